@@ -1,0 +1,81 @@
+// Internal declarations shared by the HIP translation units of libgla_hip.so.
+// gfx950 (MI355X) only.  Matrices are column-major with explicit leading
+// dimensions, exactly the Julia layout the reference works on.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <string>
+
+#include "gla_hip.h"
+
+namespace gla {
+
+typedef double d4 __attribute__((ext_vector_type(4)));
+typedef float f4 __attribute__((ext_vector_type(4)));
+
+// Thread-local error text returned by gla_last_error_string().
+void set_error(const char *fmt, ...);
+
+#define GLA_HIP_TRY(expr)                                                              \
+    do {                                                                               \
+        hipError_t _e = (expr);                                                        \
+        if (_e != hipSuccess) {                                                        \
+            ::gla::set_error("%s:%d %s -> %s", __FILE__, __LINE__, #expr,              \
+                             hipGetErrorString(_e));                                   \
+            return GLA_ERR_HIP;                                                        \
+        }                                                                              \
+    } while (0)
+
+#define GLA_LAUNCH_CHECK()                                                             \
+    do {                                                                               \
+        hipError_t _e = hipGetLastError();                                             \
+        if (_e != hipSuccess) {                                                        \
+            ::gla::set_error("%s:%d launch failed: %s", __FILE__, __LINE__,            \
+                             hipGetErrorString(_e));                                   \
+            return GLA_ERR_HIP;                                                        \
+        }                                                                              \
+    } while (0)
+
+// ------------------------------------------------------------------ handle --
+// Workspace slots carved out of one device allocation that only grows.
+struct Handle {
+    int device = 0;
+    hipStream_t side = nullptr;       // second stream for look-ahead overlap
+    hipEvent_t ev_fork = nullptr;     // main -> side
+    hipEvent_t ev_join = nullptr;     // side -> main
+    hipEvent_t ev_step[2] = {nullptr, nullptr};
+    int *dinfo = nullptr;             // device: first failing pivot (1-based), 0 = ok
+    unsigned *dbar = nullptr;         // device: grid-barrier counters
+    void *ws = nullptr;               // device workspace
+    size_t ws_bytes = 0;
+    int num_cus = 256;
+};
+
+// Ensure the workspace holds at least `bytes` (grows with hipMalloc, which
+// synchronises the device; the first call of a given size pays for it).
+int ensure_workspace(Handle *h, size_t bytes);
+
+// ------------------------------------------------------------------ GEMM --
+// C = alpha*op(A)*op(B) + beta*C  with op(A) M×K, op(B) K×N.
+// mode 0: every element; mode 1: only elements with (row + diag_off) >= col
+// ("lower" relative to a diagonal), tiles wholly above it are skipped.
+// ta/tb: 'N' or 'T'.  Supported (ta,tb,mode): (N,N,0) (N,T,0) (T,N,0) (N,T,1).
+template <typename T>
+int gemm(hipStream_t s, char ta, char tb, int mode, int64_t M, int64_t N, int64_t K, T alpha,
+         const T *A, int64_t lda, const T *B, int64_t ldb, T beta, T *C, int64_t ldc,
+         int64_t diag_off = 0, T *ws = nullptr, size_t ws_elems = 0);
+
+// ------------------------------------------------------------- Cholesky --
+template <typename T>
+int chol_device(Handle *h, hipStream_t s, int64_t n, T *A, int64_t lda, int algo,
+                int64_t param);
+
+// Small helpers used across units.
+__global__ void k_set_int(int *p, int v);
+
+inline int64_t cdiv(int64_t a, int64_t b) { return (a + b - 1) / b; }
+
+}  // namespace gla

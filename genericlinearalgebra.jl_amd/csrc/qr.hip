@@ -1,0 +1,702 @@
+// Householder QR on gfx950: qrUnblocked!, qrBlocked!, QR2[Tuple{:QBlocked}]
+// and the compact-WY applications lmul!(H, A, M) / lmul!(H', A, M)
+// (src/qr.jl:64-146, src/householder.jl:82-159, src/juliaBLAS.jl:154-166).
+//
+// Panel (qr_panel_kernel).  A block column of width w ≤ 128 is factored
+// column by column exactly as qrUnblocked! does — reflector! then the left
+// reflectorApply! on the remaining panel columns — with the panel spread over
+// P workgroups by rows and held in registers (thread (c, g) owns column c and
+// 64 consecutive rows).  Each Householder step needs sums over all rows, so
+// each column costs one grid-wide exchange: every workgroup publishes, for
+// every panel column c, the partial Σ_{r>j} x_r a_rc (x = column j) and its
+// row-j entry a_jc; after a barrier every workgroup adds the P partials in
+// the same order, so all of them hold identical totals.  From these:
+//   ‖x‖² = x_j² + Σ x_r²,  ν = copysign(‖x‖, x_j),  ξ = x_j + ν,  τ = ξ/ν,
+//   v_r = x_r/ξ (r > j),   vᵀa_c = a_jc + (Σ x_r a_rc)/ξ,
+//   a_c -= τ (vᵀa_c) v,    and for c < j, v_cᵀv_j = V_jc + (Σ V_rc x_r)/ξ,
+// the last giving column j of the compact-WY T by the forward recurrence
+// T[0:j, j] = -τ_j T[0:j,0:j] (V[:,0:j]ᵀ v_j), kept in LDS by workgroup 0.
+// The kernel also writes V with its unit diagonal and zero upper part
+// explicit (Vw), which the WY update reads as a plain matrix.
+//
+// Trailing update (src/householder.jl:119-157):  W = Vwᵀ A2 (MFMA GEMM,
+// split-K), W = Tᵀ W, A2 -= Vw W — the same three contractions the
+// reference makes, with V1 folded into Vw.
+#include <algorithm>
+
+#include "gla_internal.hpp"
+
+namespace gla {
+
+template <typename T>
+int unit_upper_explicit(hipStream_t s, int64_t n, const T *src, int64_t lds, T *dst, int64_t ldd);
+
+namespace {
+
+constexpr int kQrThreads = 256;
+constexpr int kQrNB = 128;   // max panel width
+constexpr int kQrRPT = 64;   // rows per thread
+constexpr unsigned kSpinLimit = 1u << 25;
+
+#define GLA_TRY(expr)                  \
+    do {                               \
+        int _rc = (expr);              \
+        if (_rc != GLA_OK) return _rc; \
+    } while (0)
+
+template <typename T>
+struct QrPanelP {
+    int64_t m;
+    int w, kmax;
+    T *A;
+    int64_t lda;
+    T *tau;
+    T *Vw;
+    int64_t ldv;
+    T *Tw;
+    int64_t ldt;
+    T *part;
+    unsigned *bar;
+    unsigned bar_base;
+    int *err;
+};
+
+template <typename T>
+__device__ __forceinline__ T dsqrt(T x);
+template <>
+__device__ __forceinline__ double dsqrt<double>(double x) { return __builtin_sqrt(x); }
+template <>
+__device__ __forceinline__ float dsqrt<float>(float x) { return __builtin_sqrtf(x); }
+template <typename T>
+__device__ __forceinline__ T dcopysign(T a, T b);
+template <>
+__device__ __forceinline__ double dcopysign<double>(double a, double b) { return __builtin_copysign(a, b); }
+template <>
+__device__ __forceinline__ float dcopysign<float>(float a, float b) { return __builtin_copysignf(a, b); }
+
+template <typename T>
+__device__ __forceinline__ void store_coherent(T *p, T v)
+{
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+template <typename T>
+__device__ __forceinline__ T load_coherent(const T *p)
+{
+    return __hip_atomic_load(const_cast<T *>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// All workgroups of the grid meet here.  The waits are bounded: after
+// kSpinLimit polls the flag *err is raised and the workgroup carries on, so
+// no launch can hang.
+__device__ __forceinline__ void grid_barrier(unsigned *bar, unsigned target, int *err)
+{
+    __threadfence();
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        atomicAdd(bar, 1u);
+        unsigned spins = 0;
+        while (__hip_atomic_load(bar, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
+            __builtin_amdgcn_s_sleep(1);
+            if (++spins > kSpinLimit) {
+                atomicExch(err, 1);
+                break;
+            }
+        }
+        __threadfence();
+    }
+    __syncthreads();
+}
+
+template <typename T, int NB, int RPT>
+__global__ __launch_bounds__(kQrThreads) void qr_panel_kernel(QrPanelP<T> p)
+{
+    constexpr int G = kQrThreads / NB;   // row groups
+    constexpr int R = G * RPT;           // rows per workgroup
+    static_assert(NB % 64 == 0 || G == 1, "wave-uniform row groups");
+
+    __shared__ T xs[R];             // column j of this workgroup's rows, then v
+    __shared__ T red[2][G][NB];     // per-group partial sums
+    __shared__ T tot[2 * NB];       // grid totals for the current column
+    __shared__ T gsh[NB];           // v_iᵀ v_j for i < j (workgroup 0)
+    __shared__ T Ts[NB * NB];       // T, column-major (workgroup 0)
+
+    const int t = threadIdx.x;
+    const int c = t % NB, g = t / NB;
+    const int wg = blockIdx.x, P = gridDim.x;
+    const int64_t m = p.m;
+    const int64_t row0 = (int64_t)wg * R;             // first row of this workgroup
+    const int64_t rb = row0 + (int64_t)g * RPT;       // first row of this thread
+    const bool cact = c < p.w;
+    const int64_t lda = p.lda;
+
+    T a[RPT];
+#pragma unroll
+    for (int q = 0; q < RPT; ++q) {
+        const int64_t r = rb + q;
+        a[q] = (cact && r < m) ? p.A[r + (int64_t)c * lda] : T(0);
+    }
+    if (wg == 0)
+        for (int e = t; e < NB * NB; e += kQrThreads) Ts[e] = T(0);
+
+    for (int j = 0; j < p.kmax; ++j) {
+        // (1) column j of this workgroup's rows to LDS
+        if (c == j) {
+#pragma unroll
+            for (int q = 0; q < RPT; ++q) xs[g * RPT + q] = a[q];
+        }
+        __syncthreads();
+
+        // (2) partial Σ_{r>j} a_rc x_r and the row-j entry of column c
+        T dot = T(0), rowj = T(0);
+        if (cact) {
+#pragma unroll
+            for (int q = 0; q < RPT; ++q) {
+                const int64_t r = rb + q;
+                if (r > j) dot += a[q] * xs[g * RPT + q];
+                if (r == j) rowj = a[q];
+            }
+        }
+        red[0][g][c] = dot;
+        red[1][g][c] = rowj;
+        __syncthreads();
+
+        T *part = p.part + (size_t)(j & 1) * P * 2 * NB;
+        if (P > 1) {
+            if (t < NB) {
+                T s0 = T(0), s1 = T(0);
+#pragma unroll
+                for (int gg = 0; gg < G; ++gg) {
+                    s0 += red[0][gg][t];
+                    s1 += red[1][gg][t];
+                }
+                store_coherent(part + (size_t)wg * 2 * NB + t, s0);
+                store_coherent(part + (size_t)wg * 2 * NB + NB + t, s1);
+            }
+            grid_barrier(p.bar, p.bar_base + (unsigned)(j + 1) * (unsigned)P, p.err);
+            // (3) identical totals in every workgroup: fixed summation order
+            for (int e = t; e < 2 * NB; e += kQrThreads) {
+                T s = T(0);
+                for (int pp = 0; pp < P; ++pp) s += load_coherent(part + (size_t)pp * 2 * NB + e);
+                tot[e] = s;
+            }
+        } else {
+            if (t < NB) {
+                T s0 = T(0), s1 = T(0);
+#pragma unroll
+                for (int gg = 0; gg < G; ++gg) {
+                    s0 += red[0][gg][t];
+                    s1 += red[1][gg][t];
+                }
+                tot[t] = s0;
+                tot[NB + t] = s1;
+            }
+        }
+        __syncthreads();
+
+        // (4) the reflector of column j (Julia's reflector!)
+        const T xj = tot[NB + j];
+        const T nrm = dsqrt(xj * xj + tot[j]);
+        const bool live = nrm != T(0);
+        const T nu = dcopysign(nrm, xj);
+        const T xi = xj + nu;
+        const T tauj = live ? xi / nu : T(0);
+        // v_r = x_r / ξ below the diagonal (x unchanged when the norm is 0)
+        if (live)
+            for (int e = t; e < R; e += kQrThreads)
+                if (row0 + e > j) xs[e] = xs[e] / xi;
+        if (wg == 0 && t < j) gsh[t] = live ? tot[NB + t] + tot[t] / xi : T(0);
+        __syncthreads();
+
+        // (5) apply to the remaining columns; column j becomes (β, v)
+        if (cact && live) {
+            if (c > j) {
+                const T coef = tauj * (tot[NB + c] + tot[c] / xi);
+#pragma unroll
+                for (int q = 0; q < RPT; ++q) {
+                    const int64_t r = rb + q;
+                    if (r > j) a[q] -= coef * xs[g * RPT + q];
+                    if (r == j) a[q] -= coef;
+                }
+            } else if (c == j) {
+#pragma unroll
+                for (int q = 0; q < RPT; ++q) {
+                    const int64_t r = rb + q;
+                    if (r > j) a[q] = xs[g * RPT + q];
+                    if (r == j) a[q] = -nu;
+                }
+            }
+        }
+        // (6) τ_j and column j of T
+        if (wg == 0) {
+            if (t == 0) p.tau[j] = tauj;
+            if (t < j) {
+                T s = T(0);
+                for (int l = t; l < j; ++l) s += Ts[t + l * NB] * gsh[l];
+                Ts[t + j * NB] = -tauj * s;
+            }
+            if (t == j) Ts[j + j * NB] = tauj;
+        }
+        __syncthreads();
+    }
+
+    // Write back the factored panel and the explicit V.
+#pragma unroll
+    for (int q = 0; q < RPT; ++q) {
+        const int64_t r = rb + q;
+        if (cact && r < m) {
+            p.A[r + (int64_t)c * lda] = a[q];
+            if (p.Vw && c < p.kmax)
+                p.Vw[r + (int64_t)c * p.ldv] = r < c ? T(0) : (r == c ? T(1) : a[q]);
+        }
+    }
+    if (wg == 0 && p.Tw) {
+        for (int e = t; e < p.kmax * p.kmax; e += kQrThreads) {
+            const int i = e % p.kmax, jj = e / p.kmax;
+            p.Tw[i + (int64_t)jj * p.ldt] = Ts[i + jj * NB];
+        }
+    }
+}
+
+// ---------------------------------------------------------- small kernels --
+// dst (m×n) = unit-lower explicit copy of src: 1 on the diagonal, 0 above.
+template <typename T>
+__global__ void unit_lower_copy_kernel(int64_t m, int64_t n, const T *src, int64_t lds, T *dst,
+                                       int64_t ldd)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t j = blockIdx.y;
+    if (i >= m) return;
+    dst[i + j * ldd] = i < j ? T(0) : (i == j ? T(1) : src[i + j * lds]);
+}
+
+// dst (n×n) = upper triangle of src (incl. diagonal), zeros below.
+template <typename T>
+__global__ void upper_copy_kernel(int64_t n, const T *src, int64_t lds, T *dst, int64_t ldd)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t j = blockIdx.y;
+    if (i >= n) return;
+    dst[i + j * ldd] = i <= j ? src[i + j * lds] : T(0);
+}
+
+template <typename T>
+__global__ void copy_kernel(int64_t m, int64_t n, const T *src, int64_t lds, T *dst, int64_t ldd)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t j = blockIdx.y;
+    if (i < m) dst[i + j * ldd] = src[i + j * lds];
+}
+
+// Tm = I + D·striu(G) as a full k×k matrix (src/qr.jl:69-74 before inv!).
+template <typename T>
+__global__ void tb_init_kernel(int64_t k, const T *Gm, int64_t ldg, const T *tau, T *Tm,
+                               int64_t ldt)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t j = blockIdx.y;
+    if (i >= k) return;
+    Tm[i + j * ldt] = i < j ? tau[i] * Gm[i + j * ldg] : (i == j ? T(1) : T(0));
+}
+
+// T[j,j] = τ_j; T[i,j] *= τ_j (src/qr.jl:76-81).
+template <typename T>
+__global__ void tb_scale_kernel(int64_t k, const T *tau, T *Tm, int64_t ldt)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t j = blockIdx.y;
+    if (i >= k) return;
+    if (i == j) Tm[i + j * ldt] = tau[j];
+    else if (i < j) Tm[i + j * ldt] *= tau[j];
+}
+
+// In-place inverse of a unit upper-triangular k×k block (k ≤ 128), one
+// workgroup: column j of the inverse is -X[0:j,0:j]·U[0:j,j] with the
+// columns before j already inverted (the column sweep of LAPACK trti2).
+template <typename T, int NB>
+__global__ __launch_bounds__(256) void trtri_unit_upper_leaf(int k, T *U, int64_t ldu)
+{
+    __shared__ T S[NB * (NB + 1)];
+    constexpr int LD = NB + 1;
+    for (int e = threadIdx.x; e < k * k; e += 256) {
+        const int i = e % k, j = e / k;
+        S[i + j * LD] = i < j ? U[i + (int64_t)j * ldu] : (i == j ? T(1) : T(0));
+    }
+    __syncthreads();
+    for (int j = 1; j < k; ++j) {
+        T s = T(0);
+        const int i = threadIdx.x;
+        if (i < j) {
+            s = S[i + j * LD];
+            for (int l = i + 1; l < j; ++l) s += S[i + l * LD] * S[l + j * LD];
+        }
+        __syncthreads();
+        if (i < j) S[i + j * LD] = -s;
+        __syncthreads();
+    }
+    for (int e = threadIdx.x; e < k * k; e += 256) {
+        const int i = e % k, j = e / k;
+        if (i < j) U[i + (int64_t)j * ldu] = S[i + j * LD];
+    }
+}
+
+template <typename T>
+int launch2d(hipStream_t s, void (*kern)(int64_t, int64_t, const T *, int64_t, T *, int64_t),
+             int64_t m, int64_t n, const T *src, int64_t lds, T *dst, int64_t ldd)
+{
+    if (m <= 0 || n <= 0) return GLA_OK;
+    hipLaunchKernelGGL(kern, dim3((unsigned)cdiv(m, 256), (unsigned)n), dim3(256), 0, s, m, n, src,
+                       lds, dst, ldd);
+    GLA_LAUNCH_CHECK();
+    return GLA_OK;
+}
+
+template <typename T>
+int unit_lower_copy(hipStream_t s, int64_t m, int64_t n, const T *src, int64_t lds, T *dst,
+                    int64_t ldd)
+{
+    return launch2d<T>(s, unit_lower_copy_kernel<T>, m, n, src, lds, dst, ldd);
+}
+template <typename T>
+int copy_mat(hipStream_t s, int64_t m, int64_t n, const T *src, int64_t lds, T *dst, int64_t ldd)
+{
+    return launch2d<T>(s, copy_kernel<T>, m, n, src, lds, dst, ldd);
+}
+template <typename T>
+int upper_copy(hipStream_t s, int64_t n, const T *src, int64_t lds, T *dst, int64_t ldd)
+{
+    if (n <= 0) return GLA_OK;
+    hipLaunchKernelGGL(upper_copy_kernel<T>, dim3((unsigned)cdiv(n, 256), (unsigned)n), dim3(256),
+                       0, s, n, src, lds, dst, ldd);
+    GLA_LAUNCH_CHECK();
+    return GLA_OK;
+}
+
+// Unit upper-triangular inverse, recursive: X12 = -X11 (U12 X22).
+// ws must hold k*k/4 + 1 elements.
+template <typename T>
+int trtri_unit_upper(hipStream_t s, int64_t k, T *U, int64_t ldu, T *ws)
+{
+    if (k <= 0) return GLA_OK;
+    if (k <= kQrNB) {
+        hipLaunchKernelGGL((trtri_unit_upper_leaf<T, kQrNB>), dim3(1), dim3(256), 0, s, (int)k,
+                           U, ldu);
+        GLA_LAUNCH_CHECK();
+        return GLA_OK;
+    }
+    int64_t k1 = std::max<int64_t>(kQrNB, (k / 2) / kQrNB * kQrNB);
+    int64_t k2 = k - k1;
+    T *U12 = U + k1 * ldu, *U22 = U + k1 + k1 * ldu;
+    GLA_TRY(trtri_unit_upper<T>(s, k1, U, ldu, ws));
+    GLA_TRY(trtri_unit_upper<T>(s, k2, U22, ldu, ws));
+    // The leaf inverse keeps U's strict lower part untouched, so the GEMMs
+    // read explicit-triangular copies.
+    T *X11 = ws;                         // k1×k1 (upper incl. unit diag)
+    T *W = ws + k1 * k1;                 // k1×k2
+    T *X22 = W + k1 * k2;                // k2×k2
+    GLA_TRY(unit_upper_explicit(s, k1, U, ldu, X11, k1));
+    GLA_TRY(unit_upper_explicit(s, k2, U22, ldu, X22, k2));
+    GLA_TRY(gemm<T>(s, 'N', 'N', 0, k1, k2, k2, T(1), U12, ldu, X22, k2, T(0), W, k1));
+    GLA_TRY(gemm<T>(s, 'N', 'N', 0, k1, k2, k1, T(-1), X11, k1, W, k1, T(0), U12, ldu));
+    return GLA_OK;
+}
+
+template <typename T>
+__global__ void unit_upper_copy_kernel(int64_t n, const T *src, int64_t lds, T *dst, int64_t ldd)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t j = blockIdx.y;
+    if (i >= n) return;
+    dst[i + j * ldd] = i < j ? src[i + j * lds] : (i == j ? T(1) : T(0));
+}
+
+// Workspace layout of one qrBlocked! call.
+template <typename T>
+struct QrWs {
+    T *Vw, *Tw, *W1, *W2, *part, *split;
+    size_t split_elems;
+};
+
+template <typename T>
+size_t qr_ws_elems(int64_t m, int64_t n, int nb, int64_t *split_elems)
+{
+    const int64_t R = (kQrThreads / kQrNB) * kQrRPT;
+    const int64_t P = std::max<int64_t>(1, cdiv(m, R));
+    const int64_t sp = 8 * (int64_t)nb * std::max<int64_t>(n, 1);
+    *split_elems = sp;
+    return (size_t)m * nb + (size_t)nb * nb + 2 * (size_t)nb * std::max<int64_t>(n, 1) +
+           2 * (size_t)P * 2 * kQrNB + (size_t)sp + 64;
+}
+
+template <typename T>
+int qr_panel(Handle *h, hipStream_t s, int64_t m, int w, T *A, int64_t lda, T *tau, T *Vw,
+             int64_t ldv, T *Tw, int64_t ldt, T *part, unsigned &bar_base, int *err)
+{
+    constexpr int R = (kQrThreads / kQrNB) * kQrRPT;
+    const int64_t P = std::max<int64_t>(1, cdiv(m, R));
+    // every workgroup must be resident at once for the grid barrier
+    if (P > h->num_cus) {
+        set_error("qr panel: %lld rows exceed the register-resident panel capacity", (long long)m);
+        return GLA_ERR_UNSUPPORTED;
+    }
+    QrPanelP<T> p;
+    p.m = m;
+    p.w = w;
+    p.kmax = (int)std::min<int64_t>(m, w);
+    p.A = A;
+    p.lda = lda;
+    p.tau = tau;
+    p.Vw = Vw;
+    p.ldv = ldv;
+    p.Tw = Tw;
+    p.ldt = ldt;
+    p.part = part;
+    p.bar = h->dbar;
+    p.bar_base = bar_base;
+    p.err = h->dinfo + 1;
+    hipLaunchKernelGGL((qr_panel_kernel<T, kQrNB, kQrRPT>), dim3((unsigned)P), dim3(kQrThreads),
+                       0, s, p);
+    GLA_LAUNCH_CHECK();
+    if (P > 1) bar_base += (unsigned)(p.kmax * P);
+    (void)err;
+    return GLA_OK;
+}
+
+// A2 (mm×n2) <- (I - V T Vᵀ)ᵀ A2 with explicit V (mm×k) and T (k×k upper);
+// trans 'T' applies Qᵀ (src/householder.jl:119-157), 'N' applies Q (:82-115).
+template <typename T>
+int wy_update(hipStream_t s, char trans, int64_t mm, int64_t n2, int64_t k, const T *Vw,
+              int64_t ldv, const T *Tw, int64_t ldt, T *A2, int64_t lda, T *W1, T *W2, T *split,
+              size_t split_elems)
+{
+    if (n2 <= 0 || k <= 0 || mm <= 0) return GLA_OK;
+    // :142-144  W1 = V1ᵀA1 + V2ᵀA2 = Vᵀ A2
+    GLA_TRY(gemm<T>(s, 'T', 'N', 0, k, n2, mm, T(1), Vw, ldv, A2, lda, T(0), W1, k, 0, split,
+                    split_elems));
+    // :146  W2 = Tᵀ W1  (or T W1 for Q)
+    GLA_TRY(gemm<T>(s, trans == 'T' ? 'T' : 'N', 'N', 0, k, n2, k, T(1), Tw, ldt, W1, k, T(0), W2,
+                    k));
+    // :149-154  A2 -= V W2
+    GLA_TRY(gemm<T>(s, 'N', 'N', 0, mm, n2, k, T(-1), Vw, ldv, W2, k, T(1), A2, lda));
+    return GLA_OK;
+}
+
+template <typename T>
+int qr_blocked_dev(Handle *h, hipStream_t s, int64_t m, int64_t n, T *A, int64_t lda, int nb,
+                   T *tau)
+{
+    if (m <= 0 || n <= 0) return GLA_OK;
+    int64_t split_elems = 0;
+    const size_t need = qr_ws_elems<T>(m, n, nb, &split_elems);
+    GLA_TRY(ensure_workspace(h, need * sizeof(T)));
+    T *base = reinterpret_cast<T *>(h->ws);
+    const int64_t R = (kQrThreads / kQrNB) * kQrRPT;
+    const int64_t P = std::max<int64_t>(1, cdiv(m, R));
+    T *Vw = base;
+    T *Tw = Vw + (size_t)m * nb;
+    T *W1 = Tw + (size_t)nb * nb;
+    T *W2 = W1 + (size_t)nb * n;
+    T *part = W2 + (size_t)nb * n;
+    T *split = part + 2 * (size_t)P * 2 * kQrNB;
+
+    hipLaunchKernelGGL(k_set_int, dim3(1), dim3(1), 0, s, reinterpret_cast<int *>(h->dbar), 0);
+    GLA_LAUNCH_CHECK();
+    unsigned bar_base = 0;
+    for (int64_t j0 = 0;; j0 += nb) {
+        const int64_t mm = m - j0, nn = n - j0;
+        const int w = (int)std::min<int64_t>(nn, nb);                     // src/qr.jl:121
+        const int64_t k = std::min<int64_t>(mm, w);
+        T *Ab = A + j0 + j0 * lda;
+        GLA_TRY(qr_panel<T>(h, s, mm, w, Ab, lda, tau + j0, Vw, mm, Tw, k, part, bar_base,
+                            nullptr));                                   // :123-125
+        if (nn > nb)                                                      // :126-133
+            GLA_TRY(wy_update<T>(s, 'T', mm, nn - w, k, Vw, mm, Tw, k, Ab + w * lda, lda, W1, W2,
+                                 split, (size_t)split_elems));
+        if (!(mm > nb && nn > nb)) break;                                 // :136
+    }
+    return GLA_OK;
+}
+
+}  // namespace
+
+template <typename T>
+int unit_upper_explicit(hipStream_t s, int64_t n, const T *src, int64_t lds, T *dst, int64_t ldd)
+{
+    if (n <= 0) return GLA_OK;
+    hipLaunchKernelGGL(unit_upper_copy_kernel<T>, dim3((unsigned)cdiv(n, 256), (unsigned)n),
+                       dim3(256), 0, s, n, src, lds, dst, ldd);
+    GLA_LAUNCH_CHECK();
+    return GLA_OK;
+}
+
+// ------------------------------------------------------------------ C ABI --
+namespace {
+
+template <typename T>
+int qr_entry(gla_handle_t hh, int64_t m, int64_t n, T *A, int64_t lda, int64_t bs, T *tau,
+             void *stream, bool unblocked)
+{
+    Handle *h = reinterpret_cast<Handle *>(hh);
+    if (!h) { set_error("null handle"); return -1; }
+    if (m < 0) return -2;
+    if (n < 0) return -3;
+    if (lda < std::max<int64_t>(1, m)) { set_error("lda < max(1,m)"); return -5; }
+    if (!unblocked && bs < 1) { set_error("blocksize < 1"); return -6; }
+    if (!tau && std::min(m, n) > 0) { set_error("tau is null"); return -7; }
+    hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+    GLA_HIP_TRY(hipSetDevice(h->device));
+    const int nb = (int)std::min<int64_t>(unblocked ? kQrNB : bs, kQrNB);
+    return qr_blocked_dev<T>(h, s, m, n, A, lda, nb, tau);
+}
+
+template <typename T>
+int tblock_entry(gla_handle_t hh, int64_t m, int64_t n, const T *F, int64_t ldf, const T *tau,
+                 T *Tm, int64_t ldt, void *stream)
+{
+    Handle *h = reinterpret_cast<Handle *>(hh);
+    if (!h) { set_error("null handle"); return -1; }
+    if (m < 0) return -2;
+    if (n < 0) return -3;
+    if (ldf < std::max<int64_t>(1, m)) return -5;
+    const int64_t k = std::min(m, n);
+    if (ldt < std::max<int64_t>(1, k)) return -8;
+    if (k == 0) return GLA_OK;
+    hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+    GLA_HIP_TRY(hipSetDevice(h->device));
+    // Vw (m×k) explicit, G = VwᵀVw (k×k), inverse scratch k*k
+    const size_t need = (size_t)m * k + 2 * (size_t)k * k + 64;
+    GLA_TRY(ensure_workspace(h, need * sizeof(T)));
+    T *Vw = reinterpret_cast<T *>(h->ws);
+    T *Gm = Vw + (size_t)m * k;
+    T *iw = Gm + (size_t)k * k;
+    GLA_TRY(unit_lower_copy<T>(s, m, k, F, ldf, Vw, m));
+    GLA_TRY(gemm<T>(s, 'T', 'N', 0, k, k, m, T(1), Vw, m, Vw, m, T(0), Gm, k));   // :69-74
+    hipLaunchKernelGGL(tb_init_kernel<T>, dim3((unsigned)cdiv(k, 256), (unsigned)k), dim3(256), 0,
+                       s, k, Gm, k, tau, Tm, ldt);
+    GLA_LAUNCH_CHECK();
+    GLA_TRY(trtri_unit_upper<T>(s, k, Tm, ldt, iw));                                // :75
+    hipLaunchKernelGGL(tb_scale_kernel<T>, dim3((unsigned)cdiv(k, 256), (unsigned)k), dim3(256),
+                       0, s, k, tau, Tm, ldt);                                      // :76-81
+    GLA_LAUNCH_CHECK();
+    return GLA_OK;
+}
+
+template <typename T>
+int wy_entry(gla_handle_t hh, char trans, int64_t mA, int64_t nA, const T *V, int64_t ldv,
+             int64_t nV, const T *Tm, int64_t ldt, T *A, int64_t lda, T *M, int64_t ldm,
+             void *stream)
+{
+    Handle *h = reinterpret_cast<Handle *>(hh);
+    if (!h) { set_error("null handle"); return -1; }
+    if (trans != 'N' && trans != 'T' && trans != 'C') return -2;
+    if (mA < 0) return -3;
+    if (nA < 0) return -4;
+    if (ldv < std::max<int64_t>(1, mA)) return -6;
+    if (nV < 0) return -7;
+    const int64_t b = std::min(mA, nV);
+    if (ldt < std::max<int64_t>(1, b)) return -9;
+    if (lda < std::max<int64_t>(1, mA)) return -11;
+    if (M && ldm < std::max<int64_t>(1, b)) return -13;
+    if (b == 0 || nA == 0) return GLA_OK;
+    hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+    GLA_HIP_TRY(hipSetDevice(h->device));
+    const int64_t split_elems = 8 * b * nA;
+    const size_t need = (size_t)mA * b + (size_t)b * b + 2 * (size_t)b * nA + split_elems + 64;
+    GLA_TRY(ensure_workspace(h, need * sizeof(T)));
+    T *Vw = reinterpret_cast<T *>(h->ws);
+    T *Tw = Vw + (size_t)mA * b;
+    T *W1 = Tw + (size_t)b * b;
+    T *W2 = W1 + (size_t)b * nA;
+    T *split = W2 + (size_t)b * nA;
+    GLA_TRY(unit_lower_copy<T>(s, mA, b, V, ldv, Vw, mA));
+    GLA_TRY(upper_copy<T>(s, b, Tm, ldt, Tw, b));
+    GLA_TRY(wy_update<T>(s, trans == 'N' ? 'N' : 'T', mA, nA, b, Vw, mA, Tw, b, A, lda, W1, W2,
+                         split, (size_t)split_elems));
+    if (M) GLA_TRY(copy_mat<T>(s, b, nA, W2, b, M, ldm));
+    return GLA_OK;
+}
+
+template <typename T>
+int lmul_ul_entry(gla_handle_t hh, int64_t m, int64_t n, const T *L, int64_t ldl, T *B,
+                  int64_t ldb, T alpha, void *stream)
+{
+    Handle *h = reinterpret_cast<Handle *>(hh);
+    if (!h) { set_error("null handle"); return -1; }
+    if (m < 0) return -2;
+    if (n < 0) return -3;
+    if (ldl < std::max<int64_t>(1, m)) return -5;
+    if (ldb < std::max<int64_t>(1, m)) return -7;
+    if (m == 0 || n == 0) return GLA_OK;
+    hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+    GLA_HIP_TRY(hipSetDevice(h->device));
+    const size_t need = (size_t)m * m + (size_t)m * n + 64;
+    GLA_TRY(ensure_workspace(h, need * sizeof(T)));
+    T *Lw = reinterpret_cast<T *>(h->ws);
+    T *W = Lw + (size_t)m * m;
+    GLA_TRY(unit_lower_copy<T>(s, m, m, L, ldl, Lw, m));
+    GLA_TRY(gemm<T>(s, 'N', 'N', 0, m, n, m, alpha, Lw, m, B, ldb, T(0), W, m));
+    return copy_mat<T>(s, m, n, W, m, B, ldb);
+}
+
+}  // namespace
+}  // namespace gla
+
+extern "C" {
+
+int gla_qr_unblocked_d(gla_handle_t h, int64_t m, int64_t n, double *A, int64_t lda, double *tau,
+                       void *stream)
+{
+    return gla::qr_entry<double>(h, m, n, A, lda, 0, tau, stream, true);
+}
+int gla_qr_unblocked_s(gla_handle_t h, int64_t m, int64_t n, float *A, int64_t lda, float *tau,
+                       void *stream)
+{
+    return gla::qr_entry<float>(h, m, n, A, lda, 0, tau, stream, true);
+}
+int gla_qr_blocked_d(gla_handle_t h, int64_t m, int64_t n, double *A, int64_t lda,
+                     int64_t blocksize, double *tau, double *work, int64_t ldwork, void *stream)
+{
+    (void)work;
+    (void)ldwork;
+    return gla::qr_entry<double>(h, m, n, A, lda, blocksize, tau, stream, false);
+}
+int gla_qr_blocked_s(gla_handle_t h, int64_t m, int64_t n, float *A, int64_t lda,
+                     int64_t blocksize, float *tau, float *work, int64_t ldwork, void *stream)
+{
+    (void)work;
+    (void)ldwork;
+    return gla::qr_entry<float>(h, m, n, A, lda, blocksize, tau, stream, false);
+}
+int gla_qr_tblock_d(gla_handle_t h, int64_t m, int64_t n, const double *F, int64_t ldf,
+                    const double *tau, double *T, int64_t ldt, void *stream)
+{
+    return gla::tblock_entry<double>(h, m, n, F, ldf, tau, T, ldt, stream);
+}
+int gla_qr_tblock_s(gla_handle_t h, int64_t m, int64_t n, const float *F, int64_t ldf,
+                    const float *tau, float *T, int64_t ldt, void *stream)
+{
+    return gla::tblock_entry<float>(h, m, n, F, ldf, tau, T, ldt, stream);
+}
+int gla_wy_apply_d(gla_handle_t h, char trans, int64_t mA, int64_t nA, const double *V,
+                   int64_t ldv, int64_t nV, const double *T, int64_t ldt, double *A, int64_t lda,
+                   double *M, int64_t ldm, void *stream)
+{
+    return gla::wy_entry<double>(h, trans, mA, nA, V, ldv, nV, T, ldt, A, lda, M, ldm, stream);
+}
+int gla_wy_apply_s(gla_handle_t h, char trans, int64_t mA, int64_t nA, const float *V,
+                   int64_t ldv, int64_t nV, const float *T, int64_t ldt, float *A, int64_t lda,
+                   float *M, int64_t ldm, void *stream)
+{
+    return gla::wy_entry<float>(h, trans, mA, nA, V, ldv, nV, T, ldt, A, lda, M, ldm, stream);
+}
+int gla_lmul_unit_lower_d(gla_handle_t h, int64_t m, int64_t n, const double *L, int64_t ldl,
+                          double *B, int64_t ldb, double alpha, void *stream)
+{
+    return gla::lmul_ul_entry<double>(h, m, n, L, ldl, B, ldb, alpha, stream);
+}
+int gla_lmul_unit_lower_s(gla_handle_t h, int64_t m, int64_t n, const float *L, int64_t ldl,
+                          float *B, int64_t ldb, float alpha, void *stream)
+{
+    return gla::lmul_ul_entry<float>(h, m, n, L, ldl, B, ldb, alpha, stream);
+}
+
+}  // extern "C"

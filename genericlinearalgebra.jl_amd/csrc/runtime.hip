@@ -1,0 +1,118 @@
+// Handle, workspace and error reporting of libgla_hip.so.
+#include <cstdarg>
+#include <cstring>
+
+#include "gla_internal.hpp"
+
+namespace gla {
+
+namespace {
+thread_local char g_err[512] = "";
+}
+
+void set_error(const char *fmt, ...)
+{
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(g_err, sizeof(g_err), fmt, ap);
+    va_end(ap);
+}
+
+int ensure_workspace(Handle *h, size_t bytes)
+{
+    if (bytes <= h->ws_bytes) return GLA_OK;
+    // Round up to 64 MiB granules so that a sequence of growing calls
+    // reallocates rarely.
+    const size_t gran = size_t(64) << 20;
+    size_t want = (bytes + gran - 1) / gran * gran;
+    if (h->ws) {
+        // The old buffer may still be read by queued work.
+        GLA_HIP_TRY(hipDeviceSynchronize());
+        GLA_HIP_TRY(hipFree(h->ws));
+        h->ws = nullptr;
+        h->ws_bytes = 0;
+    }
+    GLA_HIP_TRY(hipMalloc(&h->ws, want));
+    h->ws_bytes = want;
+    return GLA_OK;
+}
+
+}  // namespace gla
+
+extern "C" {
+
+const char *gla_version(void) { return "gla_hip 0.1 (gfx950)"; }
+
+const char *gla_last_error_string(void) { return gla::g_err; }
+
+int gla_handle_create(gla_handle_t *out)
+{
+    using gla::Handle;
+    if (!out) return -1;
+    Handle *h = new Handle();
+    GLA_HIP_TRY(hipGetDevice(&h->device));
+    hipDeviceProp_t prop;
+    GLA_HIP_TRY(hipGetDeviceProperties(&prop, h->device));
+    h->num_cus = prop.multiProcessorCount;
+    GLA_HIP_TRY(hipStreamCreateWithFlags(&h->side, hipStreamNonBlocking));
+    GLA_HIP_TRY(hipEventCreateWithFlags(&h->ev_fork, hipEventDisableTiming));
+    GLA_HIP_TRY(hipEventCreateWithFlags(&h->ev_join, hipEventDisableTiming));
+    GLA_HIP_TRY(hipEventCreateWithFlags(&h->ev_step[0], hipEventDisableTiming));
+    GLA_HIP_TRY(hipEventCreateWithFlags(&h->ev_step[1], hipEventDisableTiming));
+    GLA_HIP_TRY(hipMalloc(reinterpret_cast<void **>(&h->dinfo), 256));
+    GLA_HIP_TRY(hipMemset(h->dinfo, 0, 256));
+    GLA_HIP_TRY(hipMalloc(reinterpret_cast<void **>(&h->dbar), 4096));
+    GLA_HIP_TRY(hipMemset(h->dbar, 0, 4096));
+    GLA_HIP_TRY(hipDeviceSynchronize());
+    *out = reinterpret_cast<gla_handle_t>(h);
+    return GLA_OK;
+}
+
+int gla_handle_destroy(gla_handle_t hh)
+{
+    using gla::Handle;
+    Handle *h = reinterpret_cast<Handle *>(hh);
+    if (!h) return GLA_OK;
+    GLA_HIP_TRY(hipSetDevice(h->device));
+    GLA_HIP_TRY(hipDeviceSynchronize());
+    if (h->ws) (void)hipFree(h->ws);
+    if (h->dinfo) (void)hipFree(h->dinfo);
+    if (h->dbar) (void)hipFree(h->dbar);
+    (void)hipEventDestroy(h->ev_fork);
+    (void)hipEventDestroy(h->ev_join);
+    (void)hipEventDestroy(h->ev_step[0]);
+    (void)hipEventDestroy(h->ev_step[1]);
+    (void)hipStreamDestroy(h->side);
+    delete h;
+    return GLA_OK;
+}
+
+int gla_gemm_d(gla_handle_t hh, char ta, char tb, int64_t m, int64_t n, int64_t k, double alpha,
+               const double *A, int64_t lda, const double *B, int64_t ldb, double beta,
+               double *C, int64_t ldc, void *stream)
+{
+    auto *h = reinterpret_cast<gla::Handle *>(hh);
+    if (!h) return -1;
+    if (m < 0) return -4;
+    if (n < 0) return -5;
+    if (k < 0) return -6;
+    GLA_HIP_TRY(hipSetDevice(h->device));
+    return gla::gemm<double>(reinterpret_cast<hipStream_t>(stream), ta, tb, 0, m, n, k, alpha, A,
+                             lda, B, ldb, beta, C, ldc, 0);
+}
+
+int gla_gemm_s(gla_handle_t hh, char ta, char tb, int64_t m, int64_t n, int64_t k, float alpha,
+               const float *A, int64_t lda, const float *B, int64_t ldb, float beta, float *C,
+               int64_t ldc, void *stream)
+{
+    auto *h = reinterpret_cast<gla::Handle *>(hh);
+    if (!h) return -1;
+    if (m < 0) return -4;
+    if (n < 0) return -5;
+    if (k < 0) return -6;
+    GLA_HIP_TRY(hipSetDevice(h->device));
+    return gla::gemm<float>(reinterpret_cast<hipStream_t>(stream), ta, tb, 0, m, n, k, alpha, A,
+                            lda, B, ldb, beta, C, ldc, 0);
+}
+
+}  // extern "C"

@@ -1,0 +1,432 @@
+"""Host-side mirror of GenericLinearAlgebra.jl's blocked factorisation API
+over libgla_hip.so (gfx950).
+
+Julia names map to Python as ``f!`` -> ``f_`` (in place); ``Val{:L}`` ->
+``Val.L``; ``F[Tuple{:R}]`` -> ``F[R]``; ``F[Tuple{:QBlocked}]`` ->
+``F[QBlocked]``; ``H'`` -> ``adjoint(H)``.  Matrices are device tensors in
+column-major layout (``stride(0) == 1``), the storage the C ABI takes; use
+:func:`colmajor` to make one.  Errors follow the reference: ``DomainError``
+for a negative pivot (sqrt of a negative number), ``DimensionMismatch`` and
+``ArgumentError`` as the reference throws them.
+
+PyTorch supplies device memory and the current stream only; every
+computation is a call into libgla_hip.so.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+import torch
+
+from ._lib import GLA_OK, build, exported_symbols, lib  # noqa: F401
+
+__all__ = [
+    "Val", "DomainError", "DimensionMismatch", "ArgumentError", "colmajor", "to_numpy",
+    "LowerTriangular", "UpperTriangular", "UnitLowerTriangular", "Hermitian",
+    "cholUnblocked_", "cholBlocked_", "cholRecursive_", "rankUpdate_", "rdiv_", "lmul_",
+    "QR2", "R", "QBlocked", "HouseholderBlock", "Adjoint", "adjoint", "qrUnblocked_",
+    "qrBlocked_", "gemm_",
+]
+
+
+class DomainError(ValueError):
+    """sqrt of a negative pivot (Julia's DomainError)."""
+
+
+class DimensionMismatch(ValueError):
+    pass
+
+
+class ArgumentError(ValueError):
+    pass
+
+
+class Val:
+    """``Val{:L}`` / ``Val{:U}``."""
+    L = "L"
+    U = "U"
+
+
+class _Tag:
+    def __init__(self, name):
+        self.name = name
+
+    def __repr__(self):
+        return f"Tuple{{:{self.name}}}"
+
+
+R = _Tag("R")
+QBlocked = _Tag("QBlocked")
+
+
+# ------------------------------------------------------------------ plumbing
+_handles: dict[int, int] = {}
+
+
+def _handle(dev: int) -> int:
+    h = _handles.get(dev)
+    if h is None:
+        with torch.cuda.device(dev):
+            hp = C.c_void_p()
+            _check(lib().gla_handle_create(C.byref(hp)), "gla_handle_create")
+            h = hp.value
+        _handles[dev] = h
+    return h
+
+
+def _check(rc: int, what: str):
+    if rc != GLA_OK:
+        msg = lib().gla_last_error_string().decode()
+        if rc < 0 and rc > -100:
+            raise ArgumentError(f"{what}: invalid argument {-rc}: {msg}")
+        raise RuntimeError(f"{what} failed ({rc}): {msg}")
+
+
+def _sfx(A: torch.Tensor) -> str:
+    if A.dtype == torch.float64:
+        return "d"
+    if A.dtype == torch.float32:
+        return "s"
+    raise TypeError(f"element type {A.dtype} not supported (Float32/Float64)")
+
+
+def _mat(A: torch.Tensor, name="A") -> int:
+    """Validates a column-major device matrix; returns its leading dimension."""
+    if not isinstance(A, torch.Tensor) or A.dim() != 2:
+        raise TypeError(f"{name} must be a 2-D torch tensor")
+    if not A.is_cuda:
+        raise TypeError(f"{name} must be a device tensor")
+    m, n = A.shape
+    if n > 1 and A.stride(0) != 1:
+        raise TypeError(f"{name} must be column-major (stride(0) == 1); see colmajor()")
+    ld = A.stride(1) if n > 1 else max(1, m)
+    if m > 1 and A.stride(0) != 1:
+        raise TypeError(f"{name} must be column-major")
+    return max(ld, 1)
+
+
+def _vec(x: torch.Tensor, n: int, dtype, name="τ") -> torch.Tensor:
+    if x.dim() != 1 or x.numel() < n or x.dtype != dtype or not x.is_cuda or x.stride(0) != 1:
+        raise TypeError(f"{name} must be a contiguous device vector of length >= {n}")
+    return x
+
+
+def _stream(A: torch.Tensor):
+    return C.c_void_p(torch.cuda.current_stream(A.device).cuda_stream)
+
+
+def colmajor(x, dtype=None, device="cuda") -> torch.Tensor:
+    """Column-major device copy of a numpy array or tensor (Julia layout)."""
+    if isinstance(x, torch.Tensor):
+        t = x.detach()
+    else:
+        t = torch.from_numpy(np.ascontiguousarray(x))
+    if dtype is not None:
+        t = t.to(dtype)
+    if t.dim() == 1:
+        return t.to(device).contiguous()
+    m, n = t.shape
+    out = torch.empty((n, m), dtype=t.dtype, device=device).t()
+    out.copy_(t)
+    return out
+
+
+def empty_colmajor(m: int, n: int, dtype=torch.float64, device="cuda") -> torch.Tensor:
+    return torch.empty((n, m), dtype=dtype, device=device).t()
+
+
+def to_numpy(A: torch.Tensor) -> np.ndarray:
+    return np.asfortranarray(A.detach().cpu().numpy())
+
+
+# ------------------------------------------------------------- wrappers ----
+class _Wrap:
+    def __init__(self, data):
+        self.data = data
+
+    def __repr__(self):
+        return f"{type(self).__name__}({self.data!r})"
+
+
+class LowerTriangular(_Wrap):
+    pass
+
+
+class UpperTriangular(_Wrap):
+    pass
+
+
+class UnitLowerTriangular(_Wrap):
+    pass
+
+
+class Hermitian:
+    """``Hermitian(C, :L)``: only the ``uplo`` triangle of ``data`` is used."""
+
+    def __init__(self, data, uplo="U"):
+        self.data = data
+        self.uplo = uplo.strip(":").upper()
+
+
+class Adjoint:
+    def __init__(self, parent):
+        self.parent = parent
+
+
+def adjoint(x):
+    if isinstance(x, Adjoint):
+        return x.parent
+    return Adjoint(x)
+
+
+# ------------------------------------------------------------- Cholesky ----
+def _uplo(v) -> str:
+    u = str(v).strip(":").upper()
+    if u not in ("L", "U"):
+        raise ArgumentError("uplo must be Val.L or Val.U")
+    return u
+
+
+def _chol(A, uplo, fn, *extra):
+    ld = _mat(A)
+    m, n = A.shape
+    if m != n:
+        raise DimensionMismatch(f"matrix is not square: dimensions are {tuple(A.shape)}")
+    info = C.c_int64(0)
+    f = getattr(lib(), f"{fn}_{_sfx(A)}")
+    with torch.cuda.device(A.device):
+        rc = f(_handle(A.device.index), _uplo(uplo).encode(), n, C.c_void_p(A.data_ptr()), ld,
+               *extra, C.byref(info), _stream(A))
+    _check(rc, fn)
+    if info.value > 0:
+        raise DomainError(f"sqrt of a negative pivot at position {info.value}")
+    return A
+
+
+def cholUnblocked_(A, uplo=Val.L):
+    """cholUnblocked!(A, Val{:L}) — src/cholesky.jl:3-15.  Returns A."""
+    return _chol(A, uplo, "gla_chol_unblocked")
+
+
+def cholBlocked_(A, uplo, blocksize: int):
+    """cholBlocked!(A, Val{:L}, blocksize) — src/cholesky.jl:17-35.  Returns A."""
+    if int(blocksize) < 1:
+        raise ArgumentError("blocksize must be positive")
+    return _chol(A, uplo, "gla_chol_blocked", int(blocksize))
+
+
+def cholRecursive_(A, uplo=Val.L, cutoff: int = 1):
+    """cholRecursive!(A, Val{:L}, cutoff) — src/cholesky.jl:37-55.
+    Returns LowerTriangular(A) (UpperTriangular for Val.U)."""
+    _chol(A, uplo, "gla_chol_recursive", int(cutoff))
+    return LowerTriangular(A) if _uplo(uplo) == "L" else UpperTriangular(A)
+
+
+def rankUpdate_(Cm: Hermitian, A, alpha):
+    """rankUpdate!(C::Hermitian, A, α) — src/juliaBLAS.jl:89-112 (and the
+    rank-1 method :53-63 when A is a vector).  Returns C."""
+    Cd = Cm.data
+    ldc = _mat(Cd, "C")
+    n = Cd.shape[0]
+    if Cd.shape[1] != n:
+        raise DimensionMismatch("C must be square")
+    if A.dim() == 1:
+        A = A.reshape(-1, 1)
+    if A.shape[0] != n:
+        raise DimensionMismatch("first dimension of A has wrong size")
+    lda = _mat(A, "A")
+    f = getattr(lib(), f"gla_rank_update_{_sfx(Cd)}")
+    with torch.cuda.device(Cd.device):
+        rc = f(_handle(Cd.device.index), Cm.uplo.encode(), n, A.shape[1], float(alpha),
+               C.c_void_p(A.data_ptr()), lda, C.c_void_p(Cd.data_ptr()), ldc, _stream(Cd))
+    _check(rc, "gla_rank_update")
+    return Cm
+
+
+def rdiv_(B, Lt):
+    """rdiv!(B, LowerTriangular(L)') as called at src/cholesky.jl:28,48:
+    B <- B L⁻ᵀ.  ``Lt`` is ``adjoint(LowerTriangular(L))``."""
+    if not (isinstance(Lt, Adjoint) and isinstance(Lt.parent, LowerTriangular)):
+        raise TypeError("rdiv_ supports B / LowerTriangular(L)' only")
+    L = Lt.parent.data
+    ldb, ldl = _mat(B, "B"), _mat(L, "L")
+    m, n = B.shape
+    if L.shape != (n, n):
+        raise DimensionMismatch("L must be n×n with n = size(B, 2)")
+    f = getattr(lib(), f"gla_trsm_rlt_{_sfx(B)}")
+    with torch.cuda.device(B.device):
+        rc = f(_handle(B.device.index), m, n, C.c_void_p(L.data_ptr()), ldl,
+               C.c_void_p(B.data_ptr()), ldb, _stream(B))
+    _check(rc, "gla_trsm_rlt")
+    return B
+
+
+def gemm_(transa: str, transb: str, alpha, A, B, beta, Cm):
+    """C = α op(A) op(B) + β C on the MFMA engine (mul!(C, A, B, α, β))."""
+    lda, ldb, ldc = _mat(A, "A"), _mat(B, "B"), _mat(Cm, "C")
+    m, n = Cm.shape
+    k = A.shape[1] if transa == "N" else A.shape[0]
+    f = getattr(lib(), f"gla_gemm_{_sfx(Cm)}")
+    with torch.cuda.device(Cm.device):
+        rc = f(_handle(Cm.device.index), transa.encode(), transb.encode(), m, n, k, float(alpha),
+               C.c_void_p(A.data_ptr()), lda, C.c_void_p(B.data_ptr()), ldb, float(beta),
+               C.c_void_p(Cm.data_ptr()), ldc, _stream(Cm))
+    _check(rc, "gla_gemm")
+    return Cm
+
+
+# ------------------------------------------------------------------- QR ----
+class HouseholderBlock:
+    """Compact WY block I − V T Vᴴ (src/householder.jl:8-11).  ``T`` is an
+    UpperTriangular wrapper of a k×k device matrix."""
+
+    def __init__(self, V, T):
+        self.V = V
+        self.T = T if isinstance(T, UpperTriangular) else UpperTriangular(T)
+
+    def adjoint(self):
+        return Adjoint(self)
+
+    def __matmul__(self, A):
+        return _wy_mul(self, A, False)
+
+    __mul__ = __matmul__
+
+
+def _adj_matmul(self, A):
+    if isinstance(self.parent, HouseholderBlock):
+        return _wy_mul(self.parent, A, True)
+    raise TypeError("unsupported adjoint product")
+
+
+Adjoint.__matmul__ = _adj_matmul
+Adjoint.__mul__ = _adj_matmul
+
+
+def lmul_(H, A, M=None, alpha=None):
+    """lmul!(H, A, M) / lmul!(H', A, M) for a HouseholderBlock
+    (src/householder.jl:82-115,119-157), or lmul!(UnitLowerTriangular(L), B, α)
+    (src/juliaBLAS.jl:154-166).  Returns A."""
+    if isinstance(H, UnitLowerTriangular):
+        if alpha is None:
+            alpha = M if M is not None else 1.0
+        L = H.data
+        ldb, ldl = _mat(A, "B"), _mat(L, "L")
+        m, n = A.shape
+        if L.shape[0] < m or L.shape[1] < m:
+            raise DimensionMismatch("L must be at least m×m")
+        f = getattr(lib(), f"gla_lmul_unit_lower_{_sfx(A)}")
+        with torch.cuda.device(A.device):
+            rc = f(_handle(A.device.index), m, n, C.c_void_p(L.data_ptr()), ldl,
+                   C.c_void_p(A.data_ptr()), ldb, float(alpha), _stream(A))
+        _check(rc, "gla_lmul_unit_lower")
+        return A
+    trans = "N"
+    if isinstance(H, Adjoint):
+        H = H.parent
+        trans = "T"
+    if not isinstance(H, HouseholderBlock):
+        raise TypeError("lmul_ expects a HouseholderBlock, its adjoint or UnitLowerTriangular")
+    V, T = H.V, H.T.data
+    ldv, ldt, lda = _mat(V, "V"), _mat(T, "T"), _mat(A, "A")
+    mA, nA = A.shape
+    if V.shape[0] != mA:
+        raise DimensionMismatch("")
+    b = min(mA, V.shape[1])
+    if M is not None:
+        ldm = _mat(M, "M")
+        if M.shape[0] < b or M.shape[1] < nA:
+            raise DimensionMismatch("workspace M is too small")
+        mp = C.c_void_p(M.data_ptr())
+    else:
+        ldm, mp = max(b, 1), C.c_void_p(None)
+    f = getattr(lib(), f"gla_wy_apply_{_sfx(A)}")
+    with torch.cuda.device(A.device):
+        rc = f(_handle(A.device.index), trans.encode(), mA, nA, C.c_void_p(V.data_ptr()), ldv,
+               V.shape[1], C.c_void_p(T.data_ptr()), ldt, C.c_void_p(A.data_ptr()), lda, mp, ldm,
+               _stream(A))
+    _check(rc, "gla_wy_apply")
+    return A
+
+
+def _wy_mul(H, A, adj):
+    """``H * A`` / ``H' * A`` (src/householder.jl:116-117,158-159): copies A."""
+    out = colmajor(A, device=A.device)
+    return lmul_(Adjoint(H) if adj else H, out)
+
+
+class QR2:
+    """Result of qrUnblocked!/qrBlocked! (src/qr.jl:6-16): ``factors`` holds R
+    in the upper triangle and the reflectors below it, ``τ`` their loadings."""
+
+    def __init__(self, factors, tau):
+        self.factors = factors
+        self.τ = tau
+
+    @property
+    def tau(self):
+        return self.τ
+
+    def size(self, *i):
+        s = tuple(self.factors.shape)
+        return s if not i else tuple(s[j - 1] for j in i)
+
+    def __getitem__(self, key):
+        m, n = self.factors.shape
+        if key is R:                                            # src/qr.jl:55-62
+            if m >= n:
+                return UpperTriangular(self.factors[:n, :n])
+            raise ArgumentError("R matrix is trapezoid and cannot be extracted with indexing")
+        if key is QBlocked:                                     # src/qr.jl:64-83
+            k = min(m, n)
+            F = self.factors
+            T = empty_colmajor(max(k, 1), max(k, 1), dtype=F.dtype, device=F.device)[:k, :k]
+            ldf = _mat(F, "F")
+            f = getattr(lib(), f"gla_qr_tblock_{_sfx(F)}")
+            with torch.cuda.device(F.device):
+                rc = f(_handle(F.device.index), m, n, C.c_void_p(F.data_ptr()), ldf,
+                       C.c_void_p(self.τ.data_ptr()), C.c_void_p(T.data_ptr()), max(k, 1),
+                       _stream(F))
+            _check(rc, "gla_qr_tblock")
+            return HouseholderBlock(F, UpperTriangular(T))
+        raise KeyError(key)
+
+
+def qrUnblocked_(A, tau=None):
+    """qrUnblocked!(A, τ) — src/qr.jl:86-111.  Returns QR2(A, τ)."""
+    lda = _mat(A)
+    m, n = A.shape
+    k = min(m, n)
+    if tau is None:
+        tau = torch.zeros(max(k, 1), dtype=A.dtype, device=A.device)[:k]
+    _vec(tau, k, A.dtype)
+    f = getattr(lib(), f"gla_qr_unblocked_{_sfx(A)}")
+    with torch.cuda.device(A.device):
+        rc = f(_handle(A.device.index), m, n, C.c_void_p(A.data_ptr()), lda,
+               C.c_void_p(tau.data_ptr()), _stream(A))
+    _check(rc, "gla_qr_unblocked")
+    return QR2(A, tau)
+
+
+def qrBlocked_(A, blocksize: int = 12, tau=None, work=None):
+    """qrBlocked!(A, blocksize = 12, τ, work) — src/qr.jl:113-146.  Returns QR2(A, τ)."""
+    lda = _mat(A)
+    m, n = A.shape
+    k = min(m, n)
+    if int(blocksize) < 1:
+        raise ArgumentError("blocksize must be positive")
+    if tau is None:
+        tau = torch.zeros(max(k, 1), dtype=A.dtype, device=A.device)[:k]
+    _vec(tau, k, A.dtype)
+    if work is not None:
+        ldw = _mat(work, "work")
+        wp = C.c_void_p(work.data_ptr())
+    else:
+        ldw, wp = max(int(blocksize), 1), C.c_void_p(None)
+    f = getattr(lib(), f"gla_qr_blocked_{_sfx(A)}")
+    with torch.cuda.device(A.device):
+        rc = f(_handle(A.device.index), m, n, C.c_void_p(A.data_ptr()), lda, int(blocksize),
+               C.c_void_p(tau.data_ptr()), wp, ldw, _stream(A))
+    _check(rc, "gla_qr_blocked")
+    return QR2(A, tau)

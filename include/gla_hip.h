@@ -1,0 +1,147 @@
+/*
+ * gla_hip.h — C ABI of libgla_hip.so, the MI355X (gfx950) implementation of
+ * GenericLinearAlgebra.jl's blocked dense factorisations.
+ *
+ * Conventions (all entry points):
+ *   - Matrices are column-major device pointers with an explicit leading
+ *     dimension (the Julia layout): element (i,j), 0-based, is A[i + j*lda].
+ *   - `stream` is a hipStream_t passed as void* (NULL = the null stream).
+ *     Work is enqueued on it; the call returns before the GPU finishes unless
+ *     a host `info` pointer is given, in which case the call synchronises the
+ *     stream and stores the result there.
+ *   - Return value: GLA_OK (0), or a negative code: -k for an invalid k-th
+ *     argument (LAPACK style), GLA_ERR_HIP for a HIP runtime error,
+ *     GLA_ERR_UNSUPPORTED for a shape this build does not handle.
+ *     gla_last_error_string() describes the last failure on this thread.
+ *   - `info` (Cholesky): 0, or the 1-based index of the first pivot whose
+ *     square root is taken of a negative number — the point where the
+ *     reference's sqrt throws DomainError (src/cholesky.jl:5,40).  A zero or
+ *     NaN pivot is not an error, as in the reference (Inf/NaN propagate).
+ *   - Suffix _d = Float64, _s = Float32.
+ *
+ * Every entry point names the reference function it replaces.
+ */
+#ifndef GLA_HIP_H
+#define GLA_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define GLA_OK 0
+#define GLA_ERR_HIP (-1000)
+#define GLA_ERR_UNSUPPORTED (-1001)
+#define GLA_ERR_NOT_SPD (-1002)
+
+typedef struct gla_handle_s *gla_handle_t;
+
+/* ------------------------------------------------------------- runtime -- */
+/* Library state for one device: workspace, side stream, info flag.  No
+ * reference counterpart (the reference allocates per call, e.g.
+ * `work = Matrix{T}(undef, blocksize, n)` at src/qr.jl:117). */
+int gla_handle_create(gla_handle_t *h);
+int gla_handle_destroy(gla_handle_t h);
+const char *gla_last_error_string(void);
+/* Version string of this build. */
+const char *gla_version(void);
+
+/* ------------------------------------------------------------ Cholesky -- */
+/* cholUnblocked!(A, Val{:L}) — src/cholesky.jl:3-15.
+ * uplo 'L' (the reference) or 'U' (extension, U = Lᵀ stored in the upper
+ * triangle).  The opposite triangle keeps its input values, as in the
+ * reference. */
+int gla_chol_unblocked_d(gla_handle_t h, char uplo, int64_t n, double *A, int64_t lda,
+                         int64_t *info, void *stream);
+int gla_chol_unblocked_s(gla_handle_t h, char uplo, int64_t n, float *A, int64_t lda,
+                         int64_t *info, void *stream);
+
+/* cholBlocked!(A, Val{:L}, blocksize) — src/cholesky.jl:17-35. */
+int gla_chol_blocked_d(gla_handle_t h, char uplo, int64_t n, double *A, int64_t lda,
+                       int64_t blocksize, int64_t *info, void *stream);
+int gla_chol_blocked_s(gla_handle_t h, char uplo, int64_t n, float *A, int64_t lda,
+                       int64_t blocksize, int64_t *info, void *stream);
+
+/* cholRecursive!(A, Val{:L}, cutoff = 1) — src/cholesky.jl:37-55. */
+int gla_chol_recursive_d(gla_handle_t h, char uplo, int64_t n, double *A, int64_t lda,
+                         int64_t cutoff, int64_t *info, void *stream);
+int gla_chol_recursive_s(gla_handle_t h, char uplo, int64_t n, float *A, int64_t lda,
+                         int64_t cutoff, int64_t *info, void *stream);
+
+/* rankUpdate!(C::Hermitian, A::StridedVecOrMat, α::Real) —
+ * src/juliaBLAS.jl:89-112 (and the rank-1 form :53-63 with k = 1).
+ * C n×n, only the `uplo` triangle is updated: C += α A Aᵀ, A n×k. */
+int gla_rank_update_d(gla_handle_t h, char uplo, int64_t n, int64_t k, double alpha,
+                      const double *A, int64_t lda, double *C, int64_t ldc, void *stream);
+int gla_rank_update_s(gla_handle_t h, char uplo, int64_t n, int64_t k, float alpha,
+                      const float *A, int64_t lda, float *C, int64_t ldc, void *stream);
+
+/* rdiv!(B, LowerTriangular(L)') as called at src/cholesky.jl:28,48:
+ * B (m×n) <- B L⁻ᵀ, L n×n lower (non-unit). */
+int gla_trsm_rlt_d(gla_handle_t h, int64_t m, int64_t n, const double *L, int64_t ldl,
+                   double *B, int64_t ldb, void *stream);
+int gla_trsm_rlt_s(gla_handle_t h, int64_t m, int64_t n, const float *L, int64_t ldl,
+                   float *B, int64_t ldb, void *stream);
+
+/* ------------------------------------------------------------------- QR -- */
+/* qrUnblocked!(A, τ) — src/qr.jl:86-111.  A m×n is overwritten with R
+ * (upper) and the Householder vectors (strict lower, unit diagonal
+ * implicit); tau has min(m,n) entries.  Reflector convention of the Julia
+ * stdlib reflector!: β = −copysign(‖x‖, x₁), τ = (x₁−β)/(−β); a column
+ * [a,0,…,0] with a ≠ 0 gives τ = 2. */
+int gla_qr_unblocked_d(gla_handle_t h, int64_t m, int64_t n, double *A, int64_t lda,
+                       double *tau, void *stream);
+int gla_qr_unblocked_s(gla_handle_t h, int64_t m, int64_t n, float *A, int64_t lda,
+                       float *tau, void *stream);
+
+/* qrBlocked!(A, blocksize = 12, τ, work) — src/qr.jl:113-146.  `work`
+ * (blocksize × n, ld ldwork) may be NULL; the handle's workspace is used. */
+int gla_qr_blocked_d(gla_handle_t h, int64_t m, int64_t n, double *A, int64_t lda,
+                     int64_t blocksize, double *tau, double *work, int64_t ldwork,
+                     void *stream);
+int gla_qr_blocked_s(gla_handle_t h, int64_t m, int64_t n, float *A, int64_t lda,
+                     int64_t blocksize, float *tau, float *work, int64_t ldwork,
+                     void *stream);
+
+/* getindex(::QR2, Tuple{:QBlocked}) — src/qr.jl:64-83: the compact-WY
+ * T = (I + D striu(VᵀV))⁻¹ D (k×k upper, k = min(m,n)) of QR2(F, τ). */
+int gla_qr_tblock_d(gla_handle_t h, int64_t m, int64_t n, const double *F, int64_t ldf,
+                    const double *tau, double *T, int64_t ldt, void *stream);
+int gla_qr_tblock_s(gla_handle_t h, int64_t m, int64_t n, const float *F, int64_t ldf,
+                    const float *tau, float *T, int64_t ldt, void *stream);
+
+/* lmul!(H::HouseholderBlock, A, M) (trans 'N') and
+ * lmul!(adjoint(H), A, M) (trans 'T') — src/householder.jl:82-115,119-157.
+ * H = I − V T Vᵀ with V mA×nV (unit lower, strict-lower storage), T b×b,
+ * b = min(mA, nV); A mA×nA; M b×nA workspace (may be NULL). */
+int gla_wy_apply_d(gla_handle_t h, char trans, int64_t mA, int64_t nA, const double *V,
+                   int64_t ldv, int64_t nV, const double *T, int64_t ldt, double *A,
+                   int64_t lda, double *M, int64_t ldm, void *stream);
+int gla_wy_apply_s(gla_handle_t h, char trans, int64_t mA, int64_t nA, const float *V,
+                   int64_t ldv, int64_t nV, const float *T, int64_t ldt, float *A,
+                   int64_t lda, float *M, int64_t ldm, void *stream);
+
+/* lmul!(A::UnitLowerTriangular, B, α) — src/juliaBLAS.jl:154-166:
+ * B (m×n) <- α L_unit B, L m×m. */
+int gla_lmul_unit_lower_d(gla_handle_t h, int64_t m, int64_t n, const double *L,
+                          int64_t ldl, double *B, int64_t ldb, double alpha, void *stream);
+int gla_lmul_unit_lower_s(gla_handle_t h, int64_t m, int64_t n, const float *L,
+                          int64_t ldl, float *B, int64_t ldb, float alpha, void *stream);
+
+/* ------------------------------------------------------ GEMM primitive -- */
+/* The MFMA GEMM the factorisations are built on, exported for the
+ * juliaBLAS-style callers (`mul!(C, A, B, α, β)` at src/householder.jl:108,144,
+ * 149): C = α op(A) op(B) + β C, (transa,transb) in {NN, NT, TN}. */
+int gla_gemm_d(gla_handle_t h, char transa, char transb, int64_t m, int64_t n, int64_t k,
+               double alpha, const double *A, int64_t lda, const double *B, int64_t ldb,
+               double beta, double *C, int64_t ldc, void *stream);
+int gla_gemm_s(gla_handle_t h, char transa, char transb, int64_t m, int64_t n, int64_t k,
+               float alpha, const float *A, int64_t lda, const float *B, int64_t ldb,
+               float beta, float *C, int64_t ldc, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* GLA_HIP_H */

@@ -1,0 +1,243 @@
+"""ctypes front end of the CPU oracle (oracle/gla_oracle.c).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, by ``__graft_entry__.smoke()``
+and by the benchmark's cpu_baseline leg, always as the checker or the CPU
+baseline, never as the product path.
+
+Every function takes and returns numpy arrays in column-major (Fortran)
+order and copies its inputs, so callers keep their originals.  The suffix
+of the C symbol is picked from the dtype (float32 -> _s, float64 -> _d);
+``*_q`` helpers compute in __float128 (the stand-in for the reference's
+BigFloat runs) from float64 inputs.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "build", "libgla_oracle.so")
+_lib = None
+
+_i64 = C.c_int64
+_p = C.c_void_p
+
+
+def build() -> str:
+    """Compile the oracle with its Makefile; returns the library path."""
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+    return _LIB_PATH
+
+
+def lib() -> C.CDLL:
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            build()
+        L = C.CDLL(_LIB_PATH)
+        for sfx, ft in (("d", C.c_double), ("s", C.c_float)):
+            g = lambda n: getattr(L, f"{n}_{sfx}")  # noqa: E731
+            g("orc_chol_unblocked").argtypes = [_i64, _p, _i64]
+            g("orc_chol_unblocked").restype = _i64
+            g("orc_chol_blocked").argtypes = [_i64, _p, _i64, _i64]
+            g("orc_chol_blocked").restype = _i64
+            g("orc_chol_recursive").argtypes = [_i64, _p, _i64, _i64]
+            g("orc_chol_recursive").restype = _i64
+            g("orc_syr_l").argtypes = [_i64, ft, _p, _i64, _p, _i64]
+            g("orc_syrk_l").argtypes = [_i64, _i64, ft, _p, _i64, _p, _i64]
+            g("orc_trsm_rlt").argtypes = [_i64, _i64, _p, _i64, _p, _i64]
+            g("orc_reflector").argtypes = [_i64, _p]
+            g("orc_reflector").restype = ft
+            g("orc_reflector_apply").argtypes = [_i64, _i64, _p, ft, _p, _i64]
+            g("orc_qr_unblocked").argtypes = [_i64, _i64, _p, _i64, _p]
+            g("orc_qr_tblock").argtypes = [_i64, _i64, _p, _i64, _p, _p, _i64]
+            g("orc_lmul_unit_lower").argtypes = [_i64, _i64, _p, _i64, _p, _i64, ft]
+            g("orc_wy_apply").argtypes = [C.c_int, _i64, _i64, _p, _i64, _i64, _p, _i64,
+                                          _p, _i64, _p, _i64]
+            g("orc_qr_blocked").argtypes = [_i64, _i64, _p, _i64, _i64, _p, _p, _i64]
+        L.orq_chol_recursive.argtypes = [_i64, _p, _i64, _i64]
+        L.orq_chol_recursive.restype = _i64
+        L.orq_chol_unblocked.argtypes = [_i64, _p, _i64]
+        L.orq_chol_unblocked.restype = _i64
+        L.orq_qr_unblocked.argtypes = [_i64, _i64, _p, _i64, _p]
+        L.orq_chol_residual.argtypes = [_i64, _p, _i64, _p, _i64]
+        L.orq_chol_residual.restype = C.c_double
+        _lib = L
+    return _lib
+
+
+def _sfx(a: np.ndarray) -> str:
+    if a.dtype == np.float64:
+        return "d"
+    if a.dtype == np.float32:
+        return "s"
+    raise TypeError(f"oracle supports float32/float64, got {a.dtype}")
+
+
+def _f(a, dtype=None) -> np.ndarray:
+    a = np.array(a, dtype=dtype, order="F", copy=True)
+    if a.ndim == 1:
+        a = a.reshape(-1)
+    return a
+
+
+def _ptr(a: np.ndarray):
+    return a.ctypes.data_as(_p)
+
+
+def _fn(name: str, a: np.ndarray):
+    return getattr(lib(), f"{name}_{_sfx(a)}")
+
+
+# --------------------------------------------------------------- Cholesky --
+def chol_unblocked(A):
+    """cholUnblocked!(copy(A), Val{:L}) -> (A_out, info); src/cholesky.jl:3-15."""
+    A = _f(A)
+    info = _fn("orc_chol_unblocked", A)(A.shape[0], _ptr(A), A.shape[0])
+    return A, int(info)
+
+
+def chol_blocked(A, blocksize: int):
+    """cholBlocked!(copy(A), Val{:L}, blocksize) -> (A_out, info); src/cholesky.jl:17-35."""
+    A = _f(A)
+    info = _fn("orc_chol_blocked", A)(A.shape[0], _ptr(A), A.shape[0], int(blocksize))
+    return A, int(info)
+
+
+def chol_recursive(A, cutoff: int = 1):
+    """cholRecursive!(copy(A), Val{:L}, cutoff) -> (A_out, info); src/cholesky.jl:37-55."""
+    A = _f(A)
+    info = _fn("orc_chol_recursive", A)(A.shape[0], _ptr(A), A.shape[0], int(cutoff))
+    return A, int(info)
+
+
+def chol_recursive_q(A, cutoff: int = 1):
+    """cholRecursive! evaluated in __float128 from float64 input, rounded back."""
+    A = _f(A, np.float64)
+    info = lib().orq_chol_recursive(A.shape[0], _ptr(A), A.shape[0], int(cutoff))
+    return A, int(info)
+
+
+def chol_unblocked_q(A):
+    A = _f(A, np.float64)
+    info = lib().orq_chol_unblocked(A.shape[0], _ptr(A), A.shape[0])
+    return A, int(info)
+
+
+def chol_residual_q(A, L) -> float:
+    """‖A − LLᵀ‖_F/‖A‖_F in __float128 (lower triangles of A and L)."""
+    A = _f(A, np.float64)
+    L = _f(L, np.float64)
+    return float(lib().orq_chol_residual(A.shape[0], _ptr(A), A.shape[0], _ptr(L), L.shape[0]))
+
+
+def syr_l(A, a, alpha):
+    """rankUpdate!(Hermitian(A,:L), a, α) generic rank-1; src/juliaBLAS.jl:53-63."""
+    A = _f(A)
+    a = _f(a, A.dtype)
+    _fn("orc_syr_l", A)(A.shape[0], alpha, _ptr(a), 1, _ptr(A), A.shape[0])
+    return A
+
+
+def syrk_l(C_, A, alpha):
+    """rankUpdate!(Hermitian(C,:L), A, α) generic rank-k; src/juliaBLAS.jl:89-112."""
+    C_ = _f(C_)
+    A = _f(A, C_.dtype)
+    if A.ndim == 1:
+        A = A.reshape(-1, 1, order="F")
+    _fn("orc_syrk_l", C_)(C_.shape[0], A.shape[1], alpha, _ptr(A), A.shape[0], _ptr(C_),
+                          C_.shape[0])
+    return C_
+
+
+def trsm_rlt(B, L):
+    """rdiv!(B, LowerTriangular(L)') -> B L^{-T}."""
+    B = _f(B)
+    L = _f(L, B.dtype)
+    _fn("orc_trsm_rlt", B)(B.shape[0], B.shape[1], _ptr(L), L.shape[0], _ptr(B), B.shape[0])
+    return B
+
+
+# --------------------------------------------------------------------- QR --
+def reflector(x):
+    """LinearAlgebra.reflector!(copy(x)) -> (x_out, τ)."""
+    x = _f(x)
+    tau = _fn("orc_reflector", x)(x.shape[0], _ptr(x))
+    return x, float(tau)
+
+
+def reflector_apply(x, tau, A):
+    """LinearAlgebra.reflectorApply!(x, τ, copy(A)) (left)."""
+    A = _f(A)
+    x = _f(x, A.dtype)
+    if x.shape[0] != A.shape[0]:
+        raise ValueError("reflector must have same length as first dimension of matrix")
+    _fn("orc_reflector_apply", A)(A.shape[0], A.shape[1], _ptr(x), tau, _ptr(A), A.shape[0])
+    return A
+
+
+def qr_unblocked(A):
+    """qrUnblocked!(copy(A)) -> (factors, τ); src/qr.jl:86-111."""
+    A = _f(A)
+    m, n = A.shape
+    tau = np.zeros(min(m, n), dtype=A.dtype)
+    _fn("orc_qr_unblocked", A)(m, n, _ptr(A), m, _ptr(tau))
+    return A, tau
+
+
+def qr_unblocked_q(A):
+    A = _f(A, np.float64)
+    m, n = A.shape
+    tau = np.zeros(min(m, n), dtype=np.float64)
+    lib().orq_qr_unblocked(m, n, _ptr(A), m, _ptr(tau))
+    return A, tau
+
+
+def qr_blocked(A, blocksize: int = 12):
+    """qrBlocked!(copy(A), blocksize) -> (factors, τ); src/qr.jl:113-146."""
+    A = _f(A)
+    m, n = A.shape
+    tau = np.zeros(min(m, n), dtype=A.dtype)
+    work = np.zeros((blocksize, max(n, 1)), dtype=A.dtype, order="F")
+    _fn("orc_qr_blocked", A)(m, n, _ptr(A), m, int(blocksize), _ptr(tau), _ptr(work),
+                             blocksize)
+    return A, tau
+
+
+def qr_tblock(F, tau):
+    """F[Tuple{:QBlocked}].T of a QR2(F, τ); src/qr.jl:64-83."""
+    F = _f(F)
+    tau = _f(tau, F.dtype)
+    m, n = F.shape
+    k = min(m, n)
+    T = np.zeros((k, k), dtype=F.dtype, order="F")
+    _fn("orc_qr_tblock", F)(m, n, _ptr(F), m, _ptr(tau), _ptr(T), max(k, 1))
+    return T
+
+
+def wy_apply(V, T, A, adjoint: bool):
+    """lmul!(H, copy(A), M) or lmul!(H', copy(A), M) for H = HouseholderBlock(V, T);
+    src/householder.jl:82-115,119-157."""
+    A = _f(A)
+    V = _f(V, A.dtype)
+    T = _f(T, A.dtype)
+    mA, nA = A.shape
+    if V.shape[0] != mA:
+        raise ValueError("DimensionMismatch")
+    b = min(mA, V.shape[1])
+    M = np.zeros((max(b, 1), max(nA, 1)), dtype=A.dtype, order="F")
+    _fn("orc_wy_apply", A)(1 if adjoint else 0, mA, nA, _ptr(V), mA, V.shape[1], _ptr(T),
+                           T.shape[0], _ptr(A), mA, _ptr(M), M.shape[0])
+    return A
+
+
+def lmul_unit_lower(L, B, alpha):
+    """lmul!(UnitLowerTriangular(L), copy(B), α); src/juliaBLAS.jl:154-166."""
+    B = _f(B)
+    L = _f(L, B.dtype)
+    _fn("orc_lmul_unit_lower", B)(B.shape[0], B.shape[1], _ptr(L), L.shape[0], _ptr(B),
+                                  B.shape[0], alpha)
+    return B

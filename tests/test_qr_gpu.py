@@ -1,0 +1,151 @@
+"""Householder QR on the device against the oracle (restatement of
+src/qr.jl and src/householder.jl) and the reference's invariants
+(test/qr.jl:6-36).  Bar: normwise relative difference ≤ 10·n·eps for the
+packed factors (R and V), τ, and T."""
+import numpy as np
+import pytest
+
+import oracle as O
+from conftest import eps_of, rel
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+
+
+def _gla():
+    import gla
+
+    return gla
+
+
+def _tdt(dtype):
+    return torch.float64 if dtype == np.float64 else torch.float32
+
+
+DIMS = [(10, 5), (10, 10), (5, 10), (100, 50), (100, 100), (50, 100)]
+BZS = [1, 2, 3, 4, 7, 8, 9, 15, 16, 17, 31, 32, 33]
+
+
+@pytest.mark.parametrize("m,n", DIMS)
+@pytest.mark.parametrize("bz", BZS)
+def test_reference_grid(m, n, bz):
+    """test/qr.jl:7-26 — Q'A = R (m ≥ n) or triu(factors) (m < n), Q'(QA) = A —
+    plus the factors themselves against the oracle's qrBlocked!."""
+    gla = _gla()
+    rng = np.random.default_rng(m * 1000 + n * 10 + bz)
+    A = rng.standard_normal((m, n))
+    F = gla.qrBlocked_(gla.colmajor(A), bz)
+    fac, tau = gla.to_numpy(F.factors), F.tau.cpu().numpy()
+    ofac, otau = O.qr_blocked(A, bz)
+    k = max(m, n)
+    tol = 10 * k * eps_of(np.float64)
+    assert rel(fac, ofac) <= tol
+    assert rel(tau, otau) <= tol
+    Q = F[gla.QBlocked]
+    QtA = gla.to_numpy(gla.adjoint(Q) @ gla.colmajor(A))
+    if m >= n:
+        Rm = gla.to_numpy(F[gla.R].data)
+        assert rel(QtA[:min(m, n), :], np.triu(Rm)) <= tol
+    else:
+        with pytest.raises(gla.ArgumentError):
+            F[gla.R]
+        assert rel(QtA, np.triu(fac)) <= tol
+    QQA = gla.to_numpy(gla.adjoint(Q) @ (Q @ gla.colmajor(A)))
+    assert rel(QQA, A) <= tol
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+@pytest.mark.parametrize("m,n", [(1, 1), (1, 5), (7, 1), (64, 64), (129, 128), (300, 128),
+                                 (1000, 257), (4100, 300), (20000, 256)])
+def test_unblocked_and_blocked_vs_oracle(rng, dtype, m, n):
+    gla = _gla()
+    A = rng.standard_normal((m, n)).astype(dtype)
+    tol = 10 * max(m, n) * eps_of(dtype)
+    for nb in (None, 32, 128):
+        Ag = gla.colmajor(A, _tdt(dtype))
+        F = gla.qrUnblocked_(Ag) if nb is None else gla.qrBlocked_(Ag, nb)
+        fac, tau = gla.to_numpy(F.factors), F.tau.cpu().numpy()
+        ofac, otau = O.qr_blocked(A.astype(np.float64), nb or min(n, 128))
+        assert rel(fac, ofac) <= tol, nb
+        assert rel(tau, otau) <= tol, nb
+
+
+def test_known_reflector():
+    """[3;4]: β = −5, v = [1, 0.5], τ = 1.6 (Julia's reflector! convention)."""
+    gla = _gla()
+    F = gla.qrUnblocked_(gla.colmajor(np.array([[3.0], [4.0]])))
+    fac = gla.to_numpy(F.factors)
+    assert np.allclose(fac[:, 0], [-5.0, 0.5], rtol=0, atol=1e-15)
+    assert abs(float(F.tau[0]) - 1.6) < 1e-15
+
+
+def test_reduced_column_gives_tau_two():
+    """A column [a, 0, …] with a ≠ 0 gives τ = 2 and β = −a (unlike LAPACK's τ = 0);
+    a zero column gives τ = 0 and stays zero."""
+    gla = _gla()
+    A = np.zeros((6, 3))
+    A[0, 0] = 2.5
+    A[:, 2] = np.arange(1, 7)
+    F = gla.qrUnblocked_(gla.colmajor(A))
+    fac, tau = gla.to_numpy(F.factors), F.tau.cpu().numpy()
+    ofac, otau = O.qr_unblocked(A)
+    assert tau[0] == 2.0 and fac[0, 0] == -2.5
+    assert tau[1] == 0.0
+    assert rel(fac, ofac) <= 1e-14 and rel(tau, otau) <= 1e-14
+
+
+@pytest.mark.parametrize("m,n", [(40, 17), (17, 40), (300, 200), (600, 600)])
+def test_tblock_matches_explicit_reflector_product(rng, m, n):
+    """F[Tuple{:QBlocked}]: I − V T Vᵀ equals H₁H₂⋯H_k formed explicitly."""
+    gla = _gla()
+    A = rng.standard_normal((m, n))
+    F = gla.qrBlocked_(gla.colmajor(A), 16)
+    H = F[gla.QBlocked]
+    T = gla.to_numpy(H.T.data)
+    fac, tau = gla.to_numpy(F.factors), F.tau.cpu().numpy()
+    k = min(m, n)
+    Tref = O.qr_tblock(fac, tau)
+    assert rel(np.triu(T), Tref) <= 10 * max(m, n) * eps_of(np.float64)
+    if m <= 300:
+        V = np.tril(fac[:, :k], -1) + np.eye(m, k)
+        Q = np.eye(m)
+        for j in range(k):
+            Q = Q @ (np.eye(m) - tau[j] * np.outer(V[:, j], V[:, j]))
+        assert np.abs(np.eye(m) - V @ np.triu(T) @ V.T - Q).max() <= 100 * m * eps_of(np.float64)
+
+
+@pytest.mark.parametrize("adj", [False, True])
+@pytest.mark.parametrize("mA,nV,nA", [(50, 8, 13), (300, 64, 100), (2000, 128, 700), (5, 8, 3)])
+def test_wy_apply_vs_oracle(rng, adj, mA, nV, nA):
+    """lmul!(H, A, M) / lmul!(H', A, M) — src/householder.jl:82-157."""
+    gla = _gla()
+    V = rng.standard_normal((mA, nV))
+    b = min(mA, nV)
+    T = np.triu(rng.standard_normal((b, b))) / np.sqrt(b)
+    A = rng.standard_normal((mA, nA))
+    H = gla.HouseholderBlock(gla.colmajor(V), gla.UpperTriangular(gla.colmajor(T)))
+    Ag = gla.colmajor(A)
+    M = gla.colmajor(np.zeros((b, nA)))
+    gla.lmul_(gla.adjoint(H) if adj else H, Ag, M)
+    ref = O.wy_apply(V, T, A, adj)
+    assert rel(gla.to_numpy(Ag), ref) <= 10 * mA * eps_of(np.float64)
+
+
+def test_wy_dimension_mismatch():
+    gla = _gla()
+    H = gla.HouseholderBlock(gla.colmajor(np.ones((5, 2))), gla.colmajor(np.eye(2)))
+    with pytest.raises(gla.DimensionMismatch):
+        gla.lmul_(H, gla.colmajor(np.ones((4, 3))))
+
+
+@pytest.mark.parametrize("m,n", [(5, 5), (64, 30), (200, 77)])
+def test_lmul_unit_lower(rng, m, n):
+    """lmul!(UnitLowerTriangular(L), B, α) — src/juliaBLAS.jl:154-166."""
+    gla = _gla()
+    L = rng.standard_normal((m, m))
+    B = rng.standard_normal((m, n))
+    Bg = gla.colmajor(B)
+    gla.lmul_(gla.UnitLowerTriangular(gla.colmajor(L)), Bg, -0.5)
+    ref = O.lmul_unit_lower(L, B, -0.5)
+    assert rel(gla.to_numpy(Bg), ref) <= 10 * m * eps_of(np.float64)
