@@ -33,9 +33,9 @@ int unit_upper_explicit(hipStream_t s, int64_t n, const T *src, int64_t lds, T *
 
 namespace {
 
-constexpr int kQrThreads = 256;
+constexpr int kQrThreads = 1024;
 constexpr int kQrNB = 128;   // max panel width
-constexpr int kQrRPT = 64;   // rows per thread
+constexpr int kQrRPT = 16;   // rows per thread
 constexpr unsigned kSpinLimit = 1u << 25;
 
 #define GLA_TRY(expr)                  \
@@ -85,26 +85,96 @@ __device__ __forceinline__ T load_coherent(const T *p)
     return __hip_atomic_load(const_cast<T *>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-// All workgroups of the grid meet here.  The waits are bounded: after
-// kSpinLimit polls the flag *err is raised and the workgroup carries on, so
-// no launch can hang.
-__device__ __forceinline__ void grid_barrier(unsigned *bar, unsigned target, int *err)
+// Eight agent-coherent (sc1) loads in flight at once, then one wait.  The
+// compiler issues relaxed atomic loads one round trip at a time; this block
+// keeps the memory-level parallelism.  Outputs are early-clobber and the
+// wait is inside the block, so no register is read before its load lands.
+__device__ __forceinline__ void load8_coherent(const double *const p[8], double v[8])
 {
-    __threadfence();
+    asm volatile(
+        "global_load_dwordx2 %0, %8, off sc1\n\t"
+        "global_load_dwordx2 %1, %9, off sc1\n\t"
+        "global_load_dwordx2 %2, %10, off sc1\n\t"
+        "global_load_dwordx2 %3, %11, off sc1\n\t"
+        "global_load_dwordx2 %4, %12, off sc1\n\t"
+        "global_load_dwordx2 %5, %13, off sc1\n\t"
+        "global_load_dwordx2 %6, %14, off sc1\n\t"
+        "global_load_dwordx2 %7, %15, off sc1\n\t"
+        "s_waitcnt vmcnt(0)"
+        : "=&v"(v[0]), "=&v"(v[1]), "=&v"(v[2]), "=&v"(v[3]), "=&v"(v[4]), "=&v"(v[5]),
+          "=&v"(v[6]), "=&v"(v[7])
+        : "v"(p[0]), "v"(p[1]), "v"(p[2]), "v"(p[3]), "v"(p[4]), "v"(p[5]), "v"(p[6]), "v"(p[7])
+        : "memory");
+}
+__device__ __forceinline__ void load8_coherent(const float *const p[8], float v[8])
+{
+    asm volatile(
+        "global_load_dword %0, %8, off sc1\n\t"
+        "global_load_dword %1, %9, off sc1\n\t"
+        "global_load_dword %2, %10, off sc1\n\t"
+        "global_load_dword %3, %11, off sc1\n\t"
+        "global_load_dword %4, %12, off sc1\n\t"
+        "global_load_dword %5, %13, off sc1\n\t"
+        "global_load_dword %6, %14, off sc1\n\t"
+        "global_load_dword %7, %15, off sc1\n\t"
+        "s_waitcnt vmcnt(0)"
+        : "=&v"(v[0]), "=&v"(v[1]), "=&v"(v[2]), "=&v"(v[3]), "=&v"(v[4]), "=&v"(v[5]),
+          "=&v"(v[6]), "=&v"(v[7])
+        : "v"(p[0]), "v"(p[1]), "v"(p[2]), "v"(p[3]), "v"(p[4]), "v"(p[5]), "v"(p[6]), "v"(p[7])
+        : "memory");
+}
+
+// Grid-wide exchange.  Partials are written with agent-scope (sc1) stores,
+// each writer waits for its stores to complete (vmcnt(0)), then one thread
+// per workgroup bumps the arrival counter.  Readers poll the counter and read
+// the partials with agent-scope loads.  No agent-scope fence is used: on a
+// multi-XCD gfx950 one would write back and invalidate the whole L2 of the
+// XCD (buffer_wbl2 / buffer_inv), stalling the concurrent trailing update.
+// Waits are bounded: after kSpinLimit polls *err is raised and the workgroup
+// carries on, so no launch can hang.
+__device__ __forceinline__ void stores_done() { __builtin_amdgcn_s_waitcnt(0x0F70); }
+
+__device__ __forceinline__ void grid_arrive(unsigned *bar)
+{
     __syncthreads();
+    if (threadIdx.x == 0) __hip_atomic_fetch_add(bar, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__device__ __forceinline__ void grid_wait(unsigned *bar, unsigned target, int *err)
+{
     if (threadIdx.x == 0) {
-        atomicAdd(bar, 1u);
         unsigned spins = 0;
         while (__hip_atomic_load(bar, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
             __builtin_amdgcn_s_sleep(1);
             if (++spins > kSpinLimit) {
-                atomicExch(err, 1);
+                __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                 break;
             }
         }
-        __threadfence();
     }
     __syncthreads();
+}
+
+// Column jj of T by the forward recurrence T[0:jj,jj] = -τ T[0:jj,0:jj] g,
+// the sum over l split four ways across the workgroup (workgroup 0 only).
+template <typename T, int NB, int NTH>
+__device__ __forceinline__ void t_column(T *Ts, const T *g, T *tpart, int jj, T tau)
+{
+    constexpr int H = NTH / NB;
+    const int t = threadIdx.x, i = t % NB, h = t / NB;
+    if (i < jj) {
+        T s = T(0);
+        for (int l = i + h; l < jj; l += H) s += Ts[i + l * NB] * g[l];
+        tpart[h * NB + i] = s;
+    }
+    __syncthreads();
+    if (t < jj) {
+        T s = T(0);
+#pragma unroll
+        for (int hh = 0; hh < H; ++hh) s += tpart[hh * NB + t];
+        Ts[t + jj * NB] = -tau * s;
+    }
+    if (t == jj) Ts[jj + jj * NB] = tau;
 }
 
 template <typename T, int NB, int RPT>
@@ -112,31 +182,38 @@ __global__ __launch_bounds__(kQrThreads) void qr_panel_kernel(QrPanelP<T> p)
 {
     constexpr int G = kQrThreads / NB;   // row groups
     constexpr int R = G * RPT;           // rows per workgroup
-    static_assert(NB % 64 == 0 || G == 1, "wave-uniform row groups");
+    static_assert(NB % 64 == 0, "wave-uniform row groups");
+    static_assert(2 * NB <= kQrThreads, "one total per thread");
 
     __shared__ T xs[R];             // column j of this workgroup's rows, then v
-    __shared__ T red[2][G][NB];     // per-group partial sums
+    __shared__ T red[2][G][NB];     // per-group partial sums; also the split
+                                    // sums of the T recurrence and of the gather
     __shared__ T tot[2 * NB];       // grid totals for the current column
     __shared__ T gsh[NB];           // v_iᵀ v_j for i < j (workgroup 0)
     __shared__ T Ts[NB * NB];       // T, column-major (workgroup 0)
+    T *const tpart = &red[0][0][0];
 
     const int t = threadIdx.x;
-    const int c = t % NB, g = t / NB;
+    // g is uniform across a wave (NB is a multiple of 64): keep it, and the
+    // row indices derived from it, in scalar registers
+    const int c = t % NB, g = __builtin_amdgcn_readfirstlane(t / NB);
     const int wg = blockIdx.x, P = gridDim.x;
     const int64_t m = p.m;
     const int64_t row0 = (int64_t)wg * R;             // first row of this workgroup
-    const int64_t rb = row0 + (int64_t)g * RPT;       // first row of this thread
+    const int rb = (int)row0 + g * RPT;               // first row of this thread
     const bool cact = c < p.w;
     const int64_t lda = p.lda;
 
     T a[RPT];
+    {
+        const T *col = p.A + (int64_t)c * lda + rb;   // this thread's rows of column c
+        const int nr = cact ? (int)std::min<int64_t>(RPT, m - rb) : 0;
 #pragma unroll
-    for (int q = 0; q < RPT; ++q) {
-        const int64_t r = rb + q;
-        a[q] = (cact && r < m) ? p.A[r + (int64_t)c * lda] : T(0);
+        for (int q = 0; q < RPT; ++q) a[q] = q < nr ? col[q] : T(0);
     }
     if (wg == 0)
         for (int e = t; e < NB * NB; e += kQrThreads) Ts[e] = T(0);
+    T tau_prev = T(0);
 
     for (int j = 0; j < p.kmax; ++j) {
         // (1) column j of this workgroup's rows to LDS
@@ -148,20 +225,25 @@ __global__ __launch_bounds__(kQrThreads) void qr_panel_kernel(QrPanelP<T> p)
 
         // (2) partial Σ_{r>j} a_rc x_r and the row-j entry of column c
         T dot = T(0), rowj = T(0);
-        if (cact) {
+        {
+            T xv[RPT];
+#pragma unroll
+            for (int q = 0; q < RPT; ++q) xv[q] = xs[g * RPT + q];
 #pragma unroll
             for (int q = 0; q < RPT; ++q) {
-                const int64_t r = rb + q;
-                if (r > j) dot += a[q] * xs[g * RPT + q];
-                if (r == j) rowj = a[q];
+                const int r = rb + q;
+                const T pr = a[q] * xv[q];
+                dot += (r > j) ? pr : T(0);
+                rowj = (r == j) ? a[q] : rowj;
             }
+            if (!cact) dot = rowj = T(0);
         }
         red[0][g][c] = dot;
         red[1][g][c] = rowj;
         __syncthreads();
 
-        T *part = p.part + (size_t)(j & 1) * P * 2 * NB;
         if (P > 1) {
+            T *part = p.part + (size_t)(j & 1) * P * 2 * NB;
             if (t < NB) {
                 T s0 = T(0), s1 = T(0);
 #pragma unroll
@@ -171,13 +253,43 @@ __global__ __launch_bounds__(kQrThreads) void qr_panel_kernel(QrPanelP<T> p)
                 }
                 store_coherent(part + (size_t)wg * 2 * NB + t, s0);
                 store_coherent(part + (size_t)wg * 2 * NB + NB + t, s1);
+                stores_done();
             }
-            grid_barrier(p.bar, p.bar_base + (unsigned)(j + 1) * (unsigned)P, p.err);
-            // (3) identical totals in every workgroup: fixed summation order
-            for (int e = t; e < 2 * NB; e += kQrThreads) {
+            grid_arrive(p.bar);
+            // workgroup 0 finishes T for the previous column while the grid gathers
+            if (wg == 0 && j > 0) t_column<T, NB, kQrThreads>(Ts, gsh, tpart, j - 1, tau_prev);
+            grid_wait(p.bar, p.bar_base + (unsigned)(j + 1) * (unsigned)P, p.err);
+            // (3) identical totals in every workgroup: fixed summation order.
+            // GS threads per total each add every GS-th partial (independent
+            // loads, one round trip each), then the GS sums are added in order.
+            {
+                constexpr int GS = kQrThreads / (2 * NB);
+                const int e = t % (2 * NB), h = t / (2 * NB);
+                const T *src = part + e;
                 T s = T(0);
-                for (int pp = 0; pp < P; ++pp) s += load_coherent(part + (size_t)pp * 2 * NB + e);
-                tot[e] = s;
+                // this thread's partials are pp = h, h+GS, h+2GS, ... < P;
+                // eight per round trip, added in increasing pp
+                for (int pp0 = h; pp0 < P; pp0 += 8 * GS) {
+                    const T *ptr[8];
+                    T v[8];
+#pragma unroll
+                    for (int u = 0; u < 8; ++u) {
+                        const int pp = pp0 + u * GS;
+                        ptr[u] = src + (size_t)(pp < P ? pp : h) * 2 * NB;
+                    }
+                    load8_coherent(ptr, v);
+#pragma unroll
+                    for (int u = 0; u < 8; ++u)
+                        if (pp0 + u * GS < P) s += v[u];
+                }
+                tpart[h * 2 * NB + e] = s;
+                __syncthreads();
+                if (t < 2 * NB) {
+                    T s = T(0);
+#pragma unroll
+                    for (int hh = 0; hh < GS; ++hh) s += tpart[hh * 2 * NB + t];
+                    tot[t] = s;
+                }
             }
         } else {
             if (t < NB) {
@@ -190,6 +302,8 @@ __global__ __launch_bounds__(kQrThreads) void qr_panel_kernel(QrPanelP<T> p)
                 tot[t] = s0;
                 tot[NB + t] = s1;
             }
+            __syncthreads();   // red is reused as tpart below
+            if (wg == 0 && j > 0) t_column<T, NB, kQrThreads>(Ts, gsh, tpart, j - 1, tau_prev);
         }
         __syncthreads();
 
@@ -209,44 +323,49 @@ __global__ __launch_bounds__(kQrThreads) void qr_panel_kernel(QrPanelP<T> p)
 
         // (5) apply to the remaining columns; column j becomes (β, v)
         if (cact && live) {
+            T xv[RPT];
+#pragma unroll
+            for (int q = 0; q < RPT; ++q) xv[q] = xs[g * RPT + q];
             if (c > j) {
                 const T coef = tauj * (tot[NB + c] + tot[c] / xi);
 #pragma unroll
                 for (int q = 0; q < RPT; ++q) {
-                    const int64_t r = rb + q;
-                    if (r > j) a[q] -= coef * xs[g * RPT + q];
-                    if (r == j) a[q] -= coef;
+                    const int r = rb + q;
+                    const T v = (r == j) ? T(1) : xv[q];
+                    const T upd = a[q] - coef * v;
+                    a[q] = (r >= j) ? upd : a[q];
                 }
             } else if (c == j) {
 #pragma unroll
                 for (int q = 0; q < RPT; ++q) {
-                    const int64_t r = rb + q;
-                    if (r > j) a[q] = xs[g * RPT + q];
-                    if (r == j) a[q] = -nu;
+                    const int r = rb + q;
+                    a[q] = (r > j) ? xv[q] : (r == j ? -nu : a[q]);
                 }
             }
         }
-        // (6) τ_j and column j of T
-        if (wg == 0) {
-            if (t == 0) p.tau[j] = tauj;
-            if (t < j) {
-                T s = T(0);
-                for (int l = t; l < j; ++l) s += Ts[t + l * NB] * gsh[l];
-                Ts[t + j * NB] = -tauj * s;
-            }
-            if (t == j) Ts[j + j * NB] = tauj;
-        }
+        if (wg == 0 && t == 0) p.tau[j] = tauj;
+        tau_prev = tauj;
+        __syncthreads();
+    }
+    if (wg == 0 && p.kmax > 0) {
+        t_column<T, NB, kQrThreads>(Ts, gsh, tpart, p.kmax - 1, tau_prev);
         __syncthreads();
     }
 
     // Write back the factored panel and the explicit V.
+    {
+        T *col = p.A + (int64_t)c * lda + rb;
+        const int nr = cact ? (int)std::min<int64_t>(RPT, m - rb) : 0;
 #pragma unroll
-    for (int q = 0; q < RPT; ++q) {
-        const int64_t r = rb + q;
-        if (cact && r < m) {
-            p.A[r + (int64_t)c * lda] = a[q];
-            if (p.Vw && c < p.kmax)
-                p.Vw[r + (int64_t)c * p.ldv] = r < c ? T(0) : (r == c ? T(1) : a[q]);
+        for (int q = 0; q < RPT; ++q)
+            if (q < nr) col[q] = a[q];
+        if (p.Vw && c < p.kmax) {
+            T *vc = p.Vw + (int64_t)c * p.ldv + rb;
+#pragma unroll
+            for (int q = 0; q < RPT; ++q) {
+                const int r = rb + q;
+                if (q < nr) vc[q] = r < c ? T(0) : (r == c ? T(1) : a[q]);
+            }
         }
     }
     if (wg == 0 && p.Tw) {
