@@ -2,17 +2,21 @@
 // (src/cholesky.jl:3-55) and the rdiv!/rankUpdate! steps they call.
 //
 // Kernels
-//   chol_panel_kernel  The diagonal block (≤128×128) is factored
-//                      register-resident by every workgroup, column by column
-//                      exactly as cholUnblocked! does (sqrt, reciprocal scale,
-//                      rank-1 update), and in the same sweep each workgroup
-//                      solves its own 64 rows of the block column below,
-//                      X·L11ᵀ = A21 (the rdiv! at src/cholesky.jl:28).  One
-//                      launch therefore does potrf(diag)+trsm(panel) with no
-//                      inter-workgroup communication.  With factor = 0 the
-//                      diagonal block already holds L and only the solve runs.
-//   gemm (gemm.hip)    The trailing rankUpdate!(Hermitian(A22,:L), A21, -1)
-//                      as an MFMA SYRK over lower tiles.
+//   chol_factor_kernel  One workgroup factors a diagonal block (≤128×128) in
+//                       place.  The block is held as 16×16 MFMA accumulator
+//                       tiles; for each 16-wide column block the 16×16
+//                       diagonal tile is factored in LDS by one wave, column
+//                       by column as cholUnblocked! does (sqrt, reciprocal
+//                       scale, rank-1 update); the tiles below solve
+//                       x·L_kkᵀ = b by substitution and the trailing tiles
+//                       take the rank-16 update C -= X_i X_jᵀ on MFMA.
+//   chol_solve_kernel   rdiv!(A21, LowerTriangular(A11)') (src/cholesky.jl:28,
+//                       48) for a ≤128-wide L11: each workgroup takes 64 rows
+//                       of A21, with the same 16-column blocking (substitution
+//                       against the 16×16 diagonal tile, MFMA rank-16 update
+//                       of the remaining columns against L11's tiles).
+//   gemm (gemm.hip)     The trailing rankUpdate!(Hermitian(A22,:L), A21, -1)
+//                       as an MFMA SYRK over lower tiles.
 //
 // Drivers
 //   blocked   right-looking, block nb ≤ 128, depth-1 look-ahead: the update
@@ -21,8 +25,8 @@
 //             handle's side stream and overlaps the next panel.
 //   recursive halves n as cholRecursive! does; below a threshold the blocked
 //             driver factors the leaf.  rdiv! with a large L11 is itself
-//             recursive (two half solves and a GEMM), its leaves are
-//             factor = 0 panel launches.
+//             recursive (two half solves and a GEMM); its leaves are
+//             chol_solve_kernel launches.
 #include <algorithm>
 
 #include "gla_internal.hpp"
@@ -34,8 +38,15 @@ __global__ void k_set_int(int *p, int v) { *p = v; }
 namespace {
 
 constexpr int kPanelNB = 128;   // max diagonal block
-constexpr int kPanelRB = 64;    // rows of A21 per workgroup
-constexpr int kPanelThreads = 512;
+constexpr int kPanelRB = 64;    // rows of A21 per solve workgroup
+constexpr int kPanelThreads = 256;
+constexpr int kXLD = 17;        // padded row length of the LDS column blocks
+
+#define GLA_TRY(expr)                  \
+    do {                               \
+        int _rc = (expr);              \
+        if (_rc != GLA_OK) return _rc; \
+    } while (0)
 
 template <typename T>
 __device__ __forceinline__ T dev_sqrt(T x);
@@ -44,146 +55,261 @@ __device__ __forceinline__ double dev_sqrt<double>(double x) { return __builtin_
 template <>
 __device__ __forceinline__ float dev_sqrt<float>(float x) { return __builtin_sqrtf(x); }
 
-template <typename T, int NB, int RB, int NTH>
-__global__ __launch_bounds__(NTH) void chol_panel_kernel(int nb, const T *__restrict__ A11,
-                                                         int64_t lda, T *__restrict__ Lw,
-                                                         int64_t ldw, T *__restrict__ B,
-                                                         int64_t ldb, int64_t nrows, int factor,
-                                                         int *info, int64_t info_base)
+// 16×16×4 MFMA with one A and one B element per lane; accumulator element r
+// of lane l sits at (row(l, r), l % 16) of the 16×16 tile.
+template <typename T>
+struct PanelMfma;
+template <>
+struct PanelMfma<double> {
+    typedef d4 acc_t;
+    static __device__ __forceinline__ acc_t mma(double a, double b, acc_t c)
+    {
+        return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+    }
+    static __device__ __forceinline__ int row(int l, int r) { return (l >> 4) + 4 * r; }
+};
+template <>
+struct PanelMfma<float> {
+    typedef f4 acc_t;
+    static __device__ __forceinline__ acc_t mma(float a, float b, acc_t c)
+    {
+        return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+    }
+    static __device__ __forceinline__ int row(int l, int r) { return 4 * (l >> 4) + r; }
+};
+
+__device__ __forceinline__ void wave_order() { __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront"); }
+
+// Factors the 16×16 tile D (row-major, row length kXLD) in place by one
+// wave, column by column as cholUnblocked! does (src/cholesky.jl:3-15), and
+// stores the reciprocals of its diagonal in Di.  `gbase` is the 1-based
+// global index of the tile's first pivot minus one; the first negative
+// pivot is recorded in *info (the reference's sqrt DomainError).
+template <typename T>
+__device__ __noinline__ void factor16(T *D, T *Di, int nvalid, int *info, int64_t gbase)
 {
-    constexpr int GD = NTH / NB;   // column groups of the diagonal block
-    constexpr int QD = NB / GD;    // diagonal elements per thread
-    constexpr int GB = NTH / RB;   // column groups of the row block
-    constexpr int QB = NB / GB;    // row-block elements per thread
-    static_assert(NB % 64 == 0 && RB % 64 == 0, "wave-uniform column groups");
-
-    __shared__ T lbuf[NB];   // column k of L below the diagonal
-    __shared__ T xbuf[RB];   // column k of X for this row block
-    __shared__ T sd[2];      // L[k][k] and its reciprocal
-
-    const int t = threadIdx.x;
-    const int di = t % NB, dg = t / NB;
-    const int bi = t % RB, bg = t / RB;
-    const int64_t row = (int64_t)blockIdx.x * RB + bi;
-    const bool rowok = row < nrows;
-
-    T dr[QD], br[QB];
-#pragma unroll
-    for (int q = 0; q < QD; ++q) {
-        const int dj = dg + GD * q;
-        dr[q] = (di < nb && dj <= di) ? A11[di + (int64_t)dj * lda] : T(0);
-    }
-#pragma unroll
-    for (int q = 0; q < QB; ++q) {
-        const int bj = bg + GB * q;
-        br[q] = (rowok && bj < nb) ? B[row + (int64_t)bj * ldb] : T(0);
-    }
-
-    // Pivot of column 0.
-    if (di == 0 && dg == 0) {
-        T akk = dr[0];
-        T d = akk;
-        if (factor) {
-            if (akk < T(0) && blockIdx.x == 0 && info) atomicCAS(info, 0, (int)(info_base + 1));
-            d = dev_sqrt(akk);
-            dr[0] = d;
-        }
-        sd[0] = d;
-        sd[1] = T(1) / d;
-    }
-    __syncthreads();
-
-    for (int k = 0; k < nb; ++k) {
-        const T rinv = sd[1];
-        // Column k of L (cholesky.jl:8, rmul! by the reciprocal) and of X.
-        if (dg == k % GD && di > k && di < nb) {
-            const int q = k / GD;
-            T cur = T(0);
-#pragma unroll
-            for (int qq = 0; qq < QD; ++qq)
-                if (qq == q) cur = dr[qq];
-            const T l = factor ? cur * rinv : cur;
-#pragma unroll
-            for (int qq = 0; qq < QD; ++qq)
-                if (qq == q) dr[qq] = l;
-            lbuf[di] = l;
-        }
-        if (bg == k % GB) {
-            const int q = k / GB;
-            T x = T(0);
-#pragma unroll
-            for (int qq = 0; qq < QB; ++qq)
-                if (qq == q) x = br[qq];
-            x = x * rinv;
-#pragma unroll
-            for (int qq = 0; qq < QB; ++qq)
-                if (qq == q) br[qq] = x;
-            xbuf[bi] = x;
-        }
-        __syncthreads();
-
-        // Rank-1 updates (juliaBLAS.jl:53-63 for the diagonal block; the
-        // right-looking substitution for the rows below).
-        if (factor && di > k && di < nb) {
-            const T li = lbuf[di];
-#pragma unroll
-            for (int q = 0; q < QD; ++q) {
-                const int dj = dg + GD * q;
-                if (dj > k && dj <= di) dr[q] -= li * lbuf[dj];
+    const int lane = threadIdx.x & 63;
+#pragma unroll 1
+    for (int c = 0; c < 16; ++c) {
+        T d = D[c * kXLD + c];
+        if (d < T(0) && lane == 0 && info && c < nvalid)
+            atomicCAS(info, 0, (int)(gbase + c + 1));
+        d = dev_sqrt(d);
+        const T rinv = T(1) / d;
+        wave_order();
+        if (lane < 16) {
+            if (lane > c) D[lane * kXLD + c] *= rinv;
+            if (lane == c) {
+                D[c * kXLD + c] = d;
+                Di[c] = rinv;
             }
         }
-        {
-            const T xi = xbuf[bi];
-#pragma unroll
-            for (int q = 0; q < QB; ++q) {
-                const int bj = bg + GB * q;
-                if (bj > k && bj < nb) br[q] -= xi * lbuf[bj];
-            }
+        wave_order();
+#pragma unroll 1
+        for (int e = lane; e < 256; e += 64) {
+            const int i = e >> 4, j = e & 15;
+            if (j > c && j <= i) D[i * kXLD + j] -= D[i * kXLD + c] * D[j * kXLD + c];
         }
-        // Pivot of column k+1, by its owner, right after its own update.
-        if (k + 1 < nb && di == k + 1 && dg == (k + 1) % GD) {
-            const int q = (k + 1) / GD;
-            T akk = T(0);
-#pragma unroll
-            for (int qq = 0; qq < QD; ++qq)
-                if (qq == q) akk = dr[qq];
-            T d = akk;
-            if (factor) {
-                if (akk < T(0) && blockIdx.x == 0 && info)
-                    atomicCAS(info, 0, (int)(info_base + k + 2));
-                d = dev_sqrt(akk);
-#pragma unroll
-                for (int qq = 0; qq < QD; ++qq)
-                    if (qq == q) dr[qq] = d;
-            }
-            sd[0] = d;
-            sd[1] = T(1) / d;
-        }
-        __syncthreads();
-    }
-
-    if (factor && blockIdx.x == 0 && Lw) {
-#pragma unroll
-        for (int q = 0; q < QD; ++q) {
-            const int dj = dg + GD * q;
-            if (di < nb && dj <= di) Lw[di + (int64_t)dj * ldw] = dr[q];
-        }
-    }
-#pragma unroll
-    for (int q = 0; q < QB; ++q) {
-        const int bj = bg + GB * q;
-        if (rowok && bj < nb) B[row + (int64_t)bj * ldb] = br[q];
+        wave_order();
     }
 }
 
-// Copies the lower triangle (incl. diagonal) of an nb×nb block.
+// Rows [r0, r1) of the LDS column block X solve x·Lᵀ = b against the 16×16
+// lower tile D (reciprocal diagonal Di), one row per thread.
 template <typename T>
-__global__ void copy_lower_kernel(int nb, const T *__restrict__ src, int64_t lds,
-                                  T *__restrict__ dst, int64_t ldd)
+__device__ __forceinline__ void solve_rows(T *X, const T *D, const T *Di, int r0, int r1)
 {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    const int j = blockIdx.y;
-    if (i < nb && j <= i) dst[i + (int64_t)j * ldd] = src[i + (int64_t)j * lds];
+    for (int rr = r0 + (int)threadIdx.x; rr < r1; rr += kPanelThreads) {
+        T x[16];
+#pragma unroll
+        for (int c = 0; c < 16; ++c) x[c] = X[rr * kXLD + c];
+#pragma unroll
+        for (int c = 0; c < 16; ++c) {
+            T sum = x[c];
+#pragma unroll
+            for (int k = 0; k < c; ++k) sum -= x[k] * D[c * kXLD + k];
+            x[c] = sum * Di[c];
+        }
+#pragma unroll
+        for (int c = 0; c < 16; ++c) X[rr * kXLD + c] = x[c];
+    }
+}
+
+// Tile (bi, bj) of the diagonal block held in slot q of wave w: slots 0..w
+// are tile row w (columns 0..w), slots w+1..8 tile row 7-w (columns
+// 0..7-w).  Nine tiles per wave; tile rows are paired so that every 16-wide
+// step gives each wave nearly the same number of updates.
+__host__ __device__ constexpr int fslot_bi(int w, int q) { return q <= w ? w : 7 - w; }
+__host__ __device__ constexpr int fslot_bj(int w, int q) { return q <= w ? q : q - w - 1; }
+
+template <typename T, int W>
+__device__ __forceinline__ void factor_body(int nb, T *__restrict__ A, int64_t lda, int *info,
+                                            int64_t info_base, T *Xb, T *Db, T *Dinv)
+{
+    typedef PanelMfma<T> M;
+    typedef typename M::acc_t acc_t;
+    constexpr int NQ = 9;
+    const int lane = threadIdx.x & 63;
+    const int lc = lane & 15;
+
+    acc_t acc[NQ];
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) {
+        const int bi = fslot_bi(W, q), bj = fslot_bj(W, q);
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const int gi = bi * 16 + M::row(lane, r), gj = bj * 16 + lc;
+            T v = T(0);
+            if (gj <= gi) v = gi < nb ? A[gi + (int64_t)gj * lda] : (gi == gj ? T(1) : T(0));
+            acc[q][r] = v;
+        }
+    }
+
+    const int nkb = (nb + 15) / 16;
+    for (int kb = 0; kb < nkb; ++kb) {
+        T *X = Xb + (kb & 1) * (kPanelNB * kXLD);
+        T *D = Db + (kb & 1) * (16 * kXLD);
+        T *Di = Dinv + (kb & 1) * 16;
+#pragma unroll
+        for (int q = 0; q < NQ; ++q) {
+            const int bi = fslot_bi(W, q), bj = fslot_bj(W, q);
+            if (bj != kb) continue;
+            T *dst = bi == kb ? D : X + bi * 16 * kXLD;
+#pragma unroll
+            for (int r = 0; r < 4; ++r) dst[M::row(lane, r) * kXLD + lc] = acc[q][r];
+        }
+        __syncthreads();
+        if (W == 0) factor16<T>(D, Di, nb - kb * 16, info, info_base + kb * 16);
+        __syncthreads();
+        solve_rows<T>(X, D, Di, (kb + 1) * 16, nkb * 16);
+        __syncthreads();
+#pragma unroll
+        for (int q = 0; q < NQ; ++q) {
+            const int bi = fslot_bi(W, q), bj = fslot_bj(W, q);
+            if (bj < kb) continue;
+            if (bj == kb) {
+                const T *src = bi == kb ? D : X + bi * 16 * kXLD;
+#pragma unroll
+                for (int r = 0; r < 4; ++r) acc[q][r] = src[M::row(lane, r) * kXLD + lc];
+                continue;
+            }
+#pragma unroll
+            for (int kq = 0; kq < 4; ++kq) {
+                const int kk = (lane >> 4) + 4 * kq;
+                const T a = -X[(bi * 16 + lc) * kXLD + kk];
+                const T b = X[(bj * 16 + lc) * kXLD + kk];
+                acc[q] = M::mma(a, b, acc[q]);
+            }
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) {
+        const int bi = fslot_bi(W, q), bj = fslot_bj(W, q);
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const int gi = bi * 16 + M::row(lane, r), gj = bj * 16 + lc;
+            if (gi < nb && gj <= gi) A[gi + (int64_t)gj * lda] = acc[q][r];
+        }
+    }
+}
+
+// Factors the nb×nb (nb ≤ 128) diagonal block at A in place (lower
+// triangle; the strict upper triangle is not touched).  One workgroup.
+template <typename T>
+__global__ __launch_bounds__(kPanelThreads) void chol_factor_kernel(int nb, T *__restrict__ A,
+                                                                    int64_t lda, int *info,
+                                                                    int64_t info_base)
+{
+    __shared__ T Xb[2 * kPanelNB * kXLD];
+    __shared__ T Db[2 * 16 * kXLD];
+    __shared__ T Dinv[2 * 16];
+    switch (threadIdx.x >> 6) {
+    case 0: factor_body<T, 0>(nb, A, lda, info, info_base, Xb, Db, Dinv); break;
+    case 1: factor_body<T, 1>(nb, A, lda, info, info_base, Xb, Db, Dinv); break;
+    case 2: factor_body<T, 2>(nb, A, lda, info, info_base, Xb, Db, Dinv); break;
+    default: factor_body<T, 3>(nb, A, lda, info, info_base, Xb, Db, Dinv); break;
+    }
+}
+
+// B (nrows×nb) <- B L⁻ᵀ for the lower nb×nb L (nb ≤ 128): each workgroup
+// takes 64 rows; wave w holds tile row w of them (8 tiles, columns 0..7).
+// Column block kb: L's tiles (j, kb) for j ≥ kb come from global memory into
+// LDS, the rows' tile kb solves against L_kk, and tiles j > kb take
+// B_j -= X_kb L_(j,kb)ᵀ on MFMA.
+template <typename T>
+__global__ __launch_bounds__(kPanelThreads) void chol_solve_kernel(int nb, const T *__restrict__ L,
+                                                                   int64_t ldl, T *__restrict__ B,
+                                                                   int64_t ldb, int64_t nrows)
+{
+    typedef PanelMfma<T> M;
+    typedef typename M::acc_t acc_t;
+    constexpr int NBLK = kPanelNB / 16;
+    constexpr int XR = kPanelNB + kPanelRB;        // rows: L column block, then B's
+    __shared__ T Xb[2 * XR * kXLD];
+    __shared__ T Dinv[2 * 16];
+    const int lane = threadIdx.x & 63;
+    const int w = threadIdx.x >> 6;
+    const int lc = lane & 15;
+    const int64_t row0 = (int64_t)blockIdx.x * kPanelRB;
+
+    acc_t acc[NBLK];
+#pragma unroll
+    for (int bj = 0; bj < NBLK; ++bj)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const int64_t gr = row0 + w * 16 + M::row(lane, r);
+            const int gj = bj * 16 + lc;
+            acc[bj][r] = (gr < nrows && gj < nb) ? B[gr + (int64_t)gj * ldb] : T(0);
+        }
+
+    const int nkb = (nb + 15) / 16;
+    for (int kb = 0; kb < nkb; ++kb) {
+        T *X = Xb + (kb & 1) * (XR * kXLD);
+        T *Di = Dinv + (kb & 1) * 16;
+        // L[kb*16 : nkb*16, kb*16 : kb*16+16] into X rows (identity padded)
+        for (int e = threadIdx.x; e < (nkb - kb) * 16 * 16; e += kPanelThreads) {
+            const int i = kb * 16 + (e >> 4), j = kb * 16 + (e & 15);
+            T v = T(0);
+            if (i < nb && j < nb) v = j <= i ? L[i + (int64_t)j * ldl] : T(0);
+            else if (i == j) v = T(1);
+            X[i * kXLD + (e & 15)] = v;
+            if (i == j) Di[e & 15] = T(1) / v;
+        }
+#pragma unroll
+        for (int bj = 0; bj < NBLK; ++bj)
+            if (bj == kb)
+#pragma unroll
+                for (int r = 0; r < 4; ++r)
+                    X[(kPanelNB + w * 16 + M::row(lane, r)) * kXLD + lc] = acc[bj][r];
+        __syncthreads();
+        solve_rows<T>(X, X + kb * 16 * kXLD, Di, kPanelNB, XR);
+        __syncthreads();
+#pragma unroll
+        for (int bj = 0; bj < NBLK; ++bj) {
+            if (bj < kb || bj >= nkb) continue;
+            if (bj == kb) {
+#pragma unroll
+                for (int r = 0; r < 4; ++r)
+                    acc[bj][r] = X[(kPanelNB + w * 16 + M::row(lane, r)) * kXLD + lc];
+                continue;
+            }
+#pragma unroll
+            for (int kq = 0; kq < 4; ++kq) {
+                const int kk = (lane >> 4) + 4 * kq;
+                const T a = -X[(kPanelNB + w * 16 + lc) * kXLD + kk];
+                const T b = X[(bj * 16 + lc) * kXLD + kk];
+                acc[bj] = M::mma(a, b, acc[bj]);
+            }
+        }
+    }
+#pragma unroll
+    for (int bj = 0; bj < NBLK; ++bj)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const int64_t gr = row0 + w * 16 + M::row(lane, r);
+            const int gj = bj * 16 + lc;
+            if (gr < nrows && gj < nb) B[gr + (int64_t)gj * ldb] = acc[bj][r];
+        }
 }
 
 // Swaps the strict upper and strict lower triangles of an n×n matrix, in
@@ -202,37 +328,28 @@ __global__ void swap_triangles_kernel(int64_t n, T *A, int64_t lda)
 }
 
 template <typename T>
-int launch_panel(hipStream_t s, int nb, const T *A11, int64_t lda, T *Lw, int64_t ldw, T *B,
-                 int64_t ldb, int64_t nrows, int factor, int *info, int64_t info_base)
+int launch_factor(hipStream_t s, int nb, T *A, int64_t lda, int *info, int64_t info_base)
 {
-    const int64_t grid = nrows > 0 ? cdiv(nrows, kPanelRB) : 1;
-    hipLaunchKernelGGL((chol_panel_kernel<T, kPanelNB, kPanelRB, kPanelThreads>),
-                       dim3((unsigned)grid), dim3(kPanelThreads), 0, s, nb, A11, lda, Lw, ldw,
-                       B, ldb, nrows, factor, info, info_base);
+    hipLaunchKernelGGL((chol_factor_kernel<T>), dim3(1), dim3(kPanelThreads), 0, s, nb, A, lda,
+                       info, info_base);
     GLA_LAUNCH_CHECK();
     return GLA_OK;
 }
 
 template <typename T>
-int launch_copy_lower(hipStream_t s, int nb, const T *src, int64_t lds, T *dst, int64_t ldd)
+int launch_solve(hipStream_t s, int nb, const T *L, int64_t ldl, T *B, int64_t ldb,
+                 int64_t nrows)
 {
-    hipLaunchKernelGGL((copy_lower_kernel<T>), dim3((unsigned)cdiv(nb, 128), nb), dim3(128), 0,
-                       s, nb, src, lds, dst, ldd);
+    if (nrows <= 0) return GLA_OK;
+    hipLaunchKernelGGL((chol_solve_kernel<T>), dim3((unsigned)cdiv(nrows, kPanelRB)),
+                       dim3(kPanelThreads), 0, s, nb, L, ldl, B, ldb, nrows);
     GLA_LAUNCH_CHECK();
     return GLA_OK;
 }
-
-#define GLA_TRY(expr)                  \
-    do {                               \
-        int _rc = (expr);              \
-        if (_rc != GLA_OK) return _rc; \
-    } while (0)
 
 // Right-looking blocked Cholesky of the lower triangle with look-ahead.
-// Diagonal blocks are factored into the workspace Lw (n × nb) and copied
-// into A once every panel that reads them has run.
 template <typename T>
-int chol_blocked_dev(Handle *h, hipStream_t s, int64_t n, T *A, int64_t lda, int nb, T *Lw,
+int chol_blocked_dev(Handle *h, hipStream_t s, int64_t n, T *A, int64_t lda, int nb,
                      int64_t info_base)
 {
     if (n <= 0) return GLA_OK;
@@ -248,22 +365,17 @@ int chol_blocked_dev(Handle *h, hipStream_t s, int64_t n, T *A, int64_t lda, int
         const int64_t below = n - k - kb;
         T *Akk = A + k + k * lda;
         T *A21 = Akk + kb;
-        GLA_TRY(launch_panel<T>(s, kb, Akk, lda, Lw + k, n, A21, lda, below, 1, h->dinfo,
-                                info_base + k));
+        GLA_TRY(launch_factor<T>(s, kb, Akk, lda, h->dinfo, info_base + k));     // :25
+        GLA_TRY(launch_solve<T>(s, kb, Akk, lda, A21, lda, below));             // :27-28
         if (below == 0) break;
         const int kb2 = (int)std::min<int64_t>(nb, below);
         const int64_t rest = below - kb2;
-        if (overlap && rest > 0) {
-            // panel(step) done -> side may update columns beyond the next panel
-            GLA_HIP_TRY(hipEventRecord(h->ev_step[0], s));
-        }
-        if (overlap && step > 0) {
-            // the side update of the previous step touched the next panel's columns
-            GLA_HIP_TRY(hipStreamWaitEvent(s, h->ev_step[1], 0));
-        }
-        // Next block column: A[k+kb:n, k+kb:k+kb+kb2] -= A21 * A21[0:kb2]ᵀ (lower part)
+        if (overlap && rest > 0) GLA_HIP_TRY(hipEventRecord(h->ev_step[0], s));
+        if (overlap && step > 0) GLA_HIP_TRY(hipStreamWaitEvent(s, h->ev_step[1], 0));
+        // :30-31 rankUpdate! on the next block column first (look-ahead) ...
         GLA_TRY(gemm<T>(s, 'N', 'T', 1, below, kb2, kb, T(-1), A21, lda, A21, lda, T(1),
                         A21 + kb * lda, lda, 0));
+        // ... and on the rest of A22 beside the next panels
         if (rest > 0) {
             hipStream_t us = overlap ? h->side : s;
             if (overlap) GLA_HIP_TRY(hipStreamWaitEvent(h->side, h->ev_step[0], 0));
@@ -277,12 +389,6 @@ int chol_blocked_dev(Handle *h, hipStream_t s, int64_t n, T *A, int64_t lda, int
         GLA_HIP_TRY(hipEventRecord(h->ev_join, h->side));
         GLA_HIP_TRY(hipStreamWaitEvent(s, h->ev_join, 0));
     }
-    // Diagonal blocks from Lw into A.
-    k = 0;
-    for (int64_t step = 0; step < nsteps; ++step, k += nb) {
-        const int kb = (int)std::min<int64_t>(nb, n - k);
-        GLA_TRY(launch_copy_lower<T>(s, kb, Lw + k, n, A + k + k * lda, lda));
-    }
     return GLA_OK;
 }
 
@@ -292,8 +398,7 @@ int trsm_rlt_dev(hipStream_t s, int64_t m, int64_t n, const T *L, int64_t ldl, T
                  int64_t ldb)
 {
     if (m <= 0 || n <= 0) return GLA_OK;
-    if (n <= kPanelNB)
-        return launch_panel<T>(s, (int)n, L, ldl, nullptr, 0, B, ldb, m, 0, nullptr, 0);
+    if (n <= kPanelNB) return launch_solve<T>(s, (int)n, L, ldl, B, ldb, m);
     // split at a multiple of the panel width
     int64_t n1 = ((n / 2) + kPanelNB - 1) / kPanelNB * kPanelNB;
     if (n1 >= n) n1 = n - kPanelNB;
@@ -308,16 +413,16 @@ int trsm_rlt_dev(hipStream_t s, int64_t m, int64_t n, const T *L, int64_t ldl, T
 // blocked driver.
 template <typename T>
 int chol_recursive_dev(Handle *h, hipStream_t s, int64_t n, T *A, int64_t lda, int64_t leaf,
-                       T *Lw, int64_t info_base)
+                       int64_t info_base)
 {
-    if (n <= leaf) return chol_blocked_dev<T>(h, s, n, A, lda, kPanelNB, Lw, info_base);
+    if (n <= leaf) return chol_blocked_dev<T>(h, s, n, A, lda, kPanelNB, info_base);
     int64_t n2 = n / 2;                                              // :44
     if (n2 > kPanelNB) n2 = n2 / kPanelNB * kPanelNB;                // keep blocks aligned
-    GLA_TRY(chol_recursive_dev<T>(h, s, n2, A, lda, leaf, Lw, info_base));   // :46
+    GLA_TRY(chol_recursive_dev<T>(h, s, n2, A, lda, leaf, info_base));   // :46
     GLA_TRY(trsm_rlt_dev<T>(s, n - n2, n2, A, lda, A + n2, lda));    // :48 rdiv!
     GLA_TRY(gemm<T>(s, 'N', 'T', 1, n - n2, n - n2, n2, T(-1), A + n2, lda, A + n2, lda, T(1),
                     A + n2 + n2 * lda, lda, 0));                     // :51 rankUpdate!
-    return chol_recursive_dev<T>(h, s, n - n2, A + n2 + n2 * lda, lda, leaf, Lw,
+    return chol_recursive_dev<T>(h, s, n - n2, A + n2 + n2 * lda, lda, leaf,
                                  info_base + n2);                    // :52
 }
 
@@ -337,22 +442,20 @@ template <typename T>
 int chol_device(Handle *h, hipStream_t s, int64_t n, T *A, int64_t lda, int algo, int64_t param)
 {
     if (n == 0) return GLA_OK;
-    GLA_TRY(ensure_workspace(h, sizeof(T) * (size_t)n * kPanelNB));
-    T *Lw = reinterpret_cast<T *>(h->ws);
     if (algo == 0) {
-        // cholUnblocked!: one panel launch when the matrix fits the register
-        // resident block, otherwise the same column sweep blocked by 128.
-        return chol_blocked_dev<T>(h, s, n, A, lda, kPanelNB, Lw, 0);
+        // cholUnblocked!: one factor launch when the matrix fits the 128×128
+        // block, otherwise the same column sweep blocked by 128.
+        return chol_blocked_dev<T>(h, s, n, A, lda, kPanelNB, 0);
     }
     if (algo == 1) {
         int nb = (int)std::min<int64_t>(std::max<int64_t>(param, 1), kPanelNB);
-        return chol_blocked_dev<T>(h, s, n, A, lda, nb, Lw, 0);
+        return chol_blocked_dev<T>(h, s, n, A, lda, nb, 0);
     }
     // cholRecursive!: the reference recurses to 1×1 (cutoff does not reach
     // the A11 branch, :46); on the device the recursion stops at a leaf of
     // at least 2048 (or `cutoff` if larger) that the blocked driver factors.
     int64_t leaf = std::max<int64_t>(param, 2048);
-    return chol_recursive_dev<T>(h, s, n, A, lda, leaf, Lw, 0);
+    return chol_recursive_dev<T>(h, s, n, A, lda, leaf, 0);
 }
 
 template int chol_device<double>(Handle *, hipStream_t, int64_t, double *, int64_t, int, int64_t);

@@ -199,7 +199,7 @@ __device__ __forceinline__ bool tile_of(const P &p, int &tm, int &tn)
 }
 
 template <typename T, int BM, int BN, bool TA, bool TB, int MODE, int VEC>
-__global__ __launch_bounds__(kThreads) void gemm_kernel(GemmP<T> p)
+__global__ __launch_bounds__(kThreads, 2) void gemm_kernel(GemmP<T> p)
 {
     constexpr int PAD = 32 / sizeof(T);
     constexpr int LDA_S = BM + PAD, LDB_S = BN + PAD;
