@@ -80,58 +80,62 @@ struct PanelMfma<float> {
 
 __device__ __forceinline__ void wave_order() { __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront"); }
 
-// Factors the 16×16 tile D (row-major, row length kXLD) in place by one
-// wave, column by column as cholUnblocked! does (src/cholesky.jl:3-15), and
-// stores the reciprocals of its diagonal in Di.  `gbase` is the 1-based
-// global index of the tile's first pivot minus one; the first negative
-// pivot is recorded in *info (the reference's sqrt DomainError).
+// One wave holds a 16×16 tile with lane (i = l % 16, g = l / 16) owning row
+// i, columns g, g+4, g+8, g+12 (v[t] = tile[i][g + 4t]).  Row i of column c
+// therefore sits in lane i + 16*(c % 4), slot c / 4, and a broadcast along a
+// column is one lane shuffle.
+
+// Factors the symmetric tile in v in place, right-looking, column by column
+// as cholUnblocked! does (src/cholesky.jl:3-15): sqrt of the pivot, scale
+// by its reciprocal, rank-1 update.  rd[c] receives 1/L[c][c].  The first
+// pivot whose square root is taken of a negative number, among the first
+// `nvalid`, is recorded in *info as gbase + c + 1.
 template <typename T>
-__device__ __noinline__ void factor16(T *D, T *Di, int nvalid, int *info, int64_t gbase)
+__device__ __forceinline__ void tile_factor(T v[4], T rd[16], int nvalid, int *info,
+                                            int64_t gbase)
 {
     const int lane = threadIdx.x & 63;
-#pragma unroll 1
+    const int i = lane & 15, g = lane >> 4;
+#pragma unroll
     for (int c = 0; c < 16; ++c) {
-        T d = D[c * kXLD + c];
-        if (d < T(0) && lane == 0 && info && c < nvalid)
-            atomicCAS(info, 0, (int)(gbase + c + 1));
+        const int ct = c >> 2, cg = c & 3;
+        T d = __shfl(v[ct], c + 16 * cg);
+        if (d < T(0) && lane == 0 && info && c < nvalid) atomicCAS(info, 0, (int)(gbase + c + 1));
         d = dev_sqrt(d);
-        const T rinv = T(1) / d;
-        wave_order();
-        if (lane < 16) {
-            if (lane > c) D[lane * kXLD + c] *= rinv;
-            if (lane == c) {
-                D[c * kXLD + c] = d;
-                Di[c] = rinv;
-            }
+        const T r = T(1) / d;
+        rd[c] = r;
+        if (g == cg) v[ct] = i > c ? v[ct] * r : (i == c ? d : v[ct]);
+        const T lic = __shfl(v[ct], i + 16 * cg);
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+            const int j = g + 4 * t;
+            const T ljc = __shfl(v[ct], j + 16 * cg);
+            if (j > c && j <= i) v[t] -= lic * ljc;
         }
-        wave_order();
-#pragma unroll 1
-        for (int e = lane; e < 256; e += 64) {
-            const int i = e >> 4, j = e & 15;
-            if (j > c && j <= i) D[i * kXLD + j] -= D[i * kXLD + c] * D[j * kXLD + c];
-        }
-        wave_order();
     }
 }
 
-// Rows [r0, r1) of the LDS column block X solve x·Lᵀ = b against the 16×16
-// lower tile D (reciprocal diagonal Di), one row per thread.
+// w <- L⁻¹ for the lower tile L in v (rd = reciprocals of its diagonal):
+// forward elimination of [L | I], row k scaled then subtracted from the rows
+// below it.
 template <typename T>
-__device__ __forceinline__ void solve_rows(T *X, const T *D, const T *Di, int r0, int r1)
+__device__ __forceinline__ void tile_inverse(const T v[4], const T rd[16], T w[4])
 {
-    for (int rr = r0 + (int)threadIdx.x; rr < r1; rr += kPanelThreads) {
-        T x[16];
+    const int lane = threadIdx.x & 63;
+    const int i = lane & 15, g = lane >> 4;
 #pragma unroll
-        for (int c = 0; c < 16; ++c) x[c] = X[rr * kXLD + c];
+    for (int t = 0; t < 4; ++t) w[t] = (i == g + 4 * t) ? T(1) : T(0);
 #pragma unroll
-        for (int c = 0; c < 16; ++c) {
-            T sum = x[c];
+    for (int k = 0; k < 16; ++k) {
+        if (i == k)
 #pragma unroll
-            for (int k = 0; k < c; ++k) sum -= x[k] * D[c * kXLD + k];
-            x[c] = sum * Di[c];
+            for (int t = 0; t < 4; ++t) w[t] *= rd[k];
+        const T lik = __shfl(v[k >> 2], i + 16 * (k & 3));
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+            const T wkj = __shfl(w[t], k + 16 * g);
+            if (i > k) w[t] -= lik * wkj;
         }
-#pragma unroll
-        for (int c = 0; c < 16; ++c) X[rr * kXLD + c] = x[c];
     }
 }
 
@@ -144,7 +148,8 @@ __host__ __device__ constexpr int fslot_bj(int w, int q) { return q <= w ? q : q
 
 template <typename T, int W>
 __device__ __forceinline__ void factor_body(int nb, T *__restrict__ A, int64_t lda, int *info,
-                                            int64_t info_base, T *Xb, T *Db, T *Dinv)
+                                            int64_t info_base, T *__restrict__ LinvG, T *Xb,
+                                            T *Db, T *Lib)
 {
     typedef PanelMfma<T> M;
     typedef typename M::acc_t acc_t;
@@ -169,7 +174,8 @@ __device__ __forceinline__ void factor_body(int nb, T *__restrict__ A, int64_t l
     for (int kb = 0; kb < nkb; ++kb) {
         T *X = Xb + (kb & 1) * (kPanelNB * kXLD);
         T *D = Db + (kb & 1) * (16 * kXLD);
-        T *Di = Dinv + (kb & 1) * 16;
+        T *Li = Lib + (kb & 1) * (16 * kXLD);
+        // (1) column block kb to LDS: the diagonal tile to D, the tiles below to X
 #pragma unroll
         for (int q = 0; q < NQ; ++q) {
             const int bi = fslot_bi(W, q), bj = fslot_bj(W, q);
@@ -179,20 +185,48 @@ __device__ __forceinline__ void factor_body(int nb, T *__restrict__ A, int64_t l
             for (int r = 0; r < 4; ++r) dst[M::row(lane, r) * kXLD + lc] = acc[q][r];
         }
         __syncthreads();
-        if (W == 0) factor16<T>(D, Di, nb - kb * 16, info, info_base + kb * 16);
+        // (2) wave 0 factors the diagonal tile and inverts it
+        if (W == 0) {
+            const int i = lane & 15, g = lane >> 4;
+            T v[4], w[4], rd[16];
+#pragma unroll
+            for (int t = 0; t < 4; ++t) v[t] = (g + 4 * t <= i) ? D[i * kXLD + g + 4 * t] : T(0);
+            tile_factor<T>(v, rd, nb - kb * 16, info, info_base + kb * 16);
+            tile_inverse<T>(v, rd, w);
+#pragma unroll
+            for (int t = 0; t < 4; ++t) {
+                D[i * kXLD + g + 4 * t] = v[t];
+                Li[i * kXLD + g + 4 * t] = w[t];
+                if (LinvG) LinvG[kb * 256 + i * 16 + g + 4 * t] = w[t];
+            }
+        }
         __syncthreads();
-        solve_rows<T>(X, D, Di, (kb + 1) * 16, nkb * 16);
-        __syncthreads();
+        // (3) the diagonal tile takes L; tiles below: X = A L⁻ᵀ on MFMA (rdiv!)
 #pragma unroll
         for (int q = 0; q < NQ; ++q) {
             const int bi = fslot_bi(W, q), bj = fslot_bj(W, q);
-            if (bj < kb) continue;
-            if (bj == kb) {
-                const T *src = bi == kb ? D : X + bi * 16 * kXLD;
+            if (bj != kb) continue;
+            if (bi == kb) {
 #pragma unroll
-                for (int r = 0; r < 4; ++r) acc[q][r] = src[M::row(lane, r) * kXLD + lc];
+                for (int r = 0; r < 4; ++r) acc[q][r] = D[M::row(lane, r) * kXLD + lc];
                 continue;
             }
+            acc_t x = {T(0), T(0), T(0), T(0)};
+#pragma unroll
+            for (int kq = 0; kq < 4; ++kq) {
+                const int kk = (lane >> 4) + 4 * kq;
+                x = M::mma(X[(bi * 16 + lc) * kXLD + kk], Li[lc * kXLD + kk], x);
+            }
+            acc[q] = x;
+#pragma unroll
+            for (int r = 0; r < 4; ++r) X[(bi * 16 + M::row(lane, r)) * kXLD + lc] = x[r];
+        }
+        __syncthreads();
+        // (4) rank-16 trailing update C_ij -= X_i X_jᵀ (rankUpdate!)
+#pragma unroll
+        for (int q = 0; q < NQ; ++q) {
+            const int bi = fslot_bi(W, q), bj = fslot_bj(W, q);
+            if (bj <= kb) continue;
 #pragma unroll
             for (int kq = 0; kq < 4; ++kq) {
                 const int kk = (lane >> 4) + 4 * kq;
@@ -214,39 +248,69 @@ __device__ __forceinline__ void factor_body(int nb, T *__restrict__ A, int64_t l
 }
 
 // Factors the nb×nb (nb ≤ 128) diagonal block at A in place (lower
-// triangle; the strict upper triangle is not touched).  One workgroup.
+// triangle; the strict upper triangle is not touched) and stores the
+// inverses of its 16×16 diagonal tiles (row-major, 256 elements each) in
+// LinvG for the solve that follows.  One workgroup.
 template <typename T>
 __global__ __launch_bounds__(kPanelThreads) void chol_factor_kernel(int nb, T *__restrict__ A,
                                                                     int64_t lda, int *info,
-                                                                    int64_t info_base)
+                                                                    int64_t info_base,
+                                                                    T *__restrict__ LinvG)
 {
     __shared__ T Xb[2 * kPanelNB * kXLD];
     __shared__ T Db[2 * 16 * kXLD];
-    __shared__ T Dinv[2 * 16];
+    __shared__ T Lib[2 * 16 * kXLD];
     switch (threadIdx.x >> 6) {
-    case 0: factor_body<T, 0>(nb, A, lda, info, info_base, Xb, Db, Dinv); break;
-    case 1: factor_body<T, 1>(nb, A, lda, info, info_base, Xb, Db, Dinv); break;
-    case 2: factor_body<T, 2>(nb, A, lda, info, info_base, Xb, Db, Dinv); break;
-    default: factor_body<T, 3>(nb, A, lda, info, info_base, Xb, Db, Dinv); break;
+    case 0: factor_body<T, 0>(nb, A, lda, info, info_base, LinvG, Xb, Db, Lib); break;
+    case 1: factor_body<T, 1>(nb, A, lda, info, info_base, LinvG, Xb, Db, Lib); break;
+    case 2: factor_body<T, 2>(nb, A, lda, info, info_base, LinvG, Xb, Db, Lib); break;
+    default: factor_body<T, 3>(nb, A, lda, info, info_base, LinvG, Xb, Db, Lib); break;
     }
 }
 
-// B (nrows×nb) <- B L⁻ᵀ for the lower nb×nb L (nb ≤ 128): each workgroup
-// takes 64 rows; wave w holds tile row w of them (8 tiles, columns 0..7).
-// Column block kb: L's tiles (j, kb) for j ≥ kb come from global memory into
-// LDS, the rows' tile kb solves against L_kk, and tiles j > kb take
-// B_j -= X_kb L_(j,kb)ᵀ on MFMA.
+// Inverses of the 16×16 diagonal tiles of a lower n×n L (tiles past n are
+// padded with the identity); one wave per tile.
+template <typename T>
+__global__ __launch_bounds__(256) void tile_inverse_kernel(int64_t n, const T *__restrict__ L,
+                                                           int64_t ldl, T *__restrict__ LinvG)
+{
+    const int64_t tile = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (tile * 16 >= n) return;
+    const int lane = threadIdx.x & 63;
+    const int i = lane & 15, g = lane >> 4;
+    T v[4], w[4], rd[16];
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+        const int j = g + 4 * t;
+        const int64_t gi = tile * 16 + i, gj = tile * 16 + j;
+        v[t] = (j <= i) ? (gi < n ? L[gi + gj * ldl] : (i == j ? T(1) : T(0))) : T(0);
+    }
+#pragma unroll
+    for (int k = 0; k < 16; ++k) rd[k] = T(1) / __shfl(v[k >> 2], k + 16 * (k & 3));
+    tile_inverse<T>(v, rd, w);
+#pragma unroll
+    for (int t = 0; t < 4; ++t) LinvG[tile * 256 + i * 16 + g + 4 * t] = w[t];
+}
+
+// B (nrows×nb) <- B L⁻ᵀ for the lower nb×nb L (nb ≤ 128) whose 16×16
+// diagonal-tile inverses are in LinvG: each workgroup takes 64 rows; wave w
+// holds tile row w of them (8 tiles, columns 0..7).  Column block kb: L's
+// tiles (j, kb), j > kb, come into LDS; the rows' tile kb becomes
+// X = B_kb L_kk⁻ᵀ and tiles j > kb take B_j -= X L_(j,kb)ᵀ, both on MFMA.
+// rdiv!(A21, LowerTriangular(A11)') — src/cholesky.jl:28,48.
 template <typename T>
 __global__ __launch_bounds__(kPanelThreads) void chol_solve_kernel(int nb, const T *__restrict__ L,
-                                                                   int64_t ldl, T *__restrict__ B,
-                                                                   int64_t ldb, int64_t nrows)
+                                                                   int64_t ldl,
+                                                                   const T *__restrict__ LinvG,
+                                                                   T *__restrict__ B, int64_t ldb,
+                                                                   int64_t nrows)
 {
     typedef PanelMfma<T> M;
     typedef typename M::acc_t acc_t;
     constexpr int NBLK = kPanelNB / 16;
     constexpr int XR = kPanelNB + kPanelRB;        // rows: L column block, then B's
     __shared__ T Xb[2 * XR * kXLD];
-    __shared__ T Dinv[2 * 16];
+    __shared__ T Lib[2 * 16 * kXLD];
     const int lane = threadIdx.x & 63;
     const int w = threadIdx.x >> 6;
     const int lc = lane & 15;
@@ -265,40 +329,46 @@ __global__ __launch_bounds__(kPanelThreads) void chol_solve_kernel(int nb, const
     const int nkb = (nb + 15) / 16;
     for (int kb = 0; kb < nkb; ++kb) {
         T *X = Xb + (kb & 1) * (XR * kXLD);
-        T *Di = Dinv + (kb & 1) * 16;
-        // L[kb*16 : nkb*16, kb*16 : kb*16+16] into X rows (identity padded)
-        for (int e = threadIdx.x; e < (nkb - kb) * 16 * 16; e += kPanelThreads) {
-            const int i = kb * 16 + (e >> 4), j = kb * 16 + (e & 15);
-            T v = T(0);
-            if (i < nb && j < nb) v = j <= i ? L[i + (int64_t)j * ldl] : T(0);
-            else if (i == j) v = T(1);
-            X[i * kXLD + (e & 15)] = v;
-            if (i == j) Di[e & 15] = T(1) / v;
+        T *Li = Lib + (kb & 1) * (16 * kXLD);
+        // L[(kb+1)*16 : nkb*16, kb*16 : kb*16+16] into X rows, the tile inverse into Li
+        for (int e = threadIdx.x; e < (nkb - kb - 1) * 256; e += kPanelThreads) {
+            const int i = (kb + 1) * 16 + (e >> 4), j = kb * 16 + (e & 15);
+            X[i * kXLD + (e & 15)] = (i < nb && j < nb) ? L[i + (int64_t)j * ldl] : T(0);
         }
+        Li[(threadIdx.x >> 4) * kXLD + (threadIdx.x & 15)] = LinvG[kb * 256 + threadIdx.x];
+        __syncthreads();
+        // this wave's tile of column kb: X = B_kb L_kk⁻ᵀ
+        T *Xw = X + (kPanelNB + w * 16) * kXLD;
 #pragma unroll
         for (int bj = 0; bj < NBLK; ++bj)
             if (bj == kb)
 #pragma unroll
-                for (int r = 0; r < 4; ++r)
-                    X[(kPanelNB + w * 16 + M::row(lane, r)) * kXLD + lc] = acc[bj][r];
-        __syncthreads();
-        solve_rows<T>(X, X + kb * 16 * kXLD, Di, kPanelNB, XR);
-        __syncthreads();
+                for (int r = 0; r < 4; ++r) Xw[M::row(lane, r) * kXLD + lc] = acc[bj][r];
+        wave_order();   // the wave re-reads its own tile in operand layout
+        acc_t x = {T(0), T(0), T(0), T(0)};
+#pragma unroll
+        for (int kq = 0; kq < 4; ++kq) {
+            const int kk = (lane >> 4) + 4 * kq;
+            x = M::mma(Xw[lc * kXLD + kk], Li[lc * kXLD + kk], x);
+        }
+        wave_order();
+#pragma unroll
+        for (int r = 0; r < 4; ++r) Xw[M::row(lane, r) * kXLD + lc] = x[r];
+#pragma unroll
+        for (int bj = 0; bj < NBLK; ++bj)
+            if (bj == kb) acc[bj] = x;
+        wave_order();
+        // tiles j > kb: B_j -= X L_(j,kb)ᵀ
+        T a[4];
+#pragma unroll
+        for (int kq = 0; kq < 4; ++kq) a[kq] = -Xw[lc * kXLD + (lane >> 4) + 4 * kq];
 #pragma unroll
         for (int bj = 0; bj < NBLK; ++bj) {
-            if (bj < kb || bj >= nkb) continue;
-            if (bj == kb) {
-#pragma unroll
-                for (int r = 0; r < 4; ++r)
-                    acc[bj][r] = X[(kPanelNB + w * 16 + M::row(lane, r)) * kXLD + lc];
-                continue;
-            }
+            if (bj <= kb || bj >= nkb) continue;
 #pragma unroll
             for (int kq = 0; kq < 4; ++kq) {
-                const int kk = (lane >> 4) + 4 * kq;
-                const T a = -X[(kPanelNB + w * 16 + lc) * kXLD + kk];
-                const T b = X[(bj * 16 + lc) * kXLD + kk];
-                acc[bj] = M::mma(a, b, acc[bj]);
+                const T b = X[(bj * 16 + lc) * kXLD + (lane >> 4) + 4 * kq];
+                acc[bj] = M::mma(a[kq], b, acc[bj]);
             }
         }
     }
@@ -328,29 +398,44 @@ __global__ void swap_triangles_kernel(int64_t n, T *A, int64_t lda)
 }
 
 template <typename T>
-int launch_factor(hipStream_t s, int nb, T *A, int64_t lda, int *info, int64_t info_base)
+int launch_factor(hipStream_t s, int nb, T *A, int64_t lda, int *info, int64_t info_base,
+                  T *LinvG)
 {
     hipLaunchKernelGGL((chol_factor_kernel<T>), dim3(1), dim3(kPanelThreads), 0, s, nb, A, lda,
-                       info, info_base);
+                       info, info_base, LinvG);
     GLA_LAUNCH_CHECK();
     return GLA_OK;
 }
 
 template <typename T>
-int launch_solve(hipStream_t s, int nb, const T *L, int64_t ldl, T *B, int64_t ldb,
-                 int64_t nrows)
+int launch_solve(hipStream_t s, int nb, const T *L, int64_t ldl, const T *LinvG, T *B,
+                 int64_t ldb, int64_t nrows)
 {
     if (nrows <= 0) return GLA_OK;
     hipLaunchKernelGGL((chol_solve_kernel<T>), dim3((unsigned)cdiv(nrows, kPanelRB)),
-                       dim3(kPanelThreads), 0, s, nb, L, ldl, B, ldb, nrows);
+                       dim3(kPanelThreads), 0, s, nb, L, ldl, LinvG, B, ldb, nrows);
     GLA_LAUNCH_CHECK();
     return GLA_OK;
 }
+
+template <typename T>
+int launch_tile_inverse(hipStream_t s, int64_t n, const T *L, int64_t ldl, T *LinvG)
+{
+    if (n <= 0) return GLA_OK;
+    hipLaunchKernelGGL((tile_inverse_kernel<T>), dim3((unsigned)cdiv(cdiv(n, 16), 4)), dim3(256),
+                       0, s, n, L, ldl, LinvG);
+    GLA_LAUNCH_CHECK();
+    return GLA_OK;
+}
+
+// Elements of the per-call workspace: the 16×16 diagonal-tile inverses of
+// one 128-wide panel (blocked driver) or of L11 (recursive rdiv!).
+inline size_t linv_elems(int64_t n) { return (size_t)(cdiv(n, 16) + 8) * 256; }
 
 // Right-looking blocked Cholesky of the lower triangle with look-ahead.
 template <typename T>
 int chol_blocked_dev(Handle *h, hipStream_t s, int64_t n, T *A, int64_t lda, int nb,
-                     int64_t info_base)
+                     int64_t info_base, T *Linv)
 {
     if (n <= 0) return GLA_OK;
     const int64_t nsteps = cdiv(n, nb);
@@ -365,8 +450,8 @@ int chol_blocked_dev(Handle *h, hipStream_t s, int64_t n, T *A, int64_t lda, int
         const int64_t below = n - k - kb;
         T *Akk = A + k + k * lda;
         T *A21 = Akk + kb;
-        GLA_TRY(launch_factor<T>(s, kb, Akk, lda, h->dinfo, info_base + k));     // :25
-        GLA_TRY(launch_solve<T>(s, kb, Akk, lda, A21, lda, below));             // :27-28
+        GLA_TRY(launch_factor<T>(s, kb, Akk, lda, h->dinfo, info_base + k, Linv));  // :25
+        GLA_TRY(launch_solve<T>(s, kb, Akk, lda, Linv, A21, lda, below));           // :27-28
         if (below == 0) break;
         const int kb2 = (int)std::min<int64_t>(nb, below);
         const int64_t rest = below - kb2;
@@ -392,38 +477,41 @@ int chol_blocked_dev(Handle *h, hipStream_t s, int64_t n, T *A, int64_t lda, int
     return GLA_OK;
 }
 
-// B (m×n) <- B L⁻ᵀ, L n×n lower, already factored.  Recursive on L.
+// B (m×n) <- B L⁻ᵀ, L n×n lower, already factored, with the inverses of its
+// 16×16 diagonal tiles in Linv.  Recursive on L: two half solves and a GEMM.
 template <typename T>
-int trsm_rlt_dev(hipStream_t s, int64_t m, int64_t n, const T *L, int64_t ldl, T *B,
-                 int64_t ldb)
+int trsm_rlt_dev(hipStream_t s, int64_t m, int64_t n, const T *L, int64_t ldl, const T *Linv,
+                 T *B, int64_t ldb)
 {
     if (m <= 0 || n <= 0) return GLA_OK;
-    if (n <= kPanelNB) return launch_solve<T>(s, (int)n, L, ldl, B, ldb, m);
+    if (n <= kPanelNB) return launch_solve<T>(s, (int)n, L, ldl, Linv, B, ldb, m);
     // split at a multiple of the panel width
     int64_t n1 = ((n / 2) + kPanelNB - 1) / kPanelNB * kPanelNB;
     if (n1 >= n) n1 = n - kPanelNB;
-    GLA_TRY(trsm_rlt_dev<T>(s, m, n1, L, ldl, B, ldb));
+    GLA_TRY(trsm_rlt_dev<T>(s, m, n1, L, ldl, Linv, B, ldb));
     // B2 -= X1 * L21ᵀ, L21 = L[n1:n, 0:n1]
     GLA_TRY(gemm<T>(s, 'N', 'T', 0, m, n - n1, n1, T(-1), B, ldb, L + n1, ldl, T(1),
                     B + n1 * ldb, ldb, 0));
-    return trsm_rlt_dev<T>(s, m, n - n1, L + n1 + n1 * ldl, ldl, B + n1 * ldb, ldb);
+    return trsm_rlt_dev<T>(s, m, n - n1, L + n1 + n1 * ldl, ldl, Linv + (n1 / 16) * 256,
+                           B + n1 * ldb, ldb);
 }
 
 // cholRecursive! (src/cholesky.jl:37-55): halve until n <= leaf, then the
 // blocked driver.
 template <typename T>
 int chol_recursive_dev(Handle *h, hipStream_t s, int64_t n, T *A, int64_t lda, int64_t leaf,
-                       int64_t info_base)
+                       int64_t info_base, T *Linv)
 {
-    if (n <= leaf) return chol_blocked_dev<T>(h, s, n, A, lda, kPanelNB, info_base);
+    if (n <= leaf) return chol_blocked_dev<T>(h, s, n, A, lda, kPanelNB, info_base, Linv);
     int64_t n2 = n / 2;                                              // :44
     if (n2 > kPanelNB) n2 = n2 / kPanelNB * kPanelNB;                // keep blocks aligned
-    GLA_TRY(chol_recursive_dev<T>(h, s, n2, A, lda, leaf, info_base));   // :46
-    GLA_TRY(trsm_rlt_dev<T>(s, n - n2, n2, A, lda, A + n2, lda));    // :48 rdiv!
+    GLA_TRY(chol_recursive_dev<T>(h, s, n2, A, lda, leaf, info_base, Linv));   // :46
+    GLA_TRY(launch_tile_inverse<T>(s, n2, A, lda, Linv));            // :48 rdiv!
+    GLA_TRY(trsm_rlt_dev<T>(s, n - n2, n2, A, lda, Linv, A + n2, lda));
     GLA_TRY(gemm<T>(s, 'N', 'T', 1, n - n2, n - n2, n2, T(-1), A + n2, lda, A + n2, lda, T(1),
                     A + n2 + n2 * lda, lda, 0));                     // :51 rankUpdate!
     return chol_recursive_dev<T>(h, s, n - n2, A + n2 + n2 * lda, lda, leaf,
-                                 info_base + n2);                    // :52
+                                 info_base + n2, Linv);              // :52
 }
 
 template <typename T>
@@ -442,20 +530,22 @@ template <typename T>
 int chol_device(Handle *h, hipStream_t s, int64_t n, T *A, int64_t lda, int algo, int64_t param)
 {
     if (n == 0) return GLA_OK;
+    GLA_TRY(ensure_workspace(h, sizeof(T) * linv_elems(n)));
+    T *Linv = reinterpret_cast<T *>(h->ws);
     if (algo == 0) {
         // cholUnblocked!: one factor launch when the matrix fits the 128×128
         // block, otherwise the same column sweep blocked by 128.
-        return chol_blocked_dev<T>(h, s, n, A, lda, kPanelNB, 0);
+        return chol_blocked_dev<T>(h, s, n, A, lda, kPanelNB, 0, Linv);
     }
     if (algo == 1) {
         int nb = (int)std::min<int64_t>(std::max<int64_t>(param, 1), kPanelNB);
-        return chol_blocked_dev<T>(h, s, n, A, lda, nb, 0);
+        return chol_blocked_dev<T>(h, s, n, A, lda, nb, 0, Linv);
     }
     // cholRecursive!: the reference recurses to 1×1 (cutoff does not reach
     // the A11 branch, :46); on the device the recursion stops at a leaf of
     // at least 2048 (or `cutoff` if larger) that the blocked driver factors.
     int64_t leaf = std::max<int64_t>(param, 2048);
-    return chol_recursive_dev<T>(h, s, n, A, lda, leaf, 0);
+    return chol_recursive_dev<T>(h, s, n, A, lda, leaf, 0, Linv);
 }
 
 template int chol_device<double>(Handle *, hipStream_t, int64_t, double *, int64_t, int, int64_t);
@@ -524,7 +614,11 @@ int trsm_entry(gla_handle_t hh, int64_t m, int64_t n, const T *L, int64_t ldl, T
     if (ldb < std::max<int64_t>(1, m)) { set_error("ldb < max(1,m)"); return -7; }
     hipStream_t s = reinterpret_cast<hipStream_t>(stream);
     GLA_HIP_TRY(hipSetDevice(h->device));
-    return trsm_rlt_dev<T>(s, m, n, L, ldl, B, ldb);
+    if (m == 0 || n == 0) return GLA_OK;
+    GLA_TRY(ensure_workspace(h, sizeof(T) * linv_elems(n)));
+    T *Linv = reinterpret_cast<T *>(h->ws);
+    GLA_TRY(launch_tile_inverse<T>(s, n, L, ldl, Linv));
+    return trsm_rlt_dev<T>(s, m, n, L, ldl, Linv, B, ldb);
 }
 
 }  // namespace
