@@ -293,11 +293,12 @@ __global__ __launch_bounds__(256) void tile_inverse_kernel(int64_t n, const T *_
 }
 
 // B (nrows×nb) <- B L⁻ᵀ for the lower nb×nb L (nb ≤ 128) whose 16×16
-// diagonal-tile inverses are in LinvG: each workgroup takes 64 rows; wave w
-// holds tile row w of them (8 tiles, columns 0..7).  Column block kb: L's
-// tiles (j, kb), j > kb, come into LDS; the rows' tile kb becomes
-// X = B_kb L_kk⁻ᵀ and tiles j > kb take B_j -= X L_(j,kb)ᵀ, both on MFMA.
-// rdiv!(A21, LowerTriangular(A11)') — src/cholesky.jl:28,48.
+// diagonal-tile inverses are in LinvG — rdiv!(A21, LowerTriangular(A11)'),
+// src/cholesky.jl:28,48.  Each workgroup takes 64 rows; wave w holds tile row
+// w of them (8 tiles).  L's 28 strictly-lower tiles and the 8 inverses are
+// read into LDS once; then for each column block kb the wave's tile kb
+// becomes X = B_kb L_kk⁻ᵀ and its tiles j > kb take B_j -= X L_(j,kb)ᵀ, all
+// on MFMA, with no further barrier (every wave touches only its own rows).
 template <typename T>
 __global__ __launch_bounds__(kPanelThreads) void chol_solve_kernel(int nb, const T *__restrict__ L,
                                                                    int64_t ldl,
@@ -307,14 +308,34 @@ __global__ __launch_bounds__(kPanelThreads) void chol_solve_kernel(int nb, const
 {
     typedef PanelMfma<T> M;
     typedef typename M::acc_t acc_t;
-    constexpr int NBLK = kPanelNB / 16;
-    constexpr int XR = kPanelNB + kPanelRB;        // rows: L column block, then B's
-    __shared__ T Xb[2 * XR * kXLD];
-    __shared__ T Lib[2 * 16 * kXLD];
+    constexpr int NBLK = kPanelNB / 16;             // 8
+    constexpr int NLT = NBLK * (NBLK - 1) / 2;       // 28 strictly-lower tiles
+    constexpr int TS = 16 * kXLD;                    // one padded tile
+    __shared__ T Lt[NLT * TS];                       // tile (bj, kb), bj > kb
+    __shared__ T Li[NBLK * TS];                      // inverses of the diagonal tiles
+    __shared__ T Xs[4 * TS];                         // one tile per wave
     const int lane = threadIdx.x & 63;
     const int w = threadIdx.x >> 6;
     const int lc = lane & 15;
     const int64_t row0 = (int64_t)blockIdx.x * kPanelRB;
+    const int nkb = (nb + 15) / 16;
+    // index of tile (bj, kb), bj > kb, in Lt: column blocks stored one after another
+    auto lt_index = [](int bj, int kb) { return 7 * kb - kb * (kb - 1) / 2 + (bj - kb - 1); };
+
+    for (int e = threadIdx.x; e < NLT * 256; e += kPanelThreads) {
+        const int tile = e >> 8, i = (e >> 4) & 15, j = e & 15;
+        // recover (bj, kb) of the tile
+        int kb = 0, base = 0;
+        while (base + (7 - kb) <= tile) {
+            base += 7 - kb;
+            ++kb;
+        }
+        const int bj = kb + 1 + (tile - base);
+        const int gi = bj * 16 + i, gj = kb * 16 + j;
+        Lt[tile * TS + i * kXLD + j] = (gi < nb && gj < nb) ? L[gi + (int64_t)gj * ldl] : T(0);
+    }
+    for (int e = threadIdx.x; e < nkb * 256; e += kPanelThreads)
+        Li[(e >> 8) * TS + ((e >> 4) & 15) * kXLD + (e & 15)] = LinvG[e];
 
     acc_t acc[NBLK];
 #pragma unroll
@@ -325,20 +346,12 @@ __global__ __launch_bounds__(kPanelThreads) void chol_solve_kernel(int nb, const
             const int gj = bj * 16 + lc;
             acc[bj][r] = (gr < nrows && gj < nb) ? B[gr + (int64_t)gj * ldb] : T(0);
         }
+    __syncthreads();
 
-    const int nkb = (nb + 15) / 16;
+    T *Xw = Xs + w * TS;
     for (int kb = 0; kb < nkb; ++kb) {
-        T *X = Xb + (kb & 1) * (XR * kXLD);
-        T *Li = Lib + (kb & 1) * (16 * kXLD);
-        // L[(kb+1)*16 : nkb*16, kb*16 : kb*16+16] into X rows, the tile inverse into Li
-        for (int e = threadIdx.x; e < (nkb - kb - 1) * 256; e += kPanelThreads) {
-            const int i = (kb + 1) * 16 + (e >> 4), j = kb * 16 + (e & 15);
-            X[i * kXLD + (e & 15)] = (i < nb && j < nb) ? L[i + (int64_t)j * ldl] : T(0);
-        }
-        Li[(threadIdx.x >> 4) * kXLD + (threadIdx.x & 15)] = LinvG[kb * 256 + threadIdx.x];
-        __syncthreads();
+        const T *Lik = Li + kb * TS;
         // this wave's tile of column kb: X = B_kb L_kk⁻ᵀ
-        T *Xw = X + (kPanelNB + w * 16) * kXLD;
 #pragma unroll
         for (int bj = 0; bj < NBLK; ++bj)
             if (bj == kb)
@@ -346,10 +359,11 @@ __global__ __launch_bounds__(kPanelThreads) void chol_solve_kernel(int nb, const
                 for (int r = 0; r < 4; ++r) Xw[M::row(lane, r) * kXLD + lc] = acc[bj][r];
         wave_order();   // the wave re-reads its own tile in operand layout
         acc_t x = {T(0), T(0), T(0), T(0)};
+        T a[4];
 #pragma unroll
         for (int kq = 0; kq < 4; ++kq) {
             const int kk = (lane >> 4) + 4 * kq;
-            x = M::mma(Xw[lc * kXLD + kk], Li[lc * kXLD + kk], x);
+            x = M::mma(Xw[lc * kXLD + kk], Lik[lc * kXLD + kk], x);
         }
         wave_order();
 #pragma unroll
@@ -358,19 +372,20 @@ __global__ __launch_bounds__(kPanelThreads) void chol_solve_kernel(int nb, const
         for (int bj = 0; bj < NBLK; ++bj)
             if (bj == kb) acc[bj] = x;
         wave_order();
-        // tiles j > kb: B_j -= X L_(j,kb)ᵀ
-        T a[4];
 #pragma unroll
         for (int kq = 0; kq < 4; ++kq) a[kq] = -Xw[lc * kXLD + (lane >> 4) + 4 * kq];
+        // tiles j > kb: B_j -= X L_(j,kb)ᵀ
 #pragma unroll
         for (int bj = 0; bj < NBLK; ++bj) {
             if (bj <= kb || bj >= nkb) continue;
+            const T *Lt_jk = Lt + lt_index(bj, kb) * TS;
 #pragma unroll
             for (int kq = 0; kq < 4; ++kq) {
-                const T b = X[(bj * 16 + lc) * kXLD + (lane >> 4) + 4 * kq];
+                const T b = Lt_jk[lc * kXLD + (lane >> 4) + 4 * kq];
                 acc[bj] = M::mma(a[kq], b, acc[bj]);
             }
         }
+        wave_order();   // Xw is rewritten in the next step
     }
 #pragma unroll
     for (int bj = 0; bj < NBLK; ++bj)
