@@ -540,13 +540,53 @@ int swap_triangles(hipStream_t s, int64_t n, T *A, int64_t lda)
 
 }  // namespace
 
+// Building blocks shared with the mixed-precision driver.
+template <typename T>
+int chol_factor_block(hipStream_t s, int nb, T *A, int64_t lda, int *info, int64_t info_base,
+                      T *LinvG)
+{
+    return launch_factor<T>(s, nb, A, lda, info, info_base, LinvG);
+}
+template <typename T>
+int chol_solve_block(hipStream_t s, int nb, const T *L, int64_t ldl, const T *LinvG, T *B,
+                     int64_t ldb, int64_t nrows)
+{
+    return launch_solve<T>(s, nb, L, ldl, LinvG, B, ldb, nrows);
+}
+template <typename T>
+int tile_inverses(hipStream_t s, int64_t n, const T *L, int64_t ldl, T *LinvG)
+{
+    return launch_tile_inverse<T>(s, n, L, ldl, LinvG);
+}
+template <typename T>
+int trsm_rlt(hipStream_t s, int64_t m, int64_t n, const T *L, int64_t ldl, const T *Linv, T *B,
+             int64_t ldb)
+{
+    return trsm_rlt_dev<T>(s, m, n, L, ldl, Linv, B, ldb);
+}
+size_t tile_inverse_elems(int64_t n) { return linv_elems(n); }
+
+#define GLA_INST_BLOCKS(T)                                                                    \
+    template int chol_factor_block<T>(hipStream_t, int, T *, int64_t, int *, int64_t, T *);  \
+    template int chol_solve_block<T>(hipStream_t, int, const T *, int64_t, const T *, T *,   \
+                                     int64_t, int64_t);                                      \
+    template int tile_inverses<T>(hipStream_t, int64_t, const T *, int64_t, T *);            \
+    template int trsm_rlt<T>(hipStream_t, int64_t, int64_t, const T *, int64_t, const T *,   \
+                             T *, int64_t);
+GLA_INST_BLOCKS(double)
+GLA_INST_BLOCKS(float)
+#undef GLA_INST_BLOCKS
+
 // algo: 0 unblocked, 1 blocked (param = blocksize), 2 recursive (param = cutoff)
 template <typename T>
-int chol_device(Handle *h, hipStream_t s, int64_t n, T *A, int64_t lda, int algo, int64_t param)
+int chol_device(Handle *h, hipStream_t s, int64_t n, T *A, int64_t lda, int algo, int64_t param,
+                T *Linv)
 {
     if (n == 0) return GLA_OK;
-    GLA_TRY(ensure_workspace(h, sizeof(T) * linv_elems(n)));
-    T *Linv = reinterpret_cast<T *>(h->ws);
+    if (!Linv) {
+        GLA_TRY(ensure_workspace(h, sizeof(T) * linv_elems(n)));
+        Linv = reinterpret_cast<T *>(h->ws);
+    }
     if (algo == 0) {
         // cholUnblocked!: one factor launch when the matrix fits the 128×128
         // block, otherwise the same column sweep blocked by 128.
@@ -563,8 +603,10 @@ int chol_device(Handle *h, hipStream_t s, int64_t n, T *A, int64_t lda, int algo
     return chol_recursive_dev<T>(h, s, n, A, lda, leaf, 0, Linv);
 }
 
-template int chol_device<double>(Handle *, hipStream_t, int64_t, double *, int64_t, int, int64_t);
-template int chol_device<float>(Handle *, hipStream_t, int64_t, float *, int64_t, int, int64_t);
+template int chol_device<double>(Handle *, hipStream_t, int64_t, double *, int64_t, int, int64_t,
+                                 double *);
+template int chol_device<float>(Handle *, hipStream_t, int64_t, float *, int64_t, int, int64_t,
+                                float *);
 
 // ------------------------------------------------------------------ C ABI --
 namespace {

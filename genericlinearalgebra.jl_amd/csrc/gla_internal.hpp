@@ -69,9 +69,28 @@ int gemm(hipStream_t s, char ta, char tb, int mode, int64_t M, int64_t N, int64_
          int64_t diag_off = 0, T *ws = nullptr, size_t ws_elems = 0);
 
 // ------------------------------------------------------------- Cholesky --
+// Linv: scratch for the 16×16 diagonal-tile inverses (tile_inverse_elems(n)
+// elements); null takes it from the handle's workspace.
 template <typename T>
 int chol_device(Handle *h, hipStream_t s, int64_t n, T *A, int64_t lda, int algo,
-                int64_t param);
+                int64_t param, T *Linv = nullptr);
+
+// Cholesky building blocks (chol.hip): factor one ≤128 diagonal block in
+// place and store the inverses of its 16×16 diagonal tiles; solve rows
+// below it (B <- B L⁻ᵀ) with those inverses; inverses of all 16×16 diagonal
+// tiles of an n×n lower L; the recursive right-side solve B <- B L⁻ᵀ.
+template <typename T>
+int chol_factor_block(hipStream_t s, int nb, T *A, int64_t lda, int *info, int64_t info_base,
+                      T *LinvG);
+template <typename T>
+int chol_solve_block(hipStream_t s, int nb, const T *L, int64_t ldl, const T *LinvG, T *B,
+                     int64_t ldb, int64_t nrows);
+template <typename T>
+int tile_inverses(hipStream_t s, int64_t n, const T *L, int64_t ldl, T *LinvG);
+template <typename T>
+int trsm_rlt(hipStream_t s, int64_t m, int64_t n, const T *L, int64_t ldl, const T *Linv, T *B,
+             int64_t ldb);
+size_t tile_inverse_elems(int64_t n);
 
 // Small helpers used across units.
 __global__ void k_set_int(int *p, int v);

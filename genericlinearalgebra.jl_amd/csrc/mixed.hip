@@ -1,0 +1,638 @@
+// Mixed-precision Cholesky solve with FP64 iterative refinement.
+// No counterpart in the reference (its Cholesky runs in one precision,
+// src/cholesky.jl).  A x = B for SPD A (n×n, FP64) and nrhs right-hand sides:
+//
+//   factor   L Lᵀ ≈ A in a working copy W:
+//     'd'  FP64 throughout (the plain FP64 path, cholRecursive! on device);
+//     's'  FP32 throughout;
+//     'b'/'h'  FP32 storage, every 128×128 diagonal block factored in FP64,
+//          the panel solve in FP32, and the trailing SYRK/GEMM updates
+//          (rankUpdate!, src/juliaBLAS.jl:89) on BF16 / FP16 MFMA with FP32
+//          accumulation (v_mfma_f32_16x16x32_{bf16,f16});
+//   refine   r = B − A x in FP64 (MFMA GEMM), x += (L Lᵀ)⁻¹ r with the
+//            low-precision factor, until the normwise backward error
+//            η = max_k ‖r_k‖ / (‖A‖_F ‖x_k‖ + ‖b_k‖) ≤ tol.
+//
+// The triangular solves reuse the right-side solve B <- B L⁻ᵀ of the
+// Cholesky driver on the transposed right-hand sides: L y = r is
+// yᵀ = rᵀ L⁻ᵀ; Lᵀ z = y is a lower solve with the reversed matrix
+// L' = P Lᵀ P (P the reversal), z = P (L'⁻¹ (P y)).
+#include <algorithm>
+#include <cmath>
+#include <vector>
+
+#include "gla_internal.hpp"
+
+namespace gla {
+
+namespace {
+
+#define GLA_TRY(expr)                  \
+    do {                               \
+        int _rc = (expr);              \
+        if (_rc != GLA_OK) return _rc; \
+    } while (0)
+
+// ---------------------------------------------------- low-precision GEMM --
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef _Float16 f16x8 __attribute__((ext_vector_type(8)));
+
+template <typename TIN>
+struct LpMfma;
+template <>
+struct LpMfma<__bf16> {
+    typedef bf16x8 v8;
+    static __device__ __forceinline__ f4 mma(v8 a, v8 b, f4 c)
+    {
+        return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
+    }
+};
+template <>
+struct LpMfma<_Float16> {
+    typedef f16x8 v8;
+    static __device__ __forceinline__ f4 mma(v8 a, v8 b, f4 c)
+    {
+        return __builtin_amdgcn_mfma_f32_16x16x32_f16(a, b, c, 0, 0, 0);
+    }
+};
+
+struct LpP {
+    int64_t M, N, K;
+    const float *A;
+    int64_t lda;
+    const float *B;
+    int64_t ldb;
+    float *C;
+    int64_t ldc;
+    float alpha;   // ±1: C <- C + alpha A Bᵀ
+    int mode;      // 0 all, 1 lower (row >= col)
+    int tiles_m, tiles_n, total, per_xcd;
+};
+
+constexpr int kLpT = 256, kLpBM = 128, kLpBK = 32, kLpLD = kLpBK + 8;
+
+__device__ __forceinline__ void lp_tile_range(int c, int tiles_m, int mode, int &first, int &cnt)
+{
+    if (mode == 0) {
+        first = 0;
+        cnt = tiles_m;
+        return;
+    }
+    const int64_t need = (int64_t)c * kLpBM - (kLpBM - 1);
+    first = need <= 0 ? 0 : (int)((need + kLpBM - 1) / kLpBM);
+    cnt = tiles_m - first;
+    if (cnt < 0) cnt = 0;
+}
+
+// C (M×N, FP32) += alpha · A Bᵀ with A (M×K) and B (N×K) column-major FP32,
+// both rounded to TIN as they are staged into LDS; FP32 accumulation.
+template <typename TIN>
+__global__ __launch_bounds__(kLpT, 2) void lp_gemm_nt_kernel(LpP p)
+{
+    typedef typename LpMfma<TIN>::v8 v8;
+    __shared__ __attribute__((aligned(16))) TIN As[2][kLpBM * kLpLD];
+    __shared__ __attribute__((aligned(16))) TIN Bs[2][kLpBM * kLpLD];
+
+    const int b = blockIdx.x;
+    const int tid = (b % 8) * p.per_xcd + b / 8;
+    if (tid >= p.total) return;
+    int tm = 0, tn = 0;
+    {
+        int rem = tid;
+        for (int c = 0; c < p.tiles_n; ++c) {
+            int first, cnt;
+            lp_tile_range(c, p.tiles_m, p.mode, first, cnt);
+            if (rem < cnt) {
+                tm = first + rem;
+                tn = c;
+                break;
+            }
+            rem -= cnt;
+        }
+    }
+    const int64_t m0 = (int64_t)tm * kLpBM, n0 = (int64_t)tn * kLpBM;
+    const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+    const int wm = wave & 1, wn = wave >> 1;
+
+    // accumulators start at alpha·C (alpha = ±1, exact)
+    f4 acc[4][4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            acc[i][j] = f4{0.f, 0.f, 0.f, 0.f};
+            const int64_t row = m0 + wm * 64 + i * 16 + 4 * (lane >> 4);
+            const int64_t col = n0 + wn * 64 + j * 16 + (lane & 15);
+#pragma unroll
+            for (int r = 0; r < 4; ++r)
+                if (row + r < p.M && col < p.N) acc[i][j][r] = p.alpha * p.C[row + r + col * p.ldc];
+        }
+
+    // staging: thread t loads rows 4*(t%32)..+3 of k-columns t/32 + 8q, q < 4
+    float ra[16], rb[16];
+    auto gload = [&](const float *X, int64_t ldx, int64_t x0, int64_t xlim, int64_t k0, float *r) {
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int64_t gk = k0 + t / 32 + 8 * q;
+            const int64_t gx = x0 + 4 * (t % 32);
+#pragma unroll
+            for (int e = 0; e < 4; ++e)
+                r[4 * q + e] = (gk < p.K && gx + e < xlim) ? X[gx + e + gk * ldx] : 0.f;
+        }
+    };
+    auto lstore = [&](TIN *S, const float *r) {
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+#pragma unroll
+            for (int e = 0; e < 4; ++e)
+                S[(4 * (t % 32) + e) * kLpLD + t / 32 + 8 * q] = (TIN)r[4 * q + e];
+    };
+
+    const int nk = (int)((p.K + kLpBK - 1) / kLpBK);
+    if (nk > 0) {
+        gload(p.A, p.lda, m0, p.M, 0, ra);
+        gload(p.B, p.ldb, n0, p.N, 0, rb);
+        lstore(As[0], ra);
+        lstore(Bs[0], rb);
+        __syncthreads();
+    }
+    for (int kt = 0; kt < nk; ++kt) {
+        const int cur = kt & 1;
+        const bool more = kt + 1 < nk;
+        if (more) {
+            gload(p.A, p.lda, m0, p.M, (int64_t)(kt + 1) * kLpBK, ra);
+            gload(p.B, p.ldb, n0, p.N, (int64_t)(kt + 1) * kLpBK, rb);
+        }
+        v8 a[4], bb[4];
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+            a[i] = *reinterpret_cast<const v8 *>(&As[cur][(wm * 64 + i * 16 + (lane & 15)) * kLpLD +
+                                                          8 * (lane >> 4)]);
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+            bb[j] = *reinterpret_cast<const v8 *>(&Bs[cur][(wn * 64 + j * 16 + (lane & 15)) * kLpLD +
+                                                           8 * (lane >> 4)]);
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+#pragma unroll
+            for (int j = 0; j < 4; ++j) acc[i][j] = LpMfma<TIN>::mma(a[i], bb[j], acc[i][j]);
+        if (more) {
+            lstore(As[cur ^ 1], ra);
+            lstore(Bs[cur ^ 1], rb);
+        }
+        __syncthreads();
+    }
+
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const int64_t row = m0 + wm * 64 + i * 16 + 4 * (lane >> 4);
+            const int64_t col = n0 + wn * 64 + j * 16 + (lane & 15);
+            if (col >= p.N) continue;
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int64_t gr = row + r;
+                if (gr < p.M && (p.mode == 0 || gr >= col)) p.C[gr + col * p.ldc] = p.alpha * acc[i][j][r];
+            }
+        }
+}
+
+template <typename TIN>
+int lp_gemm_nt(hipStream_t s, int mode, int64_t M, int64_t N, int64_t K, float alpha,
+               const float *A, int64_t lda, const float *B, int64_t ldb, float *C, int64_t ldc)
+{
+    if (M <= 0 || N <= 0) return GLA_OK;
+    LpP p{};
+    p.M = M;
+    p.N = N;
+    p.K = K;
+    p.A = A;
+    p.lda = lda;
+    p.B = B;
+    p.ldb = ldb;
+    p.C = C;
+    p.ldc = ldc;
+    p.alpha = alpha;
+    p.mode = mode;
+    p.tiles_m = (int)cdiv(M, kLpBM);
+    p.tiles_n = (int)cdiv(N, kLpBM);
+    int64_t tot = 0;
+    for (int c = 0; c < p.tiles_n; ++c) {
+        int first, cnt;
+        if (mode == 0) {
+            cnt = p.tiles_m;
+        } else {
+            const int64_t need = (int64_t)c * kLpBM - (kLpBM - 1);
+            first = need <= 0 ? 0 : (int)((need + kLpBM - 1) / kLpBM);
+            cnt = std::max(0, p.tiles_m - first);
+        }
+        tot += cnt;
+    }
+    p.total = (int)tot;
+    if (p.total == 0) return GLA_OK;
+    p.per_xcd = (int)cdiv(p.total, 8);
+    hipLaunchKernelGGL((lp_gemm_nt_kernel<TIN>), dim3(8 * p.per_xcd), dim3(kLpT), 0, s, p);
+    GLA_LAUNCH_CHECK();
+    return GLA_OK;
+}
+
+// ------------------------------------------------------------- helpers --
+// Element-wise kernels over an m×ncols index space: x covers rows, y strides
+// over columns (the grid's y extent is capped at 65535).
+#define GLA_COLS(j, ncols) for (int64_t j = blockIdx.y; j < (ncols); j += gridDim.y)
+#define GLA_ROW(i) const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x
+
+// dst (FP32, lower triangle) <- src (FP64, lower triangle), n×n
+__global__ void cvt_lower_d2s(int64_t n, const double *src, int64_t lds, float *dst, int64_t ldd)
+{
+    GLA_ROW(i);
+    GLA_COLS(j, n) if (i < n && i >= j) dst[i + j * ldd] = (float)src[i + j * lds];
+}
+template <typename TD, typename TS>
+__global__ void cvt_block(int nb, const TS *src, int64_t lds, TD *dst, int64_t ldd)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y;
+    if (i < nb && i >= j) dst[i + (int64_t)j * ldd] = (TD)src[i + (int64_t)j * lds];
+}
+template <typename TD, typename TS>
+__global__ void cvt_vec(int64_t n, const TS *src, TD *dst)
+{
+    GLA_ROW(i);
+    if (i < n) dst[i] = (TD)src[i];
+}
+// Lr[i][j] = L[n-1-j][n-1-i] for i >= j: the reversal P Lᵀ P, lower.
+template <typename T>
+__global__ void reverse_transpose_lower(int64_t n, const T *L, int64_t ldl, T *Lr, int64_t ldr)
+{
+    GLA_ROW(i);
+    GLA_COLS(j, n) if (i < n && i >= j) Lr[i + j * ldr] = L[(n - 1 - j) + (n - 1 - i) * ldl];
+}
+// Rt (nrhs × n, type T) <- R (n × nrhs, FP64) transposed, rows optionally reversed
+template <typename T>
+__global__ void rhs_to_t(int64_t n, int64_t nrhs, const double *R, int64_t ldr, T *Rt, int rev)
+{
+    GLA_ROW(i);
+    GLA_COLS(k, nrhs) if (i < n) Rt[k + (rev ? n - 1 - i : i) * nrhs] = (T)R[i + k * ldr];
+}
+// X (n × nrhs, FP64) += Zt (nrhs × n, type T) transposed, rows optionally reversed
+template <typename T>
+__global__ void t_add_rhs(int64_t n, int64_t nrhs, const T *Zt, double *X, int64_t ldx, int rev)
+{
+    GLA_ROW(i);
+    GLA_COLS(k, nrhs) if (i < n) X[i + k * ldx] += (double)Zt[k + (rev ? n - 1 - i : i) * nrhs];
+}
+// Yt (nrhs × n) <- Zt with its columns in reverse order
+template <typename T>
+__global__ void t_reverse(int64_t n, int64_t nrhs, const T *Zt, T *Yt)
+{
+    GLA_ROW(i);
+    GLA_COLS(k, nrhs) if (i < n) Yt[k + (n - 1 - i) * nrhs] = Zt[k + i * nrhs];
+}
+__global__ void copy2d_d(int64_t m, int64_t n, const double *S, int64_t lds, double *D, int64_t ldd)
+{
+    GLA_ROW(i);
+    GLA_COLS(j, n) if (i < m) D[i + j * ldd] = S[i + j * lds];
+}
+__global__ void zero2d_d(int64_t m, int64_t n, double *D, int64_t ldd)
+{
+    GLA_ROW(i);
+    GLA_COLS(j, n) if (i < m) D[i + j * ldd] = 0.0;
+}
+#undef GLA_COLS
+#undef GLA_ROW
+
+inline dim3 g2(int64_t rows, int64_t cols)
+{
+    return dim3((unsigned)cdiv(rows, 256), (unsigned)std::min<int64_t>(cols, 65535));
+}
+
+// out[k] = Σ_i X[i + k*ldx]² over column k (one workgroup per column)
+__global__ __launch_bounds__(256) void col_sumsq(int64_t n, const double *X, int64_t ldx,
+                                                 double *out)
+{
+    __shared__ double red[256];
+    const int64_t k = blockIdx.x;
+    double s = 0.0;
+    for (int64_t i = threadIdx.x; i < n; i += 256) {
+        const double v = X[i + k * ldx];
+        s += v * v;
+    }
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[k] = red[0];
+}
+// out[blockIdx.y] = Σ of squares over the lower triangle weighted to the
+// full symmetric matrix (off-diagonal counted twice): ‖A‖_F² by columns.
+__global__ __launch_bounds__(256) void sym_sumsq_cols(int64_t n, const double *A, int64_t lda,
+                                                      double *out)
+{
+    __shared__ double red[256];
+    const int64_t j = blockIdx.x;
+    double s = 0.0;
+    for (int64_t i = j + threadIdx.x; i < n; i += 256) {
+        const double v = A[i + j * lda];
+        s += (i == j ? 1.0 : 2.0) * v * v;
+    }
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[j] = red[0];
+}
+
+
+// ------------------------------------------------------- mixed factor --
+// Blocked right-looking Cholesky of W (FP32, lower) with FP64 diagonal
+// blocks and low-precision trailing updates, depth-1 look-ahead.
+template <typename TIN>
+int chol_mixed(Handle *h, hipStream_t s, int64_t n, float *W, int64_t ldw, double *D64,
+               double *Linv64, float *Linv32)
+{
+    const int nb = 128;
+    const int64_t nsteps = cdiv(n, nb);
+    const bool overlap = nsteps > 2;
+    if (overlap) {
+        GLA_HIP_TRY(hipEventRecord(h->ev_fork, s));
+        GLA_HIP_TRY(hipStreamWaitEvent(h->side, h->ev_fork, 0));
+    }
+    int64_t k = 0;
+    for (int64_t step = 0; step < nsteps; ++step, k += nb) {
+        const int kb = (int)std::min<int64_t>(nb, n - k);
+        const int64_t below = n - k - kb;
+        float *Wkk = W + k + k * ldw;
+        float *W21 = Wkk + kb;
+        // diagonal block in FP64
+        hipLaunchKernelGGL((cvt_block<double, float>), dim3(1, kb), dim3(128), 0, s, kb, Wkk, ldw,
+                           D64, (int64_t)nb);
+        GLA_LAUNCH_CHECK();
+        GLA_TRY(chol_factor_block<double>(s, kb, D64, nb, h->dinfo, k, Linv64));
+        hipLaunchKernelGGL((cvt_block<float, double>), dim3(1, kb), dim3(128), 0, s, kb, D64,
+                           (int64_t)nb, Wkk, ldw);
+        GLA_LAUNCH_CHECK();
+        hipLaunchKernelGGL((cvt_vec<float, double>), dim3(8), dim3(256), 0, s, (int64_t)8 * 256,
+                           Linv64, Linv32);
+        GLA_LAUNCH_CHECK();
+        GLA_TRY(chol_solve_block<float>(s, kb, Wkk, ldw, Linv32, W21, ldw, below));
+        if (below == 0) break;
+        const int kb2 = (int)std::min<int64_t>(nb, below);
+        const int64_t rest = below - kb2;
+        if (overlap && rest > 0) GLA_HIP_TRY(hipEventRecord(h->ev_step[0], s));
+        if (overlap && step > 0) GLA_HIP_TRY(hipStreamWaitEvent(s, h->ev_step[1], 0));
+        GLA_TRY(lp_gemm_nt<TIN>(s, 1, below, kb2, kb, -1.f, W21, ldw, W21, ldw, W21 + kb * ldw, ldw));
+        if (rest > 0) {
+            hipStream_t us = overlap ? h->side : s;
+            if (overlap) GLA_HIP_TRY(hipStreamWaitEvent(h->side, h->ev_step[0], 0));
+            float *X2 = W21 + kb2;
+            GLA_TRY(lp_gemm_nt<TIN>(us, 1, rest, rest, kb, -1.f, X2, ldw, X2, ldw,
+                                    X2 + (kb + kb2) * ldw, ldw));
+            if (overlap) GLA_HIP_TRY(hipEventRecord(h->ev_step[1], h->side));
+        }
+    }
+    if (overlap) {
+        GLA_HIP_TRY(hipEventRecord(h->ev_join, h->side));
+        GLA_HIP_TRY(hipStreamWaitEvent(s, h->ev_join, 0));
+    }
+    return GLA_OK;
+}
+
+// X += (L Lᵀ)⁻¹ R with L (type T, lower) and its reversal Lr, both with their
+// 16×16 diagonal-tile inverses.  Rt, Yt: nrhs × n scratch of type T.
+template <typename T>
+int correction(hipStream_t s, int64_t n, int64_t nrhs, const T *L, const T *Linv, const T *Lr,
+               const T *Linvr, const double *R, int64_t ldr, double *X, int64_t ldx, T *Rt, T *Yt)
+{
+    hipLaunchKernelGGL((rhs_to_t<T>), g2(n, nrhs), dim3(256), 0, s, n, nrhs, R, ldr, Rt, 0);
+    GLA_LAUNCH_CHECK();
+    GLA_TRY(trsm_rlt<T>(s, nrhs, n, L, n, Linv, Rt, nrhs));          // yᵀ = rᵀ L⁻ᵀ
+    hipLaunchKernelGGL((t_reverse<T>), g2(n, nrhs), dim3(256), 0, s, n, nrhs, Rt, Yt);
+    GLA_LAUNCH_CHECK();
+    GLA_TRY(trsm_rlt<T>(s, nrhs, n, Lr, n, Linvr, Yt, nrhs));        // (P z)ᵀ
+    hipLaunchKernelGGL((t_add_rhs<T>), g2(n, nrhs), dim3(256), 0, s, n, nrhs, Yt, X, ldx, 1);
+    GLA_LAUNCH_CHECK();
+    return GLA_OK;
+}
+
+template <typename T>
+int prepare_solves(hipStream_t s, int64_t n, const T *L, T *Lr, T *Linv, T *Linvr)
+{
+    GLA_TRY(tile_inverses<T>(s, n, L, n, Linv));
+    hipLaunchKernelGGL((reverse_transpose_lower<T>), g2(n, n), dim3(256), 0, s, n, L, n, Lr, n);
+    GLA_LAUNCH_CHECK();
+    return tile_inverses<T>(s, n, Lr, n, Linvr);
+}
+
+// η_k = ‖r_k‖ / (‖A‖ ‖x_k‖ + ‖b_k‖), returned as the max over k.
+double backward_error(const std::vector<double> &rr, const std::vector<double> &xx,
+                      const std::vector<double> &bb, double anorm)
+{
+    double eta = 0.0;
+    for (size_t k = 0; k < rr.size(); ++k) {
+        const double den = anorm * std::sqrt(xx[k]) + std::sqrt(bb[k]);
+        const double e = den > 0 ? std::sqrt(rr[k]) / den : std::sqrt(rr[k]);
+        eta = std::max(eta, e);
+    }
+    return eta;
+}
+
+template <typename T>
+int refine(Handle *h, hipStream_t s, int64_t n, const double *A, int64_t lda, int64_t nrhs,
+           const double *B, int64_t ldb, double *X, int64_t ldx, const T *L, const T *Lr,
+           const T *Linv, const T *Linvr, double *Rbuf, T *Rt, T *Yt, double *norms, double anorm,
+           double tol, int max_iter, int *iters, double *berr)
+{
+    std::vector<double> rr(nrhs), xx(nrhs), bb(nrhs);
+    hipLaunchKernelGGL(col_sumsq, dim3((unsigned)nrhs), dim3(256), 0, s, n, B, ldb, norms);
+    GLA_LAUNCH_CHECK();
+    GLA_HIP_TRY(hipMemcpyAsync(bb.data(), norms, sizeof(double) * nrhs, hipMemcpyDeviceToHost, s));
+    // x0 = 0 + (L Lᵀ)⁻¹ b
+    hipLaunchKernelGGL(zero2d_d, g2(n, nrhs), dim3(256), 0, s, n, nrhs, X, ldx);
+    GLA_LAUNCH_CHECK();
+    hipLaunchKernelGGL(copy2d_d, g2(n, nrhs), dim3(256), 0, s, n, nrhs, B, ldb, Rbuf, n);
+    GLA_LAUNCH_CHECK();
+    GLA_TRY(correction<T>(s, n, nrhs, L, Linv, Lr, Linvr, Rbuf, n, X, ldx, Rt, Yt));
+    int it = 0;
+    double eta = 0.0;
+    for (;;) {
+        // r = b − A x  (FP64 MFMA GEMM)
+        hipLaunchKernelGGL(copy2d_d, g2(n, nrhs), dim3(256), 0, s, n, nrhs, B, ldb, Rbuf, n);
+        GLA_LAUNCH_CHECK();
+        GLA_TRY(gemm<double>(s, 'N', 'N', 0, n, nrhs, n, -1.0, A, lda, X, ldx, 1.0, Rbuf, n));
+        hipLaunchKernelGGL(col_sumsq, dim3((unsigned)nrhs), dim3(256), 0, s, n, Rbuf, n, norms);
+        GLA_LAUNCH_CHECK();
+        hipLaunchKernelGGL(col_sumsq, dim3((unsigned)nrhs), dim3(256), 0, s, n, X, ldx,
+                           norms + nrhs);
+        GLA_LAUNCH_CHECK();
+        GLA_HIP_TRY(hipMemcpyAsync(rr.data(), norms, sizeof(double) * nrhs, hipMemcpyDeviceToHost, s));
+        GLA_HIP_TRY(hipMemcpyAsync(xx.data(), norms + nrhs, sizeof(double) * nrhs,
+                                   hipMemcpyDeviceToHost, s));
+        GLA_HIP_TRY(hipStreamSynchronize(s));
+        eta = backward_error(rr, xx, bb, anorm);
+        if (eta <= tol || it >= max_iter || !std::isfinite(eta)) break;
+        GLA_TRY(correction<T>(s, n, nrhs, L, Linv, Lr, Linvr, Rbuf, n, X, ldx, Rt, Yt));
+        ++it;
+    }
+    if (iters) *iters = it;
+    if (berr) *berr = eta;
+    return GLA_OK;
+}
+
+}  // namespace
+
+// Device workspace layout of one mixed solve, in bytes.
+static size_t mixed_ws_bytes(char prec, int64_t n, int64_t nrhs)
+{
+    const size_t es = prec == 'd' ? 8 : 4;
+    const size_t tiles = tile_inverse_elems(n);
+    return 2 * es * (size_t)n * n              // W, Lr
+           + 2 * es * tiles                     // tile inverses of both
+           + 8 * (size_t)n * nrhs               // FP64 residual
+           + 2 * es * (size_t)n * nrhs          // transposed right-hand sides
+           + 8 * (size_t)(128 * 128 + 8 * 256)  // FP64 diagonal block and its inverses
+           + 4 * 8 * 256                        // FP32 inverses
+           + 8 * (size_t)(2 * nrhs + n + 64)    // norms
+           + 1024;
+}
+
+int mixed_solve(Handle *h, hipStream_t s, char prec, int64_t n, const double *A, int64_t lda,
+                int64_t nrhs, const double *B, int64_t ldb, double *X, int64_t ldx, double tol,
+                int max_iter, int *iters, double *berr, int64_t *info)
+{
+    GLA_TRY(ensure_workspace(h, mixed_ws_bytes(prec, n, nrhs)));
+    char *ws = reinterpret_cast<char *>(h->ws);
+    auto take = [&](size_t bytes) {
+        char *p = ws;
+        ws += (bytes + 255) / 256 * 256;
+        return p;
+    };
+    const size_t es = prec == 'd' ? 8 : 4;
+    void *W = take(es * (size_t)n * n);
+    void *Lr = take(es * (size_t)n * n);
+    void *Linv = take(es * tile_inverse_elems(n));
+    void *Linvr = take(es * tile_inverse_elems(n));
+    double *Rbuf = reinterpret_cast<double *>(take(8 * (size_t)n * nrhs));
+    void *Rt = take(es * (size_t)n * nrhs);
+    void *Yt = take(es * (size_t)n * nrhs);
+    double *D64 = reinterpret_cast<double *>(take(8 * 128 * 128));
+    double *Linv64 = reinterpret_cast<double *>(take(8 * 8 * 256));
+    float *Linv32 = reinterpret_cast<float *>(take(4 * 8 * 256));
+    double *norms = reinterpret_cast<double *>(take(8 * (size_t)(2 * nrhs + n + 64)));
+
+    hipLaunchKernelGGL(k_set_int, dim3(1), dim3(1), 0, s, h->dinfo, 0);
+    GLA_LAUNCH_CHECK();
+    // ‖A‖_F from the lower triangle
+    hipLaunchKernelGGL(sym_sumsq_cols, dim3((unsigned)n), dim3(256), 0, s, n, A, lda,
+                       norms + 2 * nrhs);
+    GLA_LAUNCH_CHECK();
+    std::vector<double> cs(n);
+    GLA_HIP_TRY(hipMemcpyAsync(cs.data(), norms + 2 * nrhs, sizeof(double) * n,
+                               hipMemcpyDeviceToHost, s));
+
+    if (prec == 'd') {
+        double *Wd = reinterpret_cast<double *>(W);
+        hipLaunchKernelGGL(copy2d_d, g2(n, n), dim3(256), 0, s, n, n, A, lda, Wd, n);
+        GLA_LAUNCH_CHECK();
+        GLA_TRY(chol_device<double>(h, s, n, Wd, n, 2, 1, reinterpret_cast<double *>(Linv)));
+    } else {
+        float *Wf = reinterpret_cast<float *>(W);
+        hipLaunchKernelGGL(cvt_lower_d2s, g2(n, n), dim3(256), 0, s, n, A, lda, Wf, n);
+        GLA_LAUNCH_CHECK();
+        if (prec == 's')
+            GLA_TRY(chol_device<float>(h, s, n, Wf, n, 2, 1, reinterpret_cast<float *>(Linv)));
+        else if (prec == 'b') GLA_TRY(chol_mixed<__bf16>(h, s, n, Wf, n, D64, Linv64, Linv32));
+        else GLA_TRY(chol_mixed<_Float16>(h, s, n, Wf, n, D64, Linv64, Linv32));
+    }
+    int hinfo = 0;
+    GLA_HIP_TRY(hipMemcpyAsync(&hinfo, h->dinfo, sizeof(int), hipMemcpyDeviceToHost, s));
+    GLA_HIP_TRY(hipStreamSynchronize(s));
+    if (info) *info = hinfo;
+    if (hinfo != 0) {
+        if (iters) *iters = 0;
+        if (berr) *berr = INFINITY;
+        return GLA_OK;
+    }
+    double a2 = 0.0;
+    for (int64_t j = 0; j < n; ++j) a2 += cs[j];
+    const double anorm = std::sqrt(a2);
+    if (prec == 'd') {
+        double *Wd = reinterpret_cast<double *>(W);
+        GLA_TRY(prepare_solves<double>(s, n, Wd, reinterpret_cast<double *>(Lr),
+                                       reinterpret_cast<double *>(Linv),
+                                       reinterpret_cast<double *>(Linvr)));
+        return refine<double>(h, s, n, A, lda, nrhs, B, ldb, X, ldx, Wd,
+                              reinterpret_cast<double *>(Lr), reinterpret_cast<double *>(Linv),
+                              reinterpret_cast<double *>(Linvr), Rbuf,
+                              reinterpret_cast<double *>(Rt), reinterpret_cast<double *>(Yt), norms,
+                              anorm, tol, max_iter, iters, berr);
+    }
+    float *Wf = reinterpret_cast<float *>(W);
+    GLA_TRY(prepare_solves<float>(s, n, Wf, reinterpret_cast<float *>(Lr),
+                                  reinterpret_cast<float *>(Linv), reinterpret_cast<float *>(Linvr)));
+    return refine<float>(h, s, n, A, lda, nrhs, B, ldb, X, ldx, Wf, reinterpret_cast<float *>(Lr),
+                         reinterpret_cast<float *>(Linv), reinterpret_cast<float *>(Linvr), Rbuf,
+                         reinterpret_cast<float *>(Rt), reinterpret_cast<float *>(Yt), norms,
+                         anorm, tol, max_iter, iters, berr);
+}
+
+int lp_gemm_entry(char tin, int64_t m, int64_t n, int64_t k, float alpha, const float *A,
+                  int64_t lda, const float *B, int64_t ldb, float *C, int64_t ldc, int mode,
+                  hipStream_t s)
+{
+    if (tin == 'b') return lp_gemm_nt<__bf16>(s, mode, m, n, k, alpha, A, lda, B, ldb, C, ldc);
+    return lp_gemm_nt<_Float16>(s, mode, m, n, k, alpha, A, lda, B, ldb, C, ldc);
+}
+
+}  // namespace gla
+
+extern "C" {
+
+int gla_chol_solve_mixed_d(gla_handle_t hh, char prec, int64_t n, const double *A, int64_t lda,
+                           int64_t nrhs, const double *B, int64_t ldb, double *X, int64_t ldx,
+                           double tol, int32_t max_iter, int32_t *iters, double *berr,
+                           int64_t *info, void *stream)
+{
+    auto *h = reinterpret_cast<gla::Handle *>(hh);
+    if (!h) { gla::set_error("null handle"); return -1; }
+    if (prec != 'd' && prec != 's' && prec != 'b' && prec != 'h') {
+        gla::set_error("prec must be one of 'd', 's', 'b', 'h'");
+        return -2;
+    }
+    if (n < 0) return -3;
+    if (lda < std::max<int64_t>(1, n)) return -5;
+    if (nrhs < 0) return -6;
+    if (ldb < std::max<int64_t>(1, n)) return -8;
+    if (ldx < std::max<int64_t>(1, n)) return -10;
+    if (n == 0 || nrhs == 0) {
+        if (iters) *iters = 0;
+        if (berr) *berr = 0.0;
+        if (info) *info = 0;
+        return GLA_OK;
+    }
+    GLA_HIP_TRY(hipSetDevice(h->device));
+    int it = 0;
+    int rc = gla::mixed_solve(h, reinterpret_cast<hipStream_t>(stream), prec, n, A, lda, nrhs, B,
+                              ldb, X, ldx, tol, max_iter, &it, berr, info);
+    if (iters) *iters = it;
+    return rc;
+}
+
+int gla_gemm_lowp_nt(gla_handle_t hh, char tin, int64_t m, int64_t n, int64_t k, float alpha,
+                     const float *A, int64_t lda, const float *B, int64_t ldb, float *C,
+                     int64_t ldc, void *stream)
+{
+    auto *h = reinterpret_cast<gla::Handle *>(hh);
+    if (!h) return -1;
+    if (tin != 'b' && tin != 'h') return -2;
+    if (alpha != 1.f && alpha != -1.f) { gla::set_error("alpha must be +1 or -1"); return -6; }
+    GLA_HIP_TRY(hipSetDevice(h->device));
+    return gla::lp_gemm_entry(tin, m, n, k, alpha, A, lda, B, ldb, C, ldc, 0,
+                              reinterpret_cast<hipStream_t>(stream));
+}
+
+}  // extern "C"

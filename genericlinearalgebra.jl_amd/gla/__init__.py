@@ -430,3 +430,170 @@ def qrBlocked_(A, blocksize: int = 12, tau=None, work=None):
                C.c_void_p(tau.data_ptr()), wp, ldw, _stream(A))
     _check(rc, "gla_qr_blocked")
     return QR2(A, tau)
+
+
+# ----------------------------------------------------------- batched ----
+import json as _json  # noqa: E402
+import os as _os  # noqa: E402
+
+_TUNING = _os.path.join(_os.path.dirname(_os.path.abspath(__file__)), "batched_tuning.json")
+
+
+def batched_tuning(dtype) -> dict:
+    """Size-bin parameters for the batched solver: the autotuned table
+    (gla/batched_tuning.json, written by gla.autotune on an MI355X) or
+    defaults."""
+    key = "float64" if dtype in (torch.float64, np.float64, "float64") else "float32"
+    cfg = {"wave_max": 32, "grid_mult": 2}
+    if _os.path.exists(_TUNING):
+        with open(_TUNING) as f:
+            cfg.update(_json.load(f).get(key, {}))
+    return cfg
+
+
+class BatchPlan:
+    """Sizes of a batch of SPD matrices sorted into the solver's bins (see
+    gla_batch_plan_create).  Build once, reuse for every batched call with
+    the same sizes (this is what a captured graph replays)."""
+
+    def __init__(self, sizes, dtype=torch.float64, wave_max=None, grid_mult=None, device=0):
+        cfg = batched_tuning(dtype)
+        self.wave_max = int(cfg["wave_max"] if wave_max is None else wave_max)
+        self.grid_mult = int(cfg["grid_mult"] if grid_mult is None else grid_mult)
+        self.sizes = np.ascontiguousarray(np.asarray(sizes, dtype=np.int32))
+        self.device = device
+        self.dtype = dtype
+        hp = C.c_void_p()
+        with torch.cuda.device(device):
+            _check(lib().gla_batch_plan_create(_handle(device), len(self.sizes),
+                                               self.sizes.ctypes.data_as(C.c_void_p),
+                                               self.wave_max, self.grid_mult, C.byref(hp)),
+                   "gla_batch_plan_create")
+        self._plan = hp.value
+
+    def __del__(self):
+        try:
+            if getattr(self, "_plan", None):
+                lib().gla_batch_plan_destroy(self._plan)
+        except Exception:  # noqa: BLE001  (interpreter shutdown)
+            pass
+
+
+def chol_solve_batched_(plan: BatchPlan, A_ptrs, lda, b_ptrs, info=None, stream=None):
+    """Factor A_i = L_i L_iᵀ in place and overwrite b_i with A_i⁻¹ b_i for
+    every matrix of the plan, in one launch.  A_ptrs / b_ptrs are int64
+    device tensors of pointers, lda an int32 device tensor, info an optional
+    int32 device tensor (first negative pivot per matrix, 0 if none)."""
+    sfx = "d" if plan.dtype == torch.float64 else "s"
+    f = getattr(lib(), f"gla_chol_solve_batched_{sfx}")
+    st = C.c_void_p(stream if stream is not None else torch.cuda.current_stream().cuda_stream)
+    rc = f(_handle(plan.device), C.c_void_p(plan._plan), C.c_void_p(A_ptrs.data_ptr()),
+           C.c_void_p(lda.data_ptr()), C.c_void_p(b_ptrs.data_ptr()),
+           C.c_void_p(info.data_ptr() if info is not None else None), st)
+    _check(rc, "gla_chol_solve_batched")
+
+
+class PackedBatch:
+    """A batch of n_i×n_i matrices and n_i-vectors packed in two device
+    buffers (column-major, lda = n_i), with the pointer tables the batched
+    entry point takes."""
+
+    def __init__(self, sizes, dtype=torch.float64, device="cuda"):
+        self.sizes = np.asarray(sizes, dtype=np.int64)
+        self.dtype = dtype
+        esz = torch.empty((), dtype=dtype).element_size()
+        self.aoff = np.concatenate([[0], np.cumsum(self.sizes * self.sizes)])
+        self.boff = np.concatenate([[0], np.cumsum(self.sizes)])
+        self.A = torch.zeros(int(self.aoff[-1]) + 1, dtype=dtype, device=device)
+        self.b = torch.zeros(int(self.boff[-1]) + 1, dtype=dtype, device=device)
+        ab, bb = self.A.data_ptr(), self.b.data_ptr()
+        self.A_ptrs = torch.tensor(ab + self.aoff[:-1] * esz, dtype=torch.int64, device=device)
+        self.b_ptrs = torch.tensor(bb + self.boff[:-1] * esz, dtype=torch.int64, device=device)
+        self.lda = torch.tensor(np.maximum(self.sizes, 1), dtype=torch.int32, device=device)
+        self.info = torch.zeros(len(self.sizes), dtype=torch.int32, device=device)
+
+    def matrix(self, i) -> torch.Tensor:
+        n = int(self.sizes[i])
+        return self.A[int(self.aoff[i]):int(self.aoff[i]) + n * n].view(n, n).t()
+
+    def vector(self, i) -> torch.Tensor:
+        n = int(self.sizes[i])
+        return self.b[int(self.boff[i]):int(self.boff[i]) + n]
+
+
+# ------------------------------------------- mixed precision + refinement
+class MixedSolveResult:
+    """Outcome of chol_solve_mixed: corrections made, final normwise
+    backward error max_k ‖b_k − A x_k‖ / (‖A‖_F ‖x_k‖ + ‖b_k‖)."""
+
+    def __init__(self, iters: int, berr: float):
+        self.iters = iters
+        self.berr = berr
+
+    def __repr__(self):
+        return f"MixedSolveResult(iters={self.iters}, berr={self.berr:.3e})"
+
+
+_PREC = {"fp64": b"d", "fp32": b"s", "bf16": b"b", "fp16": b"h"}
+
+
+def chol_solve_mixed(A, B, precision: str = "bf16", tol: float = 1e-12, max_iter: int = 30,
+                     X=None):
+    """X = A⁻¹ B for SPD Float64 A (lower triangle read), factored in
+    `precision` ("fp64", "fp32", "bf16": FP32 storage + BF16 MFMA trailing
+    updates, "fp16": the same with FP16) and refined in Float64 to a
+    normwise backward error ≤ tol.  Raises DomainError (index = first
+    non-positive pivot) when the low-precision factorisation breaks down.
+    Returns (X, MixedSolveResult)."""
+    if precision not in _PREC:
+        raise ArgumentError(f"precision must be one of {sorted(_PREC)}")
+    if A.dtype != torch.float64 or B.dtype != torch.float64:
+        raise TypeError("A and B must be Float64")
+    lda = _mat(A)
+    vec = B.dim() == 1
+    Bm = B.view(-1, 1) if vec else B
+    ldb = _mat(Bm, "B")
+    n = A.shape[0]
+    if A.shape[1] != n:
+        raise DimensionMismatch(f"matrix is not square: dimensions are {tuple(A.shape)}")
+    if Bm.shape[0] != n:
+        raise DimensionMismatch(f"B has {Bm.shape[0]} rows, A is {n}×{n}")
+    nrhs = Bm.shape[1]
+    if X is None:
+        X = empty_colmajor(n, nrhs, torch.float64, device=A.device)
+    ldx = _mat(X, "X")
+    it = C.c_int32(0)
+    be = C.c_double(0.0)
+    info = C.c_int64(0)
+    with torch.cuda.device(A.device):
+        rc = lib().gla_chol_solve_mixed_d(_handle(A.device.index or 0), _PREC[precision], n,
+                                          C.c_void_p(A.data_ptr()), lda, nrhs,
+                                          C.c_void_p(Bm.data_ptr()), ldb,
+                                          C.c_void_p(X.data_ptr()), ldx, float(tol),
+                                          int(max_iter), C.byref(it), C.byref(be),
+                                          C.byref(info), _stream(A))
+    _check(rc, "gla_chol_solve_mixed_d")
+    if info.value != 0:
+        raise DomainError(info.value)
+    return (X.view(-1) if vec else X), MixedSolveResult(it.value, be.value)
+
+
+def gemm_lowp_nt_(Cm, A, B, alpha=-1.0, precision: str = "bf16"):
+    """Cm (FP32) += alpha · A Bᵀ with A, B rounded to bf16/fp16, FP32
+    accumulation (the trailing update of the mixed factorisation)."""
+    if precision not in ("bf16", "fp16"):
+        raise ArgumentError("precision must be bf16 or fp16")
+    for X, nm in ((Cm, "C"), (A, "A"), (B, "B")):
+        if X.dtype != torch.float32:
+            raise TypeError(f"{nm} must be Float32")
+    ldc, lda, ldb = _mat(Cm, "C"), _mat(A), _mat(B, "B")
+    m, n = Cm.shape
+    if A.shape[0] != m or B.shape[0] != n or A.shape[1] != B.shape[1]:
+        raise DimensionMismatch("C (m×n) += A (m×k) Bᵀ (k×n)")
+    with torch.cuda.device(Cm.device):
+        rc = lib().gla_gemm_lowp_nt(_handle(Cm.device.index or 0), _PREC[precision], m, n,
+                                    A.shape[1], float(alpha), C.c_void_p(A.data_ptr()), lda,
+                                    C.c_void_p(B.data_ptr()), ldb, C.c_void_p(Cm.data_ptr()),
+                                    ldc, _stream(Cm))
+    _check(rc, "gla_gemm_lowp_nt")
+    return Cm

@@ -47,6 +47,7 @@ def _sigs(ft):
         "gla_lmul_unit_lower": (_int, [_p, _i64, _i64, _p, _i64, _p, _i64, ft, _p]),
         "gla_gemm": (_int, [_p, _ch, _ch, _i64, _i64, _i64, ft, _p, _i64, _p, _i64, ft, _p,
                             _i64, _p]),
+        "gla_chol_solve_batched": (_int, [_p, _p, _p, _p, _p, _p, _p]),
     }
 
 
@@ -72,6 +73,17 @@ def lib() -> C.CDLL:
     L.gla_handle_create.restype = _int
     L.gla_handle_destroy.argtypes = [_p]
     L.gla_handle_destroy.restype = _int
+    L.gla_batch_plan_create.argtypes = [_p, _i64, _p, C.c_int32, C.c_int32, C.POINTER(_p)]
+    L.gla_batch_plan_create.restype = _int
+    L.gla_batch_plan_destroy.argtypes = [_p]
+    L.gla_batch_plan_destroy.restype = _int
+    L.gla_chol_solve_mixed_d.argtypes = [_p, _ch, _i64, _p, _i64, _i64, _p, _i64, _p, _i64,
+                                         C.c_double, C.c_int32, C.POINTER(C.c_int32),
+                                         C.POINTER(C.c_double), _pi64, _p]
+    L.gla_chol_solve_mixed_d.restype = _int
+    L.gla_gemm_lowp_nt.argtypes = [_p, _ch, _i64, _i64, _i64, C.c_float, _p, _i64, _p, _i64, _p,
+                                   _i64, _p]
+    L.gla_gemm_lowp_nt.restype = _int
     L.gla_last_error_string.restype = C.c_char_p
     L.gla_version.restype = C.c_char_p
     _lib = L

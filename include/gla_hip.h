@@ -129,6 +129,52 @@ int gla_lmul_unit_lower_d(gla_handle_t h, int64_t m, int64_t n, const double *L,
 int gla_lmul_unit_lower_s(gla_handle_t h, int64_t m, int64_t n, const float *L,
                           int64_t ldl, float *B, int64_t ldb, float alpha, void *stream);
 
+/* ------------------------------------------- batched uneven Cholesky -- */
+/* No counterpart in the reference (one matrix per call there): factor and
+ * solve many SPD matrices of uneven size n ≤ 256 in one persistent launch,
+ * each with cholUnblocked!'s result (src/cholesky.jl:3-15) followed by the
+ * solve L Lᵀ x = b.  A plan (host-side sizes, sorted into size bins: one
+ * workgroup per matrix above `wave_max`, one wave per matrix at or below
+ * it, `grid_mult` workgroups per CU) is built once; the batched call then
+ * allocates nothing and synchronises nothing, so it can be captured in a
+ * hipGraph.  Arrays A, lda, b, info are device arrays of length `batch`
+ * (pointers to each matrix / right-hand side, its leading dimension, its
+ * info = first negative pivot or 0).  A is overwritten with L (lower), b
+ * with x. */
+typedef struct gla_batch_plan_s *gla_batch_plan_t;
+int gla_batch_plan_create(gla_handle_t h, int64_t batch, const int32_t *n_host, int32_t wave_max,
+                          int32_t grid_mult, gla_batch_plan_t *plan);
+int gla_batch_plan_destroy(gla_batch_plan_t plan);
+int gla_chol_solve_batched_d(gla_handle_t h, gla_batch_plan_t plan, double *const *A,
+                             const int32_t *lda, double *const *b, int32_t *info, void *stream);
+int gla_chol_solve_batched_s(gla_handle_t h, gla_batch_plan_t plan, float *const *A,
+                             const int32_t *lda, float *const *b, int32_t *info, void *stream);
+
+/* ------------------------------- mixed precision + iterative refinement -- */
+/* Solves A X = B (A n×n SPD, FP64, lower triangle read; B, X n×nrhs FP64).
+ * No counterpart in the reference, whose Cholesky (src/cholesky.jl:3-55)
+ * factors in the element type of A only; this extends cholRecursive! with
+ * a lower-precision factorisation and FP64 refinement.  `prec` selects the
+ * factorisation: 'd' FP64 (the plain path), 's' FP32, 'b' FP32 storage with
+ * BF16 MFMA trailing updates and FP64 diagonal blocks, 'h' the same with
+ * FP16.  X is refined in FP64 (r = B − A X on the FP64 GEMM, correction
+ * with the low-precision factor) until the normwise backward error
+ * max_k ‖r_k‖₂ / (‖A‖_F ‖x_k‖₂ + ‖b_k‖₂) ≤ tol or max_iter corrections.
+ * *iters = corrections made, *berr = final backward error, *info = first
+ * non-positive pivot of the factorisation (1-based, 0 = success; then X is
+ * not computed).  A and B are not modified.  Synchronises the stream. */
+int gla_chol_solve_mixed_d(gla_handle_t h, char prec, int64_t n, const double *A, int64_t lda,
+                           int64_t nrhs, const double *B, int64_t ldb, double *X, int64_t ldx,
+                           double tol, int32_t max_iter, int32_t *iters, double *berr,
+                           int64_t *info, void *stream);
+/* Low-precision MFMA product used by the mixed factorisation:
+ * C (m×n, FP32) += alpha · A Bᵀ with A (m×k), B (n×k) FP32 in memory rounded
+ * to `tin` ('b' BF16, 'h' FP16) on the way in, FP32 accumulation,
+ * alpha ∈ {+1, −1}.  Mirrors rankUpdate!'s role (src/juliaBLAS.jl:89). */
+int gla_gemm_lowp_nt(gla_handle_t h, char tin, int64_t m, int64_t n, int64_t k, float alpha,
+                     const float *A, int64_t lda, const float *B, int64_t ldb, float *C,
+                     int64_t ldc, void *stream);
+
 /* ------------------------------------------------------ GEMM primitive -- */
 /* The MFMA GEMM the factorisations are built on, exported for the
  * juliaBLAS-style callers (`mul!(C, A, B, α, β)` at src/householder.jl:108,144,

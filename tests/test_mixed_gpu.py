@@ -1,0 +1,122 @@
+"""Mixed-precision Cholesky solve with Float64 iterative refinement, and the
+low-precision MFMA product it is built on.
+
+The product is checked against a float32 torch product of the same
+bf16/fp16-rounded operands (exact products, fp32 sums: tolerance
+k·eps32 relative to |A||B|ᵀ).  The solver is checked by what it promises:
+the backward error it reports is recomputed here in float64 on the CPU and
+must be ≤ 1e-12, and X must agree with a float64 LAPACK solve to within
+κ·1e-12.  Matrices: A = Q diag(λ) Qᵀ with λ log-spaced over [1/κ, 1]."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+
+
+def _gla():
+    import gla
+
+    return gla
+
+
+def spd_kappa(n, kappa, rng):
+    Q, _ = np.linalg.qr(rng.standard_normal((n, n)))
+    lam = np.logspace(0.0, -np.log10(kappa), n)
+    A = (Q * lam) @ Q.T
+    return (A + A.T) / 2
+
+
+def backward_error(A, X, B):
+    R = B - A @ X
+    an = np.linalg.norm(A, "fro")
+    return max(np.linalg.norm(R[:, k]) / (an * np.linalg.norm(X[:, k]) + np.linalg.norm(B[:, k]))
+               for k in range(B.shape[1]))
+
+
+@pytest.mark.parametrize("prec,tdt", [("bf16", torch.bfloat16), ("fp16", torch.float16)])
+@pytest.mark.parametrize("m,n,k", [(128, 128, 32), (300, 200, 77), (517, 129, 256), (64, 1000, 13)])
+@pytest.mark.parametrize("alpha", [1.0, -1.0])
+def test_lowp_gemm_vs_torch(prec, tdt, m, n, k, alpha):
+    gla = _gla()
+    g = torch.Generator().manual_seed(m + n + k)
+    A = torch.randn(m, k, generator=g)
+    B = torch.randn(n, k, generator=g)
+    C0 = torch.randn(m, n, generator=g)
+    Ar = A.to(tdt).float()
+    Br = B.to(tdt).float()
+    ref = C0.double() + alpha * (Ar.double() @ Br.double().T)
+    Cd = gla.colmajor(C0, torch.float32)
+    gla.gemm_lowp_nt_(Cd, gla.colmajor(A, torch.float32), gla.colmajor(B, torch.float32),
+                      alpha=alpha, precision=prec)
+    out = gla.to_numpy(Cd).astype(np.float64)
+    scale = np.abs(C0.double().numpy()) + np.abs(Ar.double().numpy()) @ np.abs(Br.double().numpy()).T
+    err = np.abs(out - ref.numpy()) / scale
+    assert err.max() <= (k + 2) * np.finfo(np.float32).eps, err.max()
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32", "bf16", "fp16"])
+@pytest.mark.parametrize("kappa", [1e2, 1e4])
+def test_refined_solve_reaches_tolerance(prec, kappa):
+    gla = _gla()
+    rng = np.random.default_rng(int(np.log10(kappa)))
+    n, nrhs = 700, 16
+    A = spd_kappa(n, kappa, rng)
+    B = rng.standard_normal((n, nrhs))
+    Ad, Bd = gla.colmajor(A), gla.colmajor(B)
+    try:
+        X, res = gla.chol_solve_mixed(Ad, Bd, precision=prec, tol=1e-12, max_iter=60)
+    except gla.DomainError:
+        pytest.skip(f"{prec} factorisation breaks down at kappa={kappa:g} (expected for "
+                    "low precision and large kappa)")
+    Xh = gla.to_numpy(X)
+    be = backward_error(A, Xh, B)
+    assert res.berr <= 1e-12, res
+    assert be <= 2e-12, (be, res)
+    assert abs(be - res.berr) <= 0.5 * max(be, res.berr) + 1e-16
+    Xr = np.linalg.solve(A, B)
+    assert np.linalg.norm(Xh - Xr) / np.linalg.norm(Xr) <= kappa * 1e-11
+    # A and B are inputs only
+    assert np.array_equal(gla.to_numpy(Ad), A)
+    assert np.array_equal(gla.to_numpy(Bd), B)
+
+
+def test_fp64_needs_no_correction_on_wellconditioned():
+    gla = _gla()
+    rng = np.random.default_rng(3)
+    A = spd_kappa(300, 10.0, rng)
+    B = rng.standard_normal((300, 4))
+    X, res = gla.chol_solve_mixed(gla.colmajor(A), gla.colmajor(B), precision="fp64",
+                                  tol=1e-12)
+    assert res.iters == 0 and res.berr <= 1e-12
+
+
+def test_ragged_sizes_and_vector_rhs():
+    gla = _gla()
+    rng = np.random.default_rng(9)
+    for n in (1, 5, 127, 129, 383):
+        A = spd_kappa(n, 100.0, rng)
+        b = rng.standard_normal(n)
+        x, res = gla.chol_solve_mixed(gla.colmajor(A), torch.from_numpy(b).cuda(),
+                                      precision="bf16", tol=1e-12)
+        xh = x.cpu().numpy()
+        assert xh.shape == (n,)
+        assert backward_error(A, xh[:, None], b[:, None]) <= 2e-12, (n, res)
+
+
+def test_not_spd_raises_domain_error():
+    gla = _gla()
+    rng = np.random.default_rng(4)
+    A = spd_kappa(256, 10.0, rng)
+    A[200, 200] = -5.0
+    with pytest.raises(gla.DomainError):
+        gla.chol_solve_mixed(gla.colmajor(A), gla.colmajor(rng.standard_normal((256, 2))),
+                             precision="bf16")
+
+
+def test_bad_precision_rejected():
+    gla = _gla()
+    A = gla.colmajor(np.eye(4))
+    with pytest.raises(gla.ArgumentError):
+        gla.chol_solve_mixed(A, gla.colmajor(np.ones((4, 1))), precision="fp8")
