@@ -80,6 +80,7 @@ struct GemmP {
     int per_xcd;  // ceil(total / 8)
     int64_t kc;   // K extent of one split (split-K along blockIdx.y)
     T *ws;        // split-K partials: ws[z][M×N], or null for a direct epilogue
+    int kfrom_row;  // 1: the K range of a tile starts at its first row (op(A)(i,k) = 0 for k < i)
 };
 
 // One operand slab: BX (M or N extent) × kBK, staged through registers into
@@ -218,8 +219,11 @@ __global__ __launch_bounds__(kThreads, 2) void gemm_kernel(GemmP<T> p)
     const int wm = wave & 1, wn = wave >> 1;
 
     // Split-K: this workgroup contracts K range [koff, koff + K).
-    const int64_t koff = (int64_t)blockIdx.y * p.kc;
-    const int64_t K = p.K - koff < p.kc ? p.K - koff : p.kc;
+    // Triangular operands (kfrom_row): terms with k below the tile's first
+    // row are zero and skipped.
+    const int64_t koff = (p.kfrom_row ? m0 : 0) + (int64_t)blockIdx.y * p.kc;
+    int64_t K = p.K - koff < p.kc ? p.K - koff : p.kc;
+    if (K < 0) K = 0;
     const T *Ap = p.A + (TA ? koff : koff * p.lda);
     const T *Bp = p.B + (TB ? koff * p.ldb : koff);
 
@@ -422,10 +426,12 @@ int dispatch(hipStream_t s, char ta, char tb, int mode, const GemmP<T> &p, size_
 template <typename T>
 int gemm(hipStream_t s, char ta, char tb, int mode, int64_t M, int64_t N, int64_t K, T alpha,
          const T *A, int64_t lda, const T *B, int64_t ldb, T beta, T *C, int64_t ldc,
-         int64_t diag_off, T *ws, size_t ws_elems)
+         int64_t diag_off, T *ws, size_t ws_elems, int kfrom_row)
 {
     if (M <= 0 || N <= 0) return GLA_OK;
     GemmP<T> p{};
+    p.kfrom_row = kfrom_row;
+    if (kfrom_row) ws_elems = 0;
     p.M = M;
     p.N = N;
     p.K = K < 0 ? 0 : K;
@@ -448,9 +454,9 @@ int gemm(hipStream_t s, char ta, char tb, int mode, int64_t M, int64_t N, int64_
 
 template int gemm<double>(hipStream_t, char, char, int, int64_t, int64_t, int64_t, double,
                           const double *, int64_t, const double *, int64_t, double, double *,
-                          int64_t, int64_t, double *, size_t);
+                          int64_t, int64_t, double *, size_t, int);
 template int gemm<float>(hipStream_t, char, char, int, int64_t, int64_t, int64_t, float,
                          const float *, int64_t, const float *, int64_t, float, float *,
-                         int64_t, int64_t, float *, size_t);
+                         int64_t, int64_t, float *, size_t, int);
 
 }  // namespace gla

@@ -62,11 +62,13 @@ int ensure_workspace(Handle *h, size_t bytes);
 // C = alpha*op(A)*op(B) + beta*C  with op(A) M×K, op(B) K×N.
 // mode 0: every element; mode 1: only elements with (row + diag_off) >= col
 // ("lower" relative to a diagonal), tiles wholly above it are skipped.
-// ta/tb: 'N' or 'T'.  Supported (ta,tb,mode): (N,N,0) (N,T,0) (T,N,0) (N,T,1).
+// ta/tb: 'N' or 'T'.  Supported (ta,tb,mode): (N,N,0) (N,T,0) (T,N,0) (N,T,1)
+// (N,T,2).  kfrom_row = 1: op(A)(i,k) = 0 for k < i (an upper-triangular
+// op(A)), so each tile contracts k from its first row on (no split-K).
 template <typename T>
 int gemm(hipStream_t s, char ta, char tb, int mode, int64_t M, int64_t N, int64_t K, T alpha,
          const T *A, int64_t lda, const T *B, int64_t ldb, T beta, T *C, int64_t ldc,
-         int64_t diag_off = 0, T *ws = nullptr, size_t ws_elems = 0);
+         int64_t diag_off = 0, T *ws = nullptr, size_t ws_elems = 0, int kfrom_row = 0);
 
 // ------------------------------------------------------------- Cholesky --
 // Linv: scratch for the 16×16 diagonal-tile inverses (tile_inverse_elems(n)
@@ -91,6 +93,17 @@ template <typename T>
 int trsm_rlt(hipStream_t s, int64_t m, int64_t n, const T *L, int64_t ldl, const T *Linv, T *B,
              int64_t ldb);
 size_t tile_inverse_elems(int64_t n);
+
+// QR building blocks (qr.hip): T of the first k reflectors of F (m×k), and
+// A (mA×nA) <- op(Q) A, trans 'N' or 'T', Q from the first b reflectors.
+size_t qr_tblock_scratch(int64_t m, int64_t k);
+size_t qr_apply_scratch(int64_t mA, int64_t nA, int64_t b);
+template <typename T>
+int qr_tblock_dev(hipStream_t s, int64_t m, int64_t k, const T *F, int64_t ldf, const T *tau,
+                  T *Tm, int64_t ldt, T *scratch);
+template <typename T>
+int qr_apply_dev(hipStream_t s, char trans, int64_t mA, int64_t nA, int64_t b, const T *V,
+                 int64_t ldv, const T *Tm, int64_t ldt, T *A, int64_t lda, T *scratch);
 
 // Small helpers used across units.
 __global__ void k_set_int(int *p, int v);

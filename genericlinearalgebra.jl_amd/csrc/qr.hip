@@ -637,6 +637,61 @@ int qr_blocked_dev(Handle *h, hipStream_t s, int64_t m, int64_t n, T *A, int64_t
 
 }  // namespace
 
+// Building blocks shared with the adjoints: the compact-WY T of the first k
+// reflectors stored in F (m×k), and A (mA×nA) <- op(Q) A for Q = I − V T Vᵀ
+// with V the first b reflectors of F.
+size_t qr_tblock_scratch(int64_t m, int64_t k) { return (size_t)m * k + 2 * (size_t)k * k + 64; }
+size_t qr_apply_scratch(int64_t mA, int64_t nA, int64_t b)
+{
+    return (size_t)mA * b + (size_t)b * b + 2 * (size_t)b * nA + 8 * (size_t)b * nA + 64;
+}
+
+template <typename T>
+int qr_tblock_dev(hipStream_t s, int64_t m, int64_t k, const T *F, int64_t ldf, const T *tau,
+                  T *Tm, int64_t ldt, T *scratch)
+{
+    if (k <= 0) return GLA_OK;
+    // Vw (m×k) explicit, G = VwᵀVw (k×k), inverse scratch k*k
+    T *Vw = scratch;
+    T *Gm = Vw + (size_t)m * k;
+    T *iw = Gm + (size_t)k * k;
+    GLA_TRY(unit_lower_copy<T>(s, m, k, F, ldf, Vw, m));
+    GLA_TRY(gemm<T>(s, 'T', 'N', 0, k, k, m, T(1), Vw, m, Vw, m, T(0), Gm, k));   // :69-74
+    hipLaunchKernelGGL(tb_init_kernel<T>, dim3((unsigned)cdiv(k, 256), (unsigned)k), dim3(256), 0,
+                       s, k, Gm, k, tau, Tm, ldt);
+    GLA_LAUNCH_CHECK();
+    GLA_TRY(trtri_unit_upper<T>(s, k, Tm, ldt, iw));                                // :75
+    hipLaunchKernelGGL(tb_scale_kernel<T>, dim3((unsigned)cdiv(k, 256), (unsigned)k), dim3(256),
+                       0, s, k, tau, Tm, ldt);                                      // :76-81
+    GLA_LAUNCH_CHECK();
+    return GLA_OK;
+}
+
+template <typename T>
+int qr_apply_dev(hipStream_t s, char trans, int64_t mA, int64_t nA, int64_t b, const T *V,
+                 int64_t ldv, const T *Tm, int64_t ldt, T *A, int64_t lda, T *scratch)
+{
+    if (b <= 0 || nA <= 0) return GLA_OK;
+    T *Vw = scratch;
+    T *Tw = Vw + (size_t)mA * b;
+    T *W1 = Tw + (size_t)b * b;
+    T *W2 = W1 + (size_t)b * nA;
+    T *split = W2 + (size_t)b * nA;
+    GLA_TRY(unit_lower_copy<T>(s, mA, b, V, ldv, Vw, mA));
+    GLA_TRY(upper_copy<T>(s, b, Tm, ldt, Tw, b));
+    return wy_update<T>(s, trans, mA, nA, b, Vw, mA, Tw, b, A, lda, W1, W2, split,
+                        8 * (size_t)b * nA);
+}
+
+#define GLA_INST_QR(T)                                                                           \
+    template int qr_tblock_dev<T>(hipStream_t, int64_t, int64_t, const T *, int64_t, const T *, \
+                                  T *, int64_t, T *);                                            \
+    template int qr_apply_dev<T>(hipStream_t, char, int64_t, int64_t, int64_t, const T *,        \
+                                 int64_t, const T *, int64_t, T *, int64_t, T *);
+GLA_INST_QR(double)
+GLA_INST_QR(float)
+#undef GLA_INST_QR
+
 template <typename T>
 int unit_upper_explicit(hipStream_t s, int64_t n, const T *src, int64_t lds, T *dst, int64_t ldd)
 {
@@ -681,22 +736,8 @@ int tblock_entry(gla_handle_t hh, int64_t m, int64_t n, const T *F, int64_t ldf,
     if (k == 0) return GLA_OK;
     hipStream_t s = reinterpret_cast<hipStream_t>(stream);
     GLA_HIP_TRY(hipSetDevice(h->device));
-    // Vw (m×k) explicit, G = VwᵀVw (k×k), inverse scratch k*k
-    const size_t need = (size_t)m * k + 2 * (size_t)k * k + 64;
-    GLA_TRY(ensure_workspace(h, need * sizeof(T)));
-    T *Vw = reinterpret_cast<T *>(h->ws);
-    T *Gm = Vw + (size_t)m * k;
-    T *iw = Gm + (size_t)k * k;
-    GLA_TRY(unit_lower_copy<T>(s, m, k, F, ldf, Vw, m));
-    GLA_TRY(gemm<T>(s, 'T', 'N', 0, k, k, m, T(1), Vw, m, Vw, m, T(0), Gm, k));   // :69-74
-    hipLaunchKernelGGL(tb_init_kernel<T>, dim3((unsigned)cdiv(k, 256), (unsigned)k), dim3(256), 0,
-                       s, k, Gm, k, tau, Tm, ldt);
-    GLA_LAUNCH_CHECK();
-    GLA_TRY(trtri_unit_upper<T>(s, k, Tm, ldt, iw));                                // :75
-    hipLaunchKernelGGL(tb_scale_kernel<T>, dim3((unsigned)cdiv(k, 256), (unsigned)k), dim3(256),
-                       0, s, k, tau, Tm, ldt);                                      // :76-81
-    GLA_LAUNCH_CHECK();
-    return GLA_OK;
+    GLA_TRY(ensure_workspace(h, qr_tblock_scratch(m, k) * sizeof(T)));
+    return qr_tblock_dev<T>(s, m, k, F, ldf, tau, Tm, ldt, reinterpret_cast<T *>(h->ws));
 }
 
 template <typename T>
@@ -718,19 +759,12 @@ int wy_entry(gla_handle_t hh, char trans, int64_t mA, int64_t nA, const T *V, in
     if (b == 0 || nA == 0) return GLA_OK;
     hipStream_t s = reinterpret_cast<hipStream_t>(stream);
     GLA_HIP_TRY(hipSetDevice(h->device));
-    const int64_t split_elems = 8 * b * nA;
-    const size_t need = (size_t)mA * b + (size_t)b * b + 2 * (size_t)b * nA + split_elems + 64;
-    GLA_TRY(ensure_workspace(h, need * sizeof(T)));
-    T *Vw = reinterpret_cast<T *>(h->ws);
-    T *Tw = Vw + (size_t)mA * b;
-    T *W1 = Tw + (size_t)b * b;
-    T *W2 = W1 + (size_t)b * nA;
-    T *split = W2 + (size_t)b * nA;
-    GLA_TRY(unit_lower_copy<T>(s, mA, b, V, ldv, Vw, mA));
-    GLA_TRY(upper_copy<T>(s, b, Tm, ldt, Tw, b));
-    GLA_TRY(wy_update<T>(s, trans == 'N' ? 'N' : 'T', mA, nA, b, Vw, mA, Tw, b, A, lda, W1, W2,
-                         split, (size_t)split_elems));
-    if (M) GLA_TRY(copy_mat<T>(s, b, nA, W2, b, M, ldm));
+    GLA_TRY(ensure_workspace(h, qr_apply_scratch(mA, nA, b) * sizeof(T)));
+    T *ws = reinterpret_cast<T *>(h->ws);
+    GLA_TRY(qr_apply_dev<T>(s, trans == 'N' ? 'N' : 'T', mA, nA, b, V, ldv, Tm, ldt, A, lda, ws));
+    // W2 = op(T) Vᵀ A, the reference's workspace M after the call
+    if (M) GLA_TRY(copy_mat<T>(s, b, nA, ws + (size_t)mA * b + (size_t)b * b + (size_t)b * nA, b,
+                               M, ldm));
     return GLA_OK;
 }
 

@@ -597,3 +597,90 @@ def gemm_lowp_nt_(Cm, A, B, alpha=-1.0, precision: str = "bf16"):
                                     ldc, _stream(Cm))
     _check(rc, "gla_gemm_lowp_nt")
     return Cm
+
+
+# ------------------------------------------------------------------ adjoints
+def chol_adjoint(L, Lbar, convention: str = "S", out=None):
+    """Ā of A = LLᵀ for the cotangent L̄ (lower parts of L and L̄ read):
+    ½ L⁻ᵀ(P + Pᵀ)L⁻¹ with P = Φ(LᵀL̄).  convention "S": symmetric; "L": the
+    derivative with respect to the lower triangle (2Ā below the diagonal, Ā
+    on it, 0 above)."""
+    if convention not in ("S", "L"):
+        raise ArgumentError("convention must be 'S' or 'L'")
+    ldl, ldg = _mat(L, "L"), _mat(Lbar, "L̄")
+    n = L.shape[0]
+    if L.shape != (n, n) or Lbar.shape != (n, n):
+        raise DimensionMismatch("L and L̄ must be n×n")
+    if Lbar.dtype != L.dtype:
+        raise TypeError("L and L̄ must have one element type")
+    if out is None:
+        out = empty_colmajor(n, n, L.dtype, device=L.device)
+    lda = _mat(out, "Ā")
+    with torch.cuda.device(L.device):
+        rc = getattr(lib(), f"gla_chol_adjoint_{_sfx(L)}")(
+            _handle(L.device.index), convention.encode(), n, C.c_void_p(L.data_ptr()), ldl,
+            C.c_void_p(Lbar.data_ptr()), ldg, C.c_void_p(out.data_ptr()), lda, _stream(L))
+    _check(rc, "gla_chol_adjoint")
+    return out
+
+
+def qr_adjoint(F, tau, Rbar, out=None):
+    """Ā (m×n, m ≥ n) of A = QR stored as (F, τ) by qrBlocked!/qrUnblocked!,
+    for the cotangent R̄ (upper part read) and Q̄ = 0:
+    Q [copyltu(R R̄ᵀ) R⁻ᵀ ; 0]."""
+    ldf, ldr = _mat(F, "F"), _mat(Rbar, "R̄")
+    m, n = F.shape
+    if n > m:
+        raise ArgumentError("the QR adjoint needs m >= n")
+    if Rbar.shape != (n, n):
+        raise DimensionMismatch("R̄ must be n×n")
+    _vec(tau, n, F.dtype)
+    if out is None:
+        out = empty_colmajor(m, n, F.dtype, device=F.device)
+    lda = _mat(out, "Ā")
+    with torch.cuda.device(F.device):
+        rc = getattr(lib(), f"gla_qr_adjoint_{_sfx(F)}")(
+            _handle(F.device.index), m, n, C.c_void_p(F.data_ptr()), ldf,
+            C.c_void_p(tau.data_ptr()), C.c_void_p(Rbar.data_ptr()), ldr,
+            C.c_void_p(out.data_ptr()), lda, _stream(F))
+    _check(rc, "gla_qr_adjoint")
+    return out
+
+
+class CholeskyFn(torch.autograd.Function):
+    """torch.autograd rule for L = cholRecursive!(copy(A)) (lower part of A
+    read; the returned L has zeros above the diagonal).  The backward is
+    gla_chol_adjoint with the symmetric convention, the analogue of a
+    ChainRulesCore rrule for the Julia binding."""
+
+    @staticmethod
+    def forward(ctx, A):
+        W = colmajor(A)
+        cholRecursive_(W, Val.L, 1)
+        L = torch.tril(W)
+        ctx.save_for_backward(L)
+        return L
+
+    @staticmethod
+    def backward(ctx, gL):
+        (L,) = ctx.saved_tensors
+        Lc = colmajor(L)
+        return chol_adjoint(Lc, colmajor(gL, L.dtype), "S").contiguous()
+
+
+class QRFn(torch.autograd.Function):
+    """torch.autograd rule for R of qrBlocked!(copy(A), blocksize) (m ≥ n,
+    R̄ only): backward by gla_qr_adjoint."""
+
+    @staticmethod
+    def forward(ctx, A, blocksize=128):
+        q = qrBlocked_(colmajor(A), blocksize)
+        F, tau = q.factors, q.tau
+        ctx.save_for_backward(F, tau)
+        return torch.triu(F[: F.shape[1], :].contiguous())
+
+    @staticmethod
+    def backward(ctx, gR):
+        F, tau = ctx.saved_tensors
+        return qr_adjoint(F, tau, colmajor(gR, F.dtype)).contiguous(), None
+

@@ -175,6 +175,33 @@ int gla_gemm_lowp_nt(gla_handle_t h, char tin, int64_t m, int64_t n, int64_t k, 
                      const float *A, int64_t lda, const float *B, int64_t ldb, float *C,
                      int64_t ldc, void *stream);
 
+/* ------------------------------------------------------------ adjoints -- */
+/* No counterpart in the reference (forward factorisations only,
+ * src/cholesky.jl, src/qr.jl): reverse-mode derivatives for an rrule.
+ *
+ * Cholesky: given L (the factor, lower; strict upper not read) and L̄
+ * (lower; strict upper not read), Ā (n×n) = ½ L⁻ᵀ (P + Pᵀ) L⁻¹ with
+ * P = Φ(Lᵀ L̄), Φ = lower triangle with halved diagonal.  conv 'S': that
+ * symmetric matrix; conv 'L': the derivative with respect to the lower
+ * triangle the factorisation reads — 2Ā below the diagonal, Ā on it, 0
+ * above. */
+int gla_chol_adjoint_d(gla_handle_t h, char conv, int64_t n, const double *L, int64_t ldl,
+                       const double *Lbar, int64_t ldlbar, double *Abar, int64_t ldabar,
+                       void *stream);
+int gla_chol_adjoint_s(gla_handle_t h, char conv, int64_t n, const float *L, int64_t ldl,
+                       const float *Lbar, int64_t ldlbar, float *Abar, int64_t ldabar,
+                       void *stream);
+/* QR (m ≥ n), as stored by qrBlocked!/qrUnblocked! (F: R on and above the
+ * diagonal, reflectors below; τ), with Q̄ = 0 and R̄ (n×n, upper; strict
+ * lower not read): Ā (m×n) = Q [copyltu(R R̄ᵀ) R⁻ᵀ ; 0], copyltu(M) =
+ * tril(M) + tril(M,−1)ᵀ, Q the thin factor of the reflectors. */
+int gla_qr_adjoint_d(gla_handle_t h, int64_t m, int64_t n, const double *F, int64_t ldf,
+                     const double *tau, const double *Rbar, int64_t ldrbar, double *Abar,
+                     int64_t ldabar, void *stream);
+int gla_qr_adjoint_s(gla_handle_t h, int64_t m, int64_t n, const float *F, int64_t ldf,
+                     const float *tau, const float *Rbar, int64_t ldrbar, float *Abar,
+                     int64_t ldabar, void *stream);
+
 /* ------------------------------------------------------ GEMM primitive -- */
 /* The MFMA GEMM the factorisations are built on, exported for the
  * juliaBLAS-style callers (`mul!(C, A, B, α, β)` at src/householder.jl:108,144,

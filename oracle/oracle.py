@@ -241,3 +241,51 @@ def lmul_unit_lower(L, B, alpha):
     _fn("orc_lmul_unit_lower", B)(B.shape[0], B.shape[1], _ptr(L), L.shape[0], _ptr(B),
                                   B.shape[0], alpha)
     return B
+
+
+# --------------------------------------------------------------- adjoints --
+# No reference code: restated from the published formulas, in float64 numpy
+# (general solves, no triangular shortcuts), for small sizes.
+def thin_q(F, tau):
+    """Q[:, :n] of the reflectors stored in F (m×n, m ≥ n) and τ, formed as
+    H₁H₂⋯Hₙ [I; 0] with Hₖ = I − τₖ vₖvₖᵀ, vₖ = [0…0, 1, F[k+1:, k]]."""
+    F = np.asarray(F, dtype=np.float64)
+    m, n = F.shape
+    E = np.zeros((m, n))
+    E[:n, :n] = np.eye(n)
+    for k in range(n - 1, -1, -1):
+        v = np.zeros(m)
+        v[k] = 1.0
+        v[k + 1:] = F[k + 1:, k]
+        E -= float(tau[k]) * np.outer(v, v @ E)
+    return E
+
+
+def chol_adjoint(L, Lbar, conv: str = "S"):
+    """Ā of A = LLᵀ for the cotangent L̄ (Murray 2016, arXiv:1602.07527,
+    eq. 10): P = Φ(Lᵀ tril(L̄)), Φ = lower triangle with halved diagonal;
+    Ā = ½ L⁻ᵀ (P + Pᵀ) L⁻¹.  conv "S": that symmetric matrix; "L": the
+    derivative with respect to the lower triangle (2Ā below the diagonal,
+    Ā on it, 0 above)."""
+    L = np.tril(np.asarray(L, dtype=np.float64))
+    G = np.tril(np.asarray(Lbar, dtype=np.float64))
+    P = np.tril(L.T @ G)
+    P[np.diag_indices_from(P)] *= 0.5
+    Y = np.linalg.solve(L.T, 0.5 * (P + P.T))     # L⁻ᵀ Psym
+    Ab = np.linalg.solve(L.T, Y.T).T              # (L⁻ᵀ Yᵀ)ᵀ = Y L⁻¹
+    if conv == "L":
+        Ab = np.tril(2.0 * Ab) - np.diag(np.diag(Ab))
+    return Ab
+
+
+def qr_adjoint(F, tau, Rbar):
+    """Ā of A = QR (m ≥ n, Q̄ = 0) for the cotangent R̄ (upper part read):
+    M = R R̄ᵀ, Ā = Q copyltu(M) R⁻ᵀ with copyltu(M) = tril(M) + tril(M,−1)ᵀ
+    (Walter & Lehmann 2018; Liao et al. 2019)."""
+    F = np.asarray(F, dtype=np.float64)
+    m, n = F.shape
+    R = np.triu(F[:n, :n])
+    M = R @ np.triu(np.asarray(Rbar, dtype=np.float64)).T
+    Cm = np.tril(M) + np.tril(M, -1).T
+    X = np.linalg.solve(R, Cm.T).T                # X Rᵀ = C
+    return thin_q(F, tau) @ X
