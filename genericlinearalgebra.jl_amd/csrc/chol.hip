@@ -450,11 +450,11 @@ inline size_t linv_elems(int64_t n) { return (size_t)(cdiv(n, 16) + 8) * 256; }
 // Right-looking blocked Cholesky of the lower triangle with look-ahead.
 template <typename T>
 int chol_blocked_dev(Handle *h, hipStream_t s, int64_t n, T *A, int64_t lda, int nb,
-                     int64_t info_base, T *Linv)
+                     int64_t info_base, T *Linv, bool use_side = true)
 {
     if (n <= 0) return GLA_OK;
     const int64_t nsteps = cdiv(n, nb);
-    const bool overlap = nsteps > 2;
+    const bool overlap = use_side && nsteps > 2;
     if (overlap) {
         GLA_HIP_TRY(hipEventRecord(h->ev_fork, s));
         GLA_HIP_TRY(hipStreamWaitEvent(h->side, h->ev_fork, 0));
@@ -509,6 +509,57 @@ int trsm_rlt_dev(hipStream_t s, int64_t m, int64_t n, const T *L, int64_t ldl, c
                     B + n1 * ldb, ldb, 0));
     return trsm_rlt_dev<T>(s, m, n - n1, L + n1 + n1 * ldl, ldl, Linv + (n1 / 16) * 256,
                            B + n1 * ldb, ldb);
+}
+
+// Blocked right-looking Cholesky with wide blocks (NB > 128) and depth-1
+// look-ahead — cholBlocked!'s loop (src/cholesky.jl:17-35) with the
+// diagonal block factored by the 128-wide driver on the caller's stream,
+// the panel solved by the recursive rdiv!, the next block column updated on
+// the caller's stream and the rest of the trailing rankUpdate! (K = NB, an
+// efficient MFMA SYRK) on the side stream, where it overlaps the next
+// diagonal factor and panel solve.
+template <typename T>
+int chol_la_dev(Handle *h, hipStream_t s, int64_t n, T *A, int64_t lda, int NB,
+                int64_t info_base, T *Linv)
+{
+    if (n <= 0) return GLA_OK;
+    if (NB <= kPanelNB || n <= NB) return chol_blocked_dev<T>(h, s, n, A, lda, kPanelNB, info_base, Linv);
+    const int64_t nsteps = cdiv(n, NB);
+    const bool overlap = nsteps > 2;
+    if (overlap) {
+        GLA_HIP_TRY(hipEventRecord(h->ev_fork, s));
+        GLA_HIP_TRY(hipStreamWaitEvent(h->side, h->ev_fork, 0));
+    }
+    int64_t k = 0;
+    for (int64_t step = 0; step < nsteps; ++step, k += NB) {
+        const int64_t kb = std::min<int64_t>(NB, n - k);
+        const int64_t below = n - k - kb;
+        T *Akk = A + k + k * lda;
+        T *A21 = Akk + kb;
+        GLA_TRY(chol_blocked_dev<T>(h, s, kb, Akk, lda, kPanelNB, info_base + k, Linv, false));
+        if (below == 0) break;
+        GLA_TRY(launch_tile_inverse<T>(s, kb, Akk, lda, Linv));
+        GLA_TRY(trsm_rlt_dev<T>(s, below, kb, Akk, lda, Linv, A21, lda));
+        const int64_t kb2 = std::min<int64_t>(NB, below);
+        const int64_t rest = below - kb2;
+        if (overlap && rest > 0) GLA_HIP_TRY(hipEventRecord(h->ev_step[0], s));
+        if (overlap && step > 0) GLA_HIP_TRY(hipStreamWaitEvent(s, h->ev_step[1], 0));
+        GLA_TRY(gemm<T>(s, 'N', 'T', 1, below, kb2, kb, T(-1), A21, lda, A21, lda, T(1),
+                        A21 + kb * lda, lda, 0));
+        if (rest > 0) {
+            hipStream_t us = overlap ? h->side : s;
+            if (overlap) GLA_HIP_TRY(hipStreamWaitEvent(h->side, h->ev_step[0], 0));
+            T *X2 = A21 + kb2;
+            GLA_TRY(gemm<T>(us, 'N', 'T', 1, rest, rest, kb, T(-1), X2, lda, X2, lda, T(1),
+                            X2 + (kb + kb2) * lda, lda, 0));
+            if (overlap) GLA_HIP_TRY(hipEventRecord(h->ev_step[1], h->side));
+        }
+    }
+    if (overlap) {
+        GLA_HIP_TRY(hipEventRecord(h->ev_join, h->side));
+        GLA_HIP_TRY(hipStreamWaitEvent(s, h->ev_join, 0));
+    }
+    return GLA_OK;
 }
 
 // cholRecursive! (src/cholesky.jl:37-55): halve until n <= leaf, then the
@@ -596,9 +647,11 @@ int chol_device(Handle *h, hipStream_t s, int64_t n, T *A, int64_t lda, int algo
         int nb = (int)std::min<int64_t>(std::max<int64_t>(param, 1), kPanelNB);
         return chol_blocked_dev<T>(h, s, n, A, lda, nb, 0, Linv);
     }
+    if (algo == 3) return chol_la_dev<T>(h, s, n, A, lda, (int)param, 0, Linv);
     // cholRecursive!: the reference recurses to 1×1 (cutoff does not reach
     // the A11 branch, :46); on the device the recursion stops at a leaf of
     // at least 2048 (or `cutoff` if larger) that the blocked driver factors.
+    if (h->chol_block > kPanelNB) return chol_la_dev<T>(h, s, n, A, lda, (int)h->chol_block, 0, Linv);
     int64_t leaf = std::max<int64_t>(param, 2048);
     return chol_recursive_dev<T>(h, s, n, A, lda, leaf, 0, Linv);
 }

@@ -52,6 +52,9 @@ struct Handle {
     void *ws = nullptr;               // device workspace
     size_t ws_bytes = 0;
     int num_cus = 256;
+    // Tuning: block width of the look-ahead driver behind cholRecursive!
+    // (0: the recursive-halving driver).  Set with gla_set_tuning.
+    int64_t chol_block = 0;
 };
 
 // Ensure the workspace holds at least `bytes` (grows with hipMalloc, which

@@ -436,7 +436,7 @@ double backward_error(const std::vector<double> &rr, const std::vector<double> &
     for (size_t k = 0; k < rr.size(); ++k) {
         const double den = anorm * std::sqrt(xx[k]) + std::sqrt(bb[k]);
         const double e = den > 0 ? std::sqrt(rr[k]) / den : std::sqrt(rr[k]);
-        eta = std::max(eta, e);
+        if (!(e <= eta)) eta = e;   // a NaN in any column makes η NaN
     }
     return eta;
 }

@@ -45,6 +45,23 @@ const char *gla_version(void) { return "gla_hip 0.1 (gfx950)"; }
 
 const char *gla_last_error_string(void) { return gla::g_err; }
 
+int gla_set_tuning(gla_handle_t hh, const char *key, int64_t value)
+{
+    auto *h = reinterpret_cast<gla::Handle *>(hh);
+    if (!h) { gla::set_error("null handle"); return -1; }
+    if (!key) return -2;
+    if (std::strcmp(key, "chol_block") == 0) {
+        if (value != 0 && (value < 128 || value % 128 != 0)) {
+            gla::set_error("chol_block must be 0 or a multiple of 128");
+            return -3;
+        }
+        h->chol_block = value;
+        return GLA_OK;
+    }
+    gla::set_error("unknown tuning key '%s'", key);
+    return -2;
+}
+
 int gla_handle_create(gla_handle_t *out)
 {
     using gla::Handle;

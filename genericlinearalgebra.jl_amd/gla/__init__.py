@@ -75,6 +75,11 @@ def _handle(dev: int) -> int:
     return h
 
 
+def set_tuning(key: str, value: int, device: int = 0):
+    """Sets a tuning knob of the device's handle (see gla_set_tuning)."""
+    _check(lib().gla_set_tuning(_handle(device), key.encode(), int(value)), "gla_set_tuning")
+
+
 def _check(rc: int, what: str):
     if rc != GLA_OK:
         msg = lib().gla_last_error_string().decode()

@@ -46,6 +46,12 @@ int gla_handle_destroy(gla_handle_t h);
 const char *gla_last_error_string(void);
 /* Version string of this build. */
 const char *gla_version(void);
+/* Tuning knobs of a handle (no reference counterpart).  "chol_block": 0
+ * (default) runs cholRecursive! as recursive halving down to 2048-wide
+ * leaves; a multiple of 128 runs it as a right-looking blocked
+ * factorisation with blocks of that width and depth-1 look-ahead.  Both
+ * compute the same factor. */
+int gla_set_tuning(gla_handle_t h, const char *key, int64_t value);
 
 /* ------------------------------------------------------------ Cholesky -- */
 /* cholUnblocked!(A, Val{:L}) — src/cholesky.jl:3-15.
