@@ -304,8 +304,13 @@ __global__ __launch_bounds__(kPanelThreads) void chol_solve_kernel(int nb, const
                                                                    int64_t ldl,
                                                                    const T *__restrict__ LinvG,
                                                                    T *__restrict__ B, int64_t ldb,
-                                                                   int64_t nrows)
+                                                                   int64_t nrows, int64_t sL,
+                                                                   int64_t sLinv, int64_t sB)
 {
+    // batched form: blockIdx.y selects (L, LinvG, B) at the given strides
+    L += blockIdx.y * sL;
+    LinvG += blockIdx.y * sLinv;
+    B += blockIdx.y * sB;
     typedef PanelMfma<T> M;
     typedef typename M::acc_t acc_t;
     constexpr int NBLK = kPanelNB / 16;             // 8
@@ -428,9 +433,32 @@ int launch_solve(hipStream_t s, int nb, const T *L, int64_t ldl, const T *LinvG,
 {
     if (nrows <= 0) return GLA_OK;
     hipLaunchKernelGGL((chol_solve_kernel<T>), dim3((unsigned)cdiv(nrows, kPanelRB)),
-                       dim3(kPanelThreads), 0, s, nb, L, ldl, LinvG, B, ldb, nrows);
+                       dim3(kPanelThreads), 0, s, nb, L, ldl, LinvG, B, ldb, nrows, 0, 0, 0);
     GLA_LAUNCH_CHECK();
     return GLA_OK;
+}
+
+// `batch` independent solves at strides sL, sLinv, sB (one grid row each).
+template <typename T>
+int launch_solve_batched(hipStream_t s, int nb, const T *L, int64_t ldl, int64_t sL,
+                         const T *LinvG, int64_t sLinv, T *B, int64_t ldb, int64_t sB,
+                         int64_t nrows, int64_t batch)
+{
+    if (nrows <= 0 || batch <= 0) return GLA_OK;
+    hipLaunchKernelGGL((chol_solve_kernel<T>), dim3((unsigned)cdiv(nrows, kPanelRB), (unsigned)batch),
+                       dim3(kPanelThreads), 0, s, nb, L, ldl, LinvG, B, ldb, nrows, sL, sLinv, sB);
+    GLA_LAUNCH_CHECK();
+    return GLA_OK;
+}
+
+template <typename T>
+__global__ void identity_blocks_kernel(int64_t nblk, T *D)
+{
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e < nblk * kPanelNB * kPanelNB) {
+        const int64_t r = e % kPanelNB, c = (e / kPanelNB) % kPanelNB;
+        D[e] = r == c ? T(1) : T(0);
+    }
 }
 
 template <typename T>
@@ -617,13 +645,39 @@ int trsm_rlt(hipStream_t s, int64_t m, int64_t n, const T *L, int64_t ldl, const
 }
 size_t tile_inverse_elems(int64_t n) { return linv_elems(n); }
 
+// Inverses of the 128×128 diagonal blocks of a lower n×n L, transposed:
+// block b (rows/columns 128b .. 128b+127, the last one possibly partial)
+// gets Dt_b = L_bb⁻ᵀ, column-major 128×128 at Dt + b·128².  Linv holds the
+// 16×16 diagonal-tile inverses of L (tile_inverses).  Two launches: the
+// identity, then every block's rdiv! I·L_bb⁻ᵀ at once.
+template <typename T>
+int diag_block_inverses(hipStream_t s, int64_t n, const T *L, int64_t ldl, const T *Linv, T *Dt)
+{
+    if (n <= 0) return GLA_OK;
+    const int64_t nblk = cdiv(n, kPanelNB), nfull = n / kPanelNB;
+    const int64_t bs = (int64_t)kPanelNB * kPanelNB;
+    hipLaunchKernelGGL((identity_blocks_kernel<T>), dim3((unsigned)cdiv(nblk * bs, 256)), dim3(256),
+                       0, s, nblk, Dt);
+    GLA_LAUNCH_CHECK();
+    const int64_t dstride = kPanelNB * (1 + ldl), lstride = (kPanelNB / 16) * 256;
+    GLA_TRY(launch_solve_batched<T>(s, kPanelNB, L, ldl, dstride, Linv, lstride, Dt, kPanelNB, bs,
+                                    kPanelNB, nfull));
+    if (nfull < nblk)
+        GLA_TRY(launch_solve_batched<T>(s, (int)(n - nfull * kPanelNB), L + nfull * dstride, ldl, 0,
+                                        Linv + nfull * lstride, 0, Dt + nfull * bs, kPanelNB, 0,
+                                        kPanelNB, 1));
+    return GLA_OK;
+}
+
 #define GLA_INST_BLOCKS(T)                                                                    \
     template int chol_factor_block<T>(hipStream_t, int, T *, int64_t, int *, int64_t, T *);  \
     template int chol_solve_block<T>(hipStream_t, int, const T *, int64_t, const T *, T *,   \
                                      int64_t, int64_t);                                      \
     template int tile_inverses<T>(hipStream_t, int64_t, const T *, int64_t, T *);            \
     template int trsm_rlt<T>(hipStream_t, int64_t, int64_t, const T *, int64_t, const T *,   \
-                             T *, int64_t);
+                             T *, int64_t); \
+    template int diag_block_inverses<T>(hipStream_t, int64_t, const T *, int64_t, const T *, T *);
+
 GLA_INST_BLOCKS(double)
 GLA_INST_BLOCKS(float)
 #undef GLA_INST_BLOCKS

@@ -96,6 +96,29 @@ template <typename T>
 int trsm_rlt(hipStream_t s, int64_t m, int64_t n, const T *L, int64_t ldl, const T *Linv, T *B,
              int64_t ldb);
 size_t tile_inverse_elems(int64_t n);
+// Dt_b = L_bb⁻ᵀ for every 128×128 diagonal block b of L (column-major
+// 128×128 at Dt + b·128²), from the tile inverses Linv.
+template <typename T>
+int diag_block_inverses(hipStream_t s, int64_t n, const T *L, int64_t ldl, const T *Linv, T *Dt);
+
+// Multi-right-hand-side lower triangular solve (trsv.hip): Y = L⁻¹ R for up
+// to 16 right-hand sides held row-major (element (i, k) at [16 i + k]),
+// blocked by 128 rows, one launch.  Dt from diag_block_inverses.  rev:
+// row i of the solve reads R row n−1−i and adds into X row n−1−i (the
+// backward solve Lᵀ z = y run as a lower solve with the reversed matrix
+// J Lᵀ J).  If X is given, X[·, k] += Y[·, k] for k < nrhs (FP64,
+// column-major).  Y is scratch of the same layout; flags: one int per
+// 128-row block, values below `epoch`; err is raised if a wait times out.
+template <typename T>
+int trsv16(Handle *h, hipStream_t s, int64_t n, const T *L, int64_t ldl, const T *Dt, const T *R,
+           T *Y, int rev, double *X, int64_t ldx, int nrhs, int *flags, int epoch, int *err);
+// R (row-major n×16, type T) = B − sym(A) X for the lower-stored FP64 A and
+// nrhs ≤ 16 columns of X, B (column-major); Xt: n×16 FP64 scratch;
+// nrm[k] += ‖r_k‖², nrm[16 + k] += ‖x_k‖² (nrm zeroed by the caller).
+template <typename T>
+int symv_residual(hipStream_t s, int64_t n, const double *A, int64_t lda, const double *X,
+                  int64_t ldx, const double *B, int64_t ldb, int nrhs, double *Xt, T *R,
+                  double *nrm);
 
 // QR building blocks (qr.hip): T of the first k reflectors of F (m×k), and
 // A (mA×nA) <- op(Q) A, trans 'N' or 'T', Q from the first b reflectors.

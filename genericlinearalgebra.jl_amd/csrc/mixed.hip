@@ -268,27 +268,6 @@ __global__ void reverse_transpose_lower(int64_t n, const T *L, int64_t ldl, T *L
     GLA_ROW(i);
     GLA_COLS(j, n) if (i < n && i >= j) Lr[i + j * ldr] = L[(n - 1 - j) + (n - 1 - i) * ldl];
 }
-// Rt (nrhs × n, type T) <- R (n × nrhs, FP64) transposed, rows optionally reversed
-template <typename T>
-__global__ void rhs_to_t(int64_t n, int64_t nrhs, const double *R, int64_t ldr, T *Rt, int rev)
-{
-    GLA_ROW(i);
-    GLA_COLS(k, nrhs) if (i < n) Rt[k + (rev ? n - 1 - i : i) * nrhs] = (T)R[i + k * ldr];
-}
-// X (n × nrhs, FP64) += Zt (nrhs × n, type T) transposed, rows optionally reversed
-template <typename T>
-__global__ void t_add_rhs(int64_t n, int64_t nrhs, const T *Zt, double *X, int64_t ldx, int rev)
-{
-    GLA_ROW(i);
-    GLA_COLS(k, nrhs) if (i < n) X[i + k * ldx] += (double)Zt[k + (rev ? n - 1 - i : i) * nrhs];
-}
-// Yt (nrhs × n) <- Zt with its columns in reverse order
-template <typename T>
-__global__ void t_reverse(int64_t n, int64_t nrhs, const T *Zt, T *Yt)
-{
-    GLA_ROW(i);
-    GLA_COLS(k, nrhs) if (i < n) Yt[k + (n - 1 - i) * nrhs] = Zt[k + i * nrhs];
-}
 __global__ void copy2d_d(int64_t m, int64_t n, const double *S, int64_t lds, double *D, int64_t ldd)
 {
     GLA_ROW(i);
@@ -307,25 +286,6 @@ inline dim3 g2(int64_t rows, int64_t cols)
     return dim3((unsigned)cdiv(rows, 256), (unsigned)std::min<int64_t>(cols, 65535));
 }
 
-// out[k] = Σ_i X[i + k*ldx]² over column k (one workgroup per column)
-__global__ __launch_bounds__(256) void col_sumsq(int64_t n, const double *X, int64_t ldx,
-                                                 double *out)
-{
-    __shared__ double red[256];
-    const int64_t k = blockIdx.x;
-    double s = 0.0;
-    for (int64_t i = threadIdx.x; i < n; i += 256) {
-        const double v = X[i + k * ldx];
-        s += v * v;
-    }
-    red[threadIdx.x] = s;
-    __syncthreads();
-    for (int w = 128; w > 0; w >>= 1) {
-        if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
-        __syncthreads();
-    }
-    if (threadIdx.x == 0) out[k] = red[0];
-}
 // out[blockIdx.y] = Σ of squares over the lower triangle weighted to the
 // full symmetric matrix (off-diagonal counted twice): ‖A‖_F² by columns.
 __global__ __launch_bounds__(256) void sym_sumsq_cols(int64_t n, const double *A, int64_t lda,
@@ -402,182 +362,252 @@ int chol_mixed(Handle *h, hipStream_t s, int64_t n, float *W, int64_t ldw, doubl
     return GLA_OK;
 }
 
-// X += (L Lᵀ)⁻¹ R with L (type T, lower) and its reversal Lr, both with their
-// 16×16 diagonal-tile inverses.  Rt, Yt: nrhs × n scratch of type T.
-template <typename T>
-int correction(hipStream_t s, int64_t n, int64_t nrhs, const T *L, const T *Linv, const T *Lr,
-               const T *Linvr, const double *R, int64_t ldr, double *X, int64_t ldx, T *Rt, T *Yt)
+__global__ void fill_int(int64_t n, int *p, int v)
 {
-    hipLaunchKernelGGL((rhs_to_t<T>), g2(n, nrhs), dim3(256), 0, s, n, nrhs, R, ldr, Rt, 0);
-    GLA_LAUNCH_CHECK();
-    GLA_TRY(trsm_rlt<T>(s, nrhs, n, L, n, Linv, Rt, nrhs));          // yᵀ = rᵀ L⁻ᵀ
-    hipLaunchKernelGGL((t_reverse<T>), g2(n, nrhs), dim3(256), 0, s, n, nrhs, Rt, Yt);
-    GLA_LAUNCH_CHECK();
-    GLA_TRY(trsm_rlt<T>(s, nrhs, n, Lr, n, Linvr, Yt, nrhs));        // (P z)ᵀ
-    hipLaunchKernelGGL((t_add_rhs<T>), g2(n, nrhs), dim3(256), 0, s, n, nrhs, Yt, X, ldx, 1);
-    GLA_LAUNCH_CHECK();
-    return GLA_OK;
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+__global__ void fill_dbl(int64_t n, double *p)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0.0;
 }
 
-template <typename T>
-int prepare_solves(hipStream_t s, int64_t n, const T *L, T *Lr, T *Linv, T *Linvr)
-{
-    GLA_TRY(tile_inverses<T>(s, n, L, n, Linv));
-    hipLaunchKernelGGL((reverse_transpose_lower<T>), g2(n, n), dim3(256), 0, s, n, L, n, Lr, n);
-    GLA_LAUNCH_CHECK();
-    return tile_inverses<T>(s, n, Lr, n, Linvr);
-}
-
-// η_k = ‖r_k‖ / (‖A‖ ‖x_k‖ + ‖b_k‖), returned as the max over k.
-double backward_error(const std::vector<double> &rr, const std::vector<double> &xx,
-                      const std::vector<double> &bb, double anorm)
+// η = max_k ‖r_k‖ / (‖A‖ ‖x_k‖ + ‖b_k‖); a NaN in any column makes η NaN.
+double backward_error(const double *rr, const double *xx, const double *bb, int64_t nrhs,
+                      double anorm)
 {
     double eta = 0.0;
-    for (size_t k = 0; k < rr.size(); ++k) {
+    for (int64_t k = 0; k < nrhs; ++k) {
         const double den = anorm * std::sqrt(xx[k]) + std::sqrt(bb[k]);
         const double e = den > 0 ? std::sqrt(rr[k]) / den : std::sqrt(rr[k]);
-        if (!(e <= eta)) eta = e;   // a NaN in any column makes η NaN
+        if (!(e <= eta)) eta = e;
     }
     return eta;
 }
 
+// Workspace of one mixed solve (element type T of the factor).
+template <typename T>
+struct MixedWS {
+    T *W, *Lr, *Linv, *Linvr, *Dt, *Dtr, *Rt, *Yt, *Zt;
+    double *Xt, *nrm, *D64, *Linv64;
+    float *Linv32;
+    int *flags, *err;
+    int64_t nblk, npad, nchunk;
+
+    static size_t bytes(int64_t n, int64_t nrhs)
+    {
+        const int64_t nblk = cdiv(n, 128), npad = nblk * 128, nch = cdiv(nrhs, 16);
+        const size_t es = sizeof(T), r = 255;
+        auto al = [&](size_t b) { return (b + r) / 256 * 256; };
+        return 2 * al(es * (size_t)n * n) + 2 * al(es * tile_inverse_elems(n)) +
+               2 * al(es * (size_t)npad * 128) + al(es * (size_t)npad * 16 * nch) +
+               2 * al(es * (size_t)npad * 16) + al(8 * (size_t)npad * 16) +
+               al(8 * (size_t)(32 * nch + n + 64)) + al(8 * 128 * 128) + al(8 * 8 * 256) +
+               al(4 * 8 * 256) + al(4 * (size_t)(nblk + 64)) + 4096;
+    }
+    void carve(void *base, int64_t n, int64_t nrhs)
+    {
+        nblk = cdiv(n, 128);
+        npad = nblk * 128;
+        nchunk = cdiv(nrhs, 16);
+        char *p = reinterpret_cast<char *>(base);
+        auto take = [&](size_t b) {
+            char *q = p;
+            p += (b + 255) / 256 * 256;
+            return q;
+        };
+        const size_t es = sizeof(T);
+        W = reinterpret_cast<T *>(take(es * (size_t)n * n));
+        Lr = reinterpret_cast<T *>(take(es * (size_t)n * n));
+        Linv = reinterpret_cast<T *>(take(es * tile_inverse_elems(n)));
+        Linvr = reinterpret_cast<T *>(take(es * tile_inverse_elems(n)));
+        Dt = reinterpret_cast<T *>(take(es * (size_t)npad * 128));
+        Dtr = reinterpret_cast<T *>(take(es * (size_t)npad * 128));
+        Rt = reinterpret_cast<T *>(take(es * (size_t)npad * 16 * nchunk));
+        Yt = reinterpret_cast<T *>(take(es * (size_t)npad * 16));
+        Zt = reinterpret_cast<T *>(take(es * (size_t)npad * 16));
+        Xt = reinterpret_cast<double *>(take(8 * (size_t)npad * 16));
+        nrm = reinterpret_cast<double *>(take(8 * (size_t)(32 * nchunk + n + 64)));
+        D64 = reinterpret_cast<double *>(take(8 * 128 * 128));
+        Linv64 = reinterpret_cast<double *>(take(8 * 8 * 256));
+        Linv32 = reinterpret_cast<float *>(take(4 * 8 * 256));
+        flags = reinterpret_cast<int *>(take(4 * (size_t)(nblk + 64)));
+        err = flags + nblk;
+    }
+};
+
+// Solve structures of the factor in ws.W: tile inverses, diagonal-block
+// inverses, and the same for the reversed Lr = J Lᵀ J (backward solve).
+template <typename T>
+int prepare_solves(hipStream_t s, int64_t n, MixedWS<T> &ws)
+{
+    GLA_TRY(tile_inverses<T>(s, n, ws.W, n, ws.Linv));
+    GLA_TRY(diag_block_inverses<T>(s, n, ws.W, n, ws.Linv, ws.Dt));
+    hipLaunchKernelGGL((reverse_transpose_lower<T>), g2(n, n), dim3(256), 0, s, n, ws.W, n, ws.Lr,
+                       n);
+    GLA_LAUNCH_CHECK();
+    GLA_TRY(tile_inverses<T>(s, n, ws.Lr, n, ws.Linvr));
+    return diag_block_inverses<T>(s, n, ws.Lr, n, ws.Linvr, ws.Dtr);
+}
+
+// Refinement outcome.
+enum { kConverged = 0, kMaxIter = 1, kStalled = 2 };
+
+// X = (L Lᵀ)⁻¹ B refined in FP64 until η ≤ tol.  Stops as stalled when η
+// fails to halve over two corrections or is not finite.
 template <typename T>
 int refine(Handle *h, hipStream_t s, int64_t n, const double *A, int64_t lda, int64_t nrhs,
-           const double *B, int64_t ldb, double *X, int64_t ldx, const T *L, const T *Lr,
-           const T *Linv, const T *Linvr, double *Rbuf, T *Rt, T *Yt, double *norms, double anorm,
-           double tol, int max_iter, int *iters, double *berr)
+           const double *B, int64_t ldb, double *X, int64_t ldx, MixedWS<T> &ws, double anorm,
+           double tol, int max_iter, int &epoch, int *iters, double *berr, int *outcome)
 {
-    std::vector<double> rr(nrhs), xx(nrhs), bb(nrhs);
-    hipLaunchKernelGGL(col_sumsq, dim3((unsigned)nrhs), dim3(256), 0, s, n, B, ldb, norms);
-    GLA_LAUNCH_CHECK();
-    GLA_HIP_TRY(hipMemcpyAsync(bb.data(), norms, sizeof(double) * nrhs, hipMemcpyDeviceToHost, s));
-    // x0 = 0 + (L Lᵀ)⁻¹ b
+    const int64_t nch = ws.nchunk;
+    std::vector<double> hn((size_t)32 * nch), bb((size_t)nrhs), rr((size_t)nrhs), xx((size_t)nrhs);
     hipLaunchKernelGGL(zero2d_d, g2(n, nrhs), dim3(256), 0, s, n, nrhs, X, ldx);
     GLA_LAUNCH_CHECK();
-    hipLaunchKernelGGL(copy2d_d, g2(n, nrhs), dim3(256), 0, s, n, nrhs, B, ldb, Rbuf, n);
-    GLA_LAUNCH_CHECK();
-    GLA_TRY(correction<T>(s, n, nrhs, L, Linv, Lr, Linvr, Rbuf, n, X, ldx, Rt, Yt));
-    int it = 0;
-    double eta = 0.0;
-    for (;;) {
-        // r = b − A x  (FP64 MFMA GEMM)
-        hipLaunchKernelGGL(copy2d_d, g2(n, nrhs), dim3(256), 0, s, n, nrhs, B, ldb, Rbuf, n);
+    double eta = INFINITY, prev2 = INFINITY, prev1 = INFINITY;
+    int corr = 0;
+    *outcome = kMaxIter;
+    for (int it = 0;; ++it) {
+        // r = b − A x per chunk of 16 columns, with ‖r_k‖², ‖x_k‖²
+        hipLaunchKernelGGL(fill_dbl, dim3((unsigned)cdiv(32 * nch, 256)), dim3(256), 0, s, 32 * nch,
+                           ws.nrm);
         GLA_LAUNCH_CHECK();
-        GLA_TRY(gemm<double>(s, 'N', 'N', 0, n, nrhs, n, -1.0, A, lda, X, ldx, 1.0, Rbuf, n));
-        hipLaunchKernelGGL(col_sumsq, dim3((unsigned)nrhs), dim3(256), 0, s, n, Rbuf, n, norms);
-        GLA_LAUNCH_CHECK();
-        hipLaunchKernelGGL(col_sumsq, dim3((unsigned)nrhs), dim3(256), 0, s, n, X, ldx,
-                           norms + nrhs);
-        GLA_LAUNCH_CHECK();
-        GLA_HIP_TRY(hipMemcpyAsync(rr.data(), norms, sizeof(double) * nrhs, hipMemcpyDeviceToHost, s));
-        GLA_HIP_TRY(hipMemcpyAsync(xx.data(), norms + nrhs, sizeof(double) * nrhs,
+        for (int64_t c = 0; c < nch; ++c) {
+            const int nr = (int)std::min<int64_t>(16, nrhs - 16 * c);
+            GLA_TRY(symv_residual<T>(s, n, A, lda, X + 16 * c * ldx, ldx, B + 16 * c * ldb, ldb, nr,
+                                     ws.Xt, ws.Rt + c * ws.npad * 16, ws.nrm + 32 * c));
+        }
+        int herr = 0;
+        GLA_HIP_TRY(hipMemcpyAsync(hn.data(), ws.nrm, sizeof(double) * 32 * nch,
                                    hipMemcpyDeviceToHost, s));
+        GLA_HIP_TRY(hipMemcpyAsync(&herr, ws.err, sizeof(int), hipMemcpyDeviceToHost, s));
         GLA_HIP_TRY(hipStreamSynchronize(s));
-        eta = backward_error(rr, xx, bb, anorm);
-        if (eta <= tol || it >= max_iter || !std::isfinite(eta)) break;
-        GLA_TRY(correction<T>(s, n, nrhs, L, Linv, Lr, Linvr, Rbuf, n, X, ldx, Rt, Yt));
-        ++it;
+        if (herr) {
+            set_error("triangular solve: a dependency wait timed out");
+            return GLA_ERR_HIP;
+        }
+        for (int64_t k = 0; k < nrhs; ++k) {
+            rr[k] = hn[32 * (k / 16) + k % 16];
+            xx[k] = hn[32 * (k / 16) + 16 + k % 16];
+        }
+        if (it == 0) {
+            bb = rr;   // x = 0: r = b
+        } else {
+            eta = backward_error(rr.data(), xx.data(), bb.data(), nrhs, anorm);
+            if (eta <= tol) {
+                *outcome = kConverged;
+                break;
+            }
+            if (!std::isfinite(eta) || (it >= 3 && eta > 0.5 * prev2)) {
+                *outcome = kStalled;
+                break;
+            }
+            if (corr >= max_iter) break;
+            ++corr;
+            prev2 = prev1;
+            prev1 = eta;
+        }
+        // x += (L Lᵀ)⁻¹ r: forward solve, then the backward one as a lower
+        // solve with Lr on reversed rows, adding into X
+        for (int64_t c = 0; c < nch; ++c) {
+            const int nr = (int)std::min<int64_t>(16, nrhs - 16 * c);
+            GLA_TRY(trsv16<T>(h, s, n, ws.W, n, ws.Dt, ws.Rt + c * ws.npad * 16, ws.Yt, 0, nullptr,
+                              0, nr, ws.flags, ++epoch, ws.err));
+            GLA_TRY(trsv16<T>(h, s, n, ws.Lr, n, ws.Dtr, ws.Yt, ws.Zt, 1, X + 16 * c * ldx, ldx, nr,
+                              ws.flags, ++epoch, ws.err));
+        }
     }
-    if (iters) *iters = it;
-    if (berr) *berr = eta;
+    *iters = corr;
+    *berr = eta;
     return GLA_OK;
 }
 
 }  // namespace
 
-// Device workspace layout of one mixed solve, in bytes.
 static size_t mixed_ws_bytes(char prec, int64_t n, int64_t nrhs)
 {
-    const size_t es = prec == 'd' ? 8 : 4;
-    const size_t tiles = tile_inverse_elems(n);
-    return 2 * es * (size_t)n * n              // W, Lr
-           + 2 * es * tiles                     // tile inverses of both
-           + 8 * (size_t)n * nrhs               // FP64 residual
-           + 2 * es * (size_t)n * nrhs          // transposed right-hand sides
-           + 8 * (size_t)(128 * 128 + 8 * 256)  // FP64 diagonal block and its inverses
-           + 4 * 8 * 256                        // FP32 inverses
-           + 8 * (size_t)(2 * nrhs + n + 64)    // norms
-           + 1024;
+    return prec == 'd' ? MixedWS<double>::bytes(n, nrhs) : MixedWS<float>::bytes(n, nrhs);
 }
 
-int mixed_solve(Handle *h, hipStream_t s, char prec, int64_t n, const double *A, int64_t lda,
-                int64_t nrhs, const double *B, int64_t ldb, double *X, int64_t ldx, double tol,
-                int max_iter, int *iters, double *berr, int64_t *info)
+// One attempt at precision `prec`: factor, then refine.  *info > 0 when the
+// factorisation meets a negative pivot.
+template <typename T>
+int mixed_attempt(Handle *h, hipStream_t s, char prec, int64_t n, const double *A, int64_t lda,
+                  int64_t nrhs, const double *B, int64_t ldb, double *X, int64_t ldx, double tol,
+                  int max_iter, double anorm, int *iters, double *berr, int64_t *info, int *outcome)
 {
-    GLA_TRY(ensure_workspace(h, mixed_ws_bytes(prec, n, nrhs)));
-    char *ws = reinterpret_cast<char *>(h->ws);
-    auto take = [&](size_t bytes) {
-        char *p = ws;
-        ws += (bytes + 255) / 256 * 256;
-        return p;
-    };
-    const size_t es = prec == 'd' ? 8 : 4;
-    void *W = take(es * (size_t)n * n);
-    void *Lr = take(es * (size_t)n * n);
-    void *Linv = take(es * tile_inverse_elems(n));
-    void *Linvr = take(es * tile_inverse_elems(n));
-    double *Rbuf = reinterpret_cast<double *>(take(8 * (size_t)n * nrhs));
-    void *Rt = take(es * (size_t)n * nrhs);
-    void *Yt = take(es * (size_t)n * nrhs);
-    double *D64 = reinterpret_cast<double *>(take(8 * 128 * 128));
-    double *Linv64 = reinterpret_cast<double *>(take(8 * 8 * 256));
-    float *Linv32 = reinterpret_cast<float *>(take(4 * 8 * 256));
-    double *norms = reinterpret_cast<double *>(take(8 * (size_t)(2 * nrhs + n + 64)));
-
+    MixedWS<T> ws;
+    ws.carve(h->ws, n, nrhs);
+    hipLaunchKernelGGL(fill_int, dim3((unsigned)cdiv(ws.nblk + 1, 256)), dim3(256), 0, s,
+                       ws.nblk + 1, ws.flags, 0);
+    GLA_LAUNCH_CHECK();
     hipLaunchKernelGGL(k_set_int, dim3(1), dim3(1), 0, s, h->dinfo, 0);
     GLA_LAUNCH_CHECK();
-    // ‖A‖_F from the lower triangle
-    hipLaunchKernelGGL(sym_sumsq_cols, dim3((unsigned)n), dim3(256), 0, s, n, A, lda,
-                       norms + 2 * nrhs);
-    GLA_LAUNCH_CHECK();
-    std::vector<double> cs(n);
-    GLA_HIP_TRY(hipMemcpyAsync(cs.data(), norms + 2 * nrhs, sizeof(double) * n,
-                               hipMemcpyDeviceToHost, s));
-
-    if (prec == 'd') {
-        double *Wd = reinterpret_cast<double *>(W);
-        hipLaunchKernelGGL(copy2d_d, g2(n, n), dim3(256), 0, s, n, n, A, lda, Wd, n);
+    if constexpr (sizeof(T) == 8) {
+        hipLaunchKernelGGL(copy2d_d, g2(n, n), dim3(256), 0, s, n, n, A, lda, ws.W, n);
         GLA_LAUNCH_CHECK();
-        GLA_TRY(chol_device<double>(h, s, n, Wd, n, 2, 1, reinterpret_cast<double *>(Linv)));
+        GLA_TRY(chol_device<double>(h, s, n, ws.W, n, 2, 1, ws.Linv));
     } else {
-        float *Wf = reinterpret_cast<float *>(W);
-        hipLaunchKernelGGL(cvt_lower_d2s, g2(n, n), dim3(256), 0, s, n, A, lda, Wf, n);
+        hipLaunchKernelGGL(cvt_lower_d2s, g2(n, n), dim3(256), 0, s, n, A, lda, ws.W, n);
         GLA_LAUNCH_CHECK();
         if (prec == 's')
-            GLA_TRY(chol_device<float>(h, s, n, Wf, n, 2, 1, reinterpret_cast<float *>(Linv)));
-        else if (prec == 'b') GLA_TRY(chol_mixed<__bf16>(h, s, n, Wf, n, D64, Linv64, Linv32));
-        else GLA_TRY(chol_mixed<_Float16>(h, s, n, Wf, n, D64, Linv64, Linv32));
+            GLA_TRY(chol_device<float>(h, s, n, ws.W, n, 2, 1, ws.Linv));
+        else if (prec == 'b')
+            GLA_TRY(chol_mixed<__bf16>(h, s, n, ws.W, n, ws.D64, ws.Linv64, ws.Linv32));
+        else
+            GLA_TRY(chol_mixed<_Float16>(h, s, n, ws.W, n, ws.D64, ws.Linv64, ws.Linv32));
     }
     int hinfo = 0;
     GLA_HIP_TRY(hipMemcpyAsync(&hinfo, h->dinfo, sizeof(int), hipMemcpyDeviceToHost, s));
     GLA_HIP_TRY(hipStreamSynchronize(s));
-    if (info) *info = hinfo;
+    *info = hinfo;
     if (hinfo != 0) {
-        if (iters) *iters = 0;
-        if (berr) *berr = INFINITY;
+        *iters = 0;
+        *berr = INFINITY;
+        *outcome = kStalled;
         return GLA_OK;
     }
+    GLA_TRY(prepare_solves<T>(s, n, ws));
+    int epoch = 0;
+    return refine<T>(h, s, n, A, lda, nrhs, B, ldb, X, ldx, ws, anorm, tol, max_iter, epoch, iters,
+                     berr, outcome);
+}
+
+int mixed_solve(Handle *h, hipStream_t s, char prec, int64_t n, const double *A, int64_t lda,
+                int64_t nrhs, const double *B, int64_t ldb, double *X, int64_t ldx, double tol,
+                int max_iter, int *iters, double *berr, int64_t *info, char *used)
+{
+    // precision ladder: 'a' tries BF16, then FP32, then FP64
+    const char ladder_a[] = {'b', 's', 'd'};
+    const char *ladder = prec == 'a' ? ladder_a : &prec;
+    const int nl = prec == 'a' ? 3 : 1;
+    size_t need = 0;
+    for (int i = 0; i < nl; ++i) need = std::max(need, mixed_ws_bytes(ladder[i], n, nrhs));
+    // ‖A‖_F from the lower triangle: column sums in scratch past the layouts
+    GLA_TRY(ensure_workspace(h, need + 8 * (size_t)(n + 64)));
+    double *cs = reinterpret_cast<double *>(reinterpret_cast<char *>(h->ws) + need);
+    std::vector<double> hcs(n);
+    hipLaunchKernelGGL(sym_sumsq_cols, dim3((unsigned)n), dim3(256), 0, s, n, A, lda, cs);
+    GLA_LAUNCH_CHECK();
+    GLA_HIP_TRY(hipMemcpyAsync(hcs.data(), cs, sizeof(double) * n, hipMemcpyDeviceToHost, s));
+    GLA_HIP_TRY(hipStreamSynchronize(s));
     double a2 = 0.0;
-    for (int64_t j = 0; j < n; ++j) a2 += cs[j];
+    for (int64_t j = 0; j < n; ++j) a2 += hcs[j];
     const double anorm = std::sqrt(a2);
-    if (prec == 'd') {
-        double *Wd = reinterpret_cast<double *>(W);
-        GLA_TRY(prepare_solves<double>(s, n, Wd, reinterpret_cast<double *>(Lr),
-                                       reinterpret_cast<double *>(Linv),
-                                       reinterpret_cast<double *>(Linvr)));
-        return refine<double>(h, s, n, A, lda, nrhs, B, ldb, X, ldx, Wd,
-                              reinterpret_cast<double *>(Lr), reinterpret_cast<double *>(Linv),
-                              reinterpret_cast<double *>(Linvr), Rbuf,
-                              reinterpret_cast<double *>(Rt), reinterpret_cast<double *>(Yt), norms,
-                              anorm, tol, max_iter, iters, berr);
+    for (int i = 0; i < nl; ++i) {
+        const char p = ladder[i];
+        int outcome = kMaxIter;
+        if (p == 'd')
+            GLA_TRY(mixed_attempt<double>(h, s, p, n, A, lda, nrhs, B, ldb, X, ldx, tol, max_iter,
+                                          anorm, iters, berr, info, &outcome));
+        else
+            GLA_TRY(mixed_attempt<float>(h, s, p, n, A, lda, nrhs, B, ldb, X, ldx, tol, max_iter,
+                                         anorm, iters, berr, info, &outcome));
+        if (used) *used = p;
+        if (outcome == kConverged || i == nl - 1) break;
     }
-    float *Wf = reinterpret_cast<float *>(W);
-    GLA_TRY(prepare_solves<float>(s, n, Wf, reinterpret_cast<float *>(Lr),
-                                  reinterpret_cast<float *>(Linv), reinterpret_cast<float *>(Linvr)));
-    return refine<float>(h, s, n, A, lda, nrhs, B, ldb, X, ldx, Wf, reinterpret_cast<float *>(Lr),
-                         reinterpret_cast<float *>(Linv), reinterpret_cast<float *>(Linvr), Rbuf,
-                         reinterpret_cast<float *>(Rt), reinterpret_cast<float *>(Yt), norms,
-                         anorm, tol, max_iter, iters, berr);
+    return GLA_OK;
 }
 
 int lp_gemm_entry(char tin, int64_t m, int64_t n, int64_t k, float alpha, const float *A,
@@ -595,12 +625,12 @@ extern "C" {
 int gla_chol_solve_mixed_d(gla_handle_t hh, char prec, int64_t n, const double *A, int64_t lda,
                            int64_t nrhs, const double *B, int64_t ldb, double *X, int64_t ldx,
                            double tol, int32_t max_iter, int32_t *iters, double *berr,
-                           int64_t *info, void *stream)
+                           int64_t *info, char *prec_used, void *stream)
 {
     auto *h = reinterpret_cast<gla::Handle *>(hh);
     if (!h) { gla::set_error("null handle"); return -1; }
-    if (prec != 'd' && prec != 's' && prec != 'b' && prec != 'h') {
-        gla::set_error("prec must be one of 'd', 's', 'b', 'h'");
+    if (prec != 'd' && prec != 's' && prec != 'b' && prec != 'h' && prec != 'a') {
+        gla::set_error("prec must be one of 'd', 's', 'b', 'h', 'a'");
         return -2;
     }
     if (n < 0) return -3;
@@ -608,17 +638,21 @@ int gla_chol_solve_mixed_d(gla_handle_t hh, char prec, int64_t n, const double *
     if (nrhs < 0) return -6;
     if (ldb < std::max<int64_t>(1, n)) return -8;
     if (ldx < std::max<int64_t>(1, n)) return -10;
-    if (n == 0 || nrhs == 0) {
-        if (iters) *iters = 0;
-        if (berr) *berr = 0.0;
-        if (info) *info = 0;
-        return GLA_OK;
-    }
-    GLA_HIP_TRY(hipSetDevice(h->device));
+    if (max_iter < 0) return -12;
     int it = 0;
-    int rc = gla::mixed_solve(h, reinterpret_cast<hipStream_t>(stream), prec, n, A, lda, nrhs, B,
-                              ldb, X, ldx, tol, max_iter, &it, berr, info);
+    double be = 0.0;
+    int64_t inf = 0;
+    char used = prec == 'a' ? 'b' : prec;
+    int rc = GLA_OK;
+    if (n > 0 && nrhs > 0) {
+        GLA_HIP_TRY(hipSetDevice(h->device));
+        rc = gla::mixed_solve(h, reinterpret_cast<hipStream_t>(stream), prec, n, A, lda, nrhs, B,
+                              ldb, X, ldx, tol, max_iter, &it, &be, &inf, &used);
+    }
     if (iters) *iters = it;
+    if (berr) *berr = be;
+    if (info) *info = inf;
+    if (prec_used) *prec_used = used;
     return rc;
 }
 

@@ -1,0 +1,333 @@
+// Solves with a Cholesky factor for a few right-hand sides, and the
+// symmetric residual, as used by the mixed-precision refinement.  No
+// counterpart in the reference (its Cholesky stops at the factor,
+// src/cholesky.jl:3-55); the solve is what `L \ (L' \ b)` would do there.
+//
+// trsv16: Y = L⁻¹ R for up to 16 right-hand sides, held row-major (row i at
+// [16 i]), in one launch.  The rows are cut into 128-row blocks; block b is
+//   Y_b = L_bb⁻¹ (R_b − Σ_{j<b} L_bj Y_j),
+// each workgroup owns blocks b ≡ blockIdx (mod grid) and streams the tiles
+// L_bj (j = 0, 1, …) through MFMA as soon as Y_j is published: a block's
+// result is written with device-coherent stores and announced by a flag
+// (epoch value) that dependent workgroups poll, with a bounded spin.  The
+// diagonal solve is one more 128×128×16 MFMA product with the explicit
+// inverse L_bb⁻¹ (from diag_block_inverses).  The grid never exceeds one
+// workgroup per CU and every wait is on a block of a smaller index, so all
+// waiters are resident and the launch cannot deadlock.
+//
+// symv_residual: R = B − A X for symmetric A given by its lower triangle
+// (FP64), 16 right-hand sides, with the column sums of squares of R and X
+// for the backward error.
+#include <algorithm>
+
+#include "gla_coherent.hpp"
+#include "gla_internal.hpp"
+
+namespace gla {
+
+namespace {
+
+constexpr int kSB = 128;                  // rows per block
+constexpr int kSR = 16;                   // right-hand sides per pass
+constexpr int kST = 256;                  // threads per workgroup
+constexpr unsigned kSolveSpin = 1u << 25;
+
+template <typename T>
+struct SMfma;
+template <>
+struct SMfma<double> {
+    typedef d4 acc_t;
+    static __device__ __forceinline__ acc_t mma(double a, double b, acc_t c)
+    {
+        return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+    }
+    static __device__ __forceinline__ int row(int l, int r) { return (l >> 4) + 4 * r; }
+};
+template <>
+struct SMfma<float> {
+    typedef f4 acc_t;
+    static __device__ __forceinline__ acc_t mma(float a, float b, acc_t c)
+    {
+        return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+    }
+    static __device__ __forceinline__ int row(int l, int r) { return 4 * (l >> 4) + r; }
+};
+
+template <typename T>
+__global__ __launch_bounds__(kST) void trsv16_kernel(int64_t n, int64_t nblk,
+                                                     const T *__restrict__ L, int64_t ldl,
+                                                     const T *__restrict__ Dt,
+                                                     const T *__restrict__ R, T *Y, int rev,
+                                                     double *__restrict__ X, int64_t ldx, int nrhs,
+                                                     int *flags, int epoch, int *err)
+{
+    typedef SMfma<T> M;
+    typedef typename M::acc_t acc_t;
+    __shared__ T Ys[2][kSB * kSR];
+    __shared__ T Ss[kSB * kSR];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int lc = lane & 15, lk = lane >> 4;
+
+    for (int64_t b = blockIdx.x; b < nblk; b += gridDim.x) {
+        const int64_t r0 = b * kSB;
+        acc_t acc[2];
+#pragma unroll
+        for (int t = 0; t < 2; ++t)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int64_t row = r0 + 32 * w + 16 * t + M::row(lane, r);
+                acc[t][r] = row < n ? R[(rev ? n - 1 - row : row) * kSR + lc] : T(0);
+            }
+        for (int64_t j = 0; j < b; ++j) {
+            T *ys = Ys[j & 1];
+            if (threadIdx.x == 0) {
+                unsigned spins = 0;
+                while (load_coherent(flags + j) < epoch) {
+                    __builtin_amdgcn_s_sleep(1);
+                    if (++spins > kSolveSpin) {
+                        store_coherent(err, 1);
+                        break;
+                    }
+                }
+            }
+            __syncthreads();
+            {
+                const T *src = Y + j * (kSB * kSR);
+                const T *p[8];
+                T v[8];
+#pragma unroll
+                for (int q = 0; q < 8; ++q) p[q] = src + threadIdx.x + q * kST;
+                load8_coherent(p, v);
+#pragma unroll
+                for (int q = 0; q < 8; ++q) ys[threadIdx.x + q * kST] = v[q];
+            }
+            __syncthreads();
+            // acc -= L_bj Y_j; block j < b is a full 128-column block
+            const T *Lj = L + (j * kSB) * ldl;
+#pragma unroll 8
+            for (int kq = 0; kq < kSB / 4; ++kq) {
+                const int k = 4 * kq + lk;
+                const T bv = ys[k * kSR + lc];
+#pragma unroll
+                for (int t = 0; t < 2; ++t) {
+                    const int64_t row = r0 + 32 * w + 16 * t + lc;
+                    const T a = row < n ? Lj[row + (int64_t)k * ldl] : T(0);
+                    acc[t] = M::mma(-a, bv, acc[t]);
+                }
+            }
+        }
+        // Y_b = L_bb⁻¹ S with L_bb⁻¹[r][k] = Dt_b[k + 128 r]
+#pragma unroll
+        for (int t = 0; t < 2; ++t)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) Ss[(32 * w + 16 * t + M::row(lane, r)) * kSR + lc] = acc[t][r];
+        __syncthreads();
+        const T *D = Dt + b * (kSB * kSB);
+        acc_t y[2];
+#pragma unroll
+        for (int t = 0; t < 2; ++t) y[t] = acc_t{T(0), T(0), T(0), T(0)};
+#pragma unroll 8
+        for (int kq = 0; kq < kSB / 4; ++kq) {
+            const int k = 4 * kq + lk;
+            const T bv = Ss[k * kSR + lc];
+#pragma unroll
+            for (int t = 0; t < 2; ++t) {
+                const int rr = 32 * w + 16 * t + lc;
+                y[t] = M::mma(D[k + (int64_t)rr * kSB], bv, y[t]);
+            }
+        }
+#pragma unroll
+        for (int t = 0; t < 2; ++t)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int64_t row = r0 + 32 * w + 16 * t + M::row(lane, r);
+                if (row < n) {
+                    store_coherent(Y + row * kSR + lc, y[t][r]);
+                    if (X && lc < nrhs) X[(rev ? n - 1 - row : row) + lc * ldx] += (double)y[t][r];
+                }
+            }
+        stores_done();
+        __syncthreads();
+        if (threadIdx.x == 0) store_coherent(flags + b, epoch);
+    }
+}
+
+// ---------------------------------------------------- symmetric residual --
+constexpr int kVB = 64;   // rows per workgroup
+
+// R (row-major n×16, type T) = B − sym(A) X for the lower-stored FP64 A,
+// Xt = X row-major FP64 (n×16, zero-padded).  nrm[k] += Σ r_k², nrm[16+k]
+// += Σ x_k² over this workgroup's rows.
+template <typename T>
+__global__ __launch_bounds__(kST) void symv_residual_kernel(int64_t n, const double *__restrict__ A,
+                                                            int64_t lda,
+                                                            const double *__restrict__ Xt,
+                                                            const double *__restrict__ B,
+                                                            int64_t ldb, int nrhs, T *__restrict__ R,
+                                                            double *nrm)
+{
+    typedef SMfma<double> M;
+    __shared__ double Tl[4][16 * 68];        // per-wave transposed staging (64 k × 16 m)
+    __shared__ double Red[4][kVB * kSR];     // per-wave partial sums
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int lc = lane & 15, lk = lane >> 4;
+    const int64_t i = blockIdx.x, r0 = i * kVB;
+    const int64_t nb = (n + kVB - 1) / kVB;
+    d4 acc[4];
+#pragma unroll
+    for (int t = 0; t < 4; ++t) acc[t] = d4{0.0, 0.0, 0.0, 0.0};
+    double *tl = Tl[w];
+    for (int64_t j = w; j < nb; j += 4) {
+        const int64_t c0 = j * kVB;
+        if (j < i) {
+            // A_ij from the lower triangle, rows contiguous
+#pragma unroll 4
+            for (int kq = 0; kq < kVB / 4; ++kq) {
+                const int64_t col = c0 + 4 * kq + lk;
+                const double bv = Xt[col * kSR + lc];
+#pragma unroll
+                for (int t = 0; t < 4; ++t) {
+                    const int64_t row = r0 + 16 * t + lc;
+                    const double a = row < n ? A[row + col * lda] : 0.0;
+                    acc[t] = M::mma(a, bv, acc[t]);
+                }
+            }
+        } else if (j == i) {
+            // diagonal block: element (r, c) from the lower triangle
+#pragma unroll 4
+            for (int kq = 0; kq < kVB / 4; ++kq) {
+                const int64_t col = c0 + 4 * kq + lk;
+                const double bv = col < n ? Xt[col * kSR + lc] : 0.0;
+#pragma unroll
+                for (int t = 0; t < 4; ++t) {
+                    const int64_t row = r0 + 16 * t + lc;
+                    double a = 0.0;
+                    if (row < n && col < n) a = row >= col ? A[row + col * lda] : A[col + row * lda];
+                    acc[t] = M::mma(a, bv, acc[t]);
+                }
+            }
+        } else {
+            // A_ij = A_jiᵀ: stage 16 columns of A_ji (64 rows each) transposed
+            // through LDS so the global reads run down the columns.
+#pragma unroll 1
+            for (int t = 0; t < 4; ++t) {
+                {
+                    const int cc = lane >> 2, rs = 16 * (lane & 3);     // column, row start
+                    const int64_t gcol = r0 + 16 * t + cc;
+#pragma unroll
+                    for (int e = 0; e < 16; ++e) {
+                        const int64_t grow = c0 + rs + e;
+                        tl[cc * 68 + rs + e] = (grow < n && gcol < n) ? A[grow + gcol * lda] : 0.0;
+                    }
+                }
+                __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+#pragma unroll 4
+                for (int kq = 0; kq < kVB / 4; ++kq) {
+                    const int64_t col = c0 + 4 * kq + lk;
+                    const double bv = col < n ? Xt[col * kSR + lc] : 0.0;
+                    acc[t] = M::mma(tl[lc * 68 + 4 * kq + lk], bv, acc[t]);
+                }
+                __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+            }
+        }
+    }
+    // sum the four waves' partials
+#pragma unroll
+    for (int t = 0; t < 4; ++t)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) Red[w][(16 * t + M::row(lane, r)) * kSR + lc] = acc[t][r];
+    __syncthreads();
+    // every element this thread visits has k = threadIdx.x % 16 (256 is a
+    // multiple of 16): accumulate locally, reduce over the 16 threads of each
+    // k, one atomic per k and workgroup
+    double rs = 0.0, xs = 0.0;
+    for (int e = threadIdx.x; e < kVB * kSR; e += kST) {
+        const int rl = e / kSR, k = e % kSR;
+        const int64_t row = r0 + rl;
+        if (row >= n) continue;
+        double v = 0.0;
+        if (k < nrhs) {
+            const double ax = Red[0][e] + Red[1][e] + Red[2][e] + Red[3][e];
+            v = B[row + k * ldb] - ax;
+            const double x = Xt[row * kSR + k];
+            xs += x * x;
+            rs += v * v;
+        }
+        R[row * kSR + k] = (T)v;
+    }
+    __syncthreads();
+    double *sr = Red[0];
+    sr[threadIdx.x] = rs;
+    sr[kST + threadIdx.x] = xs;
+    __syncthreads();
+    if (threadIdx.x < kSR && (int)threadIdx.x < nrhs) {
+        double a = 0.0, c = 0.0;
+        for (int q = threadIdx.x; q < kST; q += kSR) {
+            a += sr[q];
+            c += sr[kST + q];
+        }
+        atomicAdd(nrm + threadIdx.x, a);
+        atomicAdd(nrm + kSR + threadIdx.x, c);
+    }
+}
+
+__global__ void rowmajor16_kernel(int64_t n, int nrhs, const double *__restrict__ X, int64_t ldx,
+                                  double *__restrict__ Xt)
+{
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e < n * kSR) {
+        const int64_t row = e / kSR;
+        const int k = (int)(e % kSR);
+        Xt[e] = k < nrhs ? X[row + k * ldx] : 0.0;
+    }
+}
+
+}  // namespace
+
+template <typename T>
+int trsv16(Handle *h, hipStream_t s, int64_t n, const T *L, int64_t ldl, const T *Dt, const T *R,
+           T *Y, int rev, double *X, int64_t ldx, int nrhs, int *flags, int epoch, int *err)
+{
+    if (n <= 0) return GLA_OK;
+    if (nrhs > kSR) {
+        set_error("trsv16: at most 16 right-hand sides per pass");
+        return GLA_ERR_UNSUPPORTED;
+    }
+    const int64_t nblk = cdiv(n, kSB);
+    const int grid = (int)std::min<int64_t>(nblk, h->num_cus);
+    hipLaunchKernelGGL((trsv16_kernel<T>), dim3(grid), dim3(kST), 0, s, n, nblk, L, ldl, Dt, R, Y,
+                       rev, X, ldx, nrhs, flags, epoch, err);
+    GLA_LAUNCH_CHECK();
+    return GLA_OK;
+}
+
+// R = B − sym(A) X (see symv_residual_kernel); Xt scratch n×16 FP64; nrm[32]
+// must be zero on entry.
+template <typename T>
+int symv_residual(hipStream_t s, int64_t n, const double *A, int64_t lda, const double *X,
+                  int64_t ldx, const double *B, int64_t ldb, int nrhs, double *Xt, T *R,
+                  double *nrm)
+{
+    if (n <= 0) return GLA_OK;
+    hipLaunchKernelGGL(rowmajor16_kernel, dim3((unsigned)cdiv(n * kSR, 256)), dim3(256), 0, s, n,
+                       nrhs, X, ldx, Xt);
+    GLA_LAUNCH_CHECK();
+    hipLaunchKernelGGL((symv_residual_kernel<T>), dim3((unsigned)cdiv(n, kVB)), dim3(kST), 0, s, n,
+                       A, lda, Xt, B, ldb, nrhs, R, nrm);
+    GLA_LAUNCH_CHECK();
+    return GLA_OK;
+}
+
+template int trsv16<double>(Handle *, hipStream_t, int64_t, const double *, int64_t,
+                            const double *, const double *, double *, int, double *, int64_t, int,
+                            int *, int, int *);
+template int trsv16<float>(Handle *, hipStream_t, int64_t, const float *, int64_t, const float *,
+                           const float *, float *, int, double *, int64_t, int, int *, int, int *);
+template int symv_residual<double>(hipStream_t, int64_t, const double *, int64_t, const double *,
+                                   int64_t, const double *, int64_t, int, double *, double *,
+                                   double *);
+template int symv_residual<float>(hipStream_t, int64_t, const double *, int64_t, const double *,
+                                  int64_t, const double *, int64_t, int, double *, float *,
+                                  double *);
+
+}  // namespace gla

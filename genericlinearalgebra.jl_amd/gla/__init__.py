@@ -528,28 +528,34 @@ class PackedBatch:
 
 # ------------------------------------------- mixed precision + refinement
 class MixedSolveResult:
-    """Outcome of chol_solve_mixed: corrections made, final normwise
-    backward error max_k ‖b_k − A x_k‖ / (‖A‖_F ‖x_k‖ + ‖b_k‖)."""
+    """Outcome of chol_solve_mixed: the precision of the returned X, the
+    refinement corrections made in it, and the final normwise backward
+    error max_k ‖b_k − A x_k‖ / (‖A‖_F ‖x_k‖ + ‖b_k‖)."""
 
-    def __init__(self, iters: int, berr: float):
+    def __init__(self, iters: int, berr: float, precision: str, tol: float):
         self.iters = iters
         self.berr = berr
+        self.precision = precision
+        self.converged = berr <= tol
 
     def __repr__(self):
-        return f"MixedSolveResult(iters={self.iters}, berr={self.berr:.3e})"
+        return (f"MixedSolveResult(precision={self.precision}, iters={self.iters}, "
+                f"berr={self.berr:.3e}, converged={self.converged})")
 
 
-_PREC = {"fp64": b"d", "fp32": b"s", "bf16": b"b", "fp16": b"h"}
+_PREC = {"fp64": b"d", "fp32": b"s", "bf16": b"b", "fp16": b"h", "auto": b"a"}
+_PREC_NAME = {b"d": "fp64", b"s": "fp32", b"b": "bf16", b"h": "fp16"}
 
 
-def chol_solve_mixed(A, B, precision: str = "bf16", tol: float = 1e-12, max_iter: int = 30,
+def chol_solve_mixed(A, B, precision: str = "auto", tol: float = 1e-12, max_iter: int = 30,
                      X=None):
     """X = A⁻¹ B for SPD Float64 A (lower triangle read), factored in
     `precision` ("fp64", "fp32", "bf16": FP32 storage + BF16 MFMA trailing
-    updates, "fp16": the same with FP16) and refined in Float64 to a
-    normwise backward error ≤ tol.  Raises DomainError (index = first
-    non-positive pivot) when the low-precision factorisation breaks down.
-    Returns (X, MixedSolveResult)."""
+    updates with FP64 diagonal blocks, "fp16": the same with FP16, "auto":
+    bf16, then fp32, then fp64 until one converges) and refined in Float64
+    to a normwise backward error ≤ tol.  Raises DomainError (index = first
+    non-positive pivot) when a fixed-precision factorisation breaks down.
+    Returns (X, MixedSolveResult); check .converged for fixed precisions."""
     if precision not in _PREC:
         raise ArgumentError(f"precision must be one of {sorted(_PREC)}")
     if A.dtype != torch.float64 or B.dtype != torch.float64:
@@ -570,17 +576,19 @@ def chol_solve_mixed(A, B, precision: str = "bf16", tol: float = 1e-12, max_iter
     it = C.c_int32(0)
     be = C.c_double(0.0)
     info = C.c_int64(0)
+    used = C.create_string_buffer(1)
     with torch.cuda.device(A.device):
         rc = lib().gla_chol_solve_mixed_d(_handle(A.device.index or 0), _PREC[precision], n,
                                           C.c_void_p(A.data_ptr()), lda, nrhs,
                                           C.c_void_p(Bm.data_ptr()), ldb,
                                           C.c_void_p(X.data_ptr()), ldx, float(tol),
                                           int(max_iter), C.byref(it), C.byref(be),
-                                          C.byref(info), _stream(A))
+                                          C.byref(info), used, _stream(A))
     _check(rc, "gla_chol_solve_mixed_d")
     if info.value != 0:
         raise DomainError(info.value)
-    return (X.view(-1) if vec else X), MixedSolveResult(it.value, be.value)
+    res = MixedSolveResult(it.value, be.value, _PREC_NAME.get(used.raw[:1], "?"), tol)
+    return (X.view(-1) if vec else X), res
 
 
 def gemm_lowp_nt_(Cm, A, B, alpha=-1.0, precision: str = "bf16"):

@@ -81,7 +81,7 @@ def lib() -> C.CDLL:
     L.gla_batch_plan_destroy.restype = _int
     L.gla_chol_solve_mixed_d.argtypes = [_p, _ch, _i64, _p, _i64, _i64, _p, _i64, _p, _i64,
                                          C.c_double, C.c_int32, C.POINTER(C.c_int32),
-                                         C.POINTER(C.c_double), _pi64, _p]
+                                         C.POINTER(C.c_double), _pi64, C.c_char_p, _p]
     L.gla_chol_solve_mixed_d.restype = _int
     L.gla_gemm_lowp_nt.argtypes = [_p, _ch, _i64, _i64, _i64, C.c_float, _p, _i64, _p, _i64, _p,
                                    _i64, _p]

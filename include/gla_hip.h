@@ -163,16 +163,20 @@ int gla_chol_solve_batched_s(gla_handle_t h, gla_batch_plan_t plan, float *const
  * a lower-precision factorisation and FP64 refinement.  `prec` selects the
  * factorisation: 'd' FP64 (the plain path), 's' FP32, 'b' FP32 storage with
  * BF16 MFMA trailing updates and FP64 diagonal blocks, 'h' the same with
- * FP16.  X is refined in FP64 (r = B − A X on the FP64 GEMM, correction
- * with the low-precision factor) until the normwise backward error
- * max_k ‖r_k‖₂ / (‖A‖_F ‖x_k‖₂ + ‖b_k‖₂) ≤ tol or max_iter corrections.
- * *iters = corrections made, *berr = final backward error, *info = first
- * non-positive pivot of the factorisation (1-based, 0 = success; then X is
- * not computed).  A and B are not modified.  Synchronises the stream. */
+ * FP16, 'a' BF16 first, then FP32, then FP64 until one converges.  X is
+ * refined in FP64 (r = B − A X with the symmetric A, correction with the
+ * factor) until the normwise backward error
+ * max_k ‖r_k‖₂ / (‖A‖_F ‖x_k‖₂ + ‖b_k‖₂) ≤ tol; a precision stops when η
+ * has not halved over two corrections or after max_iter corrections.
+ * *iters = corrections of the last precision tried, *berr = its final
+ * backward error (> tol if none converged), *info = first non-positive
+ * pivot (1-based) if that factorisation broke down, *prec_used = the
+ * precision of the returned X.  A and B are not modified.  Synchronises
+ * the stream. */
 int gla_chol_solve_mixed_d(gla_handle_t h, char prec, int64_t n, const double *A, int64_t lda,
                            int64_t nrhs, const double *B, int64_t ldb, double *X, int64_t ldx,
                            double tol, int32_t max_iter, int32_t *iters, double *berr,
-                           int64_t *info, void *stream);
+                           int64_t *info, char *prec_used, void *stream);
 /* Low-precision MFMA product used by the mixed factorisation:
  * C (m×n, FP32) += alpha · A Bᵀ with A (m×k), B (n×k) FP32 in memory rounded
  * to `tin` ('b' BF16, 'h' FP16) on the way in, FP32 accumulation,

@@ -56,30 +56,68 @@ def test_lowp_gemm_vs_torch(prec, tdt, m, n, k, alpha):
     assert err.max() <= (k + 2) * np.finfo(np.float32).eps, err.max()
 
 
-@pytest.mark.parametrize("prec", ["fp64", "fp32", "bf16", "fp16"])
-@pytest.mark.parametrize("kappa", [1e2, 1e4])
-def test_refined_solve_reaches_tolerance(prec, kappa):
+def _check_solution(A, B, X, res, kappa, tol=1e-12):
+    be = backward_error(A, X, B)
+    assert res.converged and res.berr <= tol, res
+    assert be <= 2 * tol, (be, res)
+    assert abs(be - res.berr) <= 0.5 * max(be, res.berr) + 1e-16
+    Xr = np.linalg.solve(A, B)
+    assert np.linalg.norm(X - Xr) / np.linalg.norm(Xr) <= kappa * 1e-11
+
+
+@pytest.mark.parametrize("kappa", [1e2, 1e4, 1e6])
+def test_auto_precision_reaches_tolerance(kappa):
     gla = _gla()
     rng = np.random.default_rng(int(np.log10(kappa)))
     n, nrhs = 700, 16
     A = spd_kappa(n, kappa, rng)
     B = rng.standard_normal((n, nrhs))
     Ad, Bd = gla.colmajor(A), gla.colmajor(B)
-    try:
-        X, res = gla.chol_solve_mixed(Ad, Bd, precision=prec, tol=1e-12, max_iter=60)
-    except gla.DomainError:
-        pytest.skip(f"{prec} factorisation breaks down at kappa={kappa:g} (expected for "
-                    "low precision and large kappa)")
-    Xh = gla.to_numpy(X)
-    be = backward_error(A, Xh, B)
-    assert res.berr <= 1e-12, res
-    assert be <= 2e-12, (be, res)
-    assert abs(be - res.berr) <= 0.5 * max(be, res.berr) + 1e-16
-    Xr = np.linalg.solve(A, B)
-    assert np.linalg.norm(Xh - Xr) / np.linalg.norm(Xr) <= kappa * 1e-11
+    X, res = gla.chol_solve_mixed(Ad, Bd, precision="auto", tol=1e-12, max_iter=60)
+    _check_solution(A, B, gla.to_numpy(X), res, kappa)
     # A and B are inputs only
     assert np.array_equal(gla.to_numpy(Ad), A)
     assert np.array_equal(gla.to_numpy(Bd), B)
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32", "bf16", "fp16"])
+@pytest.mark.parametrize("kappa", [1e2, 1e4])
+def test_fixed_precision_converges_or_says_so(prec, kappa):
+    """A fixed precision either reaches 1e-12 or reports, truthfully, that it
+    did not (low precisions at large κ), or raises DomainError."""
+    gla = _gla()
+    rng = np.random.default_rng(int(np.log10(kappa)) + 10)
+    n, nrhs = 520, 16
+    A = spd_kappa(n, kappa, rng)
+    B = rng.standard_normal((n, nrhs))
+    try:
+        X, res = gla.chol_solve_mixed(gla.colmajor(A), gla.colmajor(B), precision=prec,
+                                      tol=1e-12, max_iter=60)
+    except gla.DomainError:
+        assert prec in ("bf16", "fp16")
+        return
+    assert res.precision == prec
+    Xh = gla.to_numpy(X)
+    if prec in ("fp64", "fp32") or kappa <= 1e2:
+        _check_solution(A, B, Xh, res, kappa)
+    elif not res.converged:
+        be = backward_error(A, Xh, B)
+        assert not (be <= 1e-12) and not (res.berr <= 1e-12)
+    else:
+        _check_solution(A, B, Xh, res, kappa)
+
+
+def test_more_than_sixteen_rhs_and_lower_triangle_only():
+    """37 right-hand sides (three passes of 16); the strict upper triangle of
+    A holds garbage that must not be read."""
+    gla = _gla()
+    rng = np.random.default_rng(21)
+    n = 300
+    A = spd_kappa(n, 1e3, rng)
+    B = rng.standard_normal((n, 37))
+    Ag = np.tril(A) + np.triu(np.full((n, n), 1e6), 1)
+    X, res = gla.chol_solve_mixed(gla.colmajor(Ag), gla.colmajor(B), precision="auto")
+    _check_solution(A, B, gla.to_numpy(X), res, 1e3)
 
 
 def test_fp64_needs_no_correction_on_wellconditioned():
@@ -89,7 +127,7 @@ def test_fp64_needs_no_correction_on_wellconditioned():
     B = rng.standard_normal((300, 4))
     X, res = gla.chol_solve_mixed(gla.colmajor(A), gla.colmajor(B), precision="fp64",
                                   tol=1e-12)
-    assert res.iters == 0 and res.berr <= 1e-12
+    assert res.iters == 0 and res.berr <= 1e-12 and res.precision == "fp64"
 
 
 def test_ragged_sizes_and_vector_rhs():
@@ -102,6 +140,7 @@ def test_ragged_sizes_and_vector_rhs():
                                       precision="bf16", tol=1e-12)
         xh = x.cpu().numpy()
         assert xh.shape == (n,)
+        assert res.converged, (n, res)
         assert backward_error(A, xh[:, None], b[:, None]) <= 2e-12, (n, res)
 
 
