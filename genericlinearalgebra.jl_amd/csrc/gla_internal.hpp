@@ -55,6 +55,8 @@ struct Handle {
     // Tuning: block width of the look-ahead driver behind cholRecursive!
     // (0: the recursive-halving driver).  Set with gla_set_tuning.
     int64_t chol_block = 0;
+    // Block width of the mixed-precision factorisation's look-ahead driver.
+    int mixed_block = 512;
 };
 
 // Ensure the workspace holds at least `bytes` (grows with hipMalloc, which
@@ -113,12 +115,13 @@ template <typename T>
 int trsv16(Handle *h, hipStream_t s, int64_t n, const T *L, int64_t ldl, const T *Dt, const T *R,
            T *Y, int rev, double *X, int64_t ldx, int nrhs, int *flags, int epoch, int *err);
 // R (row-major n×16, type T) = B − sym(A) X for the lower-stored FP64 A and
-// nrhs ≤ 16 columns of X, B (column-major); Xt: n×16 FP64 scratch;
+// nrhs ≤ 16 columns of X, B (column-major); Xt: symv_scratch(n) doubles;
 // nrm[k] += ‖r_k‖², nrm[16 + k] += ‖x_k‖² (nrm zeroed by the caller).
 template <typename T>
 int symv_residual(hipStream_t s, int64_t n, const double *A, int64_t lda, const double *X,
                   int64_t ldx, const double *B, int64_t ldb, int nrhs, double *Xt, T *R,
                   double *nrm);
+size_t symv_scratch(int64_t n);
 
 // QR building blocks (qr.hip): T of the first k reflectors of F (m×k), and
 // A (mA×nA) <- op(Q) A, trans 'N' or 'T', Q from the first b reflectors.

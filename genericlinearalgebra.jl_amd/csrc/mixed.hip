@@ -128,24 +128,30 @@ __global__ __launch_bounds__(kLpT, 2) void lp_gemm_nt_kernel(LpP p)
                 if (row + r < p.M && col < p.N) acc[i][j][r] = p.alpha * p.C[row + r + col * p.ldc];
         }
 
-    // staging: thread t loads rows 4*(t%32)..+3 of k-columns t/32 + 8q, q < 4
+    // staging: thread t holds row m = t % 128 of the slab, k = 16·(t / 128)
+    // .. +15: each load instruction reads 64 consecutive rows (coalesced);
+    // the 16 values are rounded to TIN and written as two 16-byte vectors in
+    // the [m][k] layout the MFMA operands are read from.
     float ra[16], rb[16];
+    const int sm = t & 127, sk = 16 * (t >> 7);
     auto gload = [&](const float *X, int64_t ldx, int64_t x0, int64_t xlim, int64_t k0, float *r) {
+        const int64_t gx = x0 + sm;
+        const bool okx = gx < xlim;
 #pragma unroll
-        for (int q = 0; q < 4; ++q) {
-            const int64_t gk = k0 + t / 32 + 8 * q;
-            const int64_t gx = x0 + 4 * (t % 32);
-#pragma unroll
-            for (int e = 0; e < 4; ++e)
-                r[4 * q + e] = (gk < p.K && gx + e < xlim) ? X[gx + e + gk * ldx] : 0.f;
+        for (int e = 0; e < 16; ++e) {
+            const int64_t gk = k0 + sk + e;
+            r[e] = (okx && gk < p.K) ? X[gx + gk * ldx] : 0.f;
         }
     };
     auto lstore = [&](TIN *S, const float *r) {
+        v8 lo, hi;
 #pragma unroll
-        for (int q = 0; q < 4; ++q)
-#pragma unroll
-            for (int e = 0; e < 4; ++e)
-                S[(4 * (t % 32) + e) * kLpLD + t / 32 + 8 * q] = (TIN)r[4 * q + e];
+        for (int e = 0; e < 8; ++e) {
+            lo[e] = (TIN)r[e];
+            hi[e] = (TIN)r[8 + e];
+        }
+        *reinterpret_cast<v8 *>(&S[sm * kLpLD + sk]) = lo;
+        *reinterpret_cast<v8 *>(&S[sm * kLpLD + sk + 8]) = hi;
     };
 
     const int nk = (int)((p.K + kLpBK - 1) / kLpBK);
@@ -313,11 +319,11 @@ __global__ __launch_bounds__(256) void sym_sumsq_cols(int64_t n, const double *A
 // blocks and low-precision trailing updates, depth-1 look-ahead.
 template <typename TIN>
 int chol_mixed(Handle *h, hipStream_t s, int64_t n, float *W, int64_t ldw, double *D64,
-               double *Linv64, float *Linv32)
+               double *Linv64, float *Linv32, bool use_side = true)
 {
     const int nb = 128;
     const int64_t nsteps = cdiv(n, nb);
-    const bool overlap = nsteps > 2;
+    const bool overlap = use_side && nsteps > 2;
     if (overlap) {
         GLA_HIP_TRY(hipEventRecord(h->ev_fork, s));
         GLA_HIP_TRY(hipStreamWaitEvent(h->side, h->ev_fork, 0));
@@ -342,6 +348,53 @@ int chol_mixed(Handle *h, hipStream_t s, int64_t n, float *W, int64_t ldw, doubl
         GLA_TRY(chol_solve_block<float>(s, kb, Wkk, ldw, Linv32, W21, ldw, below));
         if (below == 0) break;
         const int kb2 = (int)std::min<int64_t>(nb, below);
+        const int64_t rest = below - kb2;
+        if (overlap && rest > 0) GLA_HIP_TRY(hipEventRecord(h->ev_step[0], s));
+        if (overlap && step > 0) GLA_HIP_TRY(hipStreamWaitEvent(s, h->ev_step[1], 0));
+        GLA_TRY(lp_gemm_nt<TIN>(s, 1, below, kb2, kb, -1.f, W21, ldw, W21, ldw, W21 + kb * ldw, ldw));
+        if (rest > 0) {
+            hipStream_t us = overlap ? h->side : s;
+            if (overlap) GLA_HIP_TRY(hipStreamWaitEvent(h->side, h->ev_step[0], 0));
+            float *X2 = W21 + kb2;
+            GLA_TRY(lp_gemm_nt<TIN>(us, 1, rest, rest, kb, -1.f, X2, ldw, X2, ldw,
+                                    X2 + (kb + kb2) * ldw, ldw));
+            if (overlap) GLA_HIP_TRY(hipEventRecord(h->ev_step[1], h->side));
+        }
+    }
+    if (overlap) {
+        GLA_HIP_TRY(hipEventRecord(h->ev_join, h->side));
+        GLA_HIP_TRY(hipStreamWaitEvent(s, h->ev_join, 0));
+    }
+    return GLA_OK;
+}
+
+// Wide-block form (NB = 512): the NB×NB diagonal block by the 128-wide
+// driver above on the caller's stream, the panel below it by the FP32
+// recursive solve, then the look-ahead trailing updates of cholBlocked!'s
+// loop (src/cholesky.jl:17-35) with K = NB on the low-precision MFMA GEMM:
+// the next block column on the caller's stream, the rest on the side stream.
+template <typename TIN>
+int chol_mixed_la(Handle *h, hipStream_t s, int64_t n, float *W, int64_t ldw, double *D64,
+                  double *Linv64, float *Linv32, float *Linv, int NB)
+{
+    if (n <= NB) return chol_mixed<TIN>(h, s, n, W, ldw, D64, Linv64, Linv32, false);
+    const int64_t nsteps = cdiv(n, NB);
+    const bool overlap = nsteps > 2;
+    if (overlap) {
+        GLA_HIP_TRY(hipEventRecord(h->ev_fork, s));
+        GLA_HIP_TRY(hipStreamWaitEvent(h->side, h->ev_fork, 0));
+    }
+    int64_t k = 0;
+    for (int64_t step = 0; step < nsteps; ++step, k += NB) {
+        const int64_t kb = std::min<int64_t>(NB, n - k);
+        const int64_t below = n - k - kb;
+        float *Wkk = W + k + k * ldw;
+        float *W21 = Wkk + kb;
+        GLA_TRY(chol_mixed<TIN>(h, s, kb, Wkk, ldw, D64, Linv64, Linv32, false));
+        if (below == 0) break;
+        GLA_TRY(tile_inverses<float>(s, kb, Wkk, ldw, Linv));
+        GLA_TRY(trsm_rlt<float>(s, below, kb, Wkk, ldw, Linv, W21, ldw));
+        const int64_t kb2 = std::min<int64_t>(NB, below);
         const int64_t rest = below - kb2;
         if (overlap && rest > 0) GLA_HIP_TRY(hipEventRecord(h->ev_step[0], s));
         if (overlap && step > 0) GLA_HIP_TRY(hipStreamWaitEvent(s, h->ev_step[1], 0));
@@ -402,7 +455,7 @@ struct MixedWS {
         auto al = [&](size_t b) { return (b + r) / 256 * 256; };
         return 2 * al(es * (size_t)n * n) + 2 * al(es * tile_inverse_elems(n)) +
                2 * al(es * (size_t)npad * 128) + al(es * (size_t)npad * 16 * nch) +
-               2 * al(es * (size_t)npad * 16) + al(8 * (size_t)npad * 16) +
+               2 * al(es * (size_t)npad * 16) + al(8 * symv_scratch(n)) +
                al(8 * (size_t)(32 * nch + n + 64)) + al(8 * 128 * 128) + al(8 * 8 * 256) +
                al(4 * 8 * 256) + al(4 * (size_t)(nblk + 64)) + 4096;
     }
@@ -427,7 +480,7 @@ struct MixedWS {
         Rt = reinterpret_cast<T *>(take(es * (size_t)npad * 16 * nchunk));
         Yt = reinterpret_cast<T *>(take(es * (size_t)npad * 16));
         Zt = reinterpret_cast<T *>(take(es * (size_t)npad * 16));
-        Xt = reinterpret_cast<double *>(take(8 * (size_t)npad * 16));
+        Xt = reinterpret_cast<double *>(take(8 * symv_scratch(n)));
         nrm = reinterpret_cast<double *>(take(8 * (size_t)(32 * nchunk + n + 64)));
         D64 = reinterpret_cast<double *>(take(8 * 128 * 128));
         Linv64 = reinterpret_cast<double *>(take(8 * 8 * 256));
@@ -554,9 +607,11 @@ int mixed_attempt(Handle *h, hipStream_t s, char prec, int64_t n, const double *
         if (prec == 's')
             GLA_TRY(chol_device<float>(h, s, n, ws.W, n, 2, 1, ws.Linv));
         else if (prec == 'b')
-            GLA_TRY(chol_mixed<__bf16>(h, s, n, ws.W, n, ws.D64, ws.Linv64, ws.Linv32));
+            GLA_TRY(chol_mixed_la<__bf16>(h, s, n, ws.W, n, ws.D64, ws.Linv64, ws.Linv32, ws.Linv,
+                                           h->mixed_block));
         else
-            GLA_TRY(chol_mixed<_Float16>(h, s, n, ws.W, n, ws.D64, ws.Linv64, ws.Linv32));
+            GLA_TRY(chol_mixed_la<_Float16>(h, s, n, ws.W, n, ws.D64, ws.Linv64, ws.Linv32, ws.Linv,
+                                             h->mixed_block));
     }
     int hinfo = 0;
     GLA_HIP_TRY(hipMemcpyAsync(&hinfo, h->dinfo, sizeof(int), hipMemcpyDeviceToHost, s));

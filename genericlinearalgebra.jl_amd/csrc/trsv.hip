@@ -53,13 +53,32 @@ struct SMfma<float> {
     static __device__ __forceinline__ int row(int l, int r) { return 4 * (l >> 4) + r; }
 };
 
+// One wave's slab of L_bj: rows 32w + 16t + lc (t = 0, 1), columns
+// 128j + 4kq + lk (kq < 32), the MFMA A-operand layout.  Loaded ahead of
+// the wait for Y_j so that only the wait, Y_j and the MFMAs remain on the
+// dependency chain.
 template <typename T>
-__global__ __launch_bounds__(kST) void trsv16_kernel(int64_t n, int64_t nblk,
-                                                     const T *__restrict__ L, int64_t ldl,
-                                                     const T *__restrict__ Dt,
-                                                     const T *__restrict__ R, T *Y, int rev,
-                                                     double *__restrict__ X, int64_t ldx, int nrhs,
-                                                     int *flags, int epoch, int *err)
+__device__ __forceinline__ void load_slab(const T *__restrict__ L, int64_t ldl, int64_t n, int64_t r0,
+                                          int64_t j, int w, int lc, int lk, T (&sl)[2][kSB / 4])
+{
+    const T *Lj = L + (j * kSB) * ldl;
+#pragma unroll
+    for (int t = 0; t < 2; ++t) {
+        const int64_t row = r0 + 32 * w + 16 * t + lc;
+        const bool ok = row < n;
+#pragma unroll
+        for (int kq = 0; kq < kSB / 4; ++kq)
+            sl[t][kq] = ok ? Lj[row + (int64_t)(4 * kq + lk) * ldl] : T(0);
+    }
+}
+
+template <typename T>
+__global__ __launch_bounds__(kST, 1) void trsv16_kernel(int64_t n, int64_t nblk,
+                                                        const T *__restrict__ L, int64_t ldl,
+                                                        const T *__restrict__ Dt,
+                                                        const T *__restrict__ R, T *Y, int rev,
+                                                        double *__restrict__ X, int64_t ldx,
+                                                        int nrhs, int *flags, int epoch, int *err)
 {
     typedef SMfma<T> M;
     typedef typename M::acc_t acc_t;
@@ -78,6 +97,19 @@ __global__ __launch_bounds__(kST) void trsv16_kernel(int64_t n, int64_t nblk,
                 const int64_t row = r0 + 32 * w + 16 * t + M::row(lane, r);
                 acc[t][r] = row < n ? R[(rev ? n - 1 - row : row) * kSR + lc] : T(0);
             }
+        // this wave's rows of L_bb⁻¹ (L_bb⁻¹[r][k] = Dt_b[k + 128 r]), read early
+        T dv[2][kSB / 4];
+        {
+            const T *D = Dt + b * (kSB * kSB);
+#pragma unroll
+            for (int t = 0; t < 2; ++t) {
+                const int rr = 32 * w + 16 * t + lc;
+#pragma unroll
+                for (int kq = 0; kq < kSB / 4; ++kq) dv[t][kq] = D[4 * kq + lk + (int64_t)rr * kSB];
+            }
+        }
+        T sl[2][2][kSB / 4];
+        if (b > 0) load_slab(L, ldl, n, r0, 0, w, lc, lk, sl[0]);
         for (int64_t j = 0; j < b; ++j) {
             T *ys = Ys[j & 1];
             if (threadIdx.x == 0) {
@@ -102,39 +134,34 @@ __global__ __launch_bounds__(kST) void trsv16_kernel(int64_t n, int64_t nblk,
                 for (int q = 0; q < 8; ++q) ys[threadIdx.x + q * kST] = v[q];
             }
             __syncthreads();
-            // acc -= L_bj Y_j; block j < b is a full 128-column block
-            const T *Lj = L + (j * kSB) * ldl;
-#pragma unroll 8
-            for (int kq = 0; kq < kSB / 4; ++kq) {
-                const int k = 4 * kq + lk;
-                const T bv = ys[k * kSR + lc];
+            // prefetch the next slab, then acc -= L_bj Y_j (block j < b is full)
+            const int cur = (int)(j & 1);
+            if (j + 1 < b) {
+                if (cur == 0) load_slab(L, ldl, n, r0, j + 1, w, lc, lk, sl[1]);
+                else load_slab(L, ldl, n, r0, j + 1, w, lc, lk, sl[0]);
+            }
 #pragma unroll
-                for (int t = 0; t < 2; ++t) {
-                    const int64_t row = r0 + 32 * w + 16 * t + lc;
-                    const T a = row < n ? Lj[row + (int64_t)k * ldl] : T(0);
-                    acc[t] = M::mma(-a, bv, acc[t]);
-                }
+            for (int kq = 0; kq < kSB / 4; ++kq) {
+                const T bv = ys[(4 * kq + lk) * kSR + lc];
+#pragma unroll
+                for (int t = 0; t < 2; ++t)
+                    acc[t] = M::mma(-(cur == 0 ? sl[0][t][kq] : sl[1][t][kq]), bv, acc[t]);
             }
         }
-        // Y_b = L_bb⁻¹ S with L_bb⁻¹[r][k] = Dt_b[k + 128 r]
+        // Y_b = L_bb⁻¹ S
 #pragma unroll
         for (int t = 0; t < 2; ++t)
 #pragma unroll
             for (int r = 0; r < 4; ++r) Ss[(32 * w + 16 * t + M::row(lane, r)) * kSR + lc] = acc[t][r];
         __syncthreads();
-        const T *D = Dt + b * (kSB * kSB);
         acc_t y[2];
 #pragma unroll
         for (int t = 0; t < 2; ++t) y[t] = acc_t{T(0), T(0), T(0), T(0)};
-#pragma unroll 8
-        for (int kq = 0; kq < kSB / 4; ++kq) {
-            const int k = 4 * kq + lk;
-            const T bv = Ss[k * kSR + lc];
 #pragma unroll
-            for (int t = 0; t < 2; ++t) {
-                const int rr = 32 * w + 16 * t + lc;
-                y[t] = M::mma(D[k + (int64_t)rr * kSB], bv, y[t]);
-            }
+        for (int kq = 0; kq < kSB / 4; ++kq) {
+            const T bv = Ss[(4 * kq + lk) * kSR + lc];
+#pragma unroll
+            for (int t = 0; t < 2; ++t) y[t] = M::mma(dv[t][kq], bv, y[t]);
         }
 #pragma unroll
         for (int t = 0; t < 2; ++t)
@@ -153,18 +180,16 @@ __global__ __launch_bounds__(kST) void trsv16_kernel(int64_t n, int64_t nblk,
 }
 
 // ---------------------------------------------------- symmetric residual --
-constexpr int kVB = 64;   // rows per workgroup
+constexpr int kVB = 64;    // rows per workgroup
+constexpr int kVCH = 8;    // column chunks (grid y): partial sums per chunk
 
-// R (row-major n×16, type T) = B − sym(A) X for the lower-stored FP64 A,
-// Xt = X row-major FP64 (n×16, zero-padded).  nrm[k] += Σ r_k², nrm[16+k]
-// += Σ x_k² over this workgroup's rows.
-template <typename T>
-__global__ __launch_bounds__(kST) void symv_residual_kernel(int64_t n, const double *__restrict__ A,
-                                                            int64_t lda,
-                                                            const double *__restrict__ Xt,
-                                                            const double *__restrict__ B,
-                                                            int64_t ldb, int nrhs, T *__restrict__ R,
-                                                            double *nrm)
+// P[y] (row-major n×16, FP64) = Σ over column blocks j of chunk y of A_ij X_j
+// for row block i = blockIdx.x, A symmetric, stored in its lower triangle;
+// Xt = X row-major FP64 (n×16, zero-padded).
+__global__ __launch_bounds__(kST) void symv_partial_kernel(int64_t n, const double *__restrict__ A,
+                                                           int64_t lda,
+                                                           const double *__restrict__ Xt,
+                                                           double *__restrict__ P, int64_t npad)
 {
     typedef SMfma<double> M;
     __shared__ double Tl[4][16 * 68];        // per-wave transposed staging (64 k × 16 m)
@@ -173,11 +198,13 @@ __global__ __launch_bounds__(kST) void symv_residual_kernel(int64_t n, const dou
     const int lc = lane & 15, lk = lane >> 4;
     const int64_t i = blockIdx.x, r0 = i * kVB;
     const int64_t nb = (n + kVB - 1) / kVB;
+    const int64_t cpc = (nb + kVCH - 1) / kVCH;
+    const int64_t jb = blockIdx.y * cpc, je = std::min<int64_t>(nb, jb + cpc);
     d4 acc[4];
 #pragma unroll
     for (int t = 0; t < 4; ++t) acc[t] = d4{0.0, 0.0, 0.0, 0.0};
     double *tl = Tl[w];
-    for (int64_t j = w; j < nb; j += 4) {
+    for (int64_t j = jb + w; j < je; j += 4) {
         const int64_t c0 = j * kVB;
         if (j < i) {
             // A_ij from the lower triangle, rows contiguous
@@ -231,12 +258,28 @@ __global__ __launch_bounds__(kST) void symv_residual_kernel(int64_t n, const dou
             }
         }
     }
-    // sum the four waves' partials
 #pragma unroll
     for (int t = 0; t < 4; ++t)
 #pragma unroll
         for (int r = 0; r < 4; ++r) Red[w][(16 * t + M::row(lane, r)) * kSR + lc] = acc[t][r];
     __syncthreads();
+    double *Py = P + (int64_t)blockIdx.y * npad * kSR + r0 * kSR;
+    for (int e = threadIdx.x; e < kVB * kSR; e += kST)
+        Py[e] = Red[0][e] + Red[1][e] + Red[2][e] + Red[3][e];
+}
+
+// R = B − Σ_y P[y] (type T, row-major n×16), nrm[k] += ‖r_k‖², nrm[16+k] +=
+// ‖x_k‖² over this workgroup's 64 rows.
+template <typename T>
+__global__ __launch_bounds__(kST) void symv_finish_kernel(int64_t n, const double *__restrict__ P,
+                                                          int64_t npad,
+                                                          const double *__restrict__ Xt,
+                                                          const double *__restrict__ B,
+                                                          int64_t ldb, int nrhs, T *__restrict__ R,
+                                                          double *nrm)
+{
+    __shared__ double sr[2 * kST];
+    const int64_t r0 = (int64_t)blockIdx.x * kVB;
     // every element this thread visits has k = threadIdx.x % 16 (256 is a
     // multiple of 16): accumulate locally, reduce over the 16 threads of each
     // k, one atomic per k and workgroup
@@ -247,7 +290,9 @@ __global__ __launch_bounds__(kST) void symv_residual_kernel(int64_t n, const dou
         if (row >= n) continue;
         double v = 0.0;
         if (k < nrhs) {
-            const double ax = Red[0][e] + Red[1][e] + Red[2][e] + Red[3][e];
+            double ax = 0.0;
+#pragma unroll
+            for (int y = 0; y < kVCH; ++y) ax += P[(int64_t)y * npad * kSR + row * kSR + k];
             v = B[row + k * ldb] - ax;
             const double x = Xt[row * kSR + k];
             xs += x * x;
@@ -255,8 +300,6 @@ __global__ __launch_bounds__(kST) void symv_residual_kernel(int64_t n, const dou
         }
         R[row * kSR + k] = (T)v;
     }
-    __syncthreads();
-    double *sr = Red[0];
     sr[threadIdx.x] = rs;
     sr[kST + threadIdx.x] = xs;
     __syncthreads();
@@ -301,22 +344,29 @@ int trsv16(Handle *h, hipStream_t s, int64_t n, const T *L, int64_t ldl, const T
     return GLA_OK;
 }
 
-// R = B − sym(A) X (see symv_residual_kernel); Xt scratch n×16 FP64; nrm[32]
-// must be zero on entry.
+// R = B − sym(A) X; Xt scratch: n×16 FP64 followed by the partial sums
+// (symv_scratch(n) doubles in all); nrm[32] must be zero on entry.
 template <typename T>
 int symv_residual(hipStream_t s, int64_t n, const double *A, int64_t lda, const double *X,
                   int64_t ldx, const double *B, int64_t ldb, int nrhs, double *Xt, T *R,
                   double *nrm)
 {
     if (n <= 0) return GLA_OK;
+    const int64_t nb = cdiv(n, kVB), npad = nb * kVB;
+    double *P = Xt + npad * kSR;
     hipLaunchKernelGGL(rowmajor16_kernel, dim3((unsigned)cdiv(n * kSR, 256)), dim3(256), 0, s, n,
                        nrhs, X, ldx, Xt);
     GLA_LAUNCH_CHECK();
-    hipLaunchKernelGGL((symv_residual_kernel<T>), dim3((unsigned)cdiv(n, kVB)), dim3(kST), 0, s, n,
-                       A, lda, Xt, B, ldb, nrhs, R, nrm);
+    hipLaunchKernelGGL(symv_partial_kernel, dim3((unsigned)nb, kVCH), dim3(kST), 0, s, n, A, lda, Xt,
+                       P, npad);
+    GLA_LAUNCH_CHECK();
+    hipLaunchKernelGGL((symv_finish_kernel<T>), dim3((unsigned)nb), dim3(kST), 0, s, n, P, npad, Xt,
+                       B, ldb, nrhs, R, nrm);
     GLA_LAUNCH_CHECK();
     return GLA_OK;
 }
+
+size_t symv_scratch(int64_t n) { return (size_t)cdiv(n, kVB) * kVB * kSR * (1 + kVCH); }
 
 template int trsv16<double>(Handle *, hipStream_t, int64_t, const double *, int64_t,
                             const double *, const double *, double *, int, double *, int64_t, int,
