@@ -21,6 +21,8 @@ import subprocess
 import sys
 import time
 
+import numpy as np
+
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path[:0] = [os.path.join(ROOT, "genericlinearalgebra.jl_amd"), os.path.join(ROOT, "oracle")]
 
@@ -58,14 +60,14 @@ def cpu_baseline():
     import oracle as O
 
     rng = np.random.default_rng(1)
-    n = 1536
+    n = 3072
     Bm = rng.standard_normal((n, n))
     A = Bm.T @ Bm + n * np.eye(n)
     O.lib()
     t0 = time.perf_counter()
     O.chol_recursive(A, 1)
     tc = time.perf_counter() - t0
-    m, k = 1536, 768
+    m, k = 3072, 1536
     M = rng.standard_normal((m, k))
     t0 = time.perf_counter()
     O.qr_blocked(M, 128)
@@ -81,12 +83,153 @@ def cpu_baseline():
     }
 
 
+def ev_time(torch, fn, reps=2):
+    """Best of `reps` HIP-event timings of fn on the current stream (s)."""
+    ts = []
+    for _ in range(reps):
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record()
+        fn()
+        e1.record()
+        torch.cuda.synchronize()
+        ts.append(e0.elapsed_time(e1) / 1e3)
+    return min(ts)
+
+
+def mixed_section(torch, gla, n=N_CHOL, nrhs=16):
+    """Mixed precision + FP64 refinement at n=16384: A = Q diag(σ) Qᵀ, Q Haar
+    (QR of a Gaussian with the sign fix), σ log-spaced 1 → 1/κ; 16 N(0,1)
+    right-hand sides.  Time to η ≤ 1e-12 against the plain FP64 path
+    (FP64 factor + solve + residual check, same entry point)."""
+    dev = torch.device("cuda:0")
+    g = torch.Generator(device=dev).manual_seed(3)
+    G = torch.randn(n, n, dtype=torch.float64, device=dev, generator=g)
+    Q, Rq = torch.linalg.qr(G)
+    Q *= torch.sign(torch.diagonal(Rq))
+    del G, Rq
+    B = torch.randn(nrhs, n, dtype=torch.float64, device=dev, generator=g).t()
+    out = {"n": n, "nrhs": nrhs, "tol": 1e-12, "cases": []}
+    for kappa in (1e2, 1e4, 1e6):
+        sig = torch.logspace(0, -float(np.log10(kappa)), n, dtype=torch.float64, device=dev)
+        A = (Q * sig) @ Q.t()
+        A = A.t().contiguous().t()
+        row = {"kappa": kappa}
+        for prec in ("fp64", "auto", "bf16", "fp16"):
+            try:
+                res = {}
+
+                def run():
+                    X, r = gla.chol_solve_mixed(A, B, precision=prec, tol=1e-12, max_iter=30)
+                    res["r"] = r
+
+                t = ev_time(torch, run, reps=2)
+                r = res["r"]
+                row[prec] = {"ms": t * 1e3, "precision": r.precision, "iters": r.iters,
+                             "berr": r.berr, "converged": r.converged}
+            except gla.DomainError as e:
+                row[prec] = {"error": f"DomainError (pivot {e})"}
+        if "ms" in row["fp64"] and "ms" in row["auto"] and row["auto"]["converged"]:
+            row["speedup_auto_vs_fp64"] = row["fp64"]["ms"] / row["auto"]["ms"]
+        out["cases"].append(row)
+        del A
+    return out
+
+
+def batched_section(torch, gla, count=10000):
+    """10,000 SPD matrices, n uniform in [8, 256], A = BᵀB + n·I, one RHS,
+    factor + solve in one persistent launch, captured in a graph and
+    replayed; FP64 and FP32."""
+    sys.path.insert(0, os.path.join(ROOT, "genericlinearalgebra.jl_amd"))
+    from gla import autotune
+    out = {"count": count}
+    for dt, key in ((torch.float64, "fp64"), (torch.float32, "fp32")):
+        sizes, pb = autotune.make_batch(count, dt, seed=0)
+        A0, b0 = pb.A.clone(), pb.b.clone()
+        plan = gla.BatchPlan(sizes, dt)
+        s = torch.cuda.Stream()
+        s.wait_stream(torch.cuda.current_stream())
+        with torch.cuda.stream(s):
+            gla.chol_solve_batched_(plan, pb.A_ptrs, pb.lda, pb.b_ptrs, pb.info)
+        torch.cuda.current_stream().wait_stream(s)
+        torch.cuda.synchronize()
+        graph = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(graph):
+            gla.chol_solve_batched_(plan, pb.A_ptrs, pb.lda, pb.b_ptrs, pb.info)
+        ts = []
+        for _ in range(5):
+            pb.A.copy_(A0)
+            pb.b.copy_(b0)
+            torch.cuda.synchronize()
+            ts.append(ev_time(torch, graph.replay, reps=1))
+        t = sorted(ts)[len(ts) // 2]
+        nn = sizes.astype(np.float64)
+        esz = 8 if key == "fp64" else 4
+        flops = float(np.sum(nn ** 3 / 3 + 2 * nn ** 2))
+        nbytes = float(np.sum(esz * (nn * (nn + 1) + 2 * nn)))
+        bad = int((pb.info != 0).sum().item())
+        out[key] = {"ms": t * 1e3, "tflops": flops / t / 1e12, "gbs": nbytes / t / 1e9,
+                    "hbm_frac": nbytes / t / 8e12, "info_nonzero": bad,
+                    "wave_max": plan.wave_max, "grid_mult": plan.grid_mult}
+    return out
+
+
+def adjoint_section(torch, gla):
+    """Cholesky adjoint for 512 matrices n=128 (FP64, factors recomputed)
+    and one n=8192 (FP32); QR adjoint for 4096×256 (FP64)."""
+    dev = torch.device("cuda:0")
+    g = torch.Generator(device=dev).manual_seed(5)
+    out = {}
+    n, cnt = 128, 512
+    Bs = torch.randn(cnt, n, n, dtype=torch.float64, device=dev, generator=g)
+    As = torch.bmm(Bs.transpose(1, 2).contiguous(), Bs) + n * torch.eye(n, dtype=torch.float64,
+                                                                         device=dev)
+    Ws = torch.randn(cnt, n, n, dtype=torch.float64, device=dev, generator=g)
+    Ls = [gla.colmajor(As[i]) for i in range(cnt)]
+    Gs = [gla.colmajor(torch.tril(Ws[i])) for i in range(cnt)]
+    outs = [gla.empty_colmajor(n, n, torch.float64) for _ in range(cnt)]
+    A0s = [L.clone() for L in Ls]
+
+    def run_batch():
+        for i in range(cnt):
+            Ls[i].copy_(A0s[i])
+            gla.cholRecursive_(Ls[i], gla.Val.L)            # recompute the factor
+            gla.chol_adjoint(Ls[i], Gs[i], "S", out=outs[i])
+
+    t = ev_time(torch, run_batch, reps=2)
+    out["chol_512x128_fp64"] = {"ms": t * 1e3,
+                                "tflops": cnt * 7 * n ** 3 / 3 / t / 1e12,
+                                "note": "one adjoint call per matrix, factor recomputed"}
+    n = 8192
+    Bm = torch.randn(n, n, dtype=torch.float32, device=dev, generator=g)
+    A = (Bm.t() @ Bm + n * torch.eye(n, device=dev)).t().contiguous().t()
+    del Bm
+    L = A.clone()
+    gla.cholRecursive_(L, gla.Val.L)
+    W = gla.colmajor(torch.tril(torch.randn(n, n, device=dev, generator=g)))
+    Ab = gla.empty_colmajor(n, n, torch.float32)
+    t = ev_time(torch, lambda: gla.chol_adjoint(L, W, "S", out=Ab), reps=2)
+    out["chol_8192_fp32"] = {"ms": t * 1e3, "tflops": 7 * n ** 3 / 3 / t / 1e12}
+    del A, L, W, Ab
+    m, k = 4096, 256
+    M = gla.colmajor(torch.randn(m, k, dtype=torch.float64, device=dev, generator=g))
+    q = gla.qrBlocked_(M, 128)
+    Rb = gla.colmajor(torch.triu(torch.randn(k, k, dtype=torch.float64, device=dev, generator=g)))
+    Abq = gla.empty_colmajor(m, k, torch.float64)
+    t = ev_time(torch, lambda: gla.qr_adjoint(q.factors, q.tau, Rb, out=Abq), reps=3)
+    fl = k ** 3 / 3 + k ** 3 + (4 * m * k * k - 2 * k ** 3)
+    out["qr_4096x256_fp64"] = {"ms": t * 1e3, "tflops": fl / t / 1e12}
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-extras", action="store_true",
+                    help="skip the mixed-precision, batched and adjoint sections")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
@@ -132,47 +275,53 @@ def main():
     dt = (time.perf_counter() - t0) / args.steps
     flops = chol_flops(n) + qr_flops(M_QR, N_QR)
 
-    # each factorisation alone, HIP events on the current stream
-    def ev_time(fn, reps=2):
-        ts = []
-        for _ in range(reps):
-            e0 = torch.cuda.Event(enable_timing=True)
-            e1 = torch.cuda.Event(enable_timing=True)
-            e0.record()
-            fn()
-            e1.record()
-            torch.cuda.synchronize()
-            ts.append(e0.elapsed_time(e1) / 1e3)
-        return min(ts)
+    t_copy_a = ev_time(torch, lambda: Aw.copy_(A0))
+    t_copy_m = ev_time(torch, lambda: Mw.copy_(M0))
+    t_chol = ev_time(torch, chol)       # leaves the factors in Aw, Mw for the checks
+    t_qr = ev_time(torch, qr)
 
-    t_copy_a = ev_time(lambda: Aw.copy_(A0))
-    t_copy_m = ev_time(lambda: Mw.copy_(M0))
-    t_chol = ev_time(chol)       # leaves the factors in Aw, Mw for the checks
-    t_qr = ev_time(qr)
-
-    # size-independent parity checks at full size (Aw, Mw hold the last factors)
-    idx = torch.randperm(n, device=dev)[:64]
+    # parity at full size: the factors against rocSOLVER's (an independent
+    # library), to 10·n·eps, plus size-independent residuals
+    eps = float(np.finfo(np.float64).eps)
     L = torch.tril(Aw)
+    Lref = torch.linalg.cholesky(A0)
+    chol_diff = ((L - Lref).norm() / Lref.norm()).item()
+    del Lref
+    idx = torch.randperm(n, device=dev)[:64]
     chol_bwd = ((L[idx] @ L.t() - A0[idx]).norm() / A0[idx].norm()).item()
     del L
+    Rref = torch.linalg.qr(M0, mode="r")[1]
+    R = torch.triu(Mw[:N_QR, :N_QR])
+    sgn = torch.sign(torch.diagonal(R)) * torch.sign(torch.diagonal(Rref))
+    qr_diff = ((R - sgn[:, None] * Rref).norm() / Rref.norm()).item()
+    del Rref
     x = torch.randn(N_QR, 1, dtype=torch.float64, device=dev)
-    Rx = torch.triu(Mw[:N_QR, :N_QR]) @ x
     QRx = torch.zeros(M_QR, 1, dtype=torch.float64, device=dev)
-    QRx[:N_QR] = Rx
-    # Q·(R x) through F[Tuple{:QBlocked}] (the compact-WY block), against M x
-    Hq = gla.QR2(Mw, tau)[gla.QBlocked]
+    QRx[:N_QR] = R @ x
+    Hq = gla.QR2(Mw, tau)[gla.QBlocked]           # Q·(R x) through the compact-WY block
     QRx = Hq @ gla.colmajor(QRx)
     qr_bwd = ((QRx - M0 @ x).norm() / (M0 @ x).norm()).item()
+    del R, QRx
 
-    # roofline of the dominant kernel: the MFMA SYRK of the top recursion
-    # level of cholRecursive! at n = 16384 (C 8192×8192 lower -= A21 A21ᵀ,
-    # K = 8192), one launch timed with HIP events on the stream it runs on
-    h = n // 2
-    Cs = torch.empty_like(A0)[:h, :h]
-    Xs = A0[h:, :h]
-    t_syrk = ev_time(lambda: gla.rankUpdate_(gla.Hermitian(Cs, "L"), Xs, -1.0), reps=3)
-    syrk_flops = float(h) * float(h) * float(h)       # n(n+1)k ≈ n²k for the lower triangle
-    achieved = syrk_flops / t_syrk / 1e12
+    # roofline of the dominant kernel, the QR panel (qr_panel_kernel: the
+    # largest share of the step's kernel time): one launch on a fresh
+    # 8192×128 panel (qrUnblocked! of the panel is exactly that launch),
+    # algorithmic bytes 16·m·nb (read + write of the panel)
+    P0 = M0[:, :NB_QR].contiguous().t().contiguous().t()
+    Pw = torch.empty_like(P0)
+    ptau = torch.zeros(NB_QR, dtype=torch.float64, device=dev)
+    t_pc = ev_time(torch, lambda: Pw.copy_(P0), reps=3)
+    t_panel = ev_time(torch, lambda: (Pw.copy_(P0), gla.qrUnblocked_(Pw, ptau)), reps=5) - t_pc
+    panel_bytes = 16.0 * M_QR * NB_QR
+    achieved = panel_bytes / t_panel / 1e9
+    # the MFMA engine, for reference: the trailing SYRK of the look-ahead
+    # driver's first step (C 15360×15360 lower -= A Aᵀ, K = 512)
+    kk, mm = 512, n - 2 * 512
+    Cs = torch.empty_like(A0)[:mm, :mm]
+    Xs = A0[2 * kk:, :kk]
+    t_syrk = ev_time(torch, lambda: gla.rankUpdate_(gla.Hermitian(Cs, "L"), Xs, -1.0), reps=3)
+    syrk_tf = float(mm) * float(mm) * kk / t_syrk / 1e12
+    del Cs
 
     out = {
         "metric": "FP64 Cholesky/QR TFLOP/s & %peak; mixed+IR vs FP64 solve time at 1e-12 bwd err",
@@ -186,7 +335,7 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f64",
-        "data": "synthetic (device RNG): A = BᵀB + n·I, B iid N(0,1); M iid N(0,1)",
+        "data": "synthetic (device RNG, seed 0): A = BᵀB + n·I, B iid N(0,1); M iid N(0,1)",
         "config": {
             "workload": (f"cholRecursive!(copy(A), Val{{:L}}) n={n} + qrBlocked!(copy(M), {NB_QR}) "
                          f"{M_QR}x{N_QR}, Float64, one MI355X"),
@@ -196,15 +345,31 @@ def main():
         "chol": {"ms": t_chol * 1e3, "tflops": chol_flops(n) / (t_chol - t_copy_a) / 1e12,
                  "tflops_incl_copy": chol_flops(n) / t_chol / 1e12,
                  "pct_peak": 100 * chol_flops(n) / (t_chol - t_copy_a) / 1e12 / FP64_PEAK_TFLOPS,
+                 "factor_rel_diff_vs_rocsolver": chol_diff, "bar_10_n_eps": 10 * n * eps,
                  "backward_error_rows": chol_bwd},
         "qr": {"ms": t_qr * 1e3, "tflops": qr_flops(M_QR, N_QR) / (t_qr - t_copy_m) / 1e12,
                "pct_peak": 100 * qr_flops(M_QR, N_QR) / (t_qr - t_copy_m) / 1e12 / FP64_PEAK_TFLOPS,
-               "residual_QRx_vs_Mx": qr_bwd},
-        "roofline": {"bound": "mfma", "achieved": achieved, "peak": FP64_PEAK_TFLOPS,
-                     "unit": "TFLOP/s", "frac": achieved / FP64_PEAK_TFLOPS, "traffic": None,
-                     "kernel": "gemm_kernel<double,128,128,N,T,lower> (SYRK 8192x8192, K=8192)",
-                     "ms_per_launch": t_syrk * 1e3},
+               "R_rel_diff_vs_rocsolver_sign_normalised": qr_diff,
+               "bar_10_n_eps": 10 * M_QR * eps, "residual_QRx_vs_Mx": qr_bwd},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": 8000.0, "unit": "GB/s",
+                     "frac": achieved / 8000.0, "traffic": None,
+                     "kernel": "qr_panel_kernel<double,128,16> (panel 8192x128, 64 workgroups)",
+                     "bytes_per_launch": panel_bytes, "us_per_launch": t_panel * 1e6},
+        "roofline_gemm": {"bound": "mfma", "achieved": syrk_tf, "peak": FP64_PEAK_TFLOPS,
+                          "unit": "TFLOP/s", "frac": syrk_tf / FP64_PEAK_TFLOPS,
+                          "kernel": "gemm_kernel<double,128,128,N,T,lower> (SYRK 15360², K=512)",
+                          "ms_per_launch": t_syrk * 1e3},
     }
+    del A0, Aw, M0, Mw
+    torch.cuda.empty_cache()
+    if not args.no_extras:
+        for key, fn in (("mixed", mixed_section), ("batched", batched_section),
+                        ("adjoint", adjoint_section)):
+            try:
+                out[key] = fn(torch, gla)
+            except Exception as e:  # noqa: BLE001  (a section failure is reported, not fatal)
+                out[key] = {"error": f"{type(e).__name__}: {e}"}
+            torch.cuda.empty_cache()
     if not args.no_cpu:
         cores, model = cpu_info()
         cb = cpu_baseline()

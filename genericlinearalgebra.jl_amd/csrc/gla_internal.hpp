@@ -54,7 +54,7 @@ struct Handle {
     int num_cus = 256;
     // Tuning: block width of the look-ahead driver behind cholRecursive!
     // (0: the recursive-halving driver).  Set with gla_set_tuning.
-    int64_t chol_block = 0;
+    int64_t chol_block = 512;
     // Block width of the mixed-precision factorisation's look-ahead driver.
     int mixed_block = 512;
 };

@@ -176,13 +176,10 @@ __global__ __launch_bounds__(kQrThreads) void qr_panel_kernel(QrPanelP<T> p)
         // (2) partial Σ_{r>j} a_rc x_r and the row-j entry of column c
         T dot = T(0), rowj = T(0);
         {
-            T xv[RPT];
-#pragma unroll
-            for (int q = 0; q < RPT; ++q) xv[q] = xs[g * RPT + q];
 #pragma unroll
             for (int q = 0; q < RPT; ++q) {
                 const int r = rb + q;
-                const T pr = a[q] * xv[q];
+                const T pr = a[q] * xs[g * RPT + q];
                 dot += (r > j) ? pr : T(0);
                 rowj = (r == j) ? a[q] : rowj;
             }
@@ -273,15 +270,12 @@ __global__ __launch_bounds__(kQrThreads) void qr_panel_kernel(QrPanelP<T> p)
 
         // (5) apply to the remaining columns; column j becomes (β, v)
         if (cact && live) {
-            T xv[RPT];
-#pragma unroll
-            for (int q = 0; q < RPT; ++q) xv[q] = xs[g * RPT + q];
             if (c > j) {
                 const T coef = tauj * (tot[NB + c] + tot[c] / xi);
 #pragma unroll
                 for (int q = 0; q < RPT; ++q) {
                     const int r = rb + q;
-                    const T v = (r == j) ? T(1) : xv[q];
+                    const T v = (r == j) ? T(1) : xs[g * RPT + q];
                     const T upd = a[q] - coef * v;
                     a[q] = (r >= j) ? upd : a[q];
                 }
@@ -289,7 +283,7 @@ __global__ __launch_bounds__(kQrThreads) void qr_panel_kernel(QrPanelP<T> p)
 #pragma unroll
                 for (int q = 0; q < RPT; ++q) {
                     const int r = rb + q;
-                    a[q] = (r > j) ? xv[q] : (r == j ? -nu : a[q]);
+                    a[q] = (r > j) ? xs[g * RPT + q] : (r == j ? -nu : a[q]);
                 }
             }
         }
@@ -522,8 +516,8 @@ int qr_panel(Handle *h, hipStream_t s, int64_t m, int w, T *A, int64_t lda, T *t
     p.bar = h->dbar;
     p.bar_base = bar_base;
     p.err = h->dinfo + 1;
-    hipLaunchKernelGGL((qr_panel_kernel<T, kQrNB, kQrRPT>), dim3((unsigned)P), dim3(kQrThreads),
-                       0, s, p);
+    hipLaunchKernelGGL((qr_panel_kernel<T, kQrNB, kQrRPT>), dim3((unsigned)P), dim3(kQrThreads), 0,
+                       s, p);
     GLA_LAUNCH_CHECK();
     if (P > 1) bar_base += (unsigned)(p.kmax * P);
     (void)err;

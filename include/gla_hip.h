@@ -46,11 +46,11 @@ int gla_handle_destroy(gla_handle_t h);
 const char *gla_last_error_string(void);
 /* Version string of this build. */
 const char *gla_version(void);
-/* Tuning knobs of a handle (no reference counterpart).  "chol_block": 0
- * (default) runs cholRecursive! as recursive halving down to 2048-wide
- * leaves; a multiple of 128 runs it as a right-looking blocked
- * factorisation with blocks of that width and depth-1 look-ahead.  Both
- * compute the same factor.  "mixed_block": block width (multiple of 128,
+/* Tuning knobs of a handle (no reference counterpart).  "chol_block": a
+ * multiple of 128 (default 512) runs cholRecursive! as a right-looking
+ * blocked factorisation with blocks of that width and depth-1 look-ahead;
+ * 0 runs it as recursive halving down to 2048-wide leaves.  Both compute
+ * the same factor.  "mixed_block": block width (multiple of 128,
  * default 512) of the look-ahead driver of the BF16/FP16 factorisation in
  * gla_chol_solve_mixed_d. */
 int gla_set_tuning(gla_handle_t h, const char *key, int64_t value);

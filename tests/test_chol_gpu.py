@@ -81,12 +81,18 @@ def test_sizes_against_oracle(rng, dtype, n, algo):
 
 
 @pytest.mark.parametrize("n", [2500, 4100])
-def test_large_recursive_backward_error(rng, n):
-    """Past the 2048 leaf the recursion is exercised; checked against the
-    oracle's factor and by ‖A − LLᵀ‖/‖A‖."""
+@pytest.mark.parametrize("block", [0, 256, 512, 1024])
+def test_large_recursive_backward_error(rng, n, block):
+    """Both device drivers behind cholRecursive!: recursive halving past the
+    2048 leaf (block 0) and the wide-block look-ahead (block 256/512/1024);
+    checked against LAPACK's factor and by ‖A − LLᵀ‖/‖A‖."""
     gla = _gla()
     A = spd(n, rng)
-    out, _ = _run(lambda X: gla.cholRecursive_(X, gla.Val.L), A, np.float64)
+    gla.set_tuning("chol_block", block)
+    try:
+        out, _ = _run(lambda X: gla.cholRecursive_(X, gla.Val.L), A, np.float64)
+    finally:
+        gla.set_tuning("chol_block", 512)
     L = np.tril(out)
     Lr = np.linalg.cholesky(A)
     assert rel(L, Lr) <= 10 * n * eps_of(np.float64)
