@@ -83,6 +83,20 @@ def cpu_baseline():
     }
 
 
+def panel_traffic():
+    """HBM bytes per panel launch measured with rocprofv3 --pmc FETCH_SIZE and
+    WRITE_SIZE (profiles/qr_panel_8192x128.md records the runs); None if the
+    summary is absent."""
+    path = os.path.join(ROOT, "profiles", "qr_panel_8192x128.md")
+    try:
+        for line in open(path):
+            if line.startswith("traffic_bytes_per_launch:"):
+                return int(line.split(":", 1)[1])
+    except OSError:
+        pass
+    return None
+
+
 def ev_time(torch, fn, reps=2):
     """Best of `reps` HIP-event timings of fn on the current stream (s)."""
     ts = []
@@ -230,6 +244,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extras", action="store_true",
                     help="skip the mixed-precision, batched and adjoint sections")
+    ap.add_argument("--profile-panel", action="store_true",
+                    help="only run the QR panel launch (8192x128) and a plain copy of the "
+                         "same panel five times each, for hardware-counter runs")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
@@ -242,6 +259,16 @@ def main():
 
     dev = torch.device("cuda:0")
     torch.manual_seed(0)
+    if args.profile_panel:
+        P0 = torch.randn(NB_QR, M_QR, dtype=torch.float64, device=dev).t()
+        Pw = torch.empty_like(P0)
+        ptau = torch.zeros(NB_QR, dtype=torch.float64, device=dev)
+        for _ in range(5):
+            Pw.copy_(P0)
+            gla.qrUnblocked_(Pw, ptau)
+        torch.cuda.synchronize()
+        print(json.dumps({"profile_panel": f"{M_QR}x{NB_QR}", "launches": 5}))
+        return
     n = N_CHOL
     Bm = torch.randn(n, n, dtype=torch.float64, device=dev)
     A0 = torch.addmm(torch.eye(n, dtype=torch.float64, device=dev), Bm.t(), Bm, beta=n, alpha=1.0)
@@ -352,7 +379,7 @@ def main():
                "R_rel_diff_vs_rocsolver_sign_normalised": qr_diff,
                "bar_10_n_eps": 10 * M_QR * eps, "residual_QRx_vs_Mx": qr_bwd},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": 8000.0, "unit": "GB/s",
-                     "frac": achieved / 8000.0, "traffic": None,
+                     "frac": achieved / 8000.0, "traffic": panel_traffic(),
                      "kernel": "qr_panel_kernel<double,128,16> (panel 8192x128, 64 workgroups)",
                      "bytes_per_launch": panel_bytes, "us_per_launch": t_panel * 1e6},
         "roofline_gemm": {"bound": "mfma", "achieved": syrk_tf, "peak": FP64_PEAK_TFLOPS,
