@@ -57,6 +57,8 @@ struct Handle {
     int64_t chol_block = 512;
     // Block width of the mixed-precision factorisation's look-ahead driver.
     int mixed_block = 512;
+    // QR panel kernel: 1 LDS-reduction form, 2 wave-butterfly form.
+    int qr_panel = 2;
 };
 
 // Ensure the workspace holds at least `bytes` (grows with hipMalloc, which

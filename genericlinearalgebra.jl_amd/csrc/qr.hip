@@ -36,7 +36,8 @@ namespace {
 
 constexpr int kQrThreads = 1024;
 constexpr int kQrNB = 128;   // max panel width
-constexpr int kQrRPT = 16;   // rows per thread
+constexpr int kQrRPT = 16;   // rows per thread (LDS-reduction panel kernel)
+constexpr int kQrRPL = 2;    // rows per lane (wave-butterfly panel kernel): 128 rows per workgroup
 constexpr unsigned kSpinLimit = 1u << 25;
 
 #define GLA_TRY(expr)                  \
@@ -125,6 +126,245 @@ __device__ __forceinline__ void t_column(T *Ts, const T *g, T *tpart, int jj, T 
         Ts[t + jj * NB] = -tau * s;
     }
     if (t == jj) Ts[jj + jj * NB] = tau;
+}
+
+// Sum over the wave of v[c], c < 8, by a transposing butterfly: after the
+// xor-32/16/8 exchanges each lane keeps one column's partial, the xor-4/2/1
+// steps finish it.  Lane l returns the total of column
+// c(l) = 4·bit5(l) + 2·bit4(l) + bit3(l).  Fixed order: every run and every
+// lane adds the same values in the same order.
+template <typename T>
+__device__ __forceinline__ T wave_reduce8(const T v[8])
+{
+    const int lane = threadIdx.x & 63;
+    const bool b5 = lane & 32, b4 = lane & 16, b3 = lane & 8;
+    T u[4], w[2];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const T send = b5 ? v[k] : v[k + 4];
+        u[k] = (b5 ? v[k + 4] : v[k]) + __shfl_xor(send, 32);
+    }
+#pragma unroll
+    for (int k = 0; k < 2; ++k) {
+        const T send = b4 ? u[k] : u[k + 2];
+        w[k] = (b4 ? u[k + 2] : u[k]) + __shfl_xor(send, 16);
+    }
+    const T send = b3 ? w[0] : w[1];
+    T z = (b3 ? w[1] : w[0]) + __shfl_xor(send, 8);
+    z += __shfl_xor(z, 4);
+    z += __shfl_xor(z, 2);
+    z += __shfl_xor(z, 1);
+    return z;
+}
+
+// Householder panel, one column at a time as qrUnblocked! does
+// (src/qr.jl:86-111), over P workgroups that each hold R = 64·RPL rows of
+// the m×w panel in registers: wave v holds CPW = NB/(NTH/64) consecutive
+// columns, lane l holds rows l + 64q (q < RPL).  Per column j:
+//   (1) the wave holding column j puts x = column j in LDS;
+//   (2) every lane forms its rows' share of aᶜᵀx (rows > j) for its
+//       wave's columns, a wave butterfly sums them, one lane per column publishes
+//       the workgroup's partial (device-coherent store); workgroup 0 also
+//       publishes row j (it holds rows 0..R-1, and j < w ≤ 128 ≤ R);
+//   (3) grid barrier; every workgroup gathers the P partials of each column
+//       in a fixed order, so all hold identical totals;
+//   (4) Julia's reflector! from the totals (ν = copysign(‖x‖, x_j),
+//       ξ = x_j + ν, τ = ξ/ν, v = x/ξ), each lane dividing its own rows;
+//   (5) columns c > j take a_c -= τ(a_jc + aᶜᵀx/ξ)·v (reflectorApply!),
+//       column j becomes (β = −ν, v).
+// Workgroup 0 also builds the compact-WY T column by column from the same
+// totals (vᵢᵀv_j = a_ji + aⁱᵀx/ξ), overlapped with the grid barrier.
+template <typename T, int NB, int RPL, int NTH>
+__global__ __launch_bounds__(NTH) void qr_panel2_kernel(QrPanelP<T> p)
+{
+    constexpr int R = 64 * RPL;
+    constexpr int H = NTH / NB;                  // partial-sum groups of the gather
+    constexpr int CPW = NB / (NTH / 64);         // columns per wave
+    static_assert(CPW == 8, "eight columns per wave");
+    static_assert(R >= NB, "row j of every column lives in workgroup 0");
+
+    __shared__ T xs[2][R];
+    __shared__ T red[H][NB];        // gather sub-sums; also the T recurrence's scratch
+    __shared__ T tot[NB];           // Σ_{r>j} a_rc x_r over the panel
+    __shared__ T rowj[NB];          // a_jc
+    __shared__ T gsh[NB];           // v_iᵀ v_j, i < j (workgroup 0)
+    __shared__ T Ts[NB * NB];       // T, column-major (workgroup 0)
+    T *const tpart = &red[0][0];
+
+    const int t = threadIdx.x, lane = t & 63;
+    const int wv = __builtin_amdgcn_readfirstlane(t >> 6);
+    const int wg = blockIdx.x, P = gridDim.x;
+    const int64_t m = p.m, lda = p.lda;
+    const int64_t row0 = (int64_t)wg * R;
+    const int c0 = CPW * wv;                     // first column of this wave
+
+    T a[CPW][RPL];
+#pragma unroll
+    for (int c = 0; c < CPW; ++c)
+#pragma unroll
+        for (int q = 0; q < RPL; ++q) {
+            const int64_t r = row0 + lane + 64 * q;
+            a[c][q] = (r < m && c0 + c < p.w) ? p.A[r + (int64_t)(c0 + c) * lda] : T(0);
+        }
+    if (wg == 0)
+        for (int e = t; e < NB * NB; e += NTH) Ts[e] = T(0);
+    T tau_prev = T(0);
+
+    for (int j = 0; j < p.kmax; ++j) {
+        const int buf = j & 1;
+        const int jw = j / CPW, jc = j % CPW;
+        // (1) column j to LDS
+        if (wv == jw) {
+#pragma unroll
+            for (int q = 0; q < RPL; ++q) {
+                T v = a[0][q];
+#pragma unroll
+                for (int c = 1; c < CPW; ++c) v = (c == jc) ? a[c][q] : v;
+                xs[buf][lane + 64 * q] = v;
+            }
+        }
+        __syncthreads();
+        T x[RPL];
+#pragma unroll
+        for (int q = 0; q < RPL; ++q) x[q] = xs[buf][lane + 64 * q];
+        // (2) partial aᶜᵀx over rows > j, summed over the wave
+        T pr[CPW];
+#pragma unroll
+        for (int c = 0; c < CPW; ++c) {
+            T s = T(0);
+#pragma unroll
+            for (int q = 0; q < RPL; ++q) s += (row0 + lane + 64 * q > j) ? a[c][q] * x[q] : T(0);
+            pr[c] = s;
+        }
+        const T z = wave_reduce8(pr);
+        const int zc = c0 + 4 * ((lane >> 5) & 1) + 2 * ((lane >> 4) & 1) + ((lane >> 3) & 1);
+        const bool zl = (lane & 7) == 0;
+        // row j of this wave's columns (workgroup 0 holds rows 0..R-1)
+        const bool rl = wg == 0 && lane == (j & 63);
+        const int rq = j >> 6;
+        if (P > 1) {
+            T *part = p.part + (size_t)buf * (P + 1) * NB;
+            if (zl) store_coherent(part + (size_t)wg * NB + zc, z);
+            if (rl) {
+#pragma unroll
+                for (int c = 0; c < CPW; ++c) {
+                    T v = a[c][0];
+#pragma unroll
+                    for (int q = 1; q < RPL; ++q) v = (q == rq) ? a[c][q] : v;
+                    store_coherent(part + (size_t)P * NB + c0 + c, v);
+                }
+            }
+            stores_done();
+            grid_arrive(p.bar);
+            if (wg == 0 && j > 0) t_column<T, NB, NTH>(Ts, gsh, tpart, j - 1, tau_prev);
+            grid_wait(p.bar, p.bar_base + (unsigned)(j + 1) * (unsigned)P, p.err);
+            // (3) totals in a fixed order: H sub-sums per column over the
+            // partials p ≡ h (mod H), eight loads in flight, then the H in order
+            {
+                const int e = t % NB, h = t / NB;
+                T s = T(0);
+                for (int pp0 = h; pp0 < P; pp0 += 8 * H) {
+                    const T *ptr[8];
+                    T v[8];
+#pragma unroll
+                    for (int u = 0; u < 8; ++u) {
+                        const int pp = pp0 + u * H;
+                        ptr[u] = part + (size_t)(pp < P ? pp : h) * NB + e;
+                    }
+                    load8_coherent(ptr, v);
+#pragma unroll
+                    for (int u = 0; u < 8; ++u)
+                        if (pp0 + u * H < P) s += v[u];
+                }
+                red[h][e] = s;
+                if (t < NB) rowj[t] = load_coherent(part + (size_t)P * NB + t);
+                __syncthreads();
+                if (t < NB) {
+                    T s2 = T(0);
+#pragma unroll
+                    for (int hh = 0; hh < H; ++hh) s2 += red[hh][t];
+                    tot[t] = s2;
+                }
+            }
+        } else {
+            if (zl) tot[zc] = z;
+            if (rl) {
+#pragma unroll
+                for (int c = 0; c < CPW; ++c) {
+                    T v = a[c][0];
+#pragma unroll
+                    for (int q = 1; q < RPL; ++q) v = (q == rq) ? a[c][q] : v;
+                    rowj[c0 + c] = v;
+                }
+            }
+            __syncthreads();   // red is the T recurrence's scratch below
+            if (j > 0) t_column<T, NB, NTH>(Ts, gsh, tpart, j - 1, tau_prev);
+        }
+        __syncthreads();
+
+        // (4) the reflector of column j
+        const T xj = rowj[j];
+        const T nrm = dsqrt(xj * xj + tot[j]);
+        const bool live = nrm != T(0);
+        const T nu = dcopysign(nrm, xj);
+        const T xi = xj + nu;
+        const T tauj = live ? xi / nu : T(0);
+        const T rxi = live ? T(1) / xi : T(0);
+        if (wg == 0 && t < j) gsh[t] = live ? rowj[t] + tot[t] * rxi : T(0);
+
+        // (5) apply to the remaining columns; column j becomes (β, v)
+        if (live) {
+            T v[RPL];
+#pragma unroll
+            for (int q = 0; q < RPL; ++q) {
+                const int64_t r = row0 + lane + 64 * q;
+                v[q] = r > j ? x[q] * rxi : (r == j ? T(1) : T(0));
+            }
+#pragma unroll
+            for (int c = 0; c < CPW; ++c) {
+                const int cg = c0 + c;
+                if (cg > j && cg < p.w) {
+                    const T coef = tauj * (rowj[cg] + tot[cg] * rxi);
+#pragma unroll
+                    for (int q = 0; q < RPL; ++q) a[c][q] -= coef * v[q];
+                } else if (cg == j) {
+#pragma unroll
+                    for (int q = 0; q < RPL; ++q) {
+                        const int64_t r = row0 + lane + 64 * q;
+                        a[c][q] = r > j ? v[q] : (r == j ? -nu : a[c][q]);
+                    }
+                }
+            }
+        }
+        if (wg == 0 && t == 0) p.tau[j] = tauj;
+        tau_prev = tauj;
+    }
+    __syncthreads();
+    if (wg == 0 && p.kmax > 0) {
+        t_column<T, NB, NTH>(Ts, gsh, tpart, p.kmax - 1, tau_prev);
+        __syncthreads();
+    }
+
+    // Write back the factored panel and the explicit V.
+#pragma unroll
+    for (int c = 0; c < CPW; ++c) {
+        const int cg = c0 + c;
+        if (cg >= p.w) continue;
+#pragma unroll
+        for (int q = 0; q < RPL; ++q) {
+            const int64_t r = row0 + lane + 64 * q;
+            if (r >= m) continue;
+            p.A[r + (int64_t)cg * lda] = a[c][q];
+            if (p.Vw && cg < p.kmax)
+                p.Vw[r + (int64_t)cg * p.ldv] = r < cg ? T(0) : (r == cg ? T(1) : a[c][q]);
+        }
+    }
+    if (wg == 0 && p.Tw) {
+        for (int e = t; e < p.kmax * p.kmax; e += NTH) {
+            const int i = e % p.kmax, jj = e / p.kmax;
+            p.Tw[i + (int64_t)jj * p.ldt] = Ts[i + jj * NB];
+        }
+    }
 }
 
 template <typename T, int NB, int RPT>
@@ -482,19 +722,22 @@ struct QrWs {
 template <typename T>
 size_t qr_ws_elems(int64_t m, int64_t n, int nb, int64_t *split_elems)
 {
-    const int64_t R = (kQrThreads / kQrNB) * kQrRPT;
-    const int64_t P = std::max<int64_t>(1, cdiv(m, R));
+    // partial sums: 2 parities × (P + 1) × 2·NB covers both panel kernels
+    // (workgroups of at least 64 rows)
+    const int64_t P = std::max<int64_t>(1, cdiv(m, 64));
     const int64_t sp = 8 * (int64_t)nb * std::max<int64_t>(n, 1);
     *split_elems = sp;
     return (size_t)m * nb + (size_t)nb * nb + 2 * (size_t)nb * std::max<int64_t>(n, 1) +
-           2 * (size_t)P * 2 * kQrNB + (size_t)sp + 64;
+           2 * (size_t)(P + 1) * 2 * kQrNB + (size_t)sp + 64;
 }
 
 template <typename T>
 int qr_panel(Handle *h, hipStream_t s, int64_t m, int w, T *A, int64_t lda, T *tau, T *Vw,
              int64_t ldv, T *Tw, int64_t ldt, T *part, unsigned &bar_base, int *err)
 {
-    constexpr int R = (kQrThreads / kQrNB) * kQrRPT;
+    // 1: LDS-reduction kernel; 2: wave-butterfly kernel
+    const int panel = h->qr_panel;
+    const int R = panel == 1 ? (kQrThreads / kQrNB) * kQrRPT : 64 * kQrRPL;
     const int64_t P = std::max<int64_t>(1, cdiv(m, R));
     // every workgroup must be resident at once for the grid barrier
     if (P > h->num_cus) {
@@ -516,8 +759,12 @@ int qr_panel(Handle *h, hipStream_t s, int64_t m, int w, T *A, int64_t lda, T *t
     p.bar = h->dbar;
     p.bar_base = bar_base;
     p.err = h->dinfo + 1;
-    hipLaunchKernelGGL((qr_panel_kernel<T, kQrNB, kQrRPT>), dim3((unsigned)P), dim3(kQrThreads), 0,
-                       s, p);
+    if (panel == 1)
+        hipLaunchKernelGGL((qr_panel_kernel<T, kQrNB, kQrRPT>), dim3((unsigned)P),
+                           dim3(kQrThreads), 0, s, p);
+    else
+        hipLaunchKernelGGL((qr_panel2_kernel<T, kQrNB, kQrRPL, kQrThreads>), dim3((unsigned)P),
+                           dim3(kQrThreads), 0, s, p);
     GLA_LAUNCH_CHECK();
     if (P > 1) bar_base += (unsigned)(p.kmax * P);
     (void)err;

@@ -66,6 +66,14 @@ int gla_set_tuning(gla_handle_t hh, const char *key, int64_t value)
         h->mixed_block = (int)value;
         return GLA_OK;
     }
+    if (std::strcmp(key, "qr_panel") == 0) {
+        if (value != 1 && value != 2) {
+            gla::set_error("qr_panel must be 1 or 2");
+            return -3;
+        }
+        h->qr_panel = (int)value;
+        return GLA_OK;
+    }
     gla::set_error("unknown tuning key '%s'", key);
     return -2;
 }
