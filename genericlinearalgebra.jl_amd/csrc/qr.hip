@@ -309,8 +309,7 @@ __global__ __launch_bounds__(NTH) void qr_panel2_kernel(QrPanelP<T> p)
         const T nu = dcopysign(nrm, xj);
         const T xi = xj + nu;
         const T tauj = live ? xi / nu : T(0);
-        const T rxi = live ? T(1) / xi : T(0);
-        if (wg == 0 && t < j) gsh[t] = live ? rowj[t] + tot[t] * rxi : T(0);
+        if (wg == 0 && t < j) gsh[t] = live ? rowj[t] + tot[t] / xi : T(0);
 
         // (5) apply to the remaining columns; column j becomes (β, v)
         if (live) {
@@ -318,13 +317,13 @@ __global__ __launch_bounds__(NTH) void qr_panel2_kernel(QrPanelP<T> p)
 #pragma unroll
             for (int q = 0; q < RPL; ++q) {
                 const int64_t r = row0 + lane + 64 * q;
-                v[q] = r > j ? x[q] * rxi : (r == j ? T(1) : T(0));
+                v[q] = r > j ? x[q] / xi : (r == j ? T(1) : T(0));   // x[i] /= ξ1
             }
 #pragma unroll
             for (int c = 0; c < CPW; ++c) {
                 const int cg = c0 + c;
                 if (cg > j && cg < p.w) {
-                    const T coef = tauj * (rowj[cg] + tot[cg] * rxi);
+                    const T coef = tauj * (rowj[cg] + tot[cg] / xi);
 #pragma unroll
                     for (int q = 0; q < RPL; ++q) a[c][q] -= coef * v[q];
                 } else if (cg == j) {
