@@ -4,29 +4,35 @@
 // Kernels
 //   chol_factor_kernel  One workgroup factors a diagonal block (≤128×128) in
 //                       place.  The block is held as 16×16 MFMA accumulator
-//                       tiles; for each 16-wide column block the 16×16
-//                       diagonal tile is factored in LDS by one wave, column
-//                       by column as cholUnblocked! does (sqrt, reciprocal
-//                       scale, rank-1 update); the tiles below solve
-//                       x·L_kkᵀ = b by substitution and the trailing tiles
-//                       take the rank-16 update C -= X_i X_jᵀ on MFMA.
+//                       tiles.  For each 16-wide column block, wave 0 factors
+//                       the 16×16 diagonal tile in registers column by column
+//                       as cholUnblocked! does (sqrt, reciprocal scale,
+//                       rank-1 update; lane shuffles broadcast the column)
+//                       and inverts it; the tiles below become X = A·L_kk⁻ᵀ
+//                       and the trailing tiles take the rank-16 update
+//                       C -= X_i X_jᵀ, both on MFMA.  The tile inverses are
+//                       kept for the panel solve.
 //   chol_solve_kernel   rdiv!(A21, LowerTriangular(A11)') (src/cholesky.jl:28,
 //                       48) for a ≤128-wide L11: each workgroup takes 64 rows
-//                       of A21, with the same 16-column blocking (substitution
-//                       against the 16×16 diagonal tile, MFMA rank-16 update
-//                       of the remaining columns against L11's tiles).
+//                       of A21, L11's tiles and tile inverses sit in LDS, and
+//                       every 16-column step is X = B_kb·L_kk⁻ᵀ then
+//                       B_j -= X·L_(j,kb)ᵀ on MFMA.  A batched form (grid y)
+//                       inverts many diagonal blocks at once.
 //   gemm (gemm.hip)     The trailing rankUpdate!(Hermitian(A22,:L), A21, -1)
 //                       as an MFMA SYRK over lower tiles.
 //
 // Drivers
-//   blocked   right-looking, block nb ≤ 128, depth-1 look-ahead: the update
-//             of the next block column runs on the caller's stream right
-//             after the panel, the rest of the trailing update runs on the
-//             handle's side stream and overlaps the next panel.
-//   recursive halves n as cholRecursive! does; below a threshold the blocked
-//             driver factors the leaf.  rdiv! with a large L11 is itself
-//             recursive (two half solves and a GEMM); its leaves are
-//             chol_solve_kernel launches.
+//   blocked    right-looking, block nb ≤ 128, depth-1 look-ahead: the update
+//              of the next block column runs on the caller's stream right
+//              after the panel, the rest of the trailing update runs on the
+//              handle's side stream and overlaps the next panel.
+//   look-ahead the same loop with wide blocks (NB = 512 by default): the
+//              diagonal block by the blocked driver, the panel by the
+//              recursive rdiv!, the trailing SYRK with K = NB.
+//   recursive  halves n as cholRecursive! does; below a threshold the blocked
+//              driver factors the leaf.  rdiv! with a large L11 is itself
+//              recursive (two half solves and a GEMM); its leaves are
+//              chol_solve_kernel launches.
 #include <algorithm>
 
 #include "gla_internal.hpp"
