@@ -54,12 +54,6 @@ constexpr int kXLD = 17;        // padded row length of the LDS column blocks
         if (_rc != GLA_OK) return _rc; \
     } while (0)
 
-template <typename T>
-__device__ __forceinline__ T dev_sqrt(T x);
-template <>
-__device__ __forceinline__ double dev_sqrt<double>(double x) { return __builtin_sqrt(x); }
-template <>
-__device__ __forceinline__ float dev_sqrt<float>(float x) { return __builtin_sqrtf(x); }
 
 // 16×16×4 MFMA with one A and one B element per lane; accumulator element r
 // of lane l sits at (row(l, r), l % 16) of the 16×16 tile.
@@ -88,14 +82,79 @@ __device__ __forceinline__ void wave_order() { __builtin_amdgcn_fence(__ATOMIC_S
 
 // One wave holds a 16×16 tile with lane (i = l % 16, g = l / 16) owning row
 // i, columns g, g+4, g+8, g+12 (v[t] = tile[i][g + 4t]).  Row i of column c
-// therefore sits in lane i + 16*(c % 4), slot c / 4, and a broadcast along a
-// column is one lane shuffle.
+// therefore sits in lane i + 16*(c % 4), slot c / 4.
+
+// v from lane `src` (a wave-uniform constant) to every lane, through SGPRs.
+template <typename T>
+__device__ __forceinline__ T lane_bcast(T v, int src)
+{
+    if constexpr (sizeof(T) == 8) {
+        int2 u = __builtin_bit_cast(int2, v);
+        u.x = __builtin_amdgcn_readlane(u.x, src);
+        u.y = __builtin_amdgcn_readlane(u.y, src);
+        return __builtin_bit_cast(T, u);
+    } else {
+        return __builtin_bit_cast(T, __builtin_amdgcn_readlane(__builtin_bit_cast(int, v), src));
+    }
+}
+
+// v from lane K of each 16-lane row to the whole row (DPP row_newbcast:K).
+template <int K, typename T>
+__device__ __forceinline__ T row_bcast(T v)
+{
+    constexpr int ctrl = 0x150 + K;
+    if constexpr (sizeof(T) == 8) {
+        int2 u = __builtin_bit_cast(int2, v);
+        u.x = __builtin_amdgcn_update_dpp(0, u.x, ctrl, 0xF, 0xF, false);
+        u.y = __builtin_amdgcn_update_dpp(0, u.y, ctrl, 0xF, 0xF, false);
+        return __builtin_bit_cast(T, u);
+    } else {
+        return __builtin_bit_cast(
+            T, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), ctrl, 0xF, 0xF, false));
+    }
+}
+
+// s = √a and r = 1/√a from the hardware reciprocal square root refined by
+// Newton steps (two for FP64, one for FP32) and one correction of s = a·r:
+// both within an ulp or two of sqrt(a) and 1/sqrt(a), at a fraction of the
+// latency of a correctly rounded square root followed by a division, which
+// sit on the serial path of every column.  a = 0 gives s = 0, r = ∞ and a
+// negative or NaN a gives NaN, as sqrt and 1/sqrt do.
+__device__ __forceinline__ void sqrt_and_rsqrt(double a, double &s, double &r)
+{
+    double y = __builtin_amdgcn_rsq(a);
+    const double h = 0.5 * a;
+    double e = __builtin_fma(-h * y, y, 0.5);
+    y = __builtin_fma(y, e, y);
+    e = __builtin_fma(-h * y, y, 0.5);
+    y = __builtin_fma(y, e, y);
+    double q = a * y;
+    q = __builtin_fma(__builtin_fma(-q, q, a), 0.5 * y, q);
+    const bool zero = a == 0.0;
+    s = zero ? 0.0 : q;
+    r = zero ? __builtin_inf() : y;
+}
+__device__ __forceinline__ void sqrt_and_rsqrt(float a, float &s, float &r)
+{
+    float y = __builtin_amdgcn_rsqf(a);
+    const float h = 0.5f * a;
+    const float e = __builtin_fmaf(-h * y, y, 0.5f);
+    y = __builtin_fmaf(y, e, y);
+    float q = a * y;
+    q = __builtin_fmaf(__builtin_fmaf(-q, q, a), 0.5f * y, q);
+    const bool zero = a == 0.0f;
+    s = zero ? 0.0f : q;
+    r = zero ? __builtin_inff() : y;
+}
 
 // Factors the symmetric tile in v in place, right-looking, column by column
 // as cholUnblocked! does (src/cholesky.jl:3-15): sqrt of the pivot, scale
 // by its reciprocal, rank-1 update.  rd[c] receives 1/L[c][c].  The first
 // pivot whose square root is taken of a negative number, among the first
-// `nvalid`, is recorded in *info as gbase + c + 1.
+// `nvalid`, is recorded in *info as gbase + c + 1.  The pivot comes through
+// a scalar register; the column entries a lane needs are fetched unscaled
+// while the square root and reciprocal are computed, then scaled locally
+// (the same products the scaled column holds).
 template <typename T>
 __device__ __forceinline__ void tile_factor(T v[4], T rd[16], int nvalid, int *info,
                                             int64_t gbase)
@@ -105,44 +164,69 @@ __device__ __forceinline__ void tile_factor(T v[4], T rd[16], int nvalid, int *i
 #pragma unroll
     for (int c = 0; c < 16; ++c) {
         const int ct = c >> 2, cg = c & 3;
-        T d = __shfl(v[ct], c + 16 * cg);
+        T d = lane_bcast(v[ct], c + 16 * cg);
+        const T aic = __shfl(v[ct], i + 16 * cg);
+        T ajc[4];
+#pragma unroll
+        for (int t = 0; t < 4; ++t) ajc[t] = __shfl(v[ct], g + 4 * t + 16 * cg);
         if (d < T(0) && lane == 0 && info && c < nvalid) atomicCAS(info, 0, (int)(gbase + c + 1));
-        d = dev_sqrt(d);
-        const T r = T(1) / d;
+        T r;
+        sqrt_and_rsqrt(d, d, r);
         rd[c] = r;
         if (g == cg) v[ct] = i > c ? v[ct] * r : (i == c ? d : v[ct]);
-        const T lic = __shfl(v[ct], i + 16 * cg);
+        const T lic = aic * r;
 #pragma unroll
         for (int t = 0; t < 4; ++t) {
             const int j = g + 4 * t;
-            const T ljc = __shfl(v[ct], j + 16 * cg);
-            if (j > c && j <= i) v[t] -= lic * ljc;
+            if (j > c && j <= i) v[t] -= lic * (ajc[t] * r);
         }
     }
 }
 
 // w <- L⁻¹ for the lower tile L in v (rd = reciprocals of its diagonal):
 // forward elimination of [L | I], row k scaled then subtracted from the rows
-// below it.
+// below it.  L's entries are fetched up front (they do not change); row k
+// of W reaches the other lanes of its 16-lane row by DPP broadcast.
+template <int K, typename T>
+__device__ __forceinline__ void tile_inverse_step(const T lik, const T rdk, T w[4])
+{
+    const int i = threadIdx.x & 15;
+    if (i == K)
+#pragma unroll
+        for (int t = 0; t < 4; ++t) w[t] *= rdk;
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+        const T wkj = row_bcast<K>(w[t]);
+        if (i > K) w[t] -= lik * wkj;
+    }
+}
+
 template <typename T>
 __device__ __forceinline__ void tile_inverse(const T v[4], const T rd[16], T w[4])
 {
     const int lane = threadIdx.x & 63;
     const int i = lane & 15, g = lane >> 4;
+    T li[16];
+#pragma unroll
+    for (int k = 0; k < 16; ++k) li[k] = __shfl(v[k >> 2], i + 16 * (k & 3));
 #pragma unroll
     for (int t = 0; t < 4; ++t) w[t] = (i == g + 4 * t) ? T(1) : T(0);
-#pragma unroll
-    for (int k = 0; k < 16; ++k) {
-        if (i == k)
-#pragma unroll
-            for (int t = 0; t < 4; ++t) w[t] *= rd[k];
-        const T lik = __shfl(v[k >> 2], i + 16 * (k & 3));
-#pragma unroll
-        for (int t = 0; t < 4; ++t) {
-            const T wkj = __shfl(w[t], k + 16 * g);
-            if (i > k) w[t] -= lik * wkj;
-        }
-    }
+    tile_inverse_step<0>(li[0], rd[0], w);
+    tile_inverse_step<1>(li[1], rd[1], w);
+    tile_inverse_step<2>(li[2], rd[2], w);
+    tile_inverse_step<3>(li[3], rd[3], w);
+    tile_inverse_step<4>(li[4], rd[4], w);
+    tile_inverse_step<5>(li[5], rd[5], w);
+    tile_inverse_step<6>(li[6], rd[6], w);
+    tile_inverse_step<7>(li[7], rd[7], w);
+    tile_inverse_step<8>(li[8], rd[8], w);
+    tile_inverse_step<9>(li[9], rd[9], w);
+    tile_inverse_step<10>(li[10], rd[10], w);
+    tile_inverse_step<11>(li[11], rd[11], w);
+    tile_inverse_step<12>(li[12], rd[12], w);
+    tile_inverse_step<13>(li[13], rd[13], w);
+    tile_inverse_step<14>(li[14], rd[14], w);
+    tile_inverse_step<15>(li[15], rd[15], w);
 }
 
 // Tile (bi, bj) of the diagonal block held in slot q of wave w: slots 0..w
