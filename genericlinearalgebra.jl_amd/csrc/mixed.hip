@@ -64,7 +64,7 @@ struct LpP {
     int64_t ldb;
     float *C;
     int64_t ldc;
-    float alpha;   // ±1: C <- C + alpha A Bᵀ
+    float alpha;   // C <- C + alpha A Bᵀ
     int mode;      // 0 all, 1 lower (row >= col)
     int tiles_m, tiles_n, total, per_xcd;
 };
@@ -114,19 +114,11 @@ __global__ __launch_bounds__(kLpT, 2) void lp_gemm_nt_kernel(LpP p)
     const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
     const int wm = wave & 1, wn = wave >> 1;
 
-    // accumulators start at alpha·C (alpha = ±1, exact)
     f4 acc[4][4];
 #pragma unroll
     for (int i = 0; i < 4; ++i)
 #pragma unroll
-        for (int j = 0; j < 4; ++j) {
-            acc[i][j] = f4{0.f, 0.f, 0.f, 0.f};
-            const int64_t row = m0 + wm * 64 + i * 16 + 4 * (lane >> 4);
-            const int64_t col = n0 + wn * 64 + j * 16 + (lane & 15);
-#pragma unroll
-            for (int r = 0; r < 4; ++r)
-                if (row + r < p.M && col < p.N) acc[i][j][r] = p.alpha * p.C[row + r + col * p.ldc];
-        }
+        for (int j = 0; j < 4; ++j) acc[i][j] = f4{0.f, 0.f, 0.f, 0.f};
 
     // staging: thread t holds row m = t % 128 of the slab, k = 16·(t / 128)
     // .. +15: each load instruction reads 64 consecutive rows (coalesced);
@@ -199,7 +191,10 @@ __global__ __launch_bounds__(kLpT, 2) void lp_gemm_nt_kernel(LpP p)
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
                 const int64_t gr = row + r;
-                if (gr < p.M && (p.mode == 0 || gr >= col)) p.C[gr + col * p.ldc] = p.alpha * acc[i][j][r];
+                if (gr < p.M && (p.mode == 0 || gr >= col)) {
+                    float *c = p.C + gr + col * p.ldc;
+                    *c = *c + p.alpha * acc[i][j][r];
+                }
             }
         }
 }
