@@ -449,7 +449,7 @@ def batched_tuning(dtype) -> dict:
     (gla/batched_tuning.json, written by gla.autotune on an MI355X) or
     defaults."""
     key = "float64" if dtype in (torch.float64, np.float64, "float64") else "float32"
-    cfg = {"wave_max": 32, "grid_mult": 2}
+    cfg = {"wave_max": 256, "grid_mult": 2}
     if _os.path.exists(_TUNING):
         with open(_TUNING) as f:
             cfg.update(_json.load(f).get(key, {}))

@@ -2,8 +2,9 @@
 
 Times one persistent launch over a synthetic batch (default: 10,000 SPD
 matrices, n uniform in [8, 256], A = BᵀB + n·I) for every combination of
-  wave_max  — largest n handled by one wave (below: packed four per
-              workgroup); 0 sends every matrix to the workgroup path,
+  wave_max  — largest n handled by one wave (four matrices per workgroup);
+              larger ones take a whole workgroup; 0 sends every matrix to the
+              workgroup path, 256 none,
   grid_mult — persistent workgroups per CU,
 per element type, and writes the fastest to gla/batched_tuning.json, which
 gla.BatchPlan reads.
@@ -25,8 +26,8 @@ if os.path.dirname(HERE) not in sys.path:
 
 import gla  # noqa: E402
 
-WAVE_MAX = (0, 8, 16, 24, 32)
-GRID_MULT = (1, 2, 4, 8)
+WAVE_MAX = (0, 32, 64, 128, 192, 256)
+GRID_MULT = (1, 2, 4)
 
 
 def make_batch(count: int, dtype, seed: int = 0):

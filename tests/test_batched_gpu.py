@@ -62,7 +62,7 @@ EDGE = [1, 2, 7, 8, 16, 31, 32, 33, 63, 64, 65, 96, 127, 128, 129, 160, 200, 255
 
 
 @pytest.mark.parametrize("dtype", [np.float64, np.float32])
-@pytest.mark.parametrize("wave_max", [0, 16, 32])
+@pytest.mark.parametrize("wave_max", [0, 32, 128, 256])
 def test_uneven_sizes_vs_oracle(dtype, wave_max):
     gla = _gla()
     rng = np.random.default_rng(wave_max + (1 if dtype == np.float32 else 0))
