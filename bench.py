@@ -330,7 +330,7 @@ def main():
     qr_bwd = ((QRx - M0 @ x).norm() / (M0 @ x).norm()).item()
     del R, QRx
 
-    # roofline of the dominant kernel, the QR panel (qr_panel_kernel: the
+    # roofline of the dominant kernel, the QR panel (qr_panel2_kernel: the
     # largest share of the step's kernel time): one launch on a fresh
     # 8192×128 panel (qrUnblocked! of the panel is exactly that launch),
     # algorithmic bytes 16·m·nb (read + write of the panel)
@@ -380,7 +380,7 @@ def main():
                "bar_10_n_eps": 10 * M_QR * eps, "residual_QRx_vs_Mx": qr_bwd},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": 8000.0, "unit": "GB/s",
                      "frac": achieved / 8000.0, "traffic": panel_traffic(),
-                     "kernel": "qr_panel_kernel<double,128,16> (panel 8192x128, 64 workgroups)",
+                     "kernel": "qr_panel2_kernel<double,128,2,1024> (panel 8192x128, 64 workgroups)",
                      "bytes_per_launch": panel_bytes, "us_per_launch": t_panel * 1e6},
         "roofline_gemm": {"bound": "mfma", "achieved": syrk_tf, "peak": FP64_PEAK_TFLOPS,
                           "unit": "TFLOP/s", "frac": syrk_tf / FP64_PEAK_TFLOPS,

@@ -59,6 +59,48 @@ __device__ __forceinline__ void load8_coherent(const float *const p[8], float v[
         : "memory");
 }
 
+// load8_coherent plus a ninth agent-coherent load in the same round trip.
+__device__ __forceinline__ void load9_coherent(const double *const p[8], double v[8],
+                                               const double *p9, double &v9)
+{
+    asm volatile(
+        "global_load_dwordx2 %0, %9, off sc1\n\t"
+        "global_load_dwordx2 %1, %10, off sc1\n\t"
+        "global_load_dwordx2 %2, %11, off sc1\n\t"
+        "global_load_dwordx2 %3, %12, off sc1\n\t"
+        "global_load_dwordx2 %4, %13, off sc1\n\t"
+        "global_load_dwordx2 %5, %14, off sc1\n\t"
+        "global_load_dwordx2 %6, %15, off sc1\n\t"
+        "global_load_dwordx2 %7, %16, off sc1\n\t"
+        "global_load_dwordx2 %8, %17, off sc1\n\t"
+        "s_waitcnt vmcnt(0)"
+        : "=&v"(v[0]), "=&v"(v[1]), "=&v"(v[2]), "=&v"(v[3]), "=&v"(v[4]), "=&v"(v[5]),
+          "=&v"(v[6]), "=&v"(v[7]), "=&v"(v9)
+        : "v"(p[0]), "v"(p[1]), "v"(p[2]), "v"(p[3]), "v"(p[4]), "v"(p[5]), "v"(p[6]), "v"(p[7]),
+          "v"(p9)
+        : "memory");
+}
+__device__ __forceinline__ void load9_coherent(const float *const p[8], float v[8], const float *p9,
+                                               float &v9)
+{
+    asm volatile(
+        "global_load_dword %0, %9, off sc1\n\t"
+        "global_load_dword %1, %10, off sc1\n\t"
+        "global_load_dword %2, %11, off sc1\n\t"
+        "global_load_dword %3, %12, off sc1\n\t"
+        "global_load_dword %4, %13, off sc1\n\t"
+        "global_load_dword %5, %14, off sc1\n\t"
+        "global_load_dword %6, %15, off sc1\n\t"
+        "global_load_dword %7, %16, off sc1\n\t"
+        "global_load_dword %8, %17, off sc1\n\t"
+        "s_waitcnt vmcnt(0)"
+        : "=&v"(v[0]), "=&v"(v[1]), "=&v"(v[2]), "=&v"(v[3]), "=&v"(v[4]), "=&v"(v[5]),
+          "=&v"(v[6]), "=&v"(v[7]), "=&v"(v9)
+        : "v"(p[0]), "v"(p[1]), "v"(p[2]), "v"(p[3]), "v"(p[4]), "v"(p[5]), "v"(p[6]), "v"(p[7]),
+          "v"(p9)
+        : "memory");
+}
+
 // Waits until this thread's outstanding vector memory operations, stores
 // included, have completed (s_waitcnt vmcnt(0)).
 __device__ __forceinline__ void stores_done() { __builtin_amdgcn_s_waitcnt(0x0F70); }

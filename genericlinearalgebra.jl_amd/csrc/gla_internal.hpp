@@ -59,6 +59,9 @@ struct Handle {
     int mixed_block = 512;
     // QR panel kernel: 1 LDS-reduction form, 2 wave-butterfly form.
     int qr_panel = 2;
+    // Width of the sub-panels a QR block column is factored in (0: the whole
+    // block column in one panel launch); 32 or 64.
+    int qr_inner = 64;
 };
 
 // Ensure the workspace holds at least `bytes` (grows with hipMalloc, which
@@ -107,15 +110,20 @@ int diag_block_inverses(hipStream_t s, int64_t n, const T *L, int64_t ldl, const
 
 // Multi-right-hand-side lower triangular solve (trsv.hip): Y = L⁻¹ R for up
 // to 16 right-hand sides held row-major (element (i, k) at [16 i + k]),
-// blocked by 128 rows, one launch.  Dt from diag_block_inverses.  rev:
+// blocked by 128 rows, one launch.  Dt from diag_block_inverses, Mt from
+// diag_sub (npad × 128, npad = n rounded up to 128).  rev:
 // row i of the solve reads R row n−1−i and adds into X row n−1−i (the
 // backward solve Lᵀ z = y run as a lower solve with the reversed matrix
 // J Lᵀ J).  If X is given, X[·, k] += Y[·, k] for k < nrhs (FP64,
 // column-major).  Y is scratch of the same layout; flags: one int per
 // 128-row block, values below `epoch`; err is raised if a wait times out.
 template <typename T>
-int trsv16(Handle *h, hipStream_t s, int64_t n, const T *L, int64_t ldl, const T *Dt, const T *R,
-           T *Y, int rev, double *X, int64_t ldx, int nrhs, int *flags, int epoch, int *err);
+int trsv16(Handle *h, hipStream_t s, int64_t n, const T *L, int64_t ldl, const T *Dt, const T *Mt,
+           const T *R, T *Y, int rev, double *X, int64_t ldx, int nrhs, int *flags, int epoch,
+           int *err);
+// Mt (npad × 128) = L_bb⁻¹ L_{b,b−1} stacked over the 128-row blocks b ≥ 1.
+template <typename T>
+int diag_sub(hipStream_t s, int64_t n, const T *L, int64_t ldl, const T *Dt, T *Mt);
 // R (row-major n×16, type T) = B − sym(A) X for the lower-stored FP64 A and
 // nrhs ≤ 16 columns of X, B (column-major); Xt: symv_scratch(n) doubles;
 // nrm[k] += ‖r_k‖², nrm[16 + k] += ‖x_k‖² (nrm zeroed by the caller).

@@ -56,63 +56,118 @@ struct LpMfma<_Float16> {
     }
 };
 
-struct LpP {
-    int64_t M, N, K;
-    const float *A;
-    int64_t lda;
-    const float *B;
-    int64_t ldb;
+// Low-precision product on packed operands.  A panel is rounded to TIN once
+// and stored row-major with K padded to kPkBK (lp_pack): row r holds its K
+// values contiguously, which is the MFMA operand layout (a lane reads eight
+// consecutive k of one row with one 16-byte load).  The trailing updates of
+// the factorisation read each panel row many times, so the rounding and the
+// FP32→TIN halving of the bytes are paid once per panel.
+constexpr int kPkT = 256, kPkBM = 128, kPkBK = 64, kPkLD = kPkBK + 8;
+
+inline int64_t lp_kpad(int64_t K) { return cdiv(std::max<int64_t>(K, 1), kPkBK) * kPkBK; }
+
+// dst[r·Kp + k] = TIN(src[r + k·lds]) for k < K, 0 for K ≤ k < Kp.
+template <typename TIN>
+__global__ void lp_pack_kernel(int64_t rows, int64_t K, int64_t Kp, const float *src, int64_t lds,
+                               TIN *dst)
+{
+    typedef typename LpMfma<TIN>::v8 v8;
+    const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= rows) return;
+    for (int64_t k0 = 8 * (int64_t)blockIdx.y; k0 < Kp; k0 += 8 * (int64_t)gridDim.y) {
+        v8 v;
+#pragma unroll
+        for (int e = 0; e < 8; ++e) v[e] = k0 + e < K ? (TIN)src[r + (k0 + e) * lds] : (TIN)0.f;
+        *reinterpret_cast<v8 *>(dst + r * Kp + k0) = v;
+    }
+}
+
+template <typename TIN>
+int lp_pack(hipStream_t s, int64_t rows, int64_t K, const float *src, int64_t lds, TIN *dst)
+{
+    if (rows <= 0) return GLA_OK;
+    const int64_t Kp = lp_kpad(K);
+    hipLaunchKernelGGL((lp_pack_kernel<TIN>), dim3((unsigned)cdiv(rows, 256),
+                                                   (unsigned)std::min<int64_t>(Kp / 8, 65535)),
+                       dim3(256), 0, s, rows, K, Kp, src, lds, dst);
+    GLA_LAUNCH_CHECK();
+    return GLA_OK;
+}
+
+struct LpPk {
+    int64_t M, N, Kp;
+    const void *A;   // M×Kp, TIN, row-major
+    const void *B;   // N×Kp, TIN, row-major
     float *C;
     int64_t ldc;
-    float alpha;   // C <- C + alpha A Bᵀ
-    int mode;      // 0 all, 1 lower (row >= col)
+    float alpha;     // C <- C + alpha A Bᵀ
+    int mode;        // 0 all, 1 lower (row >= col)
     int tiles_m, tiles_n, total, per_xcd;
 };
 
-constexpr int kLpT = 256, kLpBM = 128, kLpBK = 32, kLpLD = kLpBK + 8;
+// Tile order.  Tiles are grouped in kPkST×kPkST super-tiles (1024×1024 of
+// C), super-tiles numbered column by column (in mode 1 only those on or
+// below the diagonal), tiles column by column inside each.  Consecutive
+// tile numbers go to one XCD, whose ~64 resident workgroups then share one
+// super-tile's 1024 rows of A and of B in its L2.  In mode 1 the tiles of a
+// diagonal super-tile that lie wholly above the diagonal are numbered too
+// and exit at once.
+constexpr int kPkST = 8;
 
-__device__ __forceinline__ void lp_tile_range(int c, int tiles_m, int mode, int &first, int &cnt)
+__host__ __device__ inline int lp_st_first(int J, int mode) { return mode ? J : 0; }
+
+// number of tiles in super-tile column J
+__host__ __device__ inline int64_t lp_st_col_tiles(int J, int tiles_m, int tiles_n, int mode)
 {
-    if (mode == 0) {
-        first = 0;
-        cnt = tiles_m;
-        return;
-    }
-    const int64_t need = (int64_t)c * kLpBM - (kLpBM - 1);
-    first = need <= 0 ? 0 : (int)((need + kLpBM - 1) / kLpBM);
-    cnt = tiles_m - first;
-    if (cnt < 0) cnt = 0;
+    const int wd = min(kPkST, tiles_n - J * kPkST);
+    const int rows = tiles_m - lp_st_first(J, mode) * kPkST;
+    return rows > 0 ? (int64_t)wd * rows : 0;
 }
 
-// C (M×N, FP32) += alpha · A Bᵀ with A (M×K) and B (N×K) column-major FP32,
-// both rounded to TIN as they are staged into LDS; FP32 accumulation.
+__device__ __forceinline__ void lp_tile_of(int64_t tid, int tiles_m, int tiles_n, int mode, int &tm,
+                                           int &tn)
+{
+    const int SN = (tiles_n + kPkST - 1) / kPkST;
+    int J = 0;
+    for (; J < SN; ++J) {
+        const int64_t c = lp_st_col_tiles(J, tiles_m, tiles_n, mode);
+        if (tid < c) break;
+        tid -= c;
+    }
+    const int wd = min(kPkST, tiles_n - J * kPkST);
+    const int I = lp_st_first(J, mode) + (int)(tid / ((int64_t)kPkST * wd));
+    const int inner = (int)(tid - (int64_t)(I - lp_st_first(J, mode)) * kPkST * wd);
+    const int h = min(kPkST, tiles_m - I * kPkST);
+    tn = J * kPkST + inner / h;
+    tm = I * kPkST + inner % h;
+}
+
+// C (M×N, FP32, column-major) += alpha · A Bᵀ on packed TIN operands, FP32
+// accumulation.  128×128 tiles, four waves of 64×64, K in steps of 64
+// staged through double-buffered LDS with the next step's loads in flight
+// during the MFMAs.  Tiles are numbered column by column (lower tiles only
+// in mode 1) and dealt to the XCDs in contiguous runs, so neighbouring
+// tiles share an L2.  The MFMA is issued as Cᵀ = B Aᵀ: a lane's results
+// then run down a column of C (16 consecutive rows per 16 lanes).
 template <typename TIN>
-__global__ __launch_bounds__(kLpT, 2) void lp_gemm_nt_kernel(LpP p)
+__global__ __launch_bounds__(kPkT, 2) void lp_gemm_pk_kernel(LpPk p)
 {
     typedef typename LpMfma<TIN>::v8 v8;
-    __shared__ __attribute__((aligned(16))) TIN As[2][kLpBM * kLpLD];
-    __shared__ __attribute__((aligned(16))) TIN Bs[2][kLpBM * kLpLD];
+    __shared__ __attribute__((aligned(16))) TIN As[2][kPkBM * kPkLD];
+    __shared__ __attribute__((aligned(16))) TIN Bs[2][kPkBM * kPkLD];
 
     const int b = blockIdx.x;
     const int tid = (b % 8) * p.per_xcd + b / 8;
     if (tid >= p.total) return;
-    int tm = 0, tn = 0;
-    {
-        int rem = tid;
-        for (int c = 0; c < p.tiles_n; ++c) {
-            int first, cnt;
-            lp_tile_range(c, p.tiles_m, p.mode, first, cnt);
-            if (rem < cnt) {
-                tm = first + rem;
-                tn = c;
-                break;
-            }
-            rem -= cnt;
-        }
-    }
-    const int64_t m0 = (int64_t)tm * kLpBM, n0 = (int64_t)tn * kLpBM;
+    int tm, tn;
+    lp_tile_of(tid, p.tiles_m, p.tiles_n, p.mode, tm, tn);
+    if (p.mode == 1 && tm < tn) return;          // wholly above the diagonal
+    const int64_t m0 = (int64_t)tm * kPkBM, n0 = (int64_t)tn * kPkBM;
     const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
     const int wm = wave & 1, wn = wave >> 1;
+    const TIN *__restrict__ A = reinterpret_cast<const TIN *>(p.A);
+    const TIN *__restrict__ B = reinterpret_cast<const TIN *>(p.B);
+    const int64_t Kp = p.Kp;
 
     f4 acc[4][4];
 #pragma unroll
@@ -120,60 +175,58 @@ __global__ __launch_bounds__(kLpT, 2) void lp_gemm_nt_kernel(LpP p)
 #pragma unroll
         for (int j = 0; j < 4; ++j) acc[i][j] = f4{0.f, 0.f, 0.f, 0.f};
 
-    // staging: thread t holds row m = t % 128 of the slab, k = 16·(t / 128)
-    // .. +15: each load instruction reads 64 consecutive rows (coalesced);
-    // the 16 values are rounded to TIN and written as two 16-byte vectors in
-    // the [m][k] layout the MFMA operands are read from.
-    float ra[16], rb[16];
-    const int sm = t & 127, sk = 16 * (t >> 7);
-    auto gload = [&](const float *X, int64_t ldx, int64_t x0, int64_t xlim, int64_t k0, float *r) {
-        const int64_t gx = x0 + sm;
-        const bool okx = gx < xlim;
+    // staging: 16-byte chunk c = t + 256u (u < 4) of the 128×64 slab is row
+    // c / 8, k 8·(c % 8): eight threads read one row's 128 contiguous bytes
+    v8 ra[4], rb[4];
+    auto gload = [&](const TIN *X, int64_t x0, int64_t xlim, int64_t k0, v8 *r) {
 #pragma unroll
-        for (int e = 0; e < 16; ++e) {
-            const int64_t gk = k0 + sk + e;
-            r[e] = (okx && gk < p.K) ? X[gx + gk * ldx] : 0.f;
+        for (int u = 0; u < 4; ++u) {
+            const int c = t + kPkT * u, row = c >> 3, kc = c & 7;
+            const int64_t gx = x0 + row;
+            if (gx < xlim)
+                r[u] = *reinterpret_cast<const v8 *>(X + gx * Kp + k0 + 8 * kc);
+            else
+#pragma unroll
+                for (int e = 0; e < 8; ++e) r[u][e] = (TIN)0.f;
         }
     };
-    auto lstore = [&](TIN *S, const float *r) {
-        v8 lo, hi;
+    auto lstore = [&](TIN *S, const v8 *r) {
 #pragma unroll
-        for (int e = 0; e < 8; ++e) {
-            lo[e] = (TIN)r[e];
-            hi[e] = (TIN)r[8 + e];
+        for (int u = 0; u < 4; ++u) {
+            const int c = t + kPkT * u, row = c >> 3, kc = c & 7;
+            *reinterpret_cast<v8 *>(&S[row * kPkLD + 8 * kc]) = r[u];
         }
-        *reinterpret_cast<v8 *>(&S[sm * kLpLD + sk]) = lo;
-        *reinterpret_cast<v8 *>(&S[sm * kLpLD + sk + 8]) = hi;
     };
 
-    const int nk = (int)((p.K + kLpBK - 1) / kLpBK);
-    if (nk > 0) {
-        gload(p.A, p.lda, m0, p.M, 0, ra);
-        gload(p.B, p.ldb, n0, p.N, 0, rb);
-        lstore(As[0], ra);
-        lstore(Bs[0], rb);
-        __syncthreads();
-    }
+    const int nk = (int)(Kp / kPkBK);
+    gload(A, m0, p.M, 0, ra);
+    gload(B, n0, p.N, 0, rb);
+    lstore(As[0], ra);
+    lstore(Bs[0], rb);
+    __syncthreads();
     for (int kt = 0; kt < nk; ++kt) {
         const int cur = kt & 1;
         const bool more = kt + 1 < nk;
         if (more) {
-            gload(p.A, p.lda, m0, p.M, (int64_t)(kt + 1) * kLpBK, ra);
-            gload(p.B, p.ldb, n0, p.N, (int64_t)(kt + 1) * kLpBK, rb);
+            gload(A, m0, p.M, (int64_t)(kt + 1) * kPkBK, ra);
+            gload(B, n0, p.N, (int64_t)(kt + 1) * kPkBK, rb);
         }
-        v8 a[4], bb[4];
 #pragma unroll
-        for (int i = 0; i < 4; ++i)
-            a[i] = *reinterpret_cast<const v8 *>(&As[cur][(wm * 64 + i * 16 + (lane & 15)) * kLpLD +
-                                                          8 * (lane >> 4)]);
+        for (int kk = 0; kk < kPkBK / 32; ++kk) {
+            v8 a[4], bb[4];
 #pragma unroll
-        for (int j = 0; j < 4; ++j)
-            bb[j] = *reinterpret_cast<const v8 *>(&Bs[cur][(wn * 64 + j * 16 + (lane & 15)) * kLpLD +
-                                                           8 * (lane >> 4)]);
+            for (int i = 0; i < 4; ++i)
+                a[i] = *reinterpret_cast<const v8 *>(
+                    &As[cur][(wm * 64 + i * 16 + (lane & 15)) * kPkLD + 32 * kk + 8 * (lane >> 4)]);
 #pragma unroll
-        for (int i = 0; i < 4; ++i)
+            for (int j = 0; j < 4; ++j)
+                bb[j] = *reinterpret_cast<const v8 *>(
+                    &Bs[cur][(wn * 64 + j * 16 + (lane & 15)) * kPkLD + 32 * kk + 8 * (lane >> 4)]);
 #pragma unroll
-            for (int j = 0; j < 4; ++j) acc[i][j] = LpMfma<TIN>::mma(a[i], bb[j], acc[i][j]);
+            for (int i = 0; i < 4; ++i)
+#pragma unroll
+                for (int j = 0; j < 4; ++j) acc[i][j] = LpMfma<TIN>::mma(bb[j], a[i], acc[i][j]);
+        }
         if (more) {
             lstore(As[cur ^ 1], ra);
             lstore(Bs[cur ^ 1], rb);
@@ -181,59 +234,49 @@ __global__ __launch_bounds__(kLpT, 2) void lp_gemm_nt_kernel(LpP p)
         __syncthreads();
     }
 
+    // acc[i][j][r] = C(m0 + 64wm + 16i + lane%16, n0 + 64wn + 16j + 4(lane/16) + r)
 #pragma unroll
-    for (int i = 0; i < 4; ++i)
+    for (int i = 0; i < 4; ++i) {
+        const int64_t gr = m0 + wm * 64 + i * 16 + (lane & 15);
+        if (gr >= p.M) continue;
 #pragma unroll
-        for (int j = 0; j < 4; ++j) {
-            const int64_t row = m0 + wm * 64 + i * 16 + 4 * (lane >> 4);
-            const int64_t col = n0 + wn * 64 + j * 16 + (lane & 15);
-            if (col >= p.N) continue;
+        for (int j = 0; j < 4; ++j)
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
-                const int64_t gr = row + r;
-                if (gr < p.M && (p.mode == 0 || gr >= col)) {
-                    float *c = p.C + gr + col * p.ldc;
+                const int64_t gc = n0 + wn * 64 + j * 16 + 4 * (lane >> 4) + r;
+                if (gc < p.N && (p.mode == 0 || gr >= gc)) {
+                    float *c = p.C + gr + gc * p.ldc;
                     *c = *c + p.alpha * acc[i][j][r];
                 }
             }
-        }
+    }
 }
 
+// C (M×N) += alpha · A Bᵀ with A, B packed by lp_pack (pitch lp_kpad(K)).
 template <typename TIN>
-int lp_gemm_nt(hipStream_t s, int mode, int64_t M, int64_t N, int64_t K, float alpha,
-               const float *A, int64_t lda, const float *B, int64_t ldb, float *C, int64_t ldc)
+int lp_gemm_pk(hipStream_t s, int mode, int64_t M, int64_t N, int64_t K, float alpha,
+               const TIN *A, const TIN *B, float *C, int64_t ldc)
 {
     if (M <= 0 || N <= 0) return GLA_OK;
-    LpP p{};
+    LpPk p{};
     p.M = M;
     p.N = N;
-    p.K = K;
+    p.Kp = lp_kpad(K);
     p.A = A;
-    p.lda = lda;
     p.B = B;
-    p.ldb = ldb;
     p.C = C;
     p.ldc = ldc;
     p.alpha = alpha;
     p.mode = mode;
-    p.tiles_m = (int)cdiv(M, kLpBM);
-    p.tiles_n = (int)cdiv(N, kLpBM);
+    p.tiles_m = (int)cdiv(M, kPkBM);
+    p.tiles_n = (int)cdiv(N, kPkBM);
     int64_t tot = 0;
-    for (int c = 0; c < p.tiles_n; ++c) {
-        int first, cnt;
-        if (mode == 0) {
-            cnt = p.tiles_m;
-        } else {
-            const int64_t need = (int64_t)c * kLpBM - (kLpBM - 1);
-            first = need <= 0 ? 0 : (int)((need + kLpBM - 1) / kLpBM);
-            cnt = std::max(0, p.tiles_m - first);
-        }
-        tot += cnt;
-    }
+    for (int J = 0; J < (int)cdiv(p.tiles_n, kPkST); ++J)
+        tot += lp_st_col_tiles(J, p.tiles_m, p.tiles_n, mode);
     p.total = (int)tot;
     if (p.total == 0) return GLA_OK;
     p.per_xcd = (int)cdiv(p.total, 8);
-    hipLaunchKernelGGL((lp_gemm_nt_kernel<TIN>), dim3(8 * p.per_xcd), dim3(kLpT), 0, s, p);
+    hipLaunchKernelGGL((lp_gemm_pk_kernel<TIN>), dim3(8 * p.per_xcd), dim3(kPkT), 0, s, p);
     GLA_LAUNCH_CHECK();
     return GLA_OK;
 }
@@ -311,18 +354,14 @@ __global__ __launch_bounds__(256) void sym_sumsq_cols(int64_t n, const double *A
 
 // ------------------------------------------------------- mixed factor --
 // Blocked right-looking Cholesky of W (FP32, lower) with FP64 diagonal
-// blocks and low-precision trailing updates, depth-1 look-ahead.
+// blocks and low-precision trailing updates (n ≤ the look-ahead block
+// width).  pk: room for the packed panel, n × lp_kpad(128) TIN.
 template <typename TIN>
 int chol_mixed(Handle *h, hipStream_t s, int64_t n, float *W, int64_t ldw, double *D64,
-               double *Linv64, float *Linv32, bool use_side = true)
+               double *Linv64, float *Linv32, TIN *pk)
 {
     const int nb = 128;
     const int64_t nsteps = cdiv(n, nb);
-    const bool overlap = use_side && nsteps > 2;
-    if (overlap) {
-        GLA_HIP_TRY(hipEventRecord(h->ev_fork, s));
-        GLA_HIP_TRY(hipStreamWaitEvent(h->side, h->ev_fork, 0));
-    }
     int64_t k = 0;
     for (int64_t step = 0; step < nsteps; ++step, k += nb) {
         const int kb = (int)std::min<int64_t>(nb, n - k);
@@ -342,37 +381,31 @@ int chol_mixed(Handle *h, hipStream_t s, int64_t n, float *W, int64_t ldw, doubl
         GLA_LAUNCH_CHECK();
         GLA_TRY(chol_solve_block<float>(s, kb, Wkk, ldw, Linv32, W21, ldw, below));
         if (below == 0) break;
-        const int kb2 = (int)std::min<int64_t>(nb, below);
+        const int64_t kb2 = std::min<int64_t>(nb, below);
         const int64_t rest = below - kb2;
-        if (overlap && rest > 0) GLA_HIP_TRY(hipEventRecord(h->ev_step[0], s));
-        if (overlap && step > 0) GLA_HIP_TRY(hipStreamWaitEvent(s, h->ev_step[1], 0));
-        GLA_TRY(lp_gemm_nt<TIN>(s, 1, below, kb2, kb, -1.f, W21, ldw, W21, ldw, W21 + kb * ldw, ldw));
-        if (rest > 0) {
-            hipStream_t us = overlap ? h->side : s;
-            if (overlap) GLA_HIP_TRY(hipStreamWaitEvent(h->side, h->ev_step[0], 0));
-            float *X2 = W21 + kb2;
-            GLA_TRY(lp_gemm_nt<TIN>(us, 1, rest, rest, kb, -1.f, X2, ldw, X2, ldw,
-                                    X2 + (kb + kb2) * ldw, ldw));
-            if (overlap) GLA_HIP_TRY(hipEventRecord(h->ev_step[1], h->side));
-        }
-    }
-    if (overlap) {
-        GLA_HIP_TRY(hipEventRecord(h->ev_join, h->side));
-        GLA_HIP_TRY(hipStreamWaitEvent(s, h->ev_join, 0));
+        const int64_t Kp = lp_kpad(kb);
+        GLA_TRY(lp_pack<TIN>(s, below, kb, W21, ldw, pk));
+        GLA_TRY(lp_gemm_pk<TIN>(s, 1, below, kb2, kb, -1.f, pk, pk, W21 + kb * ldw, ldw));
+        if (rest > 0)
+            GLA_TRY(lp_gemm_pk<TIN>(s, 1, rest, rest, kb, -1.f, pk + kb2 * Kp, pk + kb2 * Kp,
+                                    W21 + kb2 + (kb + kb2) * ldw, ldw));
     }
     return GLA_OK;
 }
 
-// Wide-block form (NB = 512): the NB×NB diagonal block by the 128-wide
-// driver above on the caller's stream, the panel below it by the FP32
-// recursive solve, then the look-ahead trailing updates of cholBlocked!'s
-// loop (src/cholesky.jl:17-35) with K = NB on the low-precision MFMA GEMM:
-// the next block column on the caller's stream, the rest on the side stream.
+// Wide-block form (block width NB): the NB×NB diagonal block by the
+// 128-wide driver above on the caller's stream, the panel below it by the
+// FP32 recursive solve, then the look-ahead trailing updates of
+// cholBlocked!'s loop (src/cholesky.jl:17-35) with K = NB on the
+// low-precision MFMA GEMM: the next block column on the caller's stream,
+// the rest on the side stream.  The panel is packed once per step into
+// pk[step % 2] (each n × lp_kpad(NB) TIN): the side stream may still read
+// the previous step's buffer while the caller's stream packs the next.
 template <typename TIN>
 int chol_mixed_la(Handle *h, hipStream_t s, int64_t n, float *W, int64_t ldw, double *D64,
-                  double *Linv64, float *Linv32, float *Linv, int NB)
+                  double *Linv64, float *Linv32, float *Linv, int NB, TIN *pk0, TIN *pk1)
 {
-    if (n <= NB) return chol_mixed<TIN>(h, s, n, W, ldw, D64, Linv64, Linv32, false);
+    if (n <= NB) return chol_mixed<TIN>(h, s, n, W, ldw, D64, Linv64, Linv32, pk0);
     const int64_t nsteps = cdiv(n, NB);
     const bool overlap = nsteps > 2;
     if (overlap) {
@@ -385,21 +418,23 @@ int chol_mixed_la(Handle *h, hipStream_t s, int64_t n, float *W, int64_t ldw, do
         const int64_t below = n - k - kb;
         float *Wkk = W + k + k * ldw;
         float *W21 = Wkk + kb;
-        GLA_TRY(chol_mixed<TIN>(h, s, kb, Wkk, ldw, D64, Linv64, Linv32, false));
+        TIN *pk = (step & 1) ? pk1 : pk0;
+        GLA_TRY(chol_mixed<TIN>(h, s, kb, Wkk, ldw, D64, Linv64, Linv32, pk));
         if (below == 0) break;
         GLA_TRY(tile_inverses<float>(s, kb, Wkk, ldw, Linv));
         GLA_TRY(trsm_rlt<float>(s, below, kb, Wkk, ldw, Linv, W21, ldw));
         const int64_t kb2 = std::min<int64_t>(NB, below);
         const int64_t rest = below - kb2;
+        const int64_t Kp = lp_kpad(kb);
+        GLA_TRY(lp_pack<TIN>(s, below, kb, W21, ldw, pk));
         if (overlap && rest > 0) GLA_HIP_TRY(hipEventRecord(h->ev_step[0], s));
         if (overlap && step > 0) GLA_HIP_TRY(hipStreamWaitEvent(s, h->ev_step[1], 0));
-        GLA_TRY(lp_gemm_nt<TIN>(s, 1, below, kb2, kb, -1.f, W21, ldw, W21, ldw, W21 + kb * ldw, ldw));
+        GLA_TRY(lp_gemm_pk<TIN>(s, 1, below, kb2, kb, -1.f, pk, pk, W21 + kb * ldw, ldw));
         if (rest > 0) {
             hipStream_t us = overlap ? h->side : s;
             if (overlap) GLA_HIP_TRY(hipStreamWaitEvent(h->side, h->ev_step[0], 0));
-            float *X2 = W21 + kb2;
-            GLA_TRY(lp_gemm_nt<TIN>(us, 1, rest, rest, kb, -1.f, X2, ldw, X2, ldw,
-                                    X2 + (kb + kb2) * ldw, ldw));
+            GLA_TRY(lp_gemm_pk<TIN>(us, 1, rest, rest, kb, -1.f, pk + kb2 * Kp, pk + kb2 * Kp,
+                                    W21 + kb2 + (kb + kb2) * ldw, ldw));
             if (overlap) GLA_HIP_TRY(hipEventRecord(h->ev_step[1], h->side));
         }
     }
@@ -437,24 +472,26 @@ double backward_error(const double *rr, const double *xx, const double *bb, int6
 // Workspace of one mixed solve (element type T of the factor).
 template <typename T>
 struct MixedWS {
-    T *W, *Lr, *Linv, *Linvr, *Dt, *Dtr, *Rt, *Yt, *Zt;
+    T *W, *Lr, *Linv, *Linvr, *Dt, *Dtr, *Mt, *Mtr, *Rt, *Yt, *Zt;
     double *Xt, *nrm, *D64, *Linv64;
     float *Linv32;
     int *flags, *err;
+    void *pk0, *pk1;   // packed low-precision panels, pk_bytes each
     int64_t nblk, npad, nchunk;
 
-    static size_t bytes(int64_t n, int64_t nrhs)
+    // pk_bytes: one packed panel of the BF16/FP16 factorisation (0 otherwise)
+    static size_t bytes(int64_t n, int64_t nrhs, size_t pk_bytes)
     {
         const int64_t nblk = cdiv(n, 128), npad = nblk * 128, nch = cdiv(nrhs, 16);
         const size_t es = sizeof(T), r = 255;
         auto al = [&](size_t b) { return (b + r) / 256 * 256; };
         return 2 * al(es * (size_t)n * n) + 2 * al(es * tile_inverse_elems(n)) +
-               2 * al(es * (size_t)npad * 128) + al(es * (size_t)npad * 16 * nch) +
+               4 * al(es * (size_t)npad * 128) + al(es * (size_t)npad * 16 * nch) +
                2 * al(es * (size_t)npad * 16) + al(8 * symv_scratch(n)) +
                al(8 * (size_t)(32 * nch + n + 64)) + al(8 * 128 * 128) + al(8 * 8 * 256) +
-               al(4 * 8 * 256) + al(4 * (size_t)(nblk + 64)) + 4096;
+               al(4 * 8 * 256) + al(4 * (size_t)(nblk + 64)) + 2 * al(pk_bytes) + 4096;
     }
-    void carve(void *base, int64_t n, int64_t nrhs)
+    void carve(void *base, int64_t n, int64_t nrhs, size_t pk_bytes)
     {
         nblk = cdiv(n, 128);
         npad = nblk * 128;
@@ -472,6 +509,8 @@ struct MixedWS {
         Linvr = reinterpret_cast<T *>(take(es * tile_inverse_elems(n)));
         Dt = reinterpret_cast<T *>(take(es * (size_t)npad * 128));
         Dtr = reinterpret_cast<T *>(take(es * (size_t)npad * 128));
+        Mt = reinterpret_cast<T *>(take(es * (size_t)npad * 128));
+        Mtr = reinterpret_cast<T *>(take(es * (size_t)npad * 128));
         Rt = reinterpret_cast<T *>(take(es * (size_t)npad * 16 * nchunk));
         Yt = reinterpret_cast<T *>(take(es * (size_t)npad * 16));
         Zt = reinterpret_cast<T *>(take(es * (size_t)npad * 16));
@@ -482,8 +521,16 @@ struct MixedWS {
         Linv32 = reinterpret_cast<float *>(take(4 * 8 * 256));
         flags = reinterpret_cast<int *>(take(4 * (size_t)(nblk + 64)));
         err = flags + nblk;
+        pk0 = pk_bytes ? take(pk_bytes) : nullptr;
+        pk1 = pk_bytes ? take(pk_bytes) : nullptr;
     }
 };
+
+// Bytes of one packed panel for precision prec at block width NB.
+inline size_t pk_bytes_of(char prec, int64_t n, int NB)
+{
+    return (prec == 'b' || prec == 'h') ? 2 * (size_t)n * (size_t)lp_kpad(NB) : 0;
+}
 
 // Solve structures of the factor in ws.W: tile inverses, diagonal-block
 // inverses, and the same for the reversed Lr = J Lᵀ J (backward solve).
@@ -492,11 +539,13 @@ int prepare_solves(hipStream_t s, int64_t n, MixedWS<T> &ws)
 {
     GLA_TRY(tile_inverses<T>(s, n, ws.W, n, ws.Linv));
     GLA_TRY(diag_block_inverses<T>(s, n, ws.W, n, ws.Linv, ws.Dt));
+    GLA_TRY(diag_sub<T>(s, n, ws.W, n, ws.Dt, ws.Mt));
     hipLaunchKernelGGL((reverse_transpose_lower<T>), g2(n, n), dim3(256), 0, s, n, ws.W, n, ws.Lr,
                        n);
     GLA_LAUNCH_CHECK();
     GLA_TRY(tile_inverses<T>(s, n, ws.Lr, n, ws.Linvr));
-    return diag_block_inverses<T>(s, n, ws.Lr, n, ws.Linvr, ws.Dtr);
+    GLA_TRY(diag_block_inverses<T>(s, n, ws.Lr, n, ws.Linvr, ws.Dtr));
+    return diag_sub<T>(s, n, ws.Lr, n, ws.Dtr, ws.Mtr);
 }
 
 // Refinement outcome.
@@ -560,10 +609,10 @@ int refine(Handle *h, hipStream_t s, int64_t n, const double *A, int64_t lda, in
         // solve with Lr on reversed rows, adding into X
         for (int64_t c = 0; c < nch; ++c) {
             const int nr = (int)std::min<int64_t>(16, nrhs - 16 * c);
-            GLA_TRY(trsv16<T>(h, s, n, ws.W, n, ws.Dt, ws.Rt + c * ws.npad * 16, ws.Yt, 0, nullptr,
-                              0, nr, ws.flags, ++epoch, ws.err));
-            GLA_TRY(trsv16<T>(h, s, n, ws.Lr, n, ws.Dtr, ws.Yt, ws.Zt, 1, X + 16 * c * ldx, ldx, nr,
-                              ws.flags, ++epoch, ws.err));
+            GLA_TRY(trsv16<T>(h, s, n, ws.W, n, ws.Dt, ws.Mt, ws.Rt + c * ws.npad * 16, ws.Yt, 0,
+                              nullptr, 0, nr, ws.flags, ++epoch, ws.err));
+            GLA_TRY(trsv16<T>(h, s, n, ws.Lr, n, ws.Dtr, ws.Mtr, ws.Yt, ws.Zt, 1, X + 16 * c * ldx,
+                              ldx, nr, ws.flags, ++epoch, ws.err));
         }
     }
     *iters = corr;
@@ -573,9 +622,10 @@ int refine(Handle *h, hipStream_t s, int64_t n, const double *A, int64_t lda, in
 
 }  // namespace
 
-static size_t mixed_ws_bytes(char prec, int64_t n, int64_t nrhs)
+static size_t mixed_ws_bytes(char prec, int64_t n, int64_t nrhs, int NB)
 {
-    return prec == 'd' ? MixedWS<double>::bytes(n, nrhs) : MixedWS<float>::bytes(n, nrhs);
+    return prec == 'd' ? MixedWS<double>::bytes(n, nrhs, 0)
+                       : MixedWS<float>::bytes(n, nrhs, pk_bytes_of(prec, n, NB));
 }
 
 // One attempt at precision `prec`: factor, then refine.  *info > 0 when the
@@ -586,7 +636,7 @@ int mixed_attempt(Handle *h, hipStream_t s, char prec, int64_t n, const double *
                   int max_iter, double anorm, int *iters, double *berr, int64_t *info, int *outcome)
 {
     MixedWS<T> ws;
-    ws.carve(h->ws, n, nrhs);
+    ws.carve(h->ws, n, nrhs, pk_bytes_of(prec, n, h->mixed_block));
     hipLaunchKernelGGL(fill_int, dim3((unsigned)cdiv(ws.nblk + 1, 256)), dim3(256), 0, s,
                        ws.nblk + 1, ws.flags, 0);
     GLA_LAUNCH_CHECK();
@@ -603,10 +653,12 @@ int mixed_attempt(Handle *h, hipStream_t s, char prec, int64_t n, const double *
             GLA_TRY(chol_device<float>(h, s, n, ws.W, n, 2, 1, ws.Linv));
         else if (prec == 'b')
             GLA_TRY(chol_mixed_la<__bf16>(h, s, n, ws.W, n, ws.D64, ws.Linv64, ws.Linv32, ws.Linv,
-                                           h->mixed_block));
+                                           h->mixed_block, reinterpret_cast<__bf16 *>(ws.pk0),
+                                           reinterpret_cast<__bf16 *>(ws.pk1)));
         else
             GLA_TRY(chol_mixed_la<_Float16>(h, s, n, ws.W, n, ws.D64, ws.Linv64, ws.Linv32, ws.Linv,
-                                             h->mixed_block));
+                                             h->mixed_block, reinterpret_cast<_Float16 *>(ws.pk0),
+                                             reinterpret_cast<_Float16 *>(ws.pk1)));
     }
     int hinfo = 0;
     GLA_HIP_TRY(hipMemcpyAsync(&hinfo, h->dinfo, sizeof(int), hipMemcpyDeviceToHost, s));
@@ -633,7 +685,8 @@ int mixed_solve(Handle *h, hipStream_t s, char prec, int64_t n, const double *A,
     const char *ladder = prec == 'a' ? ladder_a : &prec;
     const int nl = prec == 'a' ? 3 : 1;
     size_t need = 0;
-    for (int i = 0; i < nl; ++i) need = std::max(need, mixed_ws_bytes(ladder[i], n, nrhs));
+    for (int i = 0; i < nl; ++i)
+        need = std::max(need, mixed_ws_bytes(ladder[i], n, nrhs, h->mixed_block));
     // ‖A‖_F from the lower triangle: column sums in scratch past the layouts
     GLA_TRY(ensure_workspace(h, need + 8 * (size_t)(n + 64)));
     double *cs = reinterpret_cast<double *>(reinterpret_cast<char *>(h->ws) + need);
@@ -660,12 +713,29 @@ int mixed_solve(Handle *h, hipStream_t s, char prec, int64_t n, const double *A,
     return GLA_OK;
 }
 
-int lp_gemm_entry(char tin, int64_t m, int64_t n, int64_t k, float alpha, const float *A,
-                  int64_t lda, const float *B, int64_t ldb, float *C, int64_t ldc, int mode,
+template <typename TIN>
+int lp_gemm_packed_entry(Handle *h, int64_t m, int64_t n, int64_t k, float alpha, const float *A,
+                         int64_t lda, const float *B, int64_t ldb, float *C, int64_t ldc,
+                         hipStream_t s)
+{
+    if (m <= 0 || n <= 0) return GLA_OK;
+    const int64_t Kp = lp_kpad(k);
+    const size_t pa = ((size_t)m * Kp * sizeof(TIN) + 255) / 256 * 256;
+    GLA_TRY(ensure_workspace(h, pa + (size_t)n * Kp * sizeof(TIN)));
+    TIN *Ap = reinterpret_cast<TIN *>(h->ws);
+    TIN *Bp = reinterpret_cast<TIN *>(reinterpret_cast<char *>(h->ws) + pa);
+    GLA_TRY(lp_pack<TIN>(s, m, k, A, lda, Ap));
+    GLA_TRY(lp_pack<TIN>(s, n, k, B, ldb, Bp));
+    return lp_gemm_pk<TIN>(s, 0, m, n, k, alpha, Ap, Bp, C, ldc);
+}
+
+int lp_gemm_entry(Handle *h, char tin, int64_t m, int64_t n, int64_t k, float alpha,
+                  const float *A, int64_t lda, const float *B, int64_t ldb, float *C, int64_t ldc,
                   hipStream_t s)
 {
-    if (tin == 'b') return lp_gemm_nt<__bf16>(s, mode, m, n, k, alpha, A, lda, B, ldb, C, ldc);
-    return lp_gemm_nt<_Float16>(s, mode, m, n, k, alpha, A, lda, B, ldb, C, ldc);
+    if (tin == 'b')
+        return lp_gemm_packed_entry<__bf16>(h, m, n, k, alpha, A, lda, B, ldb, C, ldc, s);
+    return lp_gemm_packed_entry<_Float16>(h, m, n, k, alpha, A, lda, B, ldb, C, ldc, s);
 }
 
 }  // namespace gla
@@ -713,9 +783,14 @@ int gla_gemm_lowp_nt(gla_handle_t hh, char tin, int64_t m, int64_t n, int64_t k,
     auto *h = reinterpret_cast<gla::Handle *>(hh);
     if (!h) return -1;
     if (tin != 'b' && tin != 'h') return -2;
-    if (alpha != 1.f && alpha != -1.f) { gla::set_error("alpha must be +1 or -1"); return -6; }
+    if (m < 0) return -3;
+    if (n < 0) return -4;
+    if (k < 0) return -5;
+    if (lda < std::max<int64_t>(1, m)) return -8;
+    if (ldb < std::max<int64_t>(1, n)) return -10;
+    if (ldc < std::max<int64_t>(1, m)) return -12;
     GLA_HIP_TRY(hipSetDevice(h->device));
-    return gla::lp_gemm_entry(tin, m, n, k, alpha, A, lda, B, ldb, C, ldc, 0,
+    return gla::lp_gemm_entry(h, tin, m, n, k, alpha, A, lda, B, ldb, C, ldc,
                               reinterpret_cast<hipStream_t>(stream));
 }
 

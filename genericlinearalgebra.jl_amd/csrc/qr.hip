@@ -259,7 +259,8 @@ __global__ __launch_bounds__(NTH) void qr_panel2_kernel(QrPanelP<T> p)
             if (wg == 0 && j > 0) t_column<T, NB, NTH>(Ts, gsh, tpart, j - 1, tau_prev);
             grid_wait(p.bar, p.bar_base + (unsigned)(j + 1) * (unsigned)P, p.err);
             // (3) totals in a fixed order: H sub-sums per column over the
-            // partials p ≡ h (mod H), eight loads in flight, then the H in order
+            // partials p ≡ h (mod H), eight loads in flight, then the H in
+            // order; row j travels in the first batch's round trip
             {
                 const int e = t % NB, h = t / NB;
                 T s = T(0);
@@ -271,13 +272,18 @@ __global__ __launch_bounds__(NTH) void qr_panel2_kernel(QrPanelP<T> p)
                         const int pp = pp0 + u * H;
                         ptr[u] = part + (size_t)(pp < P ? pp : h) * NB + e;
                     }
-                    load8_coherent(ptr, v);
+                    if (pp0 == h) {
+                        T rv;
+                        load9_coherent(ptr, v, part + (size_t)P * NB + e, rv);
+                        if (t < NB) rowj[t] = rv;
+                    } else {
+                        load8_coherent(ptr, v);
+                    }
 #pragma unroll
                     for (int u = 0; u < 8; ++u)
                         if (pp0 + u * H < P) s += v[u];
                 }
                 red[h][e] = s;
-                if (t < NB) rowj[t] = load_coherent(part + (size_t)P * NB + t);
                 __syncthreads();
                 if (t < NB) {
                     T s2 = T(0);
@@ -302,14 +308,16 @@ __global__ __launch_bounds__(NTH) void qr_panel2_kernel(QrPanelP<T> p)
         }
         __syncthreads();
 
-        // (4) the reflector of column j
+        // (4) the reflector of column j; the divisions by ξ of reflector!
+        // (x[i] /= ξ1) are taken as products with 1/ξ, one rounding apart
         const T xj = rowj[j];
         const T nrm = dsqrt(xj * xj + tot[j]);
         const bool live = nrm != T(0);
         const T nu = dcopysign(nrm, xj);
         const T xi = xj + nu;
         const T tauj = live ? xi / nu : T(0);
-        if (wg == 0 && t < j) gsh[t] = live ? rowj[t] + tot[t] / xi : T(0);
+        const T rxi = live ? T(1) / xi : T(0);
+        if (wg == 0 && t < j) gsh[t] = live ? rowj[t] + tot[t] * rxi : T(0);
 
         // (5) apply to the remaining columns; column j becomes (β, v)
         if (live) {
@@ -317,13 +325,13 @@ __global__ __launch_bounds__(NTH) void qr_panel2_kernel(QrPanelP<T> p)
 #pragma unroll
             for (int q = 0; q < RPL; ++q) {
                 const int64_t r = row0 + lane + 64 * q;
-                v[q] = r > j ? x[q] / xi : (r == j ? T(1) : T(0));   // x[i] /= ξ1
+                v[q] = r > j ? x[q] * rxi : (r == j ? T(1) : T(0));
             }
 #pragma unroll
             for (int c = 0; c < CPW; ++c) {
                 const int cg = c0 + c;
                 if (cg > j && cg < p.w) {
-                    const T coef = tauj * (rowj[cg] + tot[cg] / xi);
+                    const T coef = tauj * (rowj[cg] + tot[cg] * rxi);
 #pragma unroll
                     for (int q = 0; q < RPL; ++q) a[c][q] -= coef * v[q];
                 } else if (cg == j) {
@@ -711,6 +719,62 @@ __global__ void unit_upper_copy_kernel(int64_t n, const T *src, int64_t lds, T *
     dst[i + j * ldd] = i < j ? src[i + j * lds] : (i == j ? T(1) : T(0));
 }
 
+// Zero the strict upper triangle of the top k rows of Vw (m×k) and all of
+// T (k×k): the sub-panel launches write only their own diagonal blocks.
+template <typename T>
+__global__ void qr_sub_clear_kernel(int k, T *Vw, int64_t ldv, T *Tw, int64_t ldt)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y;
+    if (i >= k) return;
+    if (i < j) Vw[i + (int64_t)j * ldv] = T(0);
+    Tw[i + (int64_t)j * ldt] = T(0);
+}
+
+// Compact-WY T of a block column factored as sub-panels of width ib, from
+// the sub-panels' diagonal blocks T_ss (already in T) and G = VᵀV:
+//   T[0:s0, blk] = −T[0:s0, 0:s0] · (G[0:s0, blk] · T_ss),   s0 = s·ib,
+// the blocked form of the forward recurrence of src/qr.jl:64-83 (LAPACK
+// dlarft: T12 = −T11 V1ᵀV2 T22).  One workgroup; T (k ≤ 128) in LDS, the
+// product Y kept transposed in T's zero lower part (rows s0.., columns
+// 0..s0-1), which no later step reads.
+template <typename T>
+__global__ __launch_bounds__(1024) void qr_t_merge_kernel(int k, int ib, T *Tw, int64_t ldt,
+                                                          const T *G, int64_t ldg)
+{
+    constexpr int LD = 129;
+    __shared__ T Ts[128 * LD];
+    const int t = threadIdx.x;
+    for (int e = t; e < k * k; e += 1024) {
+        const int i = e % k, j = e / k;
+        Ts[i + j * LD] = Tw[i + (int64_t)j * ldt];
+    }
+    __syncthreads();
+    for (int s0 = ib; s0 < k; s0 += ib) {
+        const int b = min(ib, k - s0);
+        // Y (s0×b) = G[0:s0, s0:s0+b] · T_ss, T_ss upper triangular
+        for (int e = t; e < s0 * b; e += 1024) {
+            const int i = e % s0, c = e / s0;
+            T acc = T(0);
+            for (int l = 0; l <= c; ++l)
+                acc += G[i + (int64_t)(s0 + l) * ldg] * Ts[s0 + l + (s0 + c) * LD];
+            Ts[s0 + c + i * LD] = acc;                      // Yᵀ
+        }
+        __syncthreads();
+        // T[0:s0, s0:s0+b] = −T[0:s0, 0:s0] · Y, T[0:s0,0:s0] upper triangular
+        for (int e = t; e < s0 * b; e += 1024) {
+            const int i = e % s0, c = e / s0;
+            T acc = T(0);
+            for (int l = i; l < s0; ++l) acc += Ts[i + l * LD] * Ts[s0 + c + l * LD];
+            Ts[i + (s0 + c) * LD] = -acc;
+        }
+        __syncthreads();
+    }
+    for (int e = t; e < k * k; e += 1024) {
+        const int i = e % k, j = e / k;
+        if (i < j) Tw[i + (int64_t)j * ldt] = Ts[i + j * LD];
+    }
+}
+
 // Workspace layout of one qrBlocked! call.
 template <typename T>
 struct QrWs {
@@ -734,9 +798,13 @@ template <typename T>
 int qr_panel(Handle *h, hipStream_t s, int64_t m, int w, T *A, int64_t lda, T *tau, T *Vw,
              int64_t ldv, T *Tw, int64_t ldt, T *part, unsigned &bar_base, int *err)
 {
-    // 1: LDS-reduction kernel; 2: wave-butterfly kernel
+    // 1: LDS-reduction kernel; 2: wave-butterfly kernel, sized to the panel
+    // width: eight columns per wave, so narrower panels run fewer waves per
+    // workgroup with more rows per lane (fewer workgroups to synchronise)
     const int panel = h->qr_panel;
-    const int R = panel == 1 ? (kQrThreads / kQrNB) * kQrRPT : 64 * kQrRPL;
+    const int wide = w <= 32 ? 32 : (w <= 64 ? 64 : 128);
+    const int R = panel == 1 ? (kQrThreads / kQrNB) * kQrRPT
+                             : (wide == 32 ? 512 : (wide == 64 ? 256 : 64 * kQrRPL));
     const int64_t P = std::max<int64_t>(1, cdiv(m, R));
     // every workgroup must be resident at once for the grid barrier
     if (P > h->num_cus) {
@@ -761,6 +829,12 @@ int qr_panel(Handle *h, hipStream_t s, int64_t m, int w, T *A, int64_t lda, T *t
     if (panel == 1)
         hipLaunchKernelGGL((qr_panel_kernel<T, kQrNB, kQrRPT>), dim3((unsigned)P),
                            dim3(kQrThreads), 0, s, p);
+    else if (wide == 32)
+        hipLaunchKernelGGL((qr_panel2_kernel<T, 32, 8, 256>), dim3((unsigned)P), dim3(256), 0, s,
+                           p);
+    else if (wide == 64)
+        hipLaunchKernelGGL((qr_panel2_kernel<T, 64, 4, 512>), dim3((unsigned)P), dim3(512), 0, s,
+                           p);
     else
         hipLaunchKernelGGL((qr_panel2_kernel<T, kQrNB, kQrRPL, kQrThreads>), dim3((unsigned)P),
                            dim3(kQrThreads), 0, s, p);
@@ -815,8 +889,37 @@ int qr_blocked_dev(Handle *h, hipStream_t s, int64_t m, int64_t n, T *A, int64_t
         const int w = (int)std::min<int64_t>(nn, nb);                     // src/qr.jl:121
         const int64_t k = std::min<int64_t>(mm, w);
         T *Ab = A + j0 + j0 * lda;
-        GLA_TRY(qr_panel<T>(h, s, mm, w, Ab, lda, tau + j0, Vw, mm, Tw, k, part, bar_base,
-                            nullptr));                                   // :123-125
+        const int ib = h->qr_inner;
+        if (h->qr_panel == 2 && ib > 0 && k > ib) {
+            // :123-125 as sub-panels of width ib: factor, update the rest of
+            // the block column, then assemble the block's T from the
+            // sub-panels' T and G = VᵀV
+            hipLaunchKernelGGL(qr_sub_clear_kernel<T>, dim3(1, (unsigned)k), dim3(128), 0, s,
+                               (int)k, Vw, mm, Tw, k);
+            GLA_LAUNCH_CHECK();
+            for (int64_t s0 = 0; s0 < k; s0 += ib) {
+                const int iw = (int)std::min<int64_t>(ib, w - s0);
+                const int64_t ms = mm - s0;
+                const int64_t ks = std::min<int64_t>(ms, iw);
+                T *Vs = Vw + s0 + s0 * mm;
+                T *Tss = Tw + s0 + s0 * k;
+                GLA_TRY(qr_panel<T>(h, s, ms, iw, Ab + s0 + s0 * lda, lda, tau + j0 + s0, Vs, mm,
+                                    Tss, k, part, bar_base, nullptr));
+                if (w - s0 - iw > 0)
+                    GLA_TRY(wy_update<T>(s, 'T', ms, w - s0 - iw, ks, Vs, mm, Tss, k,
+                                         Ab + s0 + (s0 + iw) * lda, lda, W1, W2, split,
+                                         (size_t)split_elems));
+            }
+            T *G = W1;   // k×k, k ≤ nb
+            GLA_TRY(gemm<T>(s, 'T', 'N', 0, k, k, mm, T(1), Vw, mm, Vw, mm, T(0), G, k, 0, split,
+                            (size_t)split_elems));
+            hipLaunchKernelGGL(qr_t_merge_kernel<T>, dim3(1), dim3(1024), 0, s, (int)k, ib, Tw, k,
+                               G, k);
+            GLA_LAUNCH_CHECK();
+        } else {
+            GLA_TRY(qr_panel<T>(h, s, mm, w, Ab, lda, tau + j0, Vw, mm, Tw, k, part, bar_base,
+                                nullptr));                               // :123-125
+        }
         if (nn > nb)                                                      // :126-133
             GLA_TRY(wy_update<T>(s, 'T', mm, nn - w, k, Vw, mm, Tw, k, Ab + w * lda, lda, W1, W2,
                                  split, (size_t)split_elems));

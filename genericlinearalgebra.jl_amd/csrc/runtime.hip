@@ -74,6 +74,14 @@ int gla_set_tuning(gla_handle_t hh, const char *key, int64_t value)
         h->qr_panel = (int)value;
         return GLA_OK;
     }
+    if (std::strcmp(key, "qr_inner") == 0) {
+        if (value != 0 && value != 32 && value != 64) {
+            gla::set_error("qr_inner must be 0, 32 or 64");
+            return -3;
+        }
+        h->qr_inner = (int)value;
+        return GLA_OK;
+    }
     gla::set_error("unknown tuning key '%s'", key);
     return -2;
 }

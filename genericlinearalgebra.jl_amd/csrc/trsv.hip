@@ -11,7 +11,9 @@
 // result is written with device-coherent stores and announced by a flag
 // (epoch value) that dependent workgroups poll, with a bounded spin.  The
 // diagonal solve is one more 128×128×16 MFMA product with the explicit
-// inverse L_bb⁻¹ (from diag_block_inverses).  The grid never exceeds one
+// inverse L_bb⁻¹ (from diag_block_inverses); the last link is folded into
+// it, Y_b = L_bb⁻¹(R_b − Σ_{j<b−1} L_bj Y_j) − (L_bb⁻¹ L_{b,b−1}) Y_{b−1}, so
+// only one product follows the arrival of Y_{b−1}.  The grid never exceeds one
 // workgroup per CU and every wait is on a block of a smaller index, so all
 // waiters are resident and the launch cannot deadlock.
 //
@@ -29,7 +31,8 @@ namespace {
 
 constexpr int kSB = 128;                  // rows per block
 constexpr int kSR = 16;                   // right-hand sides per pass
-constexpr int kST = 256;                  // threads per workgroup
+constexpr int kST = 256;                  // threads per workgroup (residual)
+constexpr int kTT = 256;                  // threads per workgroup (solve: four waves)
 constexpr unsigned kSolveSpin = 1u << 25;
 
 template <typename T>
@@ -53,51 +56,121 @@ struct SMfma<float> {
     static __device__ __forceinline__ int row(int l, int r) { return 4 * (l >> 4) + r; }
 };
 
-// One wave's slab of L_bj: rows 32w + 16t + lc (t = 0, 1), columns
-// 128j + 4kq + lk (kq < 32), the MFMA A-operand layout.  Loaded ahead of
-// the wait for Y_j so that only the wait, Y_j and the MFMAs remain on the
-// dependency chain.
-template <typename T>
-__device__ __forceinline__ void load_slab(const T *__restrict__ L, int64_t ldl, int64_t n, int64_t r0,
-                                          int64_t j, int w, int lc, int lk, T (&sl)[2][kSB / 4])
+// One wave's part of a 128-column block of L, KQ column quads at a time
+// (a "unit": rows r0 + 32w + 16t + lc, t < 2, columns 4(q0 + kq) + lk,
+// kq < KQ) — the MFMA A-operand layout.  Units are loaded one ahead of
+// their use, so that only the wait, the Y block and the MFMAs remain on
+// the dependency chain.  Rows at or past n read as zero.
+template <typename T, int KQ>
+__device__ __forceinline__ void load_unit(const T *__restrict__ L, int64_t ldl, int64_t n, int64_t r0,
+                                          int q0, int w, int lc, int lk, T (&u)[2][KQ])
 {
-    const T *Lj = L + (j * kSB) * ldl;
 #pragma unroll
     for (int t = 0; t < 2; ++t) {
         const int64_t row = r0 + 32 * w + 16 * t + lc;
         const bool ok = row < n;
 #pragma unroll
-        for (int kq = 0; kq < kSB / 4; ++kq)
-            sl[t][kq] = ok ? Lj[row + (int64_t)(4 * kq + lk) * ldl] : T(0);
+        for (int kq = 0; kq < KQ; ++kq)
+            u[t][kq] = ok ? L[row + (int64_t)(4 * (q0 + kq) + lk) * ldl] : T(0);
     }
 }
 
+// Block b of the solve:  Y_b = L_bb⁻¹ (R_b − Σ_{j<b−1} L_bj Y_j) − M_b Y_{b−1}
+// with M_b = L_bb⁻¹ L_{b,b−1} precomputed (diag_sub), so that once Y_{b−1}
+// is published only one 128×128×16 product stands between it and Y_b.
+// Four waves of 32 rows (two 16-row MFMA tiles) each; a block of L is
+// streamed in 32/KQ units (KQ = 16 for FP64 keeps two units and the
+// diagonal inverse's rows within the register file).
 template <typename T>
-__global__ __launch_bounds__(kST, 1) void trsv16_kernel(int64_t n, int64_t nblk,
+__global__ __launch_bounds__(kTT, 1) void trsv16_kernel(int64_t n, int64_t nblk,
                                                         const T *__restrict__ L, int64_t ldl,
                                                         const T *__restrict__ Dt,
+                                                        const T *__restrict__ Mt, int64_t npad,
                                                         const T *__restrict__ R, T *Y, int rev,
                                                         double *__restrict__ X, int64_t ldx,
                                                         int nrhs, int *flags, int epoch, int *err)
 {
     typedef SMfma<T> M;
     typedef typename M::acc_t acc_t;
+    constexpr int KQ = sizeof(T) == 8 ? 16 : 32;   // column quads per unit
+    constexpr int U = (kSB / 4) / KQ;               // units per block
     __shared__ T Ys[2][kSB * kSR];
     __shared__ T Ss[kSB * kSR];
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const int lc = lane & 15, lk = lane >> 4;
+    static_assert(kSB * kSR == 8 * kTT, "eight Y elements per thread");
+
+    // wait for block j, then stage Y_j in LDS buffer ys
+    auto fetch_y = [&](int64_t j, T *ys) {
+        if (threadIdx.x == 0) {
+            unsigned spins = 0;
+            while (load_coherent(flags + j) < epoch) {
+                __builtin_amdgcn_s_sleep(1);
+                if (++spins > kSolveSpin) {
+                    store_coherent(err, 1);
+                    break;
+                }
+            }
+        }
+        __syncthreads();
+        const T *src = Y + j * (kSB * kSR);
+        const T *p[8];
+        T v[8];
+#pragma unroll
+        for (int q = 0; q < 8; ++q) p[q] = src + threadIdx.x + q * kTT;
+        load8_coherent(p, v);
+#pragma unroll
+        for (int q = 0; q < 8; ++q) ys[threadIdx.x + q * kTT] = v[q];
+        __syncthreads();
+    };
+    // c += unit · Y[q0·4 .. q0·4 + 4KQ) rows of ys
+    auto prod = [&](const T (&u)[2][KQ], int q0, const T *ys, acc_t (&c)[2]) {
+#pragma unroll
+        for (int kq = 0; kq < KQ; ++kq) {
+            const T bv = ys[(4 * (q0 + kq) + lk) * kSR + lc];
+#pragma unroll
+            for (int t = 0; t < 2; ++t) c[t] = M::mma(u[t][kq], bv, c[t]);
+        }
+    };
 
     for (int64_t b = blockIdx.x; b < nblk; b += gridDim.x) {
         const int64_t r0 = b * kSB;
-        acc_t acc[2];
+        acc_t acc[2];                                 // Σ_{j<b−1} L_bj Y_j
 #pragma unroll
-        for (int t = 0; t < 2; ++t)
-#pragma unroll
-            for (int r = 0; r < 4; ++r) {
-                const int64_t row = r0 + 32 * w + 16 * t + M::row(lane, r);
-                acc[t][r] = row < n ? R[(rev ? n - 1 - row : row) * kSR + lc] : T(0);
-            }
-        // this wave's rows of L_bb⁻¹ (L_bb⁻¹[r][k] = Dt_b[k + 128 r]), read early
+        for (int t = 0; t < 2; ++t) acc[t] = acc_t{T(0), T(0), T(0), T(0)};
+        const int64_t nl = b > 0 ? b - 1 : 0;   // blocks applied before the diagonal solve
+        const int64_t nu = nl * U;              // units applied before it
+        // unit e < nu is block e / U, quads KQ·(e % U); unit nu + h is M_b's h-th
+        auto load_e = [&](int64_t e, T (&u)[2][KQ]) {
+            if (e < nu)
+                load_unit<T, KQ>(L + (e / U) * kSB * ldl, ldl, n, r0, KQ * (int)(e % U), w, lc, lk,
+                                 u);
+            else
+                load_unit<T, KQ>(Mt, npad, n, r0, KQ * (int)(e - nu), w, lc, lk, u);
+        };
+        T ua[2][KQ], ub[2][KQ];
+        if (b > 0) load_e(0, ua);
+        auto link = [&](int64_t e, T (&use)[2][KQ], T (&nxt)[2][KQ]) {
+            T *ys = Ys[(e / U) & 1];
+            if (e % U == 0) fetch_y(e / U, ys);
+            load_e(e + 1, nxt);
+            prod(use, KQ * (int)(e % U), ys, acc);
+        };
+        int64_t e = 0;
+        for (; e + 2 <= nu; e += 2) {
+            link(e, ua, ub);
+            link(e + 1, ub, ua);
+        }
+        if (e < nu) link(e++, ua, ub);
+        // now M_b's first unit is in ua (nu even) or ub (nu odd); with two
+        // units, the second goes into the other buffer
+        const bool m_in_b = (nu & 1) != 0;
+        if (U == 2 && b > 0) {
+            if (m_in_b) load_e(nu + 1, ua);
+            else load_e(nu + 1, ub);
+        }
+        // y = L_bb⁻¹ S, S = R_b − acc; this wave's rows of L_bb⁻¹
+        // (L_bb⁻¹[r][k] = Dt_b[k + 128 r]) load while S is staged
         T dv[2][kSB / 4];
         {
             const T *D = Dt + b * (kSB * kSB);
@@ -108,51 +181,14 @@ __global__ __launch_bounds__(kST, 1) void trsv16_kernel(int64_t n, int64_t nblk,
                 for (int kq = 0; kq < kSB / 4; ++kq) dv[t][kq] = D[4 * kq + lk + (int64_t)rr * kSB];
             }
         }
-        T sl[2][2][kSB / 4];
-        if (b > 0) load_slab(L, ldl, n, r0, 0, w, lc, lk, sl[0]);
-        for (int64_t j = 0; j < b; ++j) {
-            T *ys = Ys[j & 1];
-            if (threadIdx.x == 0) {
-                unsigned spins = 0;
-                while (load_coherent(flags + j) < epoch) {
-                    __builtin_amdgcn_s_sleep(1);
-                    if (++spins > kSolveSpin) {
-                        store_coherent(err, 1);
-                        break;
-                    }
-                }
-            }
-            __syncthreads();
-            {
-                const T *src = Y + j * (kSB * kSR);
-                const T *p[8];
-                T v[8];
-#pragma unroll
-                for (int q = 0; q < 8; ++q) p[q] = src + threadIdx.x + q * kST;
-                load8_coherent(p, v);
-#pragma unroll
-                for (int q = 0; q < 8; ++q) ys[threadIdx.x + q * kST] = v[q];
-            }
-            __syncthreads();
-            // prefetch the next slab, then acc -= L_bj Y_j (block j < b is full)
-            const int cur = (int)(j & 1);
-            if (j + 1 < b) {
-                if (cur == 0) load_slab(L, ldl, n, r0, j + 1, w, lc, lk, sl[1]);
-                else load_slab(L, ldl, n, r0, j + 1, w, lc, lk, sl[0]);
-            }
-#pragma unroll
-            for (int kq = 0; kq < kSB / 4; ++kq) {
-                const T bv = ys[(4 * kq + lk) * kSR + lc];
-#pragma unroll
-                for (int t = 0; t < 2; ++t)
-                    acc[t] = M::mma(-(cur == 0 ? sl[0][t][kq] : sl[1][t][kq]), bv, acc[t]);
-            }
-        }
-        // Y_b = L_bb⁻¹ S
 #pragma unroll
         for (int t = 0; t < 2; ++t)
 #pragma unroll
-            for (int r = 0; r < 4; ++r) Ss[(32 * w + 16 * t + M::row(lane, r)) * kSR + lc] = acc[t][r];
+            for (int r = 0; r < 4; ++r) {
+                const int64_t row = r0 + 32 * w + 16 * t + M::row(lane, r);
+                const T rv = row < n ? R[(rev ? n - 1 - row : row) * kSR + lc] : T(0);
+                Ss[(32 * w + 16 * t + M::row(lane, r)) * kSR + lc] = rv - acc[t][r];
+            }
         __syncthreads();
         acc_t y[2];
 #pragma unroll
@@ -163,20 +199,79 @@ __global__ __launch_bounds__(kST, 1) void trsv16_kernel(int64_t n, int64_t nblk,
 #pragma unroll
             for (int t = 0; t < 2; ++t) y[t] = M::mma(dv[t][kq], bv, y[t]);
         }
+        // y -= M_b Y_{b-1}
+        if (b > 0) {
+            T *ys = Ys[(b - 1) & 1];
+            fetch_y(b - 1, ys);
+            acc_t z[2];
+#pragma unroll
+            for (int t = 0; t < 2; ++t) z[t] = acc_t{T(0), T(0), T(0), T(0)};
+            if (m_in_b) {
+                prod(ub, 0, ys, z);
+                if (U == 2) prod(ua, KQ, ys, z);
+            } else {
+                prod(ua, 0, ys, z);
+                if (U == 2) prod(ub, KQ, ys, z);
+            }
+#pragma unroll
+            for (int t = 0; t < 2; ++t)
+#pragma unroll
+                for (int r = 0; r < 4; ++r) y[t][r] -= z[t][r];
+        }
 #pragma unroll
         for (int t = 0; t < 2; ++t)
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
                 const int64_t row = r0 + 32 * w + 16 * t + M::row(lane, r);
-                if (row < n) {
-                    store_coherent(Y + row * kSR + lc, y[t][r]);
-                    if (X && lc < nrhs) X[(rev ? n - 1 - row : row) + lc * ldx] += (double)y[t][r];
-                }
+                if (row < n) store_coherent(Y + row * kSR + lc, y[t][r]);
             }
         stores_done();
         __syncthreads();
         if (threadIdx.x == 0) store_coherent(flags + b, epoch);
+        // X is read by no other block: update it after the announcement
+        if (X && lc < nrhs) {
+#pragma unroll
+            for (int t = 0; t < 2; ++t)
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    const int64_t row = r0 + 32 * w + 16 * t + M::row(lane, r);
+                    if (row < n) X[(rev ? n - 1 - row : row) + lc * ldx] += (double)y[t][r];
+                }
+        }
     }
+}
+
+// M_b = L_bb⁻¹ L_{b,b−1} for every block b ≥ 1, column-major in Mt
+// (npad × 128: row r0 + r, column c at Mt[r0 + r + c·npad]).  Grid
+// (nblk − 1) × 4, each workgroup 32 columns of one M_b.
+template <typename T>
+__global__ __launch_bounds__(256) void diag_sub_kernel(int64_t n, const T *__restrict__ Dt,
+                                                       const T *__restrict__ L, int64_t ldl,
+                                                       T *__restrict__ Mt, int64_t npad)
+{
+    __shared__ T Ls[kSB][33];
+    const int t = threadIdx.x;
+    const int64_t b = blockIdx.x + 1;
+    const int c0 = 32 * blockIdx.y;
+    const int64_t r0 = b * kSB, cb = (b - 1) * kSB + c0;
+    for (int e = t; e < kSB * 32; e += 256) {
+        const int k = e % kSB, c = e / kSB;
+        const int64_t row = r0 + k;
+        Ls[k][c] = row < n ? L[row + (cb + c) * ldl] : T(0);
+    }
+    __syncthreads();
+    const int r = t >> 1, ch = 16 * (t & 1);
+    const T *D = Dt + b * (kSB * kSB) + (int64_t)r * kSB;
+    T acc[16];
+#pragma unroll
+    for (int c = 0; c < 16; ++c) acc[c] = T(0);
+    for (int k = 0; k < kSB; ++k) {
+        const T d = D[k];
+#pragma unroll
+        for (int c = 0; c < 16; ++c) acc[c] += d * Ls[k][ch + c];
+    }
+#pragma unroll
+    for (int c = 0; c < 16; ++c) Mt[r0 + r + (int64_t)(c0 + ch + c) * npad] = acc[c];
 }
 
 // ---------------------------------------------------- symmetric residual --
@@ -328,8 +423,9 @@ __global__ void rowmajor16_kernel(int64_t n, int nrhs, const double *__restrict_
 }  // namespace
 
 template <typename T>
-int trsv16(Handle *h, hipStream_t s, int64_t n, const T *L, int64_t ldl, const T *Dt, const T *R,
-           T *Y, int rev, double *X, int64_t ldx, int nrhs, int *flags, int epoch, int *err)
+int trsv16(Handle *h, hipStream_t s, int64_t n, const T *L, int64_t ldl, const T *Dt, const T *Mt,
+           const T *R, T *Y, int rev, double *X, int64_t ldx, int nrhs, int *flags, int epoch,
+           int *err)
 {
     if (n <= 0) return GLA_OK;
     if (nrhs > kSR) {
@@ -338,8 +434,19 @@ int trsv16(Handle *h, hipStream_t s, int64_t n, const T *L, int64_t ldl, const T
     }
     const int64_t nblk = cdiv(n, kSB);
     const int grid = (int)std::min<int64_t>(nblk, h->num_cus);
-    hipLaunchKernelGGL((trsv16_kernel<T>), dim3(grid), dim3(kST), 0, s, n, nblk, L, ldl, Dt, R, Y,
-                       rev, X, ldx, nrhs, flags, epoch, err);
+    hipLaunchKernelGGL((trsv16_kernel<T>), dim3(grid), dim3(kTT), 0, s, n, nblk, L, ldl, Dt, Mt,
+                       nblk * kSB, R, Y, rev, X, ldx, nrhs, flags, epoch, err);
+    GLA_LAUNCH_CHECK();
+    return GLA_OK;
+}
+
+template <typename T>
+int diag_sub(hipStream_t s, int64_t n, const T *L, int64_t ldl, const T *Dt, T *Mt)
+{
+    const int64_t nblk = cdiv(n, kSB);
+    if (nblk < 2) return GLA_OK;
+    hipLaunchKernelGGL((diag_sub_kernel<T>), dim3((unsigned)(nblk - 1), 4), dim3(256), 0, s, n, Dt,
+                       L, ldl, Mt, nblk * kSB);
     GLA_LAUNCH_CHECK();
     return GLA_OK;
 }
@@ -369,10 +476,14 @@ int symv_residual(hipStream_t s, int64_t n, const double *A, int64_t lda, const 
 size_t symv_scratch(int64_t n) { return (size_t)cdiv(n, kVB) * kVB * kSR * (1 + kVCH); }
 
 template int trsv16<double>(Handle *, hipStream_t, int64_t, const double *, int64_t,
-                            const double *, const double *, double *, int, double *, int64_t, int,
-                            int *, int, int *);
+                            const double *, const double *, const double *, double *, int,
+                            double *, int64_t, int, int *, int, int *);
 template int trsv16<float>(Handle *, hipStream_t, int64_t, const float *, int64_t, const float *,
-                           const float *, float *, int, double *, int64_t, int, int *, int, int *);
+                           const float *, const float *, float *, int, double *, int64_t, int,
+                           int *, int, int *);
+template int diag_sub<double>(hipStream_t, int64_t, const double *, int64_t, const double *,
+                              double *);
+template int diag_sub<float>(hipStream_t, int64_t, const float *, int64_t, const float *, float *);
 template int symv_residual<double>(hipStream_t, int64_t, const double *, int64_t, const double *,
                                    int64_t, const double *, int64_t, int, double *, double *,
                                    double *);

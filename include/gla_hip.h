@@ -52,7 +52,11 @@ const char *gla_version(void);
  * 0 runs it as recursive halving down to 2048-wide leaves.  Both compute
  * the same factor.  "mixed_block": block width (multiple of 128,
  * default 512) of the look-ahead driver of the BF16/FP16 factorisation in
- * gla_chol_solve_mixed_d. */
+ * gla_chol_solve_mixed_d.  "qr_panel": 1 or 2 (default), the form of the
+ * Householder panel kernel.  "qr_inner": 64 (default) or 32 factors each
+ * QR block column as sub-panels of that width with compact-WY updates
+ * between them; 0 factors it in one panel launch.  V, τ and R are the same
+ * Householder sequence either way. */
 int gla_set_tuning(gla_handle_t h, const char *key, int64_t value);
 
 /* ------------------------------------------------------------ Cholesky -- */
@@ -180,9 +184,10 @@ int gla_chol_solve_mixed_d(gla_handle_t h, char prec, int64_t n, const double *A
                            double tol, int32_t max_iter, int32_t *iters, double *berr,
                            int64_t *info, char *prec_used, void *stream);
 /* Low-precision MFMA product used by the mixed factorisation:
- * C (m×n, FP32) += alpha · A Bᵀ with A (m×k), B (n×k) FP32 in memory rounded
- * to `tin` ('b' BF16, 'h' FP16) on the way in, FP32 accumulation,
- * alpha ∈ {+1, −1}.  Mirrors rankUpdate!'s role (src/juliaBLAS.jl:89). */
+ * C (m×n, FP32) += alpha · A Bᵀ with A (m×k), B (n×k) FP32 in memory, each
+ * rounded once to `tin` ('b' BF16, 'h' FP16) into a packed copy in the
+ * handle's workspace, FP32 accumulation.  Mirrors rankUpdate!'s role
+ * (src/juliaBLAS.jl:89). */
 int gla_gemm_lowp_nt(gla_handle_t h, char tin, int64_t m, int64_t n, int64_t k, float alpha,
                      const float *A, int64_t lda, const float *B, int64_t ldb, float *C,
                      int64_t ldc, void *stream);
