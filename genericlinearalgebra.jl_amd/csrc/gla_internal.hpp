@@ -60,8 +60,10 @@ struct Handle {
     // QR panel kernel: 1 LDS-reduction form, 2 wave-butterfly form.
     int qr_panel = 2;
     // Width of the sub-panels a QR block column is factored in (0: the whole
-    // block column in one panel launch); 32 or 64.
-    int qr_inner = 64;
+    // block column in one panel launch; 32 or 64; -1: by the column height).
+    int qr_inner = -1;
+    // Depth-1 look-ahead in qrBlocked! (trailing update on the side stream).
+    int qr_lookahead = 1;
 };
 
 // Ensure the workspace holds at least `bytes` (grows with hipMalloc, which

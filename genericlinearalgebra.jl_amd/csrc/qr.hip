@@ -790,8 +790,12 @@ size_t qr_ws_elems(int64_t m, int64_t n, int nb, int64_t *split_elems)
     const int64_t P = std::max<int64_t>(1, cdiv(m, 64));
     const int64_t sp = 8 * (int64_t)nb * std::max<int64_t>(n, 1);
     *split_elems = sp;
-    return (size_t)m * nb + (size_t)nb * nb + 2 * (size_t)nb * std::max<int64_t>(n, 1) +
-           2 * (size_t)(P + 1) * 2 * kQrNB + (size_t)sp + 64;
+    // two panel sets (V, T) and two update sets (W1, W2, split-K partials):
+    // the look-ahead driver updates on the side stream with one set while
+    // the caller's stream factors the next panel with the other
+    return 2 * ((size_t)m * nb + (size_t)nb * nb + 2 * (size_t)nb * std::max<int64_t>(n, 1) +
+                (size_t)sp) +
+           2 * (size_t)(P + 1) * 2 * kQrNB + 64;
 }
 
 template <typename T>
@@ -863,6 +867,49 @@ int wy_update(hipStream_t s, char trans, int64_t mm, int64_t n2, int64_t k, cons
     return GLA_OK;
 }
 
+// Factor one block column (mm×w at Ab) into V (explicit, Vw) and T (Tw):
+// one panel launch, or sub-panels of width h->qr_inner with compact-WY
+// updates between them and the block's T assembled from theirs and G = VᵀV.
+template <typename T>
+int qr_block_column(Handle *h, hipStream_t s, int64_t mm, int w, T *Ab, int64_t lda, T *tau,
+                    T *Vw, T *Tw, T *W1, T *W2, T *split, size_t split_elems, T *part,
+                    unsigned &bar_base)
+{
+    const int64_t k = std::min<int64_t>(mm, w);
+    // auto (-1): sub-panels of 64 for block columns taller than 12288 rows,
+    // where the full-width panel spreads over more workgroups than the
+    // 64-wide one and its per-column exchange is slower
+    const int ib = h->qr_inner >= 0 ? h->qr_inner : (mm > 12288 ? 64 : 0);
+    if (!(h->qr_panel == 2 && ib > 0 && k > ib))
+        return qr_panel<T>(h, s, mm, w, Ab, lda, tau, Vw, mm, Tw, k, part, bar_base, nullptr);
+    hipLaunchKernelGGL(qr_sub_clear_kernel<T>, dim3(1, (unsigned)k), dim3(128), 0, s, (int)k, Vw,
+                       mm, Tw, k);
+    GLA_LAUNCH_CHECK();
+    for (int64_t s0 = 0; s0 < k; s0 += ib) {
+        const int iw = (int)std::min<int64_t>(ib, w - s0);
+        const int64_t ms = mm - s0;
+        const int64_t ks = std::min<int64_t>(ms, iw);
+        T *Vs = Vw + s0 + s0 * mm;
+        T *Tss = Tw + s0 + s0 * k;
+        GLA_TRY(qr_panel<T>(h, s, ms, iw, Ab + s0 + s0 * lda, lda, tau + s0, Vs, mm, Tss, k, part,
+                            bar_base, nullptr));
+        if (w - s0 - iw > 0)
+            GLA_TRY(wy_update<T>(s, 'T', ms, w - s0 - iw, ks, Vs, mm, Tss, k,
+                                 Ab + s0 + (s0 + iw) * lda, lda, W1, W2, split, split_elems));
+    }
+    T *G = W1;   // k×k, k ≤ nb
+    GLA_TRY(gemm<T>(s, 'T', 'N', 0, k, k, mm, T(1), Vw, mm, Vw, mm, T(0), G, k, 0, split,
+                    split_elems));
+    hipLaunchKernelGGL(qr_t_merge_kernel<T>, dim3(1), dim3(1024), 0, s, (int)k, ib, Tw, k, G, k);
+    GLA_LAUNCH_CHECK();
+    return GLA_OK;
+}
+
+// qrBlocked! (src/qr.jl:113-146).  With h->qr_lookahead, depth-1
+// look-ahead: after panel k the caller's stream applies its reflectors to
+// block column k+1 only and factors panel k+1 at once, while the side
+// stream applies them to the rest of the trailing matrix; panels alternate
+// between two (V, T) sets and the two streams use separate update buffers.
 template <typename T>
 int qr_blocked_dev(Handle *h, hipStream_t s, int64_t m, int64_t n, T *A, int64_t lda, int nb,
                    T *tau)
@@ -871,59 +918,63 @@ int qr_blocked_dev(Handle *h, hipStream_t s, int64_t m, int64_t n, T *A, int64_t
     int64_t split_elems = 0;
     const size_t need = qr_ws_elems<T>(m, n, nb, &split_elems);
     GLA_TRY(ensure_workspace(h, need * sizeof(T)));
-    T *base = reinterpret_cast<T *>(h->ws);
-    const int64_t R = (kQrThreads / kQrNB) * kQrRPT;
-    const int64_t P = std::max<int64_t>(1, cdiv(m, R));
-    T *Vw = base;
-    T *Tw = Vw + (size_t)m * nb;
-    T *W1 = Tw + (size_t)nb * nb;
-    T *W2 = W1 + (size_t)nb * n;
-    T *part = W2 + (size_t)nb * n;
-    T *split = part + 2 * (size_t)P * 2 * kQrNB;
+    const size_t sp = (size_t)split_elems, nw = (size_t)nb * std::max<int64_t>(n, 1);
+    T *p = reinterpret_cast<T *>(h->ws);
+    T *Vw[2], *Tw[2], *W1[2], *W2[2], *split[2];
+    for (int q = 0; q < 2; ++q) {
+        Vw[q] = p;
+        Tw[q] = Vw[q] + (size_t)m * nb;
+        W1[q] = Tw[q] + (size_t)nb * nb;
+        W2[q] = W1[q] + nw;
+        split[q] = W2[q] + nw;
+        p = split[q] + sp;
+    }
+    T *part = p;
+    const int64_t npanels = cdiv(std::min(m, n), (int64_t)nb);
+    const bool la = h->qr_lookahead && npanels > 2;
 
     hipLaunchKernelGGL(k_set_int, dim3(1), dim3(1), 0, s, reinterpret_cast<int *>(h->dbar), 0);
     GLA_LAUNCH_CHECK();
+    if (la) {
+        GLA_HIP_TRY(hipEventRecord(h->ev_fork, s));
+        GLA_HIP_TRY(hipStreamWaitEvent(h->side, h->ev_fork, 0));
+    }
     unsigned bar_base = 0;
-    for (int64_t j0 = 0;; j0 += nb) {
+    int64_t step = 0;
+    for (int64_t j0 = 0;; j0 += nb, ++step) {
         const int64_t mm = m - j0, nn = n - j0;
         const int w = (int)std::min<int64_t>(nn, nb);                     // src/qr.jl:121
         const int64_t k = std::min<int64_t>(mm, w);
         T *Ab = A + j0 + j0 * lda;
-        const int ib = h->qr_inner;
-        if (h->qr_panel == 2 && ib > 0 && k > ib) {
-            // :123-125 as sub-panels of width ib: factor, update the rest of
-            // the block column, then assemble the block's T from the
-            // sub-panels' T and G = VᵀV
-            hipLaunchKernelGGL(qr_sub_clear_kernel<T>, dim3(1, (unsigned)k), dim3(128), 0, s,
-                               (int)k, Vw, mm, Tw, k);
-            GLA_LAUNCH_CHECK();
-            for (int64_t s0 = 0; s0 < k; s0 += ib) {
-                const int iw = (int)std::min<int64_t>(ib, w - s0);
-                const int64_t ms = mm - s0;
-                const int64_t ks = std::min<int64_t>(ms, iw);
-                T *Vs = Vw + s0 + s0 * mm;
-                T *Tss = Tw + s0 + s0 * k;
-                GLA_TRY(qr_panel<T>(h, s, ms, iw, Ab + s0 + s0 * lda, lda, tau + j0 + s0, Vs, mm,
-                                    Tss, k, part, bar_base, nullptr));
-                if (w - s0 - iw > 0)
-                    GLA_TRY(wy_update<T>(s, 'T', ms, w - s0 - iw, ks, Vs, mm, Tss, k,
-                                         Ab + s0 + (s0 + iw) * lda, lda, W1, W2, split,
-                                         (size_t)split_elems));
+        const int q = (int)(step & 1);
+        GLA_TRY(qr_block_column<T>(h, s, mm, w, Ab, lda, tau + j0, Vw[q], Tw[q], W1[0], W2[0],
+                                   split[0], sp, part, bar_base));        // :123-125
+        if (nn > nb) {                                                    // :126-133
+            if (!la) {
+                GLA_TRY(wy_update<T>(s, 'T', mm, nn - w, k, Vw[q], mm, Tw[q], k, Ab + w * lda, lda,
+                                     W1[0], W2[0], split[0], sp));
+            } else {
+                // block column k+1 here; it was last touched by the side
+                // stream's update of the previous step
+                const int64_t w2 = std::min<int64_t>(nb, nn - w);
+                const int64_t rest = nn - w - w2;
+                if (step > 0) GLA_HIP_TRY(hipStreamWaitEvent(s, h->ev_step[1], 0));
+                GLA_TRY(wy_update<T>(s, 'T', mm, w2, k, Vw[q], mm, Tw[q], k, Ab + w * lda, lda,
+                                     W1[0], W2[0], split[0], sp));
+                if (rest > 0) {
+                    GLA_HIP_TRY(hipEventRecord(h->ev_step[0], s));
+                    GLA_HIP_TRY(hipStreamWaitEvent(h->side, h->ev_step[0], 0));
+                    GLA_TRY(wy_update<T>(h->side, 'T', mm, rest, k, Vw[q], mm, Tw[q], k,
+                                         Ab + (w + w2) * lda, lda, W1[1], W2[1], split[1], sp));
+                    GLA_HIP_TRY(hipEventRecord(h->ev_step[1], h->side));
+                }
             }
-            T *G = W1;   // k×k, k ≤ nb
-            GLA_TRY(gemm<T>(s, 'T', 'N', 0, k, k, mm, T(1), Vw, mm, Vw, mm, T(0), G, k, 0, split,
-                            (size_t)split_elems));
-            hipLaunchKernelGGL(qr_t_merge_kernel<T>, dim3(1), dim3(1024), 0, s, (int)k, ib, Tw, k,
-                               G, k);
-            GLA_LAUNCH_CHECK();
-        } else {
-            GLA_TRY(qr_panel<T>(h, s, mm, w, Ab, lda, tau + j0, Vw, mm, Tw, k, part, bar_base,
-                                nullptr));                               // :123-125
         }
-        if (nn > nb)                                                      // :126-133
-            GLA_TRY(wy_update<T>(s, 'T', mm, nn - w, k, Vw, mm, Tw, k, Ab + w * lda, lda, W1, W2,
-                                 split, (size_t)split_elems));
         if (!(mm > nb && nn > nb)) break;                                 // :136
+    }
+    if (la) {
+        GLA_HIP_TRY(hipEventRecord(h->ev_join, h->side));
+        GLA_HIP_TRY(hipStreamWaitEvent(s, h->ev_join, 0));
     }
     return GLA_OK;
 }

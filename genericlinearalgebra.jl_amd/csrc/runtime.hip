@@ -75,11 +75,19 @@ int gla_set_tuning(gla_handle_t hh, const char *key, int64_t value)
         return GLA_OK;
     }
     if (std::strcmp(key, "qr_inner") == 0) {
-        if (value != 0 && value != 32 && value != 64) {
-            gla::set_error("qr_inner must be 0, 32 or 64");
+        if (value != -1 && value != 0 && value != 32 && value != 64) {
+            gla::set_error("qr_inner must be -1, 0, 32 or 64");
             return -3;
         }
         h->qr_inner = (int)value;
+        return GLA_OK;
+    }
+    if (std::strcmp(key, "qr_lookahead") == 0) {
+        if (value != 0 && value != 1) {
+            gla::set_error("qr_lookahead must be 0 or 1");
+            return -3;
+        }
+        h->qr_lookahead = (int)value;
         return GLA_OK;
     }
     gla::set_error("unknown tuning key '%s'", key);

@@ -53,10 +53,14 @@ const char *gla_version(void);
  * the same factor.  "mixed_block": block width (multiple of 128,
  * default 512) of the look-ahead driver of the BF16/FP16 factorisation in
  * gla_chol_solve_mixed_d.  "qr_panel": 1 or 2 (default), the form of the
- * Householder panel kernel.  "qr_inner": 64 (default) or 32 factors each
- * QR block column as sub-panels of that width with compact-WY updates
- * between them; 0 factors it in one panel launch.  V, τ and R are the same
- * Householder sequence either way. */
+ * Householder panel kernel.  "qr_inner": 32 or 64 factors each QR block
+ * column as sub-panels of that width with compact-WY updates between them;
+ * 0 factors it in one panel launch; −1 (default) takes 64 for block columns
+ * taller than 12288 rows and 0 otherwise.  "qr_lookahead": 1 (default)
+ * factors the next QR panel while the side stream applies the current
+ * block reflector to the rest of the trailing matrix; 0 runs the
+ * reference's sequential loop.  V, τ and R are the same Householder
+ * sequence in every setting. */
 int gla_set_tuning(gla_handle_t h, const char *key, int64_t value);
 
 /* ------------------------------------------------------------ Cholesky -- */

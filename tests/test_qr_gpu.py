@@ -71,21 +71,26 @@ def test_unblocked_and_blocked_vs_oracle(rng, dtype, m, n):
         assert rel(tau, otau) <= tol, nb
 
 
+@pytest.mark.parametrize("la", [0, 1])
 @pytest.mark.parametrize("inner", [0, 32, 64])
 @pytest.mark.parametrize("m,n,nb", [(300, 128, 128), (1000, 257, 128), (2000, 700, 100),
-                                    (130, 400, 128), (5000, 192, 64), (97, 97, 128)])
-def test_subpanel_widths_agree(rng, inner, m, n, nb):
+                                    (130, 400, 128), (5000, 192, 64), (97, 97, 128),
+                                    (1500, 1500, 128)])
+def test_subpanel_widths_and_lookahead_agree(rng, la, inner, m, n, nb):
     """Factoring each block column as sub-panels (qr_inner 32/64) or in one
-    panel launch (0) computes the same Householder sequence."""
+    panel launch (0), with or without the look-ahead driver, computes the
+    same Householder sequence."""
     gla = _gla()
     A = rng.standard_normal((m, n))
     tol = 10 * max(m, n) * eps_of(np.float64)
     ofac, otau = O.qr_blocked(A, nb)
     try:
         gla.set_tuning("qr_inner", inner)
+        gla.set_tuning("qr_lookahead", la)
         F = gla.qrBlocked_(gla.colmajor(A), nb)
     finally:
-        gla.set_tuning("qr_inner", 64)
+        gla.set_tuning("qr_inner", -1)
+        gla.set_tuning("qr_lookahead", 1)
     assert rel(gla.to_numpy(F.factors), ofac) <= tol
     assert rel(F.tau.cpu().numpy(), otau) <= tol
 
