@@ -129,7 +129,7 @@ def mixed_section(torch, gla, n=N_CHOL, nrhs=16):
         A = (Q * sig) @ Q.t()
         A = A.t().contiguous().t()
         row = {"kappa": kappa}
-        for prec in ("fp64", "auto", "bf16", "fp16"):
+        for prec in ("fp64", "auto", "fp32", "bf16", "fp16"):
             try:
                 res = {}
 

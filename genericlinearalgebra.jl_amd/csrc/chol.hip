@@ -236,10 +236,13 @@ __device__ __forceinline__ void tile_inverse(const T v[4], const T rd[16], T w[4
 __host__ __device__ constexpr int fslot_bi(int w, int q) { return q <= w ? w : 7 - w; }
 __host__ __device__ constexpr int fslot_bj(int w, int q) { return q <= w ? q : q - w - 1; }
 
-template <typename T, int W>
-__device__ __forceinline__ void factor_body(int nb, T *__restrict__ A, int64_t lda, int *info,
+// TS: storage type of A (T, or FP32 for the mixed-precision driver, which
+// factors its FP32 diagonal tiles in FP64); Linv32: optional FP32 copy of
+// the tile inverses.
+template <typename T, int W, typename TS = T>
+__device__ __forceinline__ void factor_body(int nb, TS *__restrict__ A, int64_t lda, int *info,
                                             int64_t info_base, T *__restrict__ LinvG, T *Xb,
-                                            T *Db, T *Lib)
+                                            T *Db, T *Lib, float *__restrict__ Linv32 = nullptr)
 {
     typedef PanelMfma<T> M;
     typedef typename M::acc_t acc_t;
@@ -255,7 +258,7 @@ __device__ __forceinline__ void factor_body(int nb, T *__restrict__ A, int64_t l
         for (int r = 0; r < 4; ++r) {
             const int gi = bi * 16 + M::row(lane, r), gj = bj * 16 + lc;
             T v = T(0);
-            if (gj <= gi) v = gi < nb ? A[gi + (int64_t)gj * lda] : (gi == gj ? T(1) : T(0));
+            if (gj <= gi) v = gi < nb ? T(A[gi + (int64_t)gj * lda]) : (gi == gj ? T(1) : T(0));
             acc[q][r] = v;
         }
     }
@@ -288,6 +291,7 @@ __device__ __forceinline__ void factor_body(int nb, T *__restrict__ A, int64_t l
                 D[i * kXLD + g + 4 * t] = v[t];
                 Li[i * kXLD + g + 4 * t] = w[t];
                 if (LinvG) LinvG[kb * 256 + i * 16 + g + 4 * t] = w[t];
+                if (Linv32) Linv32[kb * 256 + i * 16 + g + 4 * t] = float(w[t]);
             }
         }
         __syncthreads();
@@ -332,7 +336,7 @@ __device__ __forceinline__ void factor_body(int nb, T *__restrict__ A, int64_t l
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
             const int gi = bi * 16 + M::row(lane, r), gj = bj * 16 + lc;
-            if (gi < nb && gj <= gi) A[gi + (int64_t)gj * lda] = acc[q][r];
+            if (gi < nb && gj <= gi) A[gi + (int64_t)gj * lda] = TS(acc[q][r]);
         }
     }
 }
@@ -355,6 +359,26 @@ __global__ __launch_bounds__(kPanelThreads) void chol_factor_kernel(int nb, T *_
     case 1: factor_body<T, 1>(nb, A, lda, info, info_base, LinvG, Xb, Db, Lib); break;
     case 2: factor_body<T, 2>(nb, A, lda, info, info_base, LinvG, Xb, Db, Lib); break;
     default: factor_body<T, 3>(nb, A, lda, info, info_base, LinvG, Xb, Db, Lib); break;
+    }
+}
+
+// The same factorisation of an FP32 diagonal block computed in FP64: the
+// block is read and written back in FP32, the tile inverses are stored in
+// FP32 (Linv32).
+__global__ __launch_bounds__(kPanelThreads) void chol_factor_f32io_kernel(int nb,
+                                                                          float *__restrict__ A,
+                                                                          int64_t lda, int *info,
+                                                                          int64_t info_base,
+                                                                          float *__restrict__ Linv32)
+{
+    __shared__ double Xb[2 * kPanelNB * kXLD];
+    __shared__ double Db[2 * 16 * kXLD];
+    __shared__ double Lib[2 * 16 * kXLD];
+    switch (threadIdx.x >> 6) {
+    case 0: factor_body<double, 0, float>(nb, A, lda, info, info_base, nullptr, Xb, Db, Lib, Linv32); break;
+    case 1: factor_body<double, 1, float>(nb, A, lda, info, info_base, nullptr, Xb, Db, Lib, Linv32); break;
+    case 2: factor_body<double, 2, float>(nb, A, lda, info, info_base, nullptr, Xb, Db, Lib, Linv32); break;
+    default: factor_body<double, 3, float>(nb, A, lda, info, info_base, nullptr, Xb, Db, Lib, Linv32); break;
     }
 }
 
@@ -715,6 +739,14 @@ int chol_factor_block(hipStream_t s, int nb, T *A, int64_t lda, int *info, int64
                       T *LinvG)
 {
     return launch_factor<T>(s, nb, A, lda, info, info_base, LinvG);
+}
+int chol_factor_block_f32io(hipStream_t s, int nb, float *A, int64_t lda, int *info,
+                            int64_t info_base, float *Linv32)
+{
+    hipLaunchKernelGGL(chol_factor_f32io_kernel, dim3(1), dim3(kPanelThreads), 0, s, nb, A, lda,
+                       info, info_base, Linv32);
+    GLA_LAUNCH_CHECK();
+    return GLA_OK;
 }
 template <typename T>
 int chol_solve_block(hipStream_t s, int nb, const T *L, int64_t ldl, const T *LinvG, T *B,

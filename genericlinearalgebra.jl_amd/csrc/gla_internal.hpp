@@ -96,6 +96,10 @@ int chol_device(Handle *h, hipStream_t s, int64_t n, T *A, int64_t lda, int algo
 template <typename T>
 int chol_factor_block(hipStream_t s, int nb, T *A, int64_t lda, int *info, int64_t info_base,
                       T *LinvG);
+// The diagonal-block factorisation of an FP32 block (nb ≤ 128) computed in
+// FP64: read and written in FP32, FP32 tile inverses in Linv32.
+int chol_factor_block_f32io(hipStream_t s, int nb, float *A, int64_t lda, int *info,
+                            int64_t info_base, float *Linv32);
 template <typename T>
 int chol_solve_block(hipStream_t s, int nb, const T *L, int64_t ldl, const T *LinvG, T *B,
                      int64_t ldb, int64_t nrows);

@@ -5,17 +5,18 @@
 //   factor   L Lᵀ ≈ A in a working copy W:
 //     'd'  FP64 throughout (the plain FP64 path, cholRecursive! on device);
 //     's'  FP32 throughout;
-//     'b'/'h'  FP32 storage, every 128×128 diagonal block factored in FP64,
-//          the panel solve in FP32, and the trailing SYRK/GEMM updates
-//          (rankUpdate!, src/juliaBLAS.jl:89) on BF16 / FP16 MFMA with FP32
-//          accumulation (v_mfma_f32_16x16x32_{bf16,f16});
-//   refine   r = B − A x in FP64 (MFMA GEMM), x += (L Lᵀ)⁻¹ r with the
-//            low-precision factor, until the normwise backward error
+//     'b'/'h'  FP32 storage, every 128×128 diagonal block factored in FP64
+//          (read and written back in FP32), the panel solve in FP32, and
+//          the trailing SYRK/GEMM updates (rankUpdate!, src/juliaBLAS.jl:89)
+//          on BF16 / FP16 MFMA with FP32 accumulation
+//          (v_mfma_f32_16x16x32_{bf16,f16}) from panels rounded once;
+//   refine   r = B − A x in FP64 (symmetric product from A's lower
+//            triangle), x += (L Lᵀ)⁻¹ r with the low-precision factor,
+//            until the normwise backward error
 //            η = max_k ‖r_k‖ / (‖A‖_F ‖x_k‖ + ‖b_k‖) ≤ tol.
 //
-// The triangular solves reuse the right-side solve B <- B L⁻ᵀ of the
-// Cholesky driver on the transposed right-hand sides: L y = r is
-// yᵀ = rᵀ L⁻ᵀ; Lᵀ z = y is a lower solve with the reversed matrix
+// The triangular solves are one-launch dataflow solves (trsv16): L y = r
+// directly, and Lᵀ z = y as a lower solve with the reversed matrix
 // L' = P Lᵀ P (P the reversal), z = P (L'⁻¹ (P y)).
 #include <algorithm>
 #include <cmath>
@@ -293,18 +294,6 @@ __global__ void cvt_lower_d2s(int64_t n, const double *src, int64_t lds, float *
     GLA_ROW(i);
     GLA_COLS(j, n) if (i < n && i >= j) dst[i + j * ldd] = (float)src[i + j * lds];
 }
-template <typename TD, typename TS>
-__global__ void cvt_block(int nb, const TS *src, int64_t lds, TD *dst, int64_t ldd)
-{
-    const int i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y;
-    if (i < nb && i >= j) dst[i + (int64_t)j * ldd] = (TD)src[i + (int64_t)j * lds];
-}
-template <typename TD, typename TS>
-__global__ void cvt_vec(int64_t n, const TS *src, TD *dst)
-{
-    GLA_ROW(i);
-    if (i < n) dst[i] = (TD)src[i];
-}
 // Lr[i][j] = L[n-1-j][n-1-i] for i >= j: the reversal P Lᵀ P, lower.
 template <typename T>
 __global__ void reverse_transpose_lower(int64_t n, const T *L, int64_t ldl, T *Lr, int64_t ldr)
@@ -357,8 +346,7 @@ __global__ __launch_bounds__(256) void sym_sumsq_cols(int64_t n, const double *A
 // blocks and low-precision trailing updates (n ≤ the look-ahead block
 // width).  pk: room for the packed panel, n × lp_kpad(128) TIN.
 template <typename TIN>
-int chol_mixed(Handle *h, hipStream_t s, int64_t n, float *W, int64_t ldw, double *D64,
-               double *Linv64, float *Linv32, TIN *pk)
+int chol_mixed(Handle *h, hipStream_t s, int64_t n, float *W, int64_t ldw, float *Linv32, TIN *pk)
 {
     const int nb = 128;
     const int64_t nsteps = cdiv(n, nb);
@@ -368,17 +356,8 @@ int chol_mixed(Handle *h, hipStream_t s, int64_t n, float *W, int64_t ldw, doubl
         const int64_t below = n - k - kb;
         float *Wkk = W + k + k * ldw;
         float *W21 = Wkk + kb;
-        // diagonal block in FP64
-        hipLaunchKernelGGL((cvt_block<double, float>), dim3(1, kb), dim3(128), 0, s, kb, Wkk, ldw,
-                           D64, (int64_t)nb);
-        GLA_LAUNCH_CHECK();
-        GLA_TRY(chol_factor_block<double>(s, kb, D64, nb, h->dinfo, k, Linv64));
-        hipLaunchKernelGGL((cvt_block<float, double>), dim3(1, kb), dim3(128), 0, s, kb, D64,
-                           (int64_t)nb, Wkk, ldw);
-        GLA_LAUNCH_CHECK();
-        hipLaunchKernelGGL((cvt_vec<float, double>), dim3(8), dim3(256), 0, s, (int64_t)8 * 256,
-                           Linv64, Linv32);
-        GLA_LAUNCH_CHECK();
+        // diagonal block factored in FP64 (read and written back in FP32)
+        GLA_TRY(chol_factor_block_f32io(s, kb, Wkk, ldw, h->dinfo, k, Linv32));
         GLA_TRY(chol_solve_block<float>(s, kb, Wkk, ldw, Linv32, W21, ldw, below));
         if (below == 0) break;
         const int64_t kb2 = std::min<int64_t>(nb, below);
@@ -402,10 +381,10 @@ int chol_mixed(Handle *h, hipStream_t s, int64_t n, float *W, int64_t ldw, doubl
 // pk[step % 2] (each n × lp_kpad(NB) TIN): the side stream may still read
 // the previous step's buffer while the caller's stream packs the next.
 template <typename TIN>
-int chol_mixed_la(Handle *h, hipStream_t s, int64_t n, float *W, int64_t ldw, double *D64,
-                  double *Linv64, float *Linv32, float *Linv, int NB, TIN *pk0, TIN *pk1)
+int chol_mixed_la(Handle *h, hipStream_t s, int64_t n, float *W, int64_t ldw, float *Linv32,
+                  float *Linv, int NB, TIN *pk0, TIN *pk1)
 {
-    if (n <= NB) return chol_mixed<TIN>(h, s, n, W, ldw, D64, Linv64, Linv32, pk0);
+    if (n <= NB) return chol_mixed<TIN>(h, s, n, W, ldw, Linv32, pk0);
     const int64_t nsteps = cdiv(n, NB);
     const bool overlap = nsteps > 2;
     if (overlap) {
@@ -419,8 +398,17 @@ int chol_mixed_la(Handle *h, hipStream_t s, int64_t n, float *W, int64_t ldw, do
         float *Wkk = W + k + k * ldw;
         float *W21 = Wkk + kb;
         TIN *pk = (step & 1) ? pk1 : pk0;
-        GLA_TRY(chol_mixed<TIN>(h, s, kb, Wkk, ldw, D64, Linv64, Linv32, pk));
+        GLA_TRY(chol_mixed<TIN>(h, s, kb, Wkk, ldw, Linv32, pk));
         if (below == 0) break;
+        // every fourth block, stop early once a pivot has failed: a
+        // low-precision attempt that breaks down is abandoned for the next
+        // precision without factoring the rest
+        if ((step & 3) == 3) {
+            int hinfo = 0;
+            GLA_HIP_TRY(hipMemcpyAsync(&hinfo, h->dinfo, sizeof(int), hipMemcpyDeviceToHost, s));
+            GLA_HIP_TRY(hipStreamSynchronize(s));
+            if (hinfo != 0) break;
+        }
         GLA_TRY(tile_inverses<float>(s, kb, Wkk, ldw, Linv));
         GLA_TRY(trsm_rlt<float>(s, below, kb, Wkk, ldw, Linv, W21, ldw));
         const int64_t kb2 = std::min<int64_t>(NB, below);
@@ -473,7 +461,7 @@ double backward_error(const double *rr, const double *xx, const double *bb, int6
 template <typename T>
 struct MixedWS {
     T *W, *Lr, *Linv, *Linvr, *Dt, *Dtr, *Mt, *Mtr, *Rt, *Yt, *Zt;
-    double *Xt, *nrm, *D64, *Linv64;
+    double *Xt, *nrm;
     float *Linv32;
     int *flags, *err;
     void *pk0, *pk1;   // packed low-precision panels, pk_bytes each
@@ -488,7 +476,7 @@ struct MixedWS {
         return 2 * al(es * (size_t)n * n) + 2 * al(es * tile_inverse_elems(n)) +
                4 * al(es * (size_t)npad * 128) + al(es * (size_t)npad * 16 * nch) +
                2 * al(es * (size_t)npad * 16) + al(8 * symv_scratch(n)) +
-               al(8 * (size_t)(32 * nch + n + 64)) + al(8 * 128 * 128) + al(8 * 8 * 256) +
+               al(8 * (size_t)(32 * nch + n + 64)) +
                al(4 * 8 * 256) + al(4 * (size_t)(nblk + 64)) + 2 * al(pk_bytes) + 4096;
     }
     void carve(void *base, int64_t n, int64_t nrhs, size_t pk_bytes)
@@ -516,8 +504,6 @@ struct MixedWS {
         Zt = reinterpret_cast<T *>(take(es * (size_t)npad * 16));
         Xt = reinterpret_cast<double *>(take(8 * symv_scratch(n)));
         nrm = reinterpret_cast<double *>(take(8 * (size_t)(32 * nchunk + n + 64)));
-        D64 = reinterpret_cast<double *>(take(8 * 128 * 128));
-        Linv64 = reinterpret_cast<double *>(take(8 * 8 * 256));
         Linv32 = reinterpret_cast<float *>(take(4 * 8 * 256));
         flags = reinterpret_cast<int *>(take(4 * (size_t)(nblk + 64)));
         err = flags + nblk;
@@ -652,11 +638,11 @@ int mixed_attempt(Handle *h, hipStream_t s, char prec, int64_t n, const double *
         if (prec == 's')
             GLA_TRY(chol_device<float>(h, s, n, ws.W, n, 2, 1, ws.Linv));
         else if (prec == 'b')
-            GLA_TRY(chol_mixed_la<__bf16>(h, s, n, ws.W, n, ws.D64, ws.Linv64, ws.Linv32, ws.Linv,
+            GLA_TRY(chol_mixed_la<__bf16>(h, s, n, ws.W, n, ws.Linv32, ws.Linv,
                                            h->mixed_block, reinterpret_cast<__bf16 *>(ws.pk0),
                                            reinterpret_cast<__bf16 *>(ws.pk1)));
         else
-            GLA_TRY(chol_mixed_la<_Float16>(h, s, n, ws.W, n, ws.D64, ws.Linv64, ws.Linv32, ws.Linv,
+            GLA_TRY(chol_mixed_la<_Float16>(h, s, n, ws.W, n, ws.Linv32, ws.Linv,
                                              h->mixed_block, reinterpret_cast<_Float16 *>(ws.pk0),
                                              reinterpret_cast<_Float16 *>(ws.pk1)));
     }
@@ -680,10 +666,14 @@ int mixed_solve(Handle *h, hipStream_t s, char prec, int64_t n, const double *A,
                 int64_t nrhs, const double *B, int64_t ldb, double *X, int64_t ldx, double tol,
                 int max_iter, int *iters, double *berr, int64_t *info, char *used)
 {
-    // precision ladder: 'a' tries BF16, then FP32, then FP64
-    const char ladder_a[] = {'b', 's', 'd'};
+    // precision ladder: 'a' tries FP32, then FP64.  On gfx950 at n = 16384
+    // the BF16/FP16 factorisations are no faster than FP32 (the FP64
+    // diagonal-block chain, not the trailing MFMA updates, sets their pace)
+    // and need two to four times the refinement sweeps, so they are
+    // selected explicitly ('b', 'h') rather than tried first.
+    const char ladder_a[] = {'s', 'd'};
     const char *ladder = prec == 'a' ? ladder_a : &prec;
-    const int nl = prec == 'a' ? 3 : 1;
+    const int nl = prec == 'a' ? 2 : 1;
     size_t need = 0;
     for (int i = 0; i < nl; ++i)
         need = std::max(need, mixed_ws_bytes(ladder[i], n, nrhs, h->mixed_block));
@@ -762,7 +752,7 @@ int gla_chol_solve_mixed_d(gla_handle_t hh, char prec, int64_t n, const double *
     int it = 0;
     double be = 0.0;
     int64_t inf = 0;
-    char used = prec == 'a' ? 'b' : prec;
+    char used = prec == 'a' ? 's' : prec;
     int rc = GLA_OK;
     if (n > 0 && nrhs > 0) {
         GLA_HIP_TRY(hipSetDevice(h->device));

@@ -275,99 +275,121 @@ __global__ __launch_bounds__(256) void diag_sub_kernel(int64_t n, const T *__res
 }
 
 // ---------------------------------------------------- symmetric residual --
-constexpr int kVB = 64;    // rows per workgroup
-constexpr int kVCH = 8;    // column chunks (grid y): partial sums per chunk
+constexpr int kVB = 64;    // rows of a block
+constexpr int kRB = 8;     // blocks per rectangle side (512 rows)
 
-// P[y] (row-major n×16, FP64) = Σ over column blocks j of chunk y of A_ij X_j
-// for row block i = blockIdx.x, A symmetric, stored in its lower triangle;
-// Xt = X row-major FP64 (n×16, zero-padded).
-__global__ __launch_bounds__(kST) void symv_partial_kernel(int64_t n, const double *__restrict__ A,
+// Symmetric product Y = A X over the lower triangle of A, each element of A
+// read once.  Workgroup (I, J), J ≤ I, takes the rectangle of 64×64 blocks
+// A_ij with i in rectangle row I, j in rectangle column J (and j ≤ i):
+//   y_i += A_ij x_j                 summed into its row partial  Prow[J]
+//   y_j += A_ijᵀ x_i   (j < i)      summed into its column partial Pcol[I]
+// (diagonal blocks are mirrored from their lower half in LDS).  A row of
+// rectangle row R thus has partials Prow[J] for J ≤ R and Pcol[I] for I ≥ R,
+// each written by exactly one workgroup; symv_finish_kernel adds them in a
+// fixed order, so the result does not depend on scheduling.  The next block
+// is loaded into registers while the current one is multiplied.
+__global__ __launch_bounds__(kST, 2) void symv_rect_kernel(int64_t n, const double *__restrict__ A,
                                                            int64_t lda,
                                                            const double *__restrict__ Xt,
-                                                           double *__restrict__ P, int64_t npad)
+                                                           double *__restrict__ Prow,
+                                                           double *__restrict__ Pcol,
+                                                           int64_t npad, int NR)
 {
     typedef SMfma<double> M;
-    __shared__ double Tl[4][16 * 68];        // per-wave transposed staging (64 k × 16 m)
-    __shared__ double Red[4][kVB * kSR];     // per-wave partial sums
-    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    constexpr int LDT = kVB + 1;
+    __shared__ double Ts[kVB * LDT];           // A_ij column-major: (r, c) at [c·65 + r]
+    __shared__ double xi[kVB * kSR], xj[kVB * kSR];
+    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
     const int lc = lane & 15, lk = lane >> 4;
-    const int64_t i = blockIdx.x, r0 = i * kVB;
+    int rid = blockIdx.x, J = 0;
+    while (rid >= NR - J) {
+        rid -= NR - J;
+        ++J;
+    }
+    const int I = J + rid;
     const int64_t nb = (n + kVB - 1) / kVB;
-    const int64_t cpc = (nb + kVCH - 1) / kVCH;
-    const int64_t jb = blockIdx.y * cpc, je = std::min<int64_t>(nb, jb + cpc);
-    d4 acc[4];
+    const int64_t i0 = (int64_t)I * kRB, j0 = (int64_t)J * kRB;
+
+    // thread t holds rows lane, columns w + 4u (u < 16) of a block
+    auto gload = [&](int64_t i, int64_t j, double (&r)[16]) {
+        const int64_t row = i * kVB + lane;
 #pragma unroll
-    for (int t = 0; t < 4; ++t) acc[t] = d4{0.0, 0.0, 0.0, 0.0};
-    double *tl = Tl[w];
-    for (int64_t j = jb + w; j < je; j += 4) {
-        const int64_t c0 = j * kVB;
-        if (j < i) {
-            // A_ij from the lower triangle, rows contiguous
-#pragma unroll 4
-            for (int kq = 0; kq < kVB / 4; ++kq) {
-                const int64_t col = c0 + 4 * kq + lk;
-                const double bv = Xt[col * kSR + lc];
-#pragma unroll
-                for (int t = 0; t < 4; ++t) {
-                    const int64_t row = r0 + 16 * t + lc;
-                    const double a = row < n ? A[row + col * lda] : 0.0;
-                    acc[t] = M::mma(a, bv, acc[t]);
-                }
-            }
-        } else if (j == i) {
-            // diagonal block: element (r, c) from the lower triangle
-#pragma unroll 4
-            for (int kq = 0; kq < kVB / 4; ++kq) {
-                const int64_t col = c0 + 4 * kq + lk;
-                const double bv = col < n ? Xt[col * kSR + lc] : 0.0;
-#pragma unroll
-                for (int t = 0; t < 4; ++t) {
-                    const int64_t row = r0 + 16 * t + lc;
-                    double a = 0.0;
-                    if (row < n && col < n) a = row >= col ? A[row + col * lda] : A[col + row * lda];
-                    acc[t] = M::mma(a, bv, acc[t]);
-                }
-            }
-        } else {
-            // A_ij = A_jiᵀ: stage 16 columns of A_ji (64 rows each) transposed
-            // through LDS so the global reads run down the columns.
-#pragma unroll 1
-            for (int t = 0; t < 4; ++t) {
-                {
-                    const int cc = lane >> 2, rs = 16 * (lane & 3);     // column, row start
-                    const int64_t gcol = r0 + 16 * t + cc;
-#pragma unroll
-                    for (int e = 0; e < 16; ++e) {
-                        const int64_t grow = c0 + rs + e;
-                        tl[cc * 68 + rs + e] = (grow < n && gcol < n) ? A[grow + gcol * lda] : 0.0;
-                    }
-                }
-                __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-#pragma unroll 4
-                for (int kq = 0; kq < kVB / 4; ++kq) {
-                    const int64_t col = c0 + 4 * kq + lk;
-                    const double bv = col < n ? Xt[col * kSR + lc] : 0.0;
-                    acc[t] = M::mma(tl[lc * 68 + 4 * kq + lk], bv, acc[t]);
-                }
-                __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-            }
+        for (int u = 0; u < 16; ++u) {
+            const int64_t col = j * kVB + w + 4 * u;
+            r[u] = (row < n && col < n) ? A[row + col * lda] : 0.0;
         }
+    };
+    d4 yr[kRB];
+#pragma unroll
+    for (int b = 0; b < kRB; ++b) yr[b] = d4{0.0, 0.0, 0.0, 0.0};
+    double ra[16];
+    gload(i0, j0, ra);
+    for (int bj = 0; bj < kRB; ++bj) {
+        const int64_t j = j0 + bj;
+        if (j >= nb) break;
+        d4 yc = d4{0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+        for (int bi = 0; bi < kRB; ++bi) {
+            const int64_t i = i0 + bi;
+            if (i >= nb || i < j) continue;
+#pragma unroll
+            for (int u = 0; u < 16; ++u) Ts[(w + 4 * u) * LDT + lane] = ra[u];
+#pragma unroll
+            for (int e = t; e < kVB * kSR; e += kST) {
+                xi[e] = Xt[i * kVB * kSR + e];
+                xj[e] = Xt[j * kVB * kSR + e];
+            }
+            __syncthreads();
+            if (i == j) {
+                // upper half of a diagonal block from its lower half
+                for (int e = t; e < kVB * kVB; e += kST) {
+                    const int r = e % kVB, c = e / kVB;
+                    if (r < c) Ts[c * LDT + r] = Ts[r * LDT + c];
+                }
+                __syncthreads();
+            }
+            // the next block of the rectangle: (i+1, j), else the first of column j+1
+            {
+                int64_t ni = i + 1, nj = j;
+                if (bi + 1 >= kRB || ni >= nb) {
+                    nj = j + 1;
+                    ni = nj > i0 ? nj : i0;
+                }
+                if (nj < j0 + kRB && nj < nb && ni < nb && ni < i0 + kRB) gload(ni, nj, ra);
+            }
+#pragma unroll
+            for (int kq = 0; kq < kVB / 4; ++kq)
+                yr[bi] = M::mma(Ts[(4 * kq + lk) * LDT + 16 * w + lc], xj[(4 * kq + lk) * kSR + lc],
+                                yr[bi]);
+            if (i != j) {
+#pragma unroll
+                for (int kq = 0; kq < kVB / 4; ++kq)
+                    yc = M::mma(Ts[(16 * w + lc) * LDT + 4 * kq + lk], xi[(4 * kq + lk) * kSR + lc],
+                                yc);
+            }
+            __syncthreads();
+        }
+        double *pc = Pcol + ((int64_t)I * npad + j * kVB + 16 * w) * kSR;
+#pragma unroll
+        for (int r = 0; r < 4; ++r) pc[M::row(lane, r) * kSR + lc] = yc[r];
     }
 #pragma unroll
-    for (int t = 0; t < 4; ++t)
+    for (int bi = 0; bi < kRB; ++bi) {
+        const int64_t i = i0 + bi;
+        if (i >= nb) break;
+        double *pr = Prow + ((int64_t)J * npad + i * kVB + 16 * w) * kSR;
 #pragma unroll
-        for (int r = 0; r < 4; ++r) Red[w][(16 * t + M::row(lane, r)) * kSR + lc] = acc[t][r];
-    __syncthreads();
-    double *Py = P + (int64_t)blockIdx.y * npad * kSR + r0 * kSR;
-    for (int e = threadIdx.x; e < kVB * kSR; e += kST)
-        Py[e] = Red[0][e] + Red[1][e] + Red[2][e] + Red[3][e];
+        for (int r = 0; r < 4; ++r) pr[M::row(lane, r) * kSR + lc] = yr[bi][r];
+    }
 }
 
-// R = B − Σ_y P[y] (type T, row-major n×16), nrm[k] += ‖r_k‖², nrm[16+k] +=
-// ‖x_k‖² over this workgroup's 64 rows.
+// R = B − A X with A X = Σ_{J≤R} Prow[J] + Σ_{I≥R} Pcol[I] for the rows of
+// rectangle row R, added in that order (type T, row-major n×16);
+// nrm[k] += ‖r_k‖², nrm[16+k] += ‖x_k‖² over this workgroup's 64 rows.
 template <typename T>
-__global__ __launch_bounds__(kST) void symv_finish_kernel(int64_t n, const double *__restrict__ P,
-                                                          int64_t npad,
+__global__ __launch_bounds__(kST) void symv_finish_kernel(int64_t n, const double *__restrict__ Prow,
+                                                          const double *__restrict__ Pcol,
+                                                          int64_t npad, int NR,
                                                           const double *__restrict__ Xt,
                                                           const double *__restrict__ B,
                                                           int64_t ldb, int nrhs, T *__restrict__ R,
@@ -375,6 +397,7 @@ __global__ __launch_bounds__(kST) void symv_finish_kernel(int64_t n, const doubl
 {
     __shared__ double sr[2 * kST];
     const int64_t r0 = (int64_t)blockIdx.x * kVB;
+    const int Rr = (int)(blockIdx.x / kRB);
     // every element this thread visits has k = threadIdx.x % 16 (256 is a
     // multiple of 16): accumulate locally, reduce over the 16 threads of each
     // k, one atomic per k and workgroup
@@ -386,8 +409,8 @@ __global__ __launch_bounds__(kST) void symv_finish_kernel(int64_t n, const doubl
         double v = 0.0;
         if (k < nrhs) {
             double ax = 0.0;
-#pragma unroll
-            for (int y = 0; y < kVCH; ++y) ax += P[(int64_t)y * npad * kSR + row * kSR + k];
+            for (int q = 0; q <= Rr; ++q) ax += Prow[((int64_t)q * npad + row) * kSR + k];
+            for (int q = Rr; q < NR; ++q) ax += Pcol[((int64_t)q * npad + row) * kSR + k];
             v = B[row + k * ldb] - ax;
             const double x = Xt[row * kSR + k];
             xs += x * x;
@@ -408,18 +431,17 @@ __global__ __launch_bounds__(kST) void symv_finish_kernel(int64_t n, const doubl
         atomicAdd(nrm + kSR + threadIdx.x, c);
     }
 }
-
-__global__ void rowmajor16_kernel(int64_t n, int nrhs, const double *__restrict__ X, int64_t ldx,
-                                  double *__restrict__ Xt)
+// Xt (row-major npad×16) = X's first nrhs columns, zero elsewhere.
+__global__ void rowmajor16_kernel(int64_t n, int64_t npad, int nrhs, const double *__restrict__ X,
+                                  int64_t ldx, double *__restrict__ Xt)
 {
     const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (e < n * kSR) {
+    if (e < npad * kSR) {
         const int64_t row = e / kSR;
         const int k = (int)(e % kSR);
-        Xt[e] = k < nrhs ? X[row + k * ldx] : 0.0;
+        Xt[e] = (k < nrhs && row < n) ? X[row + k * ldx] : 0.0;
     }
 }
-
 }  // namespace
 
 template <typename T>
@@ -451,8 +473,8 @@ int diag_sub(hipStream_t s, int64_t n, const T *L, int64_t ldl, const T *Dt, T *
     return GLA_OK;
 }
 
-// R = B − sym(A) X; Xt scratch: n×16 FP64 followed by the partial sums
-// (symv_scratch(n) doubles in all); nrm[32] must be zero on entry.
+// R = B − sym(A) X; Xt scratch: npad×16 FP64 followed by the rectangle
+// partials (symv_scratch(n) doubles in all); nrm[32] must be zero on entry.
 template <typename T>
 int symv_residual(hipStream_t s, int64_t n, const double *A, int64_t lda, const double *X,
                   int64_t ldx, const double *B, int64_t ldb, int nrhs, double *Xt, T *R,
@@ -460,20 +482,26 @@ int symv_residual(hipStream_t s, int64_t n, const double *A, int64_t lda, const 
 {
     if (n <= 0) return GLA_OK;
     const int64_t nb = cdiv(n, kVB), npad = nb * kVB;
-    double *P = Xt + npad * kSR;
-    hipLaunchKernelGGL(rowmajor16_kernel, dim3((unsigned)cdiv(n * kSR, 256)), dim3(256), 0, s, n,
-                       nrhs, X, ldx, Xt);
+    const int NR = (int)cdiv(nb, kRB);
+    double *Prow = Xt + npad * kSR;
+    double *Pcol = Prow + (int64_t)NR * npad * kSR;
+    hipLaunchKernelGGL(rowmajor16_kernel, dim3((unsigned)cdiv(npad * kSR, 256)), dim3(256), 0, s,
+                       n, npad, nrhs, X, ldx, Xt);
     GLA_LAUNCH_CHECK();
-    hipLaunchKernelGGL(symv_partial_kernel, dim3((unsigned)nb, kVCH), dim3(kST), 0, s, n, A, lda, Xt,
-                       P, npad);
+    hipLaunchKernelGGL(symv_rect_kernel, dim3((unsigned)(NR * (NR + 1) / 2)), dim3(kST), 0, s, n,
+                       A, lda, Xt, Prow, Pcol, npad, NR);
     GLA_LAUNCH_CHECK();
-    hipLaunchKernelGGL((symv_finish_kernel<T>), dim3((unsigned)nb), dim3(kST), 0, s, n, P, npad, Xt,
-                       B, ldb, nrhs, R, nrm);
+    hipLaunchKernelGGL((symv_finish_kernel<T>), dim3((unsigned)nb), dim3(kST), 0, s, n, Prow, Pcol,
+                       npad, NR, Xt, B, ldb, nrhs, R, nrm);
     GLA_LAUNCH_CHECK();
     return GLA_OK;
 }
 
-size_t symv_scratch(int64_t n) { return (size_t)cdiv(n, kVB) * kVB * kSR * (1 + kVCH); }
+size_t symv_scratch(int64_t n)
+{
+    const int64_t nb = cdiv(n, kVB), NR = cdiv(nb, kRB);
+    return (size_t)(nb * kVB) * kSR * (1 + 2 * NR);
+}
 
 template int trsv16<double>(Handle *, hipStream_t, int64_t, const double *, int64_t,
                             const double *, const double *, const double *, double *, int,

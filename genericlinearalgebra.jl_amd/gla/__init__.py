@@ -552,7 +552,7 @@ def chol_solve_mixed(A, B, precision: str = "auto", tol: float = 1e-12, max_iter
     """X = A⁻¹ B for SPD Float64 A (lower triangle read), factored in
     `precision` ("fp64", "fp32", "bf16": FP32 storage + BF16 MFMA trailing
     updates with FP64 diagonal blocks, "fp16": the same with FP16, "auto":
-    bf16, then fp32, then fp64 until one converges) and refined in Float64
+    fp32, then fp64 if fp32 does not converge) and refined in Float64
     to a normwise backward error ≤ tol.  Raises DomainError (index = first
     non-positive pivot) when a fixed-precision factorisation breaks down.
     Returns (X, MixedSolveResult); check .converged for fixed precisions."""

@@ -173,7 +173,9 @@ int gla_chol_solve_batched_s(gla_handle_t h, gla_batch_plan_t plan, float *const
  * a lower-precision factorisation and FP64 refinement.  `prec` selects the
  * factorisation: 'd' FP64 (the plain path), 's' FP32, 'b' FP32 storage with
  * BF16 MFMA trailing updates and FP64 diagonal blocks, 'h' the same with
- * FP16, 'a' BF16 first, then FP32, then FP64 until one converges.  X is
+ * FP16, 'a' FP32 first, then FP64 if FP32 does not converge (a BF16/FP16
+ * factorisation that breaks down stops within four 512-column blocks of the
+ * failing pivot).  X is
  * refined in FP64 (r = B − A X with the symmetric A, correction with the
  * factor) until the normwise backward error
  * max_k ‖r_k‖₂ / (‖A‖_F ‖x_k‖₂ + ‖b_k‖₂) ≤ tol; a precision stops when η
