@@ -199,21 +199,14 @@ def adjoint_section(torch, gla):
     As = torch.bmm(Bs.transpose(1, 2).contiguous(), Bs) + n * torch.eye(n, dtype=torch.float64,
                                                                          device=dev)
     Ws = torch.randn(cnt, n, n, dtype=torch.float64, device=dev, generator=g)
-    Ls = [gla.colmajor(As[i]) for i in range(cnt)]
-    Gs = [gla.colmajor(torch.tril(Ws[i])) for i in range(cnt)]
-    outs = [gla.empty_colmajor(n, n, torch.float64) for _ in range(cnt)]
-    A0s = [L.clone() for L in Ls]
-
-    def run_batch():
-        for i in range(cnt):
-            Ls[i].copy_(A0s[i])
-            gla.cholRecursive_(Ls[i], gla.Val.L)            # recompute the factor
-            gla.chol_adjoint(Ls[i], Gs[i], "S", out=outs[i])
-
-    t = ev_time(torch, run_batch, reps=2)
+    Acm, Wcm = As.transpose(1, 2), Ws.transpose(1, 2)     # column-major matrices
+    outb = torch.empty_like(As).transpose(1, 2)
+    t = ev_time(torch, lambda: gla.chol_adjoint_batched(Acm, Wcm, "S", out=outb), reps=3)
     out["chol_512x128_fp64"] = {"ms": t * 1e3,
                                 "tflops": cnt * 7 * n ** 3 / 3 / t / 1e12,
-                                "note": "one adjoint call per matrix, factor recomputed"}
+                                "note": "one batched call; factors recomputed inside (the "
+                                        "n^3/3 per matrix of the recompute is not counted)"}
+    del As, Ws, Bs, outb
     n = 8192
     Bm = torch.randn(n, n, dtype=torch.float32, device=dev, generator=g)
     A = (Bm.t() @ Bm + n * torch.eye(n, device=dev)).t().contiguous().t()

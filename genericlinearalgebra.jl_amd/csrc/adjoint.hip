@@ -35,18 +35,23 @@ namespace {
 
 #define GLA_COLS(j, ncols) for (int64_t j = blockIdx.y; j < (ncols); j += gridDim.y)
 #define GLA_ROW(i) const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x
+// strided batch along grid z (stride 0: one matrix)
+#define GLA_BATCH(P, st) P += (int64_t)blockIdx.z * (st)
 
-inline dim3 g2(int64_t rows, int64_t cols)
+inline dim3 g2(int64_t rows, int64_t cols, int64_t batch = 1)
 {
-    return dim3((unsigned)cdiv(rows, 256), (unsigned)std::min<int64_t>(cols, 65535));
+    return dim3((unsigned)cdiv(rows, 256), (unsigned)std::min<int64_t>(cols, 65535),
+                (unsigned)batch);
 }
 
 // D (n×n) <- op(tri(S)), zeros elsewhere.  src_lower: the lower triangle of
 // S is taken (else the upper); trans: D = tri(S)ᵀ.
 template <typename T>
 __global__ void tri_copy(int64_t n, const T *S, int64_t lds, T *D, int64_t ldd, int src_lower,
-                         int trans)
+                         int trans, int64_t sS, int64_t sD)
 {
+    GLA_BATCH(S, sS);
+    GLA_BATCH(D, sD);
     GLA_ROW(i);
     GLA_COLS(j, n)
     if (i < n) {
@@ -59,8 +64,11 @@ __global__ void tri_copy(int64_t n, const T *S, int64_t lds, T *D, int64_t ldd, 
 // D[i][n-1-j] = scale · M[max(i,j)][min(i,j)]: scale·copyltu(M), columns
 // reversed (reads the lower triangle of M only).
 template <typename T>
-__global__ void sym_lower_rev(int64_t n, const T *M, int64_t ldm, T scale, T *D, int64_t ldd)
+__global__ void sym_lower_rev(int64_t n, const T *M, int64_t ldm, T scale, T *D, int64_t ldd,
+                              int64_t sM, int64_t sD)
 {
+    GLA_BATCH(M, sM);
+    GLA_BATCH(D, sD);
     GLA_ROW(i);
     GLA_COLS(j, n)
     if (i < n) {
@@ -71,8 +79,11 @@ __global__ void sym_lower_rev(int64_t n, const T *M, int64_t ldm, T scale, T *D,
 
 // D = J Sᵀ J: D[i][j] = S[n-1-j][n-1-i], n×n.
 template <typename T>
-__global__ void rev_transpose(int64_t n, const T *S, int64_t lds, T *D, int64_t ldd)
+__global__ void rev_transpose(int64_t n, const T *S, int64_t lds, T *D, int64_t ldd, int64_t sS,
+                              int64_t sD)
 {
+    GLA_BATCH(S, sS);
+    GLA_BATCH(D, sD);
     GLA_ROW(i);
     GLA_COLS(j, n) if (i < n) D[i + j * ldd] = S[(n - 1 - j) + (n - 1 - i) * lds];
 }
@@ -81,8 +92,11 @@ __global__ void rev_transpose(int64_t n, const T *S, int64_t lds, T *D, int64_t 
 // transpose = 1: Lr = J Lᵀ J of a lower L, Lr[i][j] = L[n-1-j][n-1-i];
 // transpose = 0: Kr = J R J of an upper R,  Kr[i][j] = R[n-1-i][n-1-j].
 template <typename T>
-__global__ void rev_lower(int64_t n, const T *S, int64_t lds, T *D, int64_t ldd, int transpose)
+__global__ void rev_lower(int64_t n, const T *S, int64_t lds, T *D, int64_t ldd, int transpose,
+                          int64_t sS, int64_t sD)
 {
+    GLA_BATCH(S, sS);
+    GLA_BATCH(D, sD);
     GLA_ROW(i);
     GLA_COLS(j, n)
     if (i < n && i >= j)
@@ -95,8 +109,10 @@ __global__ void rev_lower(int64_t n, const T *S, int64_t lds, T *D, int64_t ldd,
 // lower_only: 2Z below the diagonal, Z on it, 0 above.
 template <typename T>
 __global__ void chol_adj_out(int64_t n, const T *R, int64_t ldr, T *Ab, int64_t ldab,
-                             int lower_only)
+                             int lower_only, int64_t sR, int64_t sAb)
 {
+    GLA_BATCH(R, sR);
+    GLA_BATCH(Ab, sAb);
     GLA_ROW(i);
     GLA_COLS(j, n)
     if (i < n) {
@@ -117,13 +133,14 @@ __global__ void qr_adj_top(int64_t m, int64_t n, const T *R, int64_t ldr, T *Ab,
 
 #undef GLA_COLS
 #undef GLA_ROW
+#undef GLA_BATCH
 
 template <typename T>
 int launch_tri_copy(hipStream_t s, int64_t n, const T *S, int64_t lds, T *D, int src_lower,
-                    int trans)
+                    int trans, int64_t sS = 0, int64_t batch = 1)
 {
-    hipLaunchKernelGGL((tri_copy<T>), g2(n, n), dim3(256), 0, s, n, S, lds, D, n, src_lower,
-                       trans);
+    hipLaunchKernelGGL((tri_copy<T>), g2(n, n, batch), dim3(256), 0, s, n, S, lds, D, n,
+                       src_lower, trans, sS, n * n);
     GLA_LAUNCH_CHECK();
     return GLA_OK;
 }
@@ -145,17 +162,63 @@ int chol_adjoint_dev(hipStream_t s, int64_t n, const T *L, int64_t ldl, const T 
     // W (lower) = Lt · Gtᵀ = Lᵀ L̄, k from each tile's first row
     GLA_TRY(gemm<T>(s, 'N', 'T', 1, n, n, n, T(1), Lt, n, Gt, n, T(0), W, n, 0, nullptr, 0, 1));
     // Lt <- ½ copyltu(W) J
-    hipLaunchKernelGGL((sym_lower_rev<T>), g2(n, n), dim3(256), 0, s, n, W, n, T(0.5), Lt, n);
+    hipLaunchKernelGGL((sym_lower_rev<T>), g2(n, n), dim3(256), 0, s, n, W, n, T(0.5), Lt, n,
+                       (int64_t)0, (int64_t)0);
     GLA_LAUNCH_CHECK();
-    hipLaunchKernelGGL((rev_lower<T>), g2(n, n), dim3(256), 0, s, n, L, ldl, Lr, n, 1);
+    hipLaunchKernelGGL((rev_lower<T>), g2(n, n), dim3(256), 0, s, n, L, ldl, Lr, n, 1, (int64_t)0,
+                       (int64_t)0);
     GLA_LAUNCH_CHECK();
     GLA_TRY(tile_inverses<T>(s, n, Lr, n, Linv));
     GLA_TRY(trsm_rlt<T>(s, n, n, Lr, n, Linv, Lt, n));        // Res1 = Psym J Lr⁻ᵀ
-    hipLaunchKernelGGL((rev_transpose<T>), g2(n, n), dim3(256), 0, s, n, Lt, n, W, n);
+    hipLaunchKernelGGL((rev_transpose<T>), g2(n, n), dim3(256), 0, s, n, Lt, n, W, n, (int64_t)0,
+                       (int64_t)0);
     GLA_LAUNCH_CHECK();
     GLA_TRY(trsm_rlt<T>(s, n, n, Lr, n, Linv, W, n));         // Res2 = J Res1ᵀ J Lr⁻ᵀ
     hipLaunchKernelGGL((chol_adj_out<T>), g2(n, n), dim3(256), 0, s, n, W, n, Ab, ldab,
-                       lower_only);
+                       lower_only, (int64_t)0, (int64_t)0);
+    GLA_LAUNCH_CHECK();
+    return GLA_OK;
+}
+
+// Per-matrix scratch of the batched form (n ≤ 128): L, Lᵀ, L̄ᵀ, W, Lr and
+// the tile inverses.
+size_t chol_adjoint_batched_scratch(int64_t n)
+{
+    return 5 * (size_t)n * n + tile_inverse_elems(n);
+}
+
+// Ā for a strided batch of SPD matrices of order n ≤ 128, recomputing each
+// factor L from A: the single-matrix sequence above with every step one
+// launch over the whole batch.  info[b] = first failing pivot of matrix b.
+template <typename T>
+int chol_adjoint_batched_dev(hipStream_t s, int64_t n, int64_t batch, const T *A, int64_t lda,
+                             int64_t sA, const T *Lb, int64_t ldlb, int64_t sLb, T *Ab,
+                             int64_t ldab, int64_t sAb, int lower_only, int *info, T *scratch)
+{
+    if (n <= 0 || batch <= 0) return GLA_OK;
+    const int64_t nn = n * n, sl = (int64_t)tile_inverse_elems(n);
+    T *Lw = scratch, *Lt = Lw + batch * nn, *Gt = Lt + batch * nn, *W = Gt + batch * nn;
+    T *Lr = W + batch * nn, *Linv = Lr + batch * nn;
+    GLA_TRY(launch_tri_copy<T>(s, n, A, lda, Lw, 1, 0, sA, batch));   // L <- tril(A)
+    GLA_TRY(chol_factor_batched<T>(s, (int)n, Lw, n, nn, info, Linv, sl, batch));
+    GLA_TRY(launch_tri_copy<T>(s, n, Lw, n, Lt, 1, 1, nn, batch));    // Lt = Lᵀ
+    GLA_TRY(launch_tri_copy<T>(s, n, Lb, ldlb, Gt, 1, 1, sLb, batch)); // Gt = tril(L̄)ᵀ
+    GLA_TRY(gemm_batched<T>(s, 'N', 'T', 1, n, n, n, T(1), Lt, n, nn, Gt, n, nn, T(0), W, n, nn,
+                            (int)batch, 1));                          // W = Lᵀ L̄ (lower)
+    hipLaunchKernelGGL((sym_lower_rev<T>), g2(n, n, batch), dim3(256), 0, s, n, W, n, T(0.5), Lt,
+                       n, nn, nn);
+    GLA_LAUNCH_CHECK();
+    hipLaunchKernelGGL((rev_lower<T>), g2(n, n, batch), dim3(256), 0, s, n, Lw, n, Lr, n, 1, nn,
+                       nn);
+    GLA_LAUNCH_CHECK();
+    GLA_TRY(tile_inverses_batched<T>(s, n, Lr, n, nn, Linv, sl, batch));
+    GLA_TRY(trsm_rlt_batched<T>(s, n, (int)n, Lr, n, nn, Linv, sl, Lt, n, nn, batch));
+    hipLaunchKernelGGL((rev_transpose<T>), g2(n, n, batch), dim3(256), 0, s, n, Lt, n, W, n, nn,
+                       nn);
+    GLA_LAUNCH_CHECK();
+    GLA_TRY(trsm_rlt_batched<T>(s, n, (int)n, Lr, n, nn, Linv, sl, W, n, nn, batch));
+    hipLaunchKernelGGL((chol_adj_out<T>), g2(n, n, batch), dim3(256), 0, s, n, W, n, Ab, ldab,
+                       lower_only, nn, sAb);
     GLA_LAUNCH_CHECK();
     return GLA_OK;
 }
@@ -179,9 +242,11 @@ int qr_adjoint_dev(hipStream_t s, int64_t m, int64_t n, const T *F, int64_t ldf,
     GLA_TRY(launch_tri_copy<T>(s, n, Rb, ldrb, Gz, 0, 0));   // triu(R̄)
     // W (lower) = R R̄ᵀ
     GLA_TRY(gemm<T>(s, 'N', 'T', 1, n, n, n, T(1), Rz, n, Gz, n, T(0), W, n, 0, nullptr, 0, 1));
-    hipLaunchKernelGGL((sym_lower_rev<T>), g2(n, n), dim3(256), 0, s, n, W, n, T(1), Gz, n);
+    hipLaunchKernelGGL((sym_lower_rev<T>), g2(n, n), dim3(256), 0, s, n, W, n, T(1), Gz, n,
+                       (int64_t)0, (int64_t)0);
     GLA_LAUNCH_CHECK();
-    hipLaunchKernelGGL((rev_lower<T>), g2(n, n), dim3(256), 0, s, n, Rz, n, Kr, n, 0);
+    hipLaunchKernelGGL((rev_lower<T>), g2(n, n), dim3(256), 0, s, n, Rz, n, Kr, n, 0, (int64_t)0,
+                       (int64_t)0);
     GLA_LAUNCH_CHECK();
     GLA_TRY(tile_inverses<T>(s, n, Kr, n, Linv));
     GLA_TRY(trsm_rlt<T>(s, n, n, Kr, n, Linv, Gz, n));        // copyltu(M) J Kr⁻ᵀ
@@ -197,6 +262,13 @@ int qr_adjoint_dev(hipStream_t s, int64_t m, int64_t n, const T *F, int64_t ldf,
 namespace {
 
 using gla::Handle;
+
+// (a kernel rather than a memset, so the call can be captured in a graph)
+__global__ void zero_ints(int64_t n, int *p)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0;
+}
 
 template <typename T>
 int chol_adjoint_entry(gla_handle_t hh, char conv, int64_t n, const T *L, int64_t ldl,
@@ -214,6 +286,45 @@ int chol_adjoint_entry(gla_handle_t hh, char conv, int64_t n, const T *L, int64_
     GLA_TRY(gla::ensure_workspace(h, gla::chol_adjoint_scratch(n) * sizeof(T)));
     return gla::chol_adjoint_dev<T>(reinterpret_cast<hipStream_t>(stream), n, L, ldl, Lb, ldlb,
                                     Ab, ldab, conv == 'L', reinterpret_cast<T *>(h->ws));
+}
+
+template <typename T>
+int chol_adjoint_batched_entry(gla_handle_t hh, char conv, int64_t n, int64_t batch, const T *A,
+                               int64_t lda, int64_t sA, const T *Lb, int64_t ldlb, int64_t sLb,
+                               T *Ab, int64_t ldab, int64_t sAb, int32_t *info, void *stream)
+{
+    Handle *h = reinterpret_cast<Handle *>(hh);
+    if (!h) { gla::set_error("null handle"); return -1; }
+    if (conv != 'S' && conv != 'L') { gla::set_error("convention must be 'S' or 'L'"); return -2; }
+    if (n < 0) return -3;
+    if (n > 128) {
+        gla::set_error("batched Cholesky adjoint: order %lld exceeds 128", (long long)n);
+        return GLA_ERR_UNSUPPORTED;
+    }
+    if (batch < 0) return -4;
+    if (lda < std::max<int64_t>(1, n)) return -6;
+    if (ldlb < std::max<int64_t>(1, n)) return -9;
+    if (ldab < std::max<int64_t>(1, n)) return -12;
+    if (n == 0 || batch == 0) return GLA_OK;
+    GLA_HIP_TRY(hipSetDevice(h->device));
+    hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+    // chunks of at most 65535 matrices (one grid dimension each)
+    const int64_t chunk = std::min<int64_t>(batch, 65535);
+    const size_t per = gla::chol_adjoint_batched_scratch(n) * sizeof(T);
+    const size_t data = ((size_t)chunk * per + 255) / 256 * 256;
+    GLA_TRY(gla::ensure_workspace(h, data + 4 * (size_t)batch + 256));
+    int *dinfo = info ? reinterpret_cast<int *>(info)
+                      : reinterpret_cast<int *>(reinterpret_cast<char *>(h->ws) + data);
+    hipLaunchKernelGGL(zero_ints, dim3((unsigned)gla::cdiv(batch, 256)), dim3(256), 0, s, batch,
+                       dinfo);
+    GLA_LAUNCH_CHECK();
+    for (int64_t b0 = 0; b0 < batch; b0 += chunk) {
+        const int64_t nb = std::min<int64_t>(chunk, batch - b0);
+        GLA_TRY(gla::chol_adjoint_batched_dev<T>(s, n, nb, A + b0 * sA, lda, sA, Lb + b0 * sLb,
+                                                 ldlb, sLb, Ab + b0 * sAb, ldab, sAb, conv == 'L',
+                                                 dinfo + b0, reinterpret_cast<T *>(h->ws)));
+    }
+    return GLA_OK;
 }
 
 template <typename T>
@@ -249,6 +360,22 @@ int gla_chol_adjoint_s(gla_handle_t h, char conv, int64_t n, const float *L, int
                        void *stream)
 {
     return chol_adjoint_entry<float>(h, conv, n, L, ldl, Lbar, ldlbar, Abar, ldabar, stream);
+}
+int gla_chol_adjoint_batched_d(gla_handle_t h, char conv, int64_t n, int64_t batch,
+                               const double *A, int64_t lda, int64_t strideA, const double *Lbar,
+                               int64_t ldlbar, int64_t strideLbar, double *Abar, int64_t ldabar,
+                               int64_t strideAbar, int32_t *info, void *stream)
+{
+    return chol_adjoint_batched_entry<double>(h, conv, n, batch, A, lda, strideA, Lbar, ldlbar,
+                                              strideLbar, Abar, ldabar, strideAbar, info, stream);
+}
+int gla_chol_adjoint_batched_s(gla_handle_t h, char conv, int64_t n, int64_t batch,
+                               const float *A, int64_t lda, int64_t strideA, const float *Lbar,
+                               int64_t ldlbar, int64_t strideLbar, float *Abar, int64_t ldabar,
+                               int64_t strideAbar, int32_t *info, void *stream)
+{
+    return chol_adjoint_batched_entry<float>(h, conv, n, batch, A, lda, strideA, Lbar, ldlbar,
+                                             strideLbar, Abar, ldabar, strideAbar, info, stream);
 }
 int gla_qr_adjoint_d(gla_handle_t h, int64_t m, int64_t n, const double *F, int64_t ldf,
                      const double *tau, const double *Rbar, int64_t ldrbar, double *Abar,

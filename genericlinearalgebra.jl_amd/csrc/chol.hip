@@ -382,12 +382,40 @@ __global__ __launch_bounds__(kPanelThreads) void chol_factor_f32io_kernel(int nb
     }
 }
 
+// A strided batch of diagonal-block factorisations (nb ≤ 128 each), one
+// workgroup per matrix: matrix b at A + b·sA, tile inverses at LinvG + b·sLinv,
+// its first failing pivot in info[b] (info zeroed by the caller).
+template <typename T>
+__global__ __launch_bounds__(kPanelThreads) void chol_factor_batched_kernel(int nb, T *A,
+                                                                            int64_t lda,
+                                                                            int64_t sA, int *info,
+                                                                            T *LinvG,
+                                                                            int64_t sLinv)
+{
+    __shared__ T Xb[2 * kPanelNB * kXLD];
+    __shared__ T Db[2 * 16 * kXLD];
+    __shared__ T Lib[2 * 16 * kXLD];
+    const int64_t b = blockIdx.x;
+    T *Ab = A + b * sA;
+    T *Lg = LinvG + b * sLinv;
+    int *ib = info + b;
+    switch (threadIdx.x >> 6) {
+    case 0: factor_body<T, 0>(nb, Ab, lda, ib, 0, Lg, Xb, Db, Lib); break;
+    case 1: factor_body<T, 1>(nb, Ab, lda, ib, 0, Lg, Xb, Db, Lib); break;
+    case 2: factor_body<T, 2>(nb, Ab, lda, ib, 0, Lg, Xb, Db, Lib); break;
+    default: factor_body<T, 3>(nb, Ab, lda, ib, 0, Lg, Xb, Db, Lib); break;
+    }
+}
+
 // Inverses of the 16×16 diagonal tiles of a lower n×n L (tiles past n are
 // padded with the identity); one wave per tile.
 template <typename T>
 __global__ __launch_bounds__(256) void tile_inverse_kernel(int64_t n, const T *__restrict__ L,
-                                                           int64_t ldl, T *__restrict__ LinvG)
+                                                           int64_t ldl, T *__restrict__ LinvG,
+                                                           int64_t sL = 0, int64_t sLinv = 0)
 {
+    L += blockIdx.y * sL;          // strided batch along grid y
+    LinvG += blockIdx.y * sLinv;
     const int64_t tile = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
     if (tile * 16 >= n) return;
     const int lane = threadIdx.x & 63;
@@ -765,6 +793,36 @@ int trsm_rlt(hipStream_t s, int64_t m, int64_t n, const T *L, int64_t ldl, const
 {
     return trsm_rlt_dev<T>(s, m, n, L, ldl, Linv, B, ldb);
 }
+template <typename T>
+int chol_factor_batched(hipStream_t s, int nb, T *A, int64_t lda, int64_t sA, int *info, T *LinvG,
+                        int64_t sLinv, int64_t batch)
+{
+    if (batch <= 0 || nb <= 0) return GLA_OK;
+    hipLaunchKernelGGL((chol_factor_batched_kernel<T>), dim3((unsigned)batch), dim3(kPanelThreads),
+                       0, s, nb, A, lda, sA, info, LinvG, sLinv);
+    GLA_LAUNCH_CHECK();
+    return GLA_OK;
+}
+template <typename T>
+int tile_inverses_batched(hipStream_t s, int64_t n, const T *L, int64_t ldl, int64_t sL, T *LinvG,
+                          int64_t sLinv, int64_t batch)
+{
+    if (n <= 0 || batch <= 0) return GLA_OK;
+    hipLaunchKernelGGL((tile_inverse_kernel<T>), dim3((unsigned)cdiv(cdiv(n, 16), 4), (unsigned)batch),
+                       dim3(256), 0, s, n, L, ldl, LinvG, sL, sLinv);
+    GLA_LAUNCH_CHECK();
+    return GLA_OK;
+}
+template <typename T>
+int trsm_rlt_batched(hipStream_t s, int64_t m, int n, const T *L, int64_t ldl, int64_t sL,
+                     const T *Linv, int64_t sLinv, T *B, int64_t ldb, int64_t sB, int64_t batch)
+{
+    if (n > kPanelNB) {
+        set_error("trsm_rlt_batched: n = %d exceeds %d", n, kPanelNB);
+        return GLA_ERR_UNSUPPORTED;
+    }
+    return launch_solve_batched<T>(s, n, L, ldl, sL, Linv, sLinv, B, ldb, sB, m, batch);
+}
 size_t tile_inverse_elems(int64_t n) { return linv_elems(n); }
 
 // Inverses of the 128×128 diagonal blocks of a lower n×n L, transposed:
@@ -798,6 +856,12 @@ int diag_block_inverses(hipStream_t s, int64_t n, const T *L, int64_t ldl, const
     template int tile_inverses<T>(hipStream_t, int64_t, const T *, int64_t, T *);            \
     template int trsm_rlt<T>(hipStream_t, int64_t, int64_t, const T *, int64_t, const T *,   \
                              T *, int64_t); \
+    template int chol_factor_batched<T>(hipStream_t, int, T *, int64_t, int64_t, int *, T *,  \
+                                        int64_t, int64_t);                                    \
+    template int tile_inverses_batched<T>(hipStream_t, int64_t, const T *, int64_t, int64_t,  \
+                                          T *, int64_t, int64_t);                             \
+    template int trsm_rlt_batched<T>(hipStream_t, int64_t, int, const T *, int64_t, int64_t,  \
+                                     const T *, int64_t, T *, int64_t, int64_t, int64_t);     \
     template int diag_block_inverses<T>(hipStream_t, int64_t, const T *, int64_t, const T *, T *);
 
 GLA_INST_BLOCKS(double)

@@ -81,6 +81,8 @@ struct GemmP {
     int64_t kc;   // K extent of one split (split-K along blockIdx.y)
     T *ws;        // split-K partials: ws[z][M×N], or null for a direct epilogue
     int kfrom_row;  // 1: the K range of a tile starts at its first row (op(A)(i,k) = 0 for k < i)
+    int batch;      // strided batch along blockIdx.z (1: a single product)
+    int64_t sA, sB, sC;  // batch strides (elements)
 };
 
 // One operand slab: BX (M or N extent) × kBK, staged through registers into
@@ -224,8 +226,9 @@ __global__ __launch_bounds__(kThreads, 2) void gemm_kernel(GemmP<T> p)
     const int64_t koff = (p.kfrom_row ? m0 : 0) + (int64_t)blockIdx.y * p.kc;
     int64_t K = p.K - koff < p.kc ? p.K - koff : p.kc;
     if (K < 0) K = 0;
-    const T *Ap = p.A + (TA ? koff : koff * p.lda);
-    const T *Bp = p.B + (TB ? koff * p.ldb : koff);
+    const int64_t bz = blockIdx.z;
+    const T *Ap = p.A + bz * p.sA + (TA ? koff : koff * p.lda);
+    const T *Bp = p.B + bz * p.sB + (TB ? koff * p.ldb : koff);
 
     // Accumulator element r of block (i,j) is C(row, col) with col = lane%16
     // inside the 16×16 block and row = 4*(lane/16) + r for the f32
@@ -293,7 +296,7 @@ __global__ __launch_bounds__(kThreads, 2) void gemm_kernel(GemmP<T> p)
     // Epilogue.
     const T alpha = split ? T(1) : p.alpha, beta = split ? T(0) : p.beta;
     const bool beta0 = beta == T(0);
-    T *const Cp = split ? p.ws + (int64_t)blockIdx.y * p.M * p.N : p.C;
+    T *const Cp = split ? p.ws + (int64_t)blockIdx.y * p.M * p.N : p.C + bz * p.sC;
     const int64_t ldc = split ? p.M : p.ldc;
 #pragma unroll
     for (int i = 0; i < MI; ++i)
@@ -362,8 +365,14 @@ int launch(hipStream_t s, GemmP<T> p, int splits)
         p.kc = cdiv(cdiv(p.K, splits), kBK) * kBK;
         splits = (int)cdiv(p.K, p.kc);
     }
-    hipLaunchKernelGGL((gemm_kernel<T, BM, BN, TA, TB, MODE, VEC>), dim3(8 * p.per_xcd, splits),
-                       dim3(kThreads), 0, s, p);
+    if (p.batch > 1) {
+        p.ws = nullptr;   // split-K partials are single-product scratch
+        p.kc = p.K > 0 ? p.K : 1;
+        splits = 1;
+    }
+    hipLaunchKernelGGL((gemm_kernel<T, BM, BN, TA, TB, MODE, VEC>),
+                       dim3(8 * p.per_xcd, splits, p.batch > 1 ? p.batch : 1), dim3(kThreads), 0,
+                       s, p);
     GLA_LAUNCH_CHECK();
     if (p.ws) {
         hipLaunchKernelGGL((splitk_reduce_kernel<T, MODE>), dim3((unsigned)cdiv(p.M, 256), p.N),
@@ -433,11 +442,42 @@ int gemm(hipStream_t s, char ta, char tb, int mode, int64_t M, int64_t N, int64_
     p.beta = beta;
     p.diag_off = diag_off;
     p.ws = ws_elems > 0 ? ws : nullptr;
+    p.batch = 1;
     constexpr int VW = 16 / sizeof(T);
     const bool aligned = ((uintptr_t)A % 16 == 0) && ((uintptr_t)B % 16 == 0) &&
                          (lda % VW == 0) && (ldb % VW == 0);
     if (aligned) return dispatch<T, VW>(s, ta, tb, mode, p, ws_elems);
     return dispatch<T, 1>(s, ta, tb, mode, p, ws_elems);
+}
+
+template <typename T>
+int gemm_batched(hipStream_t s, char ta, char tb, int mode, int64_t M, int64_t N, int64_t K,
+                 T alpha, const T *A, int64_t lda, int64_t sA, const T *B, int64_t ldb,
+                 int64_t sB, T beta, T *C, int64_t ldc, int64_t sC, int batch, int kfrom_row)
+{
+    if (M <= 0 || N <= 0 || batch <= 0) return GLA_OK;
+    GemmP<T> p{};
+    p.kfrom_row = kfrom_row;
+    p.M = M;
+    p.N = N;
+    p.K = K < 0 ? 0 : K;
+    p.A = A;
+    p.lda = lda;
+    p.B = B;
+    p.ldb = ldb;
+    p.C = C;
+    p.ldc = ldc;
+    p.alpha = alpha;
+    p.beta = beta;
+    p.batch = batch;
+    p.sA = sA;
+    p.sB = sB;
+    p.sC = sC;
+    constexpr int VW = 16 / sizeof(T);
+    const bool aligned = ((uintptr_t)A % 16 == 0) && ((uintptr_t)B % 16 == 0) &&
+                         (lda % VW == 0) && (ldb % VW == 0) && (sA % VW == 0) && (sB % VW == 0);
+    if (aligned) return dispatch<T, VW>(s, ta, tb, mode, p, 0);
+    return dispatch<T, 1>(s, ta, tb, mode, p, 0);
 }
 
 template int gemm<double>(hipStream_t, char, char, int, int64_t, int64_t, int64_t, double,
@@ -446,5 +486,11 @@ template int gemm<double>(hipStream_t, char, char, int, int64_t, int64_t, int64_
 template int gemm<float>(hipStream_t, char, char, int, int64_t, int64_t, int64_t, float,
                          const float *, int64_t, const float *, int64_t, float, float *,
                          int64_t, int64_t, float *, size_t, int);
+template int gemm_batched<double>(hipStream_t, char, char, int, int64_t, int64_t, int64_t, double,
+                                  const double *, int64_t, int64_t, const double *, int64_t,
+                                  int64_t, double, double *, int64_t, int64_t, int, int);
+template int gemm_batched<float>(hipStream_t, char, char, int, int64_t, int64_t, int64_t, float,
+                                 const float *, int64_t, int64_t, const float *, int64_t, int64_t,
+                                 float, float *, int64_t, int64_t, int, int);
 
 }  // namespace gla

@@ -81,6 +81,12 @@ template <typename T>
 int gemm(hipStream_t s, char ta, char tb, int mode, int64_t M, int64_t N, int64_t K, T alpha,
          const T *A, int64_t lda, const T *B, int64_t ldb, T beta, T *C, int64_t ldc,
          int64_t diag_off = 0, T *ws = nullptr, size_t ws_elems = 0, int kfrom_row = 0);
+// The same over a strided batch (grid z): product b reads A + b·sA, B + b·sB
+// and writes C + b·sC; no split-K.
+template <typename T>
+int gemm_batched(hipStream_t s, char ta, char tb, int mode, int64_t M, int64_t N, int64_t K,
+                 T alpha, const T *A, int64_t lda, int64_t sA, const T *B, int64_t ldb,
+                 int64_t sB, T beta, T *C, int64_t ldc, int64_t sC, int batch, int kfrom_row = 0);
 
 // ------------------------------------------------------------- Cholesky --
 // Linv: scratch for the 16×16 diagonal-tile inverses (tile_inverse_elems(n)
@@ -105,6 +111,17 @@ int chol_solve_block(hipStream_t s, int nb, const T *L, int64_t ldl, const T *Li
                      int64_t ldb, int64_t nrows);
 template <typename T>
 int tile_inverses(hipStream_t s, int64_t n, const T *L, int64_t ldl, T *LinvG);
+// Strided-batch forms for matrices of order ≤ 128 (one diagonal block):
+// factorisation with per-matrix info, 16×16 tile inverses, and B <- B L⁻ᵀ.
+template <typename T>
+int chol_factor_batched(hipStream_t s, int nb, T *A, int64_t lda, int64_t sA, int *info, T *LinvG,
+                        int64_t sLinv, int64_t batch);
+template <typename T>
+int tile_inverses_batched(hipStream_t s, int64_t n, const T *L, int64_t ldl, int64_t sL, T *LinvG,
+                          int64_t sLinv, int64_t batch);
+template <typename T>
+int trsm_rlt_batched(hipStream_t s, int64_t m, int n, const T *L, int64_t ldl, int64_t sL,
+                     const T *Linv, int64_t sLinv, T *B, int64_t ldb, int64_t sB, int64_t batch);
 template <typename T>
 int trsm_rlt(hipStream_t s, int64_t m, int64_t n, const T *L, int64_t ldl, const T *Linv, T *B,
              int64_t ldb);

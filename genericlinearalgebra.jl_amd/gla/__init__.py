@@ -637,6 +637,48 @@ def chol_adjoint(L, Lbar, convention: str = "S", out=None):
     return out
 
 
+def _batch3(X, name):
+    """(batch, n, n) device tensor of column-major matrices: returns (ld, stride)."""
+    if not isinstance(X, torch.Tensor) or X.dim() != 3 or not X.is_cuda:
+        raise TypeError(f"{name} must be a 3-D device tensor (batch, n, n)")
+    b, m, n = X.shape
+    if m > 1 and X.stride(1) != 1:
+        raise TypeError(f"{name}: each matrix must be column-major (stride(1) == 1)")
+    return max(X.stride(2) if n > 1 else m, 1), (X.stride(0) if b > 1 else m * n)
+
+
+def chol_adjoint_batched(A, Lbar, convention: str = "S", out=None):
+    """Ā for a batch of SPD matrices A[b] (order n ≤ 128, lower triangles
+    read) and cotangents L̄[b], recomputing each factor L[b] = chol(A[b]):
+    the chol_adjoint of every matrix, in one pass of batched launches.
+    Tensors are (batch, n, n) with column-major matrices.  Raises
+    DomainError naming the first matrix whose factorisation breaks down."""
+    if convention not in ("S", "L"):
+        raise ArgumentError("convention must be 'S' or 'L'")
+    lda, sa = _batch3(A, "A")
+    ldg, sg = _batch3(Lbar, "L̄")
+    b, n, n2 = A.shape
+    if n != n2 or Lbar.shape != A.shape:
+        raise DimensionMismatch("A and L̄ must be (batch, n, n)")
+    if Lbar.dtype != A.dtype:
+        raise TypeError("A and L̄ must have one element type")
+    if out is None:
+        out = torch.empty((b, n, n), dtype=A.dtype, device=A.device).transpose(1, 2)
+    ldo, so = _batch3(out, "Ā")
+    info = torch.zeros(max(b, 1), dtype=torch.int32, device=A.device)
+    with torch.cuda.device(A.device):
+        rc = getattr(lib(), f"gla_chol_adjoint_batched_{_sfx(A)}")(
+            _handle(A.device.index), convention.encode(), n, b, C.c_void_p(A.data_ptr()), lda, sa,
+            C.c_void_p(Lbar.data_ptr()), ldg, sg, C.c_void_p(out.data_ptr()), ldo, so,
+            C.c_void_p(info.data_ptr()), _stream(A))
+    _check(rc, "gla_chol_adjoint_batched")
+    bad = torch.nonzero(info[:b]).flatten()
+    if bad.numel():
+        k = int(bad[0])
+        raise DomainError(f"matrix {k}: sqrt of a negative pivot at position {int(info[k])}")
+    return out
+
+
 def qr_adjoint(F, tau, Rbar, out=None):
     """Ā (m×n, m ≥ n) of A = QR stored as (F, τ) by qrBlocked!/qrUnblocked!,
     for the cotangent R̄ (upper part read) and Q̄ = 0:

@@ -214,6 +214,20 @@ int gla_chol_adjoint_d(gla_handle_t h, char conv, int64_t n, const double *L, in
 int gla_chol_adjoint_s(gla_handle_t h, char conv, int64_t n, const float *L, int64_t ldl,
                        const float *Lbar, int64_t ldlbar, float *Abar, int64_t ldabar,
                        void *stream);
+/* The same for a strided batch of `batch` SPD matrices of order n ≤ 128,
+ * recomputing each factor: matrix b is A + b·strideA (lower triangle read),
+ * its cotangent Lbar + b·strideLbar, its Ā written to Abar + b·strideAbar.
+ * info (device, `batch` int32, or NULL): 0, or the first non-positive pivot
+ * of that matrix (its Ā is then not meaningful).  Every step of the
+ * single-matrix sequence is one launch over the batch. */
+int gla_chol_adjoint_batched_d(gla_handle_t h, char conv, int64_t n, int64_t batch,
+                               const double *A, int64_t lda, int64_t strideA, const double *Lbar,
+                               int64_t ldlbar, int64_t strideLbar, double *Abar, int64_t ldabar,
+                               int64_t strideAbar, int32_t *info, void *stream);
+int gla_chol_adjoint_batched_s(gla_handle_t h, char conv, int64_t n, int64_t batch,
+                               const float *A, int64_t lda, int64_t strideA, const float *Lbar,
+                               int64_t ldlbar, int64_t strideLbar, float *Abar, int64_t ldabar,
+                               int64_t strideAbar, int32_t *info, void *stream);
 /* QR (m ≥ n), as stored by qrBlocked!/qrUnblocked! (F: R on and above the
  * diagonal, reflectors below; τ), with Q̄ = 0 and R̄ (n×n, upper; strict
  * lower not read): Ā (m×n) = Q [copyltu(R R̄ᵀ) R⁻ᵀ ; 0], copyltu(M) =

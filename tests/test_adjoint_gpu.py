@@ -46,6 +46,55 @@ def test_chol_adjoint_vs_oracle(dtype, n, conv):
         assert not np.triu(got, 1).any()
 
 
+def _batch_tensor(mats, tdt):
+    """(batch, n, n) device tensor holding column-major matrices."""
+    b, n = len(mats), mats[0].shape[0]
+    T = torch.empty((b, n, n), dtype=tdt, device="cuda").transpose(1, 2)
+    T.copy_(torch.from_numpy(np.stack(mats)).to(tdt))
+    return T
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+@pytest.mark.parametrize("n", [1, 16, 37, 128])
+@pytest.mark.parametrize("conv", ["S", "L"])
+def test_chol_adjoint_batched_vs_oracle(dtype, n, conv):
+    """Batched form (factor recomputed from A) against the oracle applied to
+    each matrix's LAPACK factor; strict upper parts of A and L̄ hold junk."""
+    gla = _gla()
+    tdt = torch.float64 if dtype == np.float64 else torch.float32
+    rng = np.random.default_rng(100 + n + (7 if conv == "L" else 0))
+    batch = 9
+    As = [_spd(n, rng) for _ in range(batch)]
+    Ws = [rng.standard_normal((n, n)) for _ in range(batch)]
+    Aj = [np.tril(a) + np.triu(np.full((n, n), 1e3), 1) for a in As]
+    Wj = [np.tril(w) + np.triu(np.full((n, n), -3.0), 1) for w in Ws]
+    got = gla.chol_adjoint_batched(_batch_tensor(Aj, tdt), _batch_tensor(Wj, tdt), conv)
+    got = got.cpu().numpy()
+    for b in range(batch):
+        L = np.linalg.cholesky(As[b].astype(dtype).astype(np.float64))
+        ref = O.chol_adjoint(L, np.tril(Ws[b].astype(dtype).astype(np.float64)), conv)
+        assert rel(got[b], ref) <= 50 * n * eps_of(dtype), (b, rel(got[b], ref))
+
+
+def test_chol_adjoint_batched_reports_failing_matrix():
+    gla = _gla()
+    rng = np.random.default_rng(5)
+    n, batch = 40, 6
+    As = [_spd(n, rng) for _ in range(batch)]
+    As[3][10, 10] = -1.0
+    W = [rng.standard_normal((n, n)) for _ in range(batch)]
+    with pytest.raises(gla.DomainError, match="matrix 3"):
+        gla.chol_adjoint_batched(_batch_tensor(As, torch.float64), _batch_tensor(W, torch.float64))
+
+
+def test_chol_adjoint_batched_rejects_large_order():
+    gla = _gla()
+    A = torch.eye(129, dtype=torch.float64, device="cuda").expand(2, 129, 129).contiguous()
+    A = A.transpose(1, 2)
+    with pytest.raises(RuntimeError):
+        gla.chol_adjoint_batched(A, A.clone())
+
+
 @pytest.mark.parametrize("dtype", [np.float64, np.float32])
 @pytest.mark.parametrize("m,n", [(1, 1), (10, 6), (128, 128), (300, 129), (513, 200), (700, 256)])
 def test_qr_adjoint_vs_oracle(dtype, m, n):
