@@ -83,11 +83,11 @@ def cpu_baseline():
     }
 
 
-def panel_traffic():
-    """HBM bytes per panel launch measured with rocprofv3 --pmc FETCH_SIZE and
-    WRITE_SIZE (profiles/qr_panel_8192x128.md records the runs); None if the
-    summary is absent."""
-    path = os.path.join(ROOT, "profiles", "qr_panel_8192x128.md")
+def profiled_traffic(name):
+    """HBM bytes per launch measured with rocprofv3 --pmc FETCH_SIZE and
+    WRITE_SIZE, as recorded in profiles/<name> (qr_panel_8192x128.md,
+    syrk_15360_k512.md); None if the summary is absent."""
+    path = os.path.join(ROOT, "profiles", name)
     try:
         for line in open(path):
             if line.startswith("traffic_bytes_per_launch:"):
@@ -240,6 +240,10 @@ def main():
     ap.add_argument("--profile-panel", action="store_true",
                     help="only run the QR panel launch (8192x128) and a plain copy of the "
                          "same panel five times each, for hardware-counter runs")
+    ap.add_argument("--profile-syrk", action="store_true",
+                    help="only run the trailing SYRK of the Cholesky's first look-ahead step "
+                         "(C 15360x15360 lower -= X X^T, K=512) and a plain 64 MiB copy five "
+                         "times each, for hardware-counter runs")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
@@ -252,6 +256,18 @@ def main():
 
     dev = torch.device("cuda:0")
     torch.manual_seed(0)
+    if args.profile_syrk:
+        kk, mm = 512, N_CHOL - 2 * 512
+        X = torch.randn(kk, mm, dtype=torch.float64, device=dev).t()
+        Cs = torch.randn(mm, mm, dtype=torch.float64, device=dev).t()
+        Cc = torch.empty(8 * 1024 * 1024, dtype=torch.float64, device=dev)
+        Cd = torch.empty_like(Cc)
+        for _ in range(5):
+            Cd.copy_(Cc)
+            gla.rankUpdate_(gla.Hermitian(Cs, "L"), X, -1.0)
+        torch.cuda.synchronize()
+        print(json.dumps({"profile_syrk": f"{mm}x{mm} K={kk}", "launches": 5}))
+        return
     if args.profile_panel:
         P0 = torch.randn(NB_QR, M_QR, dtype=torch.float64, device=dev).t()
         Pw = torch.empty_like(P0)
@@ -323,10 +339,10 @@ def main():
     qr_bwd = ((QRx - M0 @ x).norm() / (M0 @ x).norm()).item()
     del R, QRx
 
-    # roofline of the dominant kernel, the QR panel (qr_panel2_kernel: the
-    # largest share of the step's kernel time): one launch on a fresh
-    # 8192×128 panel (qrUnblocked! of the panel is exactly that launch),
-    # algorithmic bytes 16·m·nb (read + write of the panel)
+    # the QR panel (qr_panel2_kernel, the second largest share of the step's
+    # kernel time): one launch on a fresh 8192×128 panel (qrUnblocked! of the
+    # panel is exactly that launch), algorithmic bytes 16·m·nb (read + write
+    # of the panel)
     P0 = M0[:, :NB_QR].contiguous().t().contiguous().t()
     Pw = torch.empty_like(P0)
     ptau = torch.zeros(NB_QR, dtype=torch.float64, device=dev)
@@ -334,8 +350,11 @@ def main():
     t_panel = ev_time(torch, lambda: (Pw.copy_(P0), gla.qrUnblocked_(Pw, ptau)), reps=5) - t_pc
     panel_bytes = 16.0 * M_QR * NB_QR
     achieved = panel_bytes / t_panel / 1e9
-    # the MFMA engine, for reference: the trailing SYRK of the look-ahead
-    # driver's first step (C 15360×15360 lower -= A Aᵀ, K = 512)
+    # the dominant kernel (largest share of the step's kernel time): the
+    # MFMA SYRK gemm_kernel<double,128,128,N,T,lower> of the look-ahead
+    # Cholesky, timed on its first step's shape (C 15360×15360 lower -= X Xᵀ,
+    # K = 512; rankUpdate! of that shape is exactly one launch); algorithmic
+    # work mm²·K flop
     kk, mm = 512, n - 2 * 512
     Cs = torch.empty_like(A0)[:mm, :mm]
     Xs = A0[2 * kk:, :kk]
@@ -371,14 +390,18 @@ def main():
                "pct_peak": 100 * qr_flops(M_QR, N_QR) / (t_qr - t_copy_m) / 1e12 / FP64_PEAK_TFLOPS,
                "R_rel_diff_vs_rocsolver_sign_normalised": qr_diff,
                "bar_10_n_eps": 10 * M_QR * eps, "residual_QRx_vs_Mx": qr_bwd},
-        "roofline": {"bound": "hbm", "achieved": achieved, "peak": 8000.0, "unit": "GB/s",
-                     "frac": achieved / 8000.0, "traffic": panel_traffic(),
-                     "kernel": "qr_panel2_kernel<double,128,2,1024> (panel 8192x128, 64 workgroups)",
-                     "bytes_per_launch": panel_bytes, "us_per_launch": t_panel * 1e6},
-        "roofline_gemm": {"bound": "mfma", "achieved": syrk_tf, "peak": FP64_PEAK_TFLOPS,
-                          "unit": "TFLOP/s", "frac": syrk_tf / FP64_PEAK_TFLOPS,
-                          "kernel": "gemm_kernel<double,128,128,N,T,lower> (SYRK 15360², K=512)",
-                          "ms_per_launch": t_syrk * 1e3},
+        "roofline": {"bound": "mfma", "achieved": syrk_tf, "peak": FP64_PEAK_TFLOPS,
+                     "unit": "TFLOP/s", "frac": syrk_tf / FP64_PEAK_TFLOPS,
+                     "traffic": profiled_traffic("syrk_15360_k512.md"),
+                     "kernel": "gemm_kernel<double,128,128,N,T,lower> (SYRK 15360x15360, K=512)",
+                     "flops_per_launch": float(mm) * float(mm) * kk,
+                     "us_per_launch": t_syrk * 1e6},
+        "roofline_panel": {"bound": "hbm", "achieved": achieved, "peak": 8000.0, "unit": "GB/s",
+                           "frac": achieved / 8000.0,
+                           "traffic": profiled_traffic("qr_panel_8192x128.md"),
+                           "kernel": "qr_panel2_kernel<double,128,2,1024> (panel 8192x128, "
+                                     "64 workgroups)",
+                           "bytes_per_launch": panel_bytes, "us_per_launch": t_panel * 1e6},
     }
     del A0, Aw, M0, Mw
     torch.cuda.empty_cache()
