@@ -80,7 +80,48 @@ def cpu_baseline():
         "kind": "port",
         "sample": (f"oracle cholRecursive!(copy(A), Val{{:L}}) n={n} ({tc:.2f}s) + "
                    f"qrBlocked!(copy(M), 128) {m}x{k} ({tq:.2f}s), FP64, single thread"),
+        "lapack": lapack_line(),
     }
+
+
+def lapack_line():
+    """Host LAPACK (scipy's dpotrf + dgeqrf on its bundled OpenBLAS, all BLAS
+    threads) on the full bench step: Cholesky n = 16384 and QR 8192×4096,
+    each timed once (about 15 s of host time in all).  A second CPU line
+    beside the port."""
+    import numpy as np
+    from scipy.linalg import lapack
+
+    rng = np.random.default_rng(2)
+    n = N_CHOL
+    # SPD by diagonal dominance ((R + Rᵀ)/2 + n·I, eigenvalues ≥ n − 2√n):
+    # O(n²) to build, and LAPACK's time does not depend on the values
+    A = rng.standard_normal((n, n))
+    A += A.T
+    A *= 0.5
+    A[np.diag_indices(n)] += n
+    A = np.asfortranarray(A)
+    M = np.asfortranarray(rng.standard_normal((M_QR, N_QR)))
+    threads = os.cpu_count()
+    try:
+        import threadpoolctl
+
+        threads = max(i.get("num_threads", 0) for i in threadpoolctl.threadpool_info()) or threads
+    except Exception:  # noqa: BLE001  (thread count is informational)
+        pass
+
+    def med(fn):
+        t0 = time.perf_counter()
+        fn()
+        return time.perf_counter() - t0
+
+    tc = med(lambda: lapack.dpotrf(A, lower=1, overwrite_a=0))
+    tq = med(lambda: lapack.dgeqrf(M, overwrite_a=0))
+    fl = chol_flops(n) + qr_flops(M_QR, N_QR)
+    return {"value": fl / (tc + tq) / 1e12, "unit": "TFLOP/s", "cores": threads,
+            "kind": "lapack",
+            "sample": (f"scipy dpotrf n={n} ({tc:.2f}s, incl. the copy of A) + dgeqrf "
+                       f"{M_QR}x{N_QR} ({tq:.2f}s), FP64, one run each")}
 
 
 def profiled_traffic(name):

@@ -637,6 +637,63 @@ def chol_adjoint(L, Lbar, convention: str = "S", out=None):
     return out
 
 
+def chol_solve_(F, B, uplo=Val.L):
+    """ldiv! with a Cholesky factor (LAPACK potrs): B <- A⁻¹ B for A = L Lᵀ
+    (uplo Val.L, L in F's lower triangle as cholRecursive! leaves it) or
+    A = Uᵀ U (Val.U).  B is n×nrhs (or a vector).  Returns B."""
+    if isinstance(F, (LowerTriangular, UpperTriangular)):
+        uplo = Val.L if isinstance(F, LowerTriangular) else Val.U
+        F = F.data
+    ldf = _mat(F, "F")
+    n = F.shape[0]
+    if F.shape != (n, n):
+        raise DimensionMismatch("the factor must be square")
+    vec = B.dim() == 1
+    Bm = B.reshape(-1, 1) if vec else B
+    ldb = _mat(Bm, "B")
+    if Bm.shape[0] != n:
+        raise DimensionMismatch(f"B has {Bm.shape[0]} rows, the factor is {n}×{n}")
+    if Bm.dtype != F.dtype:
+        raise TypeError("F and B must have one element type")
+    with torch.cuda.device(F.device):
+        rc = getattr(lib(), f"gla_chol_solve_{_sfx(F)}")(
+            _handle(F.device.index), _uplo(uplo).encode(), n, Bm.shape[1],
+            C.c_void_p(F.data_ptr()), ldf, C.c_void_p(Bm.data_ptr()), ldb, _stream(F))
+    _check(rc, "gla_chol_solve")
+    return B
+
+
+def qr_solve_(F, tau, B=None):
+    """ldiv! with a QR (least squares, m ≥ n): B (m×nrhs, or a vector)
+    <- Qᵀ B with its first n rows then replaced by R⁻¹ of themselves, the
+    minimiser of ‖A X − B‖.  F, τ as qrBlocked!/qrUnblocked! leave them (or
+    a QR2 as the first argument).  Returns X = B[:n].  Raises
+    ArgumentError("R is singular ...") when R has a zero diagonal entry."""
+    if isinstance(F, QR2):
+        F, tau, B = F.factors, F.tau, tau
+    ldf = _mat(F, "F")
+    m, n = F.shape
+    if n > m:
+        raise ArgumentError("least squares through QR needs m >= n")
+    _vec(tau, n, F.dtype)
+    vec = B.dim() == 1
+    Bm = B.reshape(-1, 1) if vec else B
+    ldb = _mat(Bm, "B")
+    if Bm.shape[0] != m:
+        raise DimensionMismatch(f"B has {Bm.shape[0]} rows, A has {m}")
+    if Bm.dtype != F.dtype:
+        raise TypeError("F and B must have one element type")
+    info = C.c_int64(0)
+    with torch.cuda.device(F.device):
+        rc = getattr(lib(), f"gla_qr_solve_{_sfx(F)}")(
+            _handle(F.device.index), m, n, Bm.shape[1], C.c_void_p(F.data_ptr()), ldf,
+            C.c_void_p(tau.data_ptr()), C.c_void_p(Bm.data_ptr()), ldb, C.byref(info), _stream(F))
+    _check(rc, "gla_qr_solve")
+    if info.value > 0:
+        raise ArgumentError(f"R is singular: zero diagonal entry at {info.value}")
+    return B[:n]
+
+
 def _batch3(X, name):
     """(batch, n, n) device tensor of column-major matrices: returns (ld, stride)."""
     if not isinstance(X, torch.Tensor) or X.dim() != 3 or not X.is_cuda:

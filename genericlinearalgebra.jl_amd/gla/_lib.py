@@ -52,6 +52,8 @@ def _sigs(ft):
         "gla_chol_adjoint_batched": (_int, [_p, _ch, _i64, _i64, _p, _i64, _i64, _p, _i64, _i64,
                                             _p, _i64, _i64, _p, _p]),
         "gla_qr_adjoint": (_int, [_p, _i64, _i64, _p, _i64, _p, _p, _i64, _p, _i64, _p]),
+        "gla_chol_solve": (_int, [_p, _ch, _i64, _i64, _p, _i64, _p, _i64, _p]),
+        "gla_qr_solve": (_int, [_p, _i64, _i64, _i64, _p, _i64, _p, _p, _i64, _p, _p]),
     }
 
 

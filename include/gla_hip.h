@@ -198,6 +198,31 @@ int gla_gemm_lowp_nt(gla_handle_t h, char tin, int64_t m, int64_t n, int64_t k, 
                      const float *A, int64_t lda, const float *B, int64_t ldb, float *C,
                      int64_t ldc, void *stream);
 
+/* --------------------------------------------------------------- solves -- */
+/* No counterpart in the reference (it defines the factorisations only,
+ * src/cholesky.jl:3-55, src/qr.jl:86-146): the solves a caller applies to
+ * their results, Julia's ldiv! with a Cholesky or a QR.
+ *
+ * Cholesky (LAPACK potrs): B (n×nrhs) <- A⁻¹ B for A = L Lᵀ (uplo 'L', F
+ * holds L in its lower triangle — cholRecursive!'s output) or A = Uᵀ U
+ * (uplo 'U', U in the upper triangle); the other triangle is not read. */
+int gla_chol_solve_d(gla_handle_t h, char uplo, int64_t n, int64_t nrhs, const double *F,
+                     int64_t ldf, double *B, int64_t ldb, void *stream);
+int gla_chol_solve_s(gla_handle_t h, char uplo, int64_t n, int64_t nrhs, const float *F,
+                     int64_t ldf, float *B, int64_t ldb, void *stream);
+/* QR least squares (m ≥ n), (F, τ) as stored by qrBlocked!/qrUnblocked!:
+ * B (m×nrhs) <- Qᵀ B (lmul!(Hᴴ, B), src/householder.jl:119-157), then the
+ * first n rows <- R⁻¹ of themselves, the minimiser of ‖A X − B‖_F; rows
+ * n..m−1 keep the residual's components.  *info (host, may be NULL): 0, or
+ * the 1-based index of the first zero diagonal entry of R (X then holds
+ * Inf/NaN).  With info given the call synchronises the stream. */
+int gla_qr_solve_d(gla_handle_t h, int64_t m, int64_t n, int64_t nrhs, const double *F,
+                   int64_t ldf, const double *tau, double *B, int64_t ldb, int64_t *info,
+                   void *stream);
+int gla_qr_solve_s(gla_handle_t h, int64_t m, int64_t n, int64_t nrhs, const float *F,
+                   int64_t ldf, const float *tau, float *B, int64_t ldb, int64_t *info,
+                   void *stream);
+
 /* ------------------------------------------------------------ adjoints -- */
 /* No counterpart in the reference (forward factorisations only,
  * src/cholesky.jl, src/qr.jl): reverse-mode derivatives for an rrule.
