@@ -20,6 +20,7 @@
 // given to the same XCD to share its L2 for the A row-panels and B
 // column-panels they have in common.
 #include <algorithm>
+#include <type_traits>
 
 #include "gla_internal.hpp"
 
@@ -29,6 +30,10 @@ namespace {
 
 constexpr int kThreads = 256;
 constexpr int kBK = 16;
+
+// FP64 direct-to-LDS kernel selection (process-wide): 0 off, 1 (BK 8, ring
+// of 4), 2 (BK 8, ring of 3), 3 (BK 16, ring of 3).
+int g_gemm_dl = 0;
 
 template <typename T>
 struct Mfma;
@@ -321,6 +326,121 @@ __global__ __launch_bounds__(kThreads, 2) void gemm_kernel(GemmP<T> p)
         }
 }
 
+// Direct-to-LDS variant for FP64 with MN-contiguous operands (op(A) = A
+// 'N', op(B) = Bᵀ 'T': the SYRK and NT products), 128×128 tiles.  Each
+// K-row of a slab (128 doubles, 1 KB) is one global_load_lds_dwordx4 of one
+// wave, written straight into an NS-deep LDS ring without passing through
+// registers; the waves wait only for the slab they are about to use
+// (vmcnt), so NS − 1 slabs stay in flight behind the MFMAs.  Rows past M or
+// N load clamped (valid) addresses whose results are never written; K must
+// be a multiple of BKD.
+template <int BKD, int NS, int MODE>
+__global__ __launch_bounds__(kThreads, (2 * NS * BKD * 132 * 8 <= 81920 ? 2 : 1)) void
+gemm_dl_kernel(GemmP<double> p)
+{
+    typedef double T;
+    constexpr int BM = 128, BN = 128, LDX = 132;
+    constexpr int WM = 64, WN = 64, MI = 4, NJ = 4;
+    constexpr int LPW = BKD / 4;   // K-rows of each operand a wave loads per slab
+    static_assert(BKD % 4 == 0 && NS >= 3 && NS <= 4, "slab shape");
+    typedef Mfma<T>::acc_t acc_t;
+    __shared__ __attribute__((aligned(16))) T As[NS][BKD * LDX];
+    __shared__ __attribute__((aligned(16))) T Bs[NS][BKD * LDX];
+
+    int tm, tn;
+    if (!tile_of<BM, BN, MODE>(p, tm, tn)) return;
+    const int64_t m0 = (int64_t)tm * BM, n0 = (int64_t)tn * BN;
+    const int lane = threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int wm = wave & 1, wn = wave >> 1;
+    const int64_t koff = (p.kfrom_row ? m0 : 0);
+    int64_t K = p.K - koff;
+    if (K < 0) K = 0;
+    const int64_t bz = blockIdx.z;
+    const T *Ap = p.A + bz * p.sA + koff * p.lda;
+    const T *Bp = p.B + bz * p.sB + koff * p.ldb;
+    const int64_t xa = m0 + 2 * lane < p.M - 1 ? m0 + 2 * lane : p.M - 2;
+    const int64_t xb = n0 + 2 * lane < p.N - 1 ? n0 + 2 * lane : p.N - 2;
+
+    acc_t acc[MI][NJ];
+#pragma unroll
+    for (int i = 0; i < MI; ++i)
+#pragma unroll
+        for (int j = 0; j < NJ; ++j) acc[i][j] = acc_t{0, 0, 0, 0};
+
+    const int nk = (int)(K / BKD);
+    auto issue = [&](int sl) {
+        const int st = sl % NS;
+        const int64_t k0 = (int64_t)sl * BKD;
+#pragma unroll
+        for (int r = 0; r < LPW; ++r) {
+            const int kr = wave * LPW + r;
+            __builtin_amdgcn_global_load_lds(
+                (const void *)(Ap + xa + (k0 + kr) * p.lda),
+                (__attribute__((address_space(3))) void *)(&As[st][kr * LDX]), 16, 0, 0);
+            __builtin_amdgcn_global_load_lds(
+                (const void *)(Bp + xb + (k0 + kr) * p.ldb),
+                (__attribute__((address_space(3))) void *)(&Bs[st][kr * LDX]), 16, 0, 0);
+        }
+    };
+    // vmcnt(n): n of this wave's vector loads may still be outstanding
+    constexpr int PER = 2 * LPW;   // loads per wave per slab
+    auto wait_slab = [&](int later) {   // later = slabs issued after the one needed
+        if (NS == 4 && later >= 2)
+            __builtin_amdgcn_s_waitcnt(((2 * PER) & 15) | (((2 * PER) >> 4) << 14) | 0x0F70);
+        else if (later >= 1)
+            __builtin_amdgcn_s_waitcnt((PER & 15) | ((PER >> 4) << 14) | 0x0F70);
+        else
+            __builtin_amdgcn_s_waitcnt(0x0F70);
+    };
+    for (int sl = 0; sl < NS - 1 && sl < nk; ++sl) issue(sl);
+    for (int sl = 0; sl < nk; ++sl) {
+        wait_slab(min(NS - 2, nk - 1 - sl));
+        __syncthreads();
+        if (sl + NS - 1 < nk) issue(sl + NS - 1);
+        const T *as = As[sl % NS];
+        const T *bs = Bs[sl % NS];
+#pragma unroll
+        for (int kk = 0; kk < BKD / 4; ++kk) {
+            const int kr = kk * 4 + (lane >> 4);
+            T a[MI], b[NJ];
+#pragma unroll
+            for (int i = 0; i < MI; ++i) a[i] = as[kr * LDX + wm * WM + i * 16 + (lane & 15)];
+#pragma unroll
+            for (int j = 0; j < NJ; ++j) b[j] = bs[kr * LDX + wn * WN + j * 16 + (lane & 15)];
+#pragma unroll
+            for (int i = 0; i < MI; ++i)
+#pragma unroll
+                for (int j = 0; j < NJ; ++j) acc[i][j] = Mfma<T>::mma(a[i], b[j], acc[i][j]);
+        }
+    }
+
+    const T alpha = p.alpha, beta = p.beta;
+    const bool beta0 = beta == T(0);
+    T *const Cp = p.C + bz * p.sC;
+#pragma unroll
+    for (int i = 0; i < MI; ++i)
+#pragma unroll
+        for (int j = 0; j < NJ; ++j) {
+            const int64_t row = m0 + wm * WM + i * 16 + (lane >> 4);
+            const int64_t col = n0 + wn * WN + j * 16 + (lane & 15);
+            if (col >= p.N) continue;
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int64_t gr = row + 4 * r;
+                T *c = Cp + gr + col * p.ldc;
+                bool ok = gr < p.M;
+                if (MODE == 1) ok = ok && (gr + p.diag_off >= col);
+                if (MODE == 2) ok = ok && (gr + p.diag_off <= col);
+                if (ok) {
+                    T v = alpha * acc[i][j][r];
+                    if (!beta0) v += beta * *c;
+                    *c = v;
+                }
+            }
+        }
+}
+
 // C = alpha * Σ_z ws[z] + beta * C over the written region (split-K tail).
 template <typename T, int MODE>
 __global__ void splitk_reduce_kernel(GemmP<T> p, int splits)
@@ -369,6 +489,22 @@ int launch(hipStream_t s, GemmP<T> p, int splits)
         p.ws = nullptr;   // split-K partials are single-product scratch
         p.kc = p.K > 0 ? p.K : 1;
         splits = 1;
+    }
+    if constexpr (std::is_same<T, double>::value && BM == 128 && BN == 128 && !TA && TB &&
+                  VEC == 2) {
+        const int dl = g_gemm_dl;
+        const int bkd = dl == 3 ? 16 : 8;
+        if (dl > 0 && splits == 1 && p.M >= 2 && p.N >= 2 && p.K % bkd == 0) {
+            const dim3 grid(8 * p.per_xcd, 1, p.batch > 1 ? p.batch : 1);
+            if (dl == 1)
+                hipLaunchKernelGGL((gemm_dl_kernel<8, 4, MODE>), grid, dim3(kThreads), 0, s, p);
+            else if (dl == 2)
+                hipLaunchKernelGGL((gemm_dl_kernel<8, 3, MODE>), grid, dim3(kThreads), 0, s, p);
+            else
+                hipLaunchKernelGGL((gemm_dl_kernel<16, 3, MODE>), grid, dim3(kThreads), 0, s, p);
+            GLA_LAUNCH_CHECK();
+            return GLA_OK;
+        }
     }
     hipLaunchKernelGGL((gemm_kernel<T, BM, BN, TA, TB, MODE, VEC>),
                        dim3(8 * p.per_xcd, splits, p.batch > 1 ? p.batch : 1), dim3(kThreads), 0,
@@ -449,6 +585,8 @@ int gemm(hipStream_t s, char ta, char tb, int mode, int64_t M, int64_t N, int64_
     if (aligned) return dispatch<T, VW>(s, ta, tb, mode, p, ws_elems);
     return dispatch<T, 1>(s, ta, tb, mode, p, ws_elems);
 }
+
+void set_gemm_dl(int v) { g_gemm_dl = v; }
 
 template <typename T>
 int gemm_batched(hipStream_t s, char ta, char tb, int mode, int64_t M, int64_t N, int64_t K,

@@ -82,6 +82,15 @@ int gla_set_tuning(gla_handle_t hh, const char *key, int64_t value)
         h->qr_inner = (int)value;
         return GLA_OK;
     }
+    if (std::strcmp(key, "gemm_dl") == 0) {
+        if (value < 0 || value > 3) {
+            gla::set_error("gemm_dl must be 0, 1, 2 or 3");
+            return -3;
+        }
+        gla::set_gemm_dl((int)value);
+        return GLA_OK;
+    }
+
     if (std::strcmp(key, "qr_lookahead") == 0) {
         if (value != 0 && value != 1) {
             gla::set_error("qr_lookahead must be 0 or 1");

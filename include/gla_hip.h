@@ -60,7 +60,10 @@ const char *gla_version(void);
  * factors the next QR panel while the side stream applies the current
  * block reflector to the rest of the trailing matrix; 0 runs the
  * reference's sequential loop.  V, τ and R are the same Householder
- * sequence in every setting. */
+ * sequence in every setting.  "gemm_dl" (process-wide): 0 (default), or
+ * 1/2/3 routes FP64 NT products on 128×128 tiles (SYRK, rankUpdate!, the
+ * trailing updates) to a direct-to-LDS kernel with a 4/3/3-deep ring of
+ * 8/8/16-wide K slabs; same results bit for bit. */
 int gla_set_tuning(gla_handle_t h, const char *key, int64_t value);
 
 /* ------------------------------------------------------------ Cholesky -- */
