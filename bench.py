@@ -357,6 +357,12 @@ def main():
     t_chol = ev_time(torch, chol)       # leaves the factors in Aw, Mw for the checks
     t_qr = ev_time(torch, qr)
 
+    def qr64():
+        Mw.copy_(M0)
+        gla.qrBlocked_(Mw, 64, tau)
+
+    t_qr64 = ev_time(torch, qr64)                 # the other block size of the config
+
     # parity at full size: the factors against rocSOLVER's (an independent
     # library), to 10·n·eps, plus size-independent residuals
     eps = float(np.finfo(np.float64).eps)
@@ -429,6 +435,8 @@ def main():
                  "backward_error_rows": chol_bwd},
         "qr": {"ms": t_qr * 1e3, "tflops": qr_flops(M_QR, N_QR) / (t_qr - t_copy_m) / 1e12,
                "pct_peak": 100 * qr_flops(M_QR, N_QR) / (t_qr - t_copy_m) / 1e12 / FP64_PEAK_TFLOPS,
+               "blocksize64_ms": t_qr64 * 1e3,
+               "blocksize64_tflops": qr_flops(M_QR, N_QR) / (t_qr64 - t_copy_m) / 1e12,
                "R_rel_diff_vs_rocsolver_sign_normalised": qr_diff,
                "bar_10_n_eps": 10 * M_QR * eps, "residual_QRx_vs_Mx": qr_bwd},
         "roofline": {"bound": "mfma", "achieved": syrk_tf, "peak": FP64_PEAK_TFLOPS,
