@@ -34,6 +34,10 @@ constexpr int kBK = 16;
 // FP64 direct-to-LDS kernel selection (process-wide): 0 off, 1 (BK 8, ring
 // of 4), 2 (BK 8, ring of 3), 3 (BK 16, ring of 3).
 int g_gemm_dl = 0;
+// FP64 128×128 tiles with one LDS buffer (1, the default: measured 1–2%
+// faster for the Cholesky, whose chain kernels then find LDS beside the
+// trailing SYRK) or two (0).
+int g_gemm_lds1 = 1;
 
 template <typename T>
 struct Mfma;
@@ -206,7 +210,10 @@ __device__ __forceinline__ bool tile_of(const P &p, int &tm, int &tn)
     return false;
 }
 
-template <typename T, int BM, int BN, bool TA, bool TB, int MODE, int VEC>
+// NBUF = 1: one LDS buffer (half the LDS: kernels of the caller's stream
+// then fit beside two workgroups of this one on a CU) at the cost of a
+// second barrier per K-slab.
+template <typename T, int BM, int BN, bool TA, bool TB, int MODE, int VEC, int NBUF = 2>
 __global__ __launch_bounds__(kThreads, 2) void gemm_kernel(GemmP<T> p)
 {
     constexpr int PAD = 32 / sizeof(T);
@@ -215,8 +222,8 @@ __global__ __launch_bounds__(kThreads, 2) void gemm_kernel(GemmP<T> p)
     constexpr int MI = WM / 16, NJ = WN / 16;
     typedef typename Mfma<T>::acc_t acc_t;
 
-    __shared__ __attribute__((aligned(16))) T As[2][kBK * LDA_S];
-    __shared__ __attribute__((aligned(16))) T Bs[2][kBK * LDB_S];
+    __shared__ __attribute__((aligned(16))) T As[NBUF][kBK * LDA_S];
+    __shared__ __attribute__((aligned(16))) T Bs[NBUF][kBK * LDB_S];
 
     int tm, tn;
     if (!tile_of<BM, BN, MODE>(p, tm, tn)) return;
@@ -268,7 +275,7 @@ __global__ __launch_bounds__(kThreads, 2) void gemm_kernel(GemmP<T> p)
     }
 
     for (int kt = 0; kt < nk; ++kt) {
-        const int cur = kt & 1;
+        const int cur = NBUF == 2 ? (kt & 1) : 0;
         const bool more = kt + 1 < nk;
         if (more) {
             const int64_t k0 = (int64_t)(kt + 1) * kBK;
@@ -292,8 +299,9 @@ __global__ __launch_bounds__(kThreads, 2) void gemm_kernel(GemmP<T> p)
                 for (int j = 0; j < NJ; ++j) acc[i][j] = Mfma<T>::mma(a[i], b[j], acc[i][j]);
         }
         if (more) {
-            sa.template store<LDA_S>(As[cur ^ 1]);
-            sb.template store<LDB_S>(Bs[cur ^ 1]);
+            if (NBUF == 1) __syncthreads();   // every wave is done with the buffer
+            sa.template store<LDA_S>(As[NBUF == 2 ? cur ^ 1 : 0]);
+            sb.template store<LDB_S>(Bs[NBUF == 2 ? cur ^ 1 : 0]);
         }
         __syncthreads();
     }
@@ -506,6 +514,20 @@ int launch(hipStream_t s, GemmP<T> p, int splits)
             return GLA_OK;
         }
     }
+    if constexpr (std::is_same<T, double>::value && BM == 128 && BN == 128) {
+        if (g_gemm_lds1) {
+            hipLaunchKernelGGL((gemm_kernel<T, BM, BN, TA, TB, MODE, VEC, 1>),
+                               dim3(8 * p.per_xcd, splits, p.batch > 1 ? p.batch : 1),
+                               dim3(kThreads), 0, s, p);
+            GLA_LAUNCH_CHECK();
+            if (p.ws) {
+                hipLaunchKernelGGL((splitk_reduce_kernel<T, MODE>),
+                                   dim3((unsigned)cdiv(p.M, 256), p.N), dim3(256), 0, s, p, splits);
+                GLA_LAUNCH_CHECK();
+            }
+            return GLA_OK;
+        }
+    }
     hipLaunchKernelGGL((gemm_kernel<T, BM, BN, TA, TB, MODE, VEC>),
                        dim3(8 * p.per_xcd, splits, p.batch > 1 ? p.batch : 1), dim3(kThreads), 0,
                        s, p);
@@ -587,6 +609,7 @@ int gemm(hipStream_t s, char ta, char tb, int mode, int64_t M, int64_t N, int64_
 }
 
 void set_gemm_dl(int v) { g_gemm_dl = v; }
+void set_gemm_lds1(int v) { g_gemm_lds1 = v; }
 
 template <typename T>
 int gemm_batched(hipStream_t s, char ta, char tb, int mode, int64_t M, int64_t N, int64_t K,

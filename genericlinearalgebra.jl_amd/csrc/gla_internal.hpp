@@ -83,6 +83,7 @@ int gemm(hipStream_t s, char ta, char tb, int mode, int64_t M, int64_t N, int64_
          int64_t diag_off = 0, T *ws = nullptr, size_t ws_elems = 0, int kfrom_row = 0);
 // Selects the FP64 direct-to-LDS GEMM kernel (0: off).
 void set_gemm_dl(int v);
+void set_gemm_lds1(int v);
 // The same over a strided batch (grid z): product b reads A + b·sA, B + b·sB
 // and writes C + b·sC; no split-K.
 template <typename T>

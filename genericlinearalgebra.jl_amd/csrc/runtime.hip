@@ -91,6 +91,14 @@ int gla_set_tuning(gla_handle_t hh, const char *key, int64_t value)
         return GLA_OK;
     }
 
+    if (std::strcmp(key, "gemm_lds1") == 0) {
+        if (value != 0 && value != 1) {
+            gla::set_error("gemm_lds1 must be 0 or 1");
+            return -3;
+        }
+        gla::set_gemm_lds1((int)value);
+        return GLA_OK;
+    }
     if (std::strcmp(key, "qr_lookahead") == 0) {
         if (value != 0 && value != 1) {
             gla::set_error("qr_lookahead must be 0 or 1");

@@ -63,7 +63,9 @@ const char *gla_version(void);
  * sequence in every setting.  "gemm_dl" (process-wide): 0 (default), or
  * 1/2/3 routes FP64 NT products on 128×128 tiles (SYRK, rankUpdate!, the
  * trailing updates) to a direct-to-LDS kernel with a 4/3/3-deep ring of
- * 8/8/16-wide K slabs; same results bit for bit. */
+ * 8/8/16-wide K slabs; "gemm_lds1" (process-wide): 1 (default) stages
+ * FP64 128×128 tiles through one LDS buffer, 0 through two.  Same results
+ * bit for bit in every setting. */
 int gla_set_tuning(gla_handle_t h, const char *key, int64_t value);
 
 /* ------------------------------------------------------------ Cholesky -- */

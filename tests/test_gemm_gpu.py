@@ -54,45 +54,49 @@ def test_gemm_unaligned_submatrix(rng):
     assert rel(gla.to_numpy(Cg), ref) < 64 * 40 * eps_of(np.float64)
 
 
-@pytest.mark.parametrize("dl", [1, 2, 3])
+VARIANTS = [("gemm_dl", 1), ("gemm_dl", 2), ("gemm_dl", 3), ("gemm_lds1", 0)]
+DEFAULTS = {"gemm_dl": 0, "gemm_lds1": 1}
+
+
+@pytest.mark.parametrize("knob,val", VARIANTS)
 @pytest.mark.parametrize("m,n,k", [(1024, 768, 512), (1000, 900, 256), (129, 383, 16),
                                    (2048, 2048, 1024)])
-def test_direct_lds_kernel_matches_default(rng, dl, m, n, k):
-    """The FP64 direct-to-LDS kernel (NT products on 128×128 tiles, ragged
-    M/N edges included) gives the default kernel's result: same MFMA order,
-    so the same bits."""
+def test_kernel_variants_match_default(rng, knob, val, m, n, k):
+    """The FP64 GEMM variants (direct-to-LDS ring; double-buffered LDS) on
+    NT products with 128×128 tiles, ragged M/N edges included, give the
+    default kernel's result: same MFMA order, so the same bits."""
     gla = _gla()
     A = rng.standard_normal((m, k))
     B = rng.standard_normal((n, k))
     C0 = rng.standard_normal((m, n))
     outs = []
-    for knob in (0, dl):
+    for v in (DEFAULTS[knob], val):
         Cg = gla.colmajor(C0)
         try:
-            gla.set_tuning("gemm_dl", knob)
+            gla.set_tuning(knob, v)
             gla.gemm_("N", "T", -1.0, gla.colmajor(A), gla.colmajor(B), 1.0, Cg)
         finally:
-            gla.set_tuning("gemm_dl", 0)
+            gla.set_tuning(knob, DEFAULTS[knob])
         outs.append(gla.to_numpy(Cg))
     ref = C0 - A @ B.T
     assert rel(outs[1], ref) <= 8 * k * eps_of(np.float64)
     assert np.array_equal(outs[0], outs[1])
 
 
-@pytest.mark.parametrize("dl", [1, 2, 3])
-def test_direct_lds_kernel_syrk_lower(rng, dl):
+@pytest.mark.parametrize("knob,val", VARIANTS)
+def test_kernel_variants_syrk_lower(rng, knob, val):
     gla = _gla()
     n, k = 1536, 512
     X = rng.standard_normal((n, k))
     C0 = rng.standard_normal((n, n))
     outs = []
-    for knob in (0, dl):
+    for v in (DEFAULTS[knob], val):
         Cg = gla.colmajor(C0)
         try:
-            gla.set_tuning("gemm_dl", knob)
+            gla.set_tuning(knob, v)
             gla.rankUpdate_(gla.Hermitian(Cg, "L"), gla.colmajor(X), -1.0)
         finally:
-            gla.set_tuning("gemm_dl", 0)
+            gla.set_tuning(knob, DEFAULTS[knob])
         outs.append(gla.to_numpy(Cg))
     ref = C0 - X @ X.T
     assert rel(np.tril(outs[1]), np.tril(ref)) <= 8 * k * eps_of(np.float64)
