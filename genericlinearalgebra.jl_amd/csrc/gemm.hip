@@ -10,10 +10,13 @@
 //
 // Design: 256 threads = 4 wave64s in a 2×2 arrangement; a workgroup tile is
 // BM×BN and each wave owns (BM/2)×(BN/2) as a grid of 16×16 MFMA blocks with
-// their accumulators in registers.  K advances BK=16 at a time through two
-// LDS buffers: the next K-slab is fetched from HBM/L2 into registers while
-// the current one feeds the MFMAs, then written to the other LDS buffer
-// (one barrier per slab).  LDS holds slabs K-major ([k][mn]) so that a
+// their accumulators in registers.  K advances BK=16 at a time: the next
+// K-slab is fetched from HBM/L2 into registers while the current one feeds
+// the MFMAs, then written to LDS — to the other of two buffers (one barrier
+// per slab), or, for FP64 128×128 tiles by default, back into the single
+// buffer after a second barrier (half the LDS).  A third form for FP64 NT
+// products loads the slabs straight into an LDS ring (gemm_dl_kernel).
+// LDS holds slabs K-major ([k][mn]) so that a
 // fragment read — lane l takes element (mn = l%16, k = l/16) — is 16
 // consecutive words per k row.  Tiles are mapped to workgroups XCD-aware:
 // the dispatcher places workgroup b on XCD b%8, so consecutive tile ids are
