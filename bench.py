@@ -442,7 +442,8 @@ def main():
         "roofline": {"bound": "mfma", "achieved": syrk_tf, "peak": FP64_PEAK_TFLOPS,
                      "unit": "TFLOP/s", "frac": syrk_tf / FP64_PEAK_TFLOPS,
                      "traffic": profiled_traffic("syrk_15360_k512.md"),
-                     "kernel": "gemm_kernel<double,128,128,N,T,lower> (SYRK 15360x15360, K=512)",
+                     "kernel": ("gemm_kernel<double,128,128,N,T,lower,1 LDS buffer> "
+                                "(SYRK 15360x15360, K=512)"),
                      "flops_per_launch": float(mm) * float(mm) * kk,
                      "us_per_launch": t_syrk * 1e6},
         "roofline_panel": {"bound": "hbm", "achieved": achieved, "peak": 8000.0, "unit": "GB/s",
