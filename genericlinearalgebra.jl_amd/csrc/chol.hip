@@ -55,29 +55,6 @@ constexpr int kXLD = 17;        // padded row length of the LDS column blocks
     } while (0)
 
 
-// 16×16×4 MFMA with one A and one B element per lane; accumulator element r
-// of lane l sits at (row(l, r), l % 16) of the 16×16 tile.
-template <typename T>
-struct PanelMfma;
-template <>
-struct PanelMfma<double> {
-    typedef d4 acc_t;
-    static __device__ __forceinline__ acc_t mma(double a, double b, acc_t c)
-    {
-        return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
-    }
-    static __device__ __forceinline__ int row(int l, int r) { return (l >> 4) + 4 * r; }
-};
-template <>
-struct PanelMfma<float> {
-    typedef f4 acc_t;
-    static __device__ __forceinline__ acc_t mma(float a, float b, acc_t c)
-    {
-        return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
-    }
-    static __device__ __forceinline__ int row(int l, int r) { return 4 * (l >> 4) + r; }
-};
-
 __device__ __forceinline__ void wave_order() { __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront"); }
 
 // One wave holds a 16×16 tile with lane (i = l % 16, g = l / 16) owning row

@@ -39,6 +39,30 @@ void set_error(const char *fmt, ...);
         }                                                                              \
     } while (0)
 
+// 16×16×4 MFMA with one A and one B element per lane; accumulator element r
+// of lane l sits at (row(l, r), l % 16) of the 16×16 tile.
+template <typename T>
+struct PanelMfma;
+template <>
+struct PanelMfma<double> {
+    typedef d4 acc_t;
+    static __device__ __forceinline__ acc_t mma(double a, double b, acc_t c)
+    {
+        return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+    }
+    static __device__ __forceinline__ int row(int l, int r) { return (l >> 4) + 4 * r; }
+};
+template <>
+struct PanelMfma<float> {
+    typedef f4 acc_t;
+    static __device__ __forceinline__ acc_t mma(float a, float b, acc_t c)
+    {
+        return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+    }
+    static __device__ __forceinline__ int row(int l, int r) { return 4 * (l >> 4) + r; }
+};
+
+
 // ------------------------------------------------------------------ handle --
 // Workspace slots carved out of one device allocation that only grows.
 struct Handle {
@@ -64,6 +88,9 @@ struct Handle {
     int qr_inner = -1;
     // Depth-1 look-ahead in qrBlocked! (trailing update on the side stream).
     int qr_lookahead = 1;
+    // Block columns at least this tall are factored by the
+    // reconstructed-Householder panel (0: never; -1: 512 rows).
+    int64_t qr_hr = -1;
 };
 
 // Ensure the workspace holds at least `bytes` (grows with hipMalloc, which

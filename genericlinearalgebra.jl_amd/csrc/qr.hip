@@ -61,6 +61,9 @@ struct QrPanelP {
     unsigned *bar;
     unsigned bar_base;
     int *err;
+    // null: always run; otherwise run only if *run_if != 0 (the fallback
+    // behind the reconstructed-Householder panel)
+    const int *run_if;
 };
 
 template <typename T>
@@ -157,6 +160,21 @@ __device__ __forceinline__ T wave_reduce8(const T v[8])
     return z;
 }
 
+// A panel launched as the fallback of the reconstructed-Householder path
+// runs only if that path failed (*run_if != 0).  When it does not run it
+// still advances the grid-barrier counter by what its column loop adds
+// (kmax arrivals of each of the P workgroups), so the barrier targets of
+// later panels stay valid.
+template <typename T>
+__device__ __forceinline__ bool panel_skipped(const QrPanelP<T> &p)
+{
+    if (!p.run_if || *p.run_if != 0) return false;
+    if (gridDim.x > 1 && blockIdx.x == 0 && threadIdx.x == 0)
+        __hip_atomic_fetch_add(p.bar, (unsigned)p.kmax * gridDim.x, __ATOMIC_RELAXED,
+                               __HIP_MEMORY_SCOPE_AGENT);
+    return true;
+}
+
 // Householder panel, one column at a time as qrUnblocked! does
 // (src/qr.jl:86-111), over P workgroups that each hold R = 64·RPL rows of
 // the m×w panel in registers: wave v holds CPW = NB/(NTH/64) consecutive
@@ -191,6 +209,7 @@ __global__ __launch_bounds__(NTH) void qr_panel2_kernel(QrPanelP<T> p)
     __shared__ T Ts[NB * NB];       // T, column-major (workgroup 0)
     T *const tpart = &red[0][0];
 
+    if (panel_skipped(p)) return;
     const int t = threadIdx.x, lane = t & 63;
     const int wv = __builtin_amdgcn_readfirstlane(t >> 6);
     const int wg = blockIdx.x, P = gridDim.x;
@@ -390,6 +409,7 @@ __global__ __launch_bounds__(kQrThreads) void qr_panel_kernel(QrPanelP<T> p)
     __shared__ T Ts[NB * NB];       // T, column-major (workgroup 0)
     T *const tpart = &red[0][0][0];
 
+    if (panel_skipped(p)) return;
     const int t = threadIdx.x;
     // g is uniform across a wave (NB is a multiple of 64): keep it, and the
     // row indices derived from it, in scalar registers
@@ -775,12 +795,417 @@ __global__ __launch_bounds__(1024) void qr_t_merge_kernel(int k, int ib, T *Tw, 
     }
 }
 
+// ---------------------------------------------------------------------------
+// Reconstructed-Householder panel, for tall block columns.
+//
+// The Householder QR that qrUnblocked! computes for a panel P (m×k, m ≥ k),
+// P = Q_h R_h with Q_h = (I − V T Vᵀ)[I;0], V unit lower trapezoidal and
+// V₁ its top k×k block, satisfies
+//     [I;0] − Q_h = V · (T V₁ᵀ),
+// an LU factorisation without pivoting (V unit lower; T V₁ᵀ upper with
+// diagonal τ, since T's diagonal is τ and V₁'s is 1).  Every other thin QR
+// P = Q R' has Q = Q_h S for a diagonal sign matrix S, and LU factors are
+// unique, so the Householder factors follow from any orthonormal Q:
+//   (1) Q, R' by two CholeskyQR passes: G = XᵀX (MFMA GEMM, split-K),
+//       G = L Lᵀ (the 128-wide diagonal-block Cholesky kernel), X ← X L⁻ᵀ;
+//       R' = L₂ᵀ L₁ᵀ;
+//   (2) the LU of S − Q[0:k, :] with S chosen during the elimination:
+//       S_jj = copysign(1, −q̂_jj), q̂_jj the updated diagonal entry, which
+//       makes the pivot S_jj (1 + |q̂_jj|), i.e. τ_j = 1 + |q̂_jj| ∈ [1, 2],
+//       the range reflector! gives (τ = 1 + |ξ₁|/‖x‖); the rows below the
+//       top block follow by a right solve, V₂ = −Q[k:m, :] U⁻¹;
+//   (3) V = L, τ_j = S_jj U_jj, T = (U S) V₁⁻ᵀ, R_h = S R'.
+// These are the reflectors, τ, T and R of the column-by-column algorithm,
+// up to rounding, from O(1) grid-wide steps instead of one exchange per
+// column.  The path checks itself: a Cholesky breakdown, or a second Gram
+// matrix farther than 0.5 from I in the Frobenius norm (the first pass left
+// Q with κ² > 3), sets a device flag; the panel is then left untouched and
+// the column-by-column panel kernel, launched behind the path and gated on
+// the flag, factors it instead.
+
+// Q (m×k) = P, and the path's flag cleared.
+template <typename T>
+__global__ void hr_copy_kernel(int64_t m, const T *A, int64_t lda, T *Q, int64_t ldq, int *flag)
+{
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    const int j = blockIdx.y;
+    if (i == 0 && j == 0) flag[0] = 0;
+    if (i < m) Q[i + (int64_t)j * ldq] = A[i + (int64_t)j * lda];
+}
+
+// Gram matrix G = XᵀX of an m×k panel (k ≤ 128), lower 16×16 tiles only:
+// workgroup p sums rows of 32-row chunks p, p + P, … on MFMA (the chunk
+// staged in LDS, wave w takes tiles w, w + 4, …) and writes its partial
+// tiles to part[p][tile][256] (element (i, j) of a tile at 16 i + j);
+// hr_gram_reduce_kernel adds the P partials in a fixed order.
+constexpr int kGrRB = 32;
+constexpr int kGrLD = 132;
+
+__host__ __device__ inline int hr_ntiles(int k)
+{
+    const int nt = (k + 15) / 16;
+    return nt * (nt + 1) / 2;
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void hr_gram_kernel(int64_t m, int k, const T *X, int64_t ldx,
+                                                      T *part)
+{
+    typedef PanelMfma<T> Mf;
+    typedef typename Mf::acc_t acc_t;
+    __shared__ T Xs[kGrRB * kGrLD];
+    const int t = threadIdx.x, lane = t & 63;
+    const int w = __builtin_amdgcn_readfirstlane(t >> 6);
+    const int nt = (k + 15) / 16, ntl = nt * (nt + 1) / 2;
+    const int li = lane & 15, lk = lane >> 4;
+    acc_t acc[9];
+    int tbi[9], tbj[9];
+#pragma unroll
+    for (int q = 0; q < 9; ++q) {
+        acc[q] = acc_t{T(0), T(0), T(0), T(0)};
+        const int ti = w + 4 * q;
+        int bi = 0;
+        while ((bi + 1) * (bi + 2) / 2 <= ti) ++bi;
+        tbi[q] = bi;
+        tbj[q] = ti - bi * (bi + 1) / 2;
+    }
+    for (int64_t r0 = (int64_t)blockIdx.x * kGrRB; r0 < m; r0 += (int64_t)gridDim.x * kGrRB) {
+#pragma unroll
+        for (int e = t; e < kGrRB * 128; e += 256) {
+            const int r = e & (kGrRB - 1), c = e / kGrRB;
+            Xs[r * kGrLD + c] = (r0 + r < m && c < k) ? X[r0 + r + (int64_t)c * ldx] : T(0);
+        }
+        __syncthreads();
+#pragma unroll
+        for (int q = 0; q < 9; ++q) {
+            if (w + 4 * q >= ntl) break;
+            const int ca = tbi[q] * 16 + li, cb = tbj[q] * 16 + li;
+#pragma unroll
+            for (int kq = 0; kq < kGrRB / 4; ++kq) {
+                const int r = 4 * kq + lk;
+                acc[q] = Mf::mma(Xs[r * kGrLD + ca], Xs[r * kGrLD + cb], acc[q]);
+            }
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int q = 0; q < 9; ++q) {
+        const int ti = w + 4 * q;
+        if (ti >= ntl) break;
+        T *dst = part + ((size_t)blockIdx.x * ntl + ti) * 256;
+#pragma unroll
+        for (int r = 0; r < 4; ++r) dst[Mf::row(lane, r) * 16 + li] = acc[q][r];
+    }
+}
+
+// G (k×k, ld k, both triangles) = Σ_p part[p], one workgroup per tile, the
+// P partials summed in four interleaved groups then across the groups, a
+// fixed order.  With check: flags the path (atomicCAS(flag, 0, −1)) if the
+// tile's share of ‖G − I‖_F² exceeds 0.25 / ntiles, so an unflagged G has
+// ‖G − I‖_F ≤ 0.5; a non-finite G is flagged too.
+template <typename T>
+__global__ __launch_bounds__(1024) void hr_gram_reduce_kernel(int k, int P, const T *part, T *G,
+                                                              int *flag, int check)
+{
+    __shared__ T red[4][256];
+    __shared__ double wsum[4];
+    const int t = threadIdx.x, e = t & 255, g = t >> 8;
+    const int nt = (k + 15) / 16, ntl = nt * (nt + 1) / 2, ti = blockIdx.x;
+    int bi = 0;
+    while ((bi + 1) * (bi + 2) / 2 <= ti) ++bi;
+    const int bj = ti - bi * (bi + 1) / 2;
+    T s = T(0);
+#pragma unroll 16
+    for (int p = g; p < P; p += 4) s += part[((size_t)p * ntl + ti) * 256 + e];
+    red[g][e] = s;
+    __syncthreads();
+    if (t >= 256) return;
+    const T v = (red[0][e] + red[1][e]) + (red[2][e] + red[3][e]);
+    const int i = bi * 16 + (e >> 4), j = bj * 16 + (e & 15);
+    double c = 0.0;
+    if (i < k && j < k) {
+        G[i + (int64_t)j * k] = v;
+        G[j + (int64_t)i * k] = v;
+        const double d = (double)v - (i == j ? 1.0 : 0.0);
+        c = i > j ? 2.0 * d * d : (i == j ? d * d : 0.0);
+    }
+    if (!check) return;
+    for (int o = 32; o > 0; o >>= 1) c += __shfl_xor(c, o);
+    if ((t & 63) == 0) wsum[t >> 6] = c;
+    __syncthreads();
+    if (t == 0) {
+        const double tot = (wsum[0] + wsum[1]) + (wsum[2] + wsum[3]);
+        if (!(tot <= 0.25 / ntl)) atomicCAS(flag, 0, -1);
+    }
+}
+
+// R = R₂ R₁ (k×k, ld k, zero below the diagonal) from the two Cholesky
+// factors (R_c = L_cᵀ, L_c lower, ld k; their strict upper parts are
+// ignored): one workgroup per upper 16×16 tile, the two strips in LDS.
+template <typename T>
+__global__ __launch_bounds__(256) void hr_r_kernel(int k, const T *L1, const T *L2, T *R)
+{
+    const int bi = blockIdx.x, bj = blockIdx.y;
+    if (bi > bj) return;
+    __shared__ T A[128 * 17], B[128 * 17];   // A[l][ii] = L₂(l, i), B[l][jj] = L₁(j, l)
+    const int t = threadIdx.x;
+    for (int e = t; e < 128 * 16; e += 256) {
+        const int l = e >> 4, c = e & 15;
+        const int i = bi * 16 + c, j = bj * 16 + c;
+        A[l * 17 + c] = (l < k && i < k && l >= i) ? L2[l + (int64_t)i * k] : T(0);
+        B[l * 17 + c] = (l < k && j < k && l <= j) ? L1[j + (int64_t)l * k] : T(0);
+    }
+    __syncthreads();
+    const int ii = t & 15, jj = t >> 4;
+    const int i = bi * 16 + ii, j = bj * 16 + jj;
+    T acc = T(0);
+    for (int l = bi * 16; l < min(k, bj * 16 + 16); ++l) acc += A[l * 17 + ii] * B[l * 17 + jj];
+    if (i < k && j < k) R[i + (int64_t)j * k] = i <= j ? acc : T(0);
+}
+
+// v from lane `src` (wave-uniform) to every lane, through SGPRs.
+template <typename T>
+__device__ __forceinline__ T wave_bcast(T v, int src)
+{
+    if constexpr (sizeof(T) == 8) {
+        int2 u = __builtin_bit_cast(int2, v);
+        u.x = __builtin_amdgcn_readlane(u.x, src);
+        u.y = __builtin_amdgcn_readlane(u.y, src);
+        return __builtin_bit_cast(T, u);
+    } else {
+        return __builtin_bit_cast(T, __builtin_amdgcn_readlane(__builtin_bit_cast(int, v), src));
+    }
+}
+
+constexpr int kHrLD = 129;   // LDS leading dimension of the top block
+constexpr int kHrThreads = 256;
+
+// Steps (2)-(3) on the top block, one workgroup of 4 waves, k ≤ 128, the
+// block in LDS (M, column-major).  Both sweeps are blocked by 16 columns:
+//  LU of S − Q[0:k, :]: wave 0 eliminates the 16-column panel M[jb:k, J]
+//    column by column in registers (lane l holds local rows l and l + 64;
+//    the pivot row lives in lane c and is broadcast through SGPRs),
+//    choosing S_jj from the sign of the updated diagonal; then every wave:
+//    U12 = L11⁻¹ M[J, right] (one column per thread), and the trailing
+//    block M[below, right] −= L21 U12 (wave w takes columns w, w+4, …).
+//  T = Û V₁⁻ᵀ, Û = U S, in place over U: per column block J,
+//    T[:, J] = Acc[:, J] V₁[J, J]⁻ᵀ (one row per thread), then
+//    Acc[:, right] −= T[:, J] V₁[right, J]ᵀ.  T is upper triangular and V₁
+//    strictly lower, so both live in M at once.
+// Writes V₁ (explicit: unit diagonal, zero above) over Q's top block,
+// Un = −Uᵀ (lower, ld k) for the solve of the rows below, T (zero below the
+// diagonal), S (Sv) and τ.
+template <typename T>
+__global__ __launch_bounds__(kHrThreads) void hr_lu_kernel(int k, T *Q, int64_t ldq,
+                                                           const int *flag, T *Un, T *Tw,
+                                                           int64_t ldt, T *Sv, T *tau)
+{
+    __shared__ T M[128 * kHrLD];
+    __shared__ T Dv[128], Uv[128];
+    if (*flag != 0) return;
+    const int t = threadIdx.x, lane = t & 63;
+    const int w = __builtin_amdgcn_readfirstlane(t >> 6);
+#pragma unroll 8
+    for (int l = w; l < 128; l += kHrThreads / 64)
+#pragma unroll
+        for (int s = 0; s < 2; ++s) {
+            const int i = lane + 64 * s;
+            M[i + l * kHrLD] = (i < k && l < k) ? -Q[i + (int64_t)l * ldq] : T(0);
+        }
+    __syncthreads();
+
+    // ---- LU with sign choice
+    for (int jb = 0; jb < k; jb += 16) {
+        const int nbk = min(16, k - jb), c0 = jb + nbk;
+        if (w == 0) {
+            T pa[2][16];
+#pragma unroll
+            for (int s = 0; s < 2; ++s)
+#pragma unroll
+                for (int c = 0; c < 16; ++c) {
+                    const int i = jb + lane + 64 * s;
+                    pa[s][c] = (i < k && c < nbk) ? M[i + (jb + c) * kHrLD] : T(0);
+                }
+#pragma unroll
+            for (int c = 0; c < 16; ++c) {
+                if (c < nbk) {
+                    const T mjj = wave_bcast(pa[0][c], c);    // −q̂_jj
+                    const T dj = dcopysign<T>(T(1), mjj);
+                    const T uj = mjj + dj;                    // |uj| = 1 + |q̂_jj|
+                    const T ru = T(1) / uj;
+                    T rowc[16];
+#pragma unroll
+                    for (int c2 = 0; c2 < 16; ++c2)
+                        rowc[c2] = c2 > c ? wave_bcast(pa[0][c2], c) : T(0);
+#pragma unroll
+                    for (int s = 0; s < 2; ++s) {
+                        if (lane + 64 * s > c) {
+                            const T l = pa[s][c] * ru;
+                            pa[s][c] = l;
+#pragma unroll
+                            for (int c2 = 0; c2 < 16; ++c2)
+                                if (c2 > c) pa[s][c2] -= l * rowc[c2];
+                        }
+                    }
+                    if (lane == c) pa[0][c] = uj;
+                    if (lane == 0) {
+                        Dv[jb + c] = dj;
+                        Uv[jb + c] = uj;
+                    }
+                }
+            }
+#pragma unroll
+            for (int s = 0; s < 2; ++s)
+#pragma unroll
+                for (int c = 0; c < 16; ++c) {
+                    const int i = jb + lane + 64 * s;
+                    if (i < k && c < nbk) M[i + (jb + c) * kHrLD] = pa[s][c];
+                }
+        }
+        __syncthreads();
+        if (c0 >= k) break;
+        // U12 = L11⁻¹ M[J, c0:k]
+        if (t < k - c0) {
+            const int cc = c0 + t;
+            T x[16];
+#pragma unroll
+            for (int r = 0; r < 16; ++r) x[r] = r < nbk ? M[jb + r + cc * kHrLD] : T(0);
+#pragma unroll
+            for (int r = 1; r < 16; ++r)
+#pragma unroll
+                for (int s = 0; s < r; ++s) x[r] -= M[jb + r + (jb + s) * kHrLD] * x[s];
+#pragma unroll
+            for (int r = 0; r < 16; ++r)
+                if (r < nbk) M[jb + r + cc * kHrLD] = x[r];
+        }
+        __syncthreads();
+        // M[c0:k, c0:k] −= L21 U12
+        {
+            T lr[2][16];
+#pragma unroll
+            for (int s = 0; s < 2; ++s)
+#pragma unroll
+                for (int c = 0; c < 16; ++c) {
+                    const int i = c0 + lane + 64 * s;
+                    lr[s][c] = (i < k && c < nbk) ? M[i + (jb + c) * kHrLD] : T(0);
+                }
+            for (int cc = c0 + w; cc < k; cc += kHrThreads / 64) {
+                T u[16];
+#pragma unroll
+                for (int c = 0; c < 16; ++c) u[c] = M[jb + c + cc * kHrLD];
+#pragma unroll
+                for (int s = 0; s < 2; ++s) {
+                    const int i = c0 + lane + 64 * s;
+                    if (i < k) {
+                        T acc = M[i + cc * kHrLD];
+#pragma unroll
+                        for (int c = 0; c < 16; ++c) acc -= lr[s][c] * u[c];
+                        M[i + cc * kHrLD] = acc;
+                    }
+                }
+            }
+        }
+        __syncthreads();
+    }
+
+    // ---- outputs of the LU; Û = U S in place
+    for (int l = w; l < k; l += kHrThreads / 64)
+        for (int i = lane; i < k; i += 64) {
+            const T v = M[i + l * kHrLD];
+            Q[i + (int64_t)l * ldq] = i > l ? v : (i == l ? T(1) : T(0));
+            if (i <= l) {
+                Un[l + (int64_t)i * k] = -v;
+                M[i + l * kHrLD] = v * Dv[l];
+            }
+        }
+    if (t < k) {
+        Sv[t] = Dv[t];
+        tau[t] = Dv[t] * Uv[t];
+    }
+    __syncthreads();
+
+    // ---- T = Û V₁⁻ᵀ
+    for (int jb = 0; jb < k; jb += 16) {
+        const int nbk = min(16, k - jb), c0 = jb + nbk;
+        // rows 0 .. c0-1 hold T's nonzeros in columns J
+        if (t < c0) {
+            T x[16];
+#pragma unroll
+            for (int c = 0; c < 16; ++c) x[c] = (c < nbk && t <= jb + c) ? M[t + (jb + c) * kHrLD] : T(0);
+#pragma unroll
+            for (int c = 1; c < 16; ++c)
+#pragma unroll
+                for (int c2 = 0; c2 < c; ++c2) x[c] -= x[c2] * M[jb + c + (jb + c2) * kHrLD];
+#pragma unroll
+            for (int c = 0; c < 16; ++c)
+                if (c < nbk && t <= jb + c) M[t + (jb + c) * kHrLD] = x[c];
+        }
+        __syncthreads();
+        if (c0 >= k) break;
+        {
+            T tr[2][16];
+#pragma unroll
+            for (int s = 0; s < 2; ++s)
+#pragma unroll
+                for (int c = 0; c < 16; ++c) {
+                    const int i = lane + 64 * s;
+                    tr[s][c] = (i < c0 && c < nbk && i <= jb + c) ? M[i + (jb + c) * kHrLD] : T(0);
+                }
+            for (int cc = c0 + w; cc < k; cc += kHrThreads / 64) {
+                T v[16];
+#pragma unroll
+                for (int c = 0; c < 16; ++c) v[c] = c < nbk ? M[cc + (jb + c) * kHrLD] : T(0);
+#pragma unroll
+                for (int s = 0; s < 2; ++s) {
+                    const int i = lane + 64 * s;
+                    if (i < c0) {
+                        T acc = M[i + cc * kHrLD];
+#pragma unroll
+                        for (int c = 0; c < 16; ++c) acc -= tr[s][c] * v[c];
+                        M[i + cc * kHrLD] = acc;
+                    }
+                }
+            }
+        }
+        __syncthreads();
+    }
+    for (int l = w; l < k; l += kHrThreads / 64)
+        for (int i = lane; i < k; i += 64)
+            Tw[i + (int64_t)l * ldt] = i <= l ? M[i + l * kHrLD] : T(0);
+}
+
+// The reconstructed factors over the panel, if the path succeeded: rows
+// ≥ k take V (from Q's storage, where the solve left it), the top block V₁
+// below the diagonal and R_h = S R on and above it.
+template <typename T>
+__global__ void hr_finish_kernel(int64_t m, int k, const T *Vw, int64_t ldv, const T *R,
+                                 const T *Sv, T *A, int64_t lda, const int *flag)
+{
+    if (*flag != 0) return;
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    const int j = blockIdx.y;
+    if (i >= m) return;
+    A[i + (int64_t)j * lda] =
+        i > j ? Vw[i + (int64_t)j * ldv] : Sv[i] * R[i + (int64_t)j * k];
+}
+
 // Workspace layout of one qrBlocked! call.
 template <typename T>
 struct QrWs {
     T *Vw, *Tw, *W1, *W2, *part, *split;
     size_t split_elems;
 };
+
+// Scratch of the reconstructed-Householder panel (k ≤ nb ≤ 128): the two
+// Gram matrices / Cholesky factors, −Uᵀ, R, the 16×16 tile inverses, S,
+// and the Gram partials of hr_gram_blocks(m) workgroups.
+inline size_t hr_sq(int k) { return (size_t)cdiv((int64_t)k * k, 64) * 64 + 64; }
+inline int hr_gram_blocks(int64_t m) { return (int)std::min<int64_t>(256, cdiv(m, kGrRB)); }
+inline size_t hr_scratch_elems(int64_t m, int nb)
+{
+    return 4 * hr_sq(nb) + tile_inverse_elems(kQrNB) + 64 + 128 +
+           (size_t)hr_gram_blocks(m) * hr_ntiles(std::min(nb, kQrNB)) * 256;
+}
 
 template <typename T>
 size_t qr_ws_elems(int64_t m, int64_t n, int nb, int64_t *split_elems)
@@ -795,12 +1220,13 @@ size_t qr_ws_elems(int64_t m, int64_t n, int nb, int64_t *split_elems)
     // the caller's stream factors the next panel with the other
     return 2 * ((size_t)m * nb + (size_t)nb * nb + 2 * (size_t)nb * std::max<int64_t>(n, 1) +
                 (size_t)sp) +
-           2 * (size_t)(P + 1) * 2 * kQrNB + 64;
+           2 * (size_t)(P + 1) * 2 * kQrNB + 64 + hr_scratch_elems(m, nb) + 128;
 }
 
 template <typename T>
 int qr_panel(Handle *h, hipStream_t s, int64_t m, int w, T *A, int64_t lda, T *tau, T *Vw,
-             int64_t ldv, T *Tw, int64_t ldt, T *part, unsigned &bar_base, int *err)
+             int64_t ldv, T *Tw, int64_t ldt, T *part, unsigned &bar_base, int *err,
+             const int *run_if = nullptr)
 {
     // 1: LDS-reduction kernel; 2: wave-butterfly kernel, sized to the panel
     // width: eight columns per wave, so narrower panels run fewer waves per
@@ -830,6 +1256,7 @@ int qr_panel(Handle *h, hipStream_t s, int64_t m, int w, T *A, int64_t lda, T *t
     p.bar = h->dbar;
     p.bar_base = bar_base;
     p.err = h->dinfo + 1;
+    p.run_if = run_if;
     if (panel == 1)
         hipLaunchKernelGGL((qr_panel_kernel<T, kQrNB, kQrRPT>), dim3((unsigned)P),
                            dim3(kQrThreads), 0, s, p);
@@ -867,15 +1294,89 @@ int wy_update(hipStream_t s, char trans, int64_t mm, int64_t n2, int64_t k, cons
     return GLA_OK;
 }
 
+// Rows of the register-resident panel kernel's workgroups for width w
+// (the fallback of the reconstructed-Householder path).
+inline int panel_rows(const Handle *h, int w)
+{
+    if (h->qr_panel == 1) return (kQrThreads / kQrNB) * kQrRPT;
+    return w <= 32 ? 512 : (w <= 64 ? 256 : 64 * kQrRPL);
+}
+
+// Whether block column mm×w is factored by the reconstructed-Householder
+// path: qr_hr rows or taller (auto: 512), at least as tall as wide, and
+// within the fallback panel's capacity.
+inline bool use_hr(const Handle *h, int64_t mm, int w)
+{
+    const int64_t minrows = h->qr_hr < 0 ? 512 : h->qr_hr;
+    if (h->qr_hr == 0 || mm < w || mm < minrows || w > kQrNB) return false;
+    return cdiv(mm, panel_rows(h, w)) <= h->num_cus;
+}
+
+// The reconstructed-Householder panel (see hr_copy_kernel … hr_finish_kernel):
+// factors Ab (mm×w, mm ≥ w) in place into R and V, writes τ, the explicit
+// V (Vw, ld mm) and T (Tw, ld w).  Stream-ordered launches on s, no host
+// synchronisation; the column-by-column panel kernel runs last, gated on
+// the device flag, when the path's checks fail.
+template <typename T>
+int hr_gram(hipStream_t s, int64_t m, int k, const T *X, int64_t ldx, T *G, T *part, int *flag,
+            int check)
+{
+    const int P = hr_gram_blocks(m);
+    hipLaunchKernelGGL(hr_gram_kernel<T>, dim3((unsigned)P), dim3(256), 0, s, m, k, X, ldx, part);
+    GLA_LAUNCH_CHECK();
+    hipLaunchKernelGGL(hr_gram_reduce_kernel<T>, dim3((unsigned)hr_ntiles(k)), dim3(1024), 0, s, k,
+                       P, part, G, flag, check);
+    GLA_LAUNCH_CHECK();
+    return GLA_OK;
+}
+
+template <typename T>
+int qr_panel_hr(Handle *h, hipStream_t s, int64_t mm, int w, T *Ab, int64_t lda, T *tau, T *Vw,
+                T *Tw, T *hr, T *part, unsigned &bar_base)
+{
+    const int k = w;
+    int *flag = h->dinfo + 16;
+    const size_t sq = hr_sq(k);
+    T *G1 = hr, *G2 = G1 + sq, *Un = G2 + sq, *Rb = Un + sq, *Linv = Rb + sq;
+    T *Sv = Linv + tile_inverse_elems(kQrNB) + 64;
+    T *gpart = Sv + 128;
+    const dim3 cgrid((unsigned)cdiv(mm, 256), (unsigned)k);
+    hipLaunchKernelGGL(hr_copy_kernel<T>, cgrid, dim3(256), 0, s, mm, Ab, lda, Vw, mm, flag);
+    GLA_LAUNCH_CHECK();
+    // (1) two CholeskyQR passes: Vw <- P R₁⁻¹ R₂⁻¹
+    GLA_TRY(hr_gram<T>(s, mm, k, Ab, lda, G1, gpart, flag, 0));
+    GLA_TRY(chol_factor_block<T>(s, k, G1, k, flag, 0, Linv));
+    GLA_TRY(trsm_rlt<T>(s, mm, k, G1, k, Linv, Vw, mm));
+    GLA_TRY(hr_gram<T>(s, mm, k, Vw, mm, G2, gpart, flag, 1));
+    GLA_TRY(chol_factor_block<T>(s, k, G2, k, flag, 0, Linv));
+    GLA_TRY(trsm_rlt<T>(s, mm, k, G2, k, Linv, Vw, mm));
+    const unsigned nt = (unsigned)cdiv(k, 16);
+    hipLaunchKernelGGL(hr_r_kernel<T>, dim3(nt, nt), dim3(256), 0, s, k, G1, G2, Rb);
+    GLA_LAUNCH_CHECK();
+    // (2)-(3) the signed LU of the top block, T, τ; V₂ = Q₂ (−U)⁻¹
+    hipLaunchKernelGGL(hr_lu_kernel<T>, dim3(1), dim3(kHrThreads), 0, s, k, Vw, mm, flag, Un, Tw,
+                       (int64_t)k, Sv, tau);
+    GLA_LAUNCH_CHECK();
+    GLA_TRY(tile_inverses<T>(s, k, Un, k, Linv));
+    GLA_TRY(trsm_rlt<T>(s, mm - k, k, Un, k, Linv, Vw + k, mm));
+    hipLaunchKernelGGL(hr_finish_kernel<T>, cgrid, dim3(256), 0, s, mm, k, Vw, mm, Rb, Sv, Ab,
+                       lda, flag);
+    GLA_LAUNCH_CHECK();
+    // fallback: the column-by-column panel over the untouched panel
+    return qr_panel<T>(h, s, mm, w, Ab, lda, tau, Vw, mm, Tw, k, part, bar_base, nullptr, flag);
+}
+
 // Factor one block column (mm×w at Ab) into V (explicit, Vw) and T (Tw):
 // one panel launch, or sub-panels of width h->qr_inner with compact-WY
 // updates between them and the block's T assembled from theirs and G = VᵀV.
 template <typename T>
 int qr_block_column(Handle *h, hipStream_t s, int64_t mm, int w, T *Ab, int64_t lda, T *tau,
                     T *Vw, T *Tw, T *W1, T *W2, T *split, size_t split_elems, T *part,
-                    unsigned &bar_base)
+                    unsigned &bar_base, T *hr)
 {
     const int64_t k = std::min<int64_t>(mm, w);
+    if (use_hr(h, mm, w))
+        return qr_panel_hr<T>(h, s, mm, w, Ab, lda, tau, Vw, Tw, hr, part, bar_base);
     // auto (-1): sub-panels of 64 for block columns taller than 12288 rows,
     // where the full-width panel spreads over more workgroups than the
     // 64-wide one and its per-column exchange is slower
@@ -930,6 +1431,11 @@ int qr_blocked_dev(Handle *h, hipStream_t s, int64_t m, int64_t n, T *A, int64_t
         p = split[q] + sp;
     }
     T *part = p;
+    const int64_t P = std::max<int64_t>(1, cdiv(m, 64));
+    // 64-element aligned
+    const size_t hoff = (size_t)(part - reinterpret_cast<T *>(h->ws)) +
+                        2 * (size_t)(P + 1) * 2 * kQrNB + 64;
+    T *hr = reinterpret_cast<T *>(h->ws) + (hoff + 63) / 64 * 64;
     const int64_t npanels = cdiv(std::min(m, n), (int64_t)nb);
     const bool la = h->qr_lookahead && npanels > 2;
 
@@ -948,7 +1454,7 @@ int qr_blocked_dev(Handle *h, hipStream_t s, int64_t m, int64_t n, T *A, int64_t
         T *Ab = A + j0 + j0 * lda;
         const int q = (int)(step & 1);
         GLA_TRY(qr_block_column<T>(h, s, mm, w, Ab, lda, tau + j0, Vw[q], Tw[q], W1[0], W2[0],
-                                   split[0], sp, part, bar_base));        // :123-125
+                                   split[0], sp, part, bar_base, hr));    // :123-125
         if (nn > nb) {                                                    // :126-133
             if (!la) {
                 GLA_TRY(wy_update<T>(s, 'T', mm, nn - w, k, Vw[q], mm, Tw[q], k, Ab + w * lda, lda,

@@ -99,6 +99,14 @@ int gla_set_tuning(gla_handle_t hh, const char *key, int64_t value)
         gla::set_gemm_lds1((int)value);
         return GLA_OK;
     }
+    if (std::strcmp(key, "qr_hr") == 0) {
+        if (value < -1) {
+            gla::set_error("qr_hr must be -1, 0 or a row count");
+            return -3;
+        }
+        h->qr_hr = value;
+        return GLA_OK;
+    }
     if (std::strcmp(key, "qr_lookahead") == 0) {
         if (value != 0 && value != 1) {
             gla::set_error("qr_lookahead must be 0 or 1");

@@ -59,8 +59,13 @@ const char *gla_version(void);
  * taller than 12288 rows and 0 otherwise.  "qr_lookahead": 1 (default)
  * factors the next QR panel while the side stream applies the current
  * block reflector to the rest of the trailing matrix; 0 runs the
- * reference's sequential loop.  V, τ and R are the same Householder
- * sequence in every setting.  "gemm_dl" (process-wide): 0 (default), or
+ * reference's sequential loop.  "qr_hr": block columns at least this
+ * many rows tall (−1, default: 512; 0: none) are factored by two
+ * CholeskyQR passes and a reconstruction of the Householder vectors from
+ * the orthonormal factor (the LU of S − Q₁ with the sign choice that
+ * reflector! makes), checked on the device and falling back to the
+ * column-by-column panel when the block column is too ill-conditioned for
+ * it.  V, τ and R are the same Householder sequence in every setting.  "gemm_dl" (process-wide): 0 (default), or
  * 1/2/3 routes FP64 NT products on 128×128 tiles (SYRK, rankUpdate!, the
  * trailing updates) to a direct-to-LDS kernel with a 4/3/3-deep ring of
  * 8/8/16-wide K slabs; "gemm_lds1" (process-wide): 1 (default) stages

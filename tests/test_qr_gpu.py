@@ -95,6 +95,71 @@ def test_subpanel_widths_and_lookahead_agree(rng, la, inner, m, n, nb):
     assert rel(F.tau.cpu().numpy(), otau) <= tol
 
 
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+@pytest.mark.parametrize("la", [0, 1])
+@pytest.mark.parametrize("m,n,nb", [(300, 128, 128), (1000, 257, 128), (2000, 700, 100),
+                                    (129, 128, 128), (5000, 192, 64), (97, 97, 16),
+                                    (1500, 1500, 128), (400, 3, 1), (700, 200, 33)])
+def test_reconstructed_panel_vs_oracle(rng, dtype, la, m, n, nb):
+    """Every block column through the reconstructed-Householder panel
+    (qr_hr = 1: from one row up): CholeskyQR twice, then the signed LU of
+    the orthonormal factor's top block.  Same V, τ, R as qrBlocked!."""
+    gla = _gla()
+    A = rng.standard_normal((m, n)).astype(dtype)
+    tol = 10 * max(m, n) * eps_of(dtype)
+    ofac, otau = O.qr_blocked(A.astype(np.float64), nb)
+    try:
+        gla.set_tuning("qr_hr", 1)
+        gla.set_tuning("qr_lookahead", la)
+        F = gla.qrBlocked_(gla.colmajor(A, _tdt(dtype)), nb)
+        QtA = gla.to_numpy(gla.adjoint(F[gla.QBlocked]) @ gla.colmajor(A, _tdt(dtype)))
+    finally:
+        gla.set_tuning("qr_hr", -1)
+        gla.set_tuning("qr_lookahead", 1)
+    assert rel(gla.to_numpy(F.factors), ofac) <= tol
+    assert rel(F.tau.cpu().numpy(), otau) <= tol
+    k = min(m, n)
+    assert rel(QtA[:k, :], np.triu(ofac[:k, :])) <= tol
+
+
+@pytest.mark.parametrize("case", ["zero_column", "rank_one", "graded"])
+def test_reconstructed_panel_falls_back(rng, case):
+    """Block columns the Cholesky-based path cannot factor to working
+    accuracy — a zero column (τ = 0 in reflector!), rank deficiency, columns
+    graded over 14 orders of magnitude — fail its device checks and are
+    factored column by column instead: the result is still qrBlocked!'s."""
+    gla = _gla()
+    m, n, nb = 3000, 160, 64
+    A = rng.standard_normal((m, n))
+    if case == "zero_column":
+        A[:, 5] = 0.0
+    elif case == "rank_one":
+        A[:, 64:128] = np.outer(A[:, 64], rng.standard_normal(64))
+    else:
+        A[:, :nb] *= np.logspace(0, -14, nb)[None, :]
+    tol = 10 * max(m, n) * eps_of(np.float64)
+    ofac, otau = O.qr_blocked(A, nb)
+    try:
+        gla.set_tuning("qr_hr", 1)
+        F = gla.qrBlocked_(gla.colmajor(A), nb)
+    finally:
+        gla.set_tuning("qr_hr", -1)
+    fac, tau = gla.to_numpy(F.factors), F.tau.cpu().numpy()
+    assert np.all(np.isfinite(fac)) and np.all(np.isfinite(tau))
+    if case == "zero_column":
+        assert tau[5] == 0.0
+    if case == "rank_one":
+        # trailing columns of a rank-deficient block are rounding noise: compare
+        # the well-determined part and the factorisation itself
+        assert rel(fac[:, :65], ofac[:, :65]) <= tol
+        Q = F[gla.QBlocked]
+        QtA = gla.to_numpy(gla.adjoint(Q) @ gla.colmajor(A))
+        assert np.linalg.norm(np.tril(QtA, -1)) <= tol * np.linalg.norm(A)
+        return
+    assert rel(fac, ofac) <= tol
+    assert rel(tau, otau) <= tol
+
+
 def test_known_reflector():
     """[3;4]: β = −5, v = [1, 0.5], τ = 1.6 (Julia's reflector! convention)."""
     gla = _gla()
