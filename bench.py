@@ -279,8 +279,12 @@ def main():
     ap.add_argument("--no-extras", action="store_true",
                     help="skip the mixed-precision, batched and adjoint sections")
     ap.add_argument("--profile-panel", action="store_true",
-                    help="only run the QR panel launch (8192x128) and a plain copy of the "
+                    help="only run qrBlocked!'s panel (one 8192x128 block column, the "
+                         "reconstructed-Householder launch sequence) and a plain copy of the "
                          "same panel five times each, for hardware-counter runs")
+    ap.add_argument("--profile-column-panel", action="store_true",
+                    help="the same with qrUnblocked! of the panel (the column-by-column "
+                         "panel kernel, one launch)")
     ap.add_argument("--profile-syrk", action="store_true",
                     help="only run the trailing SYRK of the Cholesky's first look-ahead step "
                          "(C 15360x15360 lower -= X X^T, K=512) and a plain 64 MiB copy five "
@@ -309,13 +313,18 @@ def main():
         torch.cuda.synchronize()
         print(json.dumps({"profile_syrk": f"{mm}x{mm} K={kk}", "launches": 5}))
         return
-    if args.profile_panel:
+    if args.profile_panel or args.profile_column_panel:
         P0 = torch.randn(NB_QR, M_QR, dtype=torch.float64, device=dev).t()
         Pw = torch.empty_like(P0)
         ptau = torch.zeros(NB_QR, dtype=torch.float64, device=dev)
         for _ in range(5):
             Pw.copy_(P0)
-            gla.qrUnblocked_(Pw, ptau)
+            if args.profile_panel:
+                gla.qrBlocked_(Pw, NB_QR)
+            else:
+                gla.set_tuning("qr_hr", 0)
+                gla.qrUnblocked_(Pw, ptau)
+                gla.set_tuning("qr_hr", -1)
         torch.cuda.synchronize()
         print(json.dumps({"profile_panel": f"{M_QR}x{NB_QR}", "launches": 5}))
         return
@@ -386,15 +395,22 @@ def main():
     qr_bwd = ((QRx - M0 @ x).norm() / (M0 @ x).norm()).item()
     del R, QRx
 
-    # the QR panel (qr_panel2_kernel, the second largest share of the step's
-    # kernel time): one launch on a fresh 8192×128 panel (qrUnblocked! of the
-    # panel is exactly that launch), algorithmic bytes 16·m·nb (read + write
-    # of the panel)
+    # the QR panel: qrBlocked! of one fresh 8192×128 block column is exactly
+    # the panel's launch sequence (the reconstructed-Householder path, no
+    # trailing update); qrUnblocked! of the same panel with that path off is
+    # one launch of the column-by-column kernel.  Algorithmic bytes 16·m·nb (read + write of
+    # the panel).
     P0 = M0[:, :NB_QR].contiguous().t().contiguous().t()
     Pw = torch.empty_like(P0)
     ptau = torch.zeros(NB_QR, dtype=torch.float64, device=dev)
     t_pc = ev_time(torch, lambda: Pw.copy_(P0), reps=3)
-    t_panel = ev_time(torch, lambda: (Pw.copy_(P0), gla.qrUnblocked_(Pw, ptau)), reps=5) - t_pc
+    t_panel = ev_time(torch, lambda: (Pw.copy_(P0), gla.qrBlocked_(Pw, NB_QR)), reps=5) - t_pc
+    gla.set_tuning("qr_hr", 0)          # the column-by-column panel kernel
+    try:
+        t_colpanel = ev_time(torch, lambda: (Pw.copy_(P0), gla.qrUnblocked_(Pw, ptau)),
+                             reps=3) - t_pc
+    finally:
+        gla.set_tuning("qr_hr", -1)
     panel_bytes = 16.0 * M_QR * NB_QR
     achieved = panel_bytes / t_panel / 1e9
     # the dominant kernel (largest share of the step's kernel time): the
@@ -448,10 +464,11 @@ def main():
                      "us_per_launch": t_syrk * 1e6},
         "roofline_panel": {"bound": "hbm", "achieved": achieved, "peak": 8000.0, "unit": "GB/s",
                            "frac": achieved / 8000.0,
-                           "traffic": profiled_traffic("qr_panel_8192x128.md"),
-                           "kernel": "qr_panel2_kernel<double,128,2,1024> (panel 8192x128, "
-                                     "64 workgroups)",
-                           "bytes_per_launch": panel_bytes, "us_per_launch": t_panel * 1e6},
+                           "traffic": profiled_traffic("qr_hr_panel_8192x128.md"),
+                           "kernel": "reconstructed-Householder panel of qrBlocked! (block "
+                                     "column 8192x128; its 15-launch sequence timed as one)",
+                           "bytes_per_launch": panel_bytes, "us_per_launch": t_panel * 1e6,
+                           "column_kernel_us": t_colpanel * 1e6},
     }
     del A0, Aw, M0, Mw
     torch.cuda.empty_cache()

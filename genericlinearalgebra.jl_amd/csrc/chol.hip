@@ -418,6 +418,27 @@ __global__ __launch_bounds__(256) void tile_inverse_kernel(int64_t n, const T *_
 // read into LDS once; then for each column block kb the wave's tile kb
 // becomes X = B_kb L_kk⁻ᵀ and its tiles j > kb take B_j -= X L_(j,kb)ᵀ, all
 // on MFMA, with no further barrier (every wave touches only its own rows).
+// Tile q of the strictly-lower tiles (bj, kb), bj > kb, of an 8×8-tile
+// block, stored column block after column block: its kb and bj.
+__host__ __device__ constexpr int lt_kb(int q)
+{
+    int kb = 0, base = 0;
+    while (base + (7 - kb) <= q) {
+        base += 7 - kb;
+        ++kb;
+    }
+    return kb;
+}
+__host__ __device__ constexpr int lt_bj(int q)
+{
+    int kb = 0, base = 0;
+    while (base + (7 - kb) <= q) {
+        base += 7 - kb;
+        ++kb;
+    }
+    return kb + 1 + (q - base);
+}
+
 template <typename T>
 __global__ __launch_bounds__(kPanelThreads) void chol_solve_kernel(int nb, const T *__restrict__ L,
                                                                    int64_t ldl,
@@ -446,20 +467,25 @@ __global__ __launch_bounds__(kPanelThreads) void chol_solve_kernel(int nb, const
     // index of tile (bj, kb), bj > kb, in Lt: column blocks stored one after another
     auto lt_index = [](int bj, int kb) { return 7 * kb - kb * (kb - 1) / 2 + (bj - kb - 1); };
 
-    for (int e = threadIdx.x; e < NLT * 256; e += kPanelThreads) {
-        const int tile = e >> 8, i = (e >> 4) & 15, j = e & 15;
-        // recover (bj, kb) of the tile
-        int kb = 0, base = 0;
-        while (base + (7 - kb) <= tile) {
-            base += 7 - kb;
-            ++kb;
+    // thread t stages element (i, j) = (t / 16, t % 16) of every tile: all
+    // loads are issued before the first LDS store
+    static_assert(kPanelThreads == 256, "one tile element per thread");
+    {
+        const int i = (threadIdx.x >> 4) & 15, j = threadIdx.x & 15;
+        T lv[NLT], iv[NBLK];
+#pragma unroll
+        for (int q = 0; q < NLT; ++q) {
+            const int kb = lt_kb(q), bj = lt_bj(q);
+            const int gi = bj * 16 + i, gj = kb * 16 + j;
+            lv[q] = (gi < nb && gj < nb) ? L[gi + (int64_t)gj * ldl] : T(0);
         }
-        const int bj = kb + 1 + (tile - base);
-        const int gi = bj * 16 + i, gj = kb * 16 + j;
-        Lt[tile * TS + i * kXLD + j] = (gi < nb && gj < nb) ? L[gi + (int64_t)gj * ldl] : T(0);
+#pragma unroll
+        for (int q = 0; q < NBLK; ++q) iv[q] = q < nkb ? LinvG[q * 256 + threadIdx.x] : T(0);
+#pragma unroll
+        for (int q = 0; q < NLT; ++q) Lt[q * TS + i * kXLD + j] = lv[q];
+#pragma unroll
+        for (int q = 0; q < NBLK; ++q) Li[q * TS + i * kXLD + j] = iv[q];
     }
-    for (int e = threadIdx.x; e < nkb * 256; e += kPanelThreads)
-        Li[(e >> 8) * TS + ((e >> 4) & 15) * kXLD + (e & 15)] = LinvG[e];
 
     acc_t acc[NBLK];
 #pragma unroll

@@ -89,7 +89,8 @@ struct Handle {
     // Depth-1 look-ahead in qrBlocked! (trailing update on the side stream).
     int qr_lookahead = 1;
     // Block columns at least this tall are factored by the
-    // reconstructed-Householder panel (0: never; -1: 512 rows).
+    // reconstructed-Householder panel (0: never; -1: any height for block
+    // columns at least 64 wide, 2048 rows for narrower ones).
     int64_t qr_hr = -1;
 };
 

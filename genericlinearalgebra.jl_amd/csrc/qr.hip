@@ -979,199 +979,300 @@ __device__ __forceinline__ T wave_bcast(T v, int src)
 
 constexpr int kHrLD = 129;   // LDS leading dimension of the top block
 constexpr int kHrThreads = 256;
+constexpr int kHrTL = 17;    // leading dimension of the 16×16 inverses in LDS
 
-// Steps (2)-(3) on the top block, one workgroup of 4 waves, k ≤ 128, the
-// block in LDS (M, column-major).  Both sweeps are blocked by 16 columns:
-//  LU of S − Q[0:k, :]: wave 0 eliminates the 16-column panel M[jb:k, J]
-//    column by column in registers (lane l holds local rows l and l + 64;
-//    the pivot row lives in lane c and is broadcast through SGPRs),
-//    choosing S_jj from the sign of the updated diagonal; then every wave:
-//    U12 = L11⁻¹ M[J, right] (one column per thread), and the trailing
-//    block M[below, right] −= L21 U12 (wave w takes columns w, w+4, …).
-//  T = Û V₁⁻ᵀ, Û = U S, in place over U: per column block J,
-//    T[:, J] = Acc[:, J] V₁[J, J]⁻ᵀ (one row per thread), then
-//    Acc[:, right] −= T[:, J] V₁[right, J]ᵀ.  T is upper triangular and V₁
-//    strictly lower, so both live in M at once.
-// Writes V₁ (explicit: unit diagonal, zero above) over Q's top block,
-// Un = −Uᵀ (lower, ld k) for the solve of the rows below, T (zero below the
-// diagonal), S (Sv) and τ.
+__device__ __forceinline__ void hr_wave_order() { __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront"); }
+
+// 1/u for the pivots of the top-block LU (|u| ∈ [1, 2]): the hardware
+// reciprocal refined by two Newton steps, r ← r + r(1 − u r), which takes
+// it to within an ulp of the quotient.
+template <typename T>
+__device__ __forceinline__ T recip1(T u)
+{
+    T r;
+    if constexpr (sizeof(T) == 8) r = __builtin_amdgcn_rcp(u);
+    else r = __builtin_amdgcn_rcpf(u);
+    r = __builtin_fma(__builtin_fma(-u, r, T(1)), r, r);
+    r = __builtin_fma(__builtin_fma(-u, r, T(1)), r, r);
+    return r;
+}
+
+// Steps (2)-(3) on the top block: one workgroup of 4 waves, k ≤ 128, the
+// block in LDS (M, column-major), blocked by 16 columns as a right-looking
+// LU without pivoting.  Per block J = jb..jb+15:
+//   wave 0 eliminates the 16×16 diagonal block column by column (lane r
+//   holds row r; the pivot row is broadcast through SGPRs), choosing
+//   S_jj = copysign(1, −q̂_jj) from the updated diagonal entry — which
+//   depends on this block alone — and forms W_J = L_JJ⁻¹ and Z = U_JJ⁻¹;
+//   then, on MFMA over 16×16 tiles: U_J,right = W_J M_J,right,
+//   L_below,J = M_below,J Z, and M_below,right −= L_below,J U_J,right.
+// Then T = Û V₁⁻ᵀ (Û = U S) in place over U, by column blocks:
+//   T[:, J] = Acc[:, J] W_Jᵀ, Acc[:, right] −= T[:, J] V₁[right, J]ᵀ
+// (T upper and V₁ strictly lower share M).  Writes V₁ (explicit: unit
+// diagonal, zero above) over Q's top block, Un = −Uᵀ (lower, ld k) for the
+// solve of the rows below, T (zero below the diagonal), S (Sv) and τ.
 template <typename T>
 __global__ __launch_bounds__(kHrThreads) void hr_lu_kernel(int k, T *Q, int64_t ldq,
                                                            const int *flag, T *Un, T *Tw,
                                                            int64_t ldt, T *Sv, T *tau)
 {
+    typedef PanelMfma<T> Mf;
+    typedef typename Mf::acc_t acc_t;
     __shared__ T M[128 * kHrLD];
+    __shared__ T Ws[8][16 * kHrTL];
+    __shared__ T Zs[16 * kHrTL];
     __shared__ T Dv[128], Uv[128];
     if (*flag != 0) return;
     const int t = threadIdx.x, lane = t & 63;
     const int w = __builtin_amdgcn_readfirstlane(t >> 6);
-#pragma unroll 8
-    for (int l = w; l < 128; l += kHrThreads / 64)
+    const int li = lane & 15, lk = lane >> 4;
+    {
+#pragma unroll 1
+      for (int q0 = 0; q0 < 32; q0 += 8) {
+        T v[8][2];
 #pragma unroll
-        for (int s = 0; s < 2; ++s) {
-            const int i = lane + 64 * s;
-            M[i + l * kHrLD] = (i < k && l < k) ? -Q[i + (int64_t)l * ldq] : T(0);
-        }
+        for (int qq = 0; qq < 8; ++qq)
+#pragma unroll
+            for (int s = 0; s < 2; ++s) {
+                const int i = lane + 64 * s, l = w + 4 * (q0 + qq);
+                v[qq][s] = (i < k && l < k) ? -Q[i + (int64_t)l * ldq] : T(0);
+            }
+#pragma unroll
+        for (int qq = 0; qq < 8; ++qq)
+#pragma unroll
+            for (int s = 0; s < 2; ++s) M[lane + 64 * s + (w + 4 * (q0 + qq)) * kHrLD] = v[qq][s];
+      }
+    }
     __syncthreads();
 
-    // ---- LU with sign choice
+    // one 16×16 tile product on MFMA: acc += A B, A(i, kk) = a(i, kk),
+    // B(kk, j) = b(kk, j); element r of lane l of the result sits at
+    // (Mf::row(l, r), l % 16)
     for (int jb = 0; jb < k; jb += 16) {
         const int nbk = min(16, k - jb), c0 = jb + nbk;
+        T *Wb = Ws[jb >> 4];
         if (w == 0) {
-            T pa[2][16];
+            // ---- the diagonal block, lanes 0..15 (row r = lane)
+            const int r = li;
+            T d[16];
 #pragma unroll
-            for (int s = 0; s < 2; ++s)
-#pragma unroll
-                for (int c = 0; c < 16; ++c) {
-                    const int i = jb + lane + 64 * s;
-                    pa[s][c] = (i < k && c < nbk) ? M[i + (jb + c) * kHrLD] : T(0);
-                }
+            for (int c = 0; c < 16; ++c) d[c] = M[jb + r + (jb + c) * kHrLD];
+            // scheduling fences between the chained stages of this wave's
+            // work keep their loads from being hoisted into the elimination
+            __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
             for (int c = 0; c < 16; ++c) {
                 if (c < nbk) {
-                    const T mjj = wave_bcast(pa[0][c], c);    // −q̂_jj
+                    const T mjj = wave_bcast(d[c], c);        // −q̂_jj
                     const T dj = dcopysign<T>(T(1), mjj);
                     const T uj = mjj + dj;                    // |uj| = 1 + |q̂_jj|
-                    const T ru = T(1) / uj;
+                    const T ru = recip1(uj);
                     T rowc[16];
 #pragma unroll
-                    for (int c2 = 0; c2 < 16; ++c2)
-                        rowc[c2] = c2 > c ? wave_bcast(pa[0][c2], c) : T(0);
+                    for (int c2 = 0; c2 < 16; ++c2) rowc[c2] = c2 > c ? wave_bcast(d[c2], c) : T(0);
+                    if (r > c) {
+                        const T l = d[c] * ru;
+                        d[c] = l;
 #pragma unroll
-                    for (int s = 0; s < 2; ++s) {
-                        if (lane + 64 * s > c) {
-                            const T l = pa[s][c] * ru;
-                            pa[s][c] = l;
-#pragma unroll
-                            for (int c2 = 0; c2 < 16; ++c2)
-                                if (c2 > c) pa[s][c2] -= l * rowc[c2];
-                        }
+                        for (int c2 = 0; c2 < 16; ++c2)
+                            if (c2 > c) d[c2] -= l * rowc[c2];
                     }
-                    if (lane == c) pa[0][c] = uj;
+                    if (r == c) d[c] = uj;
                     if (lane == 0) {
                         Dv[jb + c] = dj;
                         Uv[jb + c] = uj;
                     }
                 }
             }
+            __builtin_amdgcn_sched_barrier(0);
+            if (lane < 16 && jb + r < 128) {
 #pragma unroll
-            for (int s = 0; s < 2; ++s)
+                for (int c = 0; c < 16; ++c) M[jb + r + (jb + c) * kHrLD] = d[c];
+            }
+            hr_wave_order();
+            __builtin_amdgcn_sched_barrier(0);
+            // W = L⁻¹ (unit lower) and Z = U⁻¹ (upper), column j = lane,
+            // by column sweeps (the updates of one step are independent)
+            if (lane < 16) {
+                const int j = lane;
+                T x[16];
 #pragma unroll
-                for (int c = 0; c < 16; ++c) {
-                    const int i = jb + lane + 64 * s;
-                    if (i < k && c < nbk) M[i + (jb + c) * kHrLD] = pa[s][c];
+                for (int rr = 0; rr < 16; ++rr) x[rr] = rr == j ? T(1) : T(0);
+#pragma unroll
+                for (int s = 0; s < 15; ++s)
+#pragma unroll
+                    for (int rr = s + 1; rr < 16; ++rr) x[rr] -= M[jb + rr + (jb + s) * kHrLD] * x[s];
+#pragma unroll
+                for (int rr = 0; rr < 16; ++rr) Wb[rr + j * kHrTL] = x[rr];
+            }
+            __builtin_amdgcn_sched_barrier(0);
+            if (lane < 16 && c0 < k) {
+                const int j = lane;
+                T x[16];
+                {
+                    T rd[16];
+#pragma unroll
+                    for (int rr = 0; rr < 16; ++rr) {
+                        rd[rr] = recip1(M[jb + rr + (jb + rr) * kHrLD]);
+                        x[rr] = rr == j ? T(1) : T(0);
+                    }
+#pragma unroll
+                    for (int s = 15; s >= 0; --s) {
+                        x[s] *= rd[s];
+#pragma unroll
+                        for (int rr = 0; rr < s; ++rr) x[rr] -= M[jb + rr + (jb + s) * kHrLD] * x[s];
+                    }
+#pragma unroll
+                    for (int rr = 0; rr < 16; ++rr) Zs[rr + j * kHrTL] = x[rr];
                 }
+            }
         }
         __syncthreads();
         if (c0 >= k) break;
-        // U12 = L11⁻¹ M[J, c0:k]
-        if (t < k - c0) {
-            const int cc = c0 + t;
-            T x[16];
+        const int nct = (k - c0 + 15) / 16;
+        // ---- U_J,right = W M_J,right and L_below,J = M_below,J Z
+        for (int it = w; it < 2 * nct; it += kHrThreads / 64) {
+            const int o = c0 + 16 * (it >> 1);
+            acc_t acc = {T(0), T(0), T(0), T(0)};
+            if ((it & 1) == 0) {
 #pragma unroll
-            for (int r = 0; r < 16; ++r) x[r] = r < nbk ? M[jb + r + cc * kHrLD] : T(0);
+                for (int kq = 0; kq < 4; ++kq) {
+                    const int kk = 4 * kq + lk;
+                    acc = Mf::mma(Wb[li + kk * kHrTL], M[jb + kk + (o + li) * kHrLD], acc);
+                }
 #pragma unroll
-            for (int r = 1; r < 16; ++r)
+                for (int q = 0; q < 4; ++q) M[jb + Mf::row(lane, q) + (o + li) * kHrLD] = acc[q];
+            } else {
 #pragma unroll
-                for (int s = 0; s < r; ++s) x[r] -= M[jb + r + (jb + s) * kHrLD] * x[s];
+                for (int kq = 0; kq < 4; ++kq) {
+                    const int kk = 4 * kq + lk;
+                    acc = Mf::mma(M[o + li + (jb + kk) * kHrLD], Zs[kk + li * kHrTL], acc);
+                }
 #pragma unroll
-            for (int r = 0; r < 16; ++r)
-                if (r < nbk) M[jb + r + cc * kHrLD] = x[r];
+                for (int q = 0; q < 4; ++q) M[o + Mf::row(lane, q) + (jb + li) * kHrLD] = acc[q];
+            }
         }
         __syncthreads();
-        // M[c0:k, c0:k] −= L21 U12
-        {
-            T lr[2][16];
+        // ---- M_below,right −= L_below,J U_J,right
+        for (int it = w; it < nct * nct; it += kHrThreads / 64) {
+            const int io = c0 + 16 * (it % nct), jo = c0 + 16 * (it / nct);
+            acc_t acc;
 #pragma unroll
-            for (int s = 0; s < 2; ++s)
+            for (int q = 0; q < 4; ++q) acc[q] = M[io + Mf::row(lane, q) + (jo + li) * kHrLD];
 #pragma unroll
-                for (int c = 0; c < 16; ++c) {
-                    const int i = c0 + lane + 64 * s;
-                    lr[s][c] = (i < k && c < nbk) ? M[i + (jb + c) * kHrLD] : T(0);
-                }
-            for (int cc = c0 + w; cc < k; cc += kHrThreads / 64) {
-                T u[16];
-#pragma unroll
-                for (int c = 0; c < 16; ++c) u[c] = M[jb + c + cc * kHrLD];
-#pragma unroll
-                for (int s = 0; s < 2; ++s) {
-                    const int i = c0 + lane + 64 * s;
-                    if (i < k) {
-                        T acc = M[i + cc * kHrLD];
-#pragma unroll
-                        for (int c = 0; c < 16; ++c) acc -= lr[s][c] * u[c];
-                        M[i + cc * kHrLD] = acc;
-                    }
-                }
+            for (int kq = 0; kq < 4; ++kq) {
+                const int kk = 4 * kq + lk;
+                acc = Mf::mma(-M[io + li + (jb + kk) * kHrLD], M[jb + kk + (jo + li) * kHrLD], acc);
             }
+#pragma unroll
+            for (int q = 0; q < 4; ++q) M[io + Mf::row(lane, q) + (jo + li) * kHrLD] = acc[q];
         }
         __syncthreads();
     }
 
-    // ---- outputs of the LU; Û = U S in place
-    for (int l = w; l < k; l += kHrThreads / 64)
-        for (int i = lane; i < k; i += 64) {
-            const T v = M[i + l * kHrLD];
-            Q[i + (int64_t)l * ldq] = i > l ? v : (i == l ? T(1) : T(0));
-            if (i <= l) {
-                Un[l + (int64_t)i * k] = -v;
-                M[i + l * kHrLD] = v * Dv[l];
+    // ---- outputs of the LU: V₁ over Q's top block, Û = U S in place
+    {
+#pragma unroll 1
+      for (int q0 = 0; q0 < 32; q0 += 8) {
+        T v[8][2];
+#pragma unroll
+        for (int qq = 0; qq < 8; ++qq)
+#pragma unroll
+            for (int s = 0; s < 2; ++s) v[qq][s] = M[lane + 64 * s + (w + 4 * (q0 + qq)) * kHrLD];
+#pragma unroll
+        for (int qq = 0; qq < 8; ++qq)
+#pragma unroll
+            for (int s = 0; s < 2; ++s) {
+                const int i = lane + 64 * s, l = w + 4 * (q0 + qq);
+                if (i < k && l < k) {
+                    Q[i + (int64_t)l * ldq] = i > l ? v[qq][s] : (i == l ? T(1) : T(0));
+                    if (i <= l) M[i + l * kHrLD] = v[qq][s] * Dv[l];
+                }
             }
-        }
+      }
+    }
     if (t < k) {
         Sv[t] = Dv[t];
         tau[t] = Dv[t] * Uv[t];
     }
     __syncthreads();
+    // Un(l, i) = −U(i, l) = −Û(i, l) S_l, written down the columns of Un
+    {
+#pragma unroll 1
+      for (int q0 = 0; q0 < 32; q0 += 8) {
+        T v[8][2];
+#pragma unroll
+        for (int qq = 0; qq < 8; ++qq)
+#pragma unroll
+            for (int s = 0; s < 2; ++s) v[qq][s] = M[w + 4 * (q0 + qq) + (lane + 64 * s) * kHrLD];
+#pragma unroll
+        for (int qq = 0; qq < 8; ++qq)
+#pragma unroll
+            for (int s = 0; s < 2; ++s) {
+                const int i = w + 4 * (q0 + qq), l = lane + 64 * s;
+                if (i <= l && l < k) Un[l + (int64_t)i * k] = -v[qq][s] * Dv[l];
+            }
+      }
+    }
+    __syncthreads();   // the T sweep below rewrites M
 
     // ---- T = Û V₁⁻ᵀ
     for (int jb = 0; jb < k; jb += 16) {
-        const int nbk = min(16, k - jb), c0 = jb + nbk;
-        // rows 0 .. c0-1 hold T's nonzeros in columns J
-        if (t < c0) {
-            T x[16];
+        const int c0 = min(k, jb + 16);
+        const T *Wb = Ws[jb >> 4];
+        // T[0:c0, J] = Acc[0:c0, J] W_Jᵀ (Acc(i, c) for i > c is V₁: zero)
+        for (int it = w; it <= (jb >> 4); it += kHrThreads / 64) {
+            const int io = 16 * it;
+            acc_t acc = {T(0), T(0), T(0), T(0)};
 #pragma unroll
-            for (int c = 0; c < 16; ++c) x[c] = (c < nbk && t <= jb + c) ? M[t + (jb + c) * kHrLD] : T(0);
+            for (int kq = 0; kq < 4; ++kq) {
+                const int kk = 4 * kq + lk;
+                const T a = (io + li <= jb + kk) ? M[io + li + (jb + kk) * kHrLD] : T(0);
+                acc = Mf::mma(a, Wb[li + kk * kHrTL], acc);
+            }
 #pragma unroll
-            for (int c = 1; c < 16; ++c)
-#pragma unroll
-                for (int c2 = 0; c2 < c; ++c2) x[c] -= x[c2] * M[jb + c + (jb + c2) * kHrLD];
-#pragma unroll
-            for (int c = 0; c < 16; ++c)
-                if (c < nbk && t <= jb + c) M[t + (jb + c) * kHrLD] = x[c];
-        }
-        __syncthreads();
-        if (c0 >= k) break;
-        {
-            T tr[2][16];
-#pragma unroll
-            for (int s = 0; s < 2; ++s)
-#pragma unroll
-                for (int c = 0; c < 16; ++c) {
-                    const int i = lane + 64 * s;
-                    tr[s][c] = (i < c0 && c < nbk && i <= jb + c) ? M[i + (jb + c) * kHrLD] : T(0);
-                }
-            for (int cc = c0 + w; cc < k; cc += kHrThreads / 64) {
-                T v[16];
-#pragma unroll
-                for (int c = 0; c < 16; ++c) v[c] = c < nbk ? M[cc + (jb + c) * kHrLD] : T(0);
-#pragma unroll
-                for (int s = 0; s < 2; ++s) {
-                    const int i = lane + 64 * s;
-                    if (i < c0) {
-                        T acc = M[i + cc * kHrLD];
-#pragma unroll
-                        for (int c = 0; c < 16; ++c) acc -= tr[s][c] * v[c];
-                        M[i + cc * kHrLD] = acc;
-                    }
-                }
+            for (int q = 0; q < 4; ++q) {
+                const int i = io + Mf::row(lane, q), c = jb + li;
+                if (i <= c) M[i + c * kHrLD] = acc[q];
             }
         }
         __syncthreads();
+        if (c0 >= k) break;
+        const int nct = (k - c0 + 15) / 16, nrt = (jb >> 4) + 1;
+        // Acc[0:c0, right] −= T[0:c0, J] V₁[right, J]ᵀ
+        for (int it = w; it < nrt * nct; it += kHrThreads / 64) {
+            const int io = 16 * (it % nrt), jo = c0 + 16 * (it / nrt);
+            acc_t acc;
+#pragma unroll
+            for (int q = 0; q < 4; ++q) acc[q] = M[io + Mf::row(lane, q) + (jo + li) * kHrLD];
+#pragma unroll
+            for (int kq = 0; kq < 4; ++kq) {
+                const int kk = 4 * kq + lk;
+                const T a = (io + li <= jb + kk) ? -M[io + li + (jb + kk) * kHrLD] : T(0);
+                acc = Mf::mma(a, M[jo + li + (jb + kk) * kHrLD], acc);
+            }
+#pragma unroll
+            for (int q = 0; q < 4; ++q) M[io + Mf::row(lane, q) + (jo + li) * kHrLD] = acc[q];
+        }
+        __syncthreads();
     }
-    for (int l = w; l < k; l += kHrThreads / 64)
-        for (int i = lane; i < k; i += 64)
-            Tw[i + (int64_t)l * ldt] = i <= l ? M[i + l * kHrLD] : T(0);
+    {
+#pragma unroll 1
+      for (int q0 = 0; q0 < 32; q0 += 8) {
+        T v[8][2];
+#pragma unroll
+        for (int qq = 0; qq < 8; ++qq)
+#pragma unroll
+            for (int s = 0; s < 2; ++s) v[qq][s] = M[lane + 64 * s + (w + 4 * (q0 + qq)) * kHrLD];
+#pragma unroll
+        for (int qq = 0; qq < 8; ++qq)
+#pragma unroll
+            for (int s = 0; s < 2; ++s) {
+                const int i = lane + 64 * s, l = w + 4 * (q0 + qq);
+                if (i < k && l < k) Tw[i + (int64_t)l * ldt] = i <= l ? v[qq][s] : T(0);
+            }
+      }
+    }
 }
 
 // The reconstructed factors over the panel, if the path succeeded: rows
@@ -1303,11 +1404,12 @@ inline int panel_rows(const Handle *h, int w)
 }
 
 // Whether block column mm×w is factored by the reconstructed-Householder
-// path: qr_hr rows or taller (auto: 512), at least as tall as wide, and
-// within the fallback panel's capacity.
+// path: at least as tall as wide, qr_hr rows or taller (auto: any height
+// for w ≥ 64, where it was faster from 128×128 up; 2048 rows for narrower
+// block columns), and within the fallback panel's capacity.
 inline bool use_hr(const Handle *h, int64_t mm, int w)
 {
-    const int64_t minrows = h->qr_hr < 0 ? 512 : h->qr_hr;
+    const int64_t minrows = h->qr_hr >= 0 ? h->qr_hr : (w >= 64 ? 1 : 2048);
     if (h->qr_hr == 0 || mm < w || mm < minrows || w > kQrNB) return false;
     return cdiv(mm, panel_rows(h, w)) <= h->num_cus;
 }

@@ -60,7 +60,8 @@ const char *gla_version(void);
  * factors the next QR panel while the side stream applies the current
  * block reflector to the rest of the trailing matrix; 0 runs the
  * reference's sequential loop.  "qr_hr": block columns at least this
- * many rows tall (−1, default: 512; 0: none) are factored by two
+ * many rows tall (−1, default: any height for block columns ≥ 64 wide,
+ * 2048 rows for narrower ones; 0: none) are factored by two
  * CholeskyQR passes and a reconstruction of the Householder vectors from
  * the orthonormal factor (the LU of S − Q₁ with the sign choice that
  * reflector! makes), checked on the device and falling back to the
