@@ -1003,8 +1003,8 @@ __device__ __forceinline__ T recip1(T u)
 //   wave 0 eliminates the 16×16 diagonal block column by column (lane r
 //   holds row r; the pivot row is broadcast through SGPRs), choosing
 //   S_jj = copysign(1, −q̂_jj) from the updated diagonal entry — which
-//   depends on this block alone — and forms W_J = L_JJ⁻¹ and Z = U_JJ⁻¹;
-//   then, on MFMA over 16×16 tiles: U_J,right = W_J M_J,right,
+//   depends on this block alone; waves 1 and 2 then form W_J = L_JJ⁻¹ and
+//   Z = U_JJ⁻¹ side by side; then, on MFMA over 16×16 tiles: U_J,right = W_J M_J,right,
 //   L_below,J = M_below,J Z, and M_below,right −= L_below,J U_J,right.
 // Then T = Û V₁⁻ᵀ (Û = U S) in place over U, by column blocks:
 //   T[:, J] = Acc[:, J] W_Jᵀ, Acc[:, right] −= T[:, J] V₁[right, J]ᵀ
@@ -1028,17 +1028,17 @@ __global__ __launch_bounds__(kHrThreads) void hr_lu_kernel(int k, T *Q, int64_t 
     const int li = lane & 15, lk = lane >> 4;
     {
 #pragma unroll 1
-      for (int q0 = 0; q0 < 32; q0 += 8) {
-        T v[8][2];
+      for (int q0 = 0; q0 < 32; q0 += 16) {
+        T v[16][2];
 #pragma unroll
-        for (int qq = 0; qq < 8; ++qq)
+        for (int qq = 0; qq < 16; ++qq)
 #pragma unroll
             for (int s = 0; s < 2; ++s) {
                 const int i = lane + 64 * s, l = w + 4 * (q0 + qq);
                 v[qq][s] = (i < k && l < k) ? -Q[i + (int64_t)l * ldq] : T(0);
             }
 #pragma unroll
-        for (int qq = 0; qq < 8; ++qq)
+        for (int qq = 0; qq < 16; ++qq)
 #pragma unroll
             for (int s = 0; s < 2; ++s) M[lane + 64 * s + (w + 4 * (q0 + qq)) * kHrLD] = v[qq][s];
       }
@@ -1089,10 +1089,12 @@ __global__ __launch_bounds__(kHrThreads) void hr_lu_kernel(int k, T *Q, int64_t 
 #pragma unroll
                 for (int c = 0; c < 16; ++c) M[jb + r + (jb + c) * kHrLD] = d[c];
             }
-            hr_wave_order();
-            __builtin_amdgcn_sched_barrier(0);
-            // W = L⁻¹ (unit lower) and Z = U⁻¹ (upper), column j = lane,
-            // by column sweeps (the updates of one step are independent)
+        }
+        __syncthreads();
+        // W = L⁻¹ (unit lower, wave 1) and Z = U⁻¹ (upper, wave 2) at once,
+        // column j = lane, by column sweeps (the updates of one step are
+        // independent)
+        if (w == 1) {
             if (lane < 16) {
                 const int j = lane;
                 T x[16];
@@ -1105,7 +1107,7 @@ __global__ __launch_bounds__(kHrThreads) void hr_lu_kernel(int k, T *Q, int64_t 
 #pragma unroll
                 for (int rr = 0; rr < 16; ++rr) Wb[rr + j * kHrTL] = x[rr];
             }
-            __builtin_amdgcn_sched_barrier(0);
+        } else if (w == 2) {
             if (lane < 16 && c0 < k) {
                 const int j = lane;
                 T x[16];
