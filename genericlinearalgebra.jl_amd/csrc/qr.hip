@@ -840,6 +840,10 @@ __global__ void hr_copy_kernel(int64_t m, const T *A, int64_t lda, T *Q, int64_t
 // hr_gram_reduce_kernel adds the P partials in a fixed order.
 constexpr int kGrRB = 32;
 constexpr int kGrLD = 132;
+// most workgroups of the Gram pass (one 32-row chunk each up to 8192 rows;
+// 64 workgroups of four chunks were slower: 31 + 5 µs against 14 + 10 µs
+// for the pass and its reduction at 8192×128)
+constexpr int kGrBlocks = 256;
 
 __host__ __device__ inline int hr_ntiles(int k)
 {
@@ -1303,7 +1307,7 @@ struct QrWs {
 // Gram matrices / Cholesky factors, −Uᵀ, R, the 16×16 tile inverses, S,
 // and the Gram partials of hr_gram_blocks(m) workgroups.
 inline size_t hr_sq(int k) { return (size_t)cdiv((int64_t)k * k, 64) * 64 + 64; }
-inline int hr_gram_blocks(int64_t m) { return (int)std::min<int64_t>(256, cdiv(m, kGrRB)); }
+inline int hr_gram_blocks(int64_t m) { return (int)std::min<int64_t>(kGrBlocks, cdiv(m, kGrRB)); }
 inline size_t hr_scratch_elems(int64_t m, int nb)
 {
     return 4 * hr_sq(nb) + tile_inverse_elems(kQrNB) + 64 + 128 +
