@@ -339,6 +339,25 @@ __global__ __launch_bounds__(kPanelThreads) void chol_factor_kernel(int nb, T *_
     }
 }
 
+// The same, run only if *run_if != 0 (a device-side choice between this
+// factorisation and a cheaper one made by an earlier kernel).
+template <typename T>
+__global__ __launch_bounds__(kPanelThreads) void chol_factor_gated_kernel(
+    int nb, T *__restrict__ A, int64_t lda, int *info, int64_t info_base, T *__restrict__ LinvG,
+    const int *run_if)
+{
+    __shared__ T Xb[2 * kPanelNB * kXLD];
+    __shared__ T Db[2 * 16 * kXLD];
+    __shared__ T Lib[2 * 16 * kXLD];
+    if (*run_if == 0) return;
+    switch (threadIdx.x >> 6) {
+    case 0: factor_body<T, 0>(nb, A, lda, info, info_base, LinvG, Xb, Db, Lib); break;
+    case 1: factor_body<T, 1>(nb, A, lda, info, info_base, LinvG, Xb, Db, Lib); break;
+    case 2: factor_body<T, 2>(nb, A, lda, info, info_base, LinvG, Xb, Db, Lib); break;
+    default: factor_body<T, 3>(nb, A, lda, info, info_base, LinvG, Xb, Db, Lib); break;
+    }
+}
+
 // The same factorisation of an FP32 diagonal block computed in FP64: the
 // block is read and written back in FP32, the tile inverses are stored in
 // FP32 (Linv32).
@@ -715,10 +734,15 @@ int chol_la_dev(Handle *h, hipStream_t s, int64_t n, T *A, int64_t lda, int NB,
         GLA_TRY(trsm_rlt_dev<T>(s, below, kb, Akk, lda, Linv, A21, lda));
         const int64_t kb2 = std::min<int64_t>(NB, below);
         const int64_t rest = below - kb2;
-        if (overlap && rest > 0) GLA_HIP_TRY(hipEventRecord(h->ev_step[0], s));
+        // the side stream's SYRK is released after the next-block-column
+        // update (chol_syrk_late, default), which then runs without the
+        // SYRK's workgroups crowding it, or right after the panel solve
+        const bool late = h->chol_syrk_late != 0;
+        if (overlap && rest > 0 && !late) GLA_HIP_TRY(hipEventRecord(h->ev_step[0], s));
         if (overlap && step > 0) GLA_HIP_TRY(hipStreamWaitEvent(s, h->ev_step[1], 0));
         GLA_TRY(gemm<T>(s, 'N', 'T', 1, below, kb2, kb, T(-1), A21, lda, A21, lda, T(1),
                         A21 + kb * lda, lda, 0));
+        if (overlap && rest > 0 && late) GLA_HIP_TRY(hipEventRecord(h->ev_step[0], s));
         if (rest > 0) {
             hipStream_t us = overlap ? h->side : s;
             if (overlap) GLA_HIP_TRY(hipStreamWaitEvent(h->side, h->ev_step[0], 0));
@@ -770,6 +794,15 @@ int chol_factor_block(hipStream_t s, int nb, T *A, int64_t lda, int *info, int64
                       T *LinvG)
 {
     return launch_factor<T>(s, nb, A, lda, info, info_base, LinvG);
+}
+template <typename T>
+int chol_factor_block_gated(hipStream_t s, int nb, T *A, int64_t lda, int *info,
+                            int64_t info_base, T *LinvG, const int *run_if)
+{
+    hipLaunchKernelGGL((chol_factor_gated_kernel<T>), dim3(1), dim3(kPanelThreads), 0, s, nb, A,
+                       lda, info, info_base, LinvG, run_if);
+    GLA_LAUNCH_CHECK();
+    return GLA_OK;
 }
 int chol_factor_block_f32io(hipStream_t s, int nb, float *A, int64_t lda, int *info,
                             int64_t info_base, float *Linv32)
@@ -854,6 +887,8 @@ int diag_block_inverses(hipStream_t s, int64_t n, const T *L, int64_t ldl, const
 
 #define GLA_INST_BLOCKS(T)                                                                    \
     template int chol_factor_block<T>(hipStream_t, int, T *, int64_t, int *, int64_t, T *);  \
+    template int chol_factor_block_gated<T>(hipStream_t, int, T *, int64_t, int *, int64_t,   \
+                                            T *, const int *);                               \
     template int chol_solve_block<T>(hipStream_t, int, const T *, int64_t, const T *, T *,   \
                                      int64_t, int64_t);                                      \
     template int tile_inverses<T>(hipStream_t, int64_t, const T *, int64_t, T *);            \

@@ -79,6 +79,9 @@ struct Handle {
     // Tuning: block width of the look-ahead driver behind cholRecursive!
     // (0: the recursive-halving driver).  Set with gla_set_tuning.
     int64_t chol_block = 512;
+    // Release the look-ahead Cholesky's side-stream SYRK after the
+    // next-block-column update (1) rather than right after the panel solve.
+    int chol_syrk_late = 1;
     // Block width of the mixed-precision factorisation's look-ahead driver.
     int mixed_block = 512;
     // QR panel kernel: 1 LDS-reduction form, 2 wave-butterfly form.
@@ -133,6 +136,10 @@ int chol_device(Handle *h, hipStream_t s, int64_t n, T *A, int64_t lda, int algo
 template <typename T>
 int chol_factor_block(hipStream_t s, int nb, T *A, int64_t lda, int *info, int64_t info_base,
                       T *LinvG);
+// The same, launched regardless but factoring only if *run_if != 0.
+template <typename T>
+int chol_factor_block_gated(hipStream_t s, int nb, T *A, int64_t lda, int *info,
+                            int64_t info_base, T *LinvG, const int *run_if);
 // The diagonal-block factorisation of an FP32 block (nb ≤ 128) computed in
 // FP64: read and written in FP32, FP32 tile inverses in Linv32.
 int chol_factor_block_f32io(hipStream_t s, int nb, float *A, int64_t lda, int *info,

@@ -415,9 +415,13 @@ int chol_mixed_la(Handle *h, hipStream_t s, int64_t n, float *W, int64_t ldw, fl
         const int64_t rest = below - kb2;
         const int64_t Kp = lp_kpad(kb);
         GLA_TRY(lp_pack<TIN>(s, below, kb, W21, ldw, pk));
-        if (overlap && rest > 0) GLA_HIP_TRY(hipEventRecord(h->ev_step[0], s));
+        // as in the FP64 look-ahead driver, the side stream's trailing update
+        // starts after the next block column's (chol_syrk_late) or with it
+        const bool late = h->chol_syrk_late != 0;
+        if (overlap && rest > 0 && !late) GLA_HIP_TRY(hipEventRecord(h->ev_step[0], s));
         if (overlap && step > 0) GLA_HIP_TRY(hipStreamWaitEvent(s, h->ev_step[1], 0));
         GLA_TRY(lp_gemm_pk<TIN>(s, 1, below, kb2, kb, -1.f, pk, pk, W21 + kb * ldw, ldw));
+        if (overlap && rest > 0 && late) GLA_HIP_TRY(hipEventRecord(h->ev_step[0], s));
         if (rest > 0) {
             hipStream_t us = overlap ? h->side : s;
             if (overlap) GLA_HIP_TRY(hipStreamWaitEvent(h->side, h->ev_step[0], 0));

@@ -107,6 +107,14 @@ int gla_set_tuning(gla_handle_t hh, const char *key, int64_t value)
         h->qr_hr = value;
         return GLA_OK;
     }
+    if (std::strcmp(key, "chol_syrk_late") == 0) {
+        if (value != 0 && value != 1) {
+            gla::set_error("chol_syrk_late must be 0 or 1");
+            return -3;
+        }
+        h->chol_syrk_late = (int)value;
+        return GLA_OK;
+    }
     if (std::strcmp(key, "qr_lookahead") == 0) {
         if (value != 0 && value != 1) {
             gla::set_error("qr_lookahead must be 0 or 1");

@@ -50,7 +50,9 @@ const char *gla_version(void);
  * multiple of 128 (default 512) runs cholRecursive! as a right-looking
  * blocked factorisation with blocks of that width and depth-1 look-ahead;
  * 0 runs it as recursive halving down to 2048-wide leaves.  Both compute
- * the same factor.  "mixed_block": block width (multiple of 128,
+ * the same factor.  "chol_syrk_late": 1 (default) starts the look-ahead
+ * driver's trailing SYRK on the side stream after the next block column's
+ * update, 0 right after the panel solve (same factor).  "mixed_block": block width (multiple of 128,
  * default 512) of the look-ahead driver of the BF16/FP16 factorisation in
  * gla_chol_solve_mixed_d.  "qr_panel": 1 or 2 (default), the form of the
  * Householder panel kernel.  "qr_inner": 32 or 64 factors each QR block

@@ -122,6 +122,36 @@ def test_reconstructed_panel_vs_oracle(rng, dtype, la, m, n, nb):
     assert rel(QtA[:k, :], np.triu(ofac[:k, :])) <= tol
 
 
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+def test_reconstructed_panel_full_second_cholesky(rng, dtype):
+    """A first block column with κ = 3e4 (FP64; 1e2 in FP32): the first
+    CholeskyQR pass leaves ‖Q₁ᵀQ₁ − I‖ above 0.1·√eps (checked here on the
+    host), so the second pass takes the full 128×128 Cholesky rather than
+    the first-order factor of I + E.  Same V, τ, R as qrBlocked!."""
+    gla = _gla()
+    m, n, nb = 3000, 160, 64
+    kappa = 3e4 if dtype == np.float64 else 1e2
+    A = rng.standard_normal((m, n))
+    U, _ = np.linalg.qr(rng.standard_normal((m, nb)))
+    W, _ = np.linalg.qr(rng.standard_normal((nb, nb)))
+    A[:, :nb] = (U * np.logspace(0, -np.log10(kappa), nb)) @ W.T
+    A = A.astype(dtype)
+    P = A[:, :nb]
+    L = np.linalg.cholesky((P.T @ P).astype(np.float64)).astype(dtype)
+    Q1 = (P @ np.linalg.inv(L.T.astype(np.float64)).astype(dtype)).astype(np.float64)
+    E = np.linalg.norm(Q1.T @ Q1 - np.eye(nb))
+    assert E > 0.2 * np.sqrt(eps_of(dtype)), E
+    tol = 10 * max(m, n) * eps_of(dtype)
+    ofac, otau = O.qr_blocked(A.astype(np.float64), nb)
+    try:
+        gla.set_tuning("qr_hr", 1)
+        F = gla.qrBlocked_(gla.colmajor(A, _tdt(dtype)), nb)
+    finally:
+        gla.set_tuning("qr_hr", -1)
+    assert rel(gla.to_numpy(F.factors), ofac) <= tol
+    assert rel(F.tau.cpu().numpy(), otau) <= tol
+
+
 @pytest.mark.parametrize("case", ["zero_column", "rank_one", "graded"])
 def test_reconstructed_panel_falls_back(rng, case):
     """Block columns the Cholesky-based path cannot factor to working
