@@ -823,25 +823,14 @@ __global__ __launch_bounds__(1024) void qr_t_merge_kernel(int k, int ib, T *Tw, 
 // the column-by-column panel kernel, launched behind the path and gated on
 // the flag, factors it instead.
 
-// Q (m×k) = P, and the path's flags cleared (flag[0]: the path failed;
-// flag[1]: the second Gram matrix is not within 0.1·√eps of I).
-template <typename T>
-__global__ void hr_copy_kernel(int64_t m, const T *A, int64_t lda, T *Q, int64_t ldq, int *flag)
-{
-    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
-    const int j = blockIdx.y;
-    if (i == 0 && j == 0) {
-        flag[0] = 0;
-        flag[1] = 0;
-    }
-    if (i < m) Q[i + (int64_t)j * ldq] = A[i + (int64_t)j * lda];
-}
-
 // Gram matrix G = XᵀX of an m×k panel (k ≤ 128), lower 16×16 tiles only:
 // workgroup p sums rows of 32-row chunks p, p + P, … on MFMA (the chunk
 // staged in LDS, wave w takes tiles w, w + 4, …) and writes its partial
 // tiles to part[p][tile][256] (element (i, j) of a tile at 16 i + j);
-// hr_gram_reduce_kernel adds the P partials in a fixed order.
+// hr_gram_reduce_kernel adds the P partials in a fixed order.  With
+// copy_to (the first pass) the chunks are also copied there (Q ← P) and
+// the path's flags are cleared (flag[0]: the path failed; flag[1]: the
+// second Gram matrix is not within 0.1·√eps of I).
 constexpr int kGrRB = 32;
 constexpr int kGrLD = 132;
 // most workgroups of the Gram pass (one 32-row chunk each up to 8192 rows;
@@ -857,7 +846,8 @@ __host__ __device__ inline int hr_ntiles(int k)
 
 template <typename T>
 __global__ __launch_bounds__(256) void hr_gram_kernel(int64_t m, int k, const T *X, int64_t ldx,
-                                                      T *part)
+                                                      T *part, T *copy_to, int64_t ldc,
+                                                      int *flag)
 {
     typedef PanelMfma<T> Mf;
     typedef typename Mf::acc_t acc_t;
@@ -866,6 +856,10 @@ __global__ __launch_bounds__(256) void hr_gram_kernel(int64_t m, int k, const T 
     const int w = __builtin_amdgcn_readfirstlane(t >> 6);
     const int nt = (k + 15) / 16, ntl = nt * (nt + 1) / 2;
     const int li = lane & 15, lk = lane >> 4;
+    if (copy_to && blockIdx.x == 0 && t == 0) {
+        flag[0] = 0;
+        flag[1] = 0;
+    }
     acc_t acc[9];
     int tbi[9], tbj[9];
 #pragma unroll
@@ -881,7 +875,10 @@ __global__ __launch_bounds__(256) void hr_gram_kernel(int64_t m, int k, const T 
 #pragma unroll
         for (int e = t; e < kGrRB * 128; e += 256) {
             const int r = e & (kGrRB - 1), c = e / kGrRB;
-            Xs[r * kGrLD + c] = (r0 + r < m && c < k) ? X[r0 + r + (int64_t)c * ldx] : T(0);
+            const bool in = r0 + r < m && c < k;
+            const T v = in ? X[r0 + r + (int64_t)c * ldx] : T(0);
+            Xs[r * kGrLD + c] = v;
+            if (copy_to && in) copy_to[r0 + r + (int64_t)c * ldc] = v;
         }
         __syncthreads();
 #pragma unroll
@@ -1047,12 +1044,13 @@ __device__ __forceinline__ T recip1(T u)
 // Then T = Û V₁⁻ᵀ (Û = U S) in place over U, by column blocks:
 //   T[:, J] = Acc[:, J] W_Jᵀ, Acc[:, right] −= T[:, J] V₁[right, J]ᵀ
 // (T upper and V₁ strictly lower share M).  Writes V₁ (explicit: unit
-// diagonal, zero above) over Q's top block, Un = −Uᵀ (lower, ld k) for the
-// solve of the rows below, T (zero below the diagonal), S (Sv) and τ.
+// diagonal, zero above) over Q's top block, Un = −Uᵀ (lower, ld k) and the
+// inverses of its 16×16 diagonal tiles (UnInv) for the solve of the rows
+// below, T (zero below the diagonal), S (Sv) and τ.
 template <typename T>
 __global__ __launch_bounds__(kHrThreads) void hr_lu_kernel(int k, T *Q, int64_t ldq,
-                                                           const int *flag, T *Un, T *Tw,
-                                                           int64_t ldt, T *Sv, T *tau)
+                                                           const int *flag, T *Un, T *UnInv,
+                                                           T *Tw, int64_t ldt, T *Sv, T *tau)
 {
     typedef PanelMfma<T> Mf;
     typedef typename Mf::acc_t acc_t;
@@ -1146,14 +1144,14 @@ __global__ __launch_bounds__(kHrThreads) void hr_lu_kernel(int k, T *Q, int64_t 
                 for (int rr = 0; rr < 16; ++rr) Wb[rr + j * kHrTL] = x[rr];
             }
         } else if (w == 2) {
-            if (lane < 16 && c0 < k) {
+            if (lane < 16) {
                 const int j = lane;
                 T x[16];
                 {
                     T rd[16];
 #pragma unroll
                     for (int rr = 0; rr < 16; ++rr) {
-                        rd[rr] = recip1(M[jb + rr + (jb + rr) * kHrLD]);
+                        rd[rr] = rr < nbk ? recip1(M[jb + rr + (jb + rr) * kHrLD]) : T(1);
                         x[rr] = rr == j ? T(1) : T(0);
                     }
 #pragma unroll
@@ -1164,6 +1162,13 @@ __global__ __launch_bounds__(kHrThreads) void hr_lu_kernel(int k, T *Q, int64_t 
                     }
 #pragma unroll
                     for (int rr = 0; rr < 16; ++rr) Zs[rr + j * kHrTL] = x[rr];
+                    // the 16×16 diagonal-tile inverse of Un = −Uᵀ, row-major
+                    // as chol_solve_kernel reads it: (−U_JJᵀ)⁻¹ = −Zᵀ (the
+                    // padding of a partial last block is the identity)
+#pragma unroll
+                    for (int rr = 0; rr < 16; ++rr)
+                        UnInv[(jb >> 4) * 256 + j * 16 + rr] =
+                            (j >= nbk || rr >= nbk) ? (j == rr ? T(1) : T(0)) : -x[rr];
                 }
             }
         }
@@ -1454,17 +1459,18 @@ inline bool use_hr(const Handle *h, int64_t mm, int w)
     return cdiv(mm, panel_rows(h, w)) <= h->num_cus;
 }
 
-// The reconstructed-Householder panel (see hr_copy_kernel … hr_finish_kernel):
+// The reconstructed-Householder panel (see hr_gram_kernel … hr_finish_kernel):
 // factors Ab (mm×w, mm ≥ w) in place into R and V, writes τ, the explicit
 // V (Vw, ld mm) and T (Tw, ld w).  Stream-ordered launches on s, no host
 // synchronisation; the column-by-column panel kernel runs last, gated on
 // the device flag, when the path's checks fail.
 template <typename T>
 int hr_gram(hipStream_t s, int64_t m, int k, const T *X, int64_t ldx, T *G, T *part, int *flag,
-            int check)
+            int check, T *copy_to = nullptr, int64_t ldc = 0)
 {
     const int P = hr_gram_blocks(m);
-    hipLaunchKernelGGL(hr_gram_kernel<T>, dim3((unsigned)P), dim3(256), 0, s, m, k, X, ldx, part);
+    hipLaunchKernelGGL(hr_gram_kernel<T>, dim3((unsigned)P), dim3(256), 0, s, m, k, X, ldx, part,
+                       copy_to, ldc, flag);
     GLA_LAUNCH_CHECK();
     hipLaunchKernelGGL(hr_gram_reduce_kernel<T>, dim3((unsigned)hr_ntiles(k)), dim3(1024), 0, s, k,
                        P, part, G, flag, check);
@@ -1483,10 +1489,9 @@ int qr_panel_hr(Handle *h, hipStream_t s, int64_t mm, int w, T *Ab, int64_t lda,
     T *Sv = Linv + tile_inverse_elems(kQrNB) + 64;
     T *gpart = Sv + 128;
     const dim3 cgrid((unsigned)cdiv(mm, 256), (unsigned)k);
-    hipLaunchKernelGGL(hr_copy_kernel<T>, cgrid, dim3(256), 0, s, mm, Ab, lda, Vw, mm, flag);
-    GLA_LAUNCH_CHECK();
-    // (1) two CholeskyQR passes: Vw <- P R₁⁻¹ R₂⁻¹
-    GLA_TRY(hr_gram<T>(s, mm, k, Ab, lda, G1, gpart, flag, 0));
+    // (1) two CholeskyQR passes: Vw <- P R₁⁻¹ R₂⁻¹ (the first Gram pass also
+    // copies P to Vw and clears the flags)
+    GLA_TRY(hr_gram<T>(s, mm, k, Ab, lda, G1, gpart, flag, 0, Vw, mm));
     GLA_TRY(chol_factor_block<T>(s, k, G1, k, flag, 0, Linv));
     GLA_TRY(trsm_rlt<T>(s, mm, k, G1, k, Linv, Vw, mm));
     GLA_TRY(hr_gram<T>(s, mm, k, Vw, mm, G2, gpart, flag, 1));
@@ -1501,10 +1506,9 @@ int qr_panel_hr(Handle *h, hipStream_t s, int64_t mm, int w, T *Ab, int64_t lda,
     hipLaunchKernelGGL(hr_r_kernel<T>, dim3(nt, nt), dim3(256), 0, s, k, G1, G2, Rb);
     GLA_LAUNCH_CHECK();
     // (2)-(3) the signed LU of the top block, T, τ; V₂ = Q₂ (−U)⁻¹
-    hipLaunchKernelGGL(hr_lu_kernel<T>, dim3(1), dim3(kHrThreads), 0, s, k, Vw, mm, flag, Un, Tw,
-                       (int64_t)k, Sv, tau);
+    hipLaunchKernelGGL(hr_lu_kernel<T>, dim3(1), dim3(kHrThreads), 0, s, k, Vw, mm, flag, Un,
+                       Linv, Tw, (int64_t)k, Sv, tau);
     GLA_LAUNCH_CHECK();
-    GLA_TRY(tile_inverses<T>(s, k, Un, k, Linv));
     GLA_TRY(trsm_rlt<T>(s, mm - k, k, Un, k, Linv, Vw + k, mm));
     hipLaunchKernelGGL(hr_finish_kernel<T>, cgrid, dim3(256), 0, s, mm, k, Vw, mm, Rb, Sv, Ab,
                        lda, flag);
