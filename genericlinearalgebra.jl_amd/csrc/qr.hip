@@ -1136,10 +1136,17 @@ __global__ __launch_bounds__(kHrThreads) void hr_lu_kernel(int k, T *Q, int64_t 
                 T x[16];
 #pragma unroll
                 for (int rr = 0; rr < 16; ++rr) x[rr] = rr == j ? T(1) : T(0);
+                // column s of L staged in registers, then the independent
+                // updates of step s (the fence keeps the steps as written)
 #pragma unroll
-                for (int s = 0; s < 15; ++s)
+                for (int s = 0; s < 15; ++s) {
+                    T cs[16];
 #pragma unroll
-                    for (int rr = s + 1; rr < 16; ++rr) x[rr] -= M[jb + rr + (jb + s) * kHrLD] * x[s];
+                    for (int rr = s + 1; rr < 16; ++rr) cs[rr] = M[jb + rr + (jb + s) * kHrLD];
+#pragma unroll
+                    for (int rr = s + 1; rr < 16; ++rr) x[rr] -= cs[rr] * x[s];
+                    __builtin_amdgcn_sched_barrier(0);
+                }
 #pragma unroll
                 for (int rr = 0; rr < 16; ++rr) Wb[rr + j * kHrTL] = x[rr];
             }
@@ -1156,9 +1163,13 @@ __global__ __launch_bounds__(kHrThreads) void hr_lu_kernel(int k, T *Q, int64_t 
                     }
 #pragma unroll
                     for (int s = 15; s >= 0; --s) {
+                        T cs[16];
+#pragma unroll
+                        for (int rr = 0; rr < s; ++rr) cs[rr] = M[jb + rr + (jb + s) * kHrLD];
                         x[s] *= rd[s];
 #pragma unroll
-                        for (int rr = 0; rr < s; ++rr) x[rr] -= M[jb + rr + (jb + s) * kHrLD] * x[s];
+                        for (int rr = 0; rr < s; ++rr) x[rr] -= cs[rr] * x[s];
+                        __builtin_amdgcn_sched_barrier(0);
                     }
 #pragma unroll
                     for (int rr = 0; rr < 16; ++rr) Zs[rr + j * kHrTL] = x[rr];
