@@ -466,7 +466,7 @@ def main():
                            "frac": achieved / 8000.0,
                            "traffic": profiled_traffic("qr_hr_panel_8192x128.md"),
                            "kernel": "reconstructed-Householder panel of qrBlocked! (block "
-                                     "column 8192x128; its 16-launch sequence timed as one)",
+                                     "column 8192x128; its 13-launch sequence timed as one)",
                            "bytes_per_launch": panel_bytes, "us_per_launch": t_panel * 1e6,
                            "column_kernel_us": t_colpanel * 1e6},
     }

@@ -339,17 +339,38 @@ __global__ __launch_bounds__(kPanelThreads) void chol_factor_kernel(int nb, T *_
     }
 }
 
-// The same, run only if *run_if != 0 (a device-side choice between this
-// factorisation and a cheaper one made by an earlier kernel).
+// The same for a block that an earlier kernel has classified through
+// flags: flags[0] != 0 — nothing to do (the caller's path failed);
+// flags[1] == 0 — A = I + E with ‖E‖_F ≤ 0.1·√eps, whose factor is
+// L = I + Φ(E), Φ(E) = the strictly lower part of E plus half its diagonal,
+// since (I + F)(I + F)ᵀ = I + E gives F = Φ(E − F Fᵀ) and ‖F Fᵀ‖ ≤ ‖E‖² ≤
+// 0.01·eps, below the rounding of any entry; likewise the 16×16 tile
+// inverses (I + F_tt)⁻¹ = I − F_tt; otherwise the full factorisation.
 template <typename T>
-__global__ __launch_bounds__(kPanelThreads) void chol_factor_gated_kernel(
+__global__ __launch_bounds__(kPanelThreads) void chol_factor_near_identity_kernel(
     int nb, T *__restrict__ A, int64_t lda, int *info, int64_t info_base, T *__restrict__ LinvG,
-    const int *run_if)
+    const int *flags)
 {
     __shared__ T Xb[2 * kPanelNB * kXLD];
     __shared__ T Db[2 * 16 * kXLD];
     __shared__ T Lib[2 * 16 * kXLD];
-    if (*run_if == 0) return;
+    if (flags[0] != 0) return;
+    if (flags[1] == 0) {
+        const int nt = (nb + 15) / 16;
+        for (int e = threadIdx.x; e < nt * 256; e += kPanelThreads) {
+            const int t = e >> 8, i = (e >> 4) & 15, j = e & 15;
+            const int gi = 16 * t + i, gj = 16 * t + j;
+            T v;
+            if (gi >= nb || gj >= nb) v = gi == gj ? T(1) : T(0);
+            else if (gi == gj) v = T(1) - (A[gi + (int64_t)gi * lda] - T(1)) * T(0.5);
+            else v = gi > gj ? -A[gi + (int64_t)gj * lda] : T(0);
+            LinvG[e] = v;
+        }
+        __syncthreads();
+        for (int e = threadIdx.x; e < nb; e += kPanelThreads)
+            A[e + (int64_t)e * lda] = T(1) + (A[e + (int64_t)e * lda] - T(1)) * T(0.5);
+        return;
+    }
     switch (threadIdx.x >> 6) {
     case 0: factor_body<T, 0>(nb, A, lda, info, info_base, LinvG, Xb, Db, Lib); break;
     case 1: factor_body<T, 1>(nb, A, lda, info, info_base, LinvG, Xb, Db, Lib); break;
@@ -796,11 +817,11 @@ int chol_factor_block(hipStream_t s, int nb, T *A, int64_t lda, int *info, int64
     return launch_factor<T>(s, nb, A, lda, info, info_base, LinvG);
 }
 template <typename T>
-int chol_factor_block_gated(hipStream_t s, int nb, T *A, int64_t lda, int *info,
-                            int64_t info_base, T *LinvG, const int *run_if)
+int chol_factor_block_near_identity(hipStream_t s, int nb, T *A, int64_t lda, int *info,
+                                    int64_t info_base, T *LinvG, const int *flags)
 {
-    hipLaunchKernelGGL((chol_factor_gated_kernel<T>), dim3(1), dim3(kPanelThreads), 0, s, nb, A,
-                       lda, info, info_base, LinvG, run_if);
+    hipLaunchKernelGGL((chol_factor_near_identity_kernel<T>), dim3(1), dim3(kPanelThreads), 0, s,
+                       nb, A, lda, info, info_base, LinvG, flags);
     GLA_LAUNCH_CHECK();
     return GLA_OK;
 }
@@ -887,8 +908,8 @@ int diag_block_inverses(hipStream_t s, int64_t n, const T *L, int64_t ldl, const
 
 #define GLA_INST_BLOCKS(T)                                                                    \
     template int chol_factor_block<T>(hipStream_t, int, T *, int64_t, int *, int64_t, T *);  \
-    template int chol_factor_block_gated<T>(hipStream_t, int, T *, int64_t, int *, int64_t,   \
-                                            T *, const int *);                               \
+    template int chol_factor_block_near_identity<T>(hipStream_t, int, T *, int64_t, int *,   \
+                                                    int64_t, T *, const int *);              \
     template int chol_solve_block<T>(hipStream_t, int, const T *, int64_t, const T *, T *,   \
                                      int64_t, int64_t);                                      \
     template int tile_inverses<T>(hipStream_t, int64_t, const T *, int64_t, T *);            \

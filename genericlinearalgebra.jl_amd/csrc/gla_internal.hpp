@@ -136,10 +136,13 @@ int chol_device(Handle *h, hipStream_t s, int64_t n, T *A, int64_t lda, int algo
 template <typename T>
 int chol_factor_block(hipStream_t s, int nb, T *A, int64_t lda, int *info, int64_t info_base,
                       T *LinvG);
-// The same, launched regardless but factoring only if *run_if != 0.
+// The same for a block classified on the device: flags[0] != 0 — skip;
+// flags[1] == 0 — A is within 0.1·√eps of I (Frobenius), factored to
+// first order (L = I + strictly-lower(A) + (diag(A) − 1)/2); otherwise the
+// full factorisation.
 template <typename T>
-int chol_factor_block_gated(hipStream_t s, int nb, T *A, int64_t lda, int *info,
-                            int64_t info_base, T *LinvG, const int *run_if);
+int chol_factor_block_near_identity(hipStream_t s, int nb, T *A, int64_t lda, int *info,
+                                    int64_t info_base, T *LinvG, const int *flags);
 // The diagonal-block factorisation of an FP32 block (nb ≤ 128) computed in
 // FP64: read and written in FP32, FP32 tile inverses in Linv32.
 int chol_factor_block_f32io(hipStream_t s, int nb, float *A, int64_t lda, int *info,

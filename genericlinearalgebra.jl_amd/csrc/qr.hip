@@ -909,7 +909,8 @@ __global__ __launch_bounds__(256) void hr_gram_kernel(int64_t m, int k, const T 
 // tile's share of ‖G − I‖_F² exceeds 0.25 / ntiles, so an unflagged G has
 // ‖G − I‖_F ≤ 0.5; a non-finite G is flagged too.  It also raises flag[1]
 // unless the share is within 0.01·eps / ntiles (‖G − I‖_F ≤ 0.1·√eps),
-// where hr_chol_near_identity_kernel replaces the Cholesky factorisation.
+// where the first-order factor of I + E replaces the full Cholesky
+// factorisation (chol_factor_block_near_identity).
 template <typename T>
 __global__ __launch_bounds__(1024) void hr_gram_reduce_kernel(int k, int P, const T *part, T *G,
                                                               int *flag, int check)
@@ -946,32 +947,6 @@ __global__ __launch_bounds__(1024) void hr_gram_reduce_kernel(int k, int P, cons
         const double eps = sizeof(T) == 8 ? 2.220446049250313e-16 : 1.1920928955078125e-7;
         if (!(tot <= 0.01 * eps / ntl)) atomicCAS(flag + 1, 0, 1);
     }
-}
-
-// Cholesky factor of G = I + E with ‖E‖_F ≤ 0.1·√eps (flag[1] clear):
-// L = I + Φ(E), Φ(E) = the strictly lower part of E plus half its
-// diagonal, since (I + F)(I + F)ᵀ = I + E gives F = Φ(E − F Fᵀ) and
-// ‖F Fᵀ‖ ≤ ‖E‖² ≤ 0.01·eps, below the rounding of any entry; likewise the
-// 16×16 diagonal-tile inverses (I + F_tt)⁻¹ = I − F_tt (row-major, as
-// chol_factor_kernel writes them).  G is lower-stored in place (ld k).
-template <typename T>
-__global__ __launch_bounds__(256) void hr_chol_near_identity_kernel(int k, T *G, T *Linv,
-                                                                    const int *flag)
-{
-    if (flag[0] != 0 || flag[1] != 0) return;
-    const int nt = (k + 15) / 16;
-    for (int e = threadIdx.x; e < nt * 256; e += 256) {
-        const int t = e >> 8, i = (e >> 4) & 15, j = e & 15;
-        const int gi = 16 * t + i, gj = 16 * t + j;
-        T v;
-        if (gi >= k || gj >= k) v = gi == gj ? T(1) : T(0);
-        else if (gi == gj) v = T(1) - (G[gi + (int64_t)gi * k] - T(1)) * T(0.5);
-        else v = gi > gj ? -G[gi + (int64_t)gj * k] : T(0);
-        Linv[e] = v;
-    }
-    __syncthreads();
-    for (int e = threadIdx.x; e < k; e += 256)
-        G[e + (int64_t)e * k] = T(1) + (G[e + (int64_t)e * k] - T(1)) * T(0.5);
 }
 
 // R = R₂ R₁ (k×k, ld k, zero below the diagonal) from the two Cholesky
@@ -1507,11 +1482,8 @@ int qr_panel_hr(Handle *h, hipStream_t s, int64_t mm, int w, T *Ab, int64_t lda,
     GLA_TRY(trsm_rlt<T>(s, mm, k, G1, k, Linv, Vw, mm));
     GLA_TRY(hr_gram<T>(s, mm, k, Vw, mm, G2, gpart, flag, 1));
     // G₂ = I + E: the first-order factor when E is below √eps (the usual
-    // case), the full factorisation otherwise (each launch gated on flag[1])
-    hipLaunchKernelGGL(hr_chol_near_identity_kernel<T>, dim3(1), dim3(256), 0, s, k, G2, Linv,
-                       flag);
-    GLA_LAUNCH_CHECK();
-    GLA_TRY(chol_factor_block_gated<T>(s, k, G2, k, flag, 0, Linv, flag + 1));
+    // case), the full factorisation otherwise (flag[1], from the reduction)
+    GLA_TRY(chol_factor_block_near_identity<T>(s, k, G2, k, flag, 0, Linv, flag));
     GLA_TRY(trsm_rlt<T>(s, mm, k, G2, k, Linv, Vw, mm));
     const unsigned nt = (unsigned)cdiv(k, 16);
     hipLaunchKernelGGL(hr_r_kernel<T>, dim3(nt, nt), dim3(256), 0, s, k, G1, G2, Rb);
