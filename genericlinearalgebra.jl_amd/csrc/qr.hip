@@ -19,6 +19,13 @@
 // The kernel also writes V with its unit diagonal and zero upper part
 // explicit (Vw), which the WY update reads as a plain matrix.
 //
+// Tall block columns (qr_panel_hr) are factored instead by two CholeskyQR
+// passes and a reconstruction of the same reflectors from the orthonormal
+// factor — the signed LU of its top block — in a fixed number of launches
+// (hr_gram_kernel … hr_finish_kernel); the column-by-column kernel stays
+// behind it as a device-gated fallback for block columns too ill-conditioned
+// for CholeskyQR.
+//
 // Trailing update (src/householder.jl:119-157):  W = Vwᵀ A2 (MFMA GEMM,
 // split-K), W = Tᵀ W, A2 -= Vw W — the same three contractions the
 // reference makes, with V1 folded into Vw.
