@@ -190,6 +190,28 @@ def test_reconstructed_panel_falls_back(rng, case):
     assert rel(tau, otau) <= tol
 
 
+@pytest.mark.parametrize("bad", [np.nan, np.inf])
+def test_reconstructed_panel_non_finite_input(rng, bad):
+    """A NaN or Inf in a tall block column makes the Gram matrices
+    non-finite: the device check flags the reconstructed path and the
+    column-by-column kernel factors the block column, propagating the value
+    as reflector! does (no error, no hang); the untouched block columns'
+    reflectors before it match the oracle."""
+    gla = _gla()
+    m, n, nb = 3000, 256, 128
+    A = rng.standard_normal((m, n))
+    A[1234, 200] = bad                      # second block column
+    ofac, otau = O.qr_blocked(A, nb)
+    F = gla.qrBlocked_(gla.colmajor(A), nb)
+    fac, tau = gla.to_numpy(F.factors), F.tau.cpu().numpy()
+    assert not np.all(np.isfinite(fac[:, 128:]))
+    tol = 10 * m * eps_of(np.float64)
+    # the first block column's reflectors and τ are computed before the bad
+    # value can reach them
+    assert rel(fac[128:, :128], ofac[128:, :128]) <= tol
+    assert rel(tau[:128], otau[:128]) <= tol
+
+
 def test_known_reflector():
     """[3;4]: β = −5, v = [1, 0.5], τ = 1.6 (Julia's reflector! convention)."""
     gla = _gla()
