@@ -95,6 +95,12 @@ struct Handle {
     // reconstructed-Householder panel (0: never; -1: any height for block
     // columns at least 64 wide, 2048 rows for narrower ones).
     int64_t qr_hr = -1;
+    // Polls before a grid-wide wait of the QR panel gives up (the call then
+    // returns GLA_ERR_TIMEOUT).
+    unsigned spin_limit = 1u << 25;
+    // Test setting: every grid-wide wait of the QR panel targets an arrival
+    // count that never comes, so each gives up after spin_limit polls.
+    int qr_wait_test = 0;
 };
 
 // Ensure the workspace holds at least `bytes` (grows with hipMalloc, which

@@ -45,7 +45,6 @@ constexpr int kQrThreads = 1024;
 constexpr int kQrNB = 128;   // max panel width
 constexpr int kQrRPT = 16;   // rows per thread (LDS-reduction panel kernel)
 constexpr int kQrRPL = 2;    // rows per lane (wave-butterfly panel kernel): 128 rows per workgroup
-constexpr unsigned kSpinLimit = 1u << 25;
 
 #define GLA_TRY(expr)                  \
     do {                               \
@@ -67,6 +66,10 @@ struct QrPanelP {
     T *part;
     unsigned *bar;
     unsigned bar_base;
+    // polls before a grid wait gives up (raising *err); added to every wait
+    // target when nonzero (a test setting: no wait can then be met)
+    unsigned spin_limit;
+    unsigned bar_skew;
     int *err;
     // null: always run; otherwise run only if *run_if != 0 (the fallback
     // behind the reconstructed-Householder panel)
@@ -92,8 +95,9 @@ __device__ __forceinline__ float dcopysign<float>(float a, float b) { return __b
 // the partials with agent-scope loads.  No agent-scope fence is used: on a
 // multi-XCD gfx950 one would write back and invalidate the whole L2 of the
 // XCD (buffer_wbl2 / buffer_inv), stalling the concurrent trailing update.
-// Waits are bounded: after kSpinLimit polls *err is raised and the workgroup
-// carries on, so no launch can hang.
+// Waits are bounded: after `limit` polls *err is raised and the workgroup
+// carries on with the partials it can see (every read stays inside the
+// exchange buffer), so no launch can hang; the host reports the flag.
 
 __device__ __forceinline__ void grid_arrive(unsigned *bar)
 {
@@ -101,13 +105,14 @@ __device__ __forceinline__ void grid_arrive(unsigned *bar)
     if (threadIdx.x == 0) __hip_atomic_fetch_add(bar, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-__device__ __forceinline__ void grid_wait(unsigned *bar, unsigned target, int *err)
+__device__ __forceinline__ void grid_wait(unsigned *bar, unsigned target, int *err,
+                                          unsigned limit)
 {
     if (threadIdx.x == 0) {
         unsigned spins = 0;
         while (__hip_atomic_load(bar, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
             __builtin_amdgcn_s_sleep(1);
-            if (++spins > kSpinLimit) {
+            if (++spins > limit) {
                 __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                 break;
             }
@@ -283,7 +288,8 @@ __global__ __launch_bounds__(NTH) void qr_panel2_kernel(QrPanelP<T> p)
             stores_done();
             grid_arrive(p.bar);
             if (wg == 0 && j > 0) t_column<T, NB, NTH>(Ts, gsh, tpart, j - 1, tau_prev);
-            grid_wait(p.bar, p.bar_base + (unsigned)(j + 1) * (unsigned)P, p.err);
+            grid_wait(p.bar, p.bar_base + (unsigned)(j + 1) * (unsigned)P + p.bar_skew, p.err,
+                      p.spin_limit);
             // (3) totals in a fixed order: H sub-sums per column over the
             // partials p ≡ h (mod H), eight loads in flight, then the H in
             // order; row j travels in the first batch's round trip
@@ -479,7 +485,8 @@ __global__ __launch_bounds__(kQrThreads) void qr_panel_kernel(QrPanelP<T> p)
             grid_arrive(p.bar);
             // workgroup 0 finishes T for the previous column while the grid gathers
             if (wg == 0 && j > 0) t_column<T, NB, kQrThreads>(Ts, gsh, tpart, j - 1, tau_prev);
-            grid_wait(p.bar, p.bar_base + (unsigned)(j + 1) * (unsigned)P, p.err);
+            grid_wait(p.bar, p.bar_base + (unsigned)(j + 1) * (unsigned)P + p.bar_skew, p.err,
+                      p.spin_limit);
             // (3) identical totals in every workgroup: fixed summation order.
             // GS threads per total each add every GS-th partial (independent
             // loads, one round trip each), then the GS sums are added in order.
@@ -1364,7 +1371,7 @@ size_t qr_ws_elems(int64_t m, int64_t n, int nb, int64_t *split_elems)
 
 template <typename T>
 int qr_panel(Handle *h, hipStream_t s, int64_t m, int w, T *A, int64_t lda, T *tau, T *Vw,
-             int64_t ldv, T *Tw, int64_t ldt, T *part, unsigned &bar_base, int *err,
+             int64_t ldv, T *Tw, int64_t ldt, T *part, unsigned &bar_base,
              const int *run_if = nullptr)
 {
     // 1: LDS-reduction kernel; 2: wave-butterfly kernel, sized to the panel
@@ -1377,7 +1384,9 @@ int qr_panel(Handle *h, hipStream_t s, int64_t m, int w, T *A, int64_t lda, T *t
     const int64_t P = std::max<int64_t>(1, cdiv(m, R));
     // every workgroup must be resident at once for the grid barrier
     if (P > h->num_cus) {
-        set_error("qr panel: %lld rows exceed the register-resident panel capacity", (long long)m);
+        set_error("qr panel: a block column of %lld rows exceeds the register-resident panel "
+                  "(at most %lld rows at width %d on %d CUs)",
+                  (long long)m, (long long)R * h->num_cus, wide, h->num_cus);
         return GLA_ERR_UNSUPPORTED;
     }
     QrPanelP<T> p;
@@ -1394,6 +1403,8 @@ int qr_panel(Handle *h, hipStream_t s, int64_t m, int w, T *A, int64_t lda, T *t
     p.part = part;
     p.bar = h->dbar;
     p.bar_base = bar_base;
+    p.spin_limit = h->spin_limit;
+    p.bar_skew = h->qr_wait_test ? (1u << 30) : 0u;
     p.err = h->dinfo + 1;
     p.run_if = run_if;
     if (panel == 1)
@@ -1410,7 +1421,45 @@ int qr_panel(Handle *h, hipStream_t s, int64_t m, int w, T *A, int64_t lda, T *t
                            dim3(kQrThreads), 0, s, p);
     GLA_LAUNCH_CHECK();
     if (P > 1) bar_base += (unsigned)(p.kmax * P);
-    (void)err;
+    return GLA_OK;
+}
+
+// Start of a factorisation: the grid-barrier counter and the panel's
+// wait-timeout flag to zero.
+__global__ void qr_reset_kernel(unsigned *bar, int *err)
+{
+    *bar = 0u;
+    *err = 0;
+}
+
+// qrBlocked!'s workspace after a step's lmul!(H', A2, M) (src/qr.jl:132,
+// src/householder.jl:142-153): M = −V₁ W2 with V₁ the unit-lower top k×k
+// block of the explicit V and W2 = Tᵀ VᵀA2 (k×n2, ld k); written to
+// work[0:k, c0:c0+n2].  One thread per element.
+template <typename T>
+__global__ void qr_work_kernel(int k, int64_t n2, const T *Vw, int64_t ldv, const T *W2, T *work,
+                               int64_t ldw)
+{
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t c = blockIdx.y;
+    if (r >= k || c >= n2) return;
+    const T *w = W2 + c * k;
+    T acc = w[r];
+    for (int i = 0; i < r; ++i) acc += Vw[r + (int64_t)i * ldv] * w[i];
+    work[r + c * ldw] = -acc;
+}
+
+template <typename T>
+int qr_work(hipStream_t s, int k, int64_t n2, const T *Vw, int64_t ldv, const T *W2, T *work,
+            int64_t ldw)
+{
+    if (!work || k <= 0 || n2 <= 0) return GLA_OK;
+    for (int64_t c0 = 0; c0 < n2; c0 += 65535) {
+        const int64_t nc = std::min<int64_t>(65535, n2 - c0);
+        hipLaunchKernelGGL(qr_work_kernel<T>, dim3((unsigned)cdiv(k, 128), (unsigned)nc),
+                           dim3(128), 0, s, k, nc, Vw, ldv, W2 + c0 * k, work + c0 * ldw, ldw);
+        GLA_LAUNCH_CHECK();
+    }
     return GLA_OK;
 }
 
@@ -1504,7 +1553,7 @@ int qr_panel_hr(Handle *h, hipStream_t s, int64_t mm, int w, T *Ab, int64_t lda,
                        lda, flag);
     GLA_LAUNCH_CHECK();
     // fallback: the column-by-column panel over the untouched panel
-    return qr_panel<T>(h, s, mm, w, Ab, lda, tau, Vw, mm, Tw, k, part, bar_base, nullptr, flag);
+    return qr_panel<T>(h, s, mm, w, Ab, lda, tau, Vw, mm, Tw, k, part, bar_base, flag);
 }
 
 // Factor one block column (mm×w at Ab) into V (explicit, Vw) and T (Tw):
@@ -1521,9 +1570,17 @@ int qr_block_column(Handle *h, hipStream_t s, int64_t mm, int w, T *Ab, int64_t 
     // auto (-1): sub-panels of 64 for block columns taller than 12288 rows,
     // where the full-width panel spreads over more workgroups than the
     // 64-wide one and its per-column exchange is slower
-    const int ib = h->qr_inner >= 0 ? h->qr_inner : (mm > 12288 ? 64 : 0);
+    int ib = h->qr_inner >= 0 ? h->qr_inner : (mm > 12288 ? 64 : 0);
+    // every workgroup of a panel launch must be resident at once (the grid
+    // exchange waits for all of them): block columns too tall for their
+    // width are factored as narrower sub-panels (64, then 32 wide)
+    if (h->qr_panel == 2) {
+        int cur = (ib > 0 && k > ib) ? ib : w;
+        while (cur > 32 && cdiv(mm, panel_rows(h, cur)) > h->num_cus) cur = cur > 64 ? 64 : 32;
+        if (cur < k) ib = cur;
+    }
     if (!(h->qr_panel == 2 && ib > 0 && k > ib))
-        return qr_panel<T>(h, s, mm, w, Ab, lda, tau, Vw, mm, Tw, k, part, bar_base, nullptr);
+        return qr_panel<T>(h, s, mm, w, Ab, lda, tau, Vw, mm, Tw, k, part, bar_base);
     hipLaunchKernelGGL(qr_sub_clear_kernel<T>, dim3(1, (unsigned)k), dim3(128), 0, s, (int)k, Vw,
                        mm, Tw, k);
     GLA_LAUNCH_CHECK();
@@ -1534,7 +1591,7 @@ int qr_block_column(Handle *h, hipStream_t s, int64_t mm, int w, T *Ab, int64_t 
         T *Vs = Vw + s0 + s0 * mm;
         T *Tss = Tw + s0 + s0 * k;
         GLA_TRY(qr_panel<T>(h, s, ms, iw, Ab + s0 + s0 * lda, lda, tau + s0, Vs, mm, Tss, k, part,
-                            bar_base, nullptr));
+                            bar_base));
         if (w - s0 - iw > 0)
             GLA_TRY(wy_update<T>(s, 'T', ms, w - s0 - iw, ks, Vs, mm, Tss, k,
                                  Ab + s0 + (s0 + iw) * lda, lda, W1, W2, split, split_elems));
@@ -1554,7 +1611,7 @@ int qr_block_column(Handle *h, hipStream_t s, int64_t mm, int w, T *Ab, int64_t 
 // between two (V, T) sets and the two streams use separate update buffers.
 template <typename T>
 int qr_blocked_dev(Handle *h, hipStream_t s, int64_t m, int64_t n, T *A, int64_t lda, int nb,
-                   T *tau)
+                   T *tau, T *work, int64_t ldw)
 {
     if (m <= 0 || n <= 0) return GLA_OK;
     int64_t split_elems = 0;
@@ -1580,7 +1637,7 @@ int qr_blocked_dev(Handle *h, hipStream_t s, int64_t m, int64_t n, T *A, int64_t
     const int64_t npanels = cdiv(std::min(m, n), (int64_t)nb);
     const bool la = h->qr_lookahead && npanels > 2;
 
-    hipLaunchKernelGGL(k_set_int, dim3(1), dim3(1), 0, s, reinterpret_cast<int *>(h->dbar), 0);
+    hipLaunchKernelGGL(qr_reset_kernel, dim3(1), dim3(1), 0, s, h->dbar, h->dinfo + 1);
     GLA_LAUNCH_CHECK();
     if (la) {
         GLA_HIP_TRY(hipEventRecord(h->ev_fork, s));
@@ -1600,6 +1657,7 @@ int qr_blocked_dev(Handle *h, hipStream_t s, int64_t m, int64_t n, T *A, int64_t
             if (!la) {
                 GLA_TRY(wy_update<T>(s, 'T', mm, nn - w, k, Vw[q], mm, Tw[q], k, Ab + w * lda, lda,
                                      W1[0], W2[0], split[0], sp));
+                GLA_TRY(qr_work<T>(s, (int)k, nn - w, Vw[q], mm, W2[0], work, ldw));
             } else {
                 // block column k+1 here; it was last touched by the side
                 // stream's update of the previous step
@@ -1608,11 +1666,17 @@ int qr_blocked_dev(Handle *h, hipStream_t s, int64_t m, int64_t n, T *A, int64_t
                 if (step > 0) GLA_HIP_TRY(hipStreamWaitEvent(s, h->ev_step[1], 0));
                 GLA_TRY(wy_update<T>(s, 'T', mm, w2, k, Vw[q], mm, Tw[q], k, Ab + w * lda, lda,
                                      W1[0], W2[0], split[0], sp));
+                GLA_TRY(qr_work<T>(s, (int)k, w2, Vw[q], mm, W2[0], work, ldw));
                 if (rest > 0) {
                     GLA_HIP_TRY(hipEventRecord(h->ev_step[0], s));
                     GLA_HIP_TRY(hipStreamWaitEvent(h->side, h->ev_step[0], 0));
                     GLA_TRY(wy_update<T>(h->side, 'T', mm, rest, k, Vw[q], mm, Tw[q], k,
                                          Ab + (w + w2) * lda, lda, W1[1], W2[1], split[1], sp));
+                    // the same stream orders this step's columns after the
+                    // previous step's; the caller's stream writes columns
+                    // [0, w2) only, disjoint from these
+                    GLA_TRY(qr_work<T>(h->side, (int)k, rest, Vw[q], mm, W2[1],
+                                       work ? work + w2 * ldw : nullptr, ldw));
                     GLA_HIP_TRY(hipEventRecord(h->ev_step[1], h->side));
                 }
             }
@@ -1698,7 +1762,7 @@ namespace {
 
 template <typename T>
 int qr_entry(gla_handle_t hh, int64_t m, int64_t n, T *A, int64_t lda, int64_t bs, T *tau,
-             void *stream, bool unblocked)
+             T *work, int64_t ldwork, void *stream, bool unblocked)
 {
     Handle *h = reinterpret_cast<Handle *>(hh);
     if (!h) { set_error("null handle"); return -1; }
@@ -1707,10 +1771,26 @@ int qr_entry(gla_handle_t hh, int64_t m, int64_t n, T *A, int64_t lda, int64_t b
     if (lda < std::max<int64_t>(1, m)) { set_error("lda < max(1,m)"); return -5; }
     if (!unblocked && bs < 1) { set_error("blocksize < 1"); return -6; }
     if (!tau && std::min(m, n) > 0) { set_error("tau is null"); return -7; }
+    const int nb = (int)std::min<int64_t>(unblocked ? kQrNB : bs, kQrNB);
+    if (work && ldwork < std::max<int64_t>(1, std::min<int64_t>(m, nb))) {
+        set_error("ldwork < min(m, blocksize)");
+        return -9;
+    }
     hipStream_t s = reinterpret_cast<hipStream_t>(stream);
     GLA_HIP_TRY(hipSetDevice(h->device));
-    const int nb = (int)std::min<int64_t>(unblocked ? kQrNB : bs, kQrNB);
-    return qr_blocked_dev<T>(h, s, m, n, A, lda, nb, tau);
+    if (m == 0 || n == 0) return GLA_OK;
+    GLA_TRY(qr_blocked_dev<T>(h, s, m, n, A, lda, nb, tau, work, ldwork));
+    // a panel whose grid wait gave up has produced meaningless factors
+    int timed_out = 0;
+    GLA_HIP_TRY(hipMemcpyAsync(&timed_out, h->dinfo + 1, sizeof(int), hipMemcpyDeviceToHost, s));
+    GLA_HIP_TRY(hipStreamSynchronize(s));
+    if (timed_out) {
+        set_error("qr panel: a grid-wide wait gave up after %u polls; A, tau and work do not "
+                  "hold a valid factorisation",
+                  h->spin_limit);
+        return GLA_ERR_TIMEOUT;
+    }
+    return GLA_OK;
 }
 
 template <typename T>
@@ -1789,26 +1869,22 @@ extern "C" {
 int gla_qr_unblocked_d(gla_handle_t h, int64_t m, int64_t n, double *A, int64_t lda, double *tau,
                        void *stream)
 {
-    return gla::qr_entry<double>(h, m, n, A, lda, 0, tau, stream, true);
+    return gla::qr_entry<double>(h, m, n, A, lda, 0, tau, nullptr, 0, stream, true);
 }
 int gla_qr_unblocked_s(gla_handle_t h, int64_t m, int64_t n, float *A, int64_t lda, float *tau,
                        void *stream)
 {
-    return gla::qr_entry<float>(h, m, n, A, lda, 0, tau, stream, true);
+    return gla::qr_entry<float>(h, m, n, A, lda, 0, tau, nullptr, 0, stream, true);
 }
 int gla_qr_blocked_d(gla_handle_t h, int64_t m, int64_t n, double *A, int64_t lda,
                      int64_t blocksize, double *tau, double *work, int64_t ldwork, void *stream)
 {
-    (void)work;
-    (void)ldwork;
-    return gla::qr_entry<double>(h, m, n, A, lda, blocksize, tau, stream, false);
+    return gla::qr_entry<double>(h, m, n, A, lda, blocksize, tau, work, ldwork, stream, false);
 }
 int gla_qr_blocked_s(gla_handle_t h, int64_t m, int64_t n, float *A, int64_t lda,
                      int64_t blocksize, float *tau, float *work, int64_t ldwork, void *stream)
 {
-    (void)work;
-    (void)ldwork;
-    return gla::qr_entry<float>(h, m, n, A, lda, blocksize, tau, stream, false);
+    return gla::qr_entry<float>(h, m, n, A, lda, blocksize, tau, work, ldwork, stream, false);
 }
 int gla_qr_tblock_d(gla_handle_t h, int64_t m, int64_t n, const double *F, int64_t ldf,
                     const double *tau, double *T, int64_t ldt, void *stream)

@@ -123,6 +123,22 @@ int gla_set_tuning(gla_handle_t hh, const char *key, int64_t value)
         h->qr_lookahead = (int)value;
         return GLA_OK;
     }
+    if (std::strcmp(key, "spin_limit") == 0) {
+        if (value < 1 || value > (int64_t)0x7fffffff) {
+            gla::set_error("spin_limit must be in [1, 2^31)");
+            return -3;
+        }
+        h->spin_limit = (unsigned)value;
+        return GLA_OK;
+    }
+    if (std::strcmp(key, "qr_wait_test") == 0) {
+        if (value != 0 && value != 1) {
+            gla::set_error("qr_wait_test must be 0 or 1");
+            return -3;
+        }
+        h->qr_wait_test = (int)value;
+        return GLA_OK;
+    }
     gla::set_error("unknown tuning key '%s'", key);
     return -2;
 }

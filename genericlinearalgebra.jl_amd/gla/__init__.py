@@ -19,10 +19,11 @@ import ctypes as C
 import numpy as np
 import torch
 
-from ._lib import GLA_OK, build, exported_symbols, lib  # noqa: F401
+from ._lib import GLA_ERR_TIMEOUT, GLA_OK, build, exported_symbols, lib  # noqa: F401
 
 __all__ = [
-    "Val", "DomainError", "DimensionMismatch", "ArgumentError", "colmajor", "to_numpy",
+    "Val", "DomainError", "DimensionMismatch", "ArgumentError", "DeviceTimeout", "colmajor",
+    "to_numpy",
     "LowerTriangular", "UpperTriangular", "UnitLowerTriangular", "Hermitian",
     "cholUnblocked_", "cholBlocked_", "cholRecursive_", "rankUpdate_", "rdiv_", "lmul_",
     "QR2", "R", "QBlocked", "HouseholderBlock", "Adjoint", "adjoint", "qrUnblocked_",
@@ -36,6 +37,11 @@ class DomainError(ValueError):
 
 class DimensionMismatch(ValueError):
     pass
+
+
+class DeviceTimeout(RuntimeError):
+    """A bounded device-side wait gave up (GLA_ERR_TIMEOUT); the call's
+    outputs are not valid."""
 
 
 class ArgumentError(ValueError):
@@ -85,6 +91,8 @@ def _check(rc: int, what: str):
         msg = lib().gla_last_error_string().decode()
         if rc < 0 and rc > -100:
             raise ArgumentError(f"{what}: invalid argument {-rc}: {msg}")
+        if rc == GLA_ERR_TIMEOUT:
+            raise DeviceTimeout(f"{what}: {msg}")
         raise RuntimeError(f"{what} failed ({rc}): {msg}")
 
 
@@ -543,17 +551,21 @@ class MixedSolveResult:
                 f"berr={self.berr:.3e}, converged={self.converged})")
 
 
-_PREC = {"fp64": b"d", "fp32": b"s", "bf16": b"b", "fp16": b"h", "auto": b"a"}
-_PREC_NAME = {b"d": "fp64", b"s": "fp32", b"b": "bf16", b"h": "fp16"}
+_PREC = {"fp64": b"d", "fp32": b"s", "bf16": b"B", "fp16": b"H", "bf16x1": b"b",
+         "fp16x1": b"h", "auto": b"a"}
+_PREC_NAME = {b"d": "fp64", b"s": "fp32", b"B": "bf16", b"H": "fp16", b"b": "bf16x1",
+              b"h": "fp16x1"}
 
 
 def chol_solve_mixed(A, B, precision: str = "auto", tol: float = 1e-12, max_iter: int = 30,
                      X=None):
     """X = A⁻¹ B for SPD Float64 A (lower triangle read), factored in
-    `precision` ("fp64", "fp32", "bf16": FP32 storage + BF16 MFMA trailing
-    updates with FP64 diagonal blocks, "fp16": the same with FP16, "auto":
-    fp32, then fp64 if fp32 does not converge) and refined in Float64
-    to a normwise backward error ≤ tol.  Raises DomainError (index = first
+    `precision` and refined in Float64 to a normwise backward error ≤ tol:
+    "fp64"; "fp32"; "bf16": FP32 storage, FP64 diagonal blocks, trailing
+    updates on BF16 MFMA with FP32 accumulation from three BF16 terms of
+    each panel; "fp16": the same with two scaled FP16 terms; "bf16x1" /
+    "fp16x1": one term (faster, for κ up to about 1e3 / 1e4); "auto": fp32,
+    then fp64 if fp32 does not converge.  Raises DomainError (index = first
     non-positive pivot) when a fixed-precision factorisation breaks down.
     Returns (X, MixedSolveResult); check .converged for fixed precisions."""
     if precision not in _PREC:
@@ -591,11 +603,16 @@ def chol_solve_mixed(A, B, precision: str = "auto", tol: float = 1e-12, max_iter
     return (X.view(-1) if vec else X), res
 
 
-def gemm_lowp_nt_(Cm, A, B, alpha=-1.0, precision: str = "bf16"):
-    """Cm (FP32) += alpha · A Bᵀ with A, B rounded to bf16/fp16, FP32
-    accumulation (the trailing update of the mixed factorisation)."""
-    if precision not in ("bf16", "fp16"):
-        raise ArgumentError("precision must be bf16 or fp16")
+_LOWP_TIN = {"bf16x1": b"b", "fp16x1": b"h", "bf16": b"B", "fp16": b"H"}
+
+
+def gemm_lowp_nt_(Cm, A, B, alpha=-1.0, precision: str = "bf16x1"):
+    """Cm (FP32) += alpha · A Bᵀ on BF16/FP16 MFMA with FP32 accumulation
+    (the trailing update of the mixed factorisation): "bf16x1"/"fp16x1" round
+    A and B once; "bf16" / "fp16" split them into three BF16 / two scaled
+    FP16 terms."""
+    if precision not in _LOWP_TIN:
+        raise ArgumentError(f"precision must be one of {sorted(_LOWP_TIN)}")
     for X, nm in ((Cm, "C"), (A, "A"), (B, "B")):
         if X.dtype != torch.float32:
             raise TypeError(f"{nm} must be Float32")
@@ -604,7 +621,7 @@ def gemm_lowp_nt_(Cm, A, B, alpha=-1.0, precision: str = "bf16"):
     if A.shape[0] != m or B.shape[0] != n or A.shape[1] != B.shape[1]:
         raise DimensionMismatch("C (m×n) += A (m×k) Bᵀ (k×n)")
     with torch.cuda.device(Cm.device):
-        rc = lib().gla_gemm_lowp_nt(_handle(Cm.device.index or 0), _PREC[precision], m, n,
+        rc = lib().gla_gemm_lowp_nt(_handle(Cm.device.index or 0), _LOWP_TIN[precision], m, n,
                                     A.shape[1], float(alpha), C.c_void_p(A.data_ptr()), lda,
                                     C.c_void_p(B.data_ptr()), ldb, C.c_void_p(Cm.data_ptr()),
                                     ldc, _stream(Cm))

@@ -22,6 +22,7 @@ _int = C.c_int
 GLA_OK = 0
 GLA_ERR_HIP = -1000
 GLA_ERR_UNSUPPORTED = -1001
+GLA_ERR_TIMEOUT = -1003
 
 
 def build(jobs: int = 8) -> str:

@@ -11,8 +11,16 @@
  *     stream and stores the result there.
  *   - Return value: GLA_OK (0), or a negative code: -k for an invalid k-th
  *     argument (LAPACK style), GLA_ERR_HIP for a HIP runtime error,
- *     GLA_ERR_UNSUPPORTED for a shape this build does not handle.
+ *     GLA_ERR_UNSUPPORTED for a shape this build does not handle,
+ *     GLA_ERR_TIMEOUT when a bounded device-side wait gave up (the outputs
+ *     of that call are then not valid).
  *     gla_last_error_string() describes the last failure on this thread.
+ *   - A handle serves one stream at a time: its workspace, grid-barrier
+ *     counter, info words and side stream are shared by every call made
+ *     with it, so calls that may overlap on the device (different streams,
+ *     different host threads) need a handle each.  The reference's
+ *     functions keep no state between calls; one handle per Julia task or
+ *     stream gives the same re-entrancy.
  *   - `info` (Cholesky): 0, or the 1-based index of the first pivot whose
  *     square root is taken of a negative number — the point where the
  *     reference's sqrt throws DomainError (src/cholesky.jl:5,40).  A zero or
@@ -34,6 +42,7 @@ extern "C" {
 #define GLA_ERR_HIP (-1000)
 #define GLA_ERR_UNSUPPORTED (-1001)
 #define GLA_ERR_NOT_SPD (-1002)
+#define GLA_ERR_TIMEOUT (-1003)
 
 typedef struct gla_handle_s *gla_handle_t;
 
@@ -44,7 +53,7 @@ typedef struct gla_handle_s *gla_handle_t;
 int gla_handle_create(gla_handle_t *h);
 int gla_handle_destroy(gla_handle_t h);
 const char *gla_last_error_string(void);
-/* Version string of this build. */
+/* Version string of this build (no reference counterpart). */
 const char *gla_version(void);
 /* Tuning knobs of a handle (no reference counterpart).  "chol_block": a
  * multiple of 128 (default 512) runs cholRecursive! as a right-looking
@@ -73,7 +82,11 @@ const char *gla_version(void);
  * trailing updates) to a direct-to-LDS kernel with a 4/3/3-deep ring of
  * 8/8/16-wide K slabs; "gemm_lds1" (process-wide): 1 (default) stages
  * FP64 128×128 tiles through one LDS buffer, 0 through two.  Same results
- * bit for bit in every setting. */
+ * bit for bit in every setting.  "spin_limit": polls (default 2^25, about a
+ * second) before a grid-wide wait of the QR panel gives up and the call
+ * returns GLA_ERR_TIMEOUT.  "qr_wait_test": 1 makes every such wait target
+ * an arrival count that never comes (a test of the timeout report; the
+ * kernels still finish, after spin_limit polls per wait). */
 int gla_set_tuning(gla_handle_t h, const char *key, int64_t value);
 
 /* ------------------------------------------------------------ Cholesky -- */
@@ -124,8 +137,21 @@ int gla_qr_unblocked_d(gla_handle_t h, int64_t m, int64_t n, double *A, int64_t 
 int gla_qr_unblocked_s(gla_handle_t h, int64_t m, int64_t n, float *A, int64_t lda,
                        float *tau, void *stream);
 
-/* qrBlocked!(A, blocksize = 12, τ, work) — src/qr.jl:113-146.  `work`
- * (blocksize × n, ld ldwork) may be NULL; the handle's workspace is used. */
+/* qrBlocked!(A, blocksize = 12, τ, work) — src/qr.jl:113-146.
+ * Block columns are min(blocksize, 128) wide; the factors do not depend on
+ * the block width (the same Householder sequence, up to rounding).
+ * `work` (ld ldwork ≥ min(m, blocksize), at least n − blocksize columns)
+ * may be NULL.  When given it is left as the reference leaves it: each step
+ * with trailing columns writes its lmul! workspace M = −V₁ Tᵀ (Vᵀ A₂)
+ * (src/householder.jl:142-153) to work[0:k, 0:n−j−w], later steps
+ * overwriting the top-left; for blocksize > 128 that is the content
+ * qrBlocked!(A, 128, τ, work) leaves.
+ * Both QR entry points synchronise the stream (to read the panel's
+ * wait-timeout flag; GLA_ERR_TIMEOUT if it was raised).  A block column of
+ * height m is factored by a register-resident panel whose workgroups must
+ * all be resident: narrower sub-panels (64, 32 wide) take block columns up
+ * to 512·CUs rows (131072 on MI355X); taller ones return
+ * GLA_ERR_UNSUPPORTED. */
 int gla_qr_blocked_d(gla_handle_t h, int64_t m, int64_t n, double *A, int64_t lda,
                      int64_t blocksize, double *tau, double *work, int64_t ldwork,
                      void *stream);
@@ -184,11 +210,14 @@ int gla_chol_solve_batched_s(gla_handle_t h, gla_batch_plan_t plan, float *const
  * No counterpart in the reference, whose Cholesky (src/cholesky.jl:3-55)
  * factors in the element type of A only; this extends cholRecursive! with
  * a lower-precision factorisation and FP64 refinement.  `prec` selects the
- * factorisation: 'd' FP64 (the plain path), 's' FP32, 'b' FP32 storage with
- * BF16 MFMA trailing updates and FP64 diagonal blocks, 'h' the same with
- * FP16, 'a' FP32 first, then FP64 if FP32 does not converge (a BF16/FP16
- * factorisation that breaks down stops within four 512-column blocks of the
- * failing pivot).  X is
+ * factorisation: 'd' FP64 (the plain path), 's' FP32, 'B' FP32 storage with
+ * FP64 diagonal blocks and the trailing updates on BF16 MFMA (FP32
+ * accumulation) from each panel split into three BF16 terms, 'H' the same
+ * with two FP16 terms of the panel scaled by a power of two, 'b' / 'h' one
+ * BF16 / FP16 term (faster; for κ up to about 1e3 / 1e4), 'a' FP32 first,
+ * then FP64 if FP32 does not converge (a BF16/FP16 factorisation that
+ * breaks down stops within four 512-column blocks of the failing pivot).
+ * X is
  * refined in FP64 (r = B − A X with the symmetric A, correction with the
  * factor) until the normwise backward error
  * max_k ‖r_k‖₂ / (‖A‖_F ‖x_k‖₂ + ‖b_k‖₂) ≤ tol; a precision stops when η
@@ -204,9 +233,10 @@ int gla_chol_solve_mixed_d(gla_handle_t h, char prec, int64_t n, const double *A
                            int64_t *info, char *prec_used, void *stream);
 /* Low-precision MFMA product used by the mixed factorisation:
  * C (m×n, FP32) += alpha · A Bᵀ with A (m×k), B (n×k) FP32 in memory, each
- * rounded once to `tin` ('b' BF16, 'h' FP16) into a packed copy in the
- * handle's workspace, FP32 accumulation.  Mirrors rankUpdate!'s role
- * (src/juliaBLAS.jl:89). */
+ * rounded once to `tin` ('b' BF16, 'h' FP16) or split into terms ('B' three
+ * BF16 terms, 'H' two FP16 terms after a common power-of-two scaling) in
+ * packed copies in the handle's workspace, FP32 accumulation.  Mirrors
+ * rankUpdate!'s role (src/juliaBLAS.jl:89). */
 int gla_gemm_lowp_nt(gla_handle_t h, char tin, int64_t m, int64_t n, int64_t k, float alpha,
                      const float *A, int64_t lda, const float *B, int64_t ldb, float *C,
                      int64_t ldc, void *stream);
@@ -252,8 +282,9 @@ int gla_chol_adjoint_d(gla_handle_t h, char conv, int64_t n, const double *L, in
 int gla_chol_adjoint_s(gla_handle_t h, char conv, int64_t n, const float *L, int64_t ldl,
                        const float *Lbar, int64_t ldlbar, float *Abar, int64_t ldabar,
                        void *stream);
-/* The same for a strided batch of `batch` SPD matrices of order n ≤ 128,
- * recomputing each factor: matrix b is A + b·strideA (lower triangle read),
+/* No counterpart in the reference: the same for a strided batch of `batch`
+ * SPD matrices of order n ≤ 128, recomputing each factor (cholUnblocked!,
+ * src/cholesky.jl:3-15): matrix b is A + b·strideA (lower triangle read),
  * its cotangent Lbar + b·strideLbar, its Ā written to Abar + b·strideAbar.
  * info (device, `batch` int32, or NULL): 0, or the first non-positive pivot
  * of that matrix (its Ā is then not meaningful).  Every step of the
@@ -266,7 +297,8 @@ int gla_chol_adjoint_batched_s(gla_handle_t h, char conv, int64_t n, int64_t bat
                                const float *A, int64_t lda, int64_t strideA, const float *Lbar,
                                int64_t ldlbar, int64_t strideLbar, float *Abar, int64_t ldabar,
                                int64_t strideAbar, int32_t *info, void *stream);
-/* QR (m ≥ n), as stored by qrBlocked!/qrUnblocked! (F: R on and above the
+/* No counterpart in the reference: QR (m ≥ n), as stored by
+ * qrBlocked!/qrUnblocked! (src/qr.jl:86-146; F: R on and above the
  * diagonal, reflectors below; τ), with Q̄ = 0 and R̄ (n×n, upper; strict
  * lower not read): Ā (m×n) = Q [copyltu(R R̄ᵀ) R⁻ᵀ ; 0], copyltu(M) =
  * tril(M) + tril(M,−1)ᵀ, Q the thin factor of the reflectors. */

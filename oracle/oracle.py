@@ -196,15 +196,17 @@ def qr_unblocked_q(A):
     return A, tau
 
 
-def qr_blocked(A, blocksize: int = 12):
-    """qrBlocked!(copy(A), blocksize) -> (factors, τ); src/qr.jl:113-146."""
+def qr_blocked(A, blocksize: int = 12, return_work: bool = False):
+    """qrBlocked!(copy(A), blocksize) -> (factors, τ), or (factors, τ, work)
+    with the blocksize × n workspace as the call leaves it (zero-initialised);
+    src/qr.jl:113-146."""
     A = _f(A)
     m, n = A.shape
     tau = np.zeros(min(m, n), dtype=A.dtype)
     work = np.zeros((blocksize, max(n, 1)), dtype=A.dtype, order="F")
     _fn("orc_qr_blocked", A)(m, n, _ptr(A), m, int(blocksize), _ptr(tau), _ptr(work),
                              blocksize)
-    return A, tau
+    return (A, tau, work) if return_work else (A, tau)
 
 
 def qr_tblock(F, tau):

@@ -275,3 +275,20 @@ def test_golden_qr():
         F, tau = O.qr_blocked(A, 16)
         assert rel(F, g["F_" + tag]) <= 10 * max(A.shape) * eps_of(np.float64), tag
         assert rel(tau, g["tau_" + tag]) <= 10 * max(A.shape) * eps_of(np.float64), tag
+
+
+@pytest.mark.parametrize("m,bz", [(40, 8), (25, 12), (9, 5)])
+def test_qr_blocked_work_closed_form(rng, m, bz):
+    """With n = 2·blocksize there is one lmul! (src/qr.jl:132) and its
+    workspace ends as M = −V₁Tᵀ(VᵀA₂), whose top-k rows satisfy
+    A₁ + M = R₁₂ (src/householder.jl:153-154): work = R₁₂ − A₁ with A₁ the
+    original rows 0:k of columns bz:2bz and R₁₂ the final R there.  A
+    transposed V₁, T in place of Tᵀ or a dropped axpy! fails this."""
+    n = 2 * bz
+    A = rng.standard_normal((m, n))
+    F, tau, work = O.qr_blocked(A, bz, return_work=True)
+    k = min(m, bz)
+    expect = F[:k, bz:n] - A[:k, bz:n]
+    assert np.abs(work[:k, :n - bz] - expect).max() <= 1e-13 * np.abs(A).max()
+    # rows the step never writes keep their initial zeros
+    assert not work[k:, :].any() and not work[:, n - bz:].any()

@@ -290,3 +290,77 @@ def test_lmul_unit_lower(rng, m, n):
     gla.lmul_(gla.UnitLowerTriangular(gla.colmajor(L)), Bg, -0.5)
     ref = O.lmul_unit_lower(L, B, -0.5)
     assert rel(gla.to_numpy(Bg), ref) <= 10 * m * eps_of(np.float64)
+
+
+@pytest.mark.parametrize("m,n,bz", [(100, 50, 7), (100, 100, 16), (50, 100, 9), (10, 10, 3),
+                                    (300, 200, 64), (700, 450, 128), (2000, 600, 128),
+                                    (400, 300, 200)])
+@pytest.mark.parametrize("la", [0, 1])
+def test_work_left_as_the_reference_leaves_it(m, n, bz, la):
+    """qrBlocked!'s `work` after the call (src/qr.jl:117,132): each step's
+    lmul! workspace M = −V₁Tᵀ(VᵀA₂), later steps overwriting the top-left;
+    compared with the oracle's work (blocksize > 128: as qrBlocked!(A, 128)
+    leaves it, the documented block width)."""
+    gla = _gla()
+    rng = np.random.default_rng(m + 3 * n + bz)
+    A = rng.standard_normal((m, n))
+    obz = min(bz, 128)
+    ofac, otau, owork = O.qr_blocked(A, obz, return_work=True)
+    work = gla.colmajor(np.zeros((obz, n)))
+    try:
+        gla.set_tuning("qr_lookahead", la)
+        F = gla.qrBlocked_(gla.colmajor(A), bz, work=work)
+    finally:
+        gla.set_tuning("qr_lookahead", 1)
+    tol = 10 * max(m, n) * eps_of(np.float64)
+    assert rel(gla.to_numpy(F.factors), ofac) <= tol
+    assert rel(gla.to_numpy(work), owork) <= tol, rel(gla.to_numpy(work), owork)
+
+
+def test_panel_wait_timeout_is_reported(rng):
+    """A grid-wide wait of the column panel that gives up makes the call
+    return GLA_ERR_TIMEOUT (DeviceTimeout): forced from the host by waits
+    that target an arrival count that never comes, each giving up after a
+    few polls, so every kernel still finishes.  The next call with the
+    normal settings succeeds and matches the oracle."""
+    gla = _gla()
+    A = rng.standard_normal((4096, 128))
+    try:
+        gla.set_tuning("qr_hr", 0)           # the column panel: 32 workgroups
+        gla.set_tuning("spin_limit", 64)
+        gla.set_tuning("qr_wait_test", 1)
+        with pytest.raises(gla.DeviceTimeout, match="gave up"):
+            gla.qrBlocked_(gla.colmajor(A), 128)
+    finally:
+        gla.set_tuning("qr_wait_test", 0)
+        gla.set_tuning("spin_limit", 1 << 25)
+    try:
+        F = gla.qrBlocked_(gla.colmajor(A), 128)
+    finally:
+        gla.set_tuning("qr_hr", -1)
+    ofac, otau = O.qr_blocked(A, 128)
+    tol = 10 * 4096 * eps_of(np.float64)
+    assert rel(gla.to_numpy(F.factors), ofac) <= tol
+    assert rel(F.tau.cpu().numpy(), otau) <= tol
+
+
+def test_block_column_taller_than_the_64_wide_panel(rng):
+    """70000 rows: more than the 64-wide panel's 256·256 rows, so the block
+    column is factored as 32-wide sub-panels; against the oracle."""
+    gla = _gla()
+    m, n = 70000, 128
+    A = rng.standard_normal((m, n))
+    F = gla.qrBlocked_(gla.colmajor(A), 128)
+    ofac, otau = O.qr_blocked(A, 128)
+    tol = 10 * m * eps_of(np.float64)
+    assert rel(gla.to_numpy(F.factors), ofac) <= tol
+    assert rel(F.tau.cpu().numpy(), otau) <= tol
+
+
+def test_block_column_beyond_capacity_is_refused_cleanly():
+    """Above 512 rows per CU even 32-wide sub-panels cannot be resident at
+    once: the call returns GLA_ERR_UNSUPPORTED with a message, nothing runs."""
+    gla = _gla()
+    A = gla.colmajor(np.ones((140000, 32)))
+    with pytest.raises(RuntimeError, match="exceeds the register-resident panel"):
+        gla.qrBlocked_(A, 32)
