@@ -20,6 +20,7 @@
 // L' = P Lᵀ P (P the reversal), z = P (L'⁻¹ (P y)).
 #include <algorithm>
 #include <cmath>
+#include <type_traits>
 #include <vector>
 
 #include "gla_internal.hpp"
@@ -58,7 +59,7 @@ struct LpMfma<_Float16> {
 };
 
 // Low-precision product on packed operands.  A panel is rounded to TIN once
-// and stored row-major with K padded to kPkBK (lp_pack): row r holds its K
+// and stored row-major with K padded to kPkBK (lp_pack_split): row r holds its K
 // values contiguously, which is the MFMA operand layout (a lane reads eight
 // consecutive k of one row with one 16-byte load).  The trailing updates of
 // the factorisation read each panel row many times, so the rounding and the
@@ -67,32 +68,158 @@ constexpr int kPkT = 256, kPkBM = 128, kPkBK = 64, kPkLD = kPkBK + 8;
 
 inline int64_t lp_kpad(int64_t K) { return cdiv(std::max<int64_t>(K, 1), kPkBK) * kPkBK; }
 
-// dst[r·Kp + k] = TIN(src[r + k·lds]) for k < K, 0 for K ≤ k < Kp.
-template <typename TIN>
-__global__ void lp_pack_kernel(int64_t rows, int64_t K, int64_t Kp, const float *src, int64_t lds,
-                               TIN *dst)
+// Multi-term operands.  An FP32 panel X is split into terms of the input
+// type, X = X₀ + X₁ (+ X₂) with each term the rounding of what the previous
+// ones leave (FP16: two terms, 22 of FP32's 24 significand bits; BF16: three
+// terms, all 24), and the update X Xᵀ is formed from the products whose
+// size is above the target accuracy as ONE product with a longer K:
+//   two terms:   A' = [X₀ X₀ X₁],        B' = [X₀ X₁ X₀]         (3 segments)
+//   three terms: A' = [X₀ X₀ X₁ X₀ X₂ X₁], B' = [X₀ X₁ X₀ X₂ X₀ X₁] (6 segments)
+// so A'B'ᵀ = X₀X₀ᵀ + X₀X₁ᵀ + X₁X₀ᵀ (+ X₀X₂ᵀ + X₂X₀ᵀ + X₁X₁ᵀ), leaving out
+// terms of relative size 2⁻²² (FP16) or 2⁻²⁴ (BF16).  FP16 panels are first
+// scaled by a power of two s that brings max|X| to [2¹³, 2¹⁴), so that the
+// low terms stay normal; the product is then scaled back by 1/s² (exact).
+__host__ __device__ constexpr int lp_nseg(int nterm) { return nterm == 1 ? 1 : (nterm == 2 ? 3 : 6); }
+
+// Scale of an FP16 panel from max|X| (its bits in *maxbits): 2^(14 − e)
+// with max|X| < 2^e; 1 without a maximum or for a zero/non-finite one.
+__device__ __forceinline__ float lp_scale(const unsigned *maxbits)
+{
+    if (!maxbits) return 1.f;
+    const float m = __uint_as_float(*maxbits);
+    if (!(m > 0.f) || !(m < INFINITY)) return 1.f;
+    int e;
+    (void)frexpf(m, &e);
+    return ldexpf(1.f, 14 - e);
+}
+
+__global__ void lp_maxabs_kernel(int64_t rows, int64_t K, const float *src, int64_t lds,
+                                 unsigned *maxbits)
+{
+    const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    float m = 0.f;
+    if (r < rows)
+        for (int64_t k = blockIdx.y; k < K; k += gridDim.y) m = fmaxf(m, fabsf(src[r + k * lds]));
+    // a wave's maximum, then one atomic per wave (non-negative floats order
+    // as their bit patterns)
+    for (int o = 32; o > 0; o >>= 1) m = fmaxf(m, __shfl_xor(m, o));
+    if ((threadIdx.x & 63) == 0 && m > 0.f) atomicMax(maxbits, __float_as_uint(m));
+}
+
+__global__ void lp_zero_u32(unsigned *p) { *p = 0u; }
+
+// A' and B' (row-major, pitch nseg·Kp) of the FP32 panel src (rows × K),
+// NT terms of TIN, scaled by lp_scale(maxbits).  Null dA or dB: that
+// operand's pattern is not written (a general product packs its two factors
+// separately).
+template <typename TIN, int NT>
+__global__ void lp_pack_split_kernel(int64_t rows, int64_t K, int64_t Kp, const float *src,
+                                     int64_t lds, TIN *dA, TIN *dB, const unsigned *maxbits)
 {
     typedef typename LpMfma<TIN>::v8 v8;
+    constexpr int NS = lp_nseg(NT);
     const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (r >= rows) return;
+    const float sc = lp_scale(maxbits);
+    const int64_t P = NS * Kp;
     for (int64_t k0 = 8 * (int64_t)blockIdx.y; k0 < Kp; k0 += 8 * (int64_t)gridDim.y) {
-        v8 v;
+        v8 t[NT];
 #pragma unroll
-        for (int e = 0; e < 8; ++e) v[e] = k0 + e < K ? (TIN)src[r + (k0 + e) * lds] : (TIN)0.f;
-        *reinterpret_cast<v8 *>(dst + r * Kp + k0) = v;
+        for (int e = 0; e < 8; ++e) {
+            float x = k0 + e < K ? src[r + (k0 + e) * lds] * sc : 0.f;
+#pragma unroll
+            for (int q = 0; q < NT; ++q) {
+                const TIN v = (TIN)x;
+                t[q][e] = v;
+                x -= (float)v;
+            }
+        }
+        // segment order of A' / B' (see above)
+        if (dA) {
+            TIN *a = dA + r * P + k0;
+            *reinterpret_cast<v8 *>(a) = t[0];
+            if constexpr (NT >= 2) {
+                *reinterpret_cast<v8 *>(a + Kp) = t[0];
+                *reinterpret_cast<v8 *>(a + 2 * Kp) = t[1];
+            }
+            if constexpr (NT >= 3) {
+                *reinterpret_cast<v8 *>(a + 3 * Kp) = t[0];
+                *reinterpret_cast<v8 *>(a + 4 * Kp) = t[2];
+                *reinterpret_cast<v8 *>(a + 5 * Kp) = t[1];
+            }
+        }
+        if (dB && dB != dA) {
+            TIN *b = dB + r * P + k0;
+            *reinterpret_cast<v8 *>(b) = t[0];
+            if constexpr (NT >= 2) {
+                *reinterpret_cast<v8 *>(b + Kp) = t[1];
+                *reinterpret_cast<v8 *>(b + 2 * Kp) = t[0];
+            }
+            if constexpr (NT >= 3) {
+                *reinterpret_cast<v8 *>(b + 3 * Kp) = t[2];
+                *reinterpret_cast<v8 *>(b + 4 * Kp) = t[0];
+                *reinterpret_cast<v8 *>(b + 5 * Kp) = t[1];
+            }
+        }
     }
 }
 
+// One packed panel of the factorisation: A' and B' (the same buffer for a
+// single term), pitch nseg·lp_kpad(K), and the FP16 scale word.
 template <typename TIN>
-int lp_pack(hipStream_t s, int64_t rows, int64_t K, const float *src, int64_t lds, TIN *dst)
+struct LpPanel {
+    TIN *a = nullptr, *b = nullptr;
+    unsigned *maxbits = nullptr;   // null: no scaling (BF16)
+    int nterm = 1;
+    int64_t pitch(int64_t K) const { return lp_nseg(nterm) * lp_kpad(K); }
+};
+
+// max|src| over rows × K into *maxbits (atomic maximum: callers zero it
+// first, and may accumulate several operands into one word).
+inline int lp_maxabs(hipStream_t s, int64_t rows, int64_t K, const float *src, int64_t lds,
+                     unsigned *maxbits)
+{
+    if (rows <= 0 || K <= 0) return GLA_OK;
+    hipLaunchKernelGGL(lp_maxabs_kernel, dim3((unsigned)cdiv(rows, 256),
+                                              (unsigned)std::min<int64_t>(K, 64)),
+                       dim3(256), 0, s, rows, K, src, lds, maxbits);
+    GLA_LAUNCH_CHECK();
+    return GLA_OK;
+}
+
+// Packs src into dA (A' pattern) and/or dB (B' pattern) with nterm terms.
+template <typename TIN>
+int lp_pack_split(hipStream_t s, int nterm, int64_t rows, int64_t K, const float *src,
+                  int64_t lds, TIN *dA, TIN *dB, const unsigned *maxbits)
 {
     if (rows <= 0) return GLA_OK;
     const int64_t Kp = lp_kpad(K);
-    hipLaunchKernelGGL((lp_pack_kernel<TIN>), dim3((unsigned)cdiv(rows, 256),
-                                                   (unsigned)std::min<int64_t>(Kp / 8, 65535)),
-                       dim3(256), 0, s, rows, K, Kp, src, lds, dst);
+    const dim3 grid((unsigned)cdiv(rows, 256), (unsigned)std::min<int64_t>(Kp / 8, 65535));
+    if (nterm == 1)
+        hipLaunchKernelGGL((lp_pack_split_kernel<TIN, 1>), grid, dim3(256), 0, s, rows, K, Kp, src,
+                           lds, dA, dB, maxbits);
+    else if (nterm == 2)
+        hipLaunchKernelGGL((lp_pack_split_kernel<TIN, 2>), grid, dim3(256), 0, s, rows, K, Kp, src,
+                           lds, dA, dB, maxbits);
+    else
+        hipLaunchKernelGGL((lp_pack_split_kernel<TIN, 3>), grid, dim3(256), 0, s, rows, K, Kp, src,
+                           lds, dA, dB, maxbits);
     GLA_LAUNCH_CHECK();
     return GLA_OK;
+}
+
+// The packed A' and B' of one panel of the factorisation (its SYRK update).
+template <typename TIN>
+int lp_pack_panel(hipStream_t s, int64_t rows, int64_t K, const float *src, int64_t lds,
+                  const LpPanel<TIN> &pp)
+{
+    if (rows <= 0) return GLA_OK;
+    if (pp.maxbits) {
+        hipLaunchKernelGGL(lp_zero_u32, dim3(1), dim3(1), 0, s, pp.maxbits);
+        GLA_LAUNCH_CHECK();
+        GLA_TRY(lp_maxabs(s, rows, K, src, lds, pp.maxbits));
+    }
+    return lp_pack_split<TIN>(s, pp.nterm, rows, K, src, lds, pp.a, pp.b, pp.maxbits);
 }
 
 struct LpPk {
@@ -101,7 +228,8 @@ struct LpPk {
     const void *B;   // N×Kp, TIN, row-major
     float *C;
     int64_t ldc;
-    float alpha;     // C <- C + alpha A Bᵀ
+    float alpha;     // C <- C + alpha A Bᵀ (divided by lp_scale(maxbits)²)
+    const unsigned *maxbits;
     int mode;        // 0 all, 1 lower (row >= col)
     int tiles_m, tiles_n, total, per_xcd;
 };
@@ -236,6 +364,11 @@ __global__ __launch_bounds__(kPkT, 2) void lp_gemm_pk_kernel(LpPk p)
     }
 
     // acc[i][j][r] = C(m0 + 64wm + 16i + lane%16, n0 + 64wn + 16j + 4(lane/16) + r)
+    float alpha = p.alpha;
+    if (p.maxbits) {
+        const float sc = lp_scale(p.maxbits);
+        alpha = alpha / (sc * sc);
+    }
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
         const int64_t gr = m0 + wm * 64 + i * 16 + (lane & 15);
@@ -247,22 +380,24 @@ __global__ __launch_bounds__(kPkT, 2) void lp_gemm_pk_kernel(LpPk p)
                 const int64_t gc = n0 + wn * 64 + j * 16 + 4 * (lane >> 4) + r;
                 if (gc < p.N && (p.mode == 0 || gr >= gc)) {
                     float *c = p.C + gr + gc * p.ldc;
-                    *c = *c + p.alpha * acc[i][j][r];
+                    *c = *c + alpha * acc[i][j][r];
                 }
             }
     }
 }
 
-// C (M×N) += alpha · A Bᵀ with A, B packed by lp_pack (pitch lp_kpad(K)).
+// C (M×N) += alpha · A Bᵀ with A, B packed by lp_pack_split (pitch lp_kpad(K)).
 template <typename TIN>
 int lp_gemm_pk(hipStream_t s, int mode, int64_t M, int64_t N, int64_t K, float alpha,
-               const TIN *A, const TIN *B, float *C, int64_t ldc)
+               const TIN *A, const TIN *B, float *C, int64_t ldc,
+               const unsigned *maxbits = nullptr)
 {
     if (M <= 0 || N <= 0) return GLA_OK;
     LpPk p{};
     p.M = M;
     p.N = N;
     p.Kp = lp_kpad(K);
+    p.maxbits = maxbits;
     p.A = A;
     p.B = B;
     p.C = C;
@@ -346,7 +481,8 @@ __global__ __launch_bounds__(256) void sym_sumsq_cols(int64_t n, const double *A
 // blocks and low-precision trailing updates (n ≤ the look-ahead block
 // width).  pk: room for the packed panel, n × lp_kpad(128) TIN.
 template <typename TIN>
-int chol_mixed(Handle *h, hipStream_t s, int64_t n, float *W, int64_t ldw, float *Linv32, TIN *pk)
+int chol_mixed(Handle *h, hipStream_t s, int64_t n, float *W, int64_t ldw, float *Linv32,
+               const LpPanel<TIN> &pk)
 {
     const int nb = 128;
     const int64_t nsteps = cdiv(n, nb);
@@ -362,12 +498,13 @@ int chol_mixed(Handle *h, hipStream_t s, int64_t n, float *W, int64_t ldw, float
         if (below == 0) break;
         const int64_t kb2 = std::min<int64_t>(nb, below);
         const int64_t rest = below - kb2;
-        const int64_t Kp = lp_kpad(kb);
-        GLA_TRY(lp_pack<TIN>(s, below, kb, W21, ldw, pk));
-        GLA_TRY(lp_gemm_pk<TIN>(s, 1, below, kb2, kb, -1.f, pk, pk, W21 + kb * ldw, ldw));
+        const int64_t P = pk.pitch(kb);
+        GLA_TRY(lp_pack_panel<TIN>(s, below, kb, W21, ldw, pk));
+        GLA_TRY(lp_gemm_pk<TIN>(s, 1, below, kb2, P, -1.f, pk.a, pk.b, W21 + kb * ldw, ldw,
+                                pk.maxbits));
         if (rest > 0)
-            GLA_TRY(lp_gemm_pk<TIN>(s, 1, rest, rest, kb, -1.f, pk + kb2 * Kp, pk + kb2 * Kp,
-                                    W21 + kb2 + (kb + kb2) * ldw, ldw));
+            GLA_TRY(lp_gemm_pk<TIN>(s, 1, rest, rest, P, -1.f, pk.a + kb2 * P, pk.b + kb2 * P,
+                                    W21 + kb2 + (kb + kb2) * ldw, ldw, pk.maxbits));
     }
     return GLA_OK;
 }
@@ -382,7 +519,7 @@ int chol_mixed(Handle *h, hipStream_t s, int64_t n, float *W, int64_t ldw, float
 // the previous step's buffer while the caller's stream packs the next.
 template <typename TIN>
 int chol_mixed_la(Handle *h, hipStream_t s, int64_t n, float *W, int64_t ldw, float *Linv32,
-                  float *Linv, int NB, TIN *pk0, TIN *pk1)
+                  float *Linv, int NB, const LpPanel<TIN> &pk0, const LpPanel<TIN> &pk1)
 {
     if (n <= NB) return chol_mixed<TIN>(h, s, n, W, ldw, Linv32, pk0);
     const int64_t nsteps = cdiv(n, NB);
@@ -397,7 +534,7 @@ int chol_mixed_la(Handle *h, hipStream_t s, int64_t n, float *W, int64_t ldw, fl
         const int64_t below = n - k - kb;
         float *Wkk = W + k + k * ldw;
         float *W21 = Wkk + kb;
-        TIN *pk = (step & 1) ? pk1 : pk0;
+        const LpPanel<TIN> &pk = (step & 1) ? pk1 : pk0;
         GLA_TRY(chol_mixed<TIN>(h, s, kb, Wkk, ldw, Linv32, pk));
         if (below == 0) break;
         // every fourth block, stop early once a pivot has failed: a
@@ -413,20 +550,21 @@ int chol_mixed_la(Handle *h, hipStream_t s, int64_t n, float *W, int64_t ldw, fl
         GLA_TRY(trsm_rlt<float>(s, below, kb, Wkk, ldw, Linv, W21, ldw));
         const int64_t kb2 = std::min<int64_t>(NB, below);
         const int64_t rest = below - kb2;
-        const int64_t Kp = lp_kpad(kb);
-        GLA_TRY(lp_pack<TIN>(s, below, kb, W21, ldw, pk));
+        const int64_t P = pk.pitch(kb);
+        GLA_TRY(lp_pack_panel<TIN>(s, below, kb, W21, ldw, pk));
         // as in the FP64 look-ahead driver, the side stream's trailing update
         // starts after the next block column's (chol_syrk_late) or with it
         const bool late = h->chol_syrk_late != 0;
         if (overlap && rest > 0 && !late) GLA_HIP_TRY(hipEventRecord(h->ev_step[0], s));
         if (overlap && step > 0) GLA_HIP_TRY(hipStreamWaitEvent(s, h->ev_step[1], 0));
-        GLA_TRY(lp_gemm_pk<TIN>(s, 1, below, kb2, kb, -1.f, pk, pk, W21 + kb * ldw, ldw));
+        GLA_TRY(lp_gemm_pk<TIN>(s, 1, below, kb2, P, -1.f, pk.a, pk.b, W21 + kb * ldw, ldw,
+                                pk.maxbits));
         if (overlap && rest > 0 && late) GLA_HIP_TRY(hipEventRecord(h->ev_step[0], s));
         if (rest > 0) {
             hipStream_t us = overlap ? h->side : s;
             if (overlap) GLA_HIP_TRY(hipStreamWaitEvent(h->side, h->ev_step[0], 0));
-            GLA_TRY(lp_gemm_pk<TIN>(us, 1, rest, rest, kb, -1.f, pk + kb2 * Kp, pk + kb2 * Kp,
-                                    W21 + kb2 + (kb + kb2) * ldw, ldw));
+            GLA_TRY(lp_gemm_pk<TIN>(us, 1, rest, rest, P, -1.f, pk.a + kb2 * P, pk.b + kb2 * P,
+                                    W21 + kb2 + (kb + kb2) * ldw, ldw, pk.maxbits));
             if (overlap) GLA_HIP_TRY(hipEventRecord(h->ev_step[1], h->side));
         }
     }
@@ -468,7 +606,10 @@ struct MixedWS {
     double *Xt, *nrm;
     float *Linv32;
     int *flags, *err;
-    void *pk0, *pk1;   // packed low-precision panels, pk_bytes each
+    // packed low-precision panels (A' and B' of two alternating steps),
+    // pk_bytes each, and their FP16 scale words
+    void *pk0a, *pk0b, *pk1a, *pk1b;
+    unsigned *maxbits;
     int64_t nblk, npad, nchunk;
 
     // pk_bytes: one packed panel of the BF16/FP16 factorisation (0 otherwise)
@@ -481,7 +622,7 @@ struct MixedWS {
                4 * al(es * (size_t)npad * 128) + al(es * (size_t)npad * 16 * nch) +
                2 * al(es * (size_t)npad * 16) + al(8 * symv_scratch(n)) +
                al(8 * (size_t)(32 * nch + n + 64)) +
-               al(4 * 8 * 256) + al(4 * (size_t)(nblk + 64)) + 2 * al(pk_bytes) + 4096;
+               al(4 * 8 * 256) + al(4 * (size_t)(nblk + 64)) + 4 * al(pk_bytes) + 4096;
     }
     void carve(void *base, int64_t n, int64_t nrhs, size_t pk_bytes)
     {
@@ -511,15 +652,40 @@ struct MixedWS {
         Linv32 = reinterpret_cast<float *>(take(4 * 8 * 256));
         flags = reinterpret_cast<int *>(take(4 * (size_t)(nblk + 64)));
         err = flags + nblk;
-        pk0 = pk_bytes ? take(pk_bytes) : nullptr;
-        pk1 = pk_bytes ? take(pk_bytes) : nullptr;
+        maxbits = reinterpret_cast<unsigned *>(flags + nblk + 8);
+        pk0a = pk_bytes ? take(pk_bytes) : nullptr;
+        pk0b = pk_bytes ? take(pk_bytes) : nullptr;
+        pk1a = pk_bytes ? take(pk_bytes) : nullptr;
+        pk1b = pk_bytes ? take(pk_bytes) : nullptr;
     }
 };
 
-// Bytes of one packed panel for precision prec at block width NB.
+// Precisions with low-precision trailing updates: 'b' / 'h' one BF16 /
+// FP16 term, 'B' BF16 in three terms, 'H' FP16 in two (scaled) terms.
+inline bool is_lowp(char prec) { return prec == 'b' || prec == 'h' || prec == 'B' || prec == 'H'; }
+inline int lowp_terms(char prec) { return prec == 'B' ? 3 : (prec == 'H' ? 2 : 1); }
+
+// Bytes of one packed operand (A' or B') for precision prec at block width NB.
 inline size_t pk_bytes_of(char prec, int64_t n, int NB)
 {
-    return (prec == 'b' || prec == 'h') ? 2 * (size_t)n * (size_t)lp_kpad(NB) : 0;
+    return is_lowp(prec) ? 2 * (size_t)n * lp_nseg(lowp_terms(prec)) * (size_t)lp_kpad(NB) : 0;
+}
+
+template <typename TIN>
+int chol_mixed_run(Handle *h, hipStream_t s, char prec, int64_t n, float *W, float *Linv32,
+                   float *Linv, MixedWS<float> &ws)
+{
+    LpPanel<TIN> p0, p1;
+    p0.nterm = p1.nterm = lowp_terms(prec);
+    p0.a = reinterpret_cast<TIN *>(ws.pk0a);
+    p1.a = reinterpret_cast<TIN *>(ws.pk1a);
+    p0.b = p0.nterm > 1 ? reinterpret_cast<TIN *>(ws.pk0b) : p0.a;
+    p1.b = p1.nterm > 1 ? reinterpret_cast<TIN *>(ws.pk1b) : p1.a;
+    if constexpr (sizeof(TIN) == 2 && !std::is_same<TIN, __bf16>::value) {
+        p0.maxbits = ws.maxbits;
+        p1.maxbits = ws.maxbits + 1;
+    }
+    return chol_mixed_la<TIN>(h, s, n, W, n, Linv32, Linv, h->mixed_block, p0, p1);
 }
 
 // Solve structures of the factor in ws.W: tile inverses, diagonal-block
@@ -641,14 +807,10 @@ int mixed_attempt(Handle *h, hipStream_t s, char prec, int64_t n, const double *
         GLA_LAUNCH_CHECK();
         if (prec == 's')
             GLA_TRY(chol_device<float>(h, s, n, ws.W, n, 2, 1, ws.Linv));
-        else if (prec == 'b')
-            GLA_TRY(chol_mixed_la<__bf16>(h, s, n, ws.W, n, ws.Linv32, ws.Linv,
-                                           h->mixed_block, reinterpret_cast<__bf16 *>(ws.pk0),
-                                           reinterpret_cast<__bf16 *>(ws.pk1)));
+        else if (prec == 'b' || prec == 'B')
+            GLA_TRY(chol_mixed_run<__bf16>(h, s, prec, n, ws.W, ws.Linv32, ws.Linv, ws));
         else
-            GLA_TRY(chol_mixed_la<_Float16>(h, s, n, ws.W, n, ws.Linv32, ws.Linv,
-                                             h->mixed_block, reinterpret_cast<_Float16 *>(ws.pk0),
-                                             reinterpret_cast<_Float16 *>(ws.pk1)));
+            GLA_TRY(chol_mixed_run<_Float16>(h, s, prec, n, ws.W, ws.Linv32, ws.Linv, ws));
     }
     int hinfo = 0;
     GLA_HIP_TRY(hipMemcpyAsync(&hinfo, h->dinfo, sizeof(int), hipMemcpyDeviceToHost, s));
@@ -708,28 +870,40 @@ int mixed_solve(Handle *h, hipStream_t s, char prec, int64_t n, const double *A,
 }
 
 template <typename TIN>
-int lp_gemm_packed_entry(Handle *h, int64_t m, int64_t n, int64_t k, float alpha, const float *A,
-                         int64_t lda, const float *B, int64_t ldb, float *C, int64_t ldc,
-                         hipStream_t s)
+int lp_gemm_packed_entry(Handle *h, int nterm, bool scaled, int64_t m, int64_t n, int64_t k,
+                         float alpha, const float *A, int64_t lda, const float *B, int64_t ldb,
+                         float *C, int64_t ldc, hipStream_t s)
 {
     if (m <= 0 || n <= 0) return GLA_OK;
-    const int64_t Kp = lp_kpad(k);
-    const size_t pa = ((size_t)m * Kp * sizeof(TIN) + 255) / 256 * 256;
-    GLA_TRY(ensure_workspace(h, pa + (size_t)n * Kp * sizeof(TIN)));
+    const int64_t P = lp_nseg(nterm) * lp_kpad(k);
+    const size_t pa = ((size_t)m * P * sizeof(TIN) + 255) / 256 * 256;
+    const size_t pb = ((size_t)n * P * sizeof(TIN) + 255) / 256 * 256;
+    GLA_TRY(ensure_workspace(h, pa + pb + 256));
     TIN *Ap = reinterpret_cast<TIN *>(h->ws);
     TIN *Bp = reinterpret_cast<TIN *>(reinterpret_cast<char *>(h->ws) + pa);
-    GLA_TRY(lp_pack<TIN>(s, m, k, A, lda, Ap));
-    GLA_TRY(lp_pack<TIN>(s, n, k, B, ldb, Bp));
-    return lp_gemm_pk<TIN>(s, 0, m, n, k, alpha, Ap, Bp, C, ldc);
+    unsigned *mb = scaled ? reinterpret_cast<unsigned *>(reinterpret_cast<char *>(h->ws) + pa + pb)
+                          : nullptr;
+    if (mb) {
+        // one scale for both factors, from the larger of their maxima
+        hipLaunchKernelGGL(lp_zero_u32, dim3(1), dim3(1), 0, s, mb);
+        GLA_LAUNCH_CHECK();
+        GLA_TRY(lp_maxabs(s, m, k, A, lda, mb));
+        GLA_TRY(lp_maxabs(s, n, k, B, ldb, mb));
+    }
+    GLA_TRY(lp_pack_split<TIN>(s, nterm, m, k, A, lda, Ap, nullptr, mb));
+    GLA_TRY(lp_pack_split<TIN>(s, nterm, n, k, B, ldb, nullptr, Bp, mb));
+    return lp_gemm_pk<TIN>(s, 0, m, n, P, alpha, Ap, Bp, C, ldc, mb);
 }
 
 int lp_gemm_entry(Handle *h, char tin, int64_t m, int64_t n, int64_t k, float alpha,
                   const float *A, int64_t lda, const float *B, int64_t ldb, float *C, int64_t ldc,
                   hipStream_t s)
 {
-    if (tin == 'b')
-        return lp_gemm_packed_entry<__bf16>(h, m, n, k, alpha, A, lda, B, ldb, C, ldc, s);
-    return lp_gemm_packed_entry<_Float16>(h, m, n, k, alpha, A, lda, B, ldb, C, ldc, s);
+    if (tin == 'b' || tin == 'B')
+        return lp_gemm_packed_entry<__bf16>(h, lowp_terms(tin), false, m, n, k, alpha, A, lda, B,
+                                            ldb, C, ldc, s);
+    return lp_gemm_packed_entry<_Float16>(h, lowp_terms(tin), tin == 'H', m, n, k, alpha, A, lda,
+                                          B, ldb, C, ldc, s);
 }
 
 }  // namespace gla
@@ -743,8 +917,9 @@ int gla_chol_solve_mixed_d(gla_handle_t hh, char prec, int64_t n, const double *
 {
     auto *h = reinterpret_cast<gla::Handle *>(hh);
     if (!h) { gla::set_error("null handle"); return -1; }
-    if (prec != 'd' && prec != 's' && prec != 'b' && prec != 'h' && prec != 'a') {
-        gla::set_error("prec must be one of 'd', 's', 'b', 'h', 'a'");
+    if (prec != 'd' && prec != 's' && prec != 'b' && prec != 'h' && prec != 'B' &&
+        prec != 'H' && prec != 'a') {
+        gla::set_error("prec must be one of 'd', 's', 'b', 'h', 'B', 'H', 'a'");
         return -2;
     }
     if (n < 0) return -3;
@@ -776,7 +951,7 @@ int gla_gemm_lowp_nt(gla_handle_t hh, char tin, int64_t m, int64_t n, int64_t k,
 {
     auto *h = reinterpret_cast<gla::Handle *>(hh);
     if (!h) return -1;
-    if (tin != 'b' && tin != 'h') return -2;
+    if (tin != 'b' && tin != 'h' && tin != 'B' && tin != 'H') return -2;
     if (m < 0) return -3;
     if (n < 0) return -4;
     if (k < 0) return -5;
