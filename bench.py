@@ -52,34 +52,70 @@ def cpu_info():
     return cores, model
 
 
+def host_threads():
+    """Host threads this run may use: the box's CPU share (OMP_NUM_THREADS)
+    or every core."""
+    cores = os.cpu_count() or 1
+    try:
+        return max(1, min(cores, int(os.environ.get("OMP_NUM_THREADS", cores))))
+    except ValueError:
+        return cores
+
+
 def cpu_baseline():
-    """The oracle (plain-C restatement of src/cholesky.jl and src/qr.jl, one
-    thread, generic loops as in the reference) on a bounded sample."""
+    """The reference's CPU path (oracle/reference_cpu.c: the package's generic
+    loops single-threaded as Julia runs them, the steps Julia hands to BLAS —
+    rdiv!, mul!, lmul! with a triangle, dot, inv! — on numpy's OpenBLAS with
+    all host threads) on a bounded sample of the step: cholRecursive!(copy(A))
+    n = 4096 and qrBlocked!(copy(M), 128) 4096×2048.  Also config C1:
+    cholRecursive!(copy(A), Val{:L}) at n = 1000, median of 20, beside the
+    reference's published 0.012098 s (README.md:36-39)."""
     import numpy as np
 
-    import oracle as O
+    import reference_cpu as R
 
+    threads = R.set_threads(host_threads())
     rng = np.random.default_rng(1)
-    n = 3072
+    n = 4096
     Bm = rng.standard_normal((n, n))
-    A = Bm.T @ Bm + n * np.eye(n)
-    O.lib()
+    A = np.asfortranarray(Bm.T @ Bm + n * np.eye(n))
+    del Bm
+    W = np.empty_like(A, order="F")
     t0 = time.perf_counter()
-    O.chol_recursive(A, 1)
+    W[...] = A
+    info = R.chol_recursive_inplace(W, 1)
     tc = time.perf_counter() - t0
-    m, k = 3072, 1536
+    m, k = 4096, 2048
     M = rng.standard_normal((m, k))
     t0 = time.perf_counter()
-    O.qr_blocked(M, 128)
+    R.qr_blocked(M, 128)
     tq = time.perf_counter() - t0
     fl = chol_flops(n) + qr_flops(m, k)
+    # C1
+    n1 = 1000
+    B1 = rng.standard_normal((n1, n1))
+    A1 = np.asfortranarray(B1.T @ B1 + n1 * np.eye(n1))
+    ts = []
+    for _ in range(21):
+        t0 = time.perf_counter()
+        W1 = A1.copy(order="F")
+        R.chol_recursive_inplace(W1, 1)
+        ts.append(time.perf_counter() - t0)
+    t1 = float(np.median(ts[1:]))
+    L1 = np.tril(W1)
+    c1_res = float(np.linalg.norm(A1 - L1 @ L1.T) / np.linalg.norm(A1))
     return {
         "value": fl / (tc + tq) / 1e12,
         "unit": "TFLOP/s",
-        "cores": 1,
+        "cores": threads,
         "kind": "port",
-        "sample": (f"oracle cholRecursive!(copy(A), Val{{:L}}) n={n} ({tc:.2f}s) + "
-                   f"qrBlocked!(copy(M), 128) {m}x{k} ({tq:.2f}s), FP64, single thread"),
+        "sample": (f"reference CPU path (generic loops 1 thread, BLAS steps on OpenBLAS with "
+                   f"{threads} threads): cholRecursive!(copy(A), Val{{:L}}) n={n} ({tc:.2f}s, "
+                   f"info {info}) + qrBlocked!(copy(M), 128) {m}x{k} ({tq:.2f}s), FP64"),
+        "c1": {"workload": "cholRecursive!(copy(A), Val{:L}) n=1000, A = BᵀB + n·I",
+               "s": t1, "gflops": chol_flops(n1) / t1 / 1e9,
+               "residual": c1_res, "published_s": 0.012098,
+               "published_note": "README.md:36-39, author's machine (unstated)"},
         "lapack": lapack_line(),
     }
 
@@ -184,8 +220,11 @@ def mixed_section(torch, gla, n=N_CHOL, nrhs=16):
                              "berr": r.berr, "converged": r.converged}
             except gla.DomainError as e:
                 row[prec] = {"error": f"DomainError (pivot {e})"}
-        if "ms" in row["fp64"] and "ms" in row["auto"] and row["auto"]["converged"]:
-            row["speedup_auto_vs_fp64"] = row["fp64"]["ms"] / row["auto"]["ms"]
+        for prec in ("auto", "fp32", "bf16", "fp16"):
+            if "ms" in row["fp64"] and "ms" in row[prec] and row[prec]["converged"]:
+                row[prec]["speedup_vs_fp64"] = row["fp64"]["ms"] / row[prec]["ms"]
+        if "speedup_vs_fp64" in row["auto"]:
+            row["speedup_auto_vs_fp64"] = row["auto"]["speedup_vs_fp64"]
         out["cases"].append(row)
         del A
     return out
@@ -372,6 +411,24 @@ def main():
 
     t_qr64 = ev_time(torch, qr64)                 # the other block size of the config
 
+    # config C1 on the device: cholRecursive!(copy(A), Val{:L}) at n = 1000,
+    # copy included, median of 21 HIP-event timings
+    g1 = torch.Generator(device=dev).manual_seed(11)
+    B1 = torch.randn(1000, 1000, dtype=torch.float64, device=dev, generator=g1)
+    A1 = (B1.t() @ B1 + 1000 * torch.eye(1000, dtype=torch.float64, device=dev))
+    A1 = A1.t().contiguous().t()
+    W1 = torch.empty_like(A1)
+
+    def c1():
+        W1.copy_(A1)
+        gla.cholRecursive_(W1, gla.Val.L)
+
+    c1()
+    t_c1 = sorted(ev_time(torch, c1, reps=1) for _ in range(21))[10]
+    L1 = torch.tril(W1)
+    c1_res = ((A1 - L1 @ L1.t()).norm() / A1.norm()).item()
+    del B1, A1, W1, L1
+
     # parity at full size: the factors against rocSOLVER's (an independent
     # library), to 10·n·eps, plus size-independent residuals
     eps = float(np.finfo(np.float64).eps)
@@ -455,6 +512,10 @@ def main():
                "blocksize64_tflops": qr_flops(M_QR, N_QR) / (t_qr64 - t_copy_m) / 1e12,
                "R_rel_diff_vs_rocsolver_sign_normalised": qr_diff,
                "bar_10_n_eps": 10 * M_QR * eps, "residual_QRx_vs_Mx": qr_bwd},
+        "c1": {"workload": "cholRecursive!(copy(A), Val{:L}) n=1000, A = BᵀB + n·I, copy "
+                           "included, median of 21",
+               "ms": t_c1 * 1e3, "gflops": chol_flops(1000) / t_c1 / 1e9,
+               "residual": c1_res, "reference_published_ms": 12.098},
         "roofline": {"bound": "mfma", "achieved": syrk_tf, "peak": FP64_PEAK_TFLOPS,
                      "unit": "TFLOP/s", "frac": syrk_tf / FP64_PEAK_TFLOPS,
                      "traffic": profiled_traffic("syrk_15360_k512.md"),
@@ -483,6 +544,7 @@ def main():
     if not args.no_cpu:
         cores, model = cpu_info()
         cb = cpu_baseline()
+        cb["c1"]["device_ms"] = out["c1"]["ms"]
         cb["host_cpu"] = model
         cb["host_cores_available"] = cores
         out["cpu_baseline"] = cb
