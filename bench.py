@@ -174,6 +174,12 @@ def profiled_traffic(name):
     return None
 
 
+def progress(msg):
+    """One line on stderr as each part of the run ends (the JSON result is
+    the only line on stdout)."""
+    print(f"bench: {msg}", file=sys.stderr, flush=True)
+
+
 def ev_time(torch, fn, reps=2):
     """Best of `reps` HIP-event timings of fn on the current stream (s)."""
     ts = []
@@ -227,6 +233,7 @@ def mixed_section(torch, gla, n=N_CHOL, nrhs=16):
             row["speedup_auto_vs_fp64"] = row["auto"]["speedup_vs_fp64"]
         out["cases"].append(row)
         del A
+        progress(f"mixed kappa {kappa:g} done")
     return out
 
 
@@ -399,6 +406,7 @@ def main():
     torch.cuda.synchronize()
     dt = (time.perf_counter() - t0) / args.steps
     flops = chol_flops(n) + qr_flops(M_QR, N_QR)
+    progress(f"timed {args.steps} steps: {dt * 1e3:.2f} ms/step")
 
     t_copy_a = ev_time(torch, lambda: Aw.copy_(A0))
     t_copy_m = ev_time(torch, lambda: Mw.copy_(M0))
@@ -428,6 +436,7 @@ def main():
     L1 = torch.tril(W1)
     c1_res = ((A1 - L1 @ L1.t()).norm() / A1.norm()).item()
     del B1, A1, W1, L1
+    progress("single-factorisation timings and C1 done")
 
     # parity at full size: the factors against rocSOLVER's (an independent
     # library), to 10·n·eps, plus size-independent residuals
@@ -541,6 +550,7 @@ def main():
             except Exception as e:  # noqa: BLE001  (a section failure is reported, not fatal)
                 out[key] = {"error": f"{type(e).__name__}: {e}"}
             torch.cuda.empty_cache()
+            progress(f"{key} section done")
     if not args.no_cpu:
         cores, model = cpu_info()
         cb = cpu_baseline()
@@ -548,6 +558,7 @@ def main():
         cb["host_cpu"] = model
         cb["host_cores_available"] = cores
         out["cpu_baseline"] = cb
+        progress("cpu baseline done")
     print(json.dumps(out))
 
 

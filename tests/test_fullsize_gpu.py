@@ -17,6 +17,8 @@ against the oracle where it is fast enough:
 """
 from concurrent.futures import ThreadPoolExecutor
 
+from threadpoolctl import threadpool_limits
+
 import numpy as np
 import pytest
 
@@ -86,7 +88,9 @@ def _check_batch(pb, As, bs, dtype):
             return f"{i} (n={n}): x differs by {ex:.2e}"
         return None
 
-    with ThreadPoolExecutor(16) as ex:
+    # one BLAS thread per worker: 16 workers each driving a 16-thread
+    # OpenBLAS oversubscribe the host and can stall for minutes
+    with threadpool_limits(1), ThreadPoolExecutor(16) as ex:
         errs = [e for e in ex.map(one, range(len(As))) if e]
     assert not errs, errs[:5]
 
@@ -270,6 +274,8 @@ def test_cholesky_adjoint_batched_512x128_vs_oracle():
         e = rel(got[b], ref)
         return None if e <= 50 * n * EPS64 else f"{b}: {e:.2e}"
 
-    with ThreadPoolExecutor(16) as ex:
+    # one BLAS thread per worker: 16 workers each driving a 16-thread
+    # OpenBLAS oversubscribe the host and can stall for minutes
+    with threadpool_limits(1), ThreadPoolExecutor(16) as ex:
         errs = [e for e in ex.map(one, range(batch)) if e]
     assert not errs, errs[:5]
