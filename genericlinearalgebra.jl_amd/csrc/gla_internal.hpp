@@ -82,6 +82,9 @@ struct Handle {
     // Release the look-ahead Cholesky's side-stream SYRK after the
     // next-block-column update (1) rather than right after the panel solve.
     int chol_syrk_late = 1;
+    // Step width of the LDLᵀ driver (128: 128-wide steps; a larger multiple
+    // of 128: that width with depth-1 look-ahead).
+    int64_t ldlt_block = 512;
     // Block width of the mixed-precision factorisation's look-ahead driver.
     int mixed_block = 512;
     // QR panel kernel: 1 LDS-reduction form, 2 wave-butterfly form.

@@ -58,6 +58,14 @@ int gla_set_tuning(gla_handle_t hh, const char *key, int64_t value)
         h->chol_block = value;
         return GLA_OK;
     }
+    if (std::strcmp(key, "ldlt_block") == 0) {
+        if (value < 128 || value % 128 != 0 || value > 4096) {
+            gla::set_error("ldlt_block must be a multiple of 128 in [128, 4096]");
+            return -3;
+        }
+        h->ldlt_block = value;
+        return GLA_OK;
+    }
     if (std::strcmp(key, "mixed_block") == 0) {
         if (value < 128 || value % 128 != 0 || value > 4096) {
             gla::set_error("mixed_block must be a multiple of 128 in [128, 4096]");

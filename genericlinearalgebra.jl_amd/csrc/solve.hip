@@ -97,6 +97,44 @@ __global__ void first_zero_diag_k(int64_t n, const T *R, int64_t ldr, int *info)
 
 __global__ void set_int_k(int *p, int v) { *p = v; }
 
+// S (rows×cols, ld lds) = 0.
+template <typename T>
+__global__ void set_zero_k(int64_t rows, int64_t cols, T *S, int64_t lds)
+{
+    GLA_ROW(i);
+    GLA_COLS(j, cols)
+    if (i < rows) S[i + j * lds] = T(0);
+}
+
+// Y (n×nrhs, column-major) <- D Y, D the diagonal of F.
+template <typename T>
+__global__ void scale_rows_k(int64_t n, int64_t nrhs, const T *F, int64_t ldf, T *Y, int64_t ldy)
+{
+    GLA_ROW(i);
+    GLA_COLS(k, nrhs)
+    if (i < n) Y[i + k * ldy] *= F[i + i * ldf];
+}
+
+// Lx (n×n, lower, explicit unit diagonal) = the unit lower factor of LDLᵀ
+// data F: L = F's strict lower triangle (upper = 0), or Uᵀ (upper = 1).
+template <typename T>
+__global__ void unit_lower_k(int64_t n, const T *F, int64_t ldf, T *Lx, int upper)
+{
+    GLA_ROW(i);
+    GLA_COLS(j, n)
+    if (i < n && i >= j)
+        Lx[i + j * n] = i == j ? T(1) : (upper ? F[j + i * ldf] : F[i + j * ldf]);
+}
+
+// Y (nrhs×n, row k = right-hand side k): Y[k][i] /= d_i, d the diagonal of F.
+template <typename T>
+__global__ void div_rows_k(int64_t n, int64_t nrhs, const T *F, int64_t ldf, T *Y)
+{
+    GLA_ROW(k);
+    GLA_COLS(i, n)
+    if (k < nrhs) Y[k + i * nrhs] = Y[k + i * nrhs] / F[i + i * ldf];
+}
+
 #undef GLA_COLS
 #undef GLA_ROW
 
@@ -150,6 +188,56 @@ int chol_solve_dev(hipStream_t s, char uplo, int64_t n, int64_t nrhs, const T *F
     hipLaunchKernelGGL((rev_lower_k<T>), g2(n, n), dim3(256), 0, s, n, L, ldl, Lr, n, 1);
     GLA_LAUNCH_CHECK();
     return reversed_tail<T>(s, n, nrhs, Lr, n, Lri, Z, B, ldb);
+}
+
+// B (n×nrhs) <- L⁻ᵀ D⁻¹ L⁻¹ B for LDLᵀ factor data F (uplo 'L': L in the
+// strict lower triangle, D on the diagonal; 'U': L = Uᵀ) — ldiv!(F::LDLt, X),
+// src/ldlt.jl:225-228.  Same scratch as the Cholesky solve.
+template <typename T>
+int ldlt_solve_dev(hipStream_t s, char uplo, int64_t n, int64_t nrhs, const T *F, int64_t ldf,
+                   T *B, int64_t ldb, T *scratch)
+{
+    if (n <= 0 || nrhs <= 0) return GLA_OK;
+    const size_t nn = (size_t)n * n, ti = tile_inverse_elems(n);
+    T *Lx = scratch, *Lr = Lx + nn, *Li = Lr + nn, *Lri = Li + ti, *Y = Lri + ti,
+      *Z = Y + (size_t)nrhs * n;
+    hipLaunchKernelGGL((unit_lower_k<T>), g2(n, n), dim3(256), 0, s, n, F, ldf, Lx,
+                       uplo == 'U' ? 1 : 0);
+    GLA_LAUNCH_CHECK();
+    // yᵀ = bᵀ L⁻ᵀ (ldiv!(L, X)), then ldiv!(D, X)
+    hipLaunchKernelGGL((to_rows_k<T>), g2(n, nrhs), dim3(256), 0, s, n, nrhs, B, ldb, Y, 0);
+    GLA_LAUNCH_CHECK();
+    GLA_TRY(tile_inverses<T>(s, n, Lx, n, Li));
+    GLA_TRY(trsm_rlt<T>(s, nrhs, n, Lx, n, Li, Y, nrhs));
+    hipLaunchKernelGGL((div_rows_k<T>), g2(nrhs, n), dim3(256), 0, s, n, nrhs, F, ldf, Y);
+    GLA_LAUNCH_CHECK();
+    // ldiv!(L', X): xᵀ = ((yᵀ J) Lr⁻ᵀ) J
+    hipLaunchKernelGGL((colrev_k<T>), g2(nrhs, n), dim3(256), 0, s, n, nrhs, Y, Z);
+    GLA_LAUNCH_CHECK();
+    hipLaunchKernelGGL((rev_lower_k<T>), g2(n, n), dim3(256), 0, s, n, Lx, n, Lr, n, 1);
+    GLA_LAUNCH_CHECK();
+    return reversed_tail<T>(s, n, nrhs, Lr, n, Lri, Z, B, ldb);
+}
+
+// B (n×nrhs) <- L D Lᵀ B — lmul!(F::LDLt, X), src/ldlt.jl:229-232 (and
+// F * X, :234): two MFMA products with the explicit unit factor.
+// Scratch: n² + n·nrhs elements.
+template <typename T>
+int ldlt_lmul_dev(hipStream_t s, char uplo, int64_t n, int64_t nrhs, const T *F, int64_t ldf,
+                  T *B, int64_t ldb, T *scratch)
+{
+    if (n <= 0 || nrhs <= 0) return GLA_OK;
+    T *Lx = scratch, *Y = Lx + (size_t)n * n;
+    hipLaunchKernelGGL((set_zero_k<T>), g2(n, n), dim3(256), 0, s, n, n, Lx, n);
+    GLA_LAUNCH_CHECK();
+    hipLaunchKernelGGL((unit_lower_k<T>), g2(n, n), dim3(256), 0, s, n, F, ldf, Lx,
+                       uplo == 'U' ? 1 : 0);
+    GLA_LAUNCH_CHECK();
+    // Y = Lᵀ B (lmul!(L', X)), then D Y (lmul!(D, X)), then B = L Y (lmul!(L, X))
+    GLA_TRY(gemm<T>(s, 'T', 'N', 0, n, nrhs, n, T(1), Lx, n, B, ldb, T(0), Y, n, 0));
+    hipLaunchKernelGGL((scale_rows_k<T>), g2(n, nrhs), dim3(256), 0, s, n, nrhs, F, ldf, Y, n);
+    GLA_LAUNCH_CHECK();
+    return gemm<T>(s, 'N', 'N', 0, n, nrhs, n, T(1), Lx, n, Y, n, T(0), B, ldb, 0);
 }
 
 // Scratch elements of the QR solve.
@@ -212,6 +300,29 @@ int chol_solve_entry(gla_handle_t hh, char uplo, int64_t n, int64_t nrhs, const 
                                   ldb, reinterpret_cast<T *>(h->ws));
 }
 
+// mode 0: ldiv!(F::LDLt, B); 1: lmul!(F::LDLt, B).
+template <typename T>
+int ldlt_apply_entry(gla_handle_t hh, int mode, char uplo, int64_t n, int64_t nrhs, const T *F,
+                     int64_t ldf, T *B, int64_t ldb, void *stream)
+{
+    Handle *h = reinterpret_cast<Handle *>(hh);
+    if (!h) { gla::set_error("null handle"); return -1; }
+    if (uplo != 'L' && uplo != 'U') { gla::set_error("uplo must be 'L' or 'U'"); return -2; }
+    if (n < 0) return -3;
+    if (nrhs < 0) return -4;
+    if (ldf < std::max<int64_t>(1, n)) { gla::set_error("ldf < max(1,n)"); return -6; }
+    if (ldb < std::max<int64_t>(1, n)) { gla::set_error("ldb < max(1,n)"); return -8; }
+    if (n == 0 || nrhs == 0) return GLA_OK;
+    GLA_HIP_TRY(hipSetDevice(h->device));
+    hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+    const size_t need = mode == 0 ? gla::chol_solve_scratch(n, nrhs)
+                                  : (size_t)n * n + (size_t)n * nrhs + 64;
+    GLA_TRY(gla::ensure_workspace(h, need * sizeof(T)));
+    T *ws = reinterpret_cast<T *>(h->ws);
+    if (mode == 0) return gla::ldlt_solve_dev<T>(s, uplo, n, nrhs, F, ldf, B, ldb, ws);
+    return gla::ldlt_lmul_dev<T>(s, uplo, n, nrhs, F, ldf, B, ldb, ws);
+}
+
 template <typename T>
 int qr_solve_entry(gla_handle_t hh, int64_t m, int64_t n, int64_t nrhs, const T *F, int64_t ldf,
                    const T *tau, T *B, int64_t ldb, int64_t *info, void *stream)
@@ -253,6 +364,26 @@ int gla_chol_solve_s(gla_handle_t h, char uplo, int64_t n, int64_t nrhs, const f
                      int64_t ldf, float *B, int64_t ldb, void *stream)
 {
     return chol_solve_entry<float>(h, uplo, n, nrhs, F, ldf, B, ldb, stream);
+}
+int gla_ldlt_solve_d(gla_handle_t h, char uplo, int64_t n, int64_t nrhs, const double *F,
+                     int64_t ldf, double *B, int64_t ldb, void *stream)
+{
+    return ldlt_apply_entry<double>(h, 0, uplo, n, nrhs, F, ldf, B, ldb, stream);
+}
+int gla_ldlt_solve_s(gla_handle_t h, char uplo, int64_t n, int64_t nrhs, const float *F,
+                     int64_t ldf, float *B, int64_t ldb, void *stream)
+{
+    return ldlt_apply_entry<float>(h, 0, uplo, n, nrhs, F, ldf, B, ldb, stream);
+}
+int gla_ldlt_lmul_d(gla_handle_t h, char uplo, int64_t n, int64_t nrhs, const double *F,
+                    int64_t ldf, double *B, int64_t ldb, void *stream)
+{
+    return ldlt_apply_entry<double>(h, 1, uplo, n, nrhs, F, ldf, B, ldb, stream);
+}
+int gla_ldlt_lmul_s(gla_handle_t h, char uplo, int64_t n, int64_t nrhs, const float *F,
+                    int64_t ldf, float *B, int64_t ldb, void *stream)
+{
+    return ldlt_apply_entry<float>(h, 1, uplo, n, nrhs, F, ldf, B, ldb, stream);
 }
 int gla_qr_solve_d(gla_handle_t h, int64_t m, int64_t n, int64_t nrhs, const double *F,
                    int64_t ldf, const double *tau, double *B, int64_t ldb, int64_t *info,

@@ -27,7 +27,7 @@ __all__ = [
     "LowerTriangular", "UpperTriangular", "UnitLowerTriangular", "Hermitian",
     "cholUnblocked_", "cholBlocked_", "cholRecursive_", "rankUpdate_", "rdiv_", "lmul_",
     "QR2", "R", "QBlocked", "HouseholderBlock", "Adjoint", "adjoint", "qrUnblocked_",
-    "qrBlocked_", "gemm_",
+    "qrBlocked_", "gemm_", "LDLt", "ldlt_", "ldlt",
 ]
 
 
@@ -709,6 +709,87 @@ def qr_solve_(F, tau, B=None):
     if info.value > 0:
         raise ArgumentError(f"R is singular: zero diagonal entry at {info.value}")
     return B[:n]
+
+
+# ------------------------------------------------------------------ LDLᵀ ----
+class LDLt:
+    """``LDLt(Hermitian(A, uplo))`` as ldlt! returns it (src/ldlt.jl:150-157):
+    ``data`` holds the factors in the ``uplo`` triangle.  Properties ``d``
+    (the diagonal of D, a view), ``L`` (unit lower as a dense tensor; Uᵀ for
+    uplo 'U'), and ``ldiv_``/``lmul_``/``@`` as src/ldlt.jl:225-234 define
+    them."""
+
+    def __init__(self, data, uplo):
+        self.data = data
+        self.uplo = uplo
+
+    @property
+    def d(self):
+        return torch.diagonal(self.data)
+
+    @property
+    def L(self):
+        n = self.data.shape[0]
+        E = torch.eye(n, dtype=self.data.dtype, device=self.data.device)
+        if self.uplo == "L":
+            return torch.tril(self.data, -1) + E
+        return torch.triu(self.data, 1).t() + E
+
+    def _apply(self, X, fn):
+        F = self.data
+        ldf = _mat(F, "F")
+        n = F.shape[0]
+        vec = X.dim() == 1
+        Xm = X.reshape(-1, 1) if vec else X
+        ldx = _mat(Xm, "X")
+        if Xm.shape[0] != n:
+            raise DimensionMismatch(f"X has {Xm.shape[0]} rows, the factorisation is {n}×{n}")
+        if Xm.dtype != F.dtype:
+            raise TypeError("F and X must have one element type")
+        with torch.cuda.device(F.device):
+            rc = getattr(lib(), f"{fn}_{_sfx(F)}")(
+                _handle(F.device.index), self.uplo.encode(), n, Xm.shape[1],
+                C.c_void_p(F.data_ptr()), ldf, C.c_void_p(Xm.data_ptr()), ldx, _stream(F))
+        _check(rc, fn)
+        return X
+
+    def ldiv_(self, X):
+        """ldiv!(F, X) — src/ldlt.jl:225-228: X <- A⁻¹ X in place."""
+        return self._apply(X, "gla_ldlt_solve")
+
+    def lmul_(self, X):
+        """lmul!(F, X) — src/ldlt.jl:229-232: X <- A X in place."""
+        return self._apply(X, "gla_ldlt_lmul")
+
+    def __matmul__(self, X):
+        """F * X — src/ldlt.jl:234 (a copy of X, then lmul!)."""
+        return self.lmul_(colmajor(X, device=X.device))
+
+
+def ldlt_(Ah, blocksize=None):
+    """ldlt!(A::Hermitian, blocksize = 128 ÷ sizeof(T)) — src/ldlt.jl:150-157.
+    Factors Ah.data in place (only its ``uplo`` triangle is read and
+    written); returns LDLt."""
+    if not isinstance(Ah, Hermitian):
+        raise TypeError("ldlt! takes a Hermitian")
+    A = Ah.data
+    lda = _mat(A)
+    n, m = A.shape
+    if n != m:
+        raise DimensionMismatch(f"matrix is not square: dimensions are {tuple(A.shape)}")
+    if blocksize is None:
+        blocksize = max(1, 128 // A.element_size())
+    with torch.cuda.device(A.device):
+        rc = getattr(lib(), f"gla_ldlt_{_sfx(A)}")(
+            _handle(A.device.index), Ah.uplo.encode(), n, C.c_void_p(A.data_ptr()), lda,
+            int(blocksize), _stream(A))
+    _check(rc, "gla_ldlt")
+    return LDLt(A, Ah.uplo)
+
+
+def ldlt(Ah, blocksize=None):
+    """ldlt(A::Hermitian) — src/ldlt.jl:202: ldlt! of a copy."""
+    return ldlt_(Hermitian(colmajor(Ah.data, device=Ah.data.device), Ah.uplo), blocksize)
 
 
 def _batch3(X, name):

@@ -80,6 +80,12 @@ const SIGNATURES = Dict{Symbol,Tuple{Any,Tuple}}(
     :gla_chol_solve_batched_s => (Cint, (Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Ptr{Float32}}, Ptr{Int32}, Ptr{Ptr{Float32}}, Ptr{Int32}, Ptr{Cvoid})),
     :gla_chol_solve_mixed_d => (Cint, (Ptr{Cvoid}, Cchar, Int64, Ptr{Float64}, Int64, Int64, Ptr{Float64}, Int64, Ptr{Float64}, Int64, Float64, Int32, Ref{Int32}, Ref{Float64}, Ref{Int64}, Ref{Cchar}, Ptr{Cvoid})),
     :gla_gemm_lowp_nt => (Cint, (Ptr{Cvoid}, Cchar, Int64, Int64, Int64, Float32, Ptr{Float32}, Int64, Ptr{Float32}, Int64, Ptr{Float32}, Int64, Ptr{Cvoid})),
+    :gla_ldlt_d => (Cint, (Ptr{Cvoid}, Cchar, Int64, Ptr{Float64}, Int64, Int64, Ptr{Cvoid})),
+    :gla_ldlt_s => (Cint, (Ptr{Cvoid}, Cchar, Int64, Ptr{Float32}, Int64, Int64, Ptr{Cvoid})),
+    :gla_ldlt_solve_d => (Cint, (Ptr{Cvoid}, Cchar, Int64, Int64, Ptr{Float64}, Int64, Ptr{Float64}, Int64, Ptr{Cvoid})),
+    :gla_ldlt_solve_s => (Cint, (Ptr{Cvoid}, Cchar, Int64, Int64, Ptr{Float32}, Int64, Ptr{Float32}, Int64, Ptr{Cvoid})),
+    :gla_ldlt_lmul_d => (Cint, (Ptr{Cvoid}, Cchar, Int64, Int64, Ptr{Float64}, Int64, Ptr{Float64}, Int64, Ptr{Cvoid})),
+    :gla_ldlt_lmul_s => (Cint, (Ptr{Cvoid}, Cchar, Int64, Int64, Ptr{Float32}, Int64, Ptr{Float32}, Int64, Ptr{Cvoid})),
     :gla_chol_solve_d => (Cint, (Ptr{Cvoid}, Cchar, Int64, Int64, Ptr{Float64}, Int64, Ptr{Float64}, Int64, Ptr{Cvoid})),
     :gla_chol_solve_s => (Cint, (Ptr{Cvoid}, Cchar, Int64, Int64, Ptr{Float32}, Int64, Ptr{Float32}, Int64, Ptr{Cvoid})),
     :gla_qr_solve_d => (Cint, (Ptr{Cvoid}, Int64, Int64, Int64, Ptr{Float64}, Int64, Ptr{Float64}, Ptr{Float64}, Int64, Ref{Int64}, Ptr{Cvoid})),
@@ -462,6 +468,41 @@ for (T, sfx) in ((Float64, "d"), (Float32, "s"))
             with_handle(h -> _check(@gla($chs, h, uplo, n, size(B, 2), pointer(A), ld(A),
                                          pointer(B), ld(B), C_NULL), "chol_solve!"))
             return B
+        end
+    end
+end
+
+# ldlt!(A::Hermitian, blocksize) and ldiv!/lmul! with its LDLt —
+# src/ldlt.jl:150-157,225-234, for a Hermitian view of a device matrix.
+for (T, sfx) in ((Float64, "d"), (Float32, "s"))
+    fac = Symbol("gla_ldlt_", sfx)
+    sol = Symbol("gla_ldlt_solve_", sfx)
+    mul = Symbol("gla_ldlt_lmul_", sfx)
+    @eval begin
+        function LinearAlgebra.ldlt!(A::Hermitian{$T,<:HIPMatrix{$T}},
+                                     blocksize::Int = max(1, 128 ÷ sizeof($T)))
+            D = A.data
+            n = _square(D)
+            blocksize >= 1 || throw(ArgumentError("blocksize must be positive"))
+            with_handle(h -> _check(@gla($fac, h, Cchar(A.uplo), n, pointer(D), ld(D),
+                                         Int64(blocksize), C_NULL), "ldlt!"))
+            return LDLt(A)
+        end
+        function LinearAlgebra.ldiv!(F::LDLt{$T,<:Hermitian{$T,<:HIPMatrix{$T}}}, X::HIPMatrix{$T})
+            D = F.data.data
+            n = _square(D)
+            size(X, 1) == n || throw(DimensionMismatch("X has $(size(X, 1)) rows, F is $n×$n"))
+            with_handle(h -> _check(@gla($sol, h, Cchar(F.data.uplo), n, size(X, 2), pointer(D),
+                                         ld(D), pointer(X), ld(X), C_NULL), "ldiv!"))
+            return X
+        end
+        function LinearAlgebra.lmul!(F::LDLt{$T,<:Hermitian{$T,<:HIPMatrix{$T}}}, X::HIPMatrix{$T})
+            D = F.data.data
+            n = _square(D)
+            size(X, 1) == n || throw(DimensionMismatch("X has $(size(X, 1)) rows, F is $n×$n"))
+            with_handle(h -> _check(@gla($mul, h, Cchar(F.data.uplo), n, size(X, 2), pointer(D),
+                                         ld(D), pointer(X), ld(X), C_NULL), "lmul!"))
+            return X
         end
     end
 end

@@ -61,7 +61,10 @@ const char *gla_version(void);
  * 0 runs it as recursive halving down to 2048-wide leaves.  Both compute
  * the same factor.  "chol_syrk_late": 1 (default) starts the look-ahead
  * driver's trailing SYRK on the side stream after the next block column's
- * update, 0 right after the panel solve (same factor).  "mixed_block": block width (multiple of 128,
+ * update, 0 right after the panel solve (same factor).  "ldlt_block": step width (multiple
+ * of 128 in [128, 4096], default 512) of gla_ldlt_{d,s}: 128 runs 128-wide
+ * steps, wider ones that width with depth-1 look-ahead (same factor up to
+ * rounding).  "mixed_block": block width (multiple of 128,
  * default 512) of the look-ahead driver of the BF16/FP16 factorisation in
  * gla_chol_solve_mixed_d.  "qr_panel": 1 or 2 (default), the form of the
  * Householder panel kernel.  "qr_inner": 32 or 64 factors each QR block
@@ -183,6 +186,34 @@ int gla_lmul_unit_lower_d(gla_handle_t h, int64_t m, int64_t n, const double *L,
                           int64_t ldl, double *B, int64_t ldb, double alpha, void *stream);
 int gla_lmul_unit_lower_s(gla_handle_t h, int64_t m, int64_t n, const float *L,
                           int64_t ldl, float *B, int64_t ldb, float alpha, void *stream);
+
+/* ----------------------------------------------------------------- LDLᵀ -- */
+/* ldlt!(Hermitian(A, uplo), blocksize) — src/ldlt.jl:150-157, the blocked
+ * factorisations without pivoting _ldlt_lower_blocked! (:80-104, A = L D Lᵀ:
+ * unit L below the diagonal, D on it) and _ldlt_upper_blocked! (:121-148,
+ * A = Uᵀ D U: unit U above the diagonal).  Only the `uplo` triangle is read
+ * and written; the other keeps its input.  `blocksize` (≥ 1; the
+ * reference's default is 128 ÷ sizeof(T)) does not change the factors, up
+ * to rounding: the device steps are "ldlt_block" wide (gla_set_tuning,
+ * default 512 with look-ahead; 128-wide diagonal blocks).  A zero pivot
+ * divides by zero and Inf/NaN propagate, as in the reference. */
+int gla_ldlt_d(gla_handle_t h, char uplo, int64_t n, double *A, int64_t lda, int64_t blocksize,
+               void *stream);
+int gla_ldlt_s(gla_handle_t h, char uplo, int64_t n, float *A, int64_t lda, int64_t blocksize,
+               void *stream);
+/* ldiv!(F::LDLt, X) — src/ldlt.jl:225-228: B (n×nrhs) <- L⁻ᵀ D⁻¹ L⁻¹ B
+ * (uplo 'L') or U⁻¹ D⁻¹ U⁻ᵀ B (uplo 'U') for factor data F from
+ * gla_ldlt_{d,s}. */
+int gla_ldlt_solve_d(gla_handle_t h, char uplo, int64_t n, int64_t nrhs, const double *F,
+                     int64_t ldf, double *B, int64_t ldb, void *stream);
+int gla_ldlt_solve_s(gla_handle_t h, char uplo, int64_t n, int64_t nrhs, const float *F,
+                     int64_t ldf, float *B, int64_t ldb, void *stream);
+/* lmul!(F::LDLt, X) — src/ldlt.jl:229-232 (and F * X, :234): B <- L D Lᵀ B
+ * (uplo 'L') or Uᵀ D U B (uplo 'U'). */
+int gla_ldlt_lmul_d(gla_handle_t h, char uplo, int64_t n, int64_t nrhs, const double *F,
+                    int64_t ldf, double *B, int64_t ldb, void *stream);
+int gla_ldlt_lmul_s(gla_handle_t h, char uplo, int64_t n, int64_t nrhs, const float *F,
+                    int64_t ldf, float *B, int64_t ldb, void *stream);
 
 /* ------------------------------------------- batched uneven Cholesky -- */
 /* No counterpart in the reference (one matrix per call there): factor and

@@ -58,6 +58,14 @@ def lib() -> C.CDLL:
             g("orc_wy_apply").argtypes = [C.c_int, _i64, _i64, _p, _i64, _i64, _p, _i64,
                                           _p, _i64, _p, _i64]
             g("orc_qr_blocked").argtypes = [_i64, _i64, _p, _i64, _i64, _p, _p, _i64]
+            g("orc_ldlt_lower").argtypes = [_i64, _p, _i64]
+            g("orc_ldlt_upper").argtypes = [_i64, _p, _i64]
+            g("orc_ldlt_lower_blocked").argtypes = [_i64, _p, _i64, _i64]
+            g("orc_ldlt_upper_blocked").argtypes = [_i64, _p, _i64, _i64]
+            g("orc_ldlt_solve").argtypes = [C.c_int, _i64, _i64, _p, _i64, _p, _i64]
+            g("orc_symtri_lower").argtypes = [_i64, _p, _i64, _p]
+            g("orc_symtri_upper").argtypes = [_i64, _p, _i64, _p]
+            g("orc_bidiag").argtypes = [_i64, _i64, _p, _i64, _p, _p]
         L.orq_chol_recursive.argtypes = [_i64, _p, _i64, _i64]
         L.orq_chol_recursive.restype = _i64
         L.orq_chol_unblocked.argtypes = [_i64, _p, _i64]
@@ -243,6 +251,144 @@ def lmul_unit_lower(L, B, alpha):
     _fn("orc_lmul_unit_lower", B)(B.shape[0], B.shape[1], _ptr(L), L.shape[0], _ptr(B),
                                   B.shape[0], alpha)
     return B
+
+
+# ------------------------------------------------------------------- LDLᵀ --
+def ldlt_default_blocksize(dtype) -> int:
+    """ldlt!'s default blocksize max(1, 128 ÷ sizeof(T)) (src/ldlt.jl:150)."""
+    return max(1, 128 // np.dtype(dtype).itemsize)
+
+
+def ldlt(A, uplo: str = "L", blocksize: int | None = None):
+    """ldlt!(Hermitian(copy(A), uplo), blocksize).data (src/ldlt.jl:150-157):
+    the blocked factorisation :80-148, only the `uplo` triangle read and
+    written.  blocksize ≥ n runs the unblocked form (:16-62)."""
+    A = _f(A)
+    n = A.shape[0]
+    bs = ldlt_default_blocksize(A.dtype) if blocksize is None else int(blocksize)
+    name = "orc_ldlt_lower_blocked" if uplo == "L" else "orc_ldlt_upper_blocked"
+    _fn(name, A)(n, _ptr(A), max(n, 1), bs)
+    return A
+
+
+def ldlt_unblocked(A, uplo: str = "L"):
+    """_ldlt_lower!(copy(A)) / _ldlt_upper!(copy(A)); src/ldlt.jl:16-62."""
+    A = _f(A)
+    n = A.shape[0]
+    _fn("orc_ldlt_lower" if uplo == "L" else "orc_ldlt_upper", A)(n, _ptr(A), max(n, 1))
+    return A
+
+
+def ldlt_solve(F, B, uplo: str = "L"):
+    """ldiv!(F::LDLt, copy(B)) for the factor data F; src/ldlt.jl:225-228."""
+    B = _f(B)
+    F = _f(F, B.dtype)
+    vec = B.ndim == 1
+    if vec:
+        B = B.reshape(-1, 1, order="F")
+    n = F.shape[0]
+    _fn("orc_ldlt_solve", B)(1 if uplo == "U" else 0, n, B.shape[1], _ptr(F), max(n, 1), _ptr(B),
+                             max(n, 1))
+    return B.reshape(-1) if vec else B
+
+
+def ldlt_factors(F, uplo: str = "L"):
+    """(L, d) of factor data F: L unit lower (L = U' for uplo 'U'), d the
+    diagonal of D (the getproperty of src/ldlt.jl:208-222)."""
+    F = np.asarray(F)
+    d = np.diag(F).copy()
+    L = np.tril(F, -1) if uplo == "L" else np.triu(F, 1).T
+    L = L + np.eye(F.shape[0], dtype=F.dtype)
+    return L, d
+
+
+# ---------------------------------------------------- two-sided reductions --
+def symtri(A, uplo: str = "L"):
+    """symtri!(Hermitian(copy(A), uplo)) -> (AS, τ) for real A: symtriLower!
+    / symtriUpper! (src/eigenSelfAdjoint.jl:450-564).  AS holds the
+    tridiagonal (diagonal and sub/superdiagonal) and the reflectors; τ has
+    n − 1 entries (the last 0)."""
+    A = _f(A)
+    n = A.shape[0]
+    tau = np.zeros(max(n - 1, 0), dtype=A.dtype)
+    name = "orc_symtri_lower" if uplo == "L" else "orc_symtri_upper"
+    _fn(name, A)(n, _ptr(A), max(n, 1), _ptr(tau) if n > 1 else None)
+    return A, tau
+
+
+def symtri_q(AS, tau, uplo: str = "L"):
+    """The orthogonal Q of symtri!'s reflectors (float64 numpy): Q = H₁H₂⋯
+    with Qᵀ A Q tridiagonal.  uplo 'L': H_k reflects rows k+1: with
+    v = [1, AS[k+2:, k]]; 'U': H_k reflects rows 1:n−k with v read upwards
+    from AS[n−k−1, n−k] (= 1) through AS[0, n−k]."""
+    AS = np.asarray(AS, dtype=np.float64)
+    n = AS.shape[0]
+    Q = np.eye(n)
+    for k in range(n - 2):
+        v = np.zeros(n)
+        if uplo == "L":
+            v[k + 1] = 1.0
+            v[k + 2:] = AS[k + 2:, k]
+        else:
+            m = n - k - 1
+            v[m - 1] = 1.0
+            v[:m - 1] = AS[:m - 1, m]
+        Q = Q - float(tau[k]) * np.outer(Q @ v, v)
+    return Q
+
+
+def symtri_diagonals(AS, uplo: str = "L"):
+    """(d, e): the tridiagonal's diagonal and off-diagonal."""
+    AS = np.asarray(AS)
+    return np.diag(AS).copy(), (np.diag(AS, -1) if uplo == "L" else np.diag(AS, 1)).copy()
+
+
+def bidiag(A):
+    """bidiagonalize!(copy(A)) -> (A_out, τl, τr) for real A (src/svd.jl:328-381)."""
+    A = _f(A)
+    m, n = A.shape
+    k = min(m, n)
+    nl, nr = (k, max(k - 1, 0)) if m >= n else (max(k - 1, 0), k)
+    taul = np.zeros(max(nl, 1), dtype=A.dtype)
+    taur = np.zeros(max(nr, 1), dtype=A.dtype)
+    _fn("orc_bidiag", A)(m, n, _ptr(A), max(m, 1), _ptr(taul), _ptr(taur))
+    return A, taul[:nl], taur[:nr]
+
+
+def bidiag_q(F, taul, taur):
+    """(Ql, Qr, B) of bidiagonalize!'s output (float64 numpy): Ql (m×m) and
+    Qr (n×n) the products of the left and right reflectors, B = Qlᵀ A Qr
+    the bidiagonal (upper for m ≥ n, lower otherwise)."""
+    F = np.asarray(F, dtype=np.float64)
+    m, n = F.shape
+    Ql, Qr = np.eye(m), np.eye(n)
+    if m >= n:
+        for i, t in enumerate(taul):
+            v = np.zeros(m)
+            v[i] = 1.0
+            v[i + 1:] = F[i + 1:, i]
+            Ql = Ql - float(t) * np.outer(Ql @ v, v)
+        for i, t in enumerate(taur):
+            v = np.zeros(n)
+            v[i + 1] = 1.0
+            v[i + 2:] = F[i, i + 2:]
+            Qr = Qr - float(t) * np.outer(Qr @ v, v)
+        B = np.diag(np.diag(F)) + np.diag(np.diag(F, 1), 1)
+        B = np.vstack([B, np.zeros((m - n, n))])
+    else:
+        for i, t in enumerate(taur):
+            v = np.zeros(n)
+            v[i] = 1.0
+            v[i + 1:] = F[i, i + 1:]
+            Qr = Qr - float(t) * np.outer(Qr @ v, v)
+        for i, t in enumerate(taul):
+            v = np.zeros(m)
+            v[i + 1] = 1.0
+            v[i + 2:] = F[i + 2:, i]
+            Ql = Ql - float(t) * np.outer(Ql @ v, v)
+        B = np.diag(np.diag(F)) + np.diag(np.diag(F, -1), -1)
+        B = np.hstack([B, np.zeros((m, n - m))])
+    return Ql, Qr, B
 
 
 # --------------------------------------------------------------- adjoints --

@@ -294,4 +294,262 @@ void OFN(orc_qr_blocked)(int64_t m, int64_t n, OT *A, int64_t lda, int64_t bs, O
     free(Tm);
 }
 
+/* ------------------------------------------------------------------ LDLᵀ -- */
+/* _ldlt_lower!(A) src/ldlt.jl:16-33: column 1 is divided by δ = A[1,1]
+ * (:19-22), the lower trailing block takes −l_i (l_j δ)' (:23-28), then the
+ * recursion on view(A, 2:n, 2:n) while n > 2 (:29-31), unrolled into k. */
+void OFN(orc_ldlt_lower)(int64_t n, OT *A, int64_t lda)
+{
+    for (int64_t k = 0; k + 1 < n; ++k) {
+        OT d = A[IX(k, k, lda)];
+        for (int64_t i = k + 1; i < n; ++i) A[IX(i, k, lda)] = A[IX(i, k, lda)] / d;   /* :19-22 */
+        for (int64_t j = k + 1; j < n; ++j) {                                             /* :23-28 */
+            OT ljd = A[IX(j, k, lda)] * d;
+            for (int64_t i = j; i < n; ++i) A[IX(i, j, lda)] -= A[IX(i, k, lda)] * ljd;
+        }
+    }
+}
+
+/* _ldlt_upper!(A) src/ldlt.jl:48-62: row 1 right of the diagonal takes
+ * A[1,j] / δ, each column j the update A[i,j] −= A[1,i]' (δ u_j) for
+ * i = 2:j with the undivided A[1,j] (:51-56); recursion while n > 2. */
+void OFN(orc_ldlt_upper)(int64_t n, OT *A, int64_t lda)
+{
+    for (int64_t k = 0; k + 1 < n; ++k) {
+        OT d = A[IX(k, k, lda)];
+        for (int64_t j = k + 1; j < n; ++j) {
+            OT dl = A[IX(k, j, lda)];
+            A[IX(k, j, lda)] = dl / d;
+            for (int64_t i = k + 1; i <= j; ++i) A[IX(i, j, lda)] -= A[IX(k, i, lda)] * dl;
+        }
+    }
+}
+
+/* _ldlt_lower_blocked!(A, blocksize) src/ldlt.jl:80-104 (tail recursion
+ * on the trailing block unrolled).  rdiv! by UnitLowerTriangular(A11)' is
+ * the row-wise substitution X[i,j] = B[i,j] − Σ_{l<j} X[i,l] A11[j,l]
+ * (:86), rdiv! by Diagonal(d₁) a division per column (:87). */
+void OFN(orc_ldlt_lower_blocked)(int64_t n, OT *A, int64_t lda, int64_t bs)
+{
+    OT *work = (OT *)malloc(sizeof(OT) * (size_t)(bs > 0 ? bs : 1));
+    int64_t k = 0;
+    while (bs < n - k) {                                        /* :82 */
+        OT *A11 = A + IX(k, k, lda), *A21 = A11 + bs, *A22 = A11 + IX(bs, bs, lda);
+        int64_t m = n - k - bs;
+        OFN(orc_ldlt_lower)(bs, A11, lda);                      /* :84 */
+        for (int64_t i = 0; i < m; ++i)                         /* :86 */
+            for (int64_t j = 0; j < bs; ++j) {
+                OT s = A21[IX(i, j, lda)];
+                for (int64_t l = 0; l < j; ++l) s -= A21[IX(i, l, lda)] * A11[IX(j, l, lda)];
+                A21[IX(i, j, lda)] = s;
+            }
+        for (int64_t j = 0; j < bs; ++j)                        /* :87 */
+            for (int64_t i = 0; i < m; ++i) A21[IX(i, j, lda)] = A21[IX(i, j, lda)] / A11[IX(j, j, lda)];
+        for (int64_t j = 0; j < m; ++j) {                       /* :88-95 */
+            for (int64_t l = 0; l < bs; ++l) work[l] = A11[IX(l, l, lda)] * A21[IX(j, l, lda)];
+            for (int64_t i = j; i < m; ++i)
+                for (int64_t l = 0; l < bs; ++l) A22[IX(i, j, lda)] -= A21[IX(i, l, lda)] * work[l];
+        }
+        k += bs;                                                /* :96 */
+    }
+    OFN(orc_ldlt_lower)(n - k, A + IX(k, k, lda), lda);         /* :101 */
+    free(work);
+}
+
+/* _ldlt_upper_blocked!(A, blocksize) src/ldlt.jl:121-148: U₁₂ <- D₁⁻¹
+ * (U₁₁')⁻¹ A₁₂ by column-wise forward substitution with the unit lower
+ * U₁₁' (:126) and a division per row (:127), then the upper trailing
+ * update A22[i,j] −= Σ_l U₁₂[l,i]' (d_l U₁₂[l,j]) for i ≤ j (:128-135). */
+void OFN(orc_ldlt_upper_blocked)(int64_t n, OT *A, int64_t lda, int64_t bs)
+{
+    OT *work = (OT *)malloc(sizeof(OT) * (size_t)(bs > 0 ? bs : 1));
+    int64_t k = 0;
+    while (bs < n - k) {                                        /* :123 */
+        OT *A11 = A + IX(k, k, lda), *U12 = A11 + IX(0, bs, lda), *A22 = A11 + IX(bs, bs, lda);
+        int64_t m = n - k - bs;
+        OFN(orc_ldlt_upper)(bs, A11, lda);                      /* :125 */
+        for (int64_t j = 0; j < m; ++j)                         /* :126 */
+            for (int64_t i = 0; i < bs; ++i) {
+                OT s = U12[IX(i, j, lda)];
+                for (int64_t l = 0; l < i; ++l) s -= A11[IX(l, i, lda)] * U12[IX(l, j, lda)];
+                U12[IX(i, j, lda)] = s;
+            }
+        for (int64_t j = 0; j < m; ++j)                         /* :127 */
+            for (int64_t i = 0; i < bs; ++i) U12[IX(i, j, lda)] = U12[IX(i, j, lda)] / A11[IX(i, i, lda)];
+        for (int64_t j = 0; j < m; ++j) {                       /* :128-135 */
+            for (int64_t l = 0; l < bs; ++l) work[l] = A11[IX(l, l, lda)] * U12[IX(l, j, lda)];
+            for (int64_t i = 0; i <= j; ++i)
+                for (int64_t l = 0; l < bs; ++l) A22[IX(i, j, lda)] -= U12[IX(l, i, lda)] * work[l];
+        }
+        k += bs;
+    }
+    OFN(orc_ldlt_upper)(n - k, A + IX(k, k, lda), lda);         /* :142 */
+    free(work);
+}
+
+/* ldiv!(F::LDLt, X) src/ldlt.jl:225-228: X <- L⁻ᴴ D⁻¹ L⁻¹ X with L unit
+ * lower (uplo 'L') or L = U' (uplo 'U'); the substitutions are the stdlib's
+ * column-oriented ones, D a division per row. */
+void OFN(orc_ldlt_solve)(int upper, int64_t n, int64_t nrhs, const OT *F, int64_t ldf, OT *B,
+                         int64_t ldb)
+{
+    for (int64_t c = 0; c < nrhs; ++c) {
+        OT *x = B + IX(0, c, ldb);
+        for (int64_t j = 0; j < n; ++j)                 /* ldiv!(L, X): L[i,j] = F[i,j] or F[j,i] */
+            for (int64_t i = j + 1; i < n; ++i)
+                x[i] -= (upper ? F[IX(j, i, ldf)] : F[IX(i, j, ldf)]) * x[j];
+        for (int64_t i = 0; i < n; ++i) x[i] = x[i] / F[IX(i, i, ldf)];   /* ldiv!(D, X) */
+        for (int64_t j = n - 1; j >= 0; --j)            /* ldiv!(L', X) */
+            for (int64_t i = 0; i < j; ++i)
+                x[i] -= (upper ? F[IX(i, j, ldf)] : F[IX(j, i, ldf)]) * x[j];
+    }
+}
+
+/* ---------------------------------------------------- two-sided reductions -- */
+/* LinearAlgebra.reflector! on the strided vector x[0], x[inc], …,
+ * x[(n−1)·inc] (inc < 0 walks a column upwards, as view(A, m:-1:1, j)
+ * does); the arithmetic of orc_reflector. */
+OT OFN(orc_reflector_inc)(int64_t n, OT *x, int64_t inc)
+{
+    if (n == 0) return (OT)0;
+    OT xi1 = x[0];
+    OT s = 0;
+    for (int64_t i = 0; i < n; ++i) s += x[i * inc] * x[i * inc];
+    OT normu = OSQRT(s);
+    if (normu == (OT)0) return (OT)0;
+    OT nu = OCOPYSIGN(normu, xi1);
+    xi1 += nu;
+    x[0] = -nu;
+    for (int64_t i = 1; i < n; ++i) x[i * inc] /= xi1;
+    return xi1 / nu;
+}
+
+/* LinearAlgebra.reflectorApply!(A, x, τ) (Julia stdlib, application from
+ * the right, real case, called at src/svd.jl:344,363): for each row i,
+ * s = τ (A[i,1] + Σ_{j≥2} A[i,j] x_j); A[i,1] −= s; A[i,j] −= s x_j.  x is
+ * strided (incx). */
+void OFN(orc_reflector_apply_right)(int64_t m, int64_t n, const OT *x, int64_t incx, OT tau,
+                                    OT *A, int64_t lda)
+{
+    if (n == 0) return;
+    for (int64_t i = 0; i < m; ++i) {
+        OT s = A[IX(i, 0, lda)];
+        for (int64_t j = 1; j < n; ++j) s += A[IX(i, j, lda)] * x[j * incx];
+        s = s * tau;
+        A[IX(i, 0, lda)] -= s;
+        for (int64_t j = 1; j < n; ++j) A[IX(i, j, lda)] -= s * x[j * incx];
+    }
+}
+
+/* symtriLower!(AS, τ, u) src/eigenSelfAdjoint.jl:450-504, real AS (n−2
+ * reflectors; τ has n−1 entries, the last left 0).  Per column k: the
+ * reflector of AS[k+1:n, k] (:466), ũ = τ·Hermitian(Ã,:L)·x (:477, the
+ * symmetric product from the lower triangle), ξ = τ xᵀũ (:480), and the
+ * lower triangle of Ã += x ξ xᵀ − x ũᵀ − ũ xᵀ (:484-491). */
+void OFN(orc_symtri_lower)(int64_t n, OT *A, int64_t lda, OT *tau)
+{
+    OT *u = (OT *)malloc(sizeof(OT) * (size_t)(n > 0 ? n : 1));
+    for (int64_t k = 0; k + 1 < n; ++k) tau[k] = 0;
+    for (int64_t k = 0; k + 2 < n; ++k) {                        /* :462 */
+        const int64_t m = n - k - 1;
+        OT *x = A + IX(k + 1, k, lda);
+        OT *At = A + IX(k + 1, k + 1, lda);
+        OT tk = OFN(orc_reflector)(m, x);                        /* :466 */
+        tau[k] = tk;                                             /* :467 */
+        OT tmp = x[0];                                           /* :470-471 */
+        x[0] = 1;
+        for (int64_t i = 0; i < m; ++i) {                        /* :477 */
+            OT sacc = 0;
+            for (int64_t j = 0; j < m; ++j)
+                sacc += (i >= j ? At[IX(i, j, lda)] : At[IX(j, i, lda)]) * x[j];
+            u[i] = tk * sacc;
+        }
+        OT xi = 0;                                               /* :480 */
+        for (int64_t i = 0; i < m; ++i) xi += x[i] * u[i];
+        xi = tk * xi;
+        for (int64_t j = 0; j < m; ++j) {                        /* :484-491 */
+            OT xj = x[j], uj = u[j], xixj = xi * xj;
+            for (int64_t i = j; i < m; ++i)
+                At[IX(i, j, lda)] += x[i] * xixj - x[i] * uj - u[i] * xj;
+        }
+        x[0] = tmp;                                              /* :494 */
+    }
+    free(u);
+}
+
+/* symtriUpper!(AS, τ, u) src/eigenSelfAdjoint.jl:507-564, real AS: step k
+ * reflects column n−k+1 above the diagonal walked upwards (:523-527, the
+ * reflector's first element is the superdiagonal entry), with the
+ * symmetric product from the upper triangle (:537) and the update of the
+ * upper triangle of Ã = AS[1:n−k, 1:n−k] (:544-551). */
+void OFN(orc_symtri_upper)(int64_t n, OT *A, int64_t lda, OT *tau)
+{
+    OT *u = (OT *)malloc(sizeof(OT) * (size_t)(n > 0 ? n : 1));
+    for (int64_t k = 0; k + 1 < n; ++k) tau[k] = 0;
+    for (int64_t k = 0; k + 2 < n; ++k) {
+        const int64_t m = n - k - 1;                             /* n − k (1-based k) */
+        OT *x = A + IX(0, m, lda);                               /* :523 */
+        OT tk = OFN(orc_reflector_inc)(m, x + (m - 1), -1);      /* :524-527 */
+        tau[k] = tk;
+        OT tmp = x[m - 1];                                       /* :530-531 */
+        x[m - 1] = 1;
+        for (int64_t i = 0; i < m; ++i) {                        /* :537 */
+            OT sacc = 0;
+            for (int64_t j = 0; j < m; ++j)
+                sacc += (i <= j ? A[IX(i, j, lda)] : A[IX(j, i, lda)]) * x[j];
+            u[i] = tk * sacc;
+        }
+        OT xi = 0;                                               /* :540 */
+        for (int64_t i = 0; i < m; ++i) xi += x[i] * u[i];
+        xi = tk * xi;
+        for (int64_t j = 0; j < m; ++j) {                        /* :544-551 */
+            OT xj = x[j], uj = u[j], xixj = xi * xj;
+            for (int64_t i = 0; i <= j; ++i)
+                A[IX(i, j, lda)] += x[i] * xixj - x[i] * uj - u[i] * xj;
+        }
+        x[m - 1] = tmp;                                          /* :554 */
+    }
+    free(u);
+}
+
+/* bidiagonalize!(A) src/svd.jl:328-381, real A.  m ≥ n (upper bidiagonal,
+ * :332-349): for each column i the left reflector of A[i:m, i] applied to
+ * A[i:m, i+1:n], then (i < n) the right reflector of row i, A[i, i+1:n],
+ * applied to A[i+1:m, i+1:n]; taul has n entries, taur n−1.  m < n (lower
+ * bidiagonal, :356-375): the right reflector of A[i, i:n] first, then the
+ * left one of A[i+1:m, i]; taur has m entries, taul m−1. */
+void OFN(orc_bidiag)(int64_t m, int64_t n, OT *A, int64_t lda, OT *taul, OT *taur)
+{
+    if (m >= n) {
+        for (int64_t i = 0; i < n; ++i) {
+            OT *x = A + IX(i, i, lda);
+            OT t = OFN(orc_reflector)(m - i, x);                                 /* :335-336 */
+            taul[i] = t;                                                         /* :337 */
+            OFN(orc_reflector_apply)(m - i, n - i - 1, x, t, A + IX(i, i + 1, lda), lda); /* :338 */
+            if (i + 1 < n) {                                                     /* :339 */
+                OT *y = A + IX(i, i + 1, lda);
+                t = OFN(orc_reflector_inc)(n - i - 1, y, lda);                   /* :340-342 */
+                taur[i] = t;                                                     /* :343 */
+                OFN(orc_reflector_apply_right)(m - i - 1, n - i - 1, y, lda, t,
+                                               A + IX(i + 1, i + 1, lda), lda);  /* :344 */
+            }
+        }
+    } else {
+        for (int64_t i = 0; i < m; ++i) {
+            OT *y = A + IX(i, i, lda);
+            OT t = OFN(orc_reflector_inc)(n - i, y, lda);                        /* :359-361 */
+            taur[i] = t;                                                         /* :362 */
+            OFN(orc_reflector_apply_right)(m - i - 1, n - i, y, lda, t, A + IX(i + 1, i, lda),
+                                           lda);                                 /* :363 */
+            if (i + 1 < m) {                                                     /* :364 */
+                OT *x = A + IX(i + 1, i, lda);
+                t = OFN(orc_reflector)(m - i - 1, x);                            /* :365-366 */
+                taul[i] = t;                                                     /* :367 */
+                OFN(orc_reflector_apply)(m - i - 1, n - i - 1, x, t, A + IX(i + 1, i + 1, lda),
+                                         lda);                                   /* :368 */
+            }
+        }
+    }
+}
+
 #undef IX
