@@ -34,6 +34,7 @@
 //              recursive (two half solves and a GEMM); its leaves are
 //              chol_solve_kernel launches.
 #include <algorithm>
+#include <mutex>
 
 #include "gla_internal.hpp"
 
@@ -782,20 +783,23 @@ int chol_la_dev(Handle *h, hipStream_t s, int64_t n, T *A, int64_t lda, int NB,
 
 // cholRecursive! (src/cholesky.jl:37-55): halve until n <= leaf, then the
 // blocked driver.
+// use_side = false keeps every launch on the caller's stream (the leaves'
+// look-ahead then runs in order: the same arithmetic, no side stream).
 template <typename T>
 int chol_recursive_dev(Handle *h, hipStream_t s, int64_t n, T *A, int64_t lda, int64_t leaf,
-                       int64_t info_base, T *Linv)
+                       int64_t info_base, T *Linv, bool use_side = true)
 {
-    if (n <= leaf) return chol_blocked_dev<T>(h, s, n, A, lda, kPanelNB, info_base, Linv);
+    if (n <= leaf)
+        return chol_blocked_dev<T>(h, s, n, A, lda, kPanelNB, info_base, Linv, use_side);
     int64_t n2 = n / 2;                                              // :44
     if (n2 > kPanelNB) n2 = n2 / kPanelNB * kPanelNB;                // keep blocks aligned
-    GLA_TRY(chol_recursive_dev<T>(h, s, n2, A, lda, leaf, info_base, Linv));   // :46
+    GLA_TRY(chol_recursive_dev<T>(h, s, n2, A, lda, leaf, info_base, Linv, use_side));   // :46
     GLA_TRY(launch_tile_inverse<T>(s, n2, A, lda, Linv));            // :48 rdiv!
     GLA_TRY(trsm_rlt_dev<T>(s, n - n2, n2, A, lda, Linv, A + n2, lda));
     GLA_TRY(gemm<T>(s, 'N', 'T', 1, n - n2, n - n2, n2, T(-1), A + n2, lda, A + n2, lda, T(1),
                     A + n2 + n2 * lda, lda, 0));                     // :51 rankUpdate!
     return chol_recursive_dev<T>(h, s, n - n2, A + n2 + n2 * lda, lda, leaf,
-                                 info_base + n2, Linv);              // :52
+                                 info_base + n2, Linv, use_side);    // :52
 }
 
 template <typename T>
@@ -1031,6 +1035,85 @@ int trsm_entry(gla_handle_t hh, int64_t m, int64_t n, const T *L, int64_t ldl, T
 }
 
 }  // namespace
+
+// ------------------------------------------------ no-allocation entry point --
+// cholRecursive! for callers that cannot allocate device memory (code under
+// HIP graph capture, serving loops): no handle, no hipMalloc.  Its scratch —
+// the 16×16 diagonal-tile inverses of the panel solves and the info word —
+// is this file-scope array, one copy per GPU, held until the process ends.
+// Host code never passes `scratch` itself: its device address on the
+// current GPU comes from hipGetSymbolAddress, taken once per device.
+constexpr int64_t kNoAllocMaxN = 32768;
+constexpr size_t kNoAllocLinvBytes = (size_t)(kNoAllocMaxN / 16 + 8) * 256 * sizeof(double);
+constexpr size_t kNoAllocBytes = kNoAllocLinvBytes + 256;   // + the info word
+static __device__ __align__(256) unsigned char scratch[kNoAllocBytes];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void *g_scratch_ptr[kMaxDevices] = {};
+
+// Device address of `scratch` on the current GPU.
+int scratch_address(void **out)
+{
+    int dev = 0;
+    GLA_HIP_TRY(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kMaxDevices) {
+        set_error("device ordinal %d beyond the %d this library tracks", dev, kMaxDevices);
+        return GLA_ERR_UNSUPPORTED;
+    }
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void *p = nullptr;
+        GLA_HIP_TRY(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        g_scratch_ptr[dev] = p;
+    }
+    *out = g_scratch_ptr[dev];
+    return GLA_OK;
+}
+
+int chol_recursive_noalloc(char uplo, int64_t n, double *A, int64_t lda, int64_t cutoff,
+                           int64_t *info, void *stream)
+{
+    if (uplo != 'L' && uplo != 'U' && uplo != 'l' && uplo != 'u') {
+        set_error("uplo must be 'L' or 'U'");
+        return -1;
+    }
+    if (n < 0) { set_error("n < 0"); return -2; }
+    if (n > kNoAllocMaxN) {
+        set_error("n = %lld exceeds the no-allocation entry point's maximum %lld (its fixed "
+                  "scratch holds the tile inverses of n <= %lld); use gla_chol_recursive_d",
+                  (long long)n, (long long)kNoAllocMaxN, (long long)kNoAllocMaxN);
+        return GLA_ERR_UNSUPPORTED;
+    }
+    if (!A && n > 0) { set_error("A is null"); return -3; }
+    if (lda < std::max<int64_t>(1, n)) { set_error("lda < max(1,n)"); return -4; }
+    void *base = nullptr;
+    GLA_TRY(scratch_address(&base));
+    double *Linv = reinterpret_cast<double *>(base);
+    int *dinfo = reinterpret_cast<int *>(static_cast<unsigned char *>(base) + kNoAllocLinvBytes);
+    hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+    hipLaunchKernelGGL(k_set_int, dim3(1), dim3(1), 0, s, dinfo, 0);   // a kernel, not a memset
+    GLA_LAUNCH_CHECK();
+    // A handle-shaped view of the scratch: the drivers read only its info
+    // word when use_side is false.
+    Handle view;
+    view.dinfo = dinfo;
+    const bool upper = (uplo == 'U' || uplo == 'u');
+    if (upper) GLA_TRY(swap_triangles<double>(s, n, A, lda));
+    if (n > 0)
+        GLA_TRY(chol_recursive_dev<double>(&view, s, n, A, lda, std::max<int64_t>(cutoff, 2048), 0,
+                                           Linv, false));
+    if (upper) GLA_TRY(swap_triangles<double>(s, n, A, lda));
+    if (info) {
+        int hv = 0;
+        GLA_HIP_TRY(hipMemcpyAsync(&hv, dinfo, sizeof(int), hipMemcpyDeviceToHost, s));
+        GLA_HIP_TRY(hipStreamSynchronize(s));
+        *info = hv;
+    }
+    return GLA_OK;
+}
+}  // namespace
 }  // namespace gla
 
 using gla::chol_entry;
@@ -1066,6 +1149,11 @@ int gla_chol_recursive_s(gla_handle_t h, char uplo, int64_t n, float *A, int64_t
                          int64_t cutoff, int64_t *info, void *stream)
 {
     return chol_entry<float>(h, uplo, n, A, lda, 2, cutoff, info, stream);
+}
+int gla_chol_recursive_noalloc_d(char uplo, int64_t n, double *A, int64_t lda, int64_t cutoff,
+                                 int64_t *info, void *stream)
+{
+    return gla::chol_recursive_noalloc(uplo, n, A, lda, cutoff, info, stream);
 }
 int gla_rank_update_d(gla_handle_t h, char uplo, int64_t n, int64_t k, double alpha,
                       const double *A, int64_t lda, double *C, int64_t ldc, void *stream)

@@ -94,6 +94,8 @@ def lib() -> C.CDLL:
     L.gla_gemm_lowp_nt.argtypes = [_p, _ch, _i64, _i64, _i64, C.c_float, _p, _i64, _p, _i64, _p,
                                    _i64, _p]
     L.gla_gemm_lowp_nt.restype = _int
+    L.gla_chol_recursive_noalloc_d.argtypes = [_ch, _i64, _p, _i64, _i64, _pi64, _p]
+    L.gla_chol_recursive_noalloc_d.restype = _int
     L.gla_set_tuning.argtypes = [_p, C.c_char_p, _i64]
     L.gla_set_tuning.restype = _int
     L.gla_last_error_string.restype = C.c_char_p

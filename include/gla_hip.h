@@ -114,6 +114,21 @@ int gla_chol_recursive_d(gla_handle_t h, char uplo, int64_t n, double *A, int64_
 int gla_chol_recursive_s(gla_handle_t h, char uplo, int64_t n, float *A, int64_t lda,
                          int64_t cutoff, int64_t *info, void *stream);
 
+/* No counterpart in the reference: cholRecursive!(A, Val{:L}, cutoff) in
+ * Float64 for callers that cannot allocate device memory (code under HIP
+ * graph capture, serving loops, harnesses that forbid hipMalloc).  No
+ * handle: the call allocates nothing and runs every launch on `stream`.
+ * Its scratch (panel tile inverses and the info word) is one fixed device
+ * array per GPU (4.2 MB, held until the process ends), so n is limited to
+ * 32768 (larger n: GLA_ERR_UNSUPPORTED, nothing launched).  Calls that
+ * overlap on one GPU (other streams, other host threads) share that array
+ * and race; serialise them.  Same factor as gla_chol_recursive_d with
+ * "chol_block" 0 (bit for bit) and to rounding with its default driver.
+ * `info` (host, may be NULL) as for gla_chol_recursive_d; with it the call
+ * synchronises the stream (leave it NULL under graph capture). */
+int gla_chol_recursive_noalloc_d(char uplo, int64_t n, double *A, int64_t lda, int64_t cutoff,
+                                 int64_t *info, void *stream);
+
 /* rankUpdate!(C::Hermitian, A::StridedVecOrMat, α::Real) —
  * src/juliaBLAS.jl:89-112 (and the rank-1 form :53-63 with k = 1).
  * C n×n, only the `uplo` triangle is updated: C += α A Aᵀ, A n×k. */

@@ -73,6 +73,18 @@ def test_null_handle_is_an_argument_error():
     assert rc == -1
 
 
+def test_noalloc_entry_rejects_arguments_before_touching_a_device():
+    """gla_chol_recursive_noalloc_d checks its arguments first: a size past
+    its fixed scratch and a bad uplo return at once (no GPU here)."""
+    L = _lib()
+    lib = L.lib()
+    rc = lib.gla_chol_recursive_noalloc_d(b"L", 32769, None, 32769, 1, None, None)
+    assert rc == L.GLA_ERR_UNSUPPORTED
+    assert b"32768" in lib.gla_last_error_string()
+    assert lib.gla_chol_recursive_noalloc_d(b"X", 4, None, 4, 1, None, None) == -1
+    assert lib.gla_chol_recursive_noalloc_d(b"L", -1, None, 4, 1, None, None) == -2
+
+
 def test_host_api_fails_loudly_without_library(monkeypatch, tmp_path):
     from gla import _lib as L
 

@@ -100,7 +100,7 @@ def test_binding_covers_every_entry_point():
     """Every header entry point is bound, except the ones that exist for
     Python tooling only (listed here with the reason)."""
     not_bound = {
-        "gla_chol_factor_noalloc_d",   # for callers that cannot allocate; C/graph users
+        "gla_chol_recursive_noalloc_d",   # for callers that cannot allocate; C/graph users
     }
     missing = sorted(set(header_decls()) - set(julia_signatures()) - not_bound)
     assert not missing, missing
