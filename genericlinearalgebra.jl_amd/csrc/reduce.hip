@@ -1,0 +1,897 @@
+// Two-sided Householder reductions (FP64/FP32, real):
+//   symtriLower!/symtriUpper!  src/eigenSelfAdjoint.jl:450-564  (A = Q T Qᵀ)
+//   bidiagonalize!             src/svd.jl:328-381               (A = Ql B Qrᵀ)
+//
+// Both produce the reference's reflectors (Julia stdlib reflector!: β =
+// −copysign(‖x‖, x₁), v₁ = 1, v = x/(x₁ − β), τ = (x₁ − β)/(−β)) in the
+// reference's packing; the per-column rank-2 (or left/right) updates the
+// reference applies at once are deferred across a panel of nb columns and
+// applied as MFMA GEMMs at the panel's end — the same matrices up to
+// rounding.
+//
+// symtri (lower; upper runs as lower on J A J, J the reversal, which maps
+// symtriUpper!'s bottom-up sweep onto symtriLower!'s).  For column c of a
+// panel starting at k (j = c − k), with the trailing matrix Ã kept as it
+// was at the panel's start and the panel's earlier reflectors in V, W
+// (Ã_now = Ã − V Wᵀ − W Vᵀ):
+//   tri_col:    a = A[c:n, c] − V W[c,:]ᵀ − W V[c,:]ᵀ (the column brought up
+//               to date; a[0] is the diagonal d_c); partial ‖a[1:]‖²;
+//   tri_symv:   the reflector's scalars from those partials, then
+//               p = Ã[c+1:n, c+1:n] v (the symmetric product from the lower
+//               triangle, each 64×64 block read once and used for both
+//               A_IJ v_J and A_IJᵀ v_I, partials per block in a fixed
+//               order) and Wᵀv, Vᵀv partials;
+//   tri_w:      v written into A and V, u = τ (p − V (Wᵀv) − W (Vᵀv)) into
+//               W, partial vᵀu;
+//   (next tri_col, or tri_wfin at the panel's end): w = u − (τ vᵀu / 2) v,
+//               so that Ã_new = Ã_now − v wᵀ − w vᵀ = H Ã_now H.
+// Panel end: Ã[k+nb:, k+nb:] −= V Wᵀ + W Vᵀ (two lower-triangle GEMMs).
+#include <algorithm>
+
+#include "gla_internal.hpp"
+
+namespace gla {
+
+namespace {
+
+#define GLA_TRY(expr)                  \
+    do {                               \
+        int _rc = (expr);              \
+        if (_rc != GLA_OK) return _rc; \
+    } while (0)
+
+constexpr int kTB = 64;       // symmetric-product block
+constexpr int kPanel = 32;    // panel width (columns of V, W)
+
+// Scalars of the current column, written by tri_symv's first workgroup.
+enum { S_TAU = 0, S_XI = 1, S_BETA = 2, S_UNEXT = 3, S_COUNT = 8 };
+
+template <typename T>
+__device__ __forceinline__ T wave_sum(T v)
+{
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o);
+    return v;
+}
+
+// Fixed-order sum of part[0..cnt) by one workgroup (every thread gets it).
+template <typename T>
+__device__ T block_sum_fixed(const T *part, int cnt, T *sh)
+{
+    T s = T(0);
+    for (int i = threadIdx.x; i < cnt; i += blockDim.x) s += part[i];
+    s = wave_sum(s);
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = s;
+    __syncthreads();
+    T r = T(0);
+    for (int w = 0; w < (int)(blockDim.x >> 6); ++w) r += sh[w];
+    __syncthreads();
+    return r;
+}
+
+// The reflector! scalars of column c from ‖a[1:]‖² (partials) and a[1]:
+// ξ₁ = a₁ + ν (the divisor of the tail), τ = ξ₁/ν, β = −ν; a zero norm
+// leaves the column alone (τ = 0, ξ₁ = 1).
+template <typename T>
+__device__ __forceinline__ void reflector_scalars(T nrm2, T x1, T &xi, T &tau, T &beta)
+{
+    const T normu = sqrt(nrm2);
+    if (normu == T(0)) {
+        xi = T(1);
+        tau = T(0);
+        beta = x1;
+        return;
+    }
+    const T nu = copysign(normu, x1);
+    xi = x1 + nu;
+    tau = xi / nu;
+    beta = -nu;
+}
+
+// Column c brought up to date with the panel's reflectors 0..j−1; before
+// that, W[:, j−1] (holding u) becomes w = u − (ξ/2) v for rows ≥ c.
+template <typename T>
+__global__ __launch_bounds__(256) void tri_col_kernel(int64_t n, int64_t c, int j, T *A,
+                                                      int64_t lda, T *V, T *W, int64_t ldv,
+                                                      const T *qpart, int qcnt, const T *scal,
+                                                      T *npart)
+{
+    __shared__ T sh[8];
+    __shared__ T wc[kPanel], vc[kPanel];
+    const int64_t i = c + (int64_t)blockIdx.x * 256 + threadIdx.x;
+    T wprev_c = T(0), half_xi = T(0);
+    if (j > 0) {
+        const T vu = block_sum_fixed(qpart, qcnt, sh);
+        half_xi = T(0.5) * scal[S_TAU] * vu;              // ξ/2 = τ vᵀu / 2
+        wprev_c = scal[S_UNEXT] - half_xi;                // row c: v_c = 1
+    }
+    if (threadIdx.x < kPanel) {
+        const int l = threadIdx.x;
+        if (l < j) {
+            wc[l] = l == j - 1 ? wprev_c : W[c + (int64_t)l * ldv];
+            vc[l] = V[c + (int64_t)l * ldv];
+        }
+    }
+    __syncthreads();
+    T a = T(0);
+    if (i < n) {
+        a = A[i + c * lda];
+        if (j > 0) {
+            T *wi = W + i + (int64_t)(j - 1) * ldv;
+            const T vi = V[i + (int64_t)(j - 1) * ldv];
+            const T w = *wi - half_xi * vi;
+            *wi = w;
+            for (int l = 0; l < j; ++l) {
+                const T wil = l == j - 1 ? w : W[i + (int64_t)l * ldv];
+                a -= V[i + (int64_t)l * ldv] * wc[l] + wil * vc[l];
+            }
+        }
+        A[i + c * lda] = a;
+    }
+    T s = (i < n && i > c) ? a * a : T(0);
+    s = wave_sum(s);
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) npart[blockIdx.x] = sh[0] + sh[1] + sh[2] + sh[3];
+}
+
+// After the panel's last column: W[:, j] (u) becomes w for rows ≥ c.
+template <typename T>
+__global__ void tri_wfin_kernel(int64_t n, int64_t c, int j, T *V, T *W, int64_t ldv,
+                                const T *qpart, int qcnt, const T *scal)
+{
+    __shared__ T sh[8];
+    const T vu = block_sum_fixed(qpart, qcnt, sh);
+    const T half_xi = T(0.5) * scal[S_TAU] * vu;
+    const int64_t i = c + (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i < n) W[i + (int64_t)j * ldv] -= half_xi * V[i + (int64_t)j * ldv];
+}
+
+// v_r of row r ≥ r0 = c + 1 from the unscaled column.
+template <typename T>
+__device__ __forceinline__ T vval(const T *A, int64_t lda, int64_t c, int64_t r, T xi)
+{
+    return r == c + 1 ? T(1) : A[r + c * lda] / xi;
+}
+
+// p = Ã v over the lower-stored trailing block, blocks (I, J), I ≥ J, of
+// 64×64; P[s·m + r] is the contribution of the block pairing row block
+// r/64 with block s (rows relative to r0).  Diagonal blocks add Wᵀv and
+// Vᵀv partials Y1[I·kPanel + l], Y2[I·kPanel + l].
+template <typename T>
+__global__ __launch_bounds__(256) void tri_symv_kernel(int64_t n, int64_t c, int j, const T *A,
+                                                       int64_t lda, const T *V, const T *W,
+                                                       int64_t ldv, const T *npart, int ncnt,
+                                                       T *P, T *Y1, T *Y2, T *scal, T *tau_out)
+{
+    __shared__ T sh[8];
+    __shared__ T srow[4][kTB];
+    const int64_t r0 = c + 1, m = n - r0;
+    // block (I, J) from the linear id: row I, J ≤ I
+    const int64_t q = blockIdx.x;
+    int64_t I = (int64_t)((sqrt(8.0 * (double)q + 1.0) - 1.0) / 2.0);
+    while ((I + 1) * (I + 2) / 2 <= q) ++I;
+    while (I * (I + 1) / 2 > q) --I;
+    const int64_t J = q - I * (I + 1) / 2;
+    const T nrm2 = block_sum_fixed(npart, ncnt, sh);
+    T xi, tau, beta;
+    reflector_scalars(nrm2, A[r0 + c * lda], xi, tau, beta);
+    if (q == 0 && threadIdx.x == 0) {
+        scal[S_TAU] = tau;
+        scal[S_XI] = xi;
+        scal[S_BETA] = beta;
+        tau_out[c] = tau;
+    }
+    const int t = threadIdx.x, a = t & 15, b = t >> 4, lane = t & 63, wv = t >> 6;
+    const int64_t R0 = r0 + I * kTB + 4 * a, C0 = r0 + J * kTB + 4 * b;
+    T x[4][4];
+    T vr[4], vcv[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+        const int64_t rr = R0 + u, cc = C0 + u;
+        vr[u] = rr < n ? vval(A, lda, c, rr, xi) : T(0);
+        vcv[u] = cc < n ? vval(A, lda, c, cc, xi) : T(0);
+    }
+#pragma unroll
+    for (int cc = 0; cc < 4; ++cc)
+#pragma unroll
+        for (int rr = 0; rr < 4; ++rr) {
+            const int64_t gi = R0 + rr, gj = C0 + cc;
+            x[cc][rr] = (gi < n && gj < n && gi >= gj) ? A[gi + gj * lda] : T(0);
+        }
+    // row sums (A_IJ v_J) and column sums (A_IJᵀ v_I, strictly lower part
+    // only on a diagonal block)
+    T s[4], qc[4];
+#pragma unroll
+    for (int rr = 0; rr < 4; ++rr) {
+        T acc = T(0);
+#pragma unroll
+        for (int cc = 0; cc < 4; ++cc) acc += x[cc][rr] * vcv[cc];
+        s[rr] = acc;
+    }
+#pragma unroll
+    for (int cc = 0; cc < 4; ++cc) {
+        T acc = T(0);
+#pragma unroll
+        for (int rr = 0; rr < 4; ++rr) {
+            const bool use = I != J || (R0 + rr > C0 + cc);
+            acc += use ? x[cc][rr] * vr[rr] : T(0);
+        }
+        qc[cc] = acc;
+    }
+    // s: reduce over b (lane bits 4, 5, then the four waves)
+#pragma unroll
+    for (int rr = 0; rr < 4; ++rr) {
+        s[rr] += __shfl_xor(s[rr], 16);
+        s[rr] += __shfl_xor(s[rr], 32);
+    }
+    if (lane < 16)
+#pragma unroll
+        for (int rr = 0; rr < 4; ++rr) srow[wv][4 * a + rr] = s[rr];
+    // qc: reduce over a (lane bits 0..3)
+#pragma unroll
+    for (int cc = 0; cc < 4; ++cc) {
+        qc[cc] += __shfl_xor(qc[cc], 1);
+        qc[cc] += __shfl_xor(qc[cc], 2);
+        qc[cc] += __shfl_xor(qc[cc], 4);
+        qc[cc] += __shfl_xor(qc[cc], 8);
+    }
+    __syncthreads();
+    if (I == J) {
+        // one slot for the diagonal block: row sums plus the transposed part
+        __shared__ T scol[kTB];
+        if (a == 0)
+#pragma unroll
+            for (int cc = 0; cc < 4; ++cc) scol[4 * b + cc] = qc[cc];
+        __syncthreads();
+        if (t < kTB) {
+            const int64_t r = I * kTB + t;
+            if (r < m) P[J * m + r] = srow[0][t] + srow[1][t] + srow[2][t] + srow[3][t] + scol[t];
+        }
+        // Wᵀv and Vᵀv over this block's rows
+        const int64_t gr = r0 + I * kTB + lane;
+        const T vg = gr < n ? vval(A, lda, c, gr, xi) : T(0);
+        for (int l = wv; l < j; l += 4) {
+            T y1 = gr < n ? W[gr + (int64_t)l * ldv] * vg : T(0);
+            T y2 = gr < n ? V[gr + (int64_t)l * ldv] * vg : T(0);
+            y1 = wave_sum(y1);
+            y2 = wave_sum(y2);
+            if (lane == 0) {
+                Y1[I * kPanel + l] = y1;
+                Y2[I * kPanel + l] = y2;
+            }
+        }
+    } else {
+        if (t < kTB) {
+            const int64_t r = I * kTB + t;
+            if (r < m) P[J * m + r] = srow[0][t] + srow[1][t] + srow[2][t] + srow[3][t];
+        }
+        if (a == 0)
+#pragma unroll
+            for (int cc = 0; cc < 4; ++cc) {
+                const int64_t r = J * kTB + 4 * b + cc;
+                if (r < m) P[I * m + r] = qc[cc];
+            }
+    }
+}
+
+// v into A (below the subdiagonal) and V, β on the subdiagonal;
+// u = τ (p − V y1 − W y2) into W[:, j]; partial vᵀu.
+template <typename T>
+__global__ __launch_bounds__(256) void tri_w_kernel(int64_t n, int64_t c, int j, T *A, int64_t lda,
+                                                    T *V, T *W, int64_t ldv, const T *P, int nbk,
+                                                    const T *Y1, const T *Y2, T *scal, T *qpart)
+{
+    __shared__ T sh[8];
+    __shared__ T y1[kPanel], y2[kPanel];
+    const int64_t r0 = c + 1, m = n - r0;
+    const T tau = scal[S_TAU], xi = scal[S_XI], beta = scal[S_BETA];
+    if (threadIdx.x < kPanel) {
+        const int l = threadIdx.x;
+        T s1 = T(0), s2 = T(0);
+        if (l < j)
+            for (int I = 0; I < nbk; ++I) {
+                s1 += Y1[I * kPanel + l];
+                s2 += Y2[I * kPanel + l];
+            }
+        y1[l] = s1;
+        y2[l] = s2;
+    }
+    const int64_t r = r0 + (int64_t)blockIdx.x * 256 + threadIdx.x;
+    T v = T(0);
+    if (r < n) v = r == r0 ? T(1) : A[r + c * lda] / xi;
+    __syncthreads();
+    T vu = T(0);
+    if (r < n) {
+        const int64_t rr = r - r0;
+        T p = T(0);
+        for (int s = 0; s < nbk; ++s) p += P[(int64_t)s * m + rr];
+        for (int l = 0; l < j; ++l)
+            p -= V[r + (int64_t)l * ldv] * y1[l] + W[r + (int64_t)l * ldv] * y2[l];
+        const T u = tau * p;
+        W[r + (int64_t)j * ldv] = u;
+        V[r + (int64_t)j * ldv] = v;
+        A[r + c * lda] = r == r0 ? beta : v;
+        if (r == r0) scal[S_UNEXT] = u;
+        vu = v * u;
+    }
+    vu = wave_sum(vu);
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = vu;
+    __syncthreads();
+    if (threadIdx.x == 0) qpart[blockIdx.x] = sh[0] + sh[1] + sh[2] + sh[3];
+}
+
+// A[i, j] <-> A[n−1−i, n−1−j] over the whole matrix (J A J in place).
+template <typename T>
+__global__ void reverse_kernel(int64_t n, T *A, int64_t lda)
+{
+    const int64_t i = (int64_t)blockIdx.x * 16 + threadIdx.x;
+    const int64_t j = (int64_t)blockIdx.y * 16 + threadIdx.y;
+    if (i >= n || j >= n) return;
+    const int64_t i2 = n - 1 - i, j2 = n - 1 - j;
+    // each pair once: the element with the smaller column-major index swaps
+    if (i + j * n < i2 + j2 * n) {
+        const T a = A[i + j * lda];
+        A[i + j * lda] = A[i2 + j2 * lda];
+        A[i2 + j2 * lda] = a;
+    }
+}
+
+template <typename T>
+__global__ void zero_kernel(int64_t cnt, T *p)
+{
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i < cnt) p[i] = T(0);
+}
+
+
+// ------------------------------------------------------ bidiagonalize! ----
+// Tall case (m ≥ n), panel [k, k+w), column i = k + j.  The trailing block
+// Ã keeps its panel-start values wherever a GEMV reads it (the column and
+// row brought up to date lie outside those regions); V, X (m rows) and
+// Y, U (n rows) hold the panel's left vectors, A u products, Aᵀ v products
+// and right vectors: A_now = Ã − V Yᵀ − X Uᵀ.
+enum { B_LTAU = 0, B_LXI = 1, B_LBETA = 2, B_RTAU = 3, B_RXI = 4, B_RBETA = 5 };
+
+// ‖A[i:m, i]‖² partials (the panel's first column, nothing to apply).
+template <typename T>
+__global__ void bd_col0_kernel(int64_t m, int64_t i, const T *A, int64_t lda, T *npart)
+{
+    __shared__ T sh[8];
+    const int64_t r = i + (int64_t)blockIdx.x * 256 + threadIdx.x;
+    T a = r < m ? A[r + i * lda] : T(0);
+    T sq = wave_sum(a * a);
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = sq;
+    __syncthreads();
+    if (threadIdx.x == 0) npart[blockIdx.x] = sh[0] + sh[1] + sh[2] + sh[3];
+}
+
+// Left reflector scalars of column i, then P[rb·C + c'] = Σ over the 64-row
+// block rb of Ã[r, i+1+c'] v_r (Ãᵀ v), and on cb = 0 blocks the partials
+// Cv1[rb·kPanel + l] = Σ V[r,l] v_r, Cv2 = Σ X[r,l] v_r (l < j).
+template <typename T>
+__global__ __launch_bounds__(256) void bd_gemvt_kernel(int64_t m, int64_t n, int64_t i, int j,
+                                                       const T *A, int64_t lda, const T *V,
+                                                       const T *X, int64_t ldm,
+                                                       const T *npart, int ncnt, int64_t ncb,
+                                                       T *P, T *Cv1, T *Cv2, T *scal, T *taul)
+{
+    __shared__ T sh[8];
+    const T nrm2 = block_sum_fixed(npart, ncnt, sh);
+    T xi, tau, beta;
+    reflector_scalars(nrm2, A[i + i * lda], xi, tau, beta);
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        scal[B_LTAU] = tau;
+        scal[B_LXI] = xi;
+        scal[B_LBETA] = beta;
+        taul[i] = tau;
+    }
+    const int64_t C = n - i - 1;
+    if (C <= 0) return;
+    const int64_t rb = blockIdx.x / ncb, cb = blockIdx.x % ncb;
+    const int t = threadIdx.x, a = t & 15, b = t >> 4, lane = t & 63, wv = t >> 6;
+    const int64_t R0 = i + rb * kTB + 4 * a, C0 = i + 1 + cb * kTB + 4 * b;
+    T vr[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+        const int64_t r = R0 + u;
+        vr[u] = r < m ? (r == i ? T(1) : A[r + i * lda] / xi) : T(0);
+    }
+    T qc[4];
+#pragma unroll
+    for (int cc = 0; cc < 4; ++cc) {
+        T acc = T(0);
+        const int64_t gc = C0 + cc;
+#pragma unroll
+        for (int rr = 0; rr < 4; ++rr) {
+            const int64_t gr = R0 + rr;
+            acc += (gr < m && gc < n) ? A[gr + gc * lda] * vr[rr] : T(0);
+        }
+        qc[cc] = acc;
+    }
+#pragma unroll
+    for (int cc = 0; cc < 4; ++cc) {
+        qc[cc] += __shfl_xor(qc[cc], 1);
+        qc[cc] += __shfl_xor(qc[cc], 2);
+        qc[cc] += __shfl_xor(qc[cc], 4);
+        qc[cc] += __shfl_xor(qc[cc], 8);
+    }
+    if (a == 0)
+#pragma unroll
+        for (int cc = 0; cc < 4; ++cc) {
+            const int64_t c = cb * kTB + 4 * b + cc;
+            if (c < C) P[rb * C + c] = qc[cc];
+        }
+    if (cb == 0) {
+        const int64_t gr = i + rb * kTB + lane;
+        const T vg = gr < m ? (gr == i ? T(1) : A[gr + i * lda] / xi) : T(0);
+        for (int l = wv; l < j; l += 4) {
+            T c1 = gr < m ? V[gr + (int64_t)l * ldm] * vg : T(0);
+            T c2 = gr < m ? X[gr + (int64_t)l * ldm] * vg : T(0);
+            c1 = wave_sum(c1);
+            c2 = wave_sum(c2);
+            if (lane == 0) {
+                Cv1[rb * kPanel + l] = c1;
+                Cv2[rb * kPanel + l] = c2;
+            }
+        }
+    }
+}
+
+// Index t covers column c = i+1+t (y_c = τ (Ãᵀv − Y c1 − U c2)_c, row i
+// brought up to date, partial ‖row‖²) and row r = i+t (v into A and V).
+template <typename T>
+__global__ __launch_bounds__(256) void bd_y_kernel(int64_t m, int64_t n, int64_t i, int j, T *A,
+                                                   int64_t lda, T *V, const T *X, int64_t ldm,
+                                                   T *Y, const T *U, int64_t ldn, const T *P,
+                                                   int64_t nrb, const T *Cv1, const T *Cv2,
+                                                   const T *scal, T *npart2)
+{
+    __shared__ T sh[8];
+    __shared__ T c1[kPanel], c2[kPanel], vi[kPanel], xi_[kPanel];
+    const T tau = scal[B_LTAU], xi = scal[B_LXI], beta = scal[B_LBETA];
+    if (threadIdx.x < kPanel) {
+        const int l = threadIdx.x;
+        T s1 = T(0), s2 = T(0);
+        if (l < j) {
+            for (int64_t rb = 0; rb < nrb; ++rb) {
+                s1 += Cv1[rb * kPanel + l];
+                s2 += Cv2[rb * kPanel + l];
+            }
+            vi[l] = V[i + (int64_t)l * ldm];
+            xi_[l] = X[i + (int64_t)l * ldm];
+        }
+        c1[l] = s1;
+        c2[l] = s2;
+    }
+    __syncthreads();
+    const int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    const int64_t C = n - i - 1;
+    T sq = T(0);
+    if (t < C) {
+        const int64_t c = i + 1 + t;
+        T p = T(0);
+        for (int64_t rb = 0; rb < nrb; ++rb) p += P[rb * C + t];
+        for (int l = 0; l < j; ++l)
+            p -= Y[c + (int64_t)l * ldn] * c1[l] + U[c + (int64_t)l * ldn] * c2[l];
+        const T y = tau * p;
+        Y[c + (int64_t)j * ldn] = y;
+        T r = A[i + c * lda] - y;
+        for (int l = 0; l < j; ++l)
+            r -= vi[l] * Y[c + (int64_t)l * ldn] + xi_[l] * U[c + (int64_t)l * ldn];
+        A[i + c * lda] = r;
+        sq = r * r;
+    }
+    const int64_t r = i + t;
+    if (r < m) {
+        const T v = r == i ? T(1) : A[r + i * lda] / xi;
+        V[r + (int64_t)j * ldm] = v;
+        A[r + i * lda] = r == i ? beta : v;
+    }
+    sq = wave_sum(sq);
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = sq;
+    __syncthreads();
+    if (threadIdx.x == 0) npart2[blockIdx.x] = sh[0] + sh[1] + sh[2] + sh[3];
+}
+
+// Right reflector scalars of row i, then P[cb·R + r'] = Σ over the 64-column
+// block cb of Ã[i+1+r', c] u_c (Ã u), and on rb = 0 blocks
+// Cu1[cb·kPanel + l] = Σ Y[c,l] u_c (l ≤ j), Cu2 = Σ U[c,l] u_c (l < j).
+template <typename T>
+__global__ __launch_bounds__(256) void bd_gemvn_kernel(int64_t m, int64_t n, int64_t i, int j,
+                                                       const T *A, int64_t lda, const T *Y,
+                                                       const T *U, int64_t ldn,
+                                                       const T *npart2, int ncnt, int64_t ncb,
+                                                       T *P, T *Cu1, T *Cu2, T *scal, T *taur)
+{
+    __shared__ T sh[8];
+    __shared__ T srow[4][kTB];
+    const T nrm2 = block_sum_fixed(npart2, ncnt, sh);
+    T xi, tau, beta;
+    reflector_scalars(nrm2, A[i + (i + 1) * lda], xi, tau, beta);
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        scal[B_RTAU] = tau;
+        scal[B_RXI] = xi;
+        scal[B_RBETA] = beta;
+        taur[i] = tau;
+    }
+    const int64_t R = m - i - 1;
+    const int64_t rb = blockIdx.x / ncb, cb = blockIdx.x % ncb;
+    const int t = threadIdx.x, a = t & 15, b = t >> 4, lane = t & 63, wv = t >> 6;
+    const int64_t R0 = i + 1 + rb * kTB + 4 * a, C0 = i + 1 + cb * kTB + 4 * b;
+    T uc[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+        const int64_t c = C0 + u;
+        uc[u] = c < n ? (c == i + 1 ? T(1) : A[i + c * lda] / xi) : T(0);
+    }
+    T s[4];
+#pragma unroll
+    for (int rr = 0; rr < 4; ++rr) {
+        T acc = T(0);
+        const int64_t gr = R0 + rr;
+#pragma unroll
+        for (int cc = 0; cc < 4; ++cc) {
+            const int64_t gc = C0 + cc;
+            acc += (gr < m && gc < n) ? A[gr + gc * lda] * uc[cc] : T(0);
+        }
+        s[rr] = acc;
+    }
+#pragma unroll
+    for (int rr = 0; rr < 4; ++rr) {
+        s[rr] += __shfl_xor(s[rr], 16);
+        s[rr] += __shfl_xor(s[rr], 32);
+    }
+    if (lane < 16)
+#pragma unroll
+        for (int rr = 0; rr < 4; ++rr) srow[wv][4 * a + rr] = s[rr];
+    __syncthreads();
+    if (t < kTB) {
+        const int64_t r = rb * kTB + t;
+        if (r < R) P[cb * R + r] = srow[0][t] + srow[1][t] + srow[2][t] + srow[3][t];
+    }
+    if (rb == 0) {
+        const int64_t gc = i + 1 + cb * kTB + lane;
+        const T ug = gc < n ? (gc == i + 1 ? T(1) : A[i + gc * lda] / xi) : T(0);
+        for (int l = wv; l <= j; l += 4) {
+            T c1 = gc < n ? Y[gc + (int64_t)l * ldn] * ug : T(0);
+            T c2 = (gc < n && l < j) ? U[gc + (int64_t)l * ldn] * ug : T(0);
+            c1 = wave_sum(c1);
+            c2 = wave_sum(c2);
+            if (lane == 0) {
+                Cu1[cb * kPanel + l] = c1;
+                Cu2[cb * kPanel + l] = c2;
+            }
+        }
+    }
+}
+
+// Index t covers row r = i+1+t (x_r = τ (Ã u − V c3 − X c4)_r; then, unless
+// `last`, column i+1 brought up to date with reflectors 0..j and its
+// partial ‖·‖²) and column c = i+1+t (u into A and U).
+template <typename T>
+__global__ __launch_bounds__(256) void bd_x_kernel(int64_t m, int64_t n, int64_t i, int j,
+                                                   int last, T *A, int64_t lda, const T *V, T *X,
+                                                   int64_t ldm, const T *Y, T *U, int64_t ldn,
+                                                   const T *P, int64_t ncb, const T *Cu1,
+                                                   const T *Cu2, const T *scal, T *npart)
+{
+    __shared__ T sh[8];
+    __shared__ T c3[kPanel], c4[kPanel], yn[kPanel], un[kPanel];
+    const T tau = scal[B_RTAU], xi = scal[B_RXI], beta = scal[B_RBETA];
+    const int64_t i1 = i + 1;
+    if (threadIdx.x < kPanel) {
+        const int l = threadIdx.x;
+        T s3 = T(0), s4 = T(0);
+        if (l <= j) {
+            for (int64_t cb = 0; cb < ncb; ++cb) {
+                s3 += Cu1[cb * kPanel + l];
+                if (l < j) s4 += Cu2[cb * kPanel + l];
+            }
+            yn[l] = Y[i1 + (int64_t)l * ldn];
+            un[l] = l == j ? T(1) : U[i1 + (int64_t)l * ldn];
+        }
+        c3[l] = s3;
+        c4[l] = s4;
+    }
+    __syncthreads();
+    const int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    const int64_t R = m - i - 1;
+    T sq = T(0);
+    if (t < R) {
+        const int64_t r = i1 + t;
+        T q = T(0);
+        for (int64_t cb = 0; cb < ncb; ++cb) q += P[cb * R + t];
+        for (int l = 0; l <= j; ++l) q -= V[r + (int64_t)l * ldm] * c3[l];
+        for (int l = 0; l < j; ++l) q -= X[r + (int64_t)l * ldm] * c4[l];
+        const T x = tau * q;
+        X[r + (int64_t)j * ldm] = x;
+        if (!last && i1 < n) {
+            T a = A[r + i1 * lda];
+            for (int l = 0; l <= j; ++l) {
+                const T xl = l == j ? x : X[r + (int64_t)l * ldm];
+                a -= V[r + (int64_t)l * ldm] * yn[l] + xl * un[l];
+            }
+            A[r + i1 * lda] = a;
+            sq = a * a;
+        }
+    }
+    const int64_t c = i1 + t;
+    if (c < n) {
+        const T u = c == i1 ? T(1) : A[i + c * lda] / xi;
+        U[c + (int64_t)j * ldn] = u;
+        A[i + c * lda] = c == i1 ? beta : u;
+    }
+    sq = wave_sum(sq);
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = sq;
+    __syncthreads();
+    if (threadIdx.x == 0) npart[blockIdx.x] = sh[0] + sh[1] + sh[2] + sh[3];
+}
+
+// B (n×m, ld ldb) = Aᵀ for A m×n.
+template <typename T>
+__global__ void transpose_kernel(int64_t m, int64_t n, const T *A, int64_t lda, T *B, int64_t ldb)
+{
+    __shared__ T tile[16][17];
+    const int64_t r0 = (int64_t)blockIdx.x * 16, c0 = (int64_t)blockIdx.y * 16;
+    const int tx = threadIdx.x, ty = threadIdx.y;
+    if (r0 + tx < m && c0 + ty < n) tile[ty][tx] = A[(r0 + tx) + (c0 + ty) * lda];
+    __syncthreads();
+    if (c0 + tx < n && r0 + ty < m) B[(c0 + tx) + (r0 + ty) * ldb] = tile[tx][ty];
+}
+
+}  // namespace
+
+// Workspace elements of symtri for order n.
+size_t symtri_scratch(int64_t n)
+{
+    const int64_t ldv = (n + 3) / 4 * 4;
+    const int64_t nbk = cdiv(n, kTB);
+    const int64_t g = cdiv(n, 256);
+    return 2 * (size_t)ldv * kPanel + (size_t)nbk * n + 2 * (size_t)nbk * kPanel + 2 * g + 64;
+}
+
+// symtriLower! on the lower triangle of A (n×n); τ (n − 1 entries, device).
+template <typename T>
+int symtri_lower_dev(hipStream_t s, int64_t n, T *A, int64_t lda, T *tau, T *ws)
+{
+    if (n < 1) return GLA_OK;
+    if (n > 1) {
+        hipLaunchKernelGGL((zero_kernel<T>), dim3((unsigned)cdiv(n - 1, 256)), dim3(256), 0, s,
+                           n - 1, tau);
+        GLA_LAUNCH_CHECK();
+    }
+    if (n <= 2) return GLA_OK;
+    const int64_t ldv = (n + 3) / 4 * 4;
+    const int64_t nbk_max = cdiv(n, kTB);
+    const int64_t gmax = cdiv(n, 256);
+    T *V = ws, *W = V + ldv * kPanel, *P = W + ldv * kPanel, *Y1 = P + nbk_max * n,
+      *Y2 = Y1 + nbk_max * kPanel, *npart = Y2 + nbk_max * kPanel, *qpart = npart + gmax,
+      *scal = qpart + gmax;
+    const int64_t ncols = n - 2;   // columns with a reflector (:462)
+    for (int64_t k = 0; k < ncols; k += kPanel) {
+        const int w = (int)std::min<int64_t>(kPanel, ncols - k);
+        for (int j = 0; j < w; ++j) {
+            const int64_t c = k + j;
+            const int64_t m = n - c - 1;
+            const unsigned g1 = (unsigned)cdiv(n - c, 256);
+            const int qcnt = (int)cdiv(n - c, 256);   // the previous column's tri_w grid
+            hipLaunchKernelGGL((tri_col_kernel<T>), dim3(g1), dim3(256), 0, s, n, c, j, A, lda, V,
+                               W, ldv, qpart, qcnt, scal, npart);
+            GLA_LAUNCH_CHECK();
+            const int64_t nbk = cdiv(m, kTB);
+            hipLaunchKernelGGL((tri_symv_kernel<T>), dim3((unsigned)(nbk * (nbk + 1) / 2)),
+                               dim3(256), 0, s, n, c, j, (const T *)A, lda, (const T *)V,
+                               (const T *)W, ldv, (const T *)npart, (int)g1, P, Y1, Y2, scal, tau);
+            GLA_LAUNCH_CHECK();
+            const unsigned g4 = (unsigned)cdiv(m, 256);
+            hipLaunchKernelGGL((tri_w_kernel<T>), dim3(g4), dim3(256), 0, s, n, c, j, A, lda, V, W,
+                               ldv, (const T *)P, (int)nbk, (const T *)Y1, (const T *)Y2, scal,
+                               qpart);
+            GLA_LAUNCH_CHECK();
+        }
+        // W's last column: u -> w over rows ≥ k + w
+        const int64_t c1 = k + w;
+        hipLaunchKernelGGL((tri_wfin_kernel<T>), dim3((unsigned)cdiv(n - c1, 256)), dim3(256), 0,
+                           s, n, c1, w - 1, V, W, ldv, (const T *)qpart, (int)cdiv(n - c1, 256),
+                           (const T *)scal);
+        GLA_LAUNCH_CHECK();
+        // Ã[c1:, c1:] −= V Wᵀ + W Vᵀ (lower triangle)
+        const int64_t mt = n - c1;
+        if (mt > 0) {
+            GLA_TRY(gemm<T>(s, 'N', 'T', 1, mt, mt, w, T(-1), V + c1, ldv, W + c1, ldv, T(1),
+                            A + c1 + c1 * lda, lda, 0));
+            GLA_TRY(gemm<T>(s, 'N', 'T', 1, mt, mt, w, T(-1), W + c1, ldv, V + c1, ldv, T(1),
+                            A + c1 + c1 * lda, lda, 0));
+        }
+    }
+    return GLA_OK;
+}
+
+template <typename T>
+int symtri_dev(hipStream_t s, char uplo, int64_t n, T *A, int64_t lda, T *tau, T *ws)
+{
+    const dim3 rg((unsigned)cdiv(n, 16), (unsigned)cdiv(n, 16));
+    if (uplo == 'U') {
+        hipLaunchKernelGGL((reverse_kernel<T>), rg, dim3(16, 16), 0, s, n, A, lda);
+        GLA_LAUNCH_CHECK();
+    }
+    GLA_TRY(symtri_lower_dev<T>(s, n, A, lda, tau, ws));
+    if (uplo == 'U') {
+        hipLaunchKernelGGL((reverse_kernel<T>), rg, dim3(16, 16), 0, s, n, A, lda);
+        GLA_LAUNCH_CHECK();
+    }
+    return GLA_OK;
+}
+
+template int symtri_dev<double>(hipStream_t, char, int64_t, double *, int64_t, double *, double *);
+template int symtri_dev<float>(hipStream_t, char, int64_t, float *, int64_t, float *, float *);
+
+// Workspace elements of bidiagonalize! for m×n.
+size_t bidiag_scratch(int64_t m, int64_t n)
+{
+    const int64_t M = std::max(m, n), N = std::min(m, n);   // the tall problem
+    const int64_t ldm = (M + 3) / 4 * 4, ldn = (N + 3) / 4 * 4;
+    const int64_t nrb = cdiv(M, kTB), ncb = cdiv(N, kTB), g = cdiv(M, 256) + 1;
+    const size_t pe = std::max((size_t)nrb * N, (size_t)ncb * M);
+    const size_t tr = m < n ? (size_t)ldm * N : 0;   // Aᵀ for the wide case
+    return 2 * (size_t)ldm * kPanel + 2 * (size_t)ldn * kPanel + pe +
+           2 * (size_t)nrb * kPanel + 2 * (size_t)ncb * kPanel + 2 * (size_t)g + 16 + tr;
+}
+
+// bidiagonalize! of a tall A (m ≥ n).
+template <typename T>
+int bidiag_tall_dev(hipStream_t s, int64_t m, int64_t n, T *A, int64_t lda, T *taul, T *taur,
+                    T *ws)
+{
+    const int64_t ldm = (m + 3) / 4 * 4, ldn = (n + 3) / 4 * 4;
+    const int64_t nrb_max = cdiv(m, kTB), ncb_max = cdiv(n, kTB), gmax = cdiv(m, 256) + 1;
+    T *V = ws, *X = V + ldm * kPanel, *Y = X + ldm * kPanel, *U = Y + ldn * kPanel;
+    T *P = U + ldn * kPanel;
+    T *Cv1 = P + std::max(nrb_max * n, ncb_max * m), *Cv2 = Cv1 + nrb_max * kPanel;
+    T *Cu1 = Cv2 + nrb_max * kPanel, *Cu2 = Cu1 + ncb_max * kPanel;
+    T *npart = Cu2 + ncb_max * kPanel, *npart2 = npart + gmax, *scal = npart2 + gmax;
+    for (int64_t k = 0; k < n; k += kPanel) {
+        const int w = (int)std::min<int64_t>(kPanel, n - k);
+        int ncnt = (int)cdiv(m - k, 256);
+        hipLaunchKernelGGL((bd_col0_kernel<T>), dim3((unsigned)ncnt), dim3(256), 0, s, m, k,
+                           (const T *)A, lda, npart);
+        GLA_LAUNCH_CHECK();
+        for (int j = 0; j < w; ++j) {
+            const int64_t i = k + j;
+            const int64_t C = n - i - 1, R = m - i;
+            const int64_t nrb = cdiv(R, kTB), ncb = std::max<int64_t>(1, cdiv(C, kTB));
+            const unsigned g2 = (unsigned)(C > 0 ? nrb * ncb : 1);
+            hipLaunchKernelGGL((bd_gemvt_kernel<T>), dim3(g2), dim3(256), 0, s, m, n, i, j,
+                               (const T *)A, lda, (const T *)V, (const T *)X, ldm,
+                               (const T *)npart, ncnt, ncb, P, Cv1, Cv2, scal, taul);
+            GLA_LAUNCH_CHECK();
+            const unsigned g3 = (unsigned)cdiv(std::max(R, C), 256);
+            hipLaunchKernelGGL((bd_y_kernel<T>), dim3(g3), dim3(256), 0, s, m, n, i, j, A, lda, V,
+                               (const T *)X, ldm, Y, (const T *)U, ldn, (const T *)P,
+                               C > 0 ? nrb : 0, (const T *)Cv1, (const T *)Cv2, (const T *)scal,
+                               npart2);
+            GLA_LAUNCH_CHECK();
+            if (C == 0) break;   // the last column: no right reflector (src/svd.jl:339)
+            const int64_t R1 = m - i - 1;
+            const int64_t nrb1 = cdiv(R1, kTB), ncb1 = cdiv(C, kTB);
+            hipLaunchKernelGGL((bd_gemvn_kernel<T>), dim3((unsigned)(nrb1 * ncb1)), dim3(256), 0,
+                               s, m, n, i, j, (const T *)A, lda, (const T *)Y, (const T *)U, ldn,
+                               (const T *)npart2, (int)g3, ncb1, P, Cu1, Cu2, scal, taur);
+            GLA_LAUNCH_CHECK();
+            const int last = j == w - 1;
+            const unsigned g5 = (unsigned)cdiv(std::max(R1, C), 256);
+            hipLaunchKernelGGL((bd_x_kernel<T>), dim3(g5), dim3(256), 0, s, m, n, i, j, last, A,
+                               lda, (const T *)V, X, ldm, (const T *)Y, U, ldn, (const T *)P,
+                               ncb1, (const T *)Cu1, (const T *)Cu2, (const T *)scal, npart);
+            GLA_LAUNCH_CHECK();
+            ncnt = (int)g5;
+        }
+        const int64_t c1 = k + w;
+        if (c1 < n) {   // Ã[c1:, c1:] −= V Yᵀ + X Uᵀ
+            GLA_TRY(gemm<T>(s, 'N', 'T', 0, m - c1, n - c1, w, T(-1), V + c1, ldm, Y + c1, ldn,
+                            T(1), A + c1 + c1 * lda, lda, 0));
+            GLA_TRY(gemm<T>(s, 'N', 'T', 0, m - c1, n - c1, w, T(-1), X + c1, ldm, U + c1, ldn,
+                            T(1), A + c1 + c1 * lda, lda, 0));
+        }
+    }
+    return GLA_OK;
+}
+
+// bidiagonalize!: tall directly, wide (m < n) as the tall problem of Aᵀ
+// (its left reflectors are A's right ones, src/svd.jl:356-375 mirrors
+// :332-349), transposed back.
+template <typename T>
+int bidiag_dev(hipStream_t s, int64_t m, int64_t n, T *A, int64_t lda, T *taul, T *taur, T *ws)
+{
+    if (m == 0 || n == 0) return GLA_OK;
+    if (m >= n) return bidiag_tall_dev<T>(s, m, n, A, lda, taul, taur, ws);
+    const int64_t ldt = (n + 3) / 4 * 4;
+    T *At = ws;
+    T *rest = ws + (size_t)ldt * m;
+    hipLaunchKernelGGL((transpose_kernel<T>), dim3((unsigned)cdiv(m, 16), (unsigned)cdiv(n, 16)),
+                       dim3(16, 16), 0, s, m, n, (const T *)A, lda, At, ldt);
+    GLA_LAUNCH_CHECK();
+    GLA_TRY(bidiag_tall_dev<T>(s, n, m, At, ldt, taur, taul, rest));
+    hipLaunchKernelGGL((transpose_kernel<T>), dim3((unsigned)cdiv(n, 16), (unsigned)cdiv(m, 16)),
+                       dim3(16, 16), 0, s, n, m, (const T *)At, ldt, A, lda);
+    GLA_LAUNCH_CHECK();
+    return GLA_OK;
+}
+
+template int bidiag_dev<double>(hipStream_t, int64_t, int64_t, double *, int64_t, double *,
+                                double *, double *);
+template int bidiag_dev<float>(hipStream_t, int64_t, int64_t, float *, int64_t, float *, float *,
+                               float *);
+
+}  // namespace gla
+
+// ------------------------------------------------------------------ C ABI --
+namespace {
+
+template <typename T>
+int symtri_entry(gla_handle_t hh, char uplo, int64_t n, T *A, int64_t lda, T *tau, void *stream)
+{
+    gla::Handle *h = reinterpret_cast<gla::Handle *>(hh);
+    if (!h) { gla::set_error("null handle"); return -1; }
+    if (uplo != 'L' && uplo != 'U') { gla::set_error("uplo must be 'L' or 'U'"); return -2; }
+    if (n < 0) { gla::set_error("n < 0"); return -3; }
+    if (!A && n > 0) { gla::set_error("A is null"); return -4; }
+    if (lda < std::max<int64_t>(1, n)) { gla::set_error("lda < max(1,n)"); return -5; }
+    if (!tau && n > 1) { gla::set_error("tau is null"); return -6; }
+    if (n == 0) return GLA_OK;
+    GLA_HIP_TRY(hipSetDevice(h->device));
+    GLA_TRY(gla::ensure_workspace(h, gla::symtri_scratch(n) * sizeof(T)));
+    return gla::symtri_dev<T>(reinterpret_cast<hipStream_t>(stream), uplo, n, A, lda, tau,
+                              reinterpret_cast<T *>(h->ws));
+}
+
+
+template <typename T>
+int bidiag_entry(gla_handle_t hh, int64_t m, int64_t n, T *A, int64_t lda, T *taul, T *taur,
+                 void *stream)
+{
+    gla::Handle *h = reinterpret_cast<gla::Handle *>(hh);
+    if (!h) { gla::set_error("null handle"); return -1; }
+    if (m < 0) { gla::set_error("m < 0"); return -2; }
+    if (n < 0) { gla::set_error("n < 0"); return -3; }
+    if (!A && m > 0 && n > 0) { gla::set_error("A is null"); return -4; }
+    if (lda < std::max<int64_t>(1, m)) { gla::set_error("lda < max(1,m)"); return -5; }
+    const int64_t k = std::min(m, n);
+    const int64_t nl = m >= n ? k : k - 1, nr = m >= n ? k - 1 : k;
+    if (!taul && nl > 0) { gla::set_error("taul is null"); return -6; }
+    if (!taur && nr > 0) { gla::set_error("taur is null"); return -7; }
+    if (k == 0) return GLA_OK;
+    GLA_HIP_TRY(hipSetDevice(h->device));
+    GLA_TRY(gla::ensure_workspace(h, gla::bidiag_scratch(m, n) * sizeof(T)));
+    return gla::bidiag_dev<T>(reinterpret_cast<hipStream_t>(stream), m, n, A, lda, taul, taur,
+                              reinterpret_cast<T *>(h->ws));
+}
+
+}  // namespace
+
+extern "C" {
+
+int gla_symtri_d(gla_handle_t h, char uplo, int64_t n, double *A, int64_t lda, double *tau,
+                 void *stream)
+{
+    return symtri_entry<double>(h, uplo, n, A, lda, tau, stream);
+}
+int gla_symtri_s(gla_handle_t h, char uplo, int64_t n, float *A, int64_t lda, float *tau,
+                 void *stream)
+{
+    return symtri_entry<float>(h, uplo, n, A, lda, tau, stream);
+}
+
+int gla_bidiag_d(gla_handle_t h, int64_t m, int64_t n, double *A, int64_t lda, double *taul,
+                 double *taur, void *stream)
+{
+    return bidiag_entry<double>(h, m, n, A, lda, taul, taur, stream);
+}
+int gla_bidiag_s(gla_handle_t h, int64_t m, int64_t n, float *A, int64_t lda, float *taul,
+                 float *taur, void *stream)
+{
+    return bidiag_entry<float>(h, m, n, A, lda, taul, taur, stream);
+}
+
+}  // extern "C"

@@ -27,7 +27,8 @@ __all__ = [
     "LowerTriangular", "UpperTriangular", "UnitLowerTriangular", "Hermitian",
     "cholUnblocked_", "cholBlocked_", "cholRecursive_", "rankUpdate_", "rdiv_", "lmul_",
     "QR2", "R", "QBlocked", "HouseholderBlock", "Adjoint", "adjoint", "qrUnblocked_",
-    "qrBlocked_", "gemm_", "LDLt", "ldlt_", "ldlt",
+    "qrBlocked_", "gemm_", "LDLt", "ldlt_", "ldlt", "SymmetricTridiagonalFactorization",
+    "symtri_", "BidiagonalFactorization", "bidiagonalize_",
 ]
 
 
@@ -790,6 +791,84 @@ def ldlt_(Ah, blocksize=None):
 def ldlt(Ah, blocksize=None):
     """ldlt(A::Hermitian) — src/ldlt.jl:202: ldlt! of a copy."""
     return ldlt_(Hermitian(colmajor(Ah.data, device=Ah.data.device), Ah.uplo), blocksize)
+
+
+# ------------------------------------------------- two-sided reductions ----
+class SymmetricTridiagonalFactorization:
+    """What symtri! returns (src/eigenSelfAdjoint.jl:450-564): the reflectors
+    (``factors`` with ``uplo`` and ``tau``, the EigenQ) and the tridiagonal
+    (``diagonals`` = (d, e))."""
+
+    def __init__(self, uplo, factors, tau):
+        self.uplo = uplo
+        self.factors = factors
+        self.tau = tau
+
+    @property
+    def diagonals(self):
+        d = torch.diagonal(self.factors).clone()
+        e = torch.diagonal(self.factors, -1 if self.uplo == "L" else 1).clone()
+        return d, e
+
+
+def symtri_(Ah, tau=None):
+    """symtri!(A::Hermitian) — symtriLower!/symtriUpper!,
+    src/eigenSelfAdjoint.jl:446-564, for real element types.  Reduces
+    Ah.data in place; returns SymmetricTridiagonalFactorization."""
+    if not isinstance(Ah, Hermitian):
+        raise TypeError("symtri! takes a Hermitian (or Symmetric) matrix")
+    A = Ah.data
+    lda = _mat(A)
+    n, m = A.shape
+    if n != m:
+        raise DimensionMismatch(f"matrix is not square: dimensions are {tuple(A.shape)}")
+    if tau is None:
+        tau = torch.zeros(max(n - 1, 0), dtype=A.dtype, device=A.device)
+    _vec(tau, max(n - 1, 0), A.dtype)
+    with torch.cuda.device(A.device):
+        rc = getattr(lib(), f"gla_symtri_{_sfx(A)}")(
+            _handle(A.device.index), Ah.uplo.encode(), n, C.c_void_p(A.data_ptr()), lda,
+            C.c_void_p(tau.data_ptr()), _stream(A))
+    _check(rc, "gla_symtri")
+    return SymmetricTridiagonalFactorization(Ah.uplo, A, tau)
+
+
+class BidiagonalFactorization:
+    """What bidiagonalize! returns (src/svd.jl:321-381): the bidiagonal
+    (``uplo`` 'U' for m ≥ n, 'L' otherwise; ``dv``, ``ev``), the reflectors
+    (``factors``) and their scalars ``taul``, ``taur``."""
+
+    def __init__(self, factors, taul, taur):
+        self.factors = factors
+        self.taul = taul
+        self.taur = taur
+        m, n = factors.shape
+        self.uplo = "U" if m >= n else "L"
+
+    @property
+    def bidiagonal(self):
+        F = self.factors
+        k = min(F.shape)
+        dv = torch.diagonal(F)[:k].clone()
+        ev = torch.diagonal(F, 1 if self.uplo == "U" else -1)[:max(k - 1, 0)].clone()
+        return dv, ev
+
+
+def bidiagonalize_(A):
+    """bidiagonalize!(A) — src/svd.jl:328-381, real element types.  Reduces A
+    in place; returns BidiagonalFactorization."""
+    lda = _mat(A)
+    m, n = A.shape
+    k = min(m, n)
+    nl, nr = (k, max(k - 1, 0)) if m >= n else (max(k - 1, 0), k)
+    taul = torch.zeros(max(nl, 1), dtype=A.dtype, device=A.device)
+    taur = torch.zeros(max(nr, 1), dtype=A.dtype, device=A.device)
+    with torch.cuda.device(A.device):
+        rc = getattr(lib(), f"gla_bidiag_{_sfx(A)}")(
+            _handle(A.device.index), m, n, C.c_void_p(A.data_ptr()), lda,
+            C.c_void_p(taul.data_ptr()), C.c_void_p(taur.data_ptr()), _stream(A))
+    _check(rc, "gla_bidiag")
+    return BidiagonalFactorization(A, taul[:nl], taur[:nr])
 
 
 def _batch3(X, name):

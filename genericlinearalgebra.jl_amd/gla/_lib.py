@@ -56,6 +56,8 @@ def _sigs(ft):
         "gla_chol_solve": (_int, [_p, _ch, _i64, _i64, _p, _i64, _p, _i64, _p]),
         "gla_qr_solve": (_int, [_p, _i64, _i64, _i64, _p, _i64, _p, _p, _i64, _p, _p]),
         "gla_ldlt": (_int, [_p, _ch, _i64, _p, _i64, _i64, _p]),
+        "gla_symtri": (_int, [_p, _ch, _i64, _p, _i64, _p, _p]),
+        "gla_bidiag": (_int, [_p, _i64, _i64, _p, _i64, _p, _p, _p]),
         "gla_ldlt_solve": (_int, [_p, _ch, _i64, _i64, _p, _i64, _p, _i64, _p]),
         "gla_ldlt_lmul": (_int, [_p, _ch, _i64, _i64, _p, _i64, _p, _i64, _p]),
     }

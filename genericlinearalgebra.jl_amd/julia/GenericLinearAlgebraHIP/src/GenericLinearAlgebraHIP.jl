@@ -86,6 +86,10 @@ const SIGNATURES = Dict{Symbol,Tuple{Any,Tuple}}(
     :gla_ldlt_solve_s => (Cint, (Ptr{Cvoid}, Cchar, Int64, Int64, Ptr{Float32}, Int64, Ptr{Float32}, Int64, Ptr{Cvoid})),
     :gla_ldlt_lmul_d => (Cint, (Ptr{Cvoid}, Cchar, Int64, Int64, Ptr{Float64}, Int64, Ptr{Float64}, Int64, Ptr{Cvoid})),
     :gla_ldlt_lmul_s => (Cint, (Ptr{Cvoid}, Cchar, Int64, Int64, Ptr{Float32}, Int64, Ptr{Float32}, Int64, Ptr{Cvoid})),
+    :gla_symtri_d => (Cint, (Ptr{Cvoid}, Cchar, Int64, Ptr{Float64}, Int64, Ptr{Float64}, Ptr{Cvoid})),
+    :gla_symtri_s => (Cint, (Ptr{Cvoid}, Cchar, Int64, Ptr{Float32}, Int64, Ptr{Float32}, Ptr{Cvoid})),
+    :gla_bidiag_d => (Cint, (Ptr{Cvoid}, Int64, Int64, Ptr{Float64}, Int64, Ptr{Float64}, Ptr{Float64}, Ptr{Cvoid})),
+    :gla_bidiag_s => (Cint, (Ptr{Cvoid}, Int64, Int64, Ptr{Float32}, Int64, Ptr{Float32}, Ptr{Float32}, Ptr{Cvoid})),
     :gla_chol_solve_d => (Cint, (Ptr{Cvoid}, Cchar, Int64, Int64, Ptr{Float64}, Int64, Ptr{Float64}, Int64, Ptr{Cvoid})),
     :gla_chol_solve_s => (Cint, (Ptr{Cvoid}, Cchar, Int64, Int64, Ptr{Float32}, Int64, Ptr{Float32}, Int64, Ptr{Cvoid})),
     :gla_qr_solve_d => (Cint, (Ptr{Cvoid}, Int64, Int64, Int64, Ptr{Float64}, Int64, Ptr{Float64}, Ptr{Float64}, Int64, Ref{Int64}, Ptr{Cvoid})),
@@ -504,6 +508,45 @@ for (T, sfx) in ((Float64, "d"), (Float32, "s"))
                                          ld(D), pointer(X), ld(X), C_NULL), "lmul!"))
             return X
         end
+    end
+end
+
+# symtri!(A::Hermitian) — src/eigenSelfAdjoint.jl:446-564, real element
+# types: the reflectors stay on the device (EigenQ), the tridiagonal is
+# downloaded (SymTridiagonal), as the reference returns them.
+for (T, sfx) in ((Float64, "d"), (Float32, "s"))
+    fn = Symbol("gla_symtri_", sfx)
+    @eval function GenericLinearAlgebra.symtri!(A::Union{Hermitian{$T,<:HIPMatrix{$T}},
+                                                         Symmetric{$T,<:HIPMatrix{$T}}})
+        D = A.data
+        n = _square(D)
+        τ = hipzeros($T, max(n - 1, 0))
+        with_handle(h -> _check(@gla($fn, h, Cchar(A.uplo), n, pointer(D), ld(D), pointer(τ),
+                                     C_NULL), "symtri!"))
+        H = Array(D)
+        d = diag(H)
+        e = A.uplo == 'L' ? diag(H, -1) : diag(H, 1)
+        return GenericLinearAlgebra.SymmetricTridiagonalFactorization(
+            GenericLinearAlgebra.EigenQ(A.uplo, D, τ), SymTridiagonal(d, e))
+    end
+end
+
+# bidiagonalize!(A) — src/svd.jl:328-381, real element types: reflectors and
+# τ vectors stay on the device, the bidiagonal is downloaded.
+for (T, sfx) in ((Float64, "d"), (Float32, "s"))
+    fn = Symbol("gla_bidiag_", sfx)
+    @eval function GenericLinearAlgebra.bidiagonalize!(A::HIPMatrix{$T})
+        m, n = size(A)
+        k = min(m, n)
+        nl, nr = m >= n ? (k, max(k - 1, 0)) : (max(k - 1, 0), k)
+        τl = hipzeros($T, max(nl, 1))
+        τr = hipzeros($T, max(nr, 1))
+        with_handle(h -> _check(@gla($fn, h, m, n, pointer(A), ld(A), pointer(τl), pointer(τr),
+                                     C_NULL), "bidiagonalize!"))
+        H = Array(A)
+        bd = m >= n ? Bidiagonal(diag(H), diag(H, 1), :U) : Bidiagonal(diag(H), diag(H, -1), :L)
+        return GenericLinearAlgebra.BidiagonalFactorization{$T,Vector{$T},typeof(A),typeof(τl)}(
+            bd, A, hipview(τl, 1:nl), hipview(τr, 1:nr))
     end
 end
 

@@ -230,6 +230,33 @@ int gla_ldlt_lmul_d(gla_handle_t h, char uplo, int64_t n, int64_t nrhs, const do
 int gla_ldlt_lmul_s(gla_handle_t h, char uplo, int64_t n, int64_t nrhs, const float *F,
                     int64_t ldf, float *B, int64_t ldb, void *stream);
 
+/* ------------------------------------------------ two-sided reductions -- */
+/* symtri!(A::Hermitian/Symmetric) — src/eigenSelfAdjoint.jl:446-564:
+ * symtriLower! (uplo 'L') or symtriUpper! (uplo 'U') of a real symmetric
+ * A (n×n; only the uplo triangle is read and written).  On return the
+ * diagonal holds the tridiagonal's diagonal, the sub- ('L') or super-
+ * ('U') diagonal its off-diagonal, and the rest of the triangle the
+ * Householder vectors (leading 1 implicit) in the reference's packing;
+ * tau (device, n − 1 entries) their τ, the last one 0.  Reflector
+ * convention of the Julia stdlib reflector! (as in gla_qr_unblocked_d). */
+int gla_symtri_d(gla_handle_t h, char uplo, int64_t n, double *A, int64_t lda, double *tau,
+                 void *stream);
+int gla_symtri_s(gla_handle_t h, char uplo, int64_t n, float *A, int64_t lda, float *tau,
+                 void *stream);
+
+/* bidiagonalize!(A) — src/svd.jl:328-381, real A (m×n).  m ≥ n: upper
+ * bidiagonal (diagonal and superdiagonal of A), left reflectors below the
+ * diagonal (taul: n entries), right reflectors right of the superdiagonal
+ * (taur: n − 1 entries); m < n: lower bidiagonal (diagonal and
+ * subdiagonal), right reflectors right of the diagonal (taur: m entries),
+ * left reflectors below the subdiagonal (taul: m − 1 entries).  The
+ * reference's packing and reflector! convention; taul/taur device vectors
+ * (may be NULL when of length 0). */
+int gla_bidiag_d(gla_handle_t h, int64_t m, int64_t n, double *A, int64_t lda, double *taul,
+                 double *taur, void *stream);
+int gla_bidiag_s(gla_handle_t h, int64_t m, int64_t n, float *A, int64_t lda, float *taul,
+                 float *taur, void *stream);
+
 /* ------------------------------------------- batched uneven Cholesky -- */
 /* No counterpart in the reference (one matrix per call there): factor and
  * solve many SPD matrices of uneven size n ≤ 256 in one persistent launch,

@@ -126,3 +126,36 @@ double orq_chol_residual(int64_t n, const double *A, int64_t lda, const double *
         }
     return (double)sqrtq(num / den);
 }
+
+/* symtri! / bidiagonalize! evaluated in __float128 from float64 input,
+ * rounded back: the stand-in for the reference's BigFloat runs, the truth
+ * the FP64 results are measured against. */
+void orq_symtri(int upper, int64_t n, double *A, int64_t lda, double *tau)
+{
+    __float128 *Q = q_from_d(n, n, A, lda);
+    __float128 *tq = (__float128 *)malloc(sizeof(__float128) * (size_t)(n > 1 ? n - 1 : 1));
+    if (upper)
+        orc_symtri_upper_q(n, Q, n, tq);
+    else
+        orc_symtri_lower_q(n, Q, n, tq);
+    d_from_q(n, n, Q, A, lda);
+    for (int64_t i = 0; i + 1 < n; ++i) tau[i] = (double)tq[i];
+    free(tq);
+    free(Q);
+}
+
+void orq_bidiag(int64_t m, int64_t n, double *A, int64_t lda, double *taul, double *taur)
+{
+    __float128 *Q = q_from_d(m, n, A, lda);
+    int64_t k = m < n ? m : n;
+    __float128 *tl = (__float128 *)malloc(sizeof(__float128) * (size_t)(k > 0 ? k : 1));
+    __float128 *tr = (__float128 *)malloc(sizeof(__float128) * (size_t)(k > 0 ? k : 1));
+    orc_bidiag_q(m, n, Q, m, tl, tr);
+    d_from_q(m, n, Q, A, lda);
+    int64_t nl = m >= n ? k : k - 1, nr = m >= n ? k - 1 : k;
+    for (int64_t i = 0; i < nl; ++i) taul[i] = (double)tl[i];
+    for (int64_t i = 0; i < nr; ++i) taur[i] = (double)tr[i];
+    free(tl);
+    free(tr);
+    free(Q);
+}

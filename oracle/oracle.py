@@ -71,6 +71,8 @@ def lib() -> C.CDLL:
         L.orq_chol_unblocked.argtypes = [_i64, _p, _i64]
         L.orq_chol_unblocked.restype = _i64
         L.orq_qr_unblocked.argtypes = [_i64, _i64, _p, _i64, _p]
+        L.orq_symtri.argtypes = [C.c_int, _i64, _p, _i64, _p]
+        L.orq_bidiag.argtypes = [_i64, _i64, _p, _i64, _p, _p]
         L.orq_chol_residual.argtypes = [_i64, _p, _i64, _p, _i64]
         L.orq_chol_residual.restype = C.c_double
         _lib = L
@@ -314,6 +316,27 @@ def symtri(A, uplo: str = "L"):
     name = "orc_symtri_lower" if uplo == "L" else "orc_symtri_upper"
     _fn(name, A)(n, _ptr(A), max(n, 1), _ptr(tau) if n > 1 else None)
     return A, tau
+
+
+def symtri_quad(A, uplo: str = "L"):
+    """symtri! evaluated in __float128 from float64 input, rounded back."""
+    A = _f(A, np.float64)
+    n = A.shape[0]
+    tau = np.zeros(max(n - 1, 1), dtype=np.float64)
+    lib().orq_symtri(1 if uplo == "U" else 0, n, _ptr(A), max(n, 1), _ptr(tau))
+    return A, tau[:max(n - 1, 0)]
+
+
+def bidiag_quad(A):
+    """bidiagonalize! evaluated in __float128 from float64 input, rounded back."""
+    A = _f(A, np.float64)
+    m, n = A.shape
+    k = min(m, n)
+    nl, nr = (k, max(k - 1, 0)) if m >= n else (max(k - 1, 0), k)
+    taul = np.zeros(max(nl, 1))
+    taur = np.zeros(max(nr, 1))
+    lib().orq_bidiag(m, n, _ptr(A), max(m, 1), _ptr(taul), _ptr(taur))
+    return A, taul[:nl], taur[:nr]
 
 
 def symtri_q(AS, tau, uplo: str = "L"):

@@ -160,3 +160,31 @@ def test_planted_slip_fails(rng):
     assert rel(H @ At @ H, bad) > 1e-3
     assert rel(Q.T @ A @ Q, np.diag(np.diag(AS)) + np.diag(np.diag(AS, -1), -1)
                + np.diag(np.diag(AS, -1), 1)) <= 1e-13
+
+
+@pytest.mark.parametrize("n", [20, 64])
+@pytest.mark.parametrize("uplo", ["L", "U"])
+def test_symtri_quad_is_the_truth_both_fp64_forms_approach(n, uplo, rng):
+    """The __float128 evaluation (the BigFloat stand-in) is what the FP64
+    oracle and LAPACK dsytrd both approximate: each within 1e-11 of it,
+    and it satisfies Qᵀ A Q = T to float64 rounding."""
+    A = sym(n, rng)
+    tru, ttru = O.symtri_quad(A, uplo)
+    ref, tref = O.symtri(A, uplo)
+    c, dl, el, tl, info = lp.dsytrd(A, lower=1 if uplo == "L" else 0)
+    tri = np.tril if uplo == "L" else np.triu
+    assert rel(tri(ref), tri(tru)) <= 1e-11
+    assert rel(tri(c), tri(tru)) <= 1e-11
+    Q = O.symtri_q(tru, ttru, uplo)
+    d, e = O.symtri_diagonals(tru, uplo)
+    T = np.diag(d) + np.diag(e, 1) + np.diag(e, -1)
+    assert np.linalg.norm(Q.T @ A @ Q - T) <= 10 * n * eps_of(np.float64) * np.linalg.norm(A)
+
+
+@pytest.mark.parametrize("m,n", [(30, 20), (20, 30)])
+def test_bidiag_quad_agrees(m, n, rng):
+    A = rng.standard_normal((m, n))
+    tru, tl, tr = O.bidiag_quad(A)
+    ref, rl, rr = O.bidiag(A)
+    assert rel(ref, tru) <= 1e-11 and rel(rl, tl) <= 1e-11 and rel(rr, tr) <= 1e-11
+
