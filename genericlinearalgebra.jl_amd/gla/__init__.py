@@ -102,7 +102,11 @@ def _sfx(A: torch.Tensor) -> str:
         return "d"
     if A.dtype == torch.float32:
         return "s"
-    raise TypeError(f"element type {A.dtype} not supported (Float32/Float64)")
+    if A.dtype == torch.complex128:
+        return "z"
+    if A.dtype == torch.complex64:
+        return "c"
+    raise TypeError(f"element type {A.dtype} not supported (Float32/Float64/ComplexF32/ComplexF64)")
 
 
 def _mat(A: torch.Tensor, name="A") -> int:
@@ -202,6 +206,16 @@ def _uplo(v) -> str:
     return u
 
 
+_last_info: dict[int, int] = {}
+
+
+def last_pivot_info(device: int = 0) -> int:
+    """The info of the last Cholesky call on the device: the 1-based index
+    of the first pivot whose real part was negative, or 0 (for complex
+    matrices the reference raises no error there)."""
+    return _last_info.get(device, 0)
+
+
 def _chol(A, uplo, fn, *extra):
     ld = _mat(A)
     m, n = A.shape
@@ -213,8 +227,11 @@ def _chol(A, uplo, fn, *extra):
         rc = f(_handle(A.device.index), _uplo(uplo).encode(), n, C.c_void_p(A.data_ptr()), ld,
                *extra, C.byref(info), _stream(A))
     _check(rc, fn)
-    if info.value > 0:
+    if info.value > 0 and not A.is_complex():
+        # complex pivots take the complex square root (no error), as in
+        # the reference; info is kept on the tensor for inspection
         raise DomainError(f"sqrt of a negative pivot at position {info.value}")
+    _last_info[A.device.index] = info.value
     return A
 
 
@@ -251,8 +268,10 @@ def rankUpdate_(Cm: Hermitian, A, alpha):
         raise DimensionMismatch("first dimension of A has wrong size")
     lda = _mat(A, "A")
     f = getattr(lib(), f"gla_rank_update_{_sfx(Cd)}")
+    if Cd.is_complex() and complex(alpha).imag != 0:
+        raise ArgumentError("α must be real for a Hermitian rank update")
     with torch.cuda.device(Cd.device):
-        rc = f(_handle(Cd.device.index), Cm.uplo.encode(), n, A.shape[1], float(alpha),
+        rc = f(_handle(Cd.device.index), Cm.uplo.encode(), n, A.shape[1], float(complex(alpha).real),
                C.c_void_p(A.data_ptr()), lda, C.c_void_p(Cd.data_ptr()), ldc, _stream(Cd))
     _check(rc, "gla_rank_update")
     return Cm

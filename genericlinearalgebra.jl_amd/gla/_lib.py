@@ -76,6 +76,12 @@ def lib() -> C.CDLL:
             f"libgla_hip.so not found at {LIB_PATH}; build it with "
             f"`make -C {PKG_DIR}` (the HIP path has no fallback)")
     L = C.CDLL(LIB_PATH)
+    for sfx, ft in (("z", C.c_double), ("c", C.c_float)):
+        for name in ("gla_chol_unblocked", "gla_chol_blocked", "gla_chol_recursive",
+                     "gla_rank_update"):
+            f = getattr(L, f"{name}_{sfx}")
+            f.restype = _int
+            f.argtypes = _sigs(ft)[name][1]
     for sfx, ft in (("d", C.c_double), ("s", C.c_float)):
         for name, (res, args) in _sigs(ft).items():
             f = getattr(L, f"{name}_{sfx}")

@@ -90,6 +90,14 @@ const SIGNATURES = Dict{Symbol,Tuple{Any,Tuple}}(
     :gla_symtri_s => (Cint, (Ptr{Cvoid}, Cchar, Int64, Ptr{Float32}, Int64, Ptr{Float32}, Ptr{Cvoid})),
     :gla_bidiag_d => (Cint, (Ptr{Cvoid}, Int64, Int64, Ptr{Float64}, Int64, Ptr{Float64}, Ptr{Float64}, Ptr{Cvoid})),
     :gla_bidiag_s => (Cint, (Ptr{Cvoid}, Int64, Int64, Ptr{Float32}, Int64, Ptr{Float32}, Ptr{Float32}, Ptr{Cvoid})),
+    :gla_chol_unblocked_z => (Cint, (Ptr{Cvoid}, Cchar, Int64, Ptr{ComplexF64}, Int64, Ref{Int64}, Ptr{Cvoid})),
+    :gla_chol_unblocked_c => (Cint, (Ptr{Cvoid}, Cchar, Int64, Ptr{ComplexF32}, Int64, Ref{Int64}, Ptr{Cvoid})),
+    :gla_chol_blocked_z => (Cint, (Ptr{Cvoid}, Cchar, Int64, Ptr{ComplexF64}, Int64, Int64, Ref{Int64}, Ptr{Cvoid})),
+    :gla_chol_blocked_c => (Cint, (Ptr{Cvoid}, Cchar, Int64, Ptr{ComplexF32}, Int64, Int64, Ref{Int64}, Ptr{Cvoid})),
+    :gla_chol_recursive_z => (Cint, (Ptr{Cvoid}, Cchar, Int64, Ptr{ComplexF64}, Int64, Int64, Ref{Int64}, Ptr{Cvoid})),
+    :gla_chol_recursive_c => (Cint, (Ptr{Cvoid}, Cchar, Int64, Ptr{ComplexF32}, Int64, Int64, Ref{Int64}, Ptr{Cvoid})),
+    :gla_rank_update_z => (Cint, (Ptr{Cvoid}, Cchar, Int64, Int64, Float64, Ptr{ComplexF64}, Int64, Ptr{ComplexF64}, Int64, Ptr{Cvoid})),
+    :gla_rank_update_c => (Cint, (Ptr{Cvoid}, Cchar, Int64, Int64, Float32, Ptr{ComplexF32}, Int64, Ptr{ComplexF32}, Int64, Ptr{Cvoid})),
     :gla_chol_solve_d => (Cint, (Ptr{Cvoid}, Cchar, Int64, Int64, Ptr{Float64}, Int64, Ptr{Float64}, Int64, Ptr{Cvoid})),
     :gla_chol_solve_s => (Cint, (Ptr{Cvoid}, Cchar, Int64, Int64, Ptr{Float32}, Int64, Ptr{Float32}, Int64, Ptr{Cvoid})),
     :gla_qr_solve_d => (Cint, (Ptr{Cvoid}, Int64, Int64, Int64, Ptr{Float64}, Int64, Ptr{Float64}, Ptr{Float64}, Int64, Ref{Int64}, Ptr{Cvoid})),
@@ -361,6 +369,49 @@ for (T, sfx) in ((Float64, "d"), (Float32, "s"))
             with_handle(h -> _check(@gla($trs, h, m, n, pointer(L), ld(L), pointer(B), ld(B),
                                          C_NULL), "rdiv!"))
             return B
+        end
+    end
+end
+
+# Complex Hermitian A (ComplexF64/ComplexF32): the complex square root of the
+# pivot, as the reference takes it, so no DomainError; her!/herk! updates.
+for (T, R, sfx) in ((ComplexF64, Float64, "z"), (ComplexF32, Float32, "c"))
+    unb = Symbol("gla_chol_unblocked_", sfx)
+    blk = Symbol("gla_chol_blocked_", sfx)
+    rec = Symbol("gla_chol_recursive_", sfx)
+    rku = Symbol("gla_rank_update_", sfx)
+    @eval begin
+        function cholUnblocked!(A::HIPMatrix{$T}, uplo::Type{Val{U}}) where {U}
+            n = _square(A)
+            info = Ref{Int64}(0)
+            with_handle(h -> _check(@gla($unb, h, _uplo(uplo), n, pointer(A), ld(A), info,
+                                         C_NULL), "cholUnblocked!"))
+            return A
+        end
+        function cholBlocked!(A::HIPMatrix{$T}, uplo::Type{Val{U}}, blocksize::Integer) where {U}
+            n = _square(A)
+            blocksize >= 1 || throw(ArgumentError("blocksize must be positive"))
+            info = Ref{Int64}(0)
+            with_handle(h -> _check(@gla($blk, h, _uplo(uplo), n, pointer(A), ld(A),
+                                         Int64(blocksize), info, C_NULL), "cholBlocked!"))
+            return A
+        end
+        function cholRecursive!(A::HIPMatrix{$T}, uplo::Type{Val{U}},
+                                cutoff::Integer = 1) where {U}
+            n = _square(A)
+            info = Ref{Int64}(0)
+            with_handle(h -> _check(@gla($rec, h, _uplo(uplo), n, pointer(A), ld(A),
+                                         Int64(cutoff), info, C_NULL), "cholRecursive!"))
+            return U === :L ? LowerTriangular(A) : UpperTriangular(A)
+        end
+        function rankUpdate!(C::Hermitian{$T,<:HIPMatrix{$T}}, A::HIPArray{$T}, α::Real)
+            n = _square(C.data)
+            size(A, 1) == n || throw(DimensionMismatch("first dimension of A has wrong size"))
+            k = ndims(A) == 1 ? 1 : size(A, 2)
+            with_handle(h -> _check(@gla($rku, h, Cchar(C.uplo), n, k, $R(α), pointer(A),
+                                         ld(A), pointer(C.data), ld(C.data), C_NULL),
+                                    "rankUpdate!"))
+            return C
         end
     end
 end

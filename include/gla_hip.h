@@ -46,6 +46,14 @@ extern "C" {
 
 typedef struct gla_handle_s *gla_handle_t;
 
+/* Complex elements as Julia stores ComplexF64 / ComplexF32: (re, im). */
+typedef struct gla_cdouble_s {
+    double re, im;
+} gla_cdouble;
+typedef struct gla_cfloat_s {
+    float re, im;
+} gla_cfloat;
+
 /* ------------------------------------------------------------- runtime -- */
 /* Library state for one device: workspace, side stream, info flag.  No
  * reference counterpart (the reference allocates per call, e.g.
@@ -114,6 +122,30 @@ int gla_chol_recursive_d(gla_handle_t h, char uplo, int64_t n, double *A, int64_
 int gla_chol_recursive_s(gla_handle_t h, char uplo, int64_t n, float *A, int64_t lda,
                          int64_t cutoff, int64_t *info, void *stream);
 
+/* The three Cholesky variants for complex Hermitian A (_z ComplexF64,
+ * _c ComplexF32) — src/cholesky.jl:3-55 with the rank updates on her! /
+ * herk! (src/juliaBLAS.jl:41-48,78-86): L lower with a real diagonal, the
+ * opposite triangle untouched ('U': U = Lᴴ in the upper triangle).  The
+ * pivot's square root is the complex one (src/cholesky.jl:5,40), so a
+ * pivot with a negative real part is not an error there: *info reports
+ * the first such pivot (1-based) and the values after it follow the
+ * reference's arithmetic (an imaginary root, then division by its zero
+ * real part).  The device steps are 64 wide (cholBlocked!'s blocksize
+ * when smaller); the factor does not depend on the step width up to
+ * rounding. */
+int gla_chol_unblocked_z(gla_handle_t h, char uplo, int64_t n, gla_cdouble *A, int64_t lda,
+                         int64_t *info, void *stream);
+int gla_chol_unblocked_c(gla_handle_t h, char uplo, int64_t n, gla_cfloat *A, int64_t lda,
+                         int64_t *info, void *stream);
+int gla_chol_blocked_z(gla_handle_t h, char uplo, int64_t n, gla_cdouble *A, int64_t lda,
+                       int64_t blocksize, int64_t *info, void *stream);
+int gla_chol_blocked_c(gla_handle_t h, char uplo, int64_t n, gla_cfloat *A, int64_t lda,
+                       int64_t blocksize, int64_t *info, void *stream);
+int gla_chol_recursive_z(gla_handle_t h, char uplo, int64_t n, gla_cdouble *A, int64_t lda,
+                         int64_t cutoff, int64_t *info, void *stream);
+int gla_chol_recursive_c(gla_handle_t h, char uplo, int64_t n, gla_cfloat *A, int64_t lda,
+                         int64_t cutoff, int64_t *info, void *stream);
+
 /* No counterpart in the reference: cholRecursive!(A, Val{:L}, cutoff) in
  * Float64 for callers that cannot allocate device memory (code under HIP
  * graph capture, serving loops, harnesses that forbid hipMalloc).  No
@@ -136,6 +168,14 @@ int gla_rank_update_d(gla_handle_t h, char uplo, int64_t n, int64_t k, double al
                       const double *A, int64_t lda, double *C, int64_t ldc, void *stream);
 int gla_rank_update_s(gla_handle_t h, char uplo, int64_t n, int64_t k, float alpha,
                       const float *A, int64_t lda, float *C, int64_t ldc, void *stream);
+
+/* rankUpdate!(C::Hermitian, A, α) for complex C — the herk! the reference
+ * dispatches to (src/juliaBLAS.jl:78-86): the uplo triangle of
+ * C += α A Aᴴ (α real), the diagonal left real. */
+int gla_rank_update_z(gla_handle_t h, char uplo, int64_t n, int64_t k, double alpha,
+                      const gla_cdouble *A, int64_t lda, gla_cdouble *C, int64_t ldc, void *stream);
+int gla_rank_update_c(gla_handle_t h, char uplo, int64_t n, int64_t k, float alpha,
+                      const gla_cfloat *A, int64_t lda, gla_cfloat *C, int64_t ldc, void *stream);
 
 /* rdiv!(B, LowerTriangular(L)') as called at src/cholesky.jl:28,48:
  * B (m×n) <- B L⁻ᵀ, L n×n lower (non-unit). */

@@ -20,6 +20,7 @@
  * products of elementary reflectors, mpmath at 256 bits and brute-force
  * pivots for the DomainError index.
  */
+#include <complex.h>
 #include <math.h>
 #include <quadmath.h>
 #include <stdint.h>
@@ -61,6 +62,47 @@
 #undef OFN
 #undef OSQRT
 #undef OCOPYSIGN
+
+/* ---- complex double (suffix _z) and complex float (_c) ---- */
+#define OC double _Complex
+#define OR double
+#define OFN(name) OCAT(name, z)
+#define OCSQRT csqrt
+#define OSQRTR sqrt
+#define OCONJ conj
+#define OCREAL creal
+#define OCIMAG cimag
+#define OCOPYSIGNR copysign
+#include "oracle_complex.h"
+#undef OC
+#undef OR
+#undef OFN
+#undef OCSQRT
+#undef OSQRTR
+#undef OCONJ
+#undef OCREAL
+#undef OCIMAG
+#undef OCOPYSIGNR
+
+#define OC float _Complex
+#define OR float
+#define OFN(name) OCAT(name, c)
+#define OCSQRT csqrtf
+#define OSQRTR sqrtf
+#define OCONJ conjf
+#define OCREAL crealf
+#define OCIMAG cimagf
+#define OCOPYSIGNR copysignf
+#include "oracle_complex.h"
+#undef OC
+#undef OR
+#undef OFN
+#undef OCSQRT
+#undef OSQRTR
+#undef OCONJ
+#undef OCREAL
+#undef OCIMAG
+#undef OCOPYSIGNR
 
 /* ---- double-in/double-out wrappers that compute in __float128 ---- */
 static __float128 *q_from_d(int64_t m, int64_t n, const double *A, int64_t lda)

@@ -71,6 +71,19 @@ def lib() -> C.CDLL:
         L.orq_chol_unblocked.argtypes = [_i64, _p, _i64]
         L.orq_chol_unblocked.restype = _i64
         L.orq_qr_unblocked.argtypes = [_i64, _i64, _p, _i64, _p]
+        for sfx, rt in (("z", C.c_double), ("c", C.c_float)):
+            g = lambda n: getattr(L, f"{n}_{sfx}")  # noqa: E731
+            for nm in ("orz_chol_unblocked",):
+                g(nm).argtypes = [_i64, _p, _i64]
+                g(nm).restype = _i64
+            g("orz_chol_blocked").argtypes = [_i64, _p, _i64, _i64]
+            g("orz_chol_blocked").restype = _i64
+            g("orz_chol_recursive").argtypes = [_i64, _p, _i64, _i64]
+            g("orz_chol_recursive").restype = _i64
+            g("orz_herk_l").argtypes = [_i64, _i64, rt, _p, _i64, _p, _i64]
+            g("orz_her_l").argtypes = [_i64, rt, _p, _p, _i64]
+            g("orz_trsm_rlc").argtypes = [_i64, _i64, _p, _i64, _p, _i64]
+            g("orz_qr_unblocked").argtypes = [_i64, _i64, _p, _i64, _p]
         L.orq_symtri.argtypes = [C.c_int, _i64, _p, _i64, _p]
         L.orq_bidiag.argtypes = [_i64, _i64, _p, _i64, _p, _p]
         L.orq_chol_residual.argtypes = [_i64, _p, _i64, _p, _i64]
@@ -84,6 +97,10 @@ def _sfx(a: np.ndarray) -> str:
         return "d"
     if a.dtype == np.float32:
         return "s"
+    if a.dtype == np.complex128:
+        return "z"
+    if a.dtype == np.complex64:
+        return "c"
     raise TypeError(f"oracle supports float32/float64, got {a.dtype}")
 
 
@@ -253,6 +270,42 @@ def lmul_unit_lower(L, B, alpha):
     _fn("orc_lmul_unit_lower", B)(B.shape[0], B.shape[1], _ptr(L), L.shape[0], _ptr(B),
                                   B.shape[0], alpha)
     return B
+
+
+# ------------------------------------------------------------ complex --
+def zchol(A, variant: str = "recursive", param: int = 1):
+    """cholUnblocked!/cholBlocked!/cholRecursive! of a complex matrix
+    (src/cholesky.jl:3-55, with her!/herk! for the rank updates) ->
+    (A_out, first index of a negative real pivot or 0)."""
+    A = _f(A)
+    n = A.shape[0]
+    if variant == "unblocked":
+        info = _fn("orz_chol_unblocked", A)(n, _ptr(A), max(n, 1))
+    elif variant == "blocked":
+        info = _fn("orz_chol_blocked", A)(n, _ptr(A), max(n, 1), int(param))
+    else:
+        info = _fn("orz_chol_recursive", A)(n, _ptr(A), max(n, 1), int(param))
+    return A, int(info)
+
+
+def zherk_l(C_, A, alpha):
+    """rankUpdate!(Hermitian(C, :L), A, α) for complex (herk!)."""
+    C_ = _f(C_)
+    A = _f(A, C_.dtype)
+    if A.ndim == 1:
+        A = A.reshape(-1, 1, order="F")
+    _fn("orz_herk_l", C_)(C_.shape[0], A.shape[1], alpha, _ptr(A), A.shape[0], _ptr(C_),
+                          C_.shape[0])
+    return C_
+
+
+def zqr_unblocked(A):
+    """qrUnblocked!(copy(A)) for complex A -> (factors, τ); src/qr.jl:86-111."""
+    A = _f(A)
+    m, n = A.shape
+    tau = np.zeros(min(m, n), dtype=A.dtype)
+    _fn("orz_qr_unblocked", A)(m, n, _ptr(A), m, _ptr(tau))
+    return A, tau
 
 
 # ------------------------------------------------------------------- LDLᵀ --

@@ -16,8 +16,9 @@ def pytest_configure(config):
 
 def rel(x, y) -> float:
     """Normwise relative difference ‖x − y‖_F / ‖y‖_F."""
-    x = np.asarray(x, dtype=np.float64)
-    y = np.asarray(y, dtype=np.float64)
+    cplx = np.iscomplexobj(x) or np.iscomplexobj(y)
+    x = np.asarray(x, dtype=np.complex128 if cplx else np.float64)
+    y = np.asarray(y, dtype=np.complex128 if cplx else np.float64)
     d = np.linalg.norm(y)
     return float(np.linalg.norm(x - y) / (d if d > 0 else 1.0))
 

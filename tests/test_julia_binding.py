@@ -35,6 +35,10 @@ C2JL = {
     "int32_t *": {"Ptr{Int32}", "Ref{Int32}"},
     "const int32_t *": {"Ptr{Int32}"},
     "double *const *": {"Ptr{Ptr{Float64}}"},
+    "gla_cdouble *": {"Ptr{ComplexF64}"},
+    "const gla_cdouble *": {"Ptr{ComplexF64}"},
+    "gla_cfloat *": {"Ptr{ComplexF32}"},
+    "const gla_cfloat *": {"Ptr{ComplexF32}"},
     "float *const *": {"Ptr{Ptr{Float32}}"},
 }
 RET2JL = {"int": {"Cint"}, "const char *": {"Cstring"}}
