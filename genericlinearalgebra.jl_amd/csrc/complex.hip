@@ -404,6 +404,461 @@ int zherk_entry(gla_handle_t hh, char uplo, int64_t n, int64_t k, T alpha, const
                                                  0);
 }
 
+
+// ------------------------------------------------------------- complex QR --
+// qrUnblocked! (src/qr.jl:86-111) with the stdlib's complex reflector! and
+// reflectorApply! (β = −copysign(‖x‖, real(x₁)), τ complex, Hᴴ = I − τ̄ v vᴴ
+// applied from the left), on one block column: per column, every
+// workgroup forms the reflector's scalars from the norm partials, writes
+// its rows of v and the partial dots Σ conj(v_i) A[i, q] of the columns
+// right of it (zqr_reflect_kernel); then each workgroup sums those dots in
+// a fixed order and applies A[i, q] −= v_i τ̄ d_q to its rows, adding the
+// next column's norm partials (zqr_apply_kernel).
+constexpr int kZQ = 32;     // widest block column of one panel pass
+
+template <typename T>
+__device__ __forceinline__ T zwave_sum(T v)
+{
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o);
+    return v;
+}
+
+template <typename T>
+__device__ T zblock_sum(const T *part, int cnt, T *sh)
+{
+    T s = T(0);
+    for (int i = threadIdx.x; i < cnt; i += blockDim.x) s += part[i];
+    s = zwave_sum(s);
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = s;
+    __syncthreads();
+    T r = sh[0] + sh[1] + sh[2] + sh[3];
+    __syncthreads();
+    return r;
+}
+
+// ‖A[j:m, j]‖² partials of the block column's first column.
+template <typename T>
+__global__ void zqr_norm_kernel(int64_t m, int j, const T *A, int64_t lda, T *part)
+{
+    __shared__ T sh[4];
+    const int64_t i = j + (int64_t)blockIdx.x * 256 + threadIdx.x;
+    T s = T(0);
+    if (i < m) {
+        const T *a = A + 2 * (i + (int64_t)j * lda);
+        s = a[0] * a[0] + a[1] * a[1];
+    }
+    s = zwave_sum(s);
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) part[blockIdx.x] = sh[0] + sh[1] + sh[2] + sh[3];
+}
+
+// Column j of the m×w block column: the reflector (v into A below the
+// diagonal, β on it, τ into tau[j]) and the partials dpart[blk·kZQ + q] =
+// Σ_i conj(v_i) A[i, q] over this workgroup's rows, q = j+1..w−1.
+template <typename T>
+__global__ __launch_bounds__(256) void zqr_reflect_kernel(int64_t m, int w, int j, T *A,
+                                                          int64_t lda, const T *part, int pcnt,
+                                                          T *tau, T *dpart, T *scal)
+{
+    __shared__ T sh[4];
+    __shared__ T red[4][kZQ][2];
+    const T nrm2 = zblock_sum(part, pcnt, sh);
+    const T *a1 = A + 2 * (j + (int64_t)j * lda);
+    const T x1r = a1[0], x1i = a1[1];
+    const T normu = sqrt(nrm2);
+    T xir = T(1), xii = T(0), taur = T(0), taui = T(0), beta = x1r;
+    bool zero = normu == T(0);
+    if (!zero) {
+        const T nu = copysign(normu, x1r);
+        xir = x1r + nu;
+        xii = x1i;
+        taur = xir / nu;
+        taui = xii / nu;
+        beta = -nu;
+    }
+    const int64_t i = j + (int64_t)blockIdx.x * 256 + threadIdx.x;
+    // v_i = x_i / ξ₁ (complex division), v_j = 1
+    T vr = T(0), vi = T(0);
+    if (i < m) {
+        if (i == j) {
+            vr = T(1);
+        } else {
+            const T *a = A + 2 * (i + (int64_t)j * lda);
+            if (zero) {
+                vr = a[0];
+                vi = a[1];
+            } else {
+                const T den = xir * xir + xii * xii;
+                vr = (a[0] * xir + a[1] * xii) / den;
+                vi = (a[1] * xir - a[0] * xii) / den;
+            }
+        }
+    }
+    // dots with the remaining columns (before writing v: column j is not among them)
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    for (int q = j + 1; q < w; ++q) {
+        T dr = T(0), di = T(0);
+        if (i < m) {
+            const T *b = A + 2 * (i + (int64_t)q * lda);
+            dr = vr * b[0] + vi * b[1];       // conj(v) · b
+            di = vr * b[1] - vi * b[0];
+        }
+        dr = zwave_sum(dr);
+        di = zwave_sum(di);
+        if (lane == 0) {
+            red[wv][q][0] = dr;
+            red[wv][q][1] = di;
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x < w && (int)threadIdx.x > j) {
+        const int q = threadIdx.x;
+        dpart[2 * ((int64_t)blockIdx.x * kZQ + q)] = red[0][q][0] + red[1][q][0] + red[2][q][0] + red[3][q][0];
+        dpart[2 * ((int64_t)blockIdx.x * kZQ + q) + 1] =
+            red[0][q][1] + red[1][q][1] + red[2][q][1] + red[3][q][1];
+    }
+    // v below the diagonal (each workgroup its own rows); A[j, j] is read
+    // by every workgroup as x₁, so β goes there in the apply kernel
+    if (i < m && i > j && !zero) {
+        T *a = A + 2 * (i + (int64_t)j * lda);
+        a[0] = vr;
+        a[1] = vi;
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        tau[2 * j] = taur;
+        tau[2 * j + 1] = taui;
+        scal[0] = taur;
+        scal[1] = taui;
+        scal[2] = beta;
+        scal[3] = zero ? x1i : T(0);
+    }
+}
+
+// A[i, q] −= v_i · conj(τ) d_q for rows i ≥ j, q = j+1..w−1 (d_q summed over
+// the reflect kernel's partials in a fixed order); β into A[j, j]; partial
+// ‖A[j+1:m, j+1]‖².
+template <typename T>
+__global__ __launch_bounds__(256) void zqr_apply_kernel(int64_t m, int w, int j, T *A, int64_t lda,
+                                                        const T *dpart, int dcnt, const T *scal,
+                                                        T *part)
+{
+    __shared__ T sh[4];
+    __shared__ T dq[kZQ][2];
+    const T taur = scal[0], taui = scal[1];
+    if (threadIdx.x < w && (int)threadIdx.x > j) {
+        const int q = threadIdx.x;
+        T dr = T(0), di = T(0);
+        for (int b = 0; b < dcnt; ++b) {
+            dr += dpart[2 * ((int64_t)b * kZQ + q)];
+            di += dpart[2 * ((int64_t)b * kZQ + q) + 1];
+        }
+        // conj(τ) d
+        dq[q][0] = taur * dr + taui * di;
+        dq[q][1] = taur * di - taui * dr;
+    }
+    __syncthreads();
+    const int64_t i = j + (int64_t)blockIdx.x * 256 + threadIdx.x;
+    T s = T(0);
+    if (i < m) {
+        T vr = T(1), vi = T(0);
+        if (i > j) {
+            const T *a = A + 2 * (i + (int64_t)j * lda);
+            vr = a[0];
+            vi = a[1];
+        }
+        for (int q = j + 1; q < w; ++q) {
+            T *b = A + 2 * (i + (int64_t)q * lda);
+            const T cr = dq[q][0], ci = dq[q][1];
+            b[0] -= vr * cr - vi * ci;
+            b[1] -= vr * ci + vi * cr;
+            if (q == j + 1 && i > j) s = b[0] * b[0] + b[1] * b[1];
+        }
+        if (i == j) {
+            T *a = A + 2 * (i + (int64_t)j * lda);
+            a[0] = scal[2];
+            a[1] = scal[3];
+        }
+    }
+    s = zwave_sum(s);
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) part[blockIdx.x] = sh[0] + sh[1] + sh[2] + sh[3];
+}
+
+// qrUnblocked! of the m×w block column at A (w ≤ kZQ).
+template <typename T>
+int zqr_panel(hipStream_t s, int64_t m, int w, T *A, int64_t lda, T *tau, T *ws)
+{
+    const int64_t g = cdiv(m, 256);
+    T *part = ws, *dpart = part + g + 8, *scal = dpart + 2 * g * kZQ;   // scal: τ, β
+    hipLaunchKernelGGL((zqr_norm_kernel<T>), dim3((unsigned)g), dim3(256), 0, s, m, 0,
+                       (const T *)A, lda, part);
+    GLA_LAUNCH_CHECK();
+    int pcnt = (int)g;
+    for (int j = 0; j < w && j < m; ++j) {
+        const unsigned gj = (unsigned)cdiv(m - j, 256);
+        hipLaunchKernelGGL((zqr_reflect_kernel<T>), dim3(gj), dim3(256), 0, s, m, w, j, A, lda,
+                           (const T *)part, pcnt, tau, dpart, scal);
+        GLA_LAUNCH_CHECK();
+        hipLaunchKernelGGL((zqr_apply_kernel<T>), dim3(gj), dim3(256), 0, s, m, w, j, A, lda,
+                           (const T *)dpart, (int)gj, (const T *)scal, part);
+        GLA_LAUNCH_CHECK();
+        pcnt = (int)gj;
+    }
+    return GLA_OK;
+}
+
+// V (m×k, ld ldv): the unit lower reflectors of F explicitly (ones on the
+// diagonal, zeros above).
+template <typename T>
+__global__ void zmake_v_kernel(int64_t m, int64_t k, const T *F, int64_t ldf, T *V, int64_t ldv)
+{
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    const int64_t c = blockIdx.y;
+    if (i >= m || c >= k) return;
+    T *v = V + 2 * (i + c * ldv);
+    if (i < c) {
+        v[0] = T(0);
+        v[1] = T(0);
+    } else if (i == c) {
+        v[0] = T(1);
+        v[1] = T(0);
+    } else {
+        v[0] = F[2 * (i + c * ldf)];
+        v[1] = F[2 * (i + c * ldf) + 1];
+    }
+}
+
+// T (k×k, k ≤ 64) from G = VᴴV (k×k, ld ldg) and τ: S = I + diag(τ)
+// striu(G), T = S⁻¹ diag(τ) (the UT transform of src/qr.jl:64-83 with the
+// first term of each entry conjugated, so that I − V T Vᴴ = H₁⋯H_k).
+// One workgroup; thread c computes column c of S⁻¹ by back substitution.
+template <typename T>
+__global__ void zqr_tsmall_kernel(int k, const T *G, int64_t ldg, const T *tau, T *Tm, int64_t ldt)
+{
+    __shared__ T Xr[kZB * kZB], Xi[kZB * kZB];
+    const int c = threadIdx.x;
+    if (c < k) {
+        // column c of S⁻¹ (unit upper): x_c = 1, x_r = −Σ_{l=r+1..c} S[r,l] x_l for r < c
+        for (int r = kZB - 1; r >= 0; --r) {
+            T xr = T(0), xi = T(0);
+            if (r == c) xr = T(1);
+            else if (r < c) {
+                T sr = T(0), si = T(0);
+                for (int l = r + 1; l <= c; ++l) {
+                    // S[r,l] = τ_r G[r,l]
+                    const T gr = G[2 * (r + (int64_t)l * ldg)], gi = G[2 * (r + (int64_t)l * ldg) + 1];
+                    const T tr = tau[2 * r], ti = tau[2 * r + 1];
+                    const T s_r = tr * gr - ti * gi, s_i = tr * gi + ti * gr;
+                    const T yr = Xr[l * kZB + c], yi = Xi[l * kZB + c];
+                    sr += s_r * yr - s_i * yi;
+                    si += s_r * yi + s_i * yr;
+                }
+                xr = -sr;
+                xi = -si;
+            }
+            Xr[r * kZB + c] = xr;
+            Xi[r * kZB + c] = xi;
+        }
+    }
+    __syncthreads();
+    // T[r, c] = (S⁻¹)[r, c] τ_c for r ≤ c, 0 below
+    for (int e = threadIdx.x; e < k * k; e += blockDim.x) {
+        const int r = e % k, cc = e / k;
+        T vr = T(0), vi = T(0);
+        if (r <= cc) {
+            const T yr = Xr[r * kZB + cc], yi = Xi[r * kZB + cc];
+            const T tr = tau[2 * cc], ti = tau[2 * cc + 1];
+            vr = yr * tr - yi * ti;
+            vi = yr * ti + yi * tr;
+        }
+        Tm[2 * (r + (int64_t)cc * ldt)] = vr;
+        Tm[2 * (r + (int64_t)cc * ldt) + 1] = vi;
+    }
+}
+
+// S[0:rows, 0:gridDim.y] = 0 (complex, ld lds).
+template <typename T>
+__global__ void zzero_kernel(int64_t rows, T *S, int64_t lds)
+{
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    const int64_t j = blockIdx.y;
+    if (i < rows) {
+        S[2 * (i + j * lds)] = T(0);
+        S[2 * (i + j * lds) + 1] = T(0);
+    }
+}
+
+// T (k×k) of the first k reflectors of the explicit V (m×k), recursively:
+// T = [T11 T12; 0 T22], T12 = −T11 (V1ᴴ V2) T22 (the blocked recurrence of
+// the same UT transform).  scratch: 2·64² + k·k complex elements.
+template <typename T>
+int zqr_tblock_dev(hipStream_t s, int64_t m, int64_t k, const T *V, int64_t ldv, const T *tau,
+                   T *Tm, int64_t ldt, T *scratch)
+{
+    if (k <= kZB) {
+        T *G = scratch;
+        GLA_TRY((zgemm_launch<T, true, false, 0, false>(s, k, k, m, T(1), V, ldv, V, ldv, T(0), G,
+                                                         k, 0)));
+        hipLaunchKernelGGL((zqr_tsmall_kernel<T>), dim3(1), dim3(kZB), 0, s, (int)k,
+                           (const T *)G, k, tau, Tm, ldt);
+        GLA_LAUNCH_CHECK();
+        return GLA_OK;
+    }
+    const int64_t k1 = (k / 2 + kZB - 1) / kZB * kZB;
+    GLA_TRY(zqr_tblock_dev<T>(s, m, k1, V, ldv, tau, Tm, ldt, scratch));
+    GLA_TRY(zqr_tblock_dev<T>(s, m - k1, k - k1, V + 2 * (k1 + k1 * ldv), ldv, tau + 2 * k1,
+                              Tm + 2 * (k1 + k1 * ldt), ldt, scratch));
+    // G = V1ᴴ V2 over rows k1..m (V2 is zero above), P = T11 G, T12 = −P T22
+    const int64_t k2 = k - k1;
+    T *Gm = scratch, *P = Gm + 2 * k1 * k2;
+    GLA_TRY((zgemm_launch<T, true, false, 0, false>(s, k1, k2, m - k1, T(1), V + 2 * k1, ldv,
+                                                     V + 2 * (k1 + k1 * ldv), ldv, T(0), Gm, k1,
+                                                     0)));
+    GLA_TRY((zgemm_launch<T, false, false, 0, false>(s, k1, k2, k1, T(1), Tm, ldt, Gm, k1, T(0), P,
+                                                      k1, 0)));
+    GLA_TRY((zgemm_launch<T, false, false, 0, false>(s, k1, k2, k2, T(-1), P, k1,
+                                                      Tm + 2 * (k1 + k1 * ldt), ldt, T(0),
+                                                      Tm + 2 * k1 * ldt, ldt, 0)));
+    // T is upper triangular: the block below T11 is zero
+    hipLaunchKernelGGL((zzero_kernel<T>), dim3((unsigned)cdiv(k2, 256), (unsigned)k1), dim3(256),
+                       0, s, k2, Tm + 2 * k1, ldt);
+    GLA_LAUNCH_CHECK();
+    return GLA_OK;
+}
+
+// A (mA×nA) <- op(H) A, H = I − V T Vᴴ (V mA×b explicit, T b×b):
+// W1 = Vᴴ A, W2 = op(T) W1 (op = ᴴ for trans 'C', none for 'N'), A −= V W2.
+template <typename T>
+int zwy_apply_dev(hipStream_t s, char trans, int64_t mA, int64_t nA, int64_t b, const T *V,
+                  int64_t ldv, const T *Tm, int64_t ldt, T *A, int64_t lda, T *W1, T *W2)
+{
+    if (mA <= 0 || nA <= 0 || b <= 0) return GLA_OK;
+    GLA_TRY((zgemm_launch<T, true, false, 0, false>(s, b, nA, mA, T(1), V, ldv, A, lda, T(0), W1,
+                                                     b, 0)));
+    if (trans == 'C')
+        GLA_TRY((zgemm_launch<T, true, false, 0, false>(s, b, nA, b, T(1), Tm, ldt, W1, b, T(0),
+                                                         W2, b, 0)));
+    else
+        GLA_TRY((zgemm_launch<T, false, false, 0, false>(s, b, nA, b, T(1), Tm, ldt, W1, b, T(0),
+                                                          W2, b, 0)));
+    return zgemm_launch<T, false, false, 0, false>(s, mA, nA, b, T(-1), V, ldv, W2, b, T(1), A,
+                                                   lda, 0);
+}
+
+// qrBlocked! (src/qr.jl:113-146) for complex A: block columns w =
+// min(blocksize, 32) wide, each factored by zqr_panel, its T formed and
+// Hᴴ applied to the columns right of it.
+template <typename T>
+int zqr_blocked_dev(hipStream_t s, int64_t m, int64_t n, T *A, int64_t lda, int64_t bs, T *tau,
+                    T *ws)
+{
+    const int64_t k = std::min(m, n);
+    const int w0 = (int)std::min<int64_t>(std::max<int64_t>(bs, 1), kZQ);
+    const int64_t g = cdiv(m, 256);
+    T *pan = ws;                                       // panel scratch
+    T *V = pan + 2 * (g + 8 + 2 * g * kZQ + 64);       // m × w0
+    T *Tm = V + 2 * m * w0;                            // w0 × w0
+    T *sc = Tm + 2 * kZQ * kZQ;                        // tblock scratch
+    T *W1 = sc + 2 * (2 * kZB * kZB + kZQ * kZQ), *W2 = W1 + 2 * (int64_t)w0 * n;
+    for (int64_t off = 0; off < k; off += w0) {
+        const int w = (int)std::min<int64_t>(w0, n - off);
+        const int64_t mm = m - off;
+        T *Ab = A + 2 * (off + off * lda);
+        GLA_TRY(zqr_panel<T>(s, mm, w, Ab, lda, tau + 2 * off, pan));
+        const int64_t rest = n - off - w;
+        if (rest > 0) {
+            const int kb = (int)std::min<int64_t>(mm, w);
+            hipLaunchKernelGGL((zmake_v_kernel<T>), dim3((unsigned)cdiv(mm, 256), (unsigned)kb),
+                               dim3(256), 0, s, mm, (int64_t)kb, (const T *)Ab, lda, V, mm);
+            GLA_LAUNCH_CHECK();
+            GLA_TRY(zqr_tblock_dev<T>(s, mm, kb, V, mm, tau + 2 * off, Tm, kZQ, sc));
+            GLA_TRY(zwy_apply_dev<T>(s, 'C', mm, rest, kb, V, mm, Tm, kZQ, Ab + 2 * w * lda, lda,
+                                     W1, W2));
+        }
+    }
+    return GLA_OK;
+}
+
+size_t zqr_scratch(int64_t m, int64_t n)
+{
+    const int64_t g = cdiv(m, 256);
+    return 2 * ((size_t)(g + 8 + 2 * g * kZQ + 64) + (size_t)m * kZQ + kZQ * kZQ +
+                (2 * kZB * kZB + kZQ * kZQ) + 2 * (size_t)kZQ * n) + 256;
+}
+
+template <typename T>
+int zqr_entry(gla_handle_t hh, int64_t m, int64_t n, void *Av, int64_t lda, int64_t bs,
+              void *tauv, void *stream)
+{
+    Handle *h = reinterpret_cast<Handle *>(hh);
+    if (!h) { set_error("null handle"); return -1; }
+    if (m < 0) return -2;
+    if (n < 0) return -3;
+    T *A = reinterpret_cast<T *>(Av);
+    T *tau = reinterpret_cast<T *>(tauv);
+    if (!A && m > 0 && n > 0) { set_error("A is null"); return -4; }
+    if (lda < std::max<int64_t>(1, m)) { set_error("lda < max(1,m)"); return -5; }
+    if (bs < 1) { set_error("blocksize < 1"); return -6; }
+    if (!tau && std::min(m, n) > 0) { set_error("tau is null"); return -7; }
+    if (std::min(m, n) == 0) return GLA_OK;
+    GLA_HIP_TRY(hipSetDevice(h->device));
+    GLA_TRY(ensure_workspace(h, zqr_scratch(m, n) * sizeof(T)));
+    return zqr_blocked_dev<T>(reinterpret_cast<hipStream_t>(stream), m, n, A, lda, bs, tau,
+                              reinterpret_cast<T *>(h->ws));
+}
+
+template <typename T>
+int ztblock_entry(gla_handle_t hh, int64_t m, int64_t n, const void *Fv, int64_t ldf,
+                  const void *tauv, void *Tv, int64_t ldt, void *stream)
+{
+    Handle *h = reinterpret_cast<Handle *>(hh);
+    if (!h) { set_error("null handle"); return -1; }
+    const int64_t k = std::min(m, n);
+    if (m < 0 || n < 0) return -2;
+    if (ldf < std::max<int64_t>(1, m)) return -5;
+    if (ldt < std::max<int64_t>(1, k)) return -8;
+    if (k == 0) return GLA_OK;
+    GLA_HIP_TRY(hipSetDevice(h->device));
+    const size_t need = 2 * ((size_t)m * k + 2 * kZB * kZB + (size_t)k * k) + 256;
+    GLA_TRY(ensure_workspace(h, need * sizeof(T)));
+    T *V = reinterpret_cast<T *>(h->ws), *sc = V + 2 * m * k;
+    hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+    hipLaunchKernelGGL((zmake_v_kernel<T>), dim3((unsigned)cdiv(m, 256), (unsigned)k), dim3(256),
+                       0, s, m, k, reinterpret_cast<const T *>(Fv), ldf, V, m);
+    GLA_LAUNCH_CHECK();
+    return zqr_tblock_dev<T>(s, m, k, V, m, reinterpret_cast<const T *>(tauv),
+                             reinterpret_cast<T *>(Tv), ldt, sc);
+}
+
+template <typename T>
+int zwy_entry(gla_handle_t hh, char trans, int64_t mA, int64_t nA, const void *Vv, int64_t ldv,
+              int64_t nV, const void *Tv, int64_t ldt, void *Av, int64_t lda, void *stream)
+{
+    Handle *h = reinterpret_cast<Handle *>(hh);
+    if (!h) { set_error("null handle"); return -1; }
+    if (trans != 'N' && trans != 'C') { set_error("trans must be 'N' or 'C'"); return -2; }
+    if (mA < 0) return -3;
+    if (nA < 0) return -4;
+    if (ldv < std::max<int64_t>(1, mA)) return -6;
+    const int64_t b = std::min(mA, nV);
+    if (ldt < std::max<int64_t>(1, b)) return -9;
+    if (lda < std::max<int64_t>(1, mA)) return -11;
+    if (b == 0 || nA == 0) return GLA_OK;
+    GLA_HIP_TRY(hipSetDevice(h->device));
+    const size_t need = 2 * ((size_t)mA * b + 2 * (size_t)b * nA) + 256;
+    GLA_TRY(ensure_workspace(h, need * sizeof(T)));
+    T *V = reinterpret_cast<T *>(h->ws), *W1 = V + 2 * mA * b, *W2 = W1 + 2 * b * nA;
+    hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+    hipLaunchKernelGGL((zmake_v_kernel<T>), dim3((unsigned)cdiv(mA, 256), (unsigned)b), dim3(256),
+                       0, s, mA, b, reinterpret_cast<const T *>(Vv), ldv, V, mA);
+    GLA_LAUNCH_CHECK();
+    return zwy_apply_dev<T>(s, trans, mA, nA, b, V, mA, reinterpret_cast<const T *>(Tv), ldt,
+                            reinterpret_cast<T *>(Av), lda, W1, W2);
+}
+
 }  // namespace
 }  // namespace gla
 
@@ -448,6 +903,49 @@ int gla_rank_update_c(gla_handle_t h, char uplo, int64_t n, int64_t k, float alp
                       const gla_cfloat *A, int64_t lda, gla_cfloat *C, int64_t ldc, void *stream)
 {
     return gla::zherk_entry<float>(h, uplo, n, k, alpha, A, lda, C, ldc, stream);
+}
+
+int gla_qr_unblocked_z(gla_handle_t h, int64_t m, int64_t n, gla_cdouble *A, int64_t lda,
+                       gla_cdouble *tau, void *stream)
+{
+    return gla::zqr_entry<double>(h, m, n, A, lda, 32, tau, stream);
+}
+int gla_qr_unblocked_c(gla_handle_t h, int64_t m, int64_t n, gla_cfloat *A, int64_t lda,
+                       gla_cfloat *tau, void *stream)
+{
+    return gla::zqr_entry<float>(h, m, n, A, lda, 32, tau, stream);
+}
+int gla_qr_blocked_z(gla_handle_t h, int64_t m, int64_t n, gla_cdouble *A, int64_t lda,
+                     int64_t blocksize, gla_cdouble *tau, void *stream)
+{
+    return gla::zqr_entry<double>(h, m, n, A, lda, blocksize, tau, stream);
+}
+int gla_qr_blocked_c(gla_handle_t h, int64_t m, int64_t n, gla_cfloat *A, int64_t lda,
+                     int64_t blocksize, gla_cfloat *tau, void *stream)
+{
+    return gla::zqr_entry<float>(h, m, n, A, lda, blocksize, tau, stream);
+}
+int gla_qr_tblock_z(gla_handle_t h, int64_t m, int64_t n, const gla_cdouble *F, int64_t ldf,
+                    const gla_cdouble *tau, gla_cdouble *T, int64_t ldt, void *stream)
+{
+    return gla::ztblock_entry<double>(h, m, n, F, ldf, tau, T, ldt, stream);
+}
+int gla_qr_tblock_c(gla_handle_t h, int64_t m, int64_t n, const gla_cfloat *F, int64_t ldf,
+                    const gla_cfloat *tau, gla_cfloat *T, int64_t ldt, void *stream)
+{
+    return gla::ztblock_entry<float>(h, m, n, F, ldf, tau, T, ldt, stream);
+}
+int gla_wy_apply_z(gla_handle_t h, char trans, int64_t mA, int64_t nA, const gla_cdouble *V,
+                   int64_t ldv, int64_t nV, const gla_cdouble *T, int64_t ldt, gla_cdouble *A,
+                   int64_t lda, void *stream)
+{
+    return gla::zwy_entry<double>(h, trans, mA, nA, V, ldv, nV, T, ldt, A, lda, stream);
+}
+int gla_wy_apply_c(gla_handle_t h, char trans, int64_t mA, int64_t nA, const gla_cfloat *V,
+                   int64_t ldv, int64_t nV, const gla_cfloat *T, int64_t ldt, gla_cfloat *A,
+                   int64_t lda, void *stream)
+{
+    return gla::zwy_entry<float>(h, trans, mA, nA, V, ldv, nV, T, ldt, A, lda, stream);
 }
 
 }  // extern "C"

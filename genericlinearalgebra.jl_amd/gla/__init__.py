@@ -376,9 +376,14 @@ def lmul_(H, A, M=None, alpha=None):
         ldm, mp = max(b, 1), C.c_void_p(None)
     f = getattr(lib(), f"gla_wy_apply_{_sfx(A)}")
     with torch.cuda.device(A.device):
-        rc = f(_handle(A.device.index), trans.encode(), mA, nA, C.c_void_p(V.data_ptr()), ldv,
-               V.shape[1], C.c_void_p(T.data_ptr()), ldt, C.c_void_p(A.data_ptr()), lda, mp, ldm,
-               _stream(A))
+        if A.is_complex():        # Hᴴ for complex blocks; the workspace is internal
+            rc = f(_handle(A.device.index), ("C" if trans == "T" else "N").encode(), mA, nA,
+                   C.c_void_p(V.data_ptr()), ldv, V.shape[1], C.c_void_p(T.data_ptr()), ldt,
+                   C.c_void_p(A.data_ptr()), lda, _stream(A))
+        else:
+            rc = f(_handle(A.device.index), trans.encode(), mA, nA, C.c_void_p(V.data_ptr()),
+                   ldv, V.shape[1], C.c_void_p(T.data_ptr()), ldt, C.c_void_p(A.data_ptr()), lda,
+                   mp, ldm, _stream(A))
     _check(rc, "gla_wy_apply")
     return A
 
@@ -459,8 +464,14 @@ def qrBlocked_(A, blocksize: int = 12, tau=None, work=None):
         ldw, wp = max(int(blocksize), 1), C.c_void_p(None)
     f = getattr(lib(), f"gla_qr_blocked_{_sfx(A)}")
     with torch.cuda.device(A.device):
-        rc = f(_handle(A.device.index), m, n, C.c_void_p(A.data_ptr()), lda, int(blocksize),
-               C.c_void_p(tau.data_ptr()), wp, ldw, _stream(A))
+        if A.is_complex():
+            if work is not None:
+                raise ArgumentError("the complex qrBlocked! keeps its workspace internal")
+            rc = f(_handle(A.device.index), m, n, C.c_void_p(A.data_ptr()), lda, int(blocksize),
+                   C.c_void_p(tau.data_ptr()), _stream(A))
+        else:
+            rc = f(_handle(A.device.index), m, n, C.c_void_p(A.data_ptr()), lda, int(blocksize),
+                   C.c_void_p(tau.data_ptr()), wp, ldw, _stream(A))
     _check(rc, "gla_qr_blocked")
     return QR2(A, tau)
 

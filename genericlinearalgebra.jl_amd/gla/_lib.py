@@ -78,10 +78,16 @@ def lib() -> C.CDLL:
     L = C.CDLL(LIB_PATH)
     for sfx, ft in (("z", C.c_double), ("c", C.c_float)):
         for name in ("gla_chol_unblocked", "gla_chol_blocked", "gla_chol_recursive",
-                     "gla_rank_update"):
+                     "gla_rank_update", "gla_qr_unblocked", "gla_qr_tblock"):
             f = getattr(L, f"{name}_{sfx}")
             f.restype = _int
             f.argtypes = _sigs(ft)[name][1]
+        f = getattr(L, f"gla_qr_blocked_{sfx}")
+        f.restype = _int
+        f.argtypes = [_p, _i64, _i64, _p, _i64, _i64, _p, _p]
+        f = getattr(L, f"gla_wy_apply_{sfx}")
+        f.restype = _int
+        f.argtypes = [_p, _ch, _i64, _i64, _p, _i64, _i64, _p, _i64, _p, _i64, _p]
     for sfx, ft in (("d", C.c_double), ("s", C.c_float)):
         for name, (res, args) in _sigs(ft).items():
             f = getattr(L, f"{name}_{sfx}")

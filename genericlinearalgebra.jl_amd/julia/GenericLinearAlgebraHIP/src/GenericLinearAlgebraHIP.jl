@@ -98,6 +98,14 @@ const SIGNATURES = Dict{Symbol,Tuple{Any,Tuple}}(
     :gla_chol_recursive_c => (Cint, (Ptr{Cvoid}, Cchar, Int64, Ptr{ComplexF32}, Int64, Int64, Ref{Int64}, Ptr{Cvoid})),
     :gla_rank_update_z => (Cint, (Ptr{Cvoid}, Cchar, Int64, Int64, Float64, Ptr{ComplexF64}, Int64, Ptr{ComplexF64}, Int64, Ptr{Cvoid})),
     :gla_rank_update_c => (Cint, (Ptr{Cvoid}, Cchar, Int64, Int64, Float32, Ptr{ComplexF32}, Int64, Ptr{ComplexF32}, Int64, Ptr{Cvoid})),
+    :gla_qr_unblocked_z => (Cint, (Ptr{Cvoid}, Int64, Int64, Ptr{ComplexF64}, Int64, Ptr{ComplexF64}, Ptr{Cvoid})),
+    :gla_qr_unblocked_c => (Cint, (Ptr{Cvoid}, Int64, Int64, Ptr{ComplexF32}, Int64, Ptr{ComplexF32}, Ptr{Cvoid})),
+    :gla_qr_blocked_z => (Cint, (Ptr{Cvoid}, Int64, Int64, Ptr{ComplexF64}, Int64, Int64, Ptr{ComplexF64}, Ptr{Cvoid})),
+    :gla_qr_blocked_c => (Cint, (Ptr{Cvoid}, Int64, Int64, Ptr{ComplexF32}, Int64, Int64, Ptr{ComplexF32}, Ptr{Cvoid})),
+    :gla_qr_tblock_z => (Cint, (Ptr{Cvoid}, Int64, Int64, Ptr{ComplexF64}, Int64, Ptr{ComplexF64}, Ptr{ComplexF64}, Int64, Ptr{Cvoid})),
+    :gla_qr_tblock_c => (Cint, (Ptr{Cvoid}, Int64, Int64, Ptr{ComplexF32}, Int64, Ptr{ComplexF32}, Ptr{ComplexF32}, Int64, Ptr{Cvoid})),
+    :gla_wy_apply_z => (Cint, (Ptr{Cvoid}, Cchar, Int64, Int64, Ptr{ComplexF64}, Int64, Int64, Ptr{ComplexF64}, Int64, Ptr{ComplexF64}, Int64, Ptr{Cvoid})),
+    :gla_wy_apply_c => (Cint, (Ptr{Cvoid}, Cchar, Int64, Int64, Ptr{ComplexF32}, Int64, Int64, Ptr{ComplexF32}, Int64, Ptr{ComplexF32}, Int64, Ptr{Cvoid})),
     :gla_chol_solve_d => (Cint, (Ptr{Cvoid}, Cchar, Int64, Int64, Ptr{Float64}, Int64, Ptr{Float64}, Int64, Ptr{Cvoid})),
     :gla_chol_solve_s => (Cint, (Ptr{Cvoid}, Cchar, Int64, Int64, Ptr{Float32}, Int64, Ptr{Float32}, Int64, Ptr{Cvoid})),
     :gla_qr_solve_d => (Cint, (Ptr{Cvoid}, Int64, Int64, Int64, Ptr{Float64}, Int64, Ptr{Float64}, Ptr{Float64}, Int64, Ref{Int64}, Ptr{Cvoid})),
@@ -598,6 +606,57 @@ for (T, sfx) in ((Float64, "d"), (Float32, "s"))
         bd = m >= n ? Bidiagonal(diag(H), diag(H, 1), :U) : Bidiagonal(diag(H), diag(H, -1), :L)
         return GenericLinearAlgebra.BidiagonalFactorization{$T,Vector{$T},typeof(A),typeof(τl)}(
             bd, A, hipview(τl, 1:nl), hipview(τr, 1:nr))
+    end
+end
+
+# Complex QR (ComplexF64/ComplexF32): qrUnblocked!, qrBlocked!, the
+# compact-WY block and lmul! with it — src/qr.jl:64-146,
+# src/householder.jl:82-157 (the workspace stays internal).
+for (T, sfx) in ((ComplexF64, "z"), (ComplexF32, "c"))
+    unb = Symbol("gla_qr_unblocked_", sfx)
+    blk = Symbol("gla_qr_blocked_", sfx)
+    tbl = Symbol("gla_qr_tblock_", sfx)
+    wya = Symbol("gla_wy_apply_", sfx)
+    @eval begin
+        function qrUnblocked!(A::HIPMatrix{$T},
+                              τ::HIPVector{$T} = hipzeros($T, min(size(A)...)))
+            m, n = size(A)
+            with_handle(h -> _check(@gla($unb, h, m, n, pointer(A), ld(A), pointer(τ), C_NULL),
+                                    "qrUnblocked!"))
+            return QR2{$T,typeof(A),typeof(τ)}(A, τ)
+        end
+        function qrBlocked!(A::HIPMatrix{$T}, blocksize::Integer = 12,
+                            τ::HIPVector{$T} = hipzeros($T, min(size(A)...)))
+            m, n = size(A)
+            blocksize >= 1 || throw(ArgumentError("blocksize must be positive"))
+            with_handle(h -> _check(@gla($blk, h, m, n, pointer(A), ld(A), Int64(blocksize),
+                                         pointer(τ), C_NULL), "qrBlocked!"))
+            return QR2{$T,typeof(A),typeof(τ)}(A, τ)
+        end
+        function Base.getindex(F::QR2{$T,<:HIPMatrix{$T}}, ::Type{Tuple{:QBlocked}})
+            m, n = size(F.factors)
+            k = min(m, n)
+            Tm = hipzeros($T, k, k)
+            with_handle(h -> _check(@gla($tbl, h, m, n, pointer(F.factors), ld(F.factors),
+                                         pointer(F.τ), pointer(Tm), ld(Tm), C_NULL),
+                                    "QR2[QBlocked]"))
+            return HIPHouseholderBlock{$T}(F.factors, UpperTriangular(Tm))
+        end
+        function LinearAlgebra.lmul!(H::HIPHouseholderBlock{$T}, A::HIPMatrix{$T})
+            V, Tm = H.V, H.T.data
+            with_handle(h -> _check(@gla($wya, h, Cchar('N'), size(A, 1), size(A, 2), pointer(V),
+                                         ld(V), size(V, 2), pointer(Tm), ld(Tm), pointer(A),
+                                         ld(A), C_NULL), "lmul!"))
+            return A
+        end
+        function LinearAlgebra.lmul!(Ha::Adjoint{$T,HIPHouseholderBlock{$T}}, A::HIPMatrix{$T})
+            H = parent(Ha)
+            V, Tm = H.V, H.T.data
+            with_handle(h -> _check(@gla($wya, h, Cchar('C'), size(A, 1), size(A, 2), pointer(V),
+                                         ld(V), size(V, 2), pointer(Tm), ld(Tm), pointer(A),
+                                         ld(A), C_NULL), "lmul!"))
+            return A
+        end
     end
 end
 

@@ -217,6 +217,40 @@ int gla_qr_blocked_s(gla_handle_t h, int64_t m, int64_t n, float *A, int64_t lda
                      int64_t blocksize, float *tau, float *work, int64_t ldwork,
                      void *stream);
 
+/* qrUnblocked!(A, τ) / qrBlocked!(A, blocksize, τ) — src/qr.jl:86-146 —
+ * for complex A (_z ComplexF64, _c ComplexF32) with the stdlib's complex
+ * reflector! (β = −copysign(‖x‖, real(x₁)) real, τ complex) and Hᴴ =
+ * I − τ̄ v vᴴ applied from the left, as qrUnblocked! does.  Block columns
+ * are min(blocksize, 32) wide (qrUnblocked!: 32); every width gives
+ * qrUnblocked!'s factors, up to rounding.  The reference's own complex
+ * qrBlocked! does not: its compact-WY T (src/qr.jl:72) takes F[j,i]
+ * where (VᴴV)[i,j] needs conj(F[j,i]), so its trailing updates are not
+ * H₁⋯H_k; this build forms the T that is (LAPACK zlarft's).  The
+ * workspace is internal (no `work` argument). */
+int gla_qr_unblocked_z(gla_handle_t h, int64_t m, int64_t n, gla_cdouble *A, int64_t lda,
+                       gla_cdouble *tau, void *stream);
+int gla_qr_unblocked_c(gla_handle_t h, int64_t m, int64_t n, gla_cfloat *A, int64_t lda,
+                       gla_cfloat *tau, void *stream);
+int gla_qr_blocked_z(gla_handle_t h, int64_t m, int64_t n, gla_cdouble *A, int64_t lda,
+                     int64_t blocksize, gla_cdouble *tau, void *stream);
+int gla_qr_blocked_c(gla_handle_t h, int64_t m, int64_t n, gla_cfloat *A, int64_t lda,
+                     int64_t blocksize, gla_cfloat *tau, void *stream);
+/* getindex(::QR2, Tuple{:QBlocked}) (src/qr.jl:64-83) for complex factors:
+ * T (k×k upper, k = min(m,n)) with I − V T Vᴴ = H₁⋯H_k (see above). */
+int gla_qr_tblock_z(gla_handle_t h, int64_t m, int64_t n, const gla_cdouble *F, int64_t ldf,
+                    const gla_cdouble *tau, gla_cdouble *T, int64_t ldt, void *stream);
+int gla_qr_tblock_c(gla_handle_t h, int64_t m, int64_t n, const gla_cfloat *F, int64_t ldf,
+                    const gla_cfloat *tau, gla_cfloat *T, int64_t ldt, void *stream);
+/* lmul!(H, A) (trans 'N') and lmul!(H', A) (trans 'C') for a complex
+ * HouseholderBlock — src/householder.jl:82-115,119-157: A <- op(I − V T Vᴴ) A,
+ * V mA×nV (unit lower, strict-lower storage); workspace internal. */
+int gla_wy_apply_z(gla_handle_t h, char trans, int64_t mA, int64_t nA, const gla_cdouble *V,
+                   int64_t ldv, int64_t nV, const gla_cdouble *T, int64_t ldt, gla_cdouble *A,
+                   int64_t lda, void *stream);
+int gla_wy_apply_c(gla_handle_t h, char trans, int64_t mA, int64_t nA, const gla_cfloat *V,
+                   int64_t ldv, int64_t nV, const gla_cfloat *T, int64_t ldt, gla_cfloat *A,
+                   int64_t lda, void *stream);
+
 /* getindex(::QR2, Tuple{:QBlocked}) — src/qr.jl:64-83: the compact-WY
  * T = (I + D striu(VᵀV))⁻¹ D (k×k upper, k = min(m,n)) of QR2(F, τ). */
 int gla_qr_tblock_d(gla_handle_t h, int64_t m, int64_t n, const double *F, int64_t ldf,

@@ -152,3 +152,61 @@ def test_complex_cholesky_4096_residual():
     R = A[rows] - L[rows] @ L.conj().t()
     err = float(torch.linalg.norm(R) / torch.linalg.norm(A[rows]))
     assert err <= 10 * n * eps_of(np.float64), err
+
+
+# ----------------------------------------------------------------- QR -------
+QR_SHAPES = [(10, 5), (10, 10), (5, 10), (100, 50), (100, 100), (50, 100)]
+QR_BS = [1, 2, 3, 4, 7, 8, 9, 15, 16, 17, 31, 32, 33]
+
+
+def _cmat(m, n, rng, dtype):
+    return (rng.standard_normal((m, n)) + 1j * rng.standard_normal((m, n))).astype(dtype)
+
+
+@pytest.mark.parametrize("m,n", QR_SHAPES)
+@pytest.mark.parametrize("dtype", CTYPES)
+def test_complex_qr_reference_grid(m, n, dtype):
+    """test/qr.jl:7-29 in complex: for each block size, the factors equal
+    qrUnblocked!'s (the oracle, which LAPACK zgeqrf pins), (Qᴴ A) = R and
+    Qᴴ(Q A) = A through the device's compact-WY block."""
+    gla = _gla()
+    rng = np.random.default_rng(m * 100 + n)
+    A = _cmat(m, n, rng, dtype)
+    ref, tref = O.zqr_unblocked(A)
+    tol = 10 * max(m, n) * eps_of(dtype)
+    F = gla.qrUnblocked_(gla.colmajor(A))
+    assert rel(gla.to_numpy(F.factors), ref) <= tol
+    assert rel(F.tau.cpu().numpy(), tref) <= tol
+    for bs in QR_BS:
+        F = gla.qrBlocked_(gla.colmajor(A), bs)
+        got = gla.to_numpy(F.factors)
+        assert rel(got, ref) <= tol, bs
+        assert rel(F.tau.cpu().numpy(), tref) <= tol, bs
+        H = F[gla.QBlocked]
+        Ad = gla.colmajor(A)
+        QhA = gla.to_numpy(gla.adjoint(H) @ Ad)
+        k = min(m, n)
+        assert rel(QhA[:k], np.triu(got)[:k]) <= tol, bs
+        back = gla.to_numpy(gla.adjoint(H) @ (H @ Ad))
+        assert rel(back, A) <= tol, bs
+
+
+@pytest.mark.parametrize("dtype", CTYPES)
+def test_complex_qr_larger(dtype):
+    gla = _gla()
+    m, n = 1000, 600
+    A = _cmat(m, n, np.random.default_rng(5), dtype)
+    ref, tref = O.zqr_unblocked(A)
+    F = gla.qrBlocked_(gla.colmajor(A), 32)
+    tol = 10 * m * eps_of(dtype)
+    assert rel(gla.to_numpy(F.factors), ref) <= tol
+    assert rel(F.tau.cpu().numpy(), tref) <= tol
+    T = gla.to_numpy(F[gla.QBlocked].T.data)
+    V = np.tril(ref[:, :n], -1) + np.eye(m, n)
+    # T against the explicit reflector product on a few columns of I
+    E = np.eye(m, 7, dtype=np.complex128)
+    Qe = E.copy()
+    for j in range(n - 1, -1, -1):
+        v = V[:, j].astype(np.complex128)
+        Qe = Qe - tref[j] * np.outer(v, v.conj() @ Qe)
+    assert rel(E - V @ (T @ (V.conj().T @ E)), Qe) <= tol

@@ -96,3 +96,30 @@ def test_complex_reflector_known_answer():
     assert F[0, 0] == pytest.approx(-13)
     assert tau[0] == pytest.approx((16 + 4j) / 13)
     assert F[1, 0] == pytest.approx(12 / (16 + 4j))
+
+
+def test_reference_complex_tblock_needs_the_conjugate(rng):
+    """src/qr.jl:72 forms T[i,j] = τ_i (F[j,i] + Σ conj(F[k,i]) F[k,j]);
+    for complex reflectors the first term must be conj(F[j,i]) for
+    I − V T Vᴴ = H₁⋯H_k.  This is why the complex qrBlocked! here builds
+    the conjugated T (equal to LAPACK zlarft's) rather than the literal
+    one, whose trailing updates would differ from qrUnblocked!."""
+    A = rng.standard_normal((12, 5)) + 1j * rng.standard_normal((12, 5))
+    F, tau = O.zqr_unblocked(A)
+    m, k = 12, 5
+    V = np.tril(F, -1) + np.eye(m, k)
+    Q = np.eye(m, dtype=complex)
+    for j in range(k):
+        Q = Q @ (np.eye(m) - tau[j] * np.outer(V[:, j], V[:, j].conj()))
+
+    def tblock(conj_first):
+        T = np.zeros((k, k), dtype=complex)
+        for j in range(k):
+            for i in range(j):
+                first = np.conj(F[j, i]) if conj_first else F[j, i]
+                T[i, j] = tau[i] * (first + np.vdot(F[j + 1:, i], F[j + 1:, j]))
+        Ui = np.linalg.inv(np.triu(T, 1) + np.eye(k))
+        return np.triu(Ui, 1) * tau[None, :] + np.diag(tau)
+
+    assert np.linalg.norm(np.eye(m) - V @ tblock(True) @ V.conj().T - Q) <= 1e-14
+    assert np.linalg.norm(np.eye(m) - V @ tblock(False) @ V.conj().T - Q) > 1e-2
