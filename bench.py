@@ -316,6 +316,139 @@ def adjoint_section(torch, gla):
     return out
 
 
+def next_rows_section(torch, gla):
+    """The rows beyond Cholesky/QR, each timed with HIP events and checked by
+    a size-independent property: blocked LDLᵀ (FP64, n = 8192, indefinite),
+    symtri! (FP64, n = 4096), bidiagonalize! (FP64, 4096²), complex
+    Cholesky (ComplexF64, n = 8192) and complex qrBlocked! (ComplexF64,
+    4096×2048)."""
+    dev = torch.device("cuda:0")
+    g = torch.Generator(device=dev).manual_seed(17)
+    out = {}
+
+    def sampled_rows(n, k=64):
+        return torch.from_numpy(np.sort(np.random.default_rng(0).choice(n, k, replace=False))).to(dev)
+
+    # LDLᵀ: A = G + Gᵀ + 3√n·diag(±1), about half the pivots negative
+    n = 8192
+    G = torch.randn(n, n, dtype=torch.float64, device=dev, generator=g)
+    A = G + G.t()
+    A.diagonal().add_(3 * n ** 0.5 * torch.sign(torch.randn(n, dtype=torch.float64, device=dev,
+                                                            generator=g)))
+    del G
+    W = gla.empty_colmajor(n, n, torch.float64)
+
+    def ldlt():
+        W.copy_(A)
+        gla.ldlt_(gla.Hermitian(W, "L"))
+
+    t_copy = ev_time(torch, lambda: W.copy_(A), reps=2)
+    t = ev_time(torch, ldlt, reps=2) - t_copy
+    F = gla.LDLt(W, "L")
+    rows = sampled_rows(n)
+    L = F.L
+    res = ((L[rows] * F.d) @ L.t() - A[rows]).norm() / A[rows].norm()
+    out["ldlt_8192_fp64"] = {"ms": t * 1e3, "tflops": n ** 3 / 3 / t / 1e12,
+                             "pct_fp64_peak": 100 * n ** 3 / 3 / t / 1e12 / FP64_PEAK_TFLOPS,
+                             "residual_rows": res.item()}
+    del A, W, F, L
+
+    # symtri!: FP64 n = 4096; the dominant kernel is the symmetric product,
+    # which reads the trailing lower triangle once per column: Σ (n−c)²/2·8 B
+    n = 4096
+    G = torch.randn(n, n, dtype=torch.float64, device=dev, generator=g)
+    A = G + G.t()
+    del G
+    W = gla.empty_colmajor(n, n, torch.float64)
+    tau = torch.zeros(n - 1, dtype=torch.float64, device=dev)
+
+    def tri():
+        W.copy_(A)
+        gla.symtri_(gla.Hermitian(W, "L"), tau)
+
+    t_copy = ev_time(torch, lambda: W.copy_(A), reps=2)
+    t = ev_time(torch, tri, reps=2) - t_copy
+    d = torch.diagonal(W).cpu().numpy()
+    e = torch.diagonal(W, -1).cpu().numpy()
+    import scipy.linalg as _sl
+    lam_t = _sl.eigvalsh_tridiagonal(d, e)
+    lam_a = torch.linalg.eigvalsh(A).cpu().numpy()
+    symv_bytes = sum((n - c - 1) ** 2 / 2 * 8 for c in range(n - 2))
+    out["symtri_4096_fp64"] = {"ms": t * 1e3, "tflops": 4 * n ** 3 / 3 / t / 1e12,
+                               "symv_bytes": symv_bytes, "symv_gbs_if_alone": symv_bytes / t / 1e9,
+                               "eigenvalue_max_abs_diff_rel": float(np.max(np.abs(lam_t - lam_a))
+                                                                    / np.max(np.abs(lam_a)))}
+    del A, W
+
+    # bidiagonalize!: FP64 4096×4096; two matrix-vector products per column
+    m = n = 4096
+    A = torch.randn(n, m, dtype=torch.float64, device=dev, generator=g).t()
+    W = torch.empty(n, m, dtype=torch.float64, device=dev).t()
+
+    def bd():
+        W.copy_(A)
+        gla.bidiagonalize_(W)
+
+    t_copy = ev_time(torch, lambda: W.copy_(A), reps=2)
+    t = ev_time(torch, bd, reps=2) - t_copy
+    F = gla.BidiagonalFactorization(W, None, None)
+    dv, ev = F.bidiagonal
+    B = torch.diag(dv) + torch.diag(ev, 1)
+    s_b = torch.linalg.svdvals(B).cpu().numpy()
+    s_a = torch.linalg.svdvals(A).cpu().numpy()
+    fl = 4 * m * n * n - 4 * n ** 3 / 3
+    out["bidiag_4096x4096_fp64"] = {"ms": t * 1e3, "tflops": fl / t / 1e12,
+                                    "svdvals_max_abs_diff_rel": float(np.max(np.abs(s_b - s_a))
+                                                                      / s_a[0])}
+    del A, W, B
+
+    # complex Cholesky: ComplexF64 n = 8192, A = BᴴB + n·I; 4n³/3 real flops
+    n = 8192
+    Bc = torch.complex(torch.randn(n, n, dtype=torch.float64, device=dev, generator=g),
+                       torch.randn(n, n, dtype=torch.float64, device=dev, generator=g))
+    A = Bc.conj().t() @ Bc
+    del Bc
+    A.diagonal().add_(n)
+    A = (0.5 * (A + A.conj().t())).t().contiguous().t()
+    W = gla.empty_colmajor(n, n, torch.complex128)
+
+    def zchol():
+        W.copy_(A)
+        gla.cholRecursive_(W, gla.Val.L)
+
+    t_copy = ev_time(torch, lambda: W.copy_(A), reps=2)
+    t = ev_time(torch, zchol, reps=2) - t_copy
+    rows = sampled_rows(n)
+    L = torch.tril(W)
+    res = (L[rows] @ L.conj().t() - A[rows]).norm() / A[rows].norm()
+    out["zchol_8192_c128"] = {"ms": t * 1e3, "tflops": 4 * n ** 3 / 3 / t / 1e12,
+                              "pct_fp64_peak": 100 * 4 * n ** 3 / 3 / t / 1e12 / FP64_PEAK_TFLOPS,
+                              "residual_rows": res.item()}
+    del A, W, L
+
+    # complex qrBlocked!: ComplexF64 4096×2048; 4 × (2mn² − 2n³/3) real flops
+    m, n = 4096, 2048
+    A = torch.complex(torch.randn(n, m, dtype=torch.float64, device=dev, generator=g),
+                      torch.randn(n, m, dtype=torch.float64, device=dev, generator=g)).t()
+    W = torch.empty(n, m, dtype=torch.complex128, device=dev).t()
+    tauc = torch.zeros(n, dtype=torch.complex128, device=dev)
+
+    def zqr():
+        W.copy_(A)
+        gla.qrBlocked_(W, 32, tauc)
+
+    t_copy = ev_time(torch, lambda: W.copy_(A), reps=2)
+    t = ev_time(torch, zqr, reps=2) - t_copy
+    Rr = torch.linalg.qr(A, mode="r")[1]
+    R = torch.triu(W[:n, :n])
+    ph = torch.diagonal(R) / torch.diagonal(Rr)
+    diffr = ((R - ph[:, None] * Rr).norm() / Rr.norm()).item()
+    fl = 4 * (2 * m * n * n - 2 * n ** 3 / 3)
+    out["zqr_4096x2048_c128"] = {"ms": t * 1e3, "tflops": fl / t / 1e12,
+                                 "R_rel_diff_vs_rocsolver_phase_normalised": diffr}
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -544,7 +677,7 @@ def main():
     torch.cuda.empty_cache()
     if not args.no_extras:
         for key, fn in (("mixed", mixed_section), ("batched", batched_section),
-                        ("adjoint", adjoint_section)):
+                        ("adjoint", adjoint_section), ("next_rows", next_rows_section)):
             try:
                 out[key] = fn(torch, gla)
             except Exception as e:  # noqa: BLE001  (a section failure is reported, not fatal)
