@@ -77,19 +77,14 @@ __device__ __forceinline__ T lane_bcast(T v, int src)
 }
 
 // v from lane K of each 16-lane row to the whole row (DPP row_newbcast:K).
+// Every lane reads a valid source, so no "old" value needs setting up; a
+// 64-bit value moves in one v_mov_b64_dpp (gfx950 has DPP on 64-bit moves
+// for row_newbcast).
 template <int K, typename T>
 __device__ __forceinline__ T row_bcast(T v)
 {
     constexpr int ctrl = 0x150 + K;
-    if constexpr (sizeof(T) == 8) {
-        int2 u = __builtin_bit_cast(int2, v);
-        u.x = __builtin_amdgcn_update_dpp(0, u.x, ctrl, 0xF, 0xF, false);
-        u.y = __builtin_amdgcn_update_dpp(0, u.y, ctrl, 0xF, 0xF, false);
-        return __builtin_bit_cast(T, u);
-    } else {
-        return __builtin_bit_cast(
-            T, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), ctrl, 0xF, 0xF, false));
-    }
+    return __builtin_amdgcn_update_dpp(v, v, ctrl, 0xF, 0xF, true);
 }
 
 // s = √a and r = 1/√a from the hardware reciprocal square root refined by
@@ -125,86 +120,129 @@ __device__ __forceinline__ void sqrt_and_rsqrt(float a, float &s, float &r)
     r = zero ? __builtin_inff() : y;
 }
 
-// Factors the symmetric tile in v in place, right-looking, column by column
-// as cholUnblocked! does (src/cholesky.jl:3-15): sqrt of the pivot, scale
-// by its reciprocal, rank-1 update.  rd[c] receives 1/L[c][c].  The first
-// pivot whose square root is taken of a negative number, among the first
-// `nvalid`, is recorded in *info as gbase + c + 1.  The pivot comes through
-// a scalar register; the column entries a lane needs are fetched unscaled
-// while the square root and reciprocal are computed, then scaled locally
-// (the same products the scaled column holds).
+// Row layout of a 16×16 tile: lane (i = l % 16, g = l / 16) holds all of row
+// i in a[0..15], the same in each of the four 16-lane groups, so that every
+// value a column step needs moves by a DPP broadcast within the lane's own
+// 16-lane row (a few cycles) rather than through the LDS crossbar.
+
+// Column C's rank-1 update of the entries a[J..15]: a[j] -= l_i l_j, with
+// l_j = L[j][C] broadcast from lane j.  Entries right of the diagonal (j > i)
+// take meaningless values and are cleared by the caller.
+template <int C, int J, typename T>
+struct TileColUpdate {
+    static __device__ __forceinline__ void run(T a[16], T l)
+    {
+        const T lj = row_bcast<J>(l);
+        if constexpr (sizeof(T) == 8)
+            a[J] = __builtin_fma(-l, lj, a[J]);
+        else
+            a[J] = __builtin_fmaf(-l, lj, a[J]);
+        TileColUpdate<C, J + 1, T>::run(a, l);
+    }
+};
+template <int C, typename T>
+struct TileColUpdate<C, 16, T> {
+    static __device__ __forceinline__ void run(T *, T) {}
+};
+
+// Factors the symmetric tile held in row layout in place, right-looking,
+// column by column as cholUnblocked! does (src/cholesky.jl:3-15): sqrt of
+// the pivot, scale by its reciprocal, rank-1 update (the update products are
+// (a_ic r)(a_jc r), each factor rounded once, as in the scaled column).
+// rd[c] receives 1/L[c][c].  The first pivot whose square root is taken of
+// a negative number, among the first `nvalid`, is recorded in *info as
+// gbase + c + 1.  Entries right of the diagonal are left meaningless.
+// The 16 column steps are one branch-free block (negative pivots are only
+// marked in `neg` and reported after the tile), so that the compiler can
+// overlap the trailing updates of column c with the square root of c + 1.
+template <int C, typename T>
+struct TileFactorCols {
+    static __device__ __forceinline__ void run(T a[16], T rd[16], int i, unsigned &neg)
+    {
+        const T d = row_bcast<C>(a[C]);
+        neg |= d < T(0) ? (1u << C) : 0u;
+        T s, r;
+        sqrt_and_rsqrt(d, s, r);
+        rd[C] = r;
+        const T l = a[C] * r;
+        a[C] = i == C ? s : l;
+        TileColUpdate<C, C + 1, T>::run(a, l);
+        TileFactorCols<C + 1, T>::run(a, rd, i, neg);
+    }
+};
 template <typename T>
-__device__ __forceinline__ void tile_factor(T v[4], T rd[16], int nvalid, int *info,
-                                            int64_t gbase)
+struct TileFactorCols<16, T> {
+    static __device__ __forceinline__ void run(T *, T *, int, unsigned &) {}
+};
+
+template <typename T>
+__device__ __forceinline__ void tile_factor_rows(T a[16], T rd[16], int nvalid, int *info,
+                                                 int64_t gbase)
+{
+    const int i = threadIdx.x & 15;
+    unsigned neg = 0;
+    TileFactorCols<0, T>::run(a, rd, i, neg);
+    if (nvalid < 16) neg &= (1u << (nvalid > 0 ? nvalid : 0)) - 1u;
+    if (neg && (threadIdx.x & 63) == 0 && info) atomicCAS(info, 0, (int)(gbase + __builtin_ctz(neg) + 1));
+#pragma unroll
+    for (int j = 0; j < 16; ++j) a[j] = j <= i ? a[j] : T(0);
+}
+
+// w <- L⁻¹ for a lower 16×16 tile L (rd = reciprocals of its diagonal), in
+// the four-column layout (w[t] = W[i][g + 4t]).  Forward elimination of
+// [L | I] with unscaled rows: V[i] = e_i − Σ_{k<i} m_ik V[k] with the
+// multipliers m_ik = L[i][k]·rd[k] formed once (zero for i ≤ k, so every
+// step is one broadcast and one unconditional FMA per column), then
+// W[i] = rd[i]·V[i].  Row k of V reaches the lanes of its 16-lane row by
+// DPP broadcast.
+template <int K, typename T>
+struct TileInvSteps {
+    static __device__ __forceinline__ void run(const T m[16], T w[4])
+    {
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+            const T vk = row_bcast<K>(w[t]);
+            if constexpr (sizeof(T) == 8)
+                w[t] = __builtin_fma(-m[K], vk, w[t]);
+            else
+                w[t] = __builtin_fmaf(-m[K], vk, w[t]);
+        }
+        TileInvSteps<K + 1, T>::run(m, w);
+    }
+};
+template <typename T>
+struct TileInvSteps<15, T> {   // row 15 has no rows below it
+    static __device__ __forceinline__ void run(const T *, T *) {}
+};
+
+// The same from L in row layout (li[k] = L[i][k]).
+template <typename T>
+__device__ __forceinline__ void tile_inverse_rows(const T li[16], const T rd[16], T w[4])
 {
     const int lane = threadIdx.x & 63;
     const int i = lane & 15, g = lane >> 4;
+    T m[16], ri = rd[0];
 #pragma unroll
-    for (int c = 0; c < 16; ++c) {
-        const int ct = c >> 2, cg = c & 3;
-        T d = lane_bcast(v[ct], c + 16 * cg);
-        const T aic = __shfl(v[ct], i + 16 * cg);
-        T ajc[4];
-#pragma unroll
-        for (int t = 0; t < 4; ++t) ajc[t] = __shfl(v[ct], g + 4 * t + 16 * cg);
-        if (d < T(0) && lane == 0 && info && c < nvalid) atomicCAS(info, 0, (int)(gbase + c + 1));
-        T r;
-        sqrt_and_rsqrt(d, d, r);
-        rd[c] = r;
-        if (g == cg) v[ct] = i > c ? v[ct] * r : (i == c ? d : v[ct]);
-        const T lic = aic * r;
-#pragma unroll
-        for (int t = 0; t < 4; ++t) {
-            const int j = g + 4 * t;
-            if (j > c && j <= i) v[t] -= lic * (ajc[t] * r);
-        }
+    for (int k = 0; k < 16; ++k) {
+        m[k] = i > k ? li[k] * rd[k] : T(0);
+        ri = i == k ? rd[k] : ri;
     }
+#pragma unroll
+    for (int t = 0; t < 4; ++t) w[t] = (i == g + 4 * t) ? T(1) : T(0);
+    TileInvSteps<0, T>::run(m, w);
+#pragma unroll
+    for (int t = 0; t < 4; ++t) w[t] *= ri;
 }
 
-// w <- L⁻¹ for the lower tile L in v (rd = reciprocals of its diagonal):
-// forward elimination of [L | I], row k scaled then subtracted from the rows
-// below it.  L's entries are fetched up front (they do not change); row k
-// of W reaches the other lanes of its 16-lane row by DPP broadcast.
-template <int K, typename T>
-__device__ __forceinline__ void tile_inverse_step(const T lik, const T rdk, T w[4])
-{
-    const int i = threadIdx.x & 15;
-    if (i == K)
-#pragma unroll
-        for (int t = 0; t < 4; ++t) w[t] *= rdk;
-#pragma unroll
-    for (int t = 0; t < 4; ++t) {
-        const T wkj = row_bcast<K>(w[t]);
-        if (i > K) w[t] -= lik * wkj;
-    }
-}
-
+// The same from L in the four-column layout (v[t] = L[i][g + 4t]).
 template <typename T>
 __device__ __forceinline__ void tile_inverse(const T v[4], const T rd[16], T w[4])
 {
-    const int lane = threadIdx.x & 63;
-    const int i = lane & 15, g = lane >> 4;
+    const int i = threadIdx.x & 15;
     T li[16];
 #pragma unroll
     for (int k = 0; k < 16; ++k) li[k] = __shfl(v[k >> 2], i + 16 * (k & 3));
-#pragma unroll
-    for (int t = 0; t < 4; ++t) w[t] = (i == g + 4 * t) ? T(1) : T(0);
-    tile_inverse_step<0>(li[0], rd[0], w);
-    tile_inverse_step<1>(li[1], rd[1], w);
-    tile_inverse_step<2>(li[2], rd[2], w);
-    tile_inverse_step<3>(li[3], rd[3], w);
-    tile_inverse_step<4>(li[4], rd[4], w);
-    tile_inverse_step<5>(li[5], rd[5], w);
-    tile_inverse_step<6>(li[6], rd[6], w);
-    tile_inverse_step<7>(li[7], rd[7], w);
-    tile_inverse_step<8>(li[8], rd[8], w);
-    tile_inverse_step<9>(li[9], rd[9], w);
-    tile_inverse_step<10>(li[10], rd[10], w);
-    tile_inverse_step<11>(li[11], rd[11], w);
-    tile_inverse_step<12>(li[12], rd[12], w);
-    tile_inverse_step<13>(li[13], rd[13], w);
-    tile_inverse_step<14>(li[14], rd[14], w);
-    tile_inverse_step<15>(li[15], rd[15], w);
+    tile_inverse_rows<T>(li, rd, w);
 }
 
 // Tile (bi, bj) of the diagonal block held in slot q of wave w: slots 0..w
@@ -259,14 +297,16 @@ __device__ __forceinline__ void factor_body(int nb, TS *__restrict__ A, int64_t 
         // (2) wave 0 factors the diagonal tile and inverts it
         if (W == 0) {
             const int i = lane & 15, g = lane >> 4;
-            T v[4], w[4], rd[16];
+            T a[16], w[4], rd[16];
 #pragma unroll
-            for (int t = 0; t < 4; ++t) v[t] = (g + 4 * t <= i) ? D[i * kXLD + g + 4 * t] : T(0);
-            tile_factor<T>(v, rd, nb - kb * 16, info, info_base + kb * 16);
-            tile_inverse<T>(v, rd, w);
+            for (int j = 0; j < 16; ++j) a[j] = j <= i ? D[i * kXLD + j] : T(0);
+            tile_factor_rows<T>(a, rd, nb - kb * 16, info, info_base + kb * 16);
+            tile_inverse_rows<T>(a, rd, w);
+            if (g == 0)
+#pragma unroll
+                for (int j = 0; j < 16; ++j) D[i * kXLD + j] = a[j];
 #pragma unroll
             for (int t = 0; t < 4; ++t) {
-                D[i * kXLD + g + 4 * t] = v[t];
                 Li[i * kXLD + g + 4 * t] = w[t];
                 if (LinvG) LinvG[kb * 256 + i * 16 + g + 4 * t] = w[t];
                 if (Linv32) Linv32[kb * 256 + i * 16 + g + 4 * t] = float(w[t]);
@@ -682,6 +722,14 @@ int chol_blocked_dev(Handle *h, hipStream_t s, int64_t n, T *A, int64_t lda, int
         GLA_TRY(launch_factor<T>(s, kb, Akk, lda, h->dinfo, info_base + k, Linv));  // :25
         GLA_TRY(launch_solve<T>(s, kb, Akk, lda, Linv, A21, lda, below));           // :27-28
         if (below == 0) break;
+        if (!overlap) {
+            // :30-31 rankUpdate! of all of A22 in one launch: with nothing
+            // beside it, splitting off the next block column only adds a
+            // launch (same products, same order)
+            GLA_TRY(gemm<T>(s, 'N', 'T', 1, below, below, kb, T(-1), A21, lda, A21, lda, T(1),
+                            A21 + kb * lda, lda, 0));
+            continue;
+        }
         const int kb2 = (int)std::min<int64_t>(nb, below);
         const int64_t rest = below - kb2;
         if (overlap && rest > 0) GLA_HIP_TRY(hipEventRecord(h->ev_step[0], s));
@@ -733,16 +781,23 @@ int trsm_rlt_dev(hipStream_t s, int64_t m, int64_t n, const T *L, int64_t ldl, c
 // efficient MFMA SYRK) on the side stream, where it overlaps the next
 // diagonal factor and panel solve.
 template <typename T>
-int chol_la_dev(Handle *h, hipStream_t s, int64_t n, T *A, int64_t lda, int NB,
+int chol_la_dev(Handle *h, hipStream_t s0, int64_t n, T *A, int64_t lda, int NB,
                 int64_t info_base, T *Linv)
 {
     if (n <= 0) return GLA_OK;
-    if (NB <= kPanelNB || n <= NB) return chol_blocked_dev<T>(h, s, n, A, lda, kPanelNB, info_base, Linv);
+    if (NB <= kPanelNB || n <= NB) return chol_blocked_dev<T>(h, s0, n, A, lda, kPanelNB, info_base, Linv);
     const int64_t nsteps = cdiv(n, NB);
     const bool overlap = nsteps > 2;
+    // With CUs reserved from the SYRK, the chain runs on an internal stream
+    // with every CU and the SYRK on the masked one, both forked from (and
+    // joined back to) the caller's stream.
+    const bool masked = overlap && h->masked != nullptr;
+    hipStream_t s = masked ? h->chain : s0;
+    hipStream_t side = masked ? h->masked : h->side;
     if (overlap) {
-        GLA_HIP_TRY(hipEventRecord(h->ev_fork, s));
-        GLA_HIP_TRY(hipStreamWaitEvent(h->side, h->ev_fork, 0));
+        GLA_HIP_TRY(hipEventRecord(h->ev_fork, s0));
+        GLA_HIP_TRY(hipStreamWaitEvent(side, h->ev_fork, 0));
+        if (masked) GLA_HIP_TRY(hipStreamWaitEvent(s, h->ev_fork, 0));
     }
     int64_t k = 0;
     for (int64_t step = 0; step < nsteps; ++step, k += NB) {
@@ -766,17 +821,21 @@ int chol_la_dev(Handle *h, hipStream_t s, int64_t n, T *A, int64_t lda, int NB,
                         A21 + kb * lda, lda, 0));
         if (overlap && rest > 0 && late) GLA_HIP_TRY(hipEventRecord(h->ev_step[0], s));
         if (rest > 0) {
-            hipStream_t us = overlap ? h->side : s;
-            if (overlap) GLA_HIP_TRY(hipStreamWaitEvent(h->side, h->ev_step[0], 0));
+            hipStream_t us = overlap ? side : s;
+            if (overlap) GLA_HIP_TRY(hipStreamWaitEvent(side, h->ev_step[0], 0));
             T *X2 = A21 + kb2;
             GLA_TRY(gemm<T>(us, 'N', 'T', 1, rest, rest, kb, T(-1), X2, lda, X2, lda, T(1),
                             X2 + (kb + kb2) * lda, lda, 0));
-            if (overlap) GLA_HIP_TRY(hipEventRecord(h->ev_step[1], h->side));
+            if (overlap) GLA_HIP_TRY(hipEventRecord(h->ev_step[1], side));
         }
     }
     if (overlap) {
-        GLA_HIP_TRY(hipEventRecord(h->ev_join, h->side));
-        GLA_HIP_TRY(hipStreamWaitEvent(s, h->ev_join, 0));
+        GLA_HIP_TRY(hipEventRecord(h->ev_join, side));
+        GLA_HIP_TRY(hipStreamWaitEvent(s0, h->ev_join, 0));
+        if (masked) {
+            GLA_HIP_TRY(hipEventRecord(h->ev_fork, s));
+            GLA_HIP_TRY(hipStreamWaitEvent(s0, h->ev_fork, 0));
+        }
     }
     return GLA_OK;
 }

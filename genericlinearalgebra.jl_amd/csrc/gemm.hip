@@ -35,8 +35,10 @@ constexpr int kThreads = 256;
 constexpr int kBK = 16;
 
 // FP64 direct-to-LDS kernel selection (process-wide): 0 off, 1 (BK 8, ring
-// of 4), 2 (BK 8, ring of 3), 3 (BK 16, ring of 3).
-int g_gemm_dl = 0;
+// of 4), 2 (BK 8, ring of 3; the default: the n = 16384 Cholesky's SYRK on
+// 7/8 of the CUs runs 5% faster than with the register-staged kernel),
+// 3 (BK 16, ring of 3).
+int g_gemm_dl = 2;
 // FP64 128×128 tiles with one LDS buffer (1, the default: measured 1–2%
 // faster for the Cholesky, whose chain kernels then find LDS beside the
 // trailing SYRK) or two (0).
@@ -613,6 +615,8 @@ int gemm(hipStream_t s, char ta, char tb, int mode, int64_t M, int64_t N, int64_
 
 void set_gemm_dl(int v) { g_gemm_dl = v; }
 void set_gemm_lds1(int v) { g_gemm_lds1 = v; }
+int get_gemm_dl() { return g_gemm_dl; }
+int get_gemm_lds1() { return g_gemm_lds1; }
 
 template <typename T>
 int gemm_batched(hipStream_t s, char ta, char tb, int mode, int64_t M, int64_t N, int64_t K,

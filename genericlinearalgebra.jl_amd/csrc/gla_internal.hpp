@@ -78,7 +78,7 @@ struct Handle {
     int num_cus = 256;
     // Tuning: block width of the look-ahead driver behind cholRecursive!
     // (0: the recursive-halving driver).  Set with gla_set_tuning.
-    int64_t chol_block = 512;
+    int64_t chol_block = 768;
     // Release the look-ahead Cholesky's side-stream SYRK after the
     // next-block-column update (1) rather than right after the panel solve.
     int chol_syrk_late = 1;
@@ -104,6 +104,13 @@ struct Handle {
     // Test setting: every grid-wide wait of the QR panel targets an arrival
     // count that never comes, so each gives up after spin_limit polls.
     int qr_wait_test = 0;
+    // CUs kept free of the look-ahead Cholesky's trailing SYRK (0: none;
+    // a new handle takes num_cus/8).  When set, the driver runs its chain on
+    // `chain` (all CUs) and the SYRK on `masked` (the other CUs), both
+    // forked from the caller's stream.
+    int side_cu_reserve = 0;
+    hipStream_t chain = nullptr;
+    hipStream_t masked = nullptr;
 };
 
 // Ensure the workspace holds at least `bytes` (grows with hipMalloc, which
@@ -124,6 +131,8 @@ int gemm(hipStream_t s, char ta, char tb, int mode, int64_t M, int64_t N, int64_
 // Selects the FP64 direct-to-LDS GEMM kernel (0: off).
 void set_gemm_dl(int v);
 void set_gemm_lds1(int v);
+int get_gemm_dl();
+int get_gemm_lds1();
 // The same over a strided batch (grid z): product b reads A + b·sA, B + b·sB
 // and writes C + b·sC; no split-K.
 template <typename T>

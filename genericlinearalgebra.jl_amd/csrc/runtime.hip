@@ -37,6 +37,30 @@ int ensure_workspace(Handle *h, size_t bytes)
     return GLA_OK;
 }
 
+namespace {
+// (Re)creates the look-ahead Cholesky's stream pair: `masked`, limited to
+// every CU but the highest-numbered `reserve`, and `chain`, a non-blocking
+// stream on all CUs (a CU-masked stream is created blocking, so the chain
+// cannot run on the caller's stream beside it).  reserve 0 drops both.
+int set_side_cu_reserve(Handle *h, int reserve)
+{
+    GLA_HIP_TRY(hipSetDevice(h->device));
+    GLA_HIP_TRY(hipDeviceSynchronize());
+    if (h->masked) { (void)hipStreamDestroy(h->masked); h->masked = nullptr; }
+    if (h->chain) { (void)hipStreamDestroy(h->chain); h->chain = nullptr; }
+    h->side_cu_reserve = reserve;
+    if (reserve > 0) {
+        const int words = (h->num_cus + 31) / 32;
+        uint32_t mask[16] = {0};
+        for (int c = 0; c < h->num_cus - reserve && c < 32 * 16; ++c)
+            mask[c / 32] |= 1u << (c % 32);
+        GLA_HIP_TRY(hipExtStreamCreateWithCUMask(&h->masked, (uint32_t)words, mask));
+        GLA_HIP_TRY(hipStreamCreateWithFlags(&h->chain, hipStreamNonBlocking));
+    }
+    return GLA_OK;
+}
+}  // namespace
+
 }  // namespace gla
 
 extern "C" {
@@ -98,7 +122,6 @@ int gla_set_tuning(gla_handle_t hh, const char *key, int64_t value)
         gla::set_gemm_dl((int)value);
         return GLA_OK;
     }
-
     if (std::strcmp(key, "gemm_lds1") == 0) {
         if (value != 0 && value != 1) {
             gla::set_error("gemm_lds1 must be 0 or 1");
@@ -139,6 +162,13 @@ int gla_set_tuning(gla_handle_t hh, const char *key, int64_t value)
         h->spin_limit = (unsigned)value;
         return GLA_OK;
     }
+    if (std::strcmp(key, "side_cu_reserve") == 0) {
+        if (value < 0 || value > h->num_cus / 2 || h->num_cus > 32 * 16) {
+            gla::set_error("side_cu_reserve must be in [0, CUs/2]");
+            return -3;
+        }
+        return gla::set_side_cu_reserve(h, (int)value);
+    }
     if (std::strcmp(key, "qr_wait_test") == 0) {
         if (value != 0 && value != 1) {
             gla::set_error("qr_wait_test must be 0 or 1");
@@ -148,6 +178,27 @@ int gla_set_tuning(gla_handle_t hh, const char *key, int64_t value)
         return GLA_OK;
     }
     gla::set_error("unknown tuning key '%s'", key);
+    return -2;
+}
+
+int gla_get_tuning(gla_handle_t hh, const char *key, int64_t *value)
+{
+    auto *h = reinterpret_cast<gla::Handle *>(hh);
+    if (!h) { gla::set_error("null handle"); return -1; }
+    if (!key) return -2;
+    if (!value) return -3;
+    const struct { const char *name; int64_t v; } knobs[] = {
+        {"chol_block", h->chol_block},       {"ldlt_block", h->ldlt_block},
+        {"mixed_block", h->mixed_block},     {"qr_panel", h->qr_panel},
+        {"qr_inner", h->qr_inner},           {"gemm_dl", gla::get_gemm_dl()},
+        {"gemm_lds1", gla::get_gemm_lds1()}, {"qr_hr", h->qr_hr},
+        {"chol_syrk_late", h->chol_syrk_late}, {"qr_lookahead", h->qr_lookahead},
+        {"spin_limit", (int64_t)h->spin_limit}, {"side_cu_reserve", h->side_cu_reserve},
+        {"qr_wait_test", h->qr_wait_test},
+    };
+    for (const auto &k : knobs)
+        if (std::strcmp(key, k.name) == 0) { *value = k.v; return GLA_OK; }
+    gla::set_error("unknown tuning knob '%s'", key);
     return -2;
 }
 
@@ -169,6 +220,12 @@ int gla_handle_create(gla_handle_t *out)
     GLA_HIP_TRY(hipMemset(h->dinfo, 0, 256));
     GLA_HIP_TRY(hipMalloc(reinterpret_cast<void **>(&h->dbar), 4096));
     GLA_HIP_TRY(hipMemset(h->dbar, 0, 4096));
+    // The trailing SYRK of the look-ahead Cholesky leaves an eighth of the
+    // CUs (32 of 256) to the chain of small kernels beside it.
+    if (h->num_cus <= 32 * 16) {
+        const int rc = gla::set_side_cu_reserve(h, h->num_cus / 8);
+        if (rc != GLA_OK) return rc;
+    }
     GLA_HIP_TRY(hipDeviceSynchronize());
     *out = reinterpret_cast<gla_handle_t>(h);
     return GLA_OK;
@@ -189,6 +246,8 @@ int gla_handle_destroy(gla_handle_t hh)
     (void)hipEventDestroy(h->ev_step[0]);
     (void)hipEventDestroy(h->ev_step[1]);
     (void)hipStreamDestroy(h->side);
+    if (h->masked) (void)hipStreamDestroy(h->masked);
+    if (h->chain) (void)hipStreamDestroy(h->chain);
     delete h;
     return GLA_OK;
 }

@@ -14,6 +14,7 @@ computation is a call into libgla_hip.so.
 """
 from __future__ import annotations
 
+import contextlib
 import ctypes as C
 
 import numpy as np
@@ -85,6 +86,27 @@ def _handle(dev: int) -> int:
 def set_tuning(key: str, value: int, device: int = 0):
     """Sets a tuning knob of the device's handle (see gla_set_tuning)."""
     _check(lib().gla_set_tuning(_handle(device), key.encode(), int(value)), "gla_set_tuning")
+
+
+def get_tuning(key: str, device: int = 0) -> int:
+    """Reads a tuning knob of the device's handle (gla_get_tuning)."""
+    v = C.c_int64(0)
+    _check(lib().gla_get_tuning(_handle(device), key.encode(), C.byref(v)), "gla_get_tuning")
+    return int(v.value)
+
+
+@contextlib.contextmanager
+def tuning(device: int = 0, **knobs):
+    """Sets knobs for the duration of a with-block and restores the values
+    they had before it, also when the block raises."""
+    old = {k: get_tuning(k, device) for k in knobs}
+    try:
+        for k, v in knobs.items():
+            set_tuning(k, v, device)
+        yield
+    finally:
+        for k, v in old.items():
+            set_tuning(k, v, device)
 
 
 def _check(rc: int, what: str):

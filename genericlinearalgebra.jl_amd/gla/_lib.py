@@ -112,6 +112,8 @@ def lib() -> C.CDLL:
     L.gla_chol_recursive_noalloc_d.restype = _int
     L.gla_set_tuning.argtypes = [_p, C.c_char_p, _i64]
     L.gla_set_tuning.restype = _int
+    L.gla_get_tuning.argtypes = [_p, C.c_char_p, _pi64]
+    L.gla_get_tuning.restype = _int
     L.gla_last_error_string.restype = C.c_char_p
     L.gla_version.restype = C.c_char_p
     _lib = L

@@ -64,12 +64,17 @@ const char *gla_last_error_string(void);
 /* Version string of this build (no reference counterpart). */
 const char *gla_version(void);
 /* Tuning knobs of a handle (no reference counterpart).  "chol_block": a
- * multiple of 128 (default 512) runs cholRecursive! as a right-looking
+ * multiple of 128 (default 768) runs cholRecursive! as a right-looking
  * blocked factorisation with blocks of that width and depth-1 look-ahead;
  * 0 runs it as recursive halving down to 2048-wide leaves.  Both compute
  * the same factor.  "chol_syrk_late": 1 (default) starts the look-ahead
  * driver's trailing SYRK on the side stream after the next block column's
- * update, 0 right after the panel solve (same factor).  "ldlt_block": step width (multiple
+ * update, 0 right after the panel solve (same factor).  "side_cu_reserve":
+ * CUs (0 to CUs/2; a new handle takes CUs/8, 32 of 256) that the look-ahead
+ * driver's trailing SYRK leaves to the chain of diagonal factor, panel
+ * solve and next-block-column update, which then runs on a stream of the
+ * handle's own and joins the caller's at the end; 0 runs the chain on the
+ * caller's stream and the SYRK on any CU (same factor bit for bit).  "ldlt_block": step width (multiple
  * of 128 in [128, 4096], default 512) of gla_ldlt_{d,s}: 128 runs 128-wide
  * steps, wider ones that width with depth-1 look-ahead (same factor up to
  * rounding).  "mixed_block": block width (multiple of 128,
@@ -88,10 +93,10 @@ const char *gla_version(void);
  * the orthonormal factor (the LU of S − Q₁ with the sign choice that
  * reflector! makes), checked on the device and falling back to the
  * column-by-column panel when the block column is too ill-conditioned for
- * it.  V, τ and R are the same Householder sequence in every setting.  "gemm_dl" (process-wide): 0 (default), or
- * 1/2/3 routes FP64 NT products on 128×128 tiles (SYRK, rankUpdate!, the
+ * it.  V, τ and R are the same Householder sequence in every setting.  "gemm_dl" (process-wide): 2 (default) or
+ * 1/3 route FP64 NT products on 128×128 tiles (SYRK, rankUpdate!, the
  * trailing updates) to a direct-to-LDS kernel with a 4/3/3-deep ring of
- * 8/8/16-wide K slabs; "gemm_lds1" (process-wide): 1 (default) stages
+ * 8/8/16-wide K slabs, 0 to the register-staged kernel; "gemm_lds1" (process-wide): 1 (default) stages
  * FP64 128×128 tiles through one LDS buffer, 0 through two.  Same results
  * bit for bit in every setting.  "spin_limit": polls (default 2^25, about a
  * second) before a grid-wide wait of the QR panel gives up and the call
@@ -99,6 +104,9 @@ const char *gla_version(void);
  * an arrival count that never comes (a test of the timeout report; the
  * kernels still finish, after spin_limit polls per wait). */
 int gla_set_tuning(gla_handle_t h, const char *key, int64_t value);
+/* Reads a tuning knob's current value, by the same keys.  No counterpart
+ * in the reference. */
+int gla_get_tuning(gla_handle_t h, const char *key, int64_t *value);
 
 /* ------------------------------------------------------------ Cholesky -- */
 /* cholUnblocked!(A, Val{:L}) — src/cholesky.jl:3-15.

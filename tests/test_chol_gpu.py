@@ -81,18 +81,15 @@ def test_sizes_against_oracle(rng, dtype, n, algo):
 
 
 @pytest.mark.parametrize("n", [2500, 4100])
-@pytest.mark.parametrize("block", [0, 256, 512, 1024])
+@pytest.mark.parametrize("block", [0, 256, 512, 768, 1024])
 def test_large_recursive_backward_error(rng, n, block):
     """Both device drivers behind cholRecursive!: recursive halving past the
-    2048 leaf (block 0) and the wide-block look-ahead (block 256/512/1024);
+    2048 leaf (block 0) and the wide-block look-ahead (block 256 to 1024);
     checked against LAPACK's factor and by ‖A − LLᵀ‖/‖A‖."""
     gla = _gla()
     A = spd(n, rng)
-    gla.set_tuning("chol_block", block)
-    try:
+    with gla.tuning(chol_block=block):
         out, _ = _run(lambda X: gla.cholRecursive_(X, gla.Val.L), A, np.float64)
-    finally:
-        gla.set_tuning("chol_block", 512)
     L = np.tril(out)
     Lr = np.linalg.cholesky(A)
     assert rel(L, Lr) <= 10 * n * eps_of(np.float64)
@@ -211,3 +208,40 @@ def test_rdiv_lower_adjoint(rng, m, n):
     gla.rdiv_(Bg, gla.adjoint(gla.LowerTriangular(gla.colmajor(L))))
     ref = O.trsm_rlt(B, L)
     assert rel(gla.to_numpy(Bg), ref) <= 10 * n * eps_of(np.float64)
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+@pytest.mark.parametrize("n,block", [(3000, 768), (5000, 512)])
+def test_side_cu_reserve_is_bitwise_neutral(rng, dtype, n, block):
+    """The look-ahead driver with its trailing SYRK on a CU-masked stream and
+    the chain on the handle's own stream (the default) computes the same
+    bits as with both on unmasked streams (side_cu_reserve 0), also when
+    the caller's stream is not the default one."""
+    gla = _gla()
+    A = spd(n, rng).astype(dtype)
+    outs = []
+    for reserve in (gla.get_tuning("side_cu_reserve"), 0):
+        with gla.tuning(chol_block=block, side_cu_reserve=reserve):
+            X = gla.colmajor(A)
+            s = torch.cuda.Stream()
+            s.wait_stream(torch.cuda.current_stream())
+            with torch.cuda.stream(s):
+                gla.cholRecursive_(X, gla.Val.L)
+            torch.cuda.current_stream().wait_stream(s)
+            torch.cuda.synchronize()
+            outs.append(gla.to_numpy(X))
+    assert np.array_equal(outs[0], outs[1])
+    assert rel(np.tril(outs[0]), np.linalg.cholesky(A.astype(np.float64))) <= 10 * n * eps_of(dtype)
+
+
+def test_tuning_defaults_and_restore():
+    gla = _gla()
+    assert gla.get_tuning("chol_block") == 768
+    assert gla.get_tuning("gemm_dl") == 2
+    assert gla.get_tuning("side_cu_reserve") == torch.cuda.get_device_properties(0).multi_processor_count // 8
+    with gla.tuning(chol_block=256, side_cu_reserve=0):
+        assert gla.get_tuning("chol_block") == 256
+        assert gla.get_tuning("side_cu_reserve") == 0
+    assert gla.get_tuning("chol_block") == 768
+    with pytest.raises(gla.ArgumentError):
+        gla.get_tuning("no_such_knob")

@@ -148,17 +148,14 @@ def _spd_device(n, seed, dtype=torch.float64):
     return A.to(dtype)
 
 
-@pytest.mark.parametrize("chol_block", [512, 0])
+@pytest.mark.parametrize("chol_block", [768, 512, 0])
 def test_cholesky_16384_residual_on_sampled_rows(chol_block):
     gla = _gla()
     n = 16384
     A = _spd_device(n, 1234)                 # symmetric: row- and column-major alike
     W = A.clone()                            # column-major view of the same values
-    try:
-        gla.set_tuning("chol_block", chol_block)
+    with gla.tuning(chol_block=chol_block):
         gla.cholRecursive_(W.t(), gla.Val.L)
-    finally:
-        gla.set_tuning("chol_block", 512)
     # W.t() is column-major: element (i, j) of the factor is W[j, i]
     L = torch.tril(W.t())
     rows = torch.from_numpy(np.sort(np.random.default_rng(5).choice(n, 96, replace=False)))

@@ -55,7 +55,8 @@ def test_gemm_unaligned_submatrix(rng):
 
 
 VARIANTS = [("gemm_dl", 1), ("gemm_dl", 2), ("gemm_dl", 3), ("gemm_lds1", 0)]
-DEFAULTS = {"gemm_dl": 0, "gemm_lds1": 1}
+# each variant is compared with the register-staged, single-LDS-buffer kernel
+BASE = {"gemm_dl": 0, "gemm_lds1": 1}
 
 
 @pytest.mark.parametrize("knob,val", VARIANTS)
@@ -70,13 +71,10 @@ def test_kernel_variants_match_default(rng, knob, val, m, n, k):
     B = rng.standard_normal((n, k))
     C0 = rng.standard_normal((m, n))
     outs = []
-    for v in (DEFAULTS[knob], val):
+    for v in (BASE[knob], val):
         Cg = gla.colmajor(C0)
-        try:
-            gla.set_tuning(knob, v)
+        with gla.tuning(gemm_dl=0, **{knob: v}) if knob != "gemm_dl" else gla.tuning(gemm_dl=v):
             gla.gemm_("N", "T", -1.0, gla.colmajor(A), gla.colmajor(B), 1.0, Cg)
-        finally:
-            gla.set_tuning(knob, DEFAULTS[knob])
         outs.append(gla.to_numpy(Cg))
     ref = C0 - A @ B.T
     assert rel(outs[1], ref) <= 8 * k * eps_of(np.float64)
@@ -90,13 +88,10 @@ def test_kernel_variants_syrk_lower(rng, knob, val):
     X = rng.standard_normal((n, k))
     C0 = rng.standard_normal((n, n))
     outs = []
-    for v in (DEFAULTS[knob], val):
+    for v in (BASE[knob], val):
         Cg = gla.colmajor(C0)
-        try:
-            gla.set_tuning(knob, v)
+        with gla.tuning(gemm_dl=0, **{knob: v}) if knob != "gemm_dl" else gla.tuning(gemm_dl=v):
             gla.rankUpdate_(gla.Hermitian(Cg, "L"), gla.colmajor(X), -1.0)
-        finally:
-            gla.set_tuning(knob, DEFAULTS[knob])
         outs.append(gla.to_numpy(Cg))
     ref = C0 - X @ X.T
     assert rel(np.tril(outs[1]), np.tril(ref)) <= 8 * k * eps_of(np.float64)

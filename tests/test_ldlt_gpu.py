@@ -34,13 +34,10 @@ def spd(n, rng):
 
 def _factor(A, uplo, dtype, block=512):
     gla = _gla()
-    gla.set_tuning("ldlt_block", block)
-    try:
+    with gla.tuning(ldlt_block=block):
         X = gla.colmajor(A.astype(dtype))
         F = gla.ldlt_(gla.Hermitian(X, uplo))
         torch.cuda.synchronize()
-    finally:
-        gla.set_tuning("ldlt_block", 512)
     return F, gla.to_numpy(F.data)
 
 

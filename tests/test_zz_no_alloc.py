@@ -36,16 +36,13 @@ def _noalloc(A, uplo="L", cutoff=1):
     return info.value
 
 
-def _handle_entry(A, uplo="L", cutoff=1, chol_block=512):
+def _handle_entry(A, uplo="L", cutoff=1, chol_block=768):
     gla = _gla()
     info = C.c_int64(-1)
-    gla.set_tuning("chol_block", chol_block)
-    try:
+    with gla.tuning(chol_block=chol_block):
         rc = gla.lib().gla_chol_recursive_d(gla._handle(0), uplo.encode(), A.shape[0],
                                             C.c_void_p(A.data_ptr()), A.stride(1), cutoff,
                                             C.byref(info), _stream())
-    finally:
-        gla.set_tuning("chol_block", 512)
     assert rc == 0, gla.lib().gla_last_error_string()
     return info.value
 
@@ -58,7 +55,7 @@ def _check_pair(A0, uplo):
     Z = gla.colmajor(A0)
     i_x = _noalloc(X, uplo)
     i_y = _handle_entry(Y, uplo, chol_block=0)
-    i_z = _handle_entry(Z, uplo, chol_block=512)
+    i_z = _handle_entry(Z, uplo)
     torch.cuda.synchronize()
     assert i_x == i_y == i_z
     if i_x == 0:                  # after a negative pivot the contents are NaN-laden
