@@ -520,6 +520,96 @@ __host__ __device__ constexpr int lt_bj(int q)
     return kb + 1 + (q - base);
 }
 
+// Index of tile (bj, kb), bj > kb, in Lt: column blocks stored one after
+// another.
+__device__ __forceinline__ int lt_index(int bj, int kb) { return 7 * kb - kb * (kb - 1) / 2 + (bj - kb - 1); }
+
+// Stages the strictly-lower 16×16 tiles of the nb×nb (nb ≤ 128) lower L and
+// the inverses of its diagonal tiles into LDS: thread t stages element
+// (t / 16, t % 16) of every tile, all loads issued before the first store.
+template <typename T>
+__device__ __forceinline__ void stage_l_tiles(int nb, const T *__restrict__ L, int64_t ldl,
+                                              const T *__restrict__ LinvG, T *Lt, T *Li)
+{
+    constexpr int NBLK = kPanelNB / 16;
+    constexpr int NLT = NBLK * (NBLK - 1) / 2;
+    constexpr int TS = 16 * kXLD;
+    static_assert(kPanelThreads == 256, "one tile element per thread");
+    const int nkb = (nb + 15) / 16;
+    const int i = (threadIdx.x >> 4) & 15, j = threadIdx.x & 15;
+    T lv[NLT], iv[NBLK];
+#pragma unroll
+    for (int q = 0; q < NLT; ++q) {
+        const int kb = lt_kb(q), bj = lt_bj(q);
+        const int gi = bj * 16 + i, gj = kb * 16 + j;
+        lv[q] = (gi < nb && gj < nb) ? L[gi + (int64_t)gj * ldl] : T(0);
+    }
+#pragma unroll
+    for (int q = 0; q < NBLK; ++q) iv[q] = q < nkb ? LinvG[q * 256 + threadIdx.x] : T(0);
+#pragma unroll
+    for (int q = 0; q < NLT; ++q) Lt[q * TS + i * kXLD + j] = lv[q];
+#pragma unroll
+    for (int q = 0; q < NBLK; ++q) Li[q * TS + i * kXLD + j] = iv[q];
+}
+
+// acc (this wave's 16 rows × 128 columns as eight 16×16 tiles) <- acc L⁻ᵀ
+// with L's tiles and tile inverses staged in LDS: for each column block kb
+// the wave's tile kb becomes X = B_kb L_kk⁻ᵀ and its tiles j > kb take
+// B_j -= X L_(j,kb)ᵀ, all on MFMA, with no barrier (every wave touches
+// only its own rows; Xw is the wave's own LDS tile).
+template <typename T>
+__device__ __forceinline__ void solve_tiles(int nkb, typename PanelMfma<T>::acc_t acc[8],
+                                            const T *Lt, const T *Li, T *Xw)
+{
+    typedef PanelMfma<T> M;
+    typedef typename M::acc_t acc_t;
+    constexpr int NBLK = kPanelNB / 16;
+    constexpr int TS = 16 * kXLD;
+    const int lane = threadIdx.x & 63;
+    const int lc = lane & 15;
+    for (int kb = 0; kb < nkb; ++kb) {
+        const T *Lik = Li + kb * TS;
+#pragma unroll
+        for (int bj = 0; bj < NBLK; ++bj)
+            if (bj == kb)
+#pragma unroll
+                for (int r = 0; r < 4; ++r) Xw[M::row(lane, r) * kXLD + lc] = acc[bj][r];
+        wave_order();   // the wave re-reads its own tile in operand layout
+        acc_t x = {T(0), T(0), T(0), T(0)};
+        T a[4];
+#pragma unroll
+        for (int kq = 0; kq < 4; ++kq) {
+            const int kk = (lane >> 4) + 4 * kq;
+            x = M::mma(Xw[lc * kXLD + kk], Lik[lc * kXLD + kk], x);
+        }
+        wave_order();
+#pragma unroll
+        for (int r = 0; r < 4; ++r) Xw[M::row(lane, r) * kXLD + lc] = x[r];
+#pragma unroll
+        for (int bj = 0; bj < NBLK; ++bj)
+            if (bj == kb) acc[bj] = x;
+        wave_order();
+#pragma unroll
+        for (int kq = 0; kq < 4; ++kq) a[kq] = -Xw[lc * kXLD + (lane >> 4) + 4 * kq];
+#pragma unroll
+        for (int bj = 0; bj < NBLK; ++bj) {
+            if (bj <= kb || bj >= nkb) continue;
+            const T *Lt_jk = Lt + lt_index(bj, kb) * TS;
+#pragma unroll
+            for (int kq = 0; kq < 4; ++kq) {
+                const T b = Lt_jk[lc * kXLD + (lane >> 4) + 4 * kq];
+                acc[bj] = M::mma(a[kq], b, acc[bj]);
+            }
+        }
+        wave_order();   // Xw is rewritten in the next step
+    }
+}
+
+// B (nrows×nb) <- B L⁻ᵀ for the lower nb×nb L (nb ≤ 128) whose 16×16
+// diagonal-tile inverses are in LinvG — rdiv!(A21, LowerTriangular(A11)'),
+// src/cholesky.jl:28,48.  Each workgroup takes 64 rows; wave w holds tile
+// row w of them (8 tiles).  L's 28 strictly-lower tiles and the 8 inverses
+// are read into LDS once, then solve_tiles runs the column sweep.
 template <typename T>
 __global__ __launch_bounds__(kPanelThreads) void chol_solve_kernel(int nb, const T *__restrict__ L,
                                                                    int64_t ldl,
@@ -545,29 +635,8 @@ __global__ __launch_bounds__(kPanelThreads) void chol_solve_kernel(int nb, const
     const int lc = lane & 15;
     const int64_t row0 = (int64_t)blockIdx.x * kPanelRB;
     const int nkb = (nb + 15) / 16;
-    // index of tile (bj, kb), bj > kb, in Lt: column blocks stored one after another
-    auto lt_index = [](int bj, int kb) { return 7 * kb - kb * (kb - 1) / 2 + (bj - kb - 1); };
 
-    // thread t stages element (i, j) = (t / 16, t % 16) of every tile: all
-    // loads are issued before the first LDS store
-    static_assert(kPanelThreads == 256, "one tile element per thread");
-    {
-        const int i = (threadIdx.x >> 4) & 15, j = threadIdx.x & 15;
-        T lv[NLT], iv[NBLK];
-#pragma unroll
-        for (int q = 0; q < NLT; ++q) {
-            const int kb = lt_kb(q), bj = lt_bj(q);
-            const int gi = bj * 16 + i, gj = kb * 16 + j;
-            lv[q] = (gi < nb && gj < nb) ? L[gi + (int64_t)gj * ldl] : T(0);
-        }
-#pragma unroll
-        for (int q = 0; q < NBLK; ++q) iv[q] = q < nkb ? LinvG[q * 256 + threadIdx.x] : T(0);
-#pragma unroll
-        for (int q = 0; q < NLT; ++q) Lt[q * TS + i * kXLD + j] = lv[q];
-#pragma unroll
-        for (int q = 0; q < NBLK; ++q) Li[q * TS + i * kXLD + j] = iv[q];
-    }
-
+    stage_l_tiles<T>(nb, L, ldl, LinvG, Lt, Li);
     acc_t acc[NBLK];
 #pragma unroll
     for (int bj = 0; bj < NBLK; ++bj)
@@ -578,46 +647,7 @@ __global__ __launch_bounds__(kPanelThreads) void chol_solve_kernel(int nb, const
             acc[bj][r] = (gr < nrows && gj < nb) ? B[gr + (int64_t)gj * ldb] : T(0);
         }
     __syncthreads();
-
-    T *Xw = Xs + w * TS;
-    for (int kb = 0; kb < nkb; ++kb) {
-        const T *Lik = Li + kb * TS;
-        // this wave's tile of column kb: X = B_kb L_kk⁻ᵀ
-#pragma unroll
-        for (int bj = 0; bj < NBLK; ++bj)
-            if (bj == kb)
-#pragma unroll
-                for (int r = 0; r < 4; ++r) Xw[M::row(lane, r) * kXLD + lc] = acc[bj][r];
-        wave_order();   // the wave re-reads its own tile in operand layout
-        acc_t x = {T(0), T(0), T(0), T(0)};
-        T a[4];
-#pragma unroll
-        for (int kq = 0; kq < 4; ++kq) {
-            const int kk = (lane >> 4) + 4 * kq;
-            x = M::mma(Xw[lc * kXLD + kk], Lik[lc * kXLD + kk], x);
-        }
-        wave_order();
-#pragma unroll
-        for (int r = 0; r < 4; ++r) Xw[M::row(lane, r) * kXLD + lc] = x[r];
-#pragma unroll
-        for (int bj = 0; bj < NBLK; ++bj)
-            if (bj == kb) acc[bj] = x;
-        wave_order();
-#pragma unroll
-        for (int kq = 0; kq < 4; ++kq) a[kq] = -Xw[lc * kXLD + (lane >> 4) + 4 * kq];
-        // tiles j > kb: B_j -= X L_(j,kb)ᵀ
-#pragma unroll
-        for (int bj = 0; bj < NBLK; ++bj) {
-            if (bj <= kb || bj >= nkb) continue;
-            const T *Lt_jk = Lt + lt_index(bj, kb) * TS;
-#pragma unroll
-            for (int kq = 0; kq < 4; ++kq) {
-                const T b = Lt_jk[lc * kXLD + (lane >> 4) + 4 * kq];
-                acc[bj] = M::mma(a[kq], b, acc[bj]);
-            }
-        }
-        wave_order();   // Xw is rewritten in the next step
-    }
+    solve_tiles<T>(nkb, acc, Lt, Li, Xs + w * TS);
 #pragma unroll
     for (int bj = 0; bj < NBLK; ++bj)
 #pragma unroll
@@ -626,6 +656,116 @@ __global__ __launch_bounds__(kPanelThreads) void chol_solve_kernel(int nb, const
             const int gj = bj * 16 + lc;
             if (gr < nrows && gj < nb) B[gr + (int64_t)gj * ldb] = acc[bj][r];
         }
+}
+
+// B (nrows×nb) <- B L⁻ᵀ for a wide lower L (nb a multiple of 128, up to
+// kPsolveMaxNB), already factored, with its 16×16 diagonal-tile inverses in
+// LinvG: rdiv!(A21, LowerTriangular(A11)') (src/cholesky.jl:28) of a whole
+// look-ahead panel in one launch.  Each workgroup takes 64 rows and sweeps
+// the 128-wide column blocks left to right: block c first takes the
+// products of the blocks already solved, B_c -= Σ_{c'<c} X_c' L(c,c')ᵀ, as
+// one MFMA product over K = 128c with operands staged through LDS in
+// 32-wide slabs, then X_c = B_c L_cc⁻ᵀ by solve_tiles.  A workgroup reads
+// only L and its own rows, so no workgroup waits for another.
+constexpr int kPsolveMaxNB = 768;   // wider panels: the recursive solve is faster
+constexpr int kPsolveKC = 32;                 // K slab of the product
+constexpr int kPsolveGLD = kPsolveKC + 1;     // padded LDS row
+template <typename T>
+__global__ __launch_bounds__(kPanelThreads) void chol_psolve_kernel(int nb, const T *__restrict__ L,
+                                                                    int64_t ldl,
+                                                                    const T *__restrict__ LinvG,
+                                                                    T *__restrict__ B, int64_t ldb,
+                                                                    int64_t nrows)
+{
+    typedef PanelMfma<T> M;
+    typedef typename M::acc_t acc_t;
+    constexpr int NBLK = kPanelNB / 16;
+    constexpr int NLT = NBLK * (NBLK - 1) / 2;
+    constexpr int TS = 16 * kXLD;
+    __shared__ T Lt[NLT * TS];
+    __shared__ T Li[NBLK * TS];
+    __shared__ T Xs[4 * TS];
+    __shared__ T Gx[kPanelRB * kPsolveGLD];   // this workgroup's solved rows, one K slab
+    __shared__ T Gl[kPanelNB * kPsolveGLD];   // rows of block c of L, the same K slab
+    const int lane = threadIdx.x & 63;
+    const int w = threadIdx.x >> 6;
+    const int lc = lane & 15;
+    const int64_t row0 = (int64_t)blockIdx.x * kPanelRB;
+    const int nc = nb / kPanelNB;
+    for (int c = 0; c < nc; ++c) {
+        acc_t acc[NBLK];
+#pragma unroll
+        for (int bj = 0; bj < NBLK; ++bj)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int64_t gr = row0 + w * 16 + M::row(lane, r);
+                const int64_t gj = (int64_t)c * kPanelNB + bj * 16 + lc;
+                acc[bj][r] = gr < nrows ? B[gr + gj * ldb] : T(0);
+            }
+        // B_c -= X[:, 0:128c] L[128c : 128c+128, 0:128c]ᵀ, the global loads
+        // of slab s + 1 in flight while slab s is multiplied
+        const T *Lc = L + (int64_t)c * kPanelNB;
+        constexpr int XE = kPanelRB * kPsolveKC / kPanelThreads;   // 8 per thread
+        constexpr int LE = kPanelNB * kPsolveKC / kPanelThreads;   // 16 per thread
+        const int kend = c * kPanelNB;
+        T xv[XE], lv[LE];
+        auto fetch = [&](int k0) {
+#pragma unroll
+            for (int e = 0; e < XE; ++e) {
+                const int idx = threadIdx.x + e * kPanelThreads;
+                const int i = idx % kPanelRB, kk = idx / kPanelRB;
+                xv[e] = row0 + i < nrows ? B[row0 + i + (int64_t)(k0 + kk) * ldb] : T(0);
+            }
+#pragma unroll
+            for (int e = 0; e < LE; ++e) {
+                const int idx = threadIdx.x + e * kPanelThreads;
+                const int i = idx % kPanelNB, kk = idx / kPanelNB;
+                lv[e] = Lc[i + (int64_t)(k0 + kk) * ldl];
+            }
+        };
+        // block c − 1 was stored by other waves at the end of the last sweep
+        __syncthreads();
+        if (kend > 0) fetch(0);
+        for (int k0 = 0; k0 < kend; k0 += kPsolveKC) {
+            __syncthreads();   // the previous slab has been read
+#pragma unroll
+            for (int e = 0; e < XE; ++e) {
+                const int idx = threadIdx.x + e * kPanelThreads;
+                Gx[(idx % kPanelRB) * kPsolveGLD + idx / kPanelRB] = xv[e];
+            }
+#pragma unroll
+            for (int e = 0; e < LE; ++e) {
+                const int idx = threadIdx.x + e * kPanelThreads;
+                Gl[(idx % kPanelNB) * kPsolveGLD + idx / kPanelNB] = lv[e];
+            }
+            __syncthreads();
+            if (k0 + kPsolveKC < kend) fetch(k0 + kPsolveKC);
+#pragma unroll
+            for (int kq = 0; kq < kPsolveKC / 4; ++kq) {
+                const int kk = 4 * kq + (lane >> 4);
+                const T a = -Gx[(w * 16 + lc) * kPsolveGLD + kk];
+#pragma unroll
+                for (int bj = 0; bj < NBLK; ++bj)
+                    acc[bj] = M::mma(a, Gl[(bj * 16 + lc) * kPsolveGLD + kk], acc[bj]);
+            }
+        }
+        // X_c = B_c L_cc⁻ᵀ
+        __syncthreads();   // Lt/Li of the previous block have been read
+        stage_l_tiles<T>(kPanelNB, L + (int64_t)c * kPanelNB * (1 + ldl), ldl,
+                         LinvG + (int64_t)c * NBLK * 256, Lt, Li);
+        __syncthreads();
+        solve_tiles<T>(NBLK, acc, Lt, Li, Xs + w * TS);
+#pragma unroll
+        for (int bj = 0; bj < NBLK; ++bj)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int64_t gr = row0 + w * 16 + M::row(lane, r);
+                const int64_t gj = (int64_t)c * kPanelNB + bj * 16 + lc;
+                if (gr < nrows) B[gr + gj * ldb] = acc[bj][r];
+            }
+        // the stores of block c are read back by this workgroup's later
+        // slabs (workgroup-scope visibility at the barrier that opens them)
+    }
 }
 
 // Swaps the strict upper and strict lower triangles of an n×n matrix, in
@@ -660,6 +800,17 @@ int launch_solve(hipStream_t s, int nb, const T *L, int64_t ldl, const T *LinvG,
     if (nrows <= 0) return GLA_OK;
     hipLaunchKernelGGL((chol_solve_kernel<T>), dim3((unsigned)cdiv(nrows, kPanelRB)),
                        dim3(kPanelThreads), 0, s, nb, L, ldl, LinvG, B, ldb, nrows, 0, 0, 0);
+    GLA_LAUNCH_CHECK();
+    return GLA_OK;
+}
+
+template <typename T>
+int launch_psolve(hipStream_t s, int nb, const T *L, int64_t ldl, const T *LinvG, T *B,
+                  int64_t ldb, int64_t nrows)
+{
+    if (nrows <= 0) return GLA_OK;
+    hipLaunchKernelGGL((chol_psolve_kernel<T>), dim3((unsigned)cdiv(nrows, kPanelRB)),
+                       dim3(kPanelThreads), 0, s, nb, L, ldl, LinvG, B, ldb, nrows);
     GLA_LAUNCH_CHECK();
     return GLA_OK;
 }
@@ -702,9 +853,12 @@ int launch_tile_inverse(hipStream_t s, int64_t n, const T *L, int64_t ldl, T *Li
 inline size_t linv_elems(int64_t n) { return (size_t)(cdiv(n, 16) + 8) * 256; }
 
 // Right-looking blocked Cholesky of the lower triangle with look-ahead.
+// keep_linv: every block's 16×16 tile inverses stay in Linv at the block's
+// own offset (tile q of the whole matrix at q·256), for a solve that
+// follows; otherwise each block reuses the start of Linv.
 template <typename T>
 int chol_blocked_dev(Handle *h, hipStream_t s, int64_t n, T *A, int64_t lda, int nb,
-                     int64_t info_base, T *Linv, bool use_side = true)
+                     int64_t info_base, T *Linv, bool use_side = true, bool keep_linv = false)
 {
     if (n <= 0) return GLA_OK;
     const int64_t nsteps = cdiv(n, nb);
@@ -719,8 +873,9 @@ int chol_blocked_dev(Handle *h, hipStream_t s, int64_t n, T *A, int64_t lda, int
         const int64_t below = n - k - kb;
         T *Akk = A + k + k * lda;
         T *A21 = Akk + kb;
-        GLA_TRY(launch_factor<T>(s, kb, Akk, lda, h->dinfo, info_base + k, Linv));  // :25
-        GLA_TRY(launch_solve<T>(s, kb, Akk, lda, Linv, A21, lda, below));           // :27-28
+        T *Lv = keep_linv ? Linv + (k / 16) * 256 : Linv;
+        GLA_TRY(launch_factor<T>(s, kb, Akk, lda, h->dinfo, info_base + k, Lv));    // :25
+        GLA_TRY(launch_solve<T>(s, kb, Akk, lda, Lv, A21, lda, below));             // :27-28
         if (below == 0) break;
         if (!overlap) {
             // :30-31 rankUpdate! of all of A22 in one launch: with nothing
@@ -776,7 +931,8 @@ int trsm_rlt_dev(hipStream_t s, int64_t m, int64_t n, const T *L, int64_t ldl, c
 // Blocked right-looking Cholesky with wide blocks (NB > 128) and depth-1
 // look-ahead — cholBlocked!'s loop (src/cholesky.jl:17-35) with the
 // diagonal block factored by the 128-wide driver on the caller's stream,
-// the panel solved by the recursive rdiv!, the next block column updated on
+// the panel solved in one chol_psolve_kernel launch (or the recursive
+// rdiv!), the next block column updated on
 // the caller's stream and the rest of the trailing rankUpdate! (K = NB, an
 // efficient MFMA SYRK) on the side stream, where it overlaps the next
 // diagonal factor and panel solve.
@@ -805,10 +961,18 @@ int chol_la_dev(Handle *h, hipStream_t s0, int64_t n, T *A, int64_t lda, int NB,
         const int64_t below = n - k - kb;
         T *Akk = A + k + k * lda;
         T *A21 = Akk + kb;
-        GLA_TRY(chol_blocked_dev<T>(h, s, kb, Akk, lda, kPanelNB, info_base + k, Linv, false));
+        // :27-28 rdiv! of the panel: one launch over all of it when the block
+        // is a whole number of 128-wide tiles, else the recursive solve
+        const bool fused = h->chol_psolve != 0 && kb % kPanelNB == 0 && kb <= kPsolveMaxNB;
+        GLA_TRY(chol_blocked_dev<T>(h, s, kb, Akk, lda, kPanelNB, info_base + k, Linv, false,
+                                    fused));
         if (below == 0) break;
-        GLA_TRY(launch_tile_inverse<T>(s, kb, Akk, lda, Linv));
-        GLA_TRY(trsm_rlt_dev<T>(s, below, kb, Akk, lda, Linv, A21, lda));
+        if (fused) {
+            GLA_TRY(launch_psolve<T>(s, (int)kb, Akk, lda, Linv, A21, lda, below));
+        } else {
+            GLA_TRY(launch_tile_inverse<T>(s, kb, Akk, lda, Linv));
+            GLA_TRY(trsm_rlt_dev<T>(s, below, kb, Akk, lda, Linv, A21, lda));
+        }
         const int64_t kb2 = std::min<int64_t>(NB, below);
         const int64_t rest = below - kb2;
         // the side stream's SYRK is released after the next-block-column
@@ -991,6 +1155,18 @@ GLA_INST_BLOCKS(float)
 #undef GLA_INST_BLOCKS
 
 // algo: 0 unblocked, 1 blocked (param = blocksize), 2 recursive (param = cutoff)
+// Step width of the look-ahead driver for chol_block −1 (the default), from
+// n = 2048…16384 sweeps on MI355X: below 3072 the recursive driver (0), then
+// 256-wide steps while the chain of diagonal factor and panel solve sets the
+// pace, wider steps as the trailing SYRK grows to dominate.
+inline int64_t auto_chol_block(int64_t n)
+{
+    if (n < 3072) return 0;
+    if (n < 10240) return 256;
+    if (n < 14336) return 512;
+    return 1024;
+}
+
 template <typename T>
 int chol_device(Handle *h, hipStream_t s, int64_t n, T *A, int64_t lda, int algo, int64_t param,
                 T *Linv)
@@ -1013,7 +1189,8 @@ int chol_device(Handle *h, hipStream_t s, int64_t n, T *A, int64_t lda, int algo
     // cholRecursive!: the reference recurses to 1×1 (cutoff does not reach
     // the A11 branch, :46); on the device the recursion stops at a leaf of
     // at least 2048 (or `cutoff` if larger) that the blocked driver factors.
-    if (h->chol_block > kPanelNB) return chol_la_dev<T>(h, s, n, A, lda, (int)h->chol_block, 0, Linv);
+    const int64_t NB = h->chol_block < 0 ? auto_chol_block(n) : h->chol_block;
+    if (NB > kPanelNB) return chol_la_dev<T>(h, s, n, A, lda, (int)NB, 0, Linv);
     int64_t leaf = std::max<int64_t>(param, 2048);
     return chol_recursive_dev<T>(h, s, n, A, lda, leaf, 0, Linv);
 }

@@ -77,8 +77,12 @@ struct Handle {
     size_t ws_bytes = 0;
     int num_cus = 256;
     // Tuning: block width of the look-ahead driver behind cholRecursive!
-    // (0: the recursive-halving driver).  Set with gla_set_tuning.
-    int64_t chol_block = 768;
+    // (0: the recursive-halving driver; −1: chosen from n).  Set with
+    // gla_set_tuning.
+    int64_t chol_block = -1;
+    // Solve the look-ahead Cholesky's panels in one launch each (1) or by the
+    // recursive rdiv! (0).
+    int chol_psolve = 1;
     // Release the look-ahead Cholesky's side-stream SYRK after the
     // next-block-column update (1) rather than right after the panel solve.
     int chol_syrk_late = 1;

@@ -75,8 +75,8 @@ int gla_set_tuning(gla_handle_t hh, const char *key, int64_t value)
     if (!h) { gla::set_error("null handle"); return -1; }
     if (!key) return -2;
     if (std::strcmp(key, "chol_block") == 0) {
-        if (value != 0 && (value < 128 || value % 128 != 0)) {
-            gla::set_error("chol_block must be 0 or a multiple of 128");
+        if (value != 0 && value != -1 && (value < 128 || value % 128 != 0)) {
+            gla::set_error("chol_block must be -1, 0 or a multiple of 128");
             return -3;
         }
         h->chol_block = value;
@@ -138,6 +138,14 @@ int gla_set_tuning(gla_handle_t hh, const char *key, int64_t value)
         h->qr_hr = value;
         return GLA_OK;
     }
+    if (std::strcmp(key, "chol_psolve") == 0) {
+        if (value != 0 && value != 1) {
+            gla::set_error("chol_psolve must be 0 or 1");
+            return -3;
+        }
+        h->chol_psolve = (int)value;
+        return GLA_OK;
+    }
     if (std::strcmp(key, "chol_syrk_late") == 0) {
         if (value != 0 && value != 1) {
             gla::set_error("chol_syrk_late must be 0 or 1");
@@ -193,6 +201,7 @@ int gla_get_tuning(gla_handle_t hh, const char *key, int64_t *value)
         {"qr_inner", h->qr_inner},           {"gemm_dl", gla::get_gemm_dl()},
         {"gemm_lds1", gla::get_gemm_lds1()}, {"qr_hr", h->qr_hr},
         {"chol_syrk_late", h->chol_syrk_late}, {"qr_lookahead", h->qr_lookahead},
+        {"chol_psolve", h->chol_psolve},
         {"spin_limit", (int64_t)h->spin_limit}, {"side_cu_reserve", h->side_cu_reserve},
         {"qr_wait_test", h->qr_wait_test},
     };

@@ -64,10 +64,13 @@ const char *gla_last_error_string(void);
 /* Version string of this build (no reference counterpart). */
 const char *gla_version(void);
 /* Tuning knobs of a handle (no reference counterpart).  "chol_block": a
- * multiple of 128 (default 768) runs cholRecursive! as a right-looking
- * blocked factorisation with blocks of that width and depth-1 look-ahead;
- * 0 runs it as recursive halving down to 2048-wide leaves.  Both compute
- * the same factor.  "chol_syrk_late": 1 (default) starts the look-ahead
+ * multiple of 128 runs cholRecursive! as a right-looking blocked
+ * factorisation with blocks of that width and depth-1 look-ahead; 0 runs it
+ * as recursive halving down to 2048-wide leaves; −1 (default) picks from n
+ * (0 below 3072, then 256, 512 from 10240, 1024 from 14336).  All compute
+ * the same factor up to rounding.  "chol_psolve": 1 (default) solves each
+ * look-ahead panel up to 768 wide in one launch (every 64 rows sweep the
+ * 128-wide column blocks), 0 by the recursive rdiv!.  "chol_syrk_late": 1 (default) starts the look-ahead
  * driver's trailing SYRK on the side stream after the next block column's
  * update, 0 right after the panel solve (same factor).  "side_cu_reserve":
  * CUs (0 to CUs/2; a new handle takes CUs/8, 32 of 256) that the look-ahead

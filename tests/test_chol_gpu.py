@@ -81,7 +81,7 @@ def test_sizes_against_oracle(rng, dtype, n, algo):
 
 
 @pytest.mark.parametrize("n", [2500, 4100])
-@pytest.mark.parametrize("block", [0, 256, 512, 768, 1024])
+@pytest.mark.parametrize("block", [-1, 0, 256, 512, 768, 1024])
 def test_large_recursive_backward_error(rng, n, block):
     """Both device drivers behind cholRecursive!: recursive halving past the
     2048 leaf (block 0) and the wide-block look-ahead (block 256 to 1024);
@@ -236,12 +236,35 @@ def test_side_cu_reserve_is_bitwise_neutral(rng, dtype, n, block):
 
 def test_tuning_defaults_and_restore():
     gla = _gla()
-    assert gla.get_tuning("chol_block") == 768
+    assert gla.get_tuning("chol_block") == -1
     assert gla.get_tuning("gemm_dl") == 2
+    assert gla.get_tuning("chol_psolve") == 1
     assert gla.get_tuning("side_cu_reserve") == torch.cuda.get_device_properties(0).multi_processor_count // 8
     with gla.tuning(chol_block=256, side_cu_reserve=0):
         assert gla.get_tuning("chol_block") == 256
         assert gla.get_tuning("side_cu_reserve") == 0
-    assert gla.get_tuning("chol_block") == 768
+    assert gla.get_tuning("chol_block") == -1
     with pytest.raises(gla.ArgumentError):
         gla.get_tuning("no_such_knob")
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+@pytest.mark.parametrize("n,block", [(1100, 256), (2000, 384), (2500, 768), (3000, 640),
+                                     (3000, 1024)])
+def test_fused_panel_solve_matches_recursive(rng, dtype, n, block):
+    """The look-ahead driver's panels solved in one launch each
+    (chol_psolve 1, the default; blocks up to 768 wide, wider ones take the
+    recursive solve either way) against the recursive rdiv! (0) and
+    LAPACK; ragged last steps and row counts that are not multiples of the
+    64-row strips included; the strict upper triangle untouched."""
+    gla = _gla()
+    A = spd(n, rng).astype(dtype)
+    outs = []
+    for ps in (1, 0):
+        with gla.tuning(chol_block=block, chol_psolve=ps):
+            out, _ = _run(lambda X: gla.cholRecursive_(X, gla.Val.L), A, dtype)
+        outs.append(out)
+    tol = 10 * n * eps_of(dtype)
+    assert rel(np.tril(outs[0]), np.tril(outs[1])) <= tol
+    assert rel(np.tril(outs[0]), np.linalg.cholesky(A.astype(np.float64))) <= tol
+    assert np.array_equal(np.triu(outs[0], 1), np.triu(A, 1))

@@ -148,7 +148,7 @@ def _spd_device(n, seed, dtype=torch.float64):
     return A.to(dtype)
 
 
-@pytest.mark.parametrize("chol_block", [768, 512, 0])
+@pytest.mark.parametrize("chol_block", [-1, 768, 0])
 def test_cholesky_16384_residual_on_sampled_rows(chol_block):
     gla = _gla()
     n = 16384

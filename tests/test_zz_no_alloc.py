@@ -36,7 +36,7 @@ def _noalloc(A, uplo="L", cutoff=1):
     return info.value
 
 
-def _handle_entry(A, uplo="L", cutoff=1, chol_block=768):
+def _handle_entry(A, uplo="L", cutoff=1, chol_block=-1):
     gla = _gla()
     info = C.c_int64(-1)
     with gla.tuning(chol_block=chol_block):
