@@ -76,16 +76,6 @@ __device__ __forceinline__ T lane_bcast(T v, int src)
     }
 }
 
-// v from lane K of each 16-lane row to the whole row (DPP row_newbcast:K).
-// Every lane reads a valid source, so no "old" value needs setting up; a
-// 64-bit value moves in one v_mov_b64_dpp (gfx950 has DPP on 64-bit moves
-// for row_newbcast).
-template <int K, typename T>
-__device__ __forceinline__ T row_bcast(T v)
-{
-    constexpr int ctrl = 0x150 + K;
-    return __builtin_amdgcn_update_dpp(v, v, ctrl, 0xF, 0xF, true);
-}
 
 // s = √a and r = 1/√a from the hardware reciprocal square root refined by
 // Newton steps (two for FP64, one for FP32) and one correction of s = a·r:

@@ -39,6 +39,21 @@ void set_error(const char *fmt, ...);
         }                                                                              \
     } while (0)
 
+// v from lane K of each 16-lane row to the whole row (DPP row_newbcast:K).
+// Every lane reads a valid source, so no "old" value needs setting up; a
+// 64-bit value moves in one v_mov_b64_dpp (gfx950 has DPP on 64-bit moves
+// for row_newbcast).
+template <int K, typename T>
+__device__ __forceinline__ T row_bcast(T v)
+{
+    static_assert(K >= 0 && K < 16, "lane of a 16-lane row");
+    return __builtin_amdgcn_update_dpp(v, v, 0x150 + K, 0xF, 0xF, true);
+}
+
+// Fused multiply-add in T's own precision.
+__device__ __forceinline__ double fma_t(double a, double b, double c) { return __builtin_fma(a, b, c); }
+__device__ __forceinline__ float fma_t(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
+
 // 16×16×4 MFMA with one A and one B element per lane; accumulator element r
 // of lane l sits at (row(l, r), l % 16) of the 16×16 tile.
 template <typename T>

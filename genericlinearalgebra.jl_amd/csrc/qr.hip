@@ -1021,6 +1021,51 @@ __device__ __forceinline__ T recip1(T u)
     return r;
 }
 
+// Column C of the elimination of a 16×16 diagonal block held in row layout
+// (lane r of each 16-lane row holds d[c] = M[jb + r][jb + c]): the pivot
+// −q̂_CC comes from lane C by DPP broadcast, S_C = copysign(1, −q̂_CC) and
+// u_C = −q̂_CC + S_C; rows r > C take l = d[C]/u_C and d[c] −= l·u_(C,c)
+// for c > C, the pivot row's entries broadcast from lane C as well.  The
+// steps are branch-free (selects), so the 16 columns form one block the
+// compiler can schedule across.
+template <int C, int J, typename T>
+struct HrLuRow {
+    static __device__ __forceinline__ void run(T d[16], T l, bool below)
+    {
+        const T u = row_bcast<C>(d[J]);
+        const T nd = fma_t(-l, u, d[J]);
+        d[J] = below ? nd : d[J];
+        HrLuRow<C, J + 1, T>::run(d, l, below);
+    }
+};
+template <int C, typename T>
+struct HrLuRow<C, 16, T> {
+    static __device__ __forceinline__ void run(T *, T, bool) {}
+};
+template <int C, typename T>
+struct HrLuCols {
+    static __device__ __forceinline__ void run(T d[16], T dv[16], T uv[16], int r, int nbk)
+    {
+        if (C < nbk) {
+            const T mjj = row_bcast<C>(d[C]);   // −q̂_CC
+            const T dj = dcopysign<T>(T(1), mjj);
+            const T uj = mjj + dj;              // |u_C| = 1 + |q̂_CC|
+            const T ru = recip1(uj);
+            const bool below = r > C;
+            const T l = d[C] * ru;
+            HrLuRow<C, C + 1, T>::run(d, l, below);
+            d[C] = below ? l : (r == C ? uj : d[C]);
+            dv[C] = dj;
+            uv[C] = uj;
+        }
+        HrLuCols<C + 1, T>::run(d, dv, uv, r, nbk);
+    }
+};
+template <typename T>
+struct HrLuCols<16, T> {
+    static __device__ __forceinline__ void run(T *, T *, T *, int, int) {}
+};
+
 // Steps (2)-(3) on the top block: one workgroup of 4 waves, k ≤ 128, the
 // block in LDS (M, column-major), blocked by 16 columns as a right-looking
 // LU without pivoting.  Per block J = jb..jb+15:
@@ -1085,31 +1130,17 @@ __global__ __launch_bounds__(kHrThreads) void hr_lu_kernel(int k, T *Q, int64_t 
             // scheduling fences between the chained stages of this wave's
             // work keep their loads from being hoisted into the elimination
             __builtin_amdgcn_sched_barrier(0);
-#pragma unroll
-            for (int c = 0; c < 16; ++c) {
-                if (c < nbk) {
-                    const T mjj = wave_bcast(d[c], c);        // −q̂_jj
-                    const T dj = dcopysign<T>(T(1), mjj);
-                    const T uj = mjj + dj;                    // |uj| = 1 + |q̂_jj|
-                    const T ru = recip1(uj);
-                    T rowc[16];
-#pragma unroll
-                    for (int c2 = 0; c2 < 16; ++c2) rowc[c2] = c2 > c ? wave_bcast(d[c2], c) : T(0);
-                    if (r > c) {
-                        const T l = d[c] * ru;
-                        d[c] = l;
-#pragma unroll
-                        for (int c2 = 0; c2 < 16; ++c2)
-                            if (c2 > c) d[c2] -= l * rowc[c2];
-                    }
-                    if (r == c) d[c] = uj;
-                    if (lane == 0) {
-                        Dv[jb + c] = dj;
-                        Uv[jb + c] = uj;
-                    }
-                }
-            }
+            T dv[16], uv[16];
+            HrLuCols<0, T>::run(d, dv, uv, r, nbk);
             __builtin_amdgcn_sched_barrier(0);
+            if (lane == 0) {
+#pragma unroll
+                for (int c = 0; c < 16; ++c)
+                    if (c < nbk) {
+                        Dv[jb + c] = dv[c];
+                        Uv[jb + c] = uv[c];
+                    }
+            }
             if (lane < 16 && jb + r < 128) {
 #pragma unroll
                 for (int c = 0; c < 16; ++c) M[jb + r + (jb + c) * kHrLD] = d[c];
@@ -1639,9 +1670,21 @@ int qr_blocked_dev(Handle *h, hipStream_t s, int64_t m, int64_t n, T *A, int64_t
 
     hipLaunchKernelGGL(qr_reset_kernel, dim3(1), dim3(1), 0, s, h->dbar, h->dinfo + 1);
     GLA_LAUNCH_CHECK();
+    // With CUs reserved from the side stream (side_cu_reserve), the panel
+    // chain runs on the handle's chain stream with every CU and the trailing
+    // updates on the masked one, both forked from (and joined back to) the
+    // caller's stream: the one-workgroup steps of the panel then find CUs
+    // beside the trailing products.
+    const bool masked = la && h->masked != nullptr;
+    hipStream_t s0 = s;
+    hipStream_t side = masked ? h->masked : h->side;
     if (la) {
-        GLA_HIP_TRY(hipEventRecord(h->ev_fork, s));
-        GLA_HIP_TRY(hipStreamWaitEvent(h->side, h->ev_fork, 0));
+        GLA_HIP_TRY(hipEventRecord(h->ev_fork, s0));
+        GLA_HIP_TRY(hipStreamWaitEvent(side, h->ev_fork, 0));
+        if (masked) {
+            GLA_HIP_TRY(hipStreamWaitEvent(h->chain, h->ev_fork, 0));
+            s = h->chain;
+        }
     }
     unsigned bar_base = 0;
     int64_t step = 0;
@@ -1669,23 +1712,27 @@ int qr_blocked_dev(Handle *h, hipStream_t s, int64_t m, int64_t n, T *A, int64_t
                 GLA_TRY(qr_work<T>(s, (int)k, w2, Vw[q], mm, W2[0], work, ldw));
                 if (rest > 0) {
                     GLA_HIP_TRY(hipEventRecord(h->ev_step[0], s));
-                    GLA_HIP_TRY(hipStreamWaitEvent(h->side, h->ev_step[0], 0));
-                    GLA_TRY(wy_update<T>(h->side, 'T', mm, rest, k, Vw[q], mm, Tw[q], k,
+                    GLA_HIP_TRY(hipStreamWaitEvent(side, h->ev_step[0], 0));
+                    GLA_TRY(wy_update<T>(side, 'T', mm, rest, k, Vw[q], mm, Tw[q], k,
                                          Ab + (w + w2) * lda, lda, W1[1], W2[1], split[1], sp));
                     // the same stream orders this step's columns after the
-                    // previous step's; the caller's stream writes columns
-                    // [0, w2) only, disjoint from these
-                    GLA_TRY(qr_work<T>(h->side, (int)k, rest, Vw[q], mm, W2[1],
+                    // previous step's; the chain writes columns [0, w2)
+                    // only, disjoint from these
+                    GLA_TRY(qr_work<T>(side, (int)k, rest, Vw[q], mm, W2[1],
                                        work ? work + w2 * ldw : nullptr, ldw));
-                    GLA_HIP_TRY(hipEventRecord(h->ev_step[1], h->side));
+                    GLA_HIP_TRY(hipEventRecord(h->ev_step[1], side));
                 }
             }
         }
         if (!(mm > nb && nn > nb)) break;                                 // :136
     }
     if (la) {
-        GLA_HIP_TRY(hipEventRecord(h->ev_join, h->side));
-        GLA_HIP_TRY(hipStreamWaitEvent(s, h->ev_join, 0));
+        GLA_HIP_TRY(hipEventRecord(h->ev_join, side));
+        GLA_HIP_TRY(hipStreamWaitEvent(s0, h->ev_join, 0));
+        if (masked) {
+            GLA_HIP_TRY(hipEventRecord(h->ev_fork, s));
+            GLA_HIP_TRY(hipStreamWaitEvent(s0, h->ev_fork, 0));
+        }
     }
     return GLA_OK;
 }

@@ -364,3 +364,27 @@ def test_block_column_beyond_capacity_is_refused_cleanly():
     A = gla.colmajor(np.ones((140000, 32)))
     with pytest.raises(RuntimeError, match="exceeds the register-resident panel"):
         gla.qrBlocked_(A, 32)
+
+
+def test_masked_side_stream_is_bitwise_neutral(rng):
+    """The look-ahead driver with its trailing updates on the CU-masked side
+    stream and the panel chain on the handle's chain stream (the default)
+    computes the same bits as with side_cu_reserve 0, also when called on a
+    stream other than the default one."""
+    gla = _gla()
+    A = rng.standard_normal((3000, 1000))
+    outs = []
+    for reserve in (gla.get_tuning("side_cu_reserve"), 0):
+        with gla.tuning(side_cu_reserve=reserve):
+            X = gla.colmajor(A)
+            st = torch.cuda.Stream()
+            st.wait_stream(torch.cuda.current_stream())
+            with torch.cuda.stream(st):
+                F = gla.qrBlocked_(X, 128)
+            torch.cuda.current_stream().wait_stream(st)
+            torch.cuda.synchronize()
+            outs.append((gla.to_numpy(F.factors), F.tau.cpu().numpy()))
+    assert np.array_equal(outs[0][0], outs[1][0])
+    assert np.array_equal(outs[0][1], outs[1][1])
+    ofac, otau = O.qr_blocked(A, 128)
+    assert rel(outs[0][0], ofac) <= 10 * 3000 * eps_of(np.float64)
