@@ -29,6 +29,9 @@ sys.path[:0] = [os.path.join(ROOT, "genericlinearalgebra.jl_amd"), os.path.join(
 FP64_PEAK_TFLOPS = 78.6
 N_CHOL = 16384
 M_QR, N_QR, NB_QR = 8192, 4096, 128
+# the look-ahead step width cholRecursive! takes at n = 16384 (handle knob
+# chol_block −1: 1024 from n = 14336 up); the first step's trailing SYRK has K = SYRK_K
+SYRK_K = 1024
 
 
 def chol_flops(n):
@@ -162,8 +165,8 @@ def lapack_line():
 
 def profiled_traffic(name):
     """HBM bytes per launch measured with rocprofv3 --pmc FETCH_SIZE and
-    WRITE_SIZE, as recorded in profiles/<name> (qr_panel_8192x128.md,
-    syrk_15360_k512.md); None if the summary is absent."""
+    WRITE_SIZE, as recorded in profiles/<name> (qr_hr_panel_8192x128.md,
+    syrk_14336_k1024.md); None if the summary is absent."""
     path = os.path.join(ROOT, "profiles", name)
     try:
         for line in open(path):
@@ -180,8 +183,9 @@ def progress(msg):
     print(f"bench: {msg}", file=sys.stderr, flush=True)
 
 
-def ev_time(torch, fn, reps=2):
-    """Best of `reps` HIP-event timings of fn on the current stream (s)."""
+def ev_time(torch, fn, reps=2, mean=False):
+    """Best (or, with mean=True, the mean) of `reps` HIP-event timings of fn
+    on the current stream (s)."""
     ts = []
     for _ in range(reps):
         e0 = torch.cuda.Event(enable_timing=True)
@@ -191,7 +195,7 @@ def ev_time(torch, fn, reps=2):
         e1.record()
         torch.cuda.synchronize()
         ts.append(e0.elapsed_time(e1) / 1e3)
-    return min(ts)
+    return sum(ts) / len(ts) if mean else min(ts)
 
 
 def mixed_section(torch, gla, n=N_CHOL, nrhs=16):
@@ -466,7 +470,7 @@ def main():
                          "panel kernel, one launch)")
     ap.add_argument("--profile-syrk", action="store_true",
                     help="only run the trailing SYRK of the Cholesky's first look-ahead step "
-                         "(C 15360x15360 lower -= X X^T, K=512) and a plain 64 MiB copy five "
+                         "(C 14336x14336 lower -= X X^T, K=1024) and a plain 64 MiB copy five "
                          "times each, for hardware-counter runs")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -481,7 +485,7 @@ def main():
     dev = torch.device("cuda:0")
     torch.manual_seed(0)
     if args.profile_syrk:
-        kk, mm = 512, N_CHOL - 2 * 512
+        kk, mm = SYRK_K, N_CHOL - 2 * SYRK_K
         X = torch.randn(kk, mm, dtype=torch.float64, device=dev).t()
         Cs = torch.randn(mm, mm, dtype=torch.float64, device=dev).t()
         Cc = torch.empty(8 * 1024 * 1024, dtype=torch.float64, device=dev)
@@ -603,7 +607,8 @@ def main():
     Pw = torch.empty_like(P0)
     ptau = torch.zeros(NB_QR, dtype=torch.float64, device=dev)
     t_pc = ev_time(torch, lambda: Pw.copy_(P0), reps=3)
-    t_panel = ev_time(torch, lambda: (Pw.copy_(P0), gla.qrBlocked_(Pw, NB_QR)), reps=5) - t_pc
+    t_panel = ev_time(torch, lambda: (Pw.copy_(P0), gla.qrBlocked_(Pw, NB_QR)), reps=5,
+                      mean=True) - t_pc
     gla.set_tuning("qr_hr", 0)          # the column-by-column panel kernel
     try:
         t_colpanel = ev_time(torch, lambda: (Pw.copy_(P0), gla.qrUnblocked_(Pw, ptau)),
@@ -613,14 +618,19 @@ def main():
     panel_bytes = 16.0 * M_QR * NB_QR
     achieved = panel_bytes / t_panel / 1e9
     # the dominant kernel (largest share of the step's kernel time): the
-    # MFMA SYRK gemm_kernel<double,128,128,N,T,lower> of the look-ahead
-    # Cholesky, timed on its first step's shape (C 15360×15360 lower -= X Xᵀ,
-    # K = 512; rankUpdate! of that shape is exactly one launch); algorithmic
-    # work mm²·K flop
-    kk, mm = 512, n - 2 * 512
+    # FP64 direct-to-LDS MFMA SYRK gemm_dl_kernel<8,3,lower> of the
+    # look-ahead Cholesky, timed on its first step's shape (C 14336×14336
+    # lower -= X Xᵀ, K = 1024, the step width cholRecursive! takes at
+    # n = 16384; rankUpdate! of that shape is exactly one launch);
+    # algorithmic work mm²·K flop
+    kk, mm = SYRK_K, n - 2 * SYRK_K
     Cs = torch.empty_like(A0)[:mm, :mm]
     Xs = A0[2 * kk:, :kk]
-    t_syrk = ev_time(torch, lambda: gla.rankUpdate_(gla.Hermitian(Cs, "L"), Xs, -1.0), reps=3)
+    # one warm launch, then the mean of five, as the rocprof summary averages
+    # its five dispatches
+    gla.rankUpdate_(gla.Hermitian(Cs, "L"), Xs, -1.0)
+    t_syrk = ev_time(torch, lambda: gla.rankUpdate_(gla.Hermitian(Cs, "L"), Xs, -1.0), reps=5,
+                     mean=True)
     syrk_tf = float(mm) * float(mm) * kk / t_syrk / 1e12
     del Cs
 
@@ -660,9 +670,9 @@ def main():
                "residual": c1_res, "reference_published_ms": 12.098},
         "roofline": {"bound": "mfma", "achieved": syrk_tf, "peak": FP64_PEAK_TFLOPS,
                      "unit": "TFLOP/s", "frac": syrk_tf / FP64_PEAK_TFLOPS,
-                     "traffic": profiled_traffic("syrk_15360_k512.md"),
-                     "kernel": ("gemm_kernel<double,128,128,N,T,lower,1 LDS buffer> "
-                                "(SYRK 15360x15360, K=512)"),
+                     "traffic": profiled_traffic("syrk_14336_k1024.md"),
+                     "kernel": ("gemm_dl_kernel<8,3,lower> (FP64 direct-to-LDS MFMA SYRK "
+                                "14336x14336, K=1024)"),
                      "flops_per_launch": float(mm) * float(mm) * kk,
                      "us_per_launch": t_syrk * 1e6},
         "roofline_panel": {"bound": "hbm", "achieved": achieved, "peak": 8000.0, "unit": "GB/s",
