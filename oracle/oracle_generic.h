@@ -424,10 +424,10 @@ OT OFN(orc_reflector_inc)(int64_t n, OT *x, int64_t inc)
     return xi1 / nu;
 }
 
-/* LinearAlgebra.reflectorApply!(A, x, τ) (Julia stdlib, application from
- * the right, real case, called at src/svd.jl:344,363): for each row i,
- * s = τ (A[i,1] + Σ_{j≥2} A[i,j] x_j); A[i,1] −= s; A[i,j] −= s x_j.  x is
- * strided (incx). */
+/* reflectorApply!(A, x, τ) from the right, the package's own method
+ * (src/qr.jl:18-40, real case), called at src/svd.jl:344,363: for each
+ * row i, s = τ (A[i,1] + Σ_{j≥2} A[i,j] x_j); A[i,1] −= s;
+ * A[i,j] −= s x_j (x_j' = x_j for real x).  x is strided (incx). */
 void OFN(orc_reflector_apply_right)(int64_t m, int64_t n, const OT *x, int64_t incx, OT tau,
                                     OT *A, int64_t lda)
 {
