@@ -370,6 +370,201 @@ __global__ __launch_bounds__(kPanelThreads) void chol_factor_kernel(int nb, T *_
     }
 }
 
+// ---------------------------------------------------------------- LDLᴴ --
+// _ldlt_lower! (src/ldlt.jl:16-33) of a 16×16 tile in row layout, column C:
+// δ = a_CC (broadcast from lane C), l_i = a_iC / δ (:21), then
+// a_ij −= l_i (l_j δ) for C < j ≤ i (:24-26).  Lane C keeps δ on the
+// diagonal, rows below take l; entries right of the diagonal are left
+// meaningless (cleared by the caller).  dv[C] receives δ.  A zero pivot
+// divides by zero and the Inf/NaN propagate, as in the reference.
+template <int C, int J, typename T>
+struct LdltColUpdate {
+    static __device__ __forceinline__ void run(T a[16], T l, T ld)
+    {
+        a[J] = fma_t(-l, row_bcast<J>(ld), a[J]);
+        LdltColUpdate<C, J + 1, T>::run(a, l, ld);
+    }
+};
+template <int C, typename T>
+struct LdltColUpdate<C, 16, T> {
+    static __device__ __forceinline__ void run(T *, T, T) {}
+};
+template <int C, typename T>
+struct LdltTileCols {
+    static __device__ __forceinline__ void run(T a[16], T dv[16], int i)
+    {
+        const T d = row_bcast<C>(a[C]);
+        dv[C] = d;
+        const T l = a[C] / d;
+        const T ld = l * d;
+        LdltColUpdate<C, C + 1, T>::run(a, l, ld);
+        a[C] = i == C ? d : l;
+        LdltTileCols<C + 1, T>::run(a, dv, i);
+    }
+};
+template <typename T>
+struct LdltTileCols<16, T> {
+    static __device__ __forceinline__ void run(T *, T *, int) {}
+};
+
+// w <- L⁻¹ (four-column layout) for the unit lower tile L in row layout
+// (li[k] = L[i][k] for k < i): forward elimination with unit pivots.
+template <typename T>
+__device__ __forceinline__ void tile_inverse_unit_rows(const T li[16], T w[4])
+{
+    const int lane = threadIdx.x & 63;
+    const int i = lane & 15, g = lane >> 4;
+    T m[16];
+#pragma unroll
+    for (int k = 0; k < 16; ++k) m[k] = i > k ? li[k] : T(0);
+#pragma unroll
+    for (int t = 0; t < 4; ++t) w[t] = (i == g + 4 * t) ? T(1) : T(0);
+    TileInvSteps<0, T>::run(m, w);
+}
+
+// The LDLᴴ of the nb×nb (nb ≤ 128) diagonal block at A, held as 16×16
+// accumulator tiles like factor_body: per 16-column step wave 0 factors the
+// diagonal tile (LdltTileCols) and inverts its unit factor; the tiles below
+// take W = A L_kk⁻ᵀ on MFMA (rdiv! by UnitLowerTriangular, :86) and
+// L = W D_kk⁻¹ (rdiv! by Diagonal, :87, a division); the trailing tiles
+// take C −= W Lᵀ (:88-95) on MFMA.  Writes D on the diagonal and L below
+// it, and the inverses of the unit 16×16 diagonal tiles (row-major, 256
+// elements each) to LinvG.
+template <typename T, int W>
+__device__ __forceinline__ void ldlt_body(int nb, T *__restrict__ A, int64_t lda,
+                                          T *__restrict__ LinvG, T *Xb, T *Yb, T *Db, T *Lib,
+                                          T *dvs)
+{
+    typedef PanelMfma<T> M;
+    typedef typename M::acc_t acc_t;
+    constexpr int NQ = 9;
+    const int lane = threadIdx.x & 63;
+    const int lc = lane & 15;
+
+    acc_t acc[NQ];
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) {
+        const int bi = fslot_bi(W, q), bj = fslot_bj(W, q);
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const int gi = bi * 16 + M::row(lane, r), gj = bj * 16 + lc;
+            T v = T(0);
+            if (gj <= gi) v = gi < nb ? A[gi + (int64_t)gj * lda] : (gi == gj ? T(1) : T(0));
+            acc[q][r] = v;
+        }
+    }
+
+    const int nkb = (nb + 15) / 16;
+    for (int kb = 0; kb < nkb; ++kb) {
+        T *X = Xb + (kb & 1) * (kPanelNB * kXLD);   // W of column block kb
+        T *Y = Yb + (kb & 1) * (kPanelNB * kXLD);   // L of column block kb
+        T *D = Db + (kb & 1) * (16 * kXLD);
+        T *Li = Lib + (kb & 1) * (16 * kXLD);
+        T *dv = dvs + (kb & 1) * 16;
+        // (1) column block kb to LDS: the diagonal tile to D, the tiles below to X
+#pragma unroll
+        for (int q = 0; q < NQ; ++q) {
+            const int bi = fslot_bi(W, q), bj = fslot_bj(W, q);
+            if (bj != kb) continue;
+            T *dst = bi == kb ? D : X + bi * 16 * kXLD;
+#pragma unroll
+            for (int r = 0; r < 4; ++r) dst[M::row(lane, r) * kXLD + lc] = acc[q][r];
+        }
+        __syncthreads();
+        // (2) wave 0: LDLᴴ of the diagonal tile and the inverse of its unit factor
+        if (W == 0) {
+            const int i = lane & 15, g = lane >> 4;
+            T a[16], w[4], d[16];
+#pragma unroll
+            for (int j = 0; j < 16; ++j) a[j] = j <= i ? D[i * kXLD + j] : T(0);
+            LdltTileCols<0, T>::run(a, d, i);
+#pragma unroll
+            for (int j = 0; j < 16; ++j) a[j] = j <= i ? a[j] : T(0);
+            tile_inverse_unit_rows<T>(a, w);
+            if (g == 0)
+#pragma unroll
+                for (int j = 0; j < 16; ++j) D[i * kXLD + j] = a[j];
+            if (lane < 16) {
+                T dl = d[0];   // the pivots are the same in every lane: lane j stores δ_j
+#pragma unroll
+                for (int j = 1; j < 16; ++j) dl = lane == j ? d[j] : dl;
+                dv[lane] = dl;
+            }
+#pragma unroll
+            for (int t = 0; t < 4; ++t) {
+                Li[i * kXLD + g + 4 * t] = w[t];
+                if (LinvG) LinvG[kb * 256 + i * 16 + g + 4 * t] = w[t];
+            }
+        }
+        __syncthreads();
+        // (3) the diagonal tile takes D and L; tiles below: W = A L⁻ᵀ, L = W D⁻¹
+        const T dcol = dv[lc];
+#pragma unroll
+        for (int q = 0; q < NQ; ++q) {
+            const int bi = fslot_bi(W, q), bj = fslot_bj(W, q);
+            if (bj != kb) continue;
+            if (bi == kb) {
+#pragma unroll
+                for (int r = 0; r < 4; ++r) acc[q][r] = D[M::row(lane, r) * kXLD + lc];
+                continue;
+            }
+            acc_t x = {T(0), T(0), T(0), T(0)};
+#pragma unroll
+            for (int kq = 0; kq < 4; ++kq) {
+                const int kk = (lane >> 4) + 4 * kq;
+                x = M::mma(X[(bi * 16 + lc) * kXLD + kk], Li[lc * kXLD + kk], x);
+            }
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                acc[q][r] = x[r] / dcol;
+                X[(bi * 16 + M::row(lane, r)) * kXLD + lc] = x[r];
+                Y[(bi * 16 + M::row(lane, r)) * kXLD + lc] = acc[q][r];
+            }
+        }
+        __syncthreads();
+        // (4) rank-16 trailing update C_ij −= W_i L_jᵀ
+#pragma unroll
+        for (int q = 0; q < NQ; ++q) {
+            const int bi = fslot_bi(W, q), bj = fslot_bj(W, q);
+            if (bj <= kb) continue;
+#pragma unroll
+            for (int kq = 0; kq < 4; ++kq) {
+                const int kk = (lane >> 4) + 4 * kq;
+                const T a = -X[(bi * 16 + lc) * kXLD + kk];
+                const T b = Y[(bj * 16 + lc) * kXLD + kk];
+                acc[q] = M::mma(a, b, acc[q]);
+            }
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) {
+        const int bi = fslot_bi(W, q), bj = fslot_bj(W, q);
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const int gi = bi * 16 + M::row(lane, r), gj = bj * 16 + lc;
+            if (gi < nb && gj <= gi) A[gi + (int64_t)gj * lda] = acc[q][r];
+        }
+    }
+}
+
+template <typename T>
+__global__ __launch_bounds__(kPanelThreads) void ldlt_tile_factor_kernel(int nb, T *__restrict__ A,
+                                                                         int64_t lda,
+                                                                         T *__restrict__ LinvG)
+{
+    __shared__ T Xb[2 * kPanelNB * kXLD];
+    __shared__ T Yb[2 * kPanelNB * kXLD];
+    __shared__ T Db[2 * 16 * kXLD];
+    __shared__ T Lib[2 * 16 * kXLD];
+    __shared__ T dvs[2 * 16];
+    switch (threadIdx.x >> 6) {
+    case 0: ldlt_body<T, 0>(nb, A, lda, LinvG, Xb, Yb, Db, Lib, dvs); break;
+    case 1: ldlt_body<T, 1>(nb, A, lda, LinvG, Xb, Yb, Db, Lib, dvs); break;
+    case 2: ldlt_body<T, 2>(nb, A, lda, LinvG, Xb, Yb, Db, Lib, dvs); break;
+    default: ldlt_body<T, 3>(nb, A, lda, LinvG, Xb, Yb, Db, Lib, dvs); break;
+    }
+}
+
 // The same for a block that an earlier kernel has classified through
 // flags: flags[0] != 0 — nothing to do (the caller's path failed);
 // flags[1] == 0 — A = I + E with ‖E‖_F ≤ 0.1·√eps, whose factor is
@@ -1034,6 +1229,14 @@ int chol_factor_block(hipStream_t s, int nb, T *A, int64_t lda, int *info, int64
     return launch_factor<T>(s, nb, A, lda, info, info_base, LinvG);
 }
 template <typename T>
+int ldlt_factor_block(hipStream_t s, int nb, T *A, int64_t lda, T *LinvG)
+{
+    hipLaunchKernelGGL((ldlt_tile_factor_kernel<T>), dim3(1), dim3(kPanelThreads), 0, s, nb, A,
+                       lda, LinvG);
+    GLA_LAUNCH_CHECK();
+    return GLA_OK;
+}
+template <typename T>
 int chol_factor_block_near_identity(hipStream_t s, int nb, T *A, int64_t lda, int *info,
                                     int64_t info_base, T *LinvG, const int *flags)
 {
@@ -1125,6 +1328,7 @@ int diag_block_inverses(hipStream_t s, int64_t n, const T *L, int64_t ldl, const
 
 #define GLA_INST_BLOCKS(T)                                                                    \
     template int chol_factor_block<T>(hipStream_t, int, T *, int64_t, int *, int64_t, T *);  \
+    template int ldlt_factor_block<T>(hipStream_t, int, T *, int64_t, T *);                  \
     template int chol_factor_block_near_identity<T>(hipStream_t, int, T *, int64_t, int *,   \
                                                     int64_t, T *, const int *);              \
     template int chol_solve_block<T>(hipStream_t, int, const T *, int64_t, const T *, T *,   \

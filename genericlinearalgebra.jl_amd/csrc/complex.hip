@@ -45,8 +45,14 @@ __global__ __launch_bounds__(256) void zgemm_kernel(int64_t M, int64_t N, int64_
                                                     const T *__restrict__ A, int64_t lda,
                                                     const T *__restrict__ B, int64_t ldb, T beta,
                                                     T *__restrict__ C, int64_t ldc,
-                                                    int64_t diag_off)
+                                                    int64_t diag_off, int64_t kchunk,
+                                                    T *__restrict__ part)
 {
+    // split K (grid z): this workgroup contracts [k_begin, k_end) and, with
+    // `part`, writes α·(its partial product) to part[z] (M×N, ld M) for
+    // zsplit_reduce_kernel instead of touching C
+    const int64_t k_begin = (int64_t)blockIdx.z * kchunk;
+    const int64_t k_end = k_begin + kchunk < K ? k_begin + kchunk : K;
     typedef PanelMfma<T> Mf;
     typedef typename Mf::acc_t acc_t;
     constexpr int LD = kZB + 4;
@@ -64,7 +70,7 @@ __global__ __launch_bounds__(256) void zgemm_kernel(int64_t M, int64_t N, int64_
             accr[i][j] = acc_t{0, 0, 0, 0};
             acci[i][j] = acc_t{0, 0, 0, 0};
         }
-    for (int64_t k0 = 0; k0 < K; k0 += kZK) {
+    for (int64_t k0 = k_begin; k0 < k_end; k0 += kZK) {
         T ar[4], ai[4], br[4], bi[4];
         // op(A) (x, k): N — A[x + k·lda]; C — conj(A[k + x·lda])
 #pragma unroll
@@ -75,7 +81,7 @@ __global__ __launch_bounds__(256) void zgemm_kernel(int64_t M, int64_t N, int64_
             const int64_t gx = m0 + x, gk = k0 + k;
             ar[l] = T(0);
             ai[l] = T(0);
-            if (gx < M && gk < K) {
+            if (gx < M && gk < k_end) {
                 const T *p = A + 2 * (TA ? gk + gx * lda : gx + gk * lda);
                 ar[l] = p[0];
                 ai[l] = TA ? -p[1] : p[1];
@@ -90,7 +96,7 @@ __global__ __launch_bounds__(256) void zgemm_kernel(int64_t M, int64_t N, int64_
             const int64_t gy = n0 + y, gk = k0 + k;
             br[l] = T(0);
             bi[l] = T(0);
-            if (gy < N && gk < K) {
+            if (gy < N && gk < k_end) {
                 const T *p = B + 2 * (TB ? gy + gk * ldb : gk + gy * ldb);
                 br[l] = p[0];
                 bi[l] = TB ? -p[1] : p[1];
@@ -146,6 +152,12 @@ __global__ __launch_bounds__(256) void zgemm_kernel(int64_t M, int64_t N, int64_
                 const int64_t gr = m0 + wm * 32 + i * 16 + Mf::row(lane, r);
                 const int64_t gc = n0 + wn * 32 + j * 16 + (lane & 15);
                 if (gr >= M || gc >= N) continue;
+                if (part) {
+                    T *q = part + 2 * ((int64_t)blockIdx.z * M * N + gr + gc * M);
+                    q[0] = alpha * accr[i][j][r];
+                    q[1] = alpha * acci[i][j][r];
+                    continue;
+                }
                 if (MODE == 1 && gr + diag_off < gc) continue;
                 if (MODE == 2 && gr + diag_off > gc) continue;
                 T *c = C + 2 * (gr + gc * ldc);
@@ -160,15 +172,60 @@ __global__ __launch_bounds__(256) void zgemm_kernel(int64_t M, int64_t N, int64_
             }
 }
 
+// C = β C + Σ_z part[z] (M×N complex, ld M), the partials in a fixed order.
+template <typename T>
+__global__ void zsplit_reduce_kernel(int64_t M, int64_t N, int S, const T *__restrict__ part,
+                                     T beta, T *__restrict__ C, int64_t ldc)
+{
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= M * N) return;
+    const int64_t r = e % M, c = e / M;
+    T re = T(0), im = T(0);
+    for (int z = 0; z < S; ++z) {
+        re += part[2 * ((int64_t)z * M * N + e)];
+        im += part[2 * ((int64_t)z * M * N + e) + 1];
+    }
+    T *p = C + 2 * (r + c * ldc);
+    if (beta != T(0)) {
+        re += beta * p[0];
+        im += beta * p[1];
+    }
+    p[0] = re;
+    p[1] = im;
+}
+
+// ws (ws_elems real elements, may be null): room for split-K partials.  A
+// product with few 64×64 tiles and a long K (Vᴴ A in the compact-WY update,
+// the Gram products of T) is split along K over grid z into ws and summed
+// by zsplit_reduce_kernel, so that it spreads over the chip.
 template <typename T, bool TA, bool TB, int MODE, bool HERM>
 int zgemm_launch(hipStream_t s, int64_t M, int64_t N, int64_t K, T alpha, const T *A, int64_t lda,
-                 const T *B, int64_t ldb, T beta, T *C, int64_t ldc, int64_t diag_off)
+                 const T *B, int64_t ldb, T beta, T *C, int64_t ldc, int64_t diag_off,
+                 T *ws = nullptr, int64_t ws_elems = 0)
 {
     if (M <= 0 || N <= 0) return GLA_OK;
-    const dim3 grid((unsigned)cdiv(M, kZB), (unsigned)cdiv(N, kZB));
+    const int64_t tiles = cdiv(M, kZB) * cdiv(N, kZB);
+    int64_t S = 1;
+    if (MODE == 0 && !HERM && ws && tiles < 128 && K >= 1024) {
+        S = std::min<int64_t>(std::min<int64_t>(cdiv(K, 256), std::max<int64_t>(1, 256 / tiles)),
+                              64);
+        while (S > 1 && 2 * S * M * N > ws_elems) --S;
+    }
+    int64_t kchunk = K > 0 ? K : 1;
+    if (S > 1) {
+        kchunk = cdiv(cdiv(K, S), kZK) * kZK;
+        S = cdiv(K, kchunk);
+    }
+    const dim3 grid((unsigned)cdiv(M, kZB), (unsigned)cdiv(N, kZB), (unsigned)S);
     hipLaunchKernelGGL((zgemm_kernel<T, TA, TB, MODE, HERM>), grid, dim3(256), 0, s, M, N, K,
-                       alpha, A, lda, B, ldb, beta, C, ldc, diag_off);
+                       alpha, A, lda, B, ldb, beta, C, ldc, diag_off, kchunk,
+                       S > 1 ? ws : (T *)nullptr);
     GLA_LAUNCH_CHECK();
+    if (S > 1) {
+        hipLaunchKernelGGL((zsplit_reduce_kernel<T>), dim3((unsigned)cdiv(M * N, 256)), dim3(256),
+                           0, s, M, N, (int)S, (const T *)ws, beta, C, ldc);
+        GLA_LAUNCH_CHECK();
+    }
     return GLA_OK;
 }
 
@@ -696,27 +753,27 @@ __global__ void zzero_kernel(int64_t rows, T *S, int64_t lds)
 // the same UT transform).  scratch: 2·64² + k·k complex elements.
 template <typename T>
 int zqr_tblock_dev(hipStream_t s, int64_t m, int64_t k, const T *V, int64_t ldv, const T *tau,
-                   T *Tm, int64_t ldt, T *scratch)
+                   T *Tm, int64_t ldt, T *scratch, T *part = nullptr, int64_t part_elems = 0)
 {
     if (k <= kZB) {
         T *G = scratch;
         GLA_TRY((zgemm_launch<T, true, false, 0, false>(s, k, k, m, T(1), V, ldv, V, ldv, T(0), G,
-                                                         k, 0)));
+                                                         k, 0, part, part_elems)));
         hipLaunchKernelGGL((zqr_tsmall_kernel<T>), dim3(1), dim3(kZB), 0, s, (int)k,
                            (const T *)G, k, tau, Tm, ldt);
         GLA_LAUNCH_CHECK();
         return GLA_OK;
     }
     const int64_t k1 = (k / 2 + kZB - 1) / kZB * kZB;
-    GLA_TRY(zqr_tblock_dev<T>(s, m, k1, V, ldv, tau, Tm, ldt, scratch));
+    GLA_TRY(zqr_tblock_dev<T>(s, m, k1, V, ldv, tau, Tm, ldt, scratch, part, part_elems));
     GLA_TRY(zqr_tblock_dev<T>(s, m - k1, k - k1, V + 2 * (k1 + k1 * ldv), ldv, tau + 2 * k1,
-                              Tm + 2 * (k1 + k1 * ldt), ldt, scratch));
+                              Tm + 2 * (k1 + k1 * ldt), ldt, scratch, part, part_elems));
     // G = V1ᴴ V2 over rows k1..m (V2 is zero above), P = T11 G, T12 = −P T22
     const int64_t k2 = k - k1;
     T *Gm = scratch, *P = Gm + 2 * k1 * k2;
     GLA_TRY((zgemm_launch<T, true, false, 0, false>(s, k1, k2, m - k1, T(1), V + 2 * k1, ldv,
                                                      V + 2 * (k1 + k1 * ldv), ldv, T(0), Gm, k1,
-                                                     0)));
+                                                     0, part, part_elems)));
     GLA_TRY((zgemm_launch<T, false, false, 0, false>(s, k1, k2, k1, T(1), Tm, ldt, Gm, k1, T(0), P,
                                                       k1, 0)));
     GLA_TRY((zgemm_launch<T, false, false, 0, false>(s, k1, k2, k2, T(-1), P, k1,
@@ -733,11 +790,12 @@ int zqr_tblock_dev(hipStream_t s, int64_t m, int64_t k, const T *V, int64_t ldv,
 // W1 = Vᴴ A, W2 = op(T) W1 (op = ᴴ for trans 'C', none for 'N'), A −= V W2.
 template <typename T>
 int zwy_apply_dev(hipStream_t s, char trans, int64_t mA, int64_t nA, int64_t b, const T *V,
-                  int64_t ldv, const T *Tm, int64_t ldt, T *A, int64_t lda, T *W1, T *W2)
+                  int64_t ldv, const T *Tm, int64_t ldt, T *A, int64_t lda, T *W1, T *W2,
+                  T *part = nullptr, int64_t part_elems = 0)
 {
     if (mA <= 0 || nA <= 0 || b <= 0) return GLA_OK;
     GLA_TRY((zgemm_launch<T, true, false, 0, false>(s, b, nA, mA, T(1), V, ldv, A, lda, T(0), W1,
-                                                     b, 0)));
+                                                     b, 0, part, part_elems)));
     if (trans == 'C')
         GLA_TRY((zgemm_launch<T, true, false, 0, false>(s, b, nA, b, T(1), Tm, ldt, W1, b, T(0),
                                                          W2, b, 0)));
@@ -747,6 +805,10 @@ int zwy_apply_dev(hipStream_t s, char trans, int64_t mA, int64_t nA, int64_t b, 
     return zgemm_launch<T, false, false, 0, false>(s, mA, nA, b, T(-1), V, ldv, W2, b, T(1), A,
                                                    lda, 0);
 }
+
+// Real elements of the split-K partials of the complex QR: 16 partial
+// products of a 32-row panel by n columns (or of a 32×32 Gram matrix).
+inline int64_t zqr_part_elems(int64_t n) { return 2 * 16 * (int64_t)kZQ * std::max<int64_t>(n, kZQ); }
 
 // qrBlocked! (src/qr.jl:113-146) for complex A: block columns w =
 // min(blocksize, 32) wide, each factored by zqr_panel, its T formed and
@@ -763,6 +825,8 @@ int zqr_blocked_dev(hipStream_t s, int64_t m, int64_t n, T *A, int64_t lda, int6
     T *Tm = V + 2 * m * w0;                            // w0 × w0
     T *sc = Tm + 2 * kZQ * kZQ;                        // tblock scratch
     T *W1 = sc + 2 * (2 * kZB * kZB + kZQ * kZQ), *W2 = W1 + 2 * (int64_t)w0 * n;
+    T *part = W2 + 2 * (int64_t)w0 * n;                // split-K partials
+    const int64_t part_elems = zqr_part_elems(n);
     for (int64_t off = 0; off < k; off += w0) {
         const int w = (int)std::min<int64_t>(w0, n - off);
         const int64_t mm = m - off;
@@ -774,9 +838,10 @@ int zqr_blocked_dev(hipStream_t s, int64_t m, int64_t n, T *A, int64_t lda, int6
             hipLaunchKernelGGL((zmake_v_kernel<T>), dim3((unsigned)cdiv(mm, 256), (unsigned)kb),
                                dim3(256), 0, s, mm, (int64_t)kb, (const T *)Ab, lda, V, mm);
             GLA_LAUNCH_CHECK();
-            GLA_TRY(zqr_tblock_dev<T>(s, mm, kb, V, mm, tau + 2 * off, Tm, kZQ, sc));
+            GLA_TRY(zqr_tblock_dev<T>(s, mm, kb, V, mm, tau + 2 * off, Tm, kZQ, sc, part,
+                                      part_elems));
             GLA_TRY(zwy_apply_dev<T>(s, 'C', mm, rest, kb, V, mm, Tm, kZQ, Ab + 2 * w * lda, lda,
-                                     W1, W2));
+                                     W1, W2, part, part_elems));
         }
     }
     return GLA_OK;
@@ -786,7 +851,8 @@ size_t zqr_scratch(int64_t m, int64_t n)
 {
     const int64_t g = cdiv(m, 256);
     return 2 * ((size_t)(g + 8 + 2 * g * kZQ + 64) + (size_t)m * kZQ + kZQ * kZQ +
-                (2 * kZB * kZB + kZQ * kZQ) + 2 * (size_t)kZQ * n) + 256;
+                (2 * kZB * kZB + kZQ * kZQ) + 2 * (size_t)kZQ * n) +
+           (size_t)zqr_part_elems(n) + 256;
 }
 
 template <typename T>

@@ -170,6 +170,11 @@ int chol_device(Handle *h, hipStream_t s, int64_t n, T *A, int64_t lda, int algo
 // place and store the inverses of its 16×16 diagonal tiles; solve rows
 // below it (B <- B L⁻ᵀ) with those inverses; inverses of all 16×16 diagonal
 // tiles of an n×n lower L; the recursive right-side solve B <- B L⁻ᵀ.
+// LDLᴴ (_ldlt_lower!, src/ldlt.jl:16-33) of the nb×nb (nb ≤ 128) block at
+// A in one workgroup: D on the diagonal, the unit factor's L below it, and
+// the inverses of its unit 16×16 diagonal tiles in LinvG.
+template <typename T>
+int ldlt_factor_block(hipStream_t s, int nb, T *A, int64_t lda, T *LinvG);
 template <typename T>
 int chol_factor_block(hipStream_t s, int nb, T *A, int64_t lda, int *info, int64_t info_base,
                       T *LinvG);

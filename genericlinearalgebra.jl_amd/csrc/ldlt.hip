@@ -4,9 +4,10 @@
 //
 // Lower (A = L D Lᵀ, L unit lower stored below the diagonal, D on it), one
 // step of width kb at column k:
-//   - the kb×kb diagonal block: _ldlt_lower! (:16-33) in one workgroup, the
-//     block's lower triangle resident in LDS; the same launch writes the
-//     inverses of the 16×16 diagonal tiles of its unit lower factor;
+//   - the kb×kb diagonal block: _ldlt_lower! (:16-33) in one workgroup as
+//     16×16 tiles (the diagonal tile factored column by column in registers,
+//     the rest on MFMA; ldlt_factor_block in chol.hip); the same launch
+//     writes the inverses of the 16×16 diagonal tiles of its unit factor;
 //   - the panel: W = A21 L11⁻ᵀ (rdiv! by UnitLowerTriangular(A11)', :86) on
 //     the Cholesky's MFMA panel solve with those unit tile inverses, then
 //     one pass keeping W and writing L21 = W D⁻¹ (rdiv! by Diagonal(d₁),
@@ -37,86 +38,6 @@ namespace {
 
 constexpr int kNB = 128;    // largest diagonal block of one factor launch
 
-// Element (i, j), i ≥ j, of a 128-row lower triangle packed by columns.
-__device__ __forceinline__ int pk(int i, int j) { return j * kNB - (j * (j - 1)) / 2 + (i - j); }
-
-// _ldlt_lower! of the nb×nb block at A (nb ≤ 128), then LinvG[q·256 + r·16 + c]
-// = (L_qq⁻¹)(r, c) for the unit lower 16×16 diagonal tiles q of the factor.
-template <typename T>
-__global__ __launch_bounds__(256) void ldlt_factor_kernel(int nb, T *__restrict__ A, int64_t lda,
-                                                          T *__restrict__ LinvG)
-{
-    __shared__ T S[kNB * (kNB + 1) / 2];
-    __shared__ T wv[kNB];
-    const int t = threadIdx.x;
-    const int i = t & (kNB - 1), p = t >> 7;      // row, column parity
-    // load the lower triangle: 16 loads in flight per thread
-    if (i < nb) {
-        for (int j0 = p; j0 <= i; j0 += 32) {
-            T v[16];
-#pragma unroll
-            for (int u = 0; u < 16; ++u) {
-                const int j = j0 + 2 * u;
-                v[u] = j <= i ? A[i + (int64_t)j * lda] : T(0);
-            }
-#pragma unroll
-            for (int u = 0; u < 16; ++u) {
-                const int j = j0 + 2 * u;
-                if (j <= i) S[pk(i, j)] = v[u];
-            }
-        }
-    }
-    // :17-31, one column per step
-    for (int k = 0; k + 1 < nb; ++k) {
-        __syncthreads();
-        const T d = S[pk(k, k)];
-        if (t > k && t < nb) {
-            const T l = S[pk(t, k)] / d;          // :21
-            S[pk(t, k)] = l;
-            wv[t] = l * d;                        // :24 lⱼδ
-        }
-        __syncthreads();
-        if (i > k && i < nb) {
-            const T li = S[pk(i, k)];
-            int j = k + 1;
-            if ((j & 1) != p) ++j;
-            for (; j <= i; j += 2) S[pk(i, j)] -= li * wv[j];   // :26
-        }
-    }
-    __syncthreads();
-    if (i < nb) {
-        for (int j0 = p; j0 <= i; j0 += 32) {
-#pragma unroll
-            for (int u = 0; u < 16; ++u) {
-                const int j = j0 + 2 * u;
-                if (j <= i) A[i + (int64_t)j * lda] = S[pk(i, j)];
-            }
-        }
-    }
-    // inverses of the unit lower diagonal tiles: thread (q, c) takes column c
-    // of tile q by forward substitution; rows past nb are the identity
-    const int nq = (nb + 15) / 16;
-    if (t < nq * 16) {
-        const int q = t >> 4, c = t & 15;
-        T x[16];
-#pragma unroll
-        for (int r = 0; r < 16; ++r) x[r] = r == c ? T(1) : T(0);
-#pragma unroll
-        for (int r = 1; r < 16; ++r) {
-            const int gi = q * 16 + r;
-            if (r > c && gi < nb) {
-                T s = T(0);
-#pragma unroll
-                for (int m = 0; m < 15; ++m)
-                    if (m >= c && m < r) s -= S[pk(gi, q * 16 + m)] * x[m];
-                x[r] = s;
-            }
-        }
-#pragma unroll
-        for (int r = 0; r < 16; ++r) LinvG[q * 256 + r * 16 + c] = x[r];
-    }
-}
-
 // W = A21 (the panel solve's output A21 L11⁻ᵀ) and A21 <- W D⁻¹, D the
 // diagonal of the factored A11 (:87).
 template <typename T>
@@ -145,12 +66,13 @@ __global__ void transpose_tri_kernel(int64_t n, T *A, int64_t lda)
     }
 }
 
+// _ldlt_lower! of the diagonal block (:84), nb ≤ 128, and the inverses of
+// its unit 16×16 diagonal tiles: the MFMA-tiled kernel beside the Cholesky's
+// diagonal factor.
 template <typename T>
 int launch_factor(hipStream_t s, int nb, T *A, int64_t lda, T *Linv)
 {
-    hipLaunchKernelGGL((ldlt_factor_kernel<T>), dim3(1), dim3(256), 0, s, nb, A, lda, Linv);
-    GLA_LAUNCH_CHECK();
-    return GLA_OK;
+    return ldlt_factor_block<T>(s, nb, A, lda, Linv);
 }
 
 template <typename T>
