@@ -183,29 +183,8 @@ __device__ __forceinline__ void tile_factor_rows(T a[16], T rd[16], int nvalid, 
 // [L | I] with unscaled rows: V[i] = e_i − Σ_{k<i} m_ik V[k] with the
 // multipliers m_ik = L[i][k]·rd[k] formed once (zero for i ≤ k, so every
 // step is one broadcast and one unconditional FMA per column), then
-// W[i] = rd[i]·V[i].  Row k of V reaches the lanes of its 16-lane row by
-// DPP broadcast.
-template <int K, typename T>
-struct TileInvSteps {
-    static __device__ __forceinline__ void run(const T m[16], T w[4])
-    {
-#pragma unroll
-        for (int t = 0; t < 4; ++t) {
-            const T vk = row_bcast<K>(w[t]);
-            if constexpr (sizeof(T) == 8)
-                w[t] = __builtin_fma(-m[K], vk, w[t]);
-            else
-                w[t] = __builtin_fmaf(-m[K], vk, w[t]);
-        }
-        TileInvSteps<K + 1, T>::run(m, w);
-    }
-};
-template <typename T>
-struct TileInvSteps<15, T> {   // row 15 has no rows below it
-    static __device__ __forceinline__ void run(const T *, T *) {}
-};
-
-// The same from L in row layout (li[k] = L[i][k]).
+// W[i] = rd[i]·V[i] (the steps are TileInvSteps).  L comes in row layout
+// (li[k] = L[i][k]).
 template <typename T>
 __device__ __forceinline__ void tile_inverse_rows(const T li[16], const T rd[16], T w[4])
 {

@@ -54,6 +54,41 @@ __device__ __forceinline__ T row_bcast(T v)
 __device__ __forceinline__ double fma_t(double a, double b, double c) { return __builtin_fma(a, b, c); }
 __device__ __forceinline__ float fma_t(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
 
+// Forward elimination steps of a 16×16 lower-triangular inverse in the
+// four-column layout (lane (i, g) holds w[t] = V[i][g + 4t]): for K = 0…14,
+// V[i] −= m[K]·V[K] with row K of V broadcast from lane K of each 16-lane
+// row; m[K] is this lane's multiplier (zero for i ≤ K), so every step is one
+// broadcast and one unconditional FMA per column.
+template <int K, typename T>
+struct TileInvSteps {
+    static __device__ __forceinline__ void run(const T m[16], T w[4])
+    {
+#pragma unroll
+        for (int t = 0; t < 4; ++t) w[t] = fma_t(-m[K], row_bcast<K>(w[t]), w[t]);
+        TileInvSteps<K + 1, T>::run(m, w);
+    }
+};
+template <typename T>
+struct TileInvSteps<15, T> {   // row 15 has no rows below it
+    static __device__ __forceinline__ void run(const T *, T *) {}
+};
+
+// The same backward, for an upper-triangular inverse: K = 15…1,
+// V[i] −= m[K]·V[K] (m[K] zero for i ≥ K).
+template <int K, typename T>
+struct TileInvStepsUp {
+    static __device__ __forceinline__ void run(const T m[16], T w[4])
+    {
+#pragma unroll
+        for (int t = 0; t < 4; ++t) w[t] = fma_t(-m[K], row_bcast<K>(w[t]), w[t]);
+        TileInvStepsUp<K - 1, T>::run(m, w);
+    }
+};
+template <typename T>
+struct TileInvStepsUp<0, T> {   // row 0 has no rows above it
+    static __device__ __forceinline__ void run(const T *, T *) {}
+};
+
 // 16×16×4 MFMA with one A and one B element per lane; accumulator element r
 // of lane l sits at (row(l, r), l % 16) of the 16×16 tile.
 template <typename T>

@@ -1025,22 +1025,21 @@ __device__ __forceinline__ T recip1(T u)
 // (lane r of each 16-lane row holds d[c] = M[jb + r][jb + c]): the pivot
 // −q̂_CC comes from lane C by DPP broadcast, S_C = copysign(1, −q̂_CC) and
 // u_C = −q̂_CC + S_C; rows r > C take l = d[C]/u_C and d[c] −= l·u_(C,c)
-// for c > C, the pivot row's entries broadcast from lane C as well.  The
-// steps are branch-free (selects), so the 16 columns form one block the
-// compiler can schedule across.
+// for c > C, the pivot row's entries broadcast from lane C as well (rows at
+// or above the pivot run the same FMAs with l = 0).  The steps are
+// branch-free, so the 16 columns form one block the compiler can schedule
+// across.
 template <int C, int J, typename T>
 struct HrLuRow {
-    static __device__ __forceinline__ void run(T d[16], T l, bool below)
+    static __device__ __forceinline__ void run(T d[16], T l)
     {
-        const T u = row_bcast<C>(d[J]);
-        const T nd = fma_t(-l, u, d[J]);
-        d[J] = below ? nd : d[J];
-        HrLuRow<C, J + 1, T>::run(d, l, below);
+        d[J] = fma_t(-l, row_bcast<C>(d[J]), d[J]);
+        HrLuRow<C, J + 1, T>::run(d, l);
     }
 };
 template <int C, typename T>
 struct HrLuRow<C, 16, T> {
-    static __device__ __forceinline__ void run(T *, T, bool) {}
+    static __device__ __forceinline__ void run(T *, T) {}
 };
 template <int C, typename T>
 struct HrLuCols {
@@ -1051,9 +1050,11 @@ struct HrLuCols {
             const T dj = dcopysign<T>(T(1), mjj);
             const T uj = mjj + dj;              // |u_C| = 1 + |q̂_CC|
             const T ru = recip1(uj);
+            // rows at or above the pivot take l = 0, which leaves them as
+            // they are (fma(−0, u, d) = d)
             const bool below = r > C;
             const T l = d[C] * ru;
-            HrLuRow<C, C + 1, T>::run(d, l, below);
+            HrLuRow<C, C + 1, T>::run(d, below ? l : T(0));
             d[C] = below ? l : (r == C ? uj : d[C]);
             dv[C] = dj;
             uv[C] = uj;
@@ -1066,14 +1067,32 @@ struct HrLuCols<16, T> {
     static __device__ __forceinline__ void run(T *, T *, T *, int, int) {}
 };
 
+// Multipliers of the backward elimination for Z = U⁻¹ (rows of U in
+// registers, myr = 1/U[r][r] in lane r): m[K] = U[r][K]/U[K][K] for K > r,
+// the reciprocal broadcast from lane K; zero otherwise.
+template <int K, typename T>
+struct HrUpMul {
+    static __device__ __forceinline__ void run(const T d[16], T myr, int r, T m[16])
+    {
+        const T rk = row_bcast<K>(myr);
+        m[K] = r < K ? d[K] * rk : T(0);
+        HrUpMul<K + 1, T>::run(d, myr, r, m);
+    }
+};
+template <typename T>
+struct HrUpMul<16, T> {
+    static __device__ __forceinline__ void run(const T *, T, int, T *) {}
+};
+
 // Steps (2)-(3) on the top block: one workgroup of 4 waves, k ≤ 128, the
 // block in LDS (M, column-major), blocked by 16 columns as a right-looking
 // LU without pivoting.  Per block J = jb..jb+15:
 //   wave 0 eliminates the 16×16 diagonal block column by column (lane r
-//   holds row r; the pivot row is broadcast through SGPRs), choosing
+//   holds row r; the pivot row moves by DPP broadcast), choosing
 //   S_jj = copysign(1, −q̂_jj) from the updated diagonal entry — which
-//   depends on this block alone; waves 1 and 2 then form W_J = L_JJ⁻¹ and
-//   Z = U_JJ⁻¹ side by side; then, on MFMA over 16×16 tiles: U_J,right = W_J M_J,right,
+//   depends on this block alone — and forms W_J = L_JJ⁻¹ and Z = U_JJ⁻¹
+//   from the rows still in its registers; then, on MFMA over 16×16 tiles:
+//   U_J,right = W_J M_J,right,
 //   L_below,J = M_below,J Z, and M_below,right −= L_below,J U_J,right.
 // Then T = Û V₁⁻ᵀ (Û = U S) in place over U, by column blocks:
 //   T[:, J] = Acc[:, J] W_Jᵀ, Acc[:, right] −= T[:, J] V₁[right, J]ᵀ
@@ -1099,13 +1118,18 @@ __global__ __launch_bounds__(kHrThreads) void hr_lu_kernel(int k, T *Q, int64_t 
     {
 #pragma unroll 1
       for (int q0 = 0; q0 < 32; q0 += 16) {
+        // unconditional loads from clamped indices (the top k×k block always
+        // exists), so that all 32 stay in flight; out-of-range values are
+        // replaced afterwards
         T v[16][2];
 #pragma unroll
         for (int qq = 0; qq < 16; ++qq)
 #pragma unroll
             for (int s = 0; s < 2; ++s) {
                 const int i = lane + 64 * s, l = w + 4 * (q0 + qq);
-                v[qq][s] = (i < k && l < k) ? -Q[i + (int64_t)l * ldq] : T(0);
+                const int ic = i < k ? i : k - 1, lcl = l < k ? l : k - 1;
+                const T q = Q[ic + (int64_t)lcl * ldq];
+                v[qq][s] = (i < k && l < k) ? -q : T(0);
             }
 #pragma unroll
         for (int qq = 0; qq < 16; ++qq)
@@ -1145,62 +1169,34 @@ __global__ __launch_bounds__(kHrThreads) void hr_lu_kernel(int k, T *Q, int64_t 
 #pragma unroll
                 for (int c = 0; c < 16; ++c) M[jb + r + (jb + c) * kHrLD] = d[c];
             }
-        }
-        __syncthreads();
-        // W = L⁻¹ (unit lower, wave 1) and Z = U⁻¹ (upper, wave 2) at once,
-        // column j = lane, by column sweeps (the updates of one step are
-        // independent)
-        if (w == 1) {
-            if (lane < 16) {
-                const int j = lane;
-                T x[16];
+            // W = L⁻¹ (unit lower) and Z = U⁻¹ (upper) from the rows in
+            // registers, four columns per lane (wz[t] = X[r][g + 4t]); rows and
+            // columns past nbk (a partial last block) are the identity
+            const int g = lane >> 4;
+            T m[16], wv[4], zv[4];
 #pragma unroll
-                for (int rr = 0; rr < 16; ++rr) x[rr] = rr == j ? T(1) : T(0);
-                // column s of L staged in registers, then the independent
-                // updates of step s (the fence keeps the steps as written)
+            for (int c = 0; c < 16; ++c) m[c] = r > c ? d[c] : T(0);
 #pragma unroll
-                for (int s = 0; s < 15; ++s) {
-                    T cs[16];
+            for (int t = 0; t < 4; ++t) wv[t] = zv[t] = (r == g + 4 * t) ? T(1) : T(0);
+            TileInvSteps<0, T>::run(m, wv);
+            T udiag = d[0];
 #pragma unroll
-                    for (int rr = s + 1; rr < 16; ++rr) cs[rr] = M[jb + rr + (jb + s) * kHrLD];
+            for (int c = 1; c < 16; ++c) udiag = r == c ? d[c] : udiag;
+            const T myr = r < nbk ? recip1(udiag) : T(1);
+            // unscaled rows Y[i] = U_ii Z[i]: Y[i] −= (U[i][k]/U[k][k]) Y[k]
+            HrUpMul<0, T>::run(d, myr, r, m);
+            TileInvStepsUp<15, T>::run(m, zv);
 #pragma unroll
-                    for (int rr = s + 1; rr < 16; ++rr) x[rr] -= cs[rr] * x[s];
-                    __builtin_amdgcn_sched_barrier(0);
-                }
-#pragma unroll
-                for (int rr = 0; rr < 16; ++rr) Wb[rr + j * kHrTL] = x[rr];
-            }
-        } else if (w == 2) {
-            if (lane < 16) {
-                const int j = lane;
-                T x[16];
-                {
-                    T rd[16];
-#pragma unroll
-                    for (int rr = 0; rr < 16; ++rr) {
-                        rd[rr] = rr < nbk ? recip1(M[jb + rr + (jb + rr) * kHrLD]) : T(1);
-                        x[rr] = rr == j ? T(1) : T(0);
-                    }
-#pragma unroll
-                    for (int s = 15; s >= 0; --s) {
-                        T cs[16];
-#pragma unroll
-                        for (int rr = 0; rr < s; ++rr) cs[rr] = M[jb + rr + (jb + s) * kHrLD];
-                        x[s] *= rd[s];
-#pragma unroll
-                        for (int rr = 0; rr < s; ++rr) x[rr] -= cs[rr] * x[s];
-                        __builtin_amdgcn_sched_barrier(0);
-                    }
-#pragma unroll
-                    for (int rr = 0; rr < 16; ++rr) Zs[rr + j * kHrTL] = x[rr];
-                    // the 16×16 diagonal-tile inverse of Un = −Uᵀ, row-major
-                    // as chol_solve_kernel reads it: (−U_JJᵀ)⁻¹ = −Zᵀ (the
-                    // padding of a partial last block is the identity)
-#pragma unroll
-                    for (int rr = 0; rr < 16; ++rr)
-                        UnInv[(jb >> 4) * 256 + j * 16 + rr] =
-                            (j >= nbk || rr >= nbk) ? (j == rr ? T(1) : T(0)) : -x[rr];
-                }
+            for (int t = 0; t < 4; ++t) {
+                const int j = g + 4 * t;
+                zv[t] *= myr;
+                Wb[r + j * kHrTL] = wv[t];
+                Zs[r + j * kHrTL] = zv[t];
+                // the 16×16 diagonal-tile inverse of Un = −Uᵀ, row-major as
+                // chol_solve_kernel reads it: (−U_JJᵀ)⁻¹ = −Zᵀ (the padding of
+                // a partial last block is the identity)
+                UnInv[(jb >> 4) * 256 + j * 16 + r] =
+                    (j >= nbk || r >= nbk) ? (j == r ? T(1) : T(0)) : -zv[t];
             }
         }
         __syncthreads();
