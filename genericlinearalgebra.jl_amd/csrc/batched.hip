@@ -1,20 +1,21 @@
 // Batched Cholesky factor + solve for many SPD matrices of uneven size
-// (n ≤ 256), FP64 and FP32, one right-hand side each, in one persistent
-// launch.  No counterpart in the reference (which factors one matrix per
+// (n ≤ 256), FP64 and FP32, one right-hand side each, in one or two
+// persistent launches.  No counterpart in the reference (which factors one matrix per
 // call, src/cholesky.jl); each matrix gets cholUnblocked!'s factor (to
 // rounding), then the solve L Lᵀ x = b.
 //
 // Plan (host): matrices are sorted by n, largest first.  Those above a tuned
-// threshold `wave_max` become one work item each, handled by a whole
-// workgroup; the rest are packed four to an item, one wave per matrix.
-// Workgroups take items round-robin (item i → workgroup i % grid).
+// threshold `wave_max` go to the workgroup kernel, one matrix per workgroup
+// (matrix i → workgroup i % grid); the rest go to the wave kernel, one wave
+// per matrix, ranks dealt to the waves in alternating order.
 //
-// Wave path (any n ≤ 256, the default): see wave_chol_solve — no workgroup
-// barrier, the panel in registers, left-looking updates streamed from L.
+// Wave path (any n ≤ 256, the default): see wave_chol_solve — 16×16 MFMA
+// tiles, no workgroup barrier, left-looking panels streamed from L.
 // Workgroup path: right-looking, 32-column panels factored in LDS with one
 // barrier per column, trailing lower triangle updated in 4×4 register blocks
 // in memory, solves with the 32×32 diagonal blocks staged in LDS.
 #include <algorithm>
+#include <type_traits>
 #include <vector>
 
 #include "gla_internal.hpp"
@@ -24,11 +25,13 @@ namespace gla {
 struct BatchPlan {
     int device = 0;
     int64_t batch = 0;
-    int nitems = 0;
-    int grid = 0;
-    int wave_max = 32;
+    int nwg = 0;               // matrices above wave_max (one workgroup each)
+    int nsmall = 0;            // matrices at or below it (one wave each)
+    int grid_wg = 0, grid_wave = 0;
+    int wave_max = 256;
     int *d_n = nullptr;        // n per matrix (device)
-    int4 *d_items = nullptr;   // work items (device)
+    int *d_wg = nullptr;       // the workgroup matrices, largest first (device)
+    int *d_small = nullptr;    // the wave matrices, largest first (device)
 };
 
 namespace {
@@ -38,7 +41,6 @@ constexpr int kBMax = 256;     // largest n
 constexpr int kPW = 32;        // panel width
 constexpr int kPLD = kPW + 1;  // padded panel row
 constexpr int kWMax = kBMax;   // the wave path takes any n ≤ 256
-constexpr int kWP = 16;        // panel width of the wave path
 
 template <typename T>
 __device__ __forceinline__ T bsqrt(T x);
@@ -50,27 +52,27 @@ __device__ __forceinline__ float bsqrt<float>(float x) { return __builtin_sqrtf(
 __device__ __forceinline__ void border() { __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront"); }
 
 // ---------------------------------------------------------------- wave --
-// One wave factors and solves one matrix (n ≤ 256) with no workgroup
-// barrier.  Lane l holds rows l + 64s (s < 4).  Left-looking by 16-column
-// panels: the panel sits in registers; the contributions of the columns
-// already factored are applied as a GEMM (L rows streamed from memory, the
-// panel's own 16 rows staged in wave-private LDS), then its 16 columns are
-// factored in registers (pivot through a scalar register, the scaled column
-// shared through LDS), then written back.  The solves keep b in registers:
-// forward by column sweeps, backward by per-column wave sums.
-
-template <typename T>
-__device__ __forceinline__ T rlane(T v, int l)
-{
-    if constexpr (sizeof(T) == 8) {
-        int2 u = __builtin_bit_cast(int2, v);
-        u.x = __builtin_amdgcn_readlane(u.x, l);
-        u.y = __builtin_amdgcn_readlane(u.y, l);
-        return __builtin_bit_cast(T, u);
-    } else {
-        return __builtin_bit_cast(T, __builtin_amdgcn_readlane(__builtin_bit_cast(int, v), l));
-    }
-}
+// One wave factors and solves one matrix (n ≤ 256) on 16×16 MFMA tiles,
+// left-looking by 16-column panels, with no workgroup barrier.  A tile sits
+// in the MFMA accumulator transposed: lane (i = l % 16, g = l / 16) holds
+// row i of the tile at the columns c_q = Mf::row(l, q), q < 4.  So every
+// load and store of a tile moves 16 consecutive rows of a column, and a tile
+// in this layout is at once the B operand of a product whose K index runs
+// through the c_q (a permutation of K that both operands share).  Per panel
+// p0 (columns p0..p0+15), the rows below in groups of kMR tiles:
+//   (1) C = A − L[rows, 0:p0] L[p0:p0+16, 0:p0]ᵀ on MFMA, the operands
+//       streamed through a per-wave LDS ring (several slabs of K in flight,
+//       which is what keeps the wave busy: the columns of L come back from
+//       HBM, not from a cache); the first group also carries the diagonal
+//       tile C_dd, whose two operands are the same panel block;
+//   (2) after the first group's K loop, C_dd is factored in row layout by
+//       the Cholesky's tile factor (cholUnblocked!'s column steps,
+//       src/cholesky.jl:3-15), L_dd is written back, y_p = L_dd⁻¹ b_p is
+//       solved by 16 broadcast steps and W = L_dd⁻¹ formed in registers;
+//   (3) each group: X = C L_dd⁻ᵀ as Xᵀ = W Cᵀ on MFMA, written back, and
+//       b_rows −= X y_p.
+// Then Lᵀ x = y from the last panel: z = y_p − L_belowᵀ x_below (16-lane row
+// sums), and a 16-step back substitution on the columns of L_dd.
 
 template <int CTRL, typename T>
 __device__ __forceinline__ T dppmov(T v)
@@ -86,153 +88,296 @@ __device__ __forceinline__ T dppmov(T v)
     }
 }
 
-// Sum over the 64 lanes, the same value in every lane and a fixed order:
-// quads (xor 1, xor 2), half rows, rows by DPP, then the four rows.
+// Sum over the 16 lanes of each row, the same bits in every lane of the row:
+// pairs, quads, half rows, rows (each step adds two equal-order partials).
 template <typename T>
-__device__ __forceinline__ T wave_sum(T v)
+__device__ __forceinline__ T row_sum16(T v)
 {
     v += dppmov<0xB1>(v);    // quad_perm [1,0,3,2]
     v += dppmov<0x4E>(v);    // quad_perm [2,3,0,1]
     v += dppmov<0x141>(v);   // row_half_mirror
     v += dppmov<0x140>(v);   // row_mirror
-    return (rlane(v, 0) + rlane(v, 16)) + (rlane(v, 32) + rlane(v, 48));
+    return v;
+}
+
+// A[r][c] for r, c < n, else 0; the load itself is unconditional (clamped
+// indices), so that a run of them stays in flight.
+// Global-memory view of a pointer read from device memory (the compiler
+// cannot infer it, and flat accesses cost more than global ones).
+template <typename T>
+using gmem = __attribute__((address_space(1))) T;
+
+template <typename T>
+__device__ __forceinline__ T ldz(const gmem<T> *__restrict__ A, int lda, int n, int r, int c)
+{
+    const int rc = r < n ? r : n - 1, cc = c < n ? c : n - 1;
+    const T v = A[rc + (int64_t)cc * lda];
+    return (r < n && c < n) ? v : T(0);
+}
+
+// Forward substitution on the diagonal tile (row layout, a[c] = L[i][c],
+// zero above the diagonal; rd[c] = 1/L[c][c] in every lane): z_i ← y_i.
+template <int C, typename T>
+struct DiagFwd {
+    static __device__ __forceinline__ void run(const T a[16], const T rd[16], int i, T &z)
+    {
+        const T y = row_bcast<C>(z) * rd[C];
+        z = i == C ? y : fma_t(-a[C], y, z);
+        DiagFwd<C + 1, T>::run(a, rd, i, z);
+    }
+};
+template <typename T>
+struct DiagFwd<16, T> {
+    static __device__ __forceinline__ void run(const T *, const T *, int, T &) {}
+};
+
+// Back substitution on the diagonal tile's transpose (column layout,
+// lc[k] = L[k][i], zero above the diagonal; rdi = 1/L[i][i]): z_i ← x_i.
+template <int C, typename T>
+struct DiagBwd {
+    static __device__ __forceinline__ void run(const T lc[16], T rdi, int i, T &z)
+    {
+        const T x = row_bcast<C>(z * rdi);
+        z = i == C ? x : fma_t(-lc[C], x, z);
+        DiagBwd<C - 1, T>::run(lc, rdi, i, z);
+    }
+};
+template <typename T>
+struct DiagBwd<-1, T> {
+    static __device__ __forceinline__ void run(const T *, T, int, T &) {}
+};
+
+constexpr int kMR = 4;       // row tiles per group below the diagonal
+constexpr int kMTL = 17;     // padded row of the staging tile
+
+// K-loop operand staging: slabs of KB columns of L (16-row blocks, one for
+// the panel's rows and one per row tile, each [k][16 rows]), an NS-deep ring
+// per wave filled by direct-to-LDS dword loads (element-aligned for any lda,
+// and no registers held by loads in flight).
+template <typename T>
+struct WaveCfg;
+template <>
+struct WaveCfg<double> {
+    static constexpr int KB = 8, NS = 3;
+};
+template <>
+struct WaveCfg<float> {
+    static constexpr int KB = 16, NS = 2;
+};
+template <typename T>
+struct WaveLds {
+    static constexpr int KB = WaveCfg<T>::KB, NS = WaveCfg<T>::NS;
+    static constexpr int kBlock = 16 * KB;                   // elements of one block
+    static constexpr int kSlab = (kMR + 1) * kBlock;
+    static constexpr int kLoads = kSlab * (int)sizeof(T) / 256;   // dword loads per slab per wave
+    static constexpr int kElems = NS * kSlab + 16 * kMTL + kBMax + 64;
+};
+
+template <int N>
+__device__ __forceinline__ void wait_vm()
+{
+    static_assert(N >= 0 && N < 64, "vmcnt range");
+    __builtin_amdgcn_s_waitcnt((N & 15) | ((N >> 4) << 14) | 0x0F70);
 }
 
 template <typename T>
-__device__ __forceinline__ T sel4(const T v[4], int s)
+__device__ void wave_chol_solve(int n, T *Ap, int lda, T *bp, int *info, T *S)
 {
-    T r = v[0];
-    r = s == 1 ? v[1] : r;
-    r = s == 2 ? v[2] : r;
-    r = s == 3 ? v[3] : r;
-    return r;
-}
-
-template <typename T>
-__device__ void wave_chol_solve(int n, T *__restrict__ A, int lda, T *__restrict__ b, int *info,
-                                T *Bt)
-{
-    const int lane = threadIdx.x & 63;
-    T *colv = Bt + kWP * kWP;
+    typedef WaveLds<T> WL;
+    constexpr int KB = WL::KB, NS = WL::NS;
+    gmem<T> *__restrict__ A = (gmem<T> *)Ap;
+    gmem<T> *__restrict__ b = (gmem<T> *)bp;
+    typedef PanelMfma<T> Mf;
+    typedef typename Mf::acc_t acc_t;
+    const int lane = threadIdx.x & 63, li = lane & 15, g = lane >> 4;
+    T *ring = S;                        // NS slabs
+    T *Ts = S + NS * WL::kSlab;         // 16 × kMTL staging tile
+    T *bv = Ts + 16 * kMTL;             // b, then y, then x
+    T *red = bv + kBMax;                // per-group partial sums
+    int cq[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) cq[q] = Mf::row(lane, q);
+    // this lane's element of each 256-byte load: (row, k) within a block, and
+    // the byte within the element (FP64 elements take two lanes)
+    constexpr int EPL = 256 / (int)sizeof(T);    // elements per load
+    const int e_l = sizeof(T) == 8 ? (lane >> 1) : lane;
+    const int lrow = e_l & 15, lk = e_l >> 4;
+    const int lbyte = sizeof(T) == 8 ? 4 * (lane & 1) : 0;
+    for (int e = lane; e < n; e += 64) bv[e] = b[e];
+    border();
     int bad = 0;
-    for (int p0 = 0; p0 < n; p0 += kWP) {
-        const int pw = min(kWP, n - p0);
-        T pan[4][kWP];
-        // (a) the panel's lower part, rows ≥ p0
+    for (int p0 = 0; p0 < n; p0 += 16) {
+        const int pw = min(16, n - p0), nslab = p0 / KB;
+        // C −= L[rows, 0:p0] L[p0:p0+16, 0:p0]ᵀ for NB − 1 row tiles starting
+        // at r0 (block 0 of each slab: the panel's own rows) through the
+        // ring.  Every slab loads all NB blocks (rows past n clamped to row
+        // n − 1: finite values whose products are never stored), so that
+        // each wait counts a fixed number of loads and the MFMAs need no
+        // branches.  Load address: a scalar base (the slab's columns) plus
+        // this lane's byte offset within the block.
+        auto kloop = [&](auto nbc, int r0, acc_t *acc) {
+            constexpr int NB = decltype(nbc)::value;
+            constexpr int LPS = NB * (WL::kBlock / EPL);     // loads per slab
+            unsigned ob[NB];
 #pragma unroll
-        for (int s = 0; s < 4; ++s) {
-            const int r = lane + 64 * s;
-#pragma unroll
-            for (int j = 0; j < kWP; ++j)
-                pan[s][j] = (r < n && j < pw && r >= p0 + j) ? A[r + (int64_t)(p0 + j) * lda] : T(0);
-        }
-        // (b) pan -= L[rows, 0:p0] · L[p0:p0+32, 0:p0]ᵀ, 32 columns of L at a time
-        for (int kp = 0; kp < p0; kp += kWP) {
-#pragma unroll
-            for (int u = 0; u < kWP * kWP / 64; ++u) {
-                const int e = lane + 64 * u, j = e & (kWP - 1), kk = e / kWP;
-                Bt[kk * kWP + j] = j < pw ? A[(p0 + j) + (int64_t)(kp + kk) * lda] : T(0);
+            for (int t = 0; t < NB; ++t) {
+                const int rb = t == 0 ? p0 : r0 + 16 * (t - 1);
+                const int rr = min(rb + lrow, n - 1);
+                ob[t] = (unsigned)(rr + lk * lda) * (unsigned)sizeof(T) + lbyte;
             }
-            border();
-#pragma unroll 2
-            for (int kk = 0; kk < kWP; ++kk) {
-                T a[4];
+            auto issue = [&](int s) {
+                T *st = ring + (s % NS) * WL::kSlab;
 #pragma unroll
-                for (int s = 0; s < 4; ++s) {
-                    const int r = lane + 64 * s;
-                    a[s] = (r < n && r >= p0) ? A[r + (int64_t)(kp + kk) * lda] : T(0);
+                for (int u = 0; u < WL::kBlock / EPL; ++u) {
+                    const char *base = (const char *)(A + (int64_t)(s * KB + u * (EPL / 16)) * lda);
+#pragma unroll
+                    for (int t = 0; t < NB; ++t)
+                        __builtin_amdgcn_global_load_lds(
+                            (const void *)(base + ob[t]),
+                            (__attribute__((address_space(3))) void *)(st + t * WL::kBlock + u * EPL),
+                            4, 0, 0);
                 }
+            };
+            for (int s = 0; s < NS - 1 && s < nslab; ++s) issue(s);
+            for (int s = 0; s < nslab; ++s) {
+                if (NS == 3 && s + 1 < nslab)
+                    wait_vm<LPS>();
+                else
+                    wait_vm<0>();
+                border();   // keeps the LDS reads below the wait
+                if (s + NS - 1 < nslab) issue(s + NS - 1);
+                const T *st = ring + (s % NS) * WL::kSlab;
 #pragma unroll
-                for (int j = 0; j < kWP; ++j) {
-                    const T bj = Bt[kk * kWP + j];
+                for (int h = 0; h < KB / 4; ++h) {
+                    const int o = (4 * h + g) * 16 + li;
+                    const T pv = -st[o];
+                    if constexpr (NB == 1)
+                        acc[0] = Mf::mma(pv, -pv, acc[0]);
+                    else
 #pragma unroll
-                    for (int s = 0; s < 4; ++s) pan[s][j] -= a[s] * bj;
-                }
-            }
-            border();
-        }
-        // (c) factor the panel's 32 columns (cholUnblocked! on them)
-        const int so = p0 >> 6, lo = p0 & 63;     // rows p0..p0+kWP-1: lanes lo.., slot so
-#pragma unroll
-        for (int c = 0; c < kWP; ++c) {
-            if (c < pw) {
-            T pv[4];
-#pragma unroll
-            for (int s = 0; s < 4; ++s) pv[s] = pan[s][c];
-            const T piv = rlane(sel4(pv, so), lo + c);
-            if (piv < T(0) && !bad) bad = p0 + c + 1;
-            const T d = bsqrt(piv);
-            const T rr = T(1) / d;
-#pragma unroll
-            for (int s = 0; s < 4; ++s) {
-                const int r = lane + 64 * s;
-                pan[s][c] = r > p0 + c ? pan[s][c] * rr : (r == p0 + c ? d : pan[s][c]);
-            }
-            if (lane >= lo && lane < lo + kWP) {
-                T q[4];
-#pragma unroll
-                for (int s = 0; s < 4; ++s) q[s] = pan[s][c];
-                colv[lane - lo] = sel4(q, so);
-            }
-            border();
-#pragma unroll
-            for (int j = c + 1; j < kWP; ++j) {
-                const T lj = colv[j];
-#pragma unroll
-                for (int s = 0; s < 4; ++s) {
-                    const int r = lane + 64 * s;
-                    if (j < pw && r >= p0 + j) pan[s][j] -= pan[s][c] * lj;
+                        for (int t = 1; t < NB; ++t)
+                            acc[t - 1] = Mf::mma(pv, st[t * WL::kBlock + o], acc[t - 1]);
                 }
             }
+        };
+        T w[4], yq[4];
+        {
+            // the diagonal tile: factor in row layout, write back,
+            // forward-solve, invert
+            acc_t d[1];
+#pragma unroll
+            for (int q = 0; q < 4; ++q) d[0][q] = ldz(A, lda, n, p0 + li, p0 + cq[q]);
+            kloop(std::integral_constant<int, 1>{}, 0, d);
+#pragma unroll
+            for (int q = 0; q < 4; ++q) Ts[li * kMTL + cq[q]] = d[0][q];
             border();
+            T a[16], rd[16];
+#pragma unroll
+            for (int j = 0; j < 16; ++j)
+                a[j] = li >= pw ? (j == li ? T(1) : T(0)) : (j <= li ? Ts[li * kMTL + j] : T(0));
+            border();
+            int fneg = 16;
+            TileFactorCols<0, T>::run(a, rd, li, fneg);
+            if (fneg < pw && !bad) bad = p0 + fneg + 1;
+#pragma unroll
+            for (int j = 0; j < 16; ++j) a[j] = j <= li ? a[j] : T(0);
+            if (g == 0 && li < pw) {
+#pragma unroll
+                for (int j = 0; j < 16; ++j)
+                    if (j <= li) A[(p0 + li) + (int64_t)(p0 + j) * lda] = a[j];
+            }
+            T z = li < pw ? bv[p0 + li] : T(0);
+            DiagFwd<0, T>::run(a, rd, li, z);
+            if (g == 0 && li < pw) bv[p0 + li] = z;
+            // W = L_dd⁻¹: a[k] becomes the multiplier L[i][k]/L[k][k] (i > k)
+            T ri = rd[0];
+#pragma unroll
+            for (int k = 0; k < 16; ++k) {
+                ri = li == k ? rd[k] : ri;
+                a[k] = li > k ? a[k] * rd[k] : T(0);
+            }
+#pragma unroll
+            for (int q = 0; q < 4; ++q) w[q] = li == cq[q] ? T(1) : T(0);
+            TileInvSteps<0, T>::run(a, w);
+#pragma unroll
+            for (int q = 0; q < 4; ++q) w[q] *= ri;     // w[q] = W[i][c_q]
+            border();
+#pragma unroll
+            for (int q = 0; q < 4; ++q) yq[q] = cq[q] < pw ? bv[p0 + cq[q]] : T(0);
+        }
+        // the tiles below, kMR at a time
+        for (int r0 = p0 + 16; r0 < n; r0 += 16 * kMR) {
+            const int nt = min(kMR, (n - r0 + 15) >> 4);
+            acc_t c[kMR];
+#pragma unroll
+            for (int t = 0; t < kMR; ++t)
+#pragma unroll
+                for (int q = 0; q < 4; ++q)
+                    c[t][q] = t < nt ? ldz(A, lda, n, r0 + 16 * t + li, p0 + cq[q]) : T(0);
+            kloop(std::integral_constant<int, kMR + 1>{}, r0, c);
+            // X = C L_dd⁻ᵀ as Xᵀ = W Cᵀ, written back; b_rows −= X y_p
+#pragma unroll
+            for (int t = 0; t < kMR; ++t) {
+                if (t >= nt) continue;
+                acc_t x = {T(0), T(0), T(0), T(0)};
+#pragma unroll
+                for (int q = 0; q < 4; ++q) x = Mf::mma(w[q], c[t][q], x);
+                const int r = r0 + 16 * t + li;
+                T part = T(0);
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    if (r < n && cq[q] < pw) A[r + (int64_t)(p0 + cq[q]) * lda] = x[q];
+                    part = fma_t(x[q], yq[q], part);
+                }
+                red[g * 16 + li] = part;
+                border();
+                const T sum = (red[li] + red[16 + li]) + (red[32 + li] + red[48 + li]);
+                if (g == 0 && r < n) bv[r] -= sum;
+                border();
             }
         }
-        // (d) write the panel back
+    }
+    // Lᵀ x = y, panels from the last
+    for (int p0 = (n - 1) & ~15; p0 >= 0; p0 -= 16) {
+        const int pw = min(16, n - p0);
+        T lt[4];
 #pragma unroll
-        for (int s = 0; s < 4; ++s) {
-            const int r = lane + 64 * s;
+        for (int q = 0; q < 4; ++q) lt[q] = ldz(A, lda, n, p0 + li, p0 + cq[q]);
+        T s[4] = {T(0), T(0), T(0), T(0)};
+        for (int r0 = p0 + 16; r0 < n; r0 += 128) {
+            T v[8][4], xr[8];
 #pragma unroll
-            for (int j = 0; j < kWP; ++j)
-                if (r < n && j < pw && r >= p0 + j) A[r + (int64_t)(p0 + j) * lda] = pan[s][j];
+            for (int u = 0; u < 8; ++u)
+#pragma unroll
+                for (int q = 0; q < 4; ++q) v[u][q] = ldz(A, lda, n, r0 + 16 * u + li, p0 + cq[q]);
+#pragma unroll
+            for (int u = 0; u < 8; ++u) xr[u] = r0 + 16 * u + li < n ? bv[r0 + 16 * u + li] : T(0);
+#pragma unroll
+            for (int u = 0; u < 8; ++u)
+#pragma unroll
+                for (int q = 0; q < 4; ++q) s[q] = fma_t(v[u][q], xr[u], s[q]);
         }
-    }
-    // L y = b, then Lᵀ x = y, with b and 1/L_rr in registers
-    T bv[4], rdg[4];
 #pragma unroll
-    for (int s = 0; s < 4; ++s) {
-        const int r = lane + 64 * s;
-        bv[s] = r < n ? b[r] : T(0);
-        rdg[s] = r < n ? T(1) / A[r + (int64_t)r * lda] : T(0);
-    }
-    for (int c = 0; c < n; ++c) {
-        const int cs = c >> 6, cl = c & 63;
-        const T yc = rlane(sel4(bv, cs), cl) * rlane(sel4(rdg, cs), cl);
+        for (int q = 0; q < 4; ++q) s[q] = row_sum16(s[q]);
+        if (li == 0) {
 #pragma unroll
-        for (int s = 0; s < 4; ++s) {
-            const int r = lane + 64 * s;
-            if (s == cs && lane == cl) bv[s] = yc;
-            if (r > c && r < n) bv[s] -= A[r + (int64_t)c * lda] * yc;
+            for (int q = 0; q < 4; ++q) red[cq[q]] = s[q];
         }
-    }
-    for (int c = n - 1; c >= 0; --c) {
-        T part = T(0);
 #pragma unroll
-        for (int s = 0; s < 4; ++s) {
-            const int r = lane + 64 * s;
-            if (r > c && r < n) part += A[r + (int64_t)c * lda] * bv[s];
-        }
-        const T sum = wave_sum(part);
-        const int cs = c >> 6, cl = c & 63;
-        const T xc = (rlane(sel4(bv, cs), cl) - sum) * rlane(sel4(rdg, cs), cl);
+        for (int q = 0; q < 4; ++q) Ts[li * kMTL + cq[q]] = lt[q];
+        border();
+        T lc[16];
 #pragma unroll
-        for (int s = 0; s < 4; ++s)
-            if (s == cs && lane == cl) bv[s] = xc;
+        for (int k = 0; k < 16; ++k)
+            lc[k] = (li < pw && k < pw) ? (k >= li ? Ts[k * kMTL + li] : T(0)) : (k == li ? T(1) : T(0));
+        const T rdi = li < pw ? T(1) / Ts[li * kMTL + li] : T(1);
+        T z = li < pw ? bv[p0 + li] - red[li] : T(0);
+        border();
+        DiagBwd<15, T>::run(lc, rdi, li, z);
+        if (g == 0 && li < pw) bv[p0 + li] = z;
+        border();
     }
-#pragma unroll
-    for (int s = 0; s < 4; ++s) {
-        const int r = lane + 64 * s;
-        if (r < n) b[r] = bv[s];
-    }
+    for (int e = lane; e < n; e += 64) b[e] = bv[e];
     if (lane == 0 && info) *info = bad;
 }
 
@@ -395,30 +540,40 @@ __device__ void wg_chol_solve(int n, T *__restrict__ A, int lda, T *__restrict__
     __syncthreads();
 }
 
+// Workgroup items: one matrix per workgroup.
 template <typename T>
-__global__ __launch_bounds__(kBThreads, 1) void batched_chol_solve_kernel(
-    int nitems, const int4 *__restrict__ items, const int *__restrict__ nvec,
+__global__ __launch_bounds__(kBThreads, 1) void batched_wg_kernel(
+    int nwg, const int *__restrict__ items, const int *__restrict__ nvec,
     T *const *__restrict__ Aptr, const int *__restrict__ ldav, T *const *__restrict__ bptr,
     int *__restrict__ info)
 {
-    __shared__ T P[kBMax * kPLD];       // panel (workgroup path) or 4 wave matrices
+    __shared__ T P[kBMax * kPLD];
     __shared__ T bs[kBMax];
     __shared__ T sc[2 * kPW];
-    static_assert(4 * (kWP * kWP + kWP) <= kBMax * kPLD, "wave scratch fits the panel buffer");
-    for (int it = blockIdx.x; it < nitems; it += gridDim.x) {
-        const int4 item = items[it];
-        if (item.y == -2) {                 // one matrix, whole workgroup
-            const int i = item.x;
-            wg_chol_solve<T>(nvec[i], Aptr[i], ldav[i], bptr[i], info ? info + i : nullptr, P, bs,
-                             sc);
-        } else {                            // up to four small matrices, one per wave
-            const int w = threadIdx.x >> 6;
-            const int i = w == 0 ? item.x : (w == 1 ? item.y : (w == 2 ? item.z : item.w));
-            if (i >= 0)
-                wave_chol_solve<T>(nvec[i], Aptr[i], ldav[i], bptr[i], info ? info + i : nullptr,
-                                   P + w * (kWP * kWP + kWP));
-            __syncthreads();
-        }
+    for (int it = blockIdx.x; it < nwg; it += gridDim.x) {
+        const int i = items[it];
+        wg_chol_solve<T>(nvec[i], Aptr[i], ldav[i], bptr[i], info ? info + i : nullptr, P, bs, sc);
+    }
+}
+
+// Wave items: the matrices at or below wave_max, largest first; global wave
+// gw of GW takes rank gw in even rounds and GW − 1 − gw in odd ones (rounds
+// of GW ranks), which evens out the sum of sizes per wave.
+template <typename T>
+__global__ __launch_bounds__(kBThreads) void batched_wave_kernel(
+    int nsmall, const int *__restrict__ order, const int *__restrict__ nvec,
+    T *const *__restrict__ Aptr, const int *__restrict__ ldav, T *const *__restrict__ bptr,
+    int *__restrict__ info)
+{
+    __shared__ T S[4 * WaveLds<T>::kElems];
+    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int GW = gridDim.x * 4, gw = blockIdx.x * 4 + w;
+    for (int base = 0, odd = 0; base < nsmall; base += GW, odd ^= 1) {
+        const int j = base + (odd ? GW - 1 - gw : gw);
+        if (j >= nsmall) continue;
+        const int i = order[j];
+        wave_chol_solve<T>(nvec[i], Aptr[i], ldav[i], bptr[i], info ? info + i : nullptr,
+                           S + w * WaveLds<T>::kElems);
     }
 }
 
@@ -428,11 +583,17 @@ template <typename T>
 int batched_launch(Handle *h, BatchPlan *pl, T *const *A, const int *lda, T *const *b, int *info,
                    hipStream_t s)
 {
-    if (pl->nitems == 0) return GLA_OK;
     (void)h;
-    hipLaunchKernelGGL((batched_chol_solve_kernel<T>), dim3(pl->grid), dim3(kBThreads), 0, s,
-                       pl->nitems, pl->d_items, pl->d_n, A, lda, b, info);
-    GLA_LAUNCH_CHECK();
+    if (pl->nwg > 0) {
+        hipLaunchKernelGGL((batched_wg_kernel<T>), dim3(pl->grid_wg), dim3(kBThreads), 0, s, pl->nwg,
+                           pl->d_wg, pl->d_n, A, lda, b, info);
+        GLA_LAUNCH_CHECK();
+    }
+    if (pl->nsmall > 0) {
+        hipLaunchKernelGGL((batched_wave_kernel<T>), dim3(pl->grid_wave), dim3(kBThreads), 0, s,
+                           pl->nsmall, pl->d_small, pl->d_n, A, lda, b, info);
+        GLA_LAUNCH_CHECK();
+    }
     return GLA_OK;
 }
 
@@ -460,31 +621,27 @@ int gla_batch_plan_create(gla_handle_t hh, int64_t batch, const int32_t *n_host,
     std::vector<int> order((size_t)batch);
     for (int64_t i = 0; i < batch; ++i) order[(size_t)i] = (int)i;
     std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return n_host[a] > n_host[b]; });
-    std::vector<int4> items;
     size_t k = 0;
-    for (; k < order.size() && n_host[order[k]] > wave_max; ++k)
-        items.push_back(make_int4(order[k], -2, -2, -2));
-    for (; k < order.size(); k += 4) {
-        int4 it = make_int4(-1, -1, -1, -1);
-        it.x = order[k];
-        if (k + 1 < order.size()) it.y = order[k + 1];
-        if (k + 2 < order.size()) it.z = order[k + 2];
-        if (k + 3 < order.size()) it.w = order[k + 3];
-        items.push_back(it);
-    }
+    while (k < order.size() && n_host[order[k]] > wave_max) ++k;
     auto *pl = new BatchPlan();
     pl->device = h->device;
     pl->batch = batch;
-    pl->nitems = (int)items.size();
+    pl->nwg = (int)k;
+    pl->nsmall = (int)(order.size() - k);
     pl->wave_max = wave_max;
-    pl->grid = std::max(1, std::min<int>(pl->nitems, h->num_cus * grid_mult));
+    pl->grid_wg = std::max(1, std::min<int>(pl->nwg, h->num_cus * grid_mult));
+    pl->grid_wave = std::max(1, std::min<int>((pl->nsmall + 3) / 4, h->num_cus * grid_mult));
     if (batch > 0) {
         GLA_HIP_TRY(hipMalloc(reinterpret_cast<void **>(&pl->d_n), sizeof(int) * batch));
         GLA_HIP_TRY(hipMemcpy(pl->d_n, n_host, sizeof(int) * batch, hipMemcpyHostToDevice));
     }
-    if (!items.empty()) {
-        GLA_HIP_TRY(hipMalloc(reinterpret_cast<void **>(&pl->d_items), sizeof(int4) * items.size()));
-        GLA_HIP_TRY(hipMemcpy(pl->d_items, items.data(), sizeof(int4) * items.size(),
+    if (pl->nwg > 0) {
+        GLA_HIP_TRY(hipMalloc(reinterpret_cast<void **>(&pl->d_wg), sizeof(int) * pl->nwg));
+        GLA_HIP_TRY(hipMemcpy(pl->d_wg, order.data(), sizeof(int) * pl->nwg, hipMemcpyHostToDevice));
+    }
+    if (pl->nsmall > 0) {
+        GLA_HIP_TRY(hipMalloc(reinterpret_cast<void **>(&pl->d_small), sizeof(int) * pl->nsmall));
+        GLA_HIP_TRY(hipMemcpy(pl->d_small, order.data() + k, sizeof(int) * pl->nsmall,
                               hipMemcpyHostToDevice));
     }
     *out = reinterpret_cast<gla_batch_plan_t>(pl);
@@ -498,7 +655,8 @@ int gla_batch_plan_destroy(gla_batch_plan_t p)
     GLA_HIP_TRY(hipSetDevice(pl->device));
     GLA_HIP_TRY(hipDeviceSynchronize());
     if (pl->d_n) (void)hipFree(pl->d_n);
-    if (pl->d_items) (void)hipFree(pl->d_items);
+    if (pl->d_wg) (void)hipFree(pl->d_wg);
+    if (pl->d_small) (void)hipFree(pl->d_small);
     delete pl;
     return GLA_OK;
 }

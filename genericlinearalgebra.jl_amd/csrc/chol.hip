@@ -77,103 +77,15 @@ __device__ __forceinline__ T lane_bcast(T v, int src)
 }
 
 
-// s = √a and r = 1/√a from the hardware reciprocal square root refined by
-// Newton steps (two for FP64, one for FP32) and one correction of s = a·r:
-// both within an ulp or two of sqrt(a) and 1/sqrt(a), at a fraction of the
-// latency of a correctly rounded square root followed by a division, which
-// sit on the serial path of every column.  a = 0 gives s = 0, r = ∞ and a
-// negative or NaN a gives NaN, as sqrt and 1/sqrt do.
-__device__ __forceinline__ void sqrt_and_rsqrt(double a, double &s, double &r)
-{
-    double y = __builtin_amdgcn_rsq(a);
-    const double h = 0.5 * a;
-    double e = __builtin_fma(-h * y, y, 0.5);
-    y = __builtin_fma(y, e, y);
-    e = __builtin_fma(-h * y, y, 0.5);
-    y = __builtin_fma(y, e, y);
-    double q = a * y;
-    q = __builtin_fma(__builtin_fma(-q, q, a), 0.5 * y, q);
-    const bool zero = a == 0.0;
-    s = zero ? 0.0 : q;
-    r = zero ? __builtin_inf() : y;
-}
-__device__ __forceinline__ void sqrt_and_rsqrt(float a, float &s, float &r)
-{
-    float y = __builtin_amdgcn_rsqf(a);
-    const float h = 0.5f * a;
-    const float e = __builtin_fmaf(-h * y, y, 0.5f);
-    y = __builtin_fmaf(y, e, y);
-    float q = a * y;
-    q = __builtin_fmaf(__builtin_fmaf(-q, q, a), 0.5f * y, q);
-    const bool zero = a == 0.0f;
-    s = zero ? 0.0f : q;
-    r = zero ? __builtin_inff() : y;
-}
-
-// Row layout of a 16×16 tile: lane (i = l % 16, g = l / 16) holds all of row
-// i in a[0..15], the same in each of the four 16-lane groups, so that every
-// value a column step needs moves by a DPP broadcast within the lane's own
-// 16-lane row (a few cycles) rather than through the LDS crossbar.
-
-// Column C's rank-1 update of the entries a[J..15]: a[j] -= l_i l_j, with
-// l_j = L[j][C] broadcast from lane j.  Entries right of the diagonal (j > i)
-// take meaningless values and are cleared by the caller.
-template <int C, int J, typename T>
-struct TileColUpdate {
-    static __device__ __forceinline__ void run(T a[16], T l)
-    {
-        const T lj = row_bcast<J>(l);
-        if constexpr (sizeof(T) == 8)
-            a[J] = __builtin_fma(-l, lj, a[J]);
-        else
-            a[J] = __builtin_fmaf(-l, lj, a[J]);
-        TileColUpdate<C, J + 1, T>::run(a, l);
-    }
-};
-template <int C, typename T>
-struct TileColUpdate<C, 16, T> {
-    static __device__ __forceinline__ void run(T *, T) {}
-};
-
-// Factors the symmetric tile held in row layout in place, right-looking,
-// column by column as cholUnblocked! does (src/cholesky.jl:3-15): sqrt of
-// the pivot, scale by its reciprocal, rank-1 update (the update products are
-// (a_ic r)(a_jc r), each factor rounded once, as in the scaled column).
-// rd[c] receives 1/L[c][c].  The first pivot whose square root is taken of
-// a negative number, among the first `nvalid`, is recorded in *info as
-// gbase + c + 1.  Entries right of the diagonal are left meaningless.
-// The 16 column steps are one branch-free block (negative pivots are only
-// marked in `neg` and reported after the tile), so that the compiler can
-// overlap the trailing updates of column c with the square root of c + 1.
-template <int C, typename T>
-struct TileFactorCols {
-    static __device__ __forceinline__ void run(T a[16], T rd[16], int i, unsigned &neg)
-    {
-        const T d = row_bcast<C>(a[C]);
-        neg |= d < T(0) ? (1u << C) : 0u;
-        T s, r;
-        sqrt_and_rsqrt(d, s, r);
-        rd[C] = r;
-        const T l = a[C] * r;
-        a[C] = i == C ? s : l;
-        TileColUpdate<C, C + 1, T>::run(a, l);
-        TileFactorCols<C + 1, T>::run(a, rd, i, neg);
-    }
-};
-template <typename T>
-struct TileFactorCols<16, T> {
-    static __device__ __forceinline__ void run(T *, T *, int, unsigned &) {}
-};
 
 template <typename T>
 __device__ __forceinline__ void tile_factor_rows(T a[16], T rd[16], int nvalid, int *info,
                                                  int64_t gbase)
 {
     const int i = threadIdx.x & 15;
-    unsigned neg = 0;
-    TileFactorCols<0, T>::run(a, rd, i, neg);
-    if (nvalid < 16) neg &= (1u << (nvalid > 0 ? nvalid : 0)) - 1u;
-    if (neg && (threadIdx.x & 63) == 0 && info) atomicCAS(info, 0, (int)(gbase + __builtin_ctz(neg) + 1));
+    int fneg = 16;
+    TileFactorCols<0, T>::run(a, rd, i, fneg);
+    if (fneg < nvalid && (threadIdx.x & 63) == 0 && info) atomicCAS(info, 0, (int)(gbase + fneg + 1));
 #pragma unroll
     for (int j = 0; j < 16; ++j) a[j] = j <= i ? a[j] : T(0);
 }

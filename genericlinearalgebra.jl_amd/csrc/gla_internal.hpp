@@ -89,6 +89,95 @@ struct TileInvStepsUp<0, T> {   // row 0 has no rows above it
     static __device__ __forceinline__ void run(const T *, T *) {}
 };
 
+// s = √a and r = 1/√a from the hardware reciprocal square root refined by
+// Newton steps (two for FP64, one for FP32) and one correction of s = a·r:
+// both within an ulp or two of sqrt(a) and 1/sqrt(a), at a fraction of the
+// latency of a correctly rounded square root followed by a division, which
+// sit on the serial path of every column.  a = 0 gives s = 0, r = ∞ and a
+// negative or NaN a gives NaN, as sqrt and 1/sqrt do.
+__device__ __forceinline__ void sqrt_and_rsqrt(double a, double &s, double &r)
+{
+    double y = __builtin_amdgcn_rsq(a);
+    const double h = 0.5 * a;
+    double e = __builtin_fma(-h * y, y, 0.5);
+    y = __builtin_fma(y, e, y);
+    e = __builtin_fma(-h * y, y, 0.5);
+    y = __builtin_fma(y, e, y);
+    double q = a * y;
+    q = __builtin_fma(__builtin_fma(-q, q, a), 0.5 * y, q);
+    const bool zero = a == 0.0;
+    s = zero ? 0.0 : q;
+    r = zero ? __builtin_inf() : y;
+}
+__device__ __forceinline__ void sqrt_and_rsqrt(float a, float &s, float &r)
+{
+    float y = __builtin_amdgcn_rsqf(a);
+    const float h = 0.5f * a;
+    const float e = __builtin_fmaf(-h * y, y, 0.5f);
+    y = __builtin_fmaf(y, e, y);
+    float q = a * y;
+    q = __builtin_fmaf(__builtin_fmaf(-q, q, a), 0.5f * y, q);
+    const bool zero = a == 0.0f;
+    s = zero ? 0.0f : q;
+    r = zero ? __builtin_inff() : y;
+}
+
+// Row layout of a 16×16 tile: lane (i = l % 16, g = l / 16) holds all of row
+// i in a[0..15], the same in each of the four 16-lane groups, so that every
+// value a column step needs moves by a DPP broadcast within the lane's own
+// 16-lane row (a few cycles) rather than through the LDS crossbar.
+
+// Column C's rank-1 update of the entries a[J..15]: a[j] -= l_i l_j, with
+// l_j = L[j][C] broadcast from lane j.  Entries right of the diagonal (j > i)
+// take meaningless values and are cleared by the caller.
+template <int C, int J, typename T>
+struct TileColUpdate {
+    static __device__ __forceinline__ void run(T a[16], T l)
+    {
+        const T lj = row_bcast<J>(l);
+        if constexpr (sizeof(T) == 8)
+            a[J] = __builtin_fma(-l, lj, a[J]);
+        else
+            a[J] = __builtin_fmaf(-l, lj, a[J]);
+        TileColUpdate<C, J + 1, T>::run(a, l);
+    }
+};
+template <int C, typename T>
+struct TileColUpdate<C, 16, T> {
+    static __device__ __forceinline__ void run(T *, T) {}
+};
+
+// Factors the symmetric tile held in row layout in place, right-looking,
+// column by column as cholUnblocked! does (src/cholesky.jl:3-15): sqrt of
+// the pivot, scale by its reciprocal, rank-1 update (the update products are
+// (a_ic r)(a_jc r), each factor rounded once, as in the scaled column).
+// rd[c] receives 1/L[c][c].  The first pivot whose square root is taken of
+// a negative number, among the first `nvalid`, is recorded in *info as
+// gbase + c + 1.  Entries right of the diagonal are left meaningless.
+// The 16 column steps are one branch-free block (the first negative pivot
+// is only noted in `fneg`, its column or 16 if none, and reported after the
+// tile), so that the compiler can overlap the trailing updates of column c
+// with the square root of c + 1.
+template <int C, typename T>
+struct TileFactorCols {
+    static __device__ __forceinline__ void run(T a[16], T rd[16], int i, int &fneg)
+    {
+        const T d = row_bcast<C>(a[C]);
+        fneg = (d < T(0) && fneg == 16) ? C : fneg;
+        T s, r;
+        sqrt_and_rsqrt(d, s, r);
+        rd[C] = r;
+        const T l = a[C] * r;
+        a[C] = i == C ? s : l;
+        TileColUpdate<C, C + 1, T>::run(a, l);
+        TileFactorCols<C + 1, T>::run(a, rd, i, fneg);
+    }
+};
+template <typename T>
+struct TileFactorCols<16, T> {
+    static __device__ __forceinline__ void run(T *, T *, int, int &) {}
+};
+
 // 16×16×4 MFMA with one A and one B element per lane; accumulator element r
 // of lane l sits at (row(l, r), l % 16) of the 16×16 tile.
 template <typename T>
