@@ -147,7 +147,8 @@ struct DiagBwd<-1, T> {
     static __device__ __forceinline__ void run(const T *, T, int, T &) {}
 };
 
-constexpr int kMR = 4;       // row tiles per group below the diagonal
+constexpr int kPW2 = 32;     // panel width of the wave path
+constexpr int kMR = 4;       // ring blocks per slab, less one
 constexpr int kMTL = 17;     // padded row of the staging tile
 
 // K-loop operand staging: slabs of KB columns of L (16-row blocks, one for
@@ -170,7 +171,8 @@ struct WaveLds {
     static constexpr int kBlock = 16 * KB;                   // elements of one block
     static constexpr int kSlab = (kMR + 1) * kBlock;
     static constexpr int kLoads = kSlab * (int)sizeof(T) / 256;   // dword loads per slab per wave
-    static constexpr int kElems = NS * kSlab + 16 * kMTL + kBMax + 64;
+    static constexpr int kRing = NS * kSlab;
+    static constexpr int kElems = kRing + 16 * kMTL + kBMax + 64;
 };
 
 template <int N>
@@ -203,178 +205,267 @@ __device__ void wave_chol_solve(int n, T *Ap, int lda, T *bp, int *info, T *S)
     const int e_l = sizeof(T) == 8 ? (lane >> 1) : lane;
     const int lrow = e_l & 15, lk = e_l >> 4;
     const int lbyte = sizeof(T) == 8 ? 4 * (lane & 1) : 0;
+    // Streams slabs 0..nslab−1 (KB columns of L each) of NB 16-row blocks
+    // (block t from row rb[t]; rows past n are clamped to row n − 1, finite
+    // values whose products are never stored) through the ring, as deep as
+    // the ring holds slabs of that size; pre() runs once the first slabs are
+    // in flight, then body(slab, s) for each slab in order.  Every slab loads
+    // all NB blocks, so each wait counts a fixed number of loads.  Load
+    // address: a scalar base (the slab's columns) plus this lane's byte
+    // offset within the block.
+    auto stream = [&](auto nbc, const int *rb, int nslab, auto &&pre, auto &&body) {
+        constexpr int NB = decltype(nbc)::value;
+        constexpr int LPS = NB * (WL::kBlock / EPL);      // loads per slab
+        constexpr int SS = NB * WL::kBlock;               // slab elements
+        constexpr int NSB = WL::kRing / SS > 16 ? 16 : WL::kRing / SS;
+        static_assert(NSB >= 2 && (NSB - 2) * LPS <= 63, "ring depth");
+        unsigned ob[NB];
+#pragma unroll
+        for (int t = 0; t < NB; ++t) {
+            const int rr = min(rb[t] + lrow, n - 1);
+            ob[t] = (unsigned)(rr + lk * lda) * (unsigned)sizeof(T) + lbyte;
+        }
+        auto issue = [&](int s) {
+            T *st = ring + (s % NSB) * SS;
+#pragma unroll
+            for (int u = 0; u < WL::kBlock / EPL; ++u) {
+                const char *base = (const char *)(A + (int64_t)(s * KB + u * (EPL / 16)) * lda);
+#pragma unroll
+                for (int t = 0; t < NB; ++t)
+                    __builtin_amdgcn_global_load_lds(
+                        (const void *)(base + ob[t]),
+                        (__attribute__((address_space(3))) void *)(st + t * WL::kBlock + u * EPL),
+                        4, 0, 0);
+            }
+        };
+        for (int s = 0; s < NSB - 1 && s < nslab; ++s) issue(s);
+        pre();
+        for (int s = 0; s < nslab; ++s) {
+            if (s + NSB - 1 < nslab)
+                wait_vm<(NSB - 2) * LPS>();
+            else
+                wait_vm<0>();
+            border();   // keeps the LDS reads below the wait
+            if (s + NSB - 1 < nslab) issue(s + NSB - 1);
+            body((const T *)(ring + (s % NSB) * SS), s);
+        }
+    };
     for (int e = lane; e < n; e += 64) bv[e] = b[e];
     border();
     int bad = 0;
-    for (int p0 = 0; p0 < n; p0 += 16) {
-        const int pw = min(16, n - p0), nslab = p0 / KB;
-        // C −= L[rows, 0:p0] L[p0:p0+16, 0:p0]ᵀ for NB − 1 row tiles starting
-        // at r0 (block 0 of each slab: the panel's own rows) through the
-        // ring.  Every slab loads all NB blocks (rows past n clamped to row
-        // n − 1: finite values whose products are never stored), so that
-        // each wait counts a fixed number of loads and the MFMAs need no
-        // branches.  Load address: a scalar base (the slab's columns) plus
-        // this lane's byte offset within the block.
-        auto kloop = [&](auto nbc, int r0, acc_t *acc) {
-            constexpr int NB = decltype(nbc)::value;
-            constexpr int LPS = NB * (WL::kBlock / EPL);     // loads per slab
-            unsigned ob[NB];
+    auto nothing = [] {};
+    // factor + invert + forward-solve one 16×16 diagonal tile given in the
+    // accumulator layout: L written back at (row0, row0) for the first nv
+    // rows, y = L⁻¹ z at bv[row0..], W = L⁻¹ to w (w[q] = W[i][c_q])
+    auto diag_tile = [&](const acc_t &t, int row0, int nv, T zin, T w[4]) {
 #pragma unroll
-            for (int t = 0; t < NB; ++t) {
-                const int rb = t == 0 ? p0 : r0 + 16 * (t - 1);
-                const int rr = min(rb + lrow, n - 1);
-                ob[t] = (unsigned)(rr + lk * lda) * (unsigned)sizeof(T) + lbyte;
+        for (int q = 0; q < 4; ++q) Ts[li * kMTL + cq[q]] = t[q];
+        border();
+        T a[16], rd[16];
+#pragma unroll
+        for (int j = 0; j < 16; ++j)
+            a[j] = li >= nv ? (j == li ? T(1) : T(0)) : (j <= li ? Ts[li * kMTL + j] : T(0));
+        border();
+        int fneg = 16;
+        TileFactorCols<0, T>::run(a, rd, li, fneg);
+        if (fneg < nv && !bad) bad = row0 + fneg + 1;
+#pragma unroll
+        for (int j = 0; j < 16; ++j) a[j] = j <= li ? a[j] : T(0);
+        if (g == 0 && li < nv) {
+#pragma unroll
+            for (int j = 0; j < 16; ++j)
+                if (j <= li) A[(row0 + li) + (int64_t)(row0 + j) * lda] = a[j];
+        }
+        T z = li < nv ? zin : T(0);
+        DiagFwd<0, T>::run(a, rd, li, z);
+        if (g == 0 && li < nv) bv[row0 + li] = z;
+        // W = L⁻¹: a[k] becomes the multiplier L[i][k]/L[k][k] (i > k)
+        T ri = rd[0];
+#pragma unroll
+        for (int k = 0; k < 16; ++k) {
+            ri = li == k ? rd[k] : ri;
+            a[k] = li > k ? a[k] * rd[k] : T(0);
+        }
+#pragma unroll
+        for (int q = 0; q < 4; ++q) w[q] = li == cq[q] ? T(1) : T(0);
+        TileInvSteps<0, T>::run(a, w);
+#pragma unroll
+        for (int q = 0; q < 4; ++q) w[q] *= ri;
+        border();
+    };
+    // Σ over the tile's 4 columns of this lane (values v, weights y) and
+    // the four lane groups: the same sum in every lane of row i
+    auto row_dot = [&](const T v[4], const T y[4]) {
+        T part = T(0);
+#pragma unroll
+        for (int q = 0; q < 4; ++q) part = fma_t(v[q], y[q], part);
+        red[g * 16 + li] = part;
+        border();
+        const T sum = (red[li] + red[16 + li]) + (red[32 + li] + red[48 + li]);
+        border();
+        return sum;
+    };
+    for (int p0 = 0; p0 < n; p0 += kPW2) {
+        // panel of 32 columns as tiles 0 (p0..p0+15) and 1 (p0+16..p0+31)
+        const int pw = min(kPW2, n - p0), pw0 = min(16, pw), pw1 = pw - pw0;
+        const int nslab = p0 / KB;
+        T w0[4], w1[4], y0q[4], y1q[4];
+        acc_t l10;
+        {
+            // the 32×32 diagonal block: T00, T10, T11
+            acc_t t00, t10, t11;
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                t00[q] = ldz(A, lda, n, p0 + li, p0 + cq[q]);
+                t10[q] = ldz(A, lda, n, p0 + 16 + li, p0 + cq[q]);
+                t11[q] = ldz(A, lda, n, p0 + 16 + li, p0 + 16 + cq[q]);
             }
-            auto issue = [&](int s) {
-                T *st = ring + (s % NS) * WL::kSlab;
-#pragma unroll
-                for (int u = 0; u < WL::kBlock / EPL; ++u) {
-                    const char *base = (const char *)(A + (int64_t)(s * KB + u * (EPL / 16)) * lda);
-#pragma unroll
-                    for (int t = 0; t < NB; ++t)
-                        __builtin_amdgcn_global_load_lds(
-                            (const void *)(base + ob[t]),
-                            (__attribute__((address_space(3))) void *)(st + t * WL::kBlock + u * EPL),
-                            4, 0, 0);
-                }
-            };
-            for (int s = 0; s < NS - 1 && s < nslab; ++s) issue(s);
-            for (int s = 0; s < nslab; ++s) {
-                if (NS == 3 && s + 1 < nslab)
-                    wait_vm<LPS>();
-                else
-                    wait_vm<0>();
-                border();   // keeps the LDS reads below the wait
-                if (s + NS - 1 < nslab) issue(s + NS - 1);
-                const T *st = ring + (s % NS) * WL::kSlab;
+            const int rb[2] = {p0, p0 + 16};
+            stream(std::integral_constant<int, 2>{}, rb, nslab, nothing, [&](const T *st, int) {
 #pragma unroll
                 for (int h = 0; h < KB / 4; ++h) {
                     const int o = (4 * h + g) * 16 + li;
-                    const T pv = -st[o];
-                    if constexpr (NB == 1)
-                        acc[0] = Mf::mma(pv, -pv, acc[0]);
-                    else
-#pragma unroll
-                        for (int t = 1; t < NB; ++t)
-                            acc[t - 1] = Mf::mma(pv, st[t * WL::kBlock + o], acc[t - 1]);
+                    const T v0 = -st[o], v1 = st[WL::kBlock + o];
+                    t00 = Mf::mma(v0, -v0, t00);
+                    t10 = Mf::mma(v0, v1, t10);
+                    t11 = Mf::mma(-v1, v1, t11);
                 }
+            });
+            diag_tile(t00, p0, pw0, bv[p0 + li], w0);
+#pragma unroll
+            for (int q = 0; q < 4; ++q) y0q[q] = cq[q] < pw0 ? bv[p0 + cq[q]] : T(0);
+            // L10 = T10 L00⁻ᵀ, T11 −= L10 L10ᵀ, z1 = b1 − L10 y0
+            l10 = acc_t{T(0), T(0), T(0), T(0)};
+#pragma unroll
+            for (int q = 0; q < 4; ++q) l10 = Mf::mma(w0[q], t10[q], l10);
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                if (li < pw1 && cq[q] < pw0) A[(p0 + 16 + li) + (int64_t)(p0 + cq[q]) * lda] = l10[q];
+                t11 = Mf::mma(-l10[q], l10[q], t11);
             }
-        };
-        T w[4], yq[4];
-        {
-            // the diagonal tile: factor in row layout, write back,
-            // forward-solve, invert
-            acc_t d[1];
+            T l10v[4] = {l10[0], l10[1], l10[2], l10[3]};
+            const T s1 = row_dot(l10v, y0q);
+            diag_tile(t11, p0 + 16, pw1, bv[p0 + 16 + li] - s1, w1);
 #pragma unroll
-            for (int q = 0; q < 4; ++q) d[0][q] = ldz(A, lda, n, p0 + li, p0 + cq[q]);
-            kloop(std::integral_constant<int, 1>{}, 0, d);
-#pragma unroll
-            for (int q = 0; q < 4; ++q) Ts[li * kMTL + cq[q]] = d[0][q];
-            border();
-            T a[16], rd[16];
-#pragma unroll
-            for (int j = 0; j < 16; ++j)
-                a[j] = li >= pw ? (j == li ? T(1) : T(0)) : (j <= li ? Ts[li * kMTL + j] : T(0));
-            border();
-            int fneg = 16;
-            TileFactorCols<0, T>::run(a, rd, li, fneg);
-            if (fneg < pw && !bad) bad = p0 + fneg + 1;
-#pragma unroll
-            for (int j = 0; j < 16; ++j) a[j] = j <= li ? a[j] : T(0);
-            if (g == 0 && li < pw) {
-#pragma unroll
-                for (int j = 0; j < 16; ++j)
-                    if (j <= li) A[(p0 + li) + (int64_t)(p0 + j) * lda] = a[j];
-            }
-            T z = li < pw ? bv[p0 + li] : T(0);
-            DiagFwd<0, T>::run(a, rd, li, z);
-            if (g == 0 && li < pw) bv[p0 + li] = z;
-            // W = L_dd⁻¹: a[k] becomes the multiplier L[i][k]/L[k][k] (i > k)
-            T ri = rd[0];
-#pragma unroll
-            for (int k = 0; k < 16; ++k) {
-                ri = li == k ? rd[k] : ri;
-                a[k] = li > k ? a[k] * rd[k] : T(0);
-            }
-#pragma unroll
-            for (int q = 0; q < 4; ++q) w[q] = li == cq[q] ? T(1) : T(0);
-            TileInvSteps<0, T>::run(a, w);
-#pragma unroll
-            for (int q = 0; q < 4; ++q) w[q] *= ri;     // w[q] = W[i][c_q]
-            border();
-#pragma unroll
-            for (int q = 0; q < 4; ++q) yq[q] = cq[q] < pw ? bv[p0 + cq[q]] : T(0);
+            for (int q = 0; q < 4; ++q) y1q[q] = cq[q] < pw1 ? bv[p0 + 16 + cq[q]] : T(0);
         }
-        // the tiles below, kMR at a time
-        for (int r0 = p0 + 16; r0 < n; r0 += 16 * kMR) {
-            const int nt = min(kMR, (n - r0 + 15) >> 4);
-            acc_t c[kMR];
+        // the rows below, two tiles at a time: C_j −= R L_jᵀ for the
+        // panel's tiles j = 0, 1, then X_0 = C_0 L00⁻ᵀ, C_1 −= X_0 L10ᵀ,
+        // X_1 = C_1 L11⁻ᵀ, written back; b_rows −= X_0 y0 + X_1 y1
+        for (int r0 = p0 + kPW2; r0 < n; r0 += 32) {
+            const int nt = min(2, (n - r0 + 15) >> 4);
+            acc_t c[2][2];
 #pragma unroll
-            for (int t = 0; t < kMR; ++t)
+            for (int t = 0; t < 2; ++t)
 #pragma unroll
-                for (int q = 0; q < 4; ++q)
-                    c[t][q] = t < nt ? ldz(A, lda, n, r0 + 16 * t + li, p0 + cq[q]) : T(0);
-            kloop(std::integral_constant<int, kMR + 1>{}, r0, c);
-            // X = C L_dd⁻ᵀ as Xᵀ = W Cᵀ, written back; b_rows −= X y_p
+                for (int j = 0; j < 2; ++j)
 #pragma unroll
-            for (int t = 0; t < kMR; ++t) {
+                    for (int q = 0; q < 4; ++q)
+                        c[t][j][q] = t < nt ? ldz(A, lda, n, r0 + 16 * t + li, p0 + 16 * j + cq[q]) : T(0);
+            const int rb[4] = {p0, p0 + 16, r0, r0 + 16};
+            auto group = [&](auto nbc) {
+                constexpr int NB = decltype(nbc)::value;
+                stream(nbc, rb, nslab, nothing, [&](const T *st, int) {
+#pragma unroll
+                    for (int h = 0; h < KB / 4; ++h) {
+                        const int o = (4 * h + g) * 16 + li;
+                        const T v0 = -st[o], v1 = -st[WL::kBlock + o];
+#pragma unroll
+                        for (int t = 0; t < NB - 2; ++t) {
+                            const T rv = st[(2 + t) * WL::kBlock + o];
+                            c[t][0] = Mf::mma(v0, rv, c[t][0]);
+                            c[t][1] = Mf::mma(v1, rv, c[t][1]);
+                        }
+                    }
+                });
+            };
+            if (nt == 1)
+                group(std::integral_constant<int, 3>{});
+            else
+                group(std::integral_constant<int, 4>{});
+#pragma unroll
+            for (int t = 0; t < 2; ++t) {
                 if (t >= nt) continue;
-                acc_t x = {T(0), T(0), T(0), T(0)};
+                acc_t x0 = {T(0), T(0), T(0), T(0)}, x1 = {T(0), T(0), T(0), T(0)};
 #pragma unroll
-                for (int q = 0; q < 4; ++q) x = Mf::mma(w[q], c[t][q], x);
+                for (int q = 0; q < 4; ++q) x0 = Mf::mma(w0[q], c[t][0][q], x0);
+#pragma unroll
+                for (int q = 0; q < 4; ++q) c[t][1] = Mf::mma(-l10[q], x0[q], c[t][1]);
+#pragma unroll
+                for (int q = 0; q < 4; ++q) x1 = Mf::mma(w1[q], c[t][1][q], x1);
                 const int r = r0 + 16 * t + li;
-                T part = T(0);
+                T xv[4], yv[4];
 #pragma unroll
                 for (int q = 0; q < 4; ++q) {
-                    if (r < n && cq[q] < pw) A[r + (int64_t)(p0 + cq[q]) * lda] = x[q];
-                    part = fma_t(x[q], yq[q], part);
+                    if (r < n && cq[q] < pw0) A[r + (int64_t)(p0 + cq[q]) * lda] = x0[q];
+                    if (r < n && cq[q] < pw1) A[r + (int64_t)(p0 + 16 + cq[q]) * lda] = x1[q];
+                    xv[q] = x0[q];
+                    yv[q] = x1[q];
                 }
-                red[g * 16 + li] = part;
-                border();
-                const T sum = (red[li] + red[16 + li]) + (red[32 + li] + red[48 + li]);
+                T part[4];
+#pragma unroll
+                for (int q = 0; q < 4; ++q) part[q] = fma_t(xv[q], y0q[q], yv[q] * y1q[q]);
+                const T one[4] = {T(1), T(1), T(1), T(1)};
+                const T sum = row_dot(part, one);
                 if (g == 0 && r < n) bv[r] -= sum;
-                border();
             }
         }
     }
-    // Lᵀ x = y, panels from the last
-    for (int p0 = (n - 1) & ~15; p0 >= 0; p0 -= 16) {
-        const int pw = min(16, n - p0);
-        T lt[4];
-#pragma unroll
-        for (int q = 0; q < 4; ++q) lt[q] = ldz(A, lda, n, p0 + li, p0 + cq[q]);
-        T s[4] = {T(0), T(0), T(0), T(0)};
-        for (int r0 = p0 + 16; r0 < n; r0 += 128) {
-            T v[8][4], xr[8];
-#pragma unroll
-            for (int u = 0; u < 8; ++u)
-#pragma unroll
-                for (int q = 0; q < 4; ++q) v[u][q] = ldz(A, lda, n, r0 + 16 * u + li, p0 + cq[q]);
-#pragma unroll
-            for (int u = 0; u < 8; ++u) xr[u] = r0 + 16 * u + li < n ? bv[r0 + 16 * u + li] : T(0);
-#pragma unroll
-            for (int u = 0; u < 8; ++u)
-#pragma unroll
-                for (int q = 0; q < 4; ++q) s[q] = fma_t(v[u][q], xr[u], s[q]);
-        }
-#pragma unroll
-        for (int q = 0; q < 4; ++q) s[q] = row_sum16(s[q]);
-        if (li == 0) {
-#pragma unroll
-            for (int q = 0; q < 4; ++q) red[cq[q]] = s[q];
-        }
+    // Lᵀ x = y, panels from the last: x_p from the 32×32 diagonal block
+    // (x1 = L11ᵀ⁻¹ y1, x0 = L00ᵀ⁻¹ (y0 − L10ᵀ x1), 16-step back
+    // substitutions on the tiles' columns), then y[0:p0] −= L[p0:p0+32, 0:p0]ᵀ x_p
+    // streamed through the ring (16-lane row sums per column)
+    auto diag_bwd = [&](const T lt[4], int row0, int nv, T zin) {
 #pragma unroll
         for (int q = 0; q < 4; ++q) Ts[li * kMTL + cq[q]] = lt[q];
         border();
         T lc[16];
 #pragma unroll
         for (int k = 0; k < 16; ++k)
-            lc[k] = (li < pw && k < pw) ? (k >= li ? Ts[k * kMTL + li] : T(0)) : (k == li ? T(1) : T(0));
-        const T rdi = li < pw ? T(1) / Ts[li * kMTL + li] : T(1);
-        T z = li < pw ? bv[p0 + li] - red[li] : T(0);
+            lc[k] = (li < nv && k < nv) ? (k >= li ? Ts[k * kMTL + li] : T(0)) : (k == li ? T(1) : T(0));
+        const T rdi = li < nv ? T(1) / Ts[li * kMTL + li] : T(1);
+        T z = li < nv ? zin : T(0);
         border();
         DiagBwd<15, T>::run(lc, rdi, li, z);
-        if (g == 0 && li < pw) bv[p0 + li] = z;
+        if (g == 0 && li < nv) bv[row0 + li] = z;
+        border();
+        return li < nv ? z : T(0);
+    };
+    for (int p0 = ((n - 1) / kPW2) * kPW2; p0 >= 0; p0 -= kPW2) {
+        const int pw = min(kPW2, n - p0), pw0 = min(16, pw), pw1 = pw - pw0;
+        const int nslab = p0 / KB;
+        T lt00[4], lt10[4], lt11[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            lt00[q] = ldz(A, lda, n, p0 + li, p0 + cq[q]);
+            lt10[q] = ldz(A, lda, n, p0 + 16 + li, p0 + cq[q]);
+            lt11[q] = ldz(A, lda, n, p0 + 16 + li, p0 + 16 + cq[q]);
+        }
+        T xr0 = T(0), xr1 = T(0);
+        auto solve = [&] {
+            xr1 = diag_bwd(lt11, p0 + 16, pw1, bv[p0 + 16 + li]);
+            // S[j] = Σ_i L10[i][j] x1[i] for the columns c_q of this lane group
+            T s[4];
+#pragma unroll
+            for (int q = 0; q < 4; ++q) s[q] = row_sum16(lt10[q] * xr1);
+            if (li == 0) {
+#pragma unroll
+                for (int q = 0; q < 4; ++q) red[cq[q]] = s[q];
+            }
+            border();
+            const T z0 = bv[p0 + li] - red[li];
+            border();
+            xr0 = diag_bwd(lt00, p0, pw0, z0);
+        };
+        const int rb[2] = {p0, p0 + 16};
+        stream(std::integral_constant<int, 2>{}, rb, nslab, solve, [&](const T *st, int s) {
+#pragma unroll
+            for (int h = 0; h < KB / 4; ++h) {
+                const int o = (4 * h + g) * 16 + li;
+                const T v = row_sum16(fma_t(st[o], xr0, st[WL::kBlock + o] * xr1));
+                if (li == 0) bv[s * KB + 4 * h + g] -= v;
+            }
+        });
         border();
     }
     for (int e = lane; e < n; e += 64) b[e] = bv[e];
@@ -560,7 +651,7 @@ __global__ __launch_bounds__(kBThreads, 1) void batched_wg_kernel(
 // gw of GW takes rank gw in even rounds and GW − 1 − gw in odd ones (rounds
 // of GW ranks), which evens out the sum of sizes per wave.
 template <typename T>
-__global__ __launch_bounds__(kBThreads) void batched_wave_kernel(
+__global__ __launch_bounds__(kBThreads) __attribute__((amdgpu_waves_per_eu(2))) void batched_wave_kernel(
     int nsmall, const int *__restrict__ order, const int *__restrict__ nvec,
     T *const *__restrict__ Aptr, const int *__restrict__ ldav, T *const *__restrict__ bptr,
     int *__restrict__ info)
