@@ -1,11 +1,11 @@
-"""Autotuner for the batched uneven Cholesky+solve size bins.
+"""Autotuner for the batched uneven Cholesky+solve launch shape.
 
-Times one persistent launch over a synthetic batch (default: 10,000 SPD
+Times the batched call over a synthetic batch (default: 10,000 SPD
 matrices, n uniform in [8, 256], A = BᵀB + n·I) for every combination of
-  wave_max  — largest n handled by one wave (four matrices per workgroup);
-              larger ones take a whole workgroup; 0 sends every matrix to the
-              workgroup path, 256 none,
-  grid_mult — persistent workgroups per CU,
+  wave_max  — largest n handled by the wave kernel (one wave per matrix,
+              16×16 MFMA tiles); larger ones go to the workgroup kernel, one
+              matrix per workgroup; 0 sends every matrix there, 256 none,
+  grid_mult — persistent workgroups per CU (four waves each),
 per element type, and writes the fastest to gla/batched_tuning.json, which
 gla.BatchPlan reads.
 
@@ -26,8 +26,8 @@ if os.path.dirname(HERE) not in sys.path:
 
 import gla  # noqa: E402
 
-WAVE_MAX = (0, 32, 64, 128, 192, 256)
-GRID_MULT = (1, 2, 4)
+WAVE_MAX = (0, 128, 256)
+GRID_MULT = (1, 2, 3, 4, 6, 8, 12, 16)
 
 
 def make_batch(count: int, dtype, seed: int = 0):
