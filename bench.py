@@ -165,8 +165,8 @@ def lapack_line():
 
 def profiled_traffic(name):
     """HBM bytes per launch measured with rocprofv3 --pmc FETCH_SIZE and
-    WRITE_SIZE, as recorded in profiles/<name> (qr_hr_panel_8192x128.md,
-    syrk_14336_k1024.md); None if the summary is absent."""
+    WRITE_SIZE (FETCH_SIZE scaled by the plain-copy calibration), as recorded
+    in profiles/<name>; None if the summary is absent."""
     path = os.path.join(ROOT, "profiles", name)
     try:
         for line in open(path):
@@ -276,6 +276,12 @@ def batched_section(torch, gla, count=10000):
         out[key] = {"ms": t * 1e3, "tflops": flops / t / 1e12, "gbs": nbytes / t / 1e9,
                     "hbm_frac": nbytes / t / 8e12, "info_nonzero": bad,
                     "wave_max": plan.wave_max, "grid_mult": plan.grid_mult}
+        if key == "fp64":
+            # measured HBM bytes of the same call (rocprofv3 counters)
+            tb = profiled_traffic("batched_chol_10000_fp64.md")
+            if tb:
+                out[key]["traffic_bytes_profiled"] = tb
+                out[key]["traffic_tbs_at_this_time"] = tb / t / 1e12
     return out
 
 
@@ -472,6 +478,10 @@ def main():
                     help="only run the trailing SYRK of the Cholesky's first look-ahead step "
                          "(C 14336x14336 lower -= X X^T, K=1024) and a plain 64 MiB copy five "
                          "times each, for hardware-counter runs")
+    ap.add_argument("--profile-batched", action="store_true",
+                    help="only run the batched Cholesky+solve of the benchmark's FP64 batch "
+                         "(10,000 matrices, n uniform in [8, 256]) five times, each after a "
+                         "plain device copy restoring the matrices, for hardware-counter runs")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
@@ -495,6 +505,20 @@ def main():
             gla.rankUpdate_(gla.Hermitian(Cs, "L"), X, -1.0)
         torch.cuda.synchronize()
         print(json.dumps({"profile_syrk": f"{mm}x{mm} K={kk}", "launches": 5}))
+        return
+    if args.profile_batched:
+        sys.path.insert(0, os.path.join(ROOT, "genericlinearalgebra.jl_amd"))
+        from gla import autotune
+        sizes, pb = autotune.make_batch(10000, torch.float64, seed=0)
+        A0, b0 = pb.A.clone(), pb.b.clone()
+        plan = gla.BatchPlan(sizes, torch.float64)
+        for _ in range(5):
+            pb.A.copy_(A0)
+            pb.b.copy_(b0)
+            gla.chol_solve_batched_(plan, pb.A_ptrs, pb.lda, pb.b_ptrs, pb.info)
+        torch.cuda.synchronize()
+        print(json.dumps({"profile_batched": "10000 FP64, n in [8, 256]", "launches": 5,
+                          "grid_mult": plan.grid_mult, "matrix_bytes": int(pb.A.numel() * 8)}))
         return
     if args.profile_panel or args.profile_column_panel:
         P0 = torch.randn(NB_QR, M_QR, dtype=torch.float64, device=dev).t()
