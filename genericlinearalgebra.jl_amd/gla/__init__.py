@@ -506,11 +506,11 @@ _TUNING = _os.path.join(_os.path.dirname(_os.path.abspath(__file__)), "batched_t
 
 
 def batched_tuning(dtype) -> dict:
-    """Size-bin parameters for the batched solver: the autotuned table
+    """Launch shape of the batched solver: the autotuned table
     (gla/batched_tuning.json, written by gla.autotune on an MI355X) or
     defaults."""
     key = "float64" if dtype in (torch.float64, np.float64, "float64") else "float32"
-    cfg = {"wave_max": 256, "grid_mult": 2}
+    cfg = {"wave_max": 256, "grid_mult": 12}
     if _os.path.exists(_TUNING):
         with open(_TUNING) as f:
             cfg.update(_json.load(f).get(key, {}))
