@@ -85,7 +85,8 @@ __device__ __forceinline__ void tile_factor_rows(T a[16], T rd[16], int nvalid, 
     const int i = threadIdx.x & 15;
     int fneg = 16;
     TileFactorCols<0, T>::run(a, rd, i, fneg);
-    if (fneg < nvalid && (threadIdx.x & 63) == 0 && info) atomicCAS(info, 0, (int)(gbase + fneg + 1));
+    if (fneg < 16 && fneg < nvalid && (threadIdx.x & 63) == 0 && info)
+        atomicCAS(info, 0, (int)(gbase + fneg + 1));
 #pragma unroll
     for (int j = 0; j < 16; ++j) a[j] = j <= i ? a[j] : T(0);
 }
