@@ -74,32 +74,6 @@ __device__ __forceinline__ void border() { __builtin_amdgcn_fence(__ATOMIC_SEQ_C
 // Then Lᵀ x = y from the last panel: z = y_p − L_belowᵀ x_below (16-lane row
 // sums), and a 16-step back substitution on the columns of L_dd.
 
-template <int CTRL, typename T>
-__device__ __forceinline__ T dppmov(T v)
-{
-    if constexpr (sizeof(T) == 8) {
-        int2 u = __builtin_bit_cast(int2, v);
-        u.x = __builtin_amdgcn_update_dpp(0, u.x, CTRL, 0xF, 0xF, false);
-        u.y = __builtin_amdgcn_update_dpp(0, u.y, CTRL, 0xF, 0xF, false);
-        return __builtin_bit_cast(T, u);
-    } else {
-        return __builtin_bit_cast(
-            T, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), CTRL, 0xF, 0xF, false));
-    }
-}
-
-// Sum over the 16 lanes of each row, the same bits in every lane of the row:
-// pairs, quads, half rows, rows (each step adds two equal-order partials).
-template <typename T>
-__device__ __forceinline__ T row_sum16(T v)
-{
-    v += dppmov<0xB1>(v);    // quad_perm [1,0,3,2]
-    v += dppmov<0x4E>(v);    // quad_perm [2,3,0,1]
-    v += dppmov<0x141>(v);   // row_half_mirror
-    v += dppmov<0x140>(v);   // row_mirror
-    return v;
-}
-
 // A[r][c] for r, c < n, else 0; the load itself is unconditional (clamped
 // indices), so that a run of them stays in flight.
 // Global-memory view of a pointer read from device memory (the compiler

@@ -50,6 +50,66 @@ __device__ __forceinline__ T row_bcast(T v)
     return __builtin_amdgcn_update_dpp(v, v, 0x150 + K, 0xF, 0xF, true);
 }
 
+// v from the lane DPP control CTRL selects within its row (bound_ctrl off:
+// a source outside the row reads 0); 64-bit values move as two halves.
+template <int CTRL, typename T>
+__device__ __forceinline__ T dppmov(T v)
+{
+    if constexpr (sizeof(T) == 8) {
+        int2 u = __builtin_bit_cast(int2, v);
+        u.x = __builtin_amdgcn_update_dpp(0, u.x, CTRL, 0xF, 0xF, false);
+        u.y = __builtin_amdgcn_update_dpp(0, u.y, CTRL, 0xF, 0xF, false);
+        return __builtin_bit_cast(T, u);
+    } else {
+        return __builtin_bit_cast(
+            T, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), CTRL, 0xF, 0xF, false));
+    }
+}
+
+// Sum over the 16 lanes of each row, the same bits in every lane of the row:
+// pairs, quads, half rows, rows (each step adds two equal-order partials).
+template <typename T>
+__device__ __forceinline__ T row_sum16(T v)
+{
+    v += dppmov<0xB1>(v);    // quad_perm [1,0,3,2]
+    v += dppmov<0x4E>(v);    // quad_perm [2,3,0,1]
+    v += dppmov<0x141>(v);   // row_half_mirror
+    v += dppmov<0x140>(v);   // row_mirror
+    return v;
+}
+
+// Sum over the 4 lanes of each quad, the same bits in every lane of it.
+template <typename T>
+__device__ __forceinline__ T quad_sum(T v)
+{
+    v += dppmov<0xB1>(v);
+    v += dppmov<0x4E>(v);
+    return v;
+}
+
+// v of lane `src` (wave-uniform) in every lane.
+template <typename T>
+__device__ __forceinline__ T lane_value(T v, int src)
+{
+    if constexpr (sizeof(T) == 8) {
+        int2 u = __builtin_bit_cast(int2, v);
+        u.x = __builtin_amdgcn_readlane(u.x, src);
+        u.y = __builtin_amdgcn_readlane(u.y, src);
+        return __builtin_bit_cast(T, u);
+    } else {
+        return __builtin_bit_cast(T, __builtin_amdgcn_readlane(__builtin_bit_cast(int, v), src));
+    }
+}
+
+// Sum over the 64 lanes (all active), the same value in every lane and a
+// fixed order: 16-lane row sums by DPP, then the four rows.
+template <typename T>
+__device__ __forceinline__ T wave_sum_dpp(T v)
+{
+    v = row_sum16(v);
+    return (lane_value(v, 0) + lane_value(v, 16)) + (lane_value(v, 32) + lane_value(v, 48));
+}
+
 // Fused multiply-add in T's own precision.
 __device__ __forceinline__ double fma_t(double a, double b, double c) { return __builtin_fma(a, b, c); }
 __device__ __forceinline__ float fma_t(float a, float b, float c) { return __builtin_fmaf(a, b, c); }

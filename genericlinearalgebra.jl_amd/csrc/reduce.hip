@@ -46,12 +46,12 @@ constexpr int kPanel = 32;    // panel width (columns of V, W)
 // Scalars of the current column, written by tri_symv's first workgroup.
 enum { S_TAU = 0, S_XI = 1, S_BETA = 2, S_UNEXT = 3, S_COUNT = 8 };
 
+constexpr int kRPB = 64;      // rows per workgroup of the row kernels (4 lanes per row)
+
 template <typename T>
 __device__ __forceinline__ T wave_sum(T v)
 {
-#pragma unroll
-    for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o);
-    return v;
+    return wave_sum_dpp(v);
 }
 
 // Fixed-order sum of part[0..cnt) by one workgroup (every thread gets it).
@@ -89,7 +89,9 @@ __device__ __forceinline__ void reflector_scalars(T nrm2, T x1, T &xi, T &tau, T
 }
 
 // Column c brought up to date with the panel's reflectors 0..j−1; before
-// that, W[:, j−1] (holding u) becomes w = u − (ξ/2) v for rows ≥ c.
+// that, W[:, j−1] (holding u) becomes w = u − (ξ/2) v for rows ≥ c.  Four
+// lanes per row, lane k taking the reflectors l ≡ k (mod 4) (loads from
+// clamped indices, all issued at once), summed over the quad.
 template <typename T>
 __global__ __launch_bounds__(256) void tri_col_kernel(int64_t n, int64_t c, int j, T *A,
                                                       int64_t lda, T *V, T *W, int64_t ldv,
@@ -98,41 +100,52 @@ __global__ __launch_bounds__(256) void tri_col_kernel(int64_t n, int64_t c, int 
 {
     __shared__ T sh[8];
     __shared__ T wc[kPanel], vc[kPanel];
-    const int64_t i = c + (int64_t)blockIdx.x * 256 + threadIdx.x;
+    const int t = threadIdx.x, k = t & 3;
+    const int64_t i = c + (int64_t)blockIdx.x * kRPB + (t >> 2);
+    const bool in = i < n;
+    const int64_t ic = in ? i : n - 1;
     T wprev_c = T(0), half_xi = T(0);
     if (j > 0) {
         const T vu = block_sum_fixed(qpart, qcnt, sh);
         half_xi = T(0.5) * scal[S_TAU] * vu;              // ξ/2 = τ vᵀu / 2
         wprev_c = scal[S_UNEXT] - half_xi;                // row c: v_c = 1
     }
-    if (threadIdx.x < kPanel) {
-        const int l = threadIdx.x;
-        if (l < j) {
-            wc[l] = l == j - 1 ? wprev_c : W[c + (int64_t)l * ldv];
-            vc[l] = V[c + (int64_t)l * ldv];
-        }
+    if (t < kPanel) {
+        wc[t] = t < j ? (t == j - 1 ? wprev_c : W[c + (int64_t)t * ldv]) : T(0);
+        vc[t] = t < j ? V[c + (int64_t)t * ldv] : T(0);
     }
     __syncthreads();
-    T a = T(0);
-    if (i < n) {
-        a = A[i + c * lda];
-        if (j > 0) {
-            T *wi = W + i + (int64_t)(j - 1) * ldv;
-            const T vi = V[i + (int64_t)(j - 1) * ldv];
-            const T w = *wi - half_xi * vi;
-            *wi = w;
-            for (int l = 0; l < j; ++l) {
-                const T wil = l == j - 1 ? w : W[i + (int64_t)l * ldv];
-                a -= V[i + (int64_t)l * ldv] * wc[l] + wil * vc[l];
+    T part = T(0);
+    if (j > 0) {
+        T wv[kPanel / 4], vv[kPanel / 4];
+#pragma unroll
+        for (int u = 0; u < kPanel / 4; ++u) {
+            const int l = k + 4 * u, lc = l < j ? l : 0;
+            wv[u] = W[ic + (int64_t)lc * ldv];
+            vv[u] = V[ic + (int64_t)lc * ldv];
+        }
+#pragma unroll
+        for (int u = 0; u < kPanel / 4; ++u) {
+            const int l = k + 4 * u;
+            T wil = wv[u];
+            if (l == j - 1) {
+                wil -= half_xi * vv[u];
+                if (in) W[i + (int64_t)l * ldv] = wil;
             }
+            if (l < j) part += vv[u] * wc[l] + wil * vc[l];
         }
-        A[i + c * lda] = a;
     }
-    T s = (i < n && i > c) ? a * a : T(0);
+    part = quad_sum(part);
+    T a = T(0);
+    if (in) {
+        a = A[i + c * lda] - part;
+        if (k == 0) A[i + c * lda] = a;
+    }
+    T s = (in && i > c && k == 0) ? a * a : T(0);
     s = wave_sum(s);
-    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = s;
+    if ((t & 63) == 0) sh[t >> 6] = s;
     __syncthreads();
-    if (threadIdx.x == 0) npart[blockIdx.x] = sh[0] + sh[1] + sh[2] + sh[3];
+    if (t == 0) npart[blockIdx.x] = (sh[0] + sh[1]) + (sh[2] + sh[3]);
 }
 
 // After the panel's last column: W[:, j] (u) becomes w for rows ≥ c.
@@ -230,12 +243,7 @@ __global__ __launch_bounds__(256) void tri_symv_kernel(int64_t n, int64_t c, int
         for (int rr = 0; rr < 4; ++rr) srow[wv][4 * a + rr] = s[rr];
     // qc: reduce over a (lane bits 0..3)
 #pragma unroll
-    for (int cc = 0; cc < 4; ++cc) {
-        qc[cc] += __shfl_xor(qc[cc], 1);
-        qc[cc] += __shfl_xor(qc[cc], 2);
-        qc[cc] += __shfl_xor(qc[cc], 4);
-        qc[cc] += __shfl_xor(qc[cc], 8);
-    }
+    for (int cc = 0; cc < 4; ++cc) qc[cc] = row_sum16(qc[cc]);
     __syncthreads();
     if (I == J) {
         // one slot for the diagonal block: row sums plus the transposed part
@@ -276,7 +284,10 @@ __global__ __launch_bounds__(256) void tri_symv_kernel(int64_t n, int64_t c, int
 }
 
 // v into A (below the subdiagonal) and V, β on the subdiagonal;
-// u = τ (p − V y1 − W y2) into W[:, j]; partial vᵀu.
+// u = τ (p − V y1 − W y2) into W[:, j]; partial vᵀu.  Four lanes per row:
+// lane k sums the block contributions s ≡ k (mod 4) of p and the
+// reflectors l ≡ k (mod 4), then the quad sums; y1, y2 (Wᵀv, Vᵀv) are
+// summed over the blocks' partials by the whole workgroup, fixed order.
 template <typename T>
 __global__ __launch_bounds__(256) void tri_w_kernel(int64_t n, int64_t c, int j, T *A, int64_t lda,
                                                     T *V, T *W, int64_t ldv, const T *P, int nbk,
@@ -284,30 +295,62 @@ __global__ __launch_bounds__(256) void tri_w_kernel(int64_t n, int64_t c, int j,
 {
     __shared__ T sh[8];
     __shared__ T y1[kPanel], y2[kPanel];
+    __shared__ T ys[2][8][kPanel];
+    const int t = threadIdx.x, k = t & 3;
     const int64_t r0 = c + 1, m = n - r0;
     const T tau = scal[S_TAU], xi = scal[S_XI], beta = scal[S_BETA];
-    if (threadIdx.x < kPanel) {
-        const int l = threadIdx.x;
+    {
+        const int l = t & (kPanel - 1), pp = t / kPanel;   // 8 parts of the block sums
         T s1 = T(0), s2 = T(0);
         if (l < j)
-            for (int I = 0; I < nbk; ++I) {
+            for (int I = pp; I < nbk; I += 8) {
                 s1 += Y1[I * kPanel + l];
                 s2 += Y2[I * kPanel + l];
             }
-        y1[l] = s1;
-        y2[l] = s2;
+        ys[0][pp][l] = s1;
+        ys[1][pp][l] = s2;
     }
-    const int64_t r = r0 + (int64_t)blockIdx.x * 256 + threadIdx.x;
-    T v = T(0);
-    if (r < n) v = r == r0 ? T(1) : A[r + c * lda] / xi;
     __syncthreads();
+    if (t < kPanel) {
+        T s1 = T(0), s2 = T(0);
+#pragma unroll
+        for (int pp = 0; pp < 8; ++pp) {
+            s1 += ys[0][pp][t];
+            s2 += ys[1][pp][t];
+        }
+        y1[t] = s1;
+        y2[t] = s2;
+    }
+    __syncthreads();
+    const int64_t r = r0 + (int64_t)blockIdx.x * kRPB + (t >> 2);
+    const bool in = r < n;
+    const int64_t rc = in ? r : n - 1, rr = rc - r0;
+    T p0 = T(0), p1 = T(0);
+    int sblk = k;
+    for (; sblk + 4 < nbk; sblk += 8) {
+        p0 += P[(int64_t)sblk * m + rr];
+        p1 += P[(int64_t)(sblk + 4) * m + rr];
+    }
+    if (sblk < nbk) p0 += P[(int64_t)sblk * m + rr];
+    T p = p0 + p1;
+    if (j > 0) {
+        T wv[kPanel / 4], vv[kPanel / 4];
+#pragma unroll
+        for (int u = 0; u < kPanel / 4; ++u) {
+            const int l = k + 4 * u, lc = l < j ? l : 0;
+            vv[u] = V[rc + (int64_t)lc * ldv];
+            wv[u] = W[rc + (int64_t)lc * ldv];
+        }
+#pragma unroll
+        for (int u = 0; u < kPanel / 4; ++u) {
+            const int l = k + 4 * u;
+            if (l < j) p -= vv[u] * y1[l] + wv[u] * y2[l];
+        }
+    }
+    p = quad_sum(p);
     T vu = T(0);
-    if (r < n) {
-        const int64_t rr = r - r0;
-        T p = T(0);
-        for (int s = 0; s < nbk; ++s) p += P[(int64_t)s * m + rr];
-        for (int l = 0; l < j; ++l)
-            p -= V[r + (int64_t)l * ldv] * y1[l] + W[r + (int64_t)l * ldv] * y2[l];
+    if (in && k == 0) {
+        const T v = r == r0 ? T(1) : A[r + c * lda] / xi;
         const T u = tau * p;
         W[r + (int64_t)j * ldv] = u;
         V[r + (int64_t)j * ldv] = v;
@@ -316,9 +359,9 @@ __global__ __launch_bounds__(256) void tri_w_kernel(int64_t n, int64_t c, int j,
         vu = v * u;
     }
     vu = wave_sum(vu);
-    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = vu;
+    if ((t & 63) == 0) sh[t >> 6] = vu;
     __syncthreads();
-    if (threadIdx.x == 0) qpart[blockIdx.x] = sh[0] + sh[1] + sh[2] + sh[3];
+    if (t == 0) qpart[blockIdx.x] = (sh[0] + sh[1]) + (sh[2] + sh[3]);
 }
 
 // A[i, j] <-> A[n−1−i, n−1−j] over the whole matrix (J A J in place).
@@ -410,12 +453,7 @@ __global__ __launch_bounds__(256) void bd_gemvt_kernel(int64_t m, int64_t n, int
         qc[cc] = acc;
     }
 #pragma unroll
-    for (int cc = 0; cc < 4; ++cc) {
-        qc[cc] += __shfl_xor(qc[cc], 1);
-        qc[cc] += __shfl_xor(qc[cc], 2);
-        qc[cc] += __shfl_xor(qc[cc], 4);
-        qc[cc] += __shfl_xor(qc[cc], 8);
-    }
+    for (int cc = 0; cc < 4; ++cc) qc[cc] = row_sum16(qc[cc]);
     if (a == 0)
 #pragma unroll
         for (int cc = 0; cc < 4; ++cc) {
@@ -647,7 +685,7 @@ size_t symtri_scratch(int64_t n)
 {
     const int64_t ldv = (n + 3) / 4 * 4;
     const int64_t nbk = cdiv(n, kTB);
-    const int64_t g = cdiv(n, 256);
+    const int64_t g = cdiv(n, kRPB);
     return 2 * (size_t)ldv * kPanel + (size_t)nbk * n + 2 * (size_t)nbk * kPanel + 2 * g + 64;
 }
 
@@ -664,7 +702,7 @@ int symtri_lower_dev(hipStream_t s, int64_t n, T *A, int64_t lda, T *tau, T *ws)
     if (n <= 2) return GLA_OK;
     const int64_t ldv = (n + 3) / 4 * 4;
     const int64_t nbk_max = cdiv(n, kTB);
-    const int64_t gmax = cdiv(n, 256);
+    const int64_t gmax = cdiv(n, kRPB);
     T *V = ws, *W = V + ldv * kPanel, *P = W + ldv * kPanel, *Y1 = P + nbk_max * n,
       *Y2 = Y1 + nbk_max * kPanel, *npart = Y2 + nbk_max * kPanel, *qpart = npart + gmax,
       *scal = qpart + gmax;
@@ -674,8 +712,8 @@ int symtri_lower_dev(hipStream_t s, int64_t n, T *A, int64_t lda, T *tau, T *ws)
         for (int j = 0; j < w; ++j) {
             const int64_t c = k + j;
             const int64_t m = n - c - 1;
-            const unsigned g1 = (unsigned)cdiv(n - c, 256);
-            const int qcnt = (int)cdiv(n - c, 256);   // the previous column's tri_w grid
+            const unsigned g1 = (unsigned)cdiv(n - c, kRPB);
+            const int qcnt = (int)cdiv(n - c, kRPB);   // the previous column's tri_w grid
             hipLaunchKernelGGL((tri_col_kernel<T>), dim3(g1), dim3(256), 0, s, n, c, j, A, lda, V,
                                W, ldv, qpart, qcnt, scal, npart);
             GLA_LAUNCH_CHECK();
@@ -684,7 +722,7 @@ int symtri_lower_dev(hipStream_t s, int64_t n, T *A, int64_t lda, T *tau, T *ws)
                                dim3(256), 0, s, n, c, j, (const T *)A, lda, (const T *)V,
                                (const T *)W, ldv, (const T *)npart, (int)g1, P, Y1, Y2, scal, tau);
             GLA_LAUNCH_CHECK();
-            const unsigned g4 = (unsigned)cdiv(m, 256);
+            const unsigned g4 = (unsigned)cdiv(m, kRPB);
             hipLaunchKernelGGL((tri_w_kernel<T>), dim3(g4), dim3(256), 0, s, n, c, j, A, lda, V, W,
                                ldv, (const T *)P, (int)nbk, (const T *)Y1, (const T *)Y2, scal,
                                qpart);
@@ -693,7 +731,7 @@ int symtri_lower_dev(hipStream_t s, int64_t n, T *A, int64_t lda, T *tau, T *ws)
         // W's last column: u -> w over rows ≥ k + w
         const int64_t c1 = k + w;
         hipLaunchKernelGGL((tri_wfin_kernel<T>), dim3((unsigned)cdiv(n - c1, 256)), dim3(256), 0,
-                           s, n, c1, w - 1, V, W, ldv, (const T *)qpart, (int)cdiv(n - c1, 256),
+                           s, n, c1, w - 1, V, W, ldv, (const T *)qpart, (int)cdiv(n - c1, kRPB),
                            (const T *)scal);
         GLA_LAUNCH_CHECK();
         // Ã[c1:, c1:] −= V Wᵀ + W Vᵀ (lower triangle)
