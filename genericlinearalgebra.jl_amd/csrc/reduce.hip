@@ -476,8 +476,32 @@ __global__ __launch_bounds__(256) void bd_gemvt_kernel(int64_t m, int64_t n, int
     }
 }
 
-// Index t covers column c = i+1+t (y_c = τ (Ãᵀv − Y c1 − U c2)_c, row i
-// brought up to date, partial ‖row‖²) and row r = i+t (v into A and V).
+// Sum of the per-block partials part[b·kPanel + l] over b < nb for l < lmax,
+// by the whole workgroup in a fixed order (eight interleaved parts, then
+// the parts in order); the result in out[l] (zero for l ≥ lmax).
+template <typename T>
+__device__ __forceinline__ void panel_partials(const T *part, int64_t nb, int lmax, T *out,
+                                               T (*cs)[kPanel])
+{
+    const int t = threadIdx.x, l = t & (kPanel - 1), pp = t / kPanel;
+    T s = T(0);
+    if (l < lmax)
+        for (int64_t b = pp; b < nb; b += 8) s += part[b * kPanel + l];
+    cs[pp][l] = s;
+    __syncthreads();
+    if (t < kPanel) {
+        T r = T(0);
+#pragma unroll
+        for (int q = 0; q < 8; ++q) r += cs[q][t];
+        out[t] = r;
+    }
+}
+
+// Item it of the workgroup (64 items, four lanes each) covers column
+// c = i+1+it (y_c = τ (Ãᵀv − Y c1 − U c2)_c, row i brought up to date,
+// partial ‖row‖²; lane k takes the row-block partials rb ≡ k and the
+// reflectors l ≡ k (mod 4), summed over the quad) and row r = i+it (v into
+// A and V).
 template <typename T>
 __global__ __launch_bounds__(256) void bd_y_kernel(int64_t m, int64_t n, int64_t i, int j, T *A,
                                                    int64_t lda, T *V, const T *X, int64_t ldm,
@@ -487,49 +511,68 @@ __global__ __launch_bounds__(256) void bd_y_kernel(int64_t m, int64_t n, int64_t
 {
     __shared__ T sh[8];
     __shared__ T c1[kPanel], c2[kPanel], vi[kPanel], xi_[kPanel];
+    __shared__ T cs[8][kPanel];
+    const int t = threadIdx.x, k = t & 3;
     const T tau = scal[B_LTAU], xi = scal[B_LXI], beta = scal[B_LBETA];
-    if (threadIdx.x < kPanel) {
-        const int l = threadIdx.x;
-        T s1 = T(0), s2 = T(0);
-        if (l < j) {
-            for (int64_t rb = 0; rb < nrb; ++rb) {
-                s1 += Cv1[rb * kPanel + l];
-                s2 += Cv2[rb * kPanel + l];
-            }
-            vi[l] = V[i + (int64_t)l * ldm];
-            xi_[l] = X[i + (int64_t)l * ldm];
-        }
-        c1[l] = s1;
-        c2[l] = s2;
+    if (t < kPanel) {
+        vi[t] = t < j ? V[i + (int64_t)t * ldm] : T(0);
+        xi_[t] = t < j ? X[i + (int64_t)t * ldm] : T(0);
     }
+    panel_partials(Cv1, nrb, j, c1, cs);
     __syncthreads();
-    const int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    panel_partials(Cv2, nrb, j, c2, cs);
+    __syncthreads();
+    const int64_t it = (int64_t)blockIdx.x * kRPB + (t >> 2);
     const int64_t C = n - i - 1;
     T sq = T(0);
-    if (t < C) {
-        const int64_t c = i + 1 + t;
-        T p = T(0);
-        for (int64_t rb = 0; rb < nrb; ++rb) p += P[rb * C + t];
-        for (int l = 0; l < j; ++l)
-            p -= Y[c + (int64_t)l * ldn] * c1[l] + U[c + (int64_t)l * ldn] * c2[l];
-        const T y = tau * p;
-        Y[c + (int64_t)j * ldn] = y;
-        T r = A[i + c * lda] - y;
-        for (int l = 0; l < j; ++l)
-            r -= vi[l] * Y[c + (int64_t)l * ldn] + xi_[l] * U[c + (int64_t)l * ldn];
-        A[i + c * lda] = r;
-        sq = r * r;
+    if (C > 0) {
+        const bool in = it < C;
+        const int64_t tc = in ? it : C - 1, c = i + 1 + tc;
+        T p0 = T(0), p1 = T(0);
+        int64_t rb = k;
+        for (; rb + 4 < nrb; rb += 8) {
+            p0 += P[rb * C + tc];
+            p1 += P[(rb + 4) * C + tc];
+        }
+        if (rb < nrb) p0 += P[rb * C + tc];
+        T p = p0 + p1, rp = T(0);
+        if (j > 0) {
+            T yv[kPanel / 4], uv[kPanel / 4];
+#pragma unroll
+            for (int u = 0; u < kPanel / 4; ++u) {
+                const int l = k + 4 * u, lc = l < j ? l : 0;
+                yv[u] = Y[c + (int64_t)lc * ldn];
+                uv[u] = U[c + (int64_t)lc * ldn];
+            }
+#pragma unroll
+            for (int u = 0; u < kPanel / 4; ++u) {
+                const int l = k + 4 * u;
+                if (l < j) {
+                    p -= yv[u] * c1[l] + uv[u] * c2[l];
+                    rp += vi[l] * yv[u] + xi_[l] * uv[u];
+                }
+            }
+        }
+        p = quad_sum(p);
+        rp = quad_sum(rp);
+        if (in && k == 0) {
+            const T y = tau * p;
+            Y[c + (int64_t)j * ldn] = y;
+            const T r = A[i + c * lda] - y - rp;
+            A[i + c * lda] = r;
+            sq = r * r;
+        }
     }
-    const int64_t r = i + t;
-    if (r < m) {
+    const int64_t r = i + it;
+    if (k == 0 && r < m) {
         const T v = r == i ? T(1) : A[r + i * lda] / xi;
         V[r + (int64_t)j * ldm] = v;
         A[r + i * lda] = r == i ? beta : v;
     }
     sq = wave_sum(sq);
-    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = sq;
+    if ((t & 63) == 0) sh[t >> 6] = sq;
     __syncthreads();
-    if (threadIdx.x == 0) npart2[blockIdx.x] = sh[0] + sh[1] + sh[2] + sh[3];
+    if (t == 0) npart2[blockIdx.x] = (sh[0] + sh[1]) + (sh[2] + sh[3]);
 }
 
 // Right reflector scalars of row i, then P[cb·R + r'] = Σ over the 64-column
@@ -604,9 +647,11 @@ __global__ __launch_bounds__(256) void bd_gemvn_kernel(int64_t m, int64_t n, int
     }
 }
 
-// Index t covers row r = i+1+t (x_r = τ (Ã u − V c3 − X c4)_r; then, unless
-// `last`, column i+1 brought up to date with reflectors 0..j and its
-// partial ‖·‖²) and column c = i+1+t (u into A and U).
+// Item it (64 per workgroup, four lanes each) covers row r = i+1+it
+// (x_r = τ (Ã u − V c3 − X c4)_r; then, unless `last`, column i+1 brought
+// up to date with reflectors 0..j and its partial ‖·‖²; lane k takes the
+// column-block partials cb ≡ k and the reflectors l ≡ k (mod 4), summed over
+// the quad) and column c = i+1+it (u into A and U).
 template <typename T>
 __global__ __launch_bounds__(256) void bd_x_kernel(int64_t m, int64_t n, int64_t i, int j,
                                                    int last, T *A, int64_t lda, const T *V, T *X,
@@ -616,54 +661,71 @@ __global__ __launch_bounds__(256) void bd_x_kernel(int64_t m, int64_t n, int64_t
 {
     __shared__ T sh[8];
     __shared__ T c3[kPanel], c4[kPanel], yn[kPanel], un[kPanel];
+    __shared__ T cs[8][kPanel];
+    const int t = threadIdx.x, k = t & 3;
     const T tau = scal[B_RTAU], xi = scal[B_RXI], beta = scal[B_RBETA];
     const int64_t i1 = i + 1;
-    if (threadIdx.x < kPanel) {
-        const int l = threadIdx.x;
-        T s3 = T(0), s4 = T(0);
-        if (l <= j) {
-            for (int64_t cb = 0; cb < ncb; ++cb) {
-                s3 += Cu1[cb * kPanel + l];
-                if (l < j) s4 += Cu2[cb * kPanel + l];
-            }
-            yn[l] = Y[i1 + (int64_t)l * ldn];
-            un[l] = l == j ? T(1) : U[i1 + (int64_t)l * ldn];
-        }
-        c3[l] = s3;
-        c4[l] = s4;
+    if (t < kPanel) {
+        yn[t] = t <= j ? Y[i1 + (int64_t)t * ldn] : T(0);
+        un[t] = t <= j ? (t == j ? T(1) : U[i1 + (int64_t)t * ldn]) : T(0);
     }
+    panel_partials(Cu1, ncb, j + 1, c3, cs);
     __syncthreads();
-    const int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    panel_partials(Cu2, ncb, j, c4, cs);
+    __syncthreads();
+    const int64_t it = (int64_t)blockIdx.x * kRPB + (t >> 2);
     const int64_t R = m - i - 1;
     T sq = T(0);
-    if (t < R) {
-        const int64_t r = i1 + t;
-        T q = T(0);
-        for (int64_t cb = 0; cb < ncb; ++cb) q += P[cb * R + t];
-        for (int l = 0; l <= j; ++l) q -= V[r + (int64_t)l * ldm] * c3[l];
-        for (int l = 0; l < j; ++l) q -= X[r + (int64_t)l * ldm] * c4[l];
-        const T x = tau * q;
-        X[r + (int64_t)j * ldm] = x;
-        if (!last && i1 < n) {
-            T a = A[r + i1 * lda];
-            for (int l = 0; l <= j; ++l) {
-                const T xl = l == j ? x : X[r + (int64_t)l * ldm];
-                a -= V[r + (int64_t)l * ldm] * yn[l] + xl * un[l];
+    if (R > 0) {
+        const bool in = it < R;
+        const int64_t tr = in ? it : R - 1, r = i1 + tr;
+        T q0 = T(0), q1 = T(0);
+        int64_t cb = k;
+        for (; cb + 4 < ncb; cb += 8) {
+            q0 += P[cb * R + tr];
+            q1 += P[(cb + 4) * R + tr];
+        }
+        if (cb < ncb) q0 += P[cb * R + tr];
+        T q = q0 + q1, ap = T(0);
+        T vv[kPanel / 4], xv[kPanel / 4];
+#pragma unroll
+        for (int u = 0; u < kPanel / 4; ++u) {
+            const int l = k + 4 * u, lv = l <= j ? l : 0, lx = l < j ? l : 0;
+            vv[u] = V[r + (int64_t)lv * ldm];
+            xv[u] = X[r + (int64_t)lx * ldm];
+        }
+#pragma unroll
+        for (int u = 0; u < kPanel / 4; ++u) {
+            const int l = k + 4 * u;
+            if (l <= j) q -= vv[u] * c3[l];
+            if (l < j) {
+                q -= xv[u] * c4[l];
+                ap += vv[u] * yn[l] + xv[u] * un[l];
             }
-            A[r + i1 * lda] = a;
-            sq = a * a;
+        }
+        q = quad_sum(q);
+        ap = quad_sum(ap);
+        if (in && k == 0) {
+            const T x = tau * q;
+            X[r + (int64_t)j * ldm] = x;
+            if (!last && i1 < n) {
+                // the l = j term: V[r][j] y_n[j] + x u_n[j]
+                const T a = A[r + i1 * lda] - ap - (V[r + (int64_t)j * ldm] * yn[j] + x * un[j]);
+                A[r + i1 * lda] = a;
+                sq = a * a;
+            }
         }
     }
-    const int64_t c = i1 + t;
-    if (c < n) {
+    const int64_t c = i1 + it;
+    if (k == 0 && c < n) {
         const T u = c == i1 ? T(1) : A[i + c * lda] / xi;
         U[c + (int64_t)j * ldn] = u;
         A[i + c * lda] = c == i1 ? beta : u;
     }
     sq = wave_sum(sq);
-    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = sq;
+    if ((t & 63) == 0) sh[t >> 6] = sq;
     __syncthreads();
-    if (threadIdx.x == 0) npart[blockIdx.x] = sh[0] + sh[1] + sh[2] + sh[3];
+    if (t == 0) npart[blockIdx.x] = (sh[0] + sh[1]) + (sh[2] + sh[3]);
 }
 
 // B (n×m, ld ldb) = Aᵀ for A m×n.
@@ -770,7 +832,7 @@ size_t bidiag_scratch(int64_t m, int64_t n)
 {
     const int64_t M = std::max(m, n), N = std::min(m, n);   // the tall problem
     const int64_t ldm = (M + 3) / 4 * 4, ldn = (N + 3) / 4 * 4;
-    const int64_t nrb = cdiv(M, kTB), ncb = cdiv(N, kTB), g = cdiv(M, 256) + 1;
+    const int64_t nrb = cdiv(M, kTB), ncb = cdiv(N, kTB), g = cdiv(M, kRPB) + 1;
     const size_t pe = std::max((size_t)nrb * N, (size_t)ncb * M);
     const size_t tr = m < n ? (size_t)ldm * N : 0;   // Aᵀ for the wide case
     return 2 * (size_t)ldm * kPanel + 2 * (size_t)ldn * kPanel + pe +
@@ -783,7 +845,7 @@ int bidiag_tall_dev(hipStream_t s, int64_t m, int64_t n, T *A, int64_t lda, T *t
                     T *ws)
 {
     const int64_t ldm = (m + 3) / 4 * 4, ldn = (n + 3) / 4 * 4;
-    const int64_t nrb_max = cdiv(m, kTB), ncb_max = cdiv(n, kTB), gmax = cdiv(m, 256) + 1;
+    const int64_t nrb_max = cdiv(m, kTB), ncb_max = cdiv(n, kTB), gmax = cdiv(m, kRPB) + 1;
     T *V = ws, *X = V + ldm * kPanel, *Y = X + ldm * kPanel, *U = Y + ldn * kPanel;
     T *P = U + ldn * kPanel;
     T *Cv1 = P + std::max(nrb_max * n, ncb_max * m), *Cv2 = Cv1 + nrb_max * kPanel;
@@ -804,7 +866,7 @@ int bidiag_tall_dev(hipStream_t s, int64_t m, int64_t n, T *A, int64_t lda, T *t
                                (const T *)A, lda, (const T *)V, (const T *)X, ldm,
                                (const T *)npart, ncnt, ncb, P, Cv1, Cv2, scal, taul);
             GLA_LAUNCH_CHECK();
-            const unsigned g3 = (unsigned)cdiv(std::max(R, C), 256);
+            const unsigned g3 = (unsigned)cdiv(std::max(R, C), kRPB);
             hipLaunchKernelGGL((bd_y_kernel<T>), dim3(g3), dim3(256), 0, s, m, n, i, j, A, lda, V,
                                (const T *)X, ldm, Y, (const T *)U, ldn, (const T *)P,
                                C > 0 ? nrb : 0, (const T *)Cv1, (const T *)Cv2, (const T *)scal,
@@ -818,7 +880,7 @@ int bidiag_tall_dev(hipStream_t s, int64_t m, int64_t n, T *A, int64_t lda, T *t
                                (const T *)npart2, (int)g3, ncb1, P, Cu1, Cu2, scal, taur);
             GLA_LAUNCH_CHECK();
             const int last = j == w - 1;
-            const unsigned g5 = (unsigned)cdiv(std::max(R1, C), 256);
+            const unsigned g5 = (unsigned)cdiv(std::max(R1, C), kRPB);
             hipLaunchKernelGGL((bd_x_kernel<T>), dim3(g5), dim3(256), 0, s, m, n, i, j, last, A,
                                lda, (const T *)V, X, ldm, (const T *)Y, U, ldn, (const T *)P,
                                ncb1, (const T *)Cu1, (const T *)Cu2, (const T *)scal, npart);
