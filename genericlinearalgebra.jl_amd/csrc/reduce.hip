@@ -196,12 +196,16 @@ __global__ __launch_bounds__(256) void tri_symv_kernel(int64_t n, int64_t c, int
         tau_out[c] = tau;
     }
     const int t = threadIdx.x, a = t & 15, b = t >> 4, lane = t & 63, wv = t >> 6;
-    const int64_t R0 = r0 + I * kTB + 4 * a, C0 = r0 + J * kTB + 4 * b;
+    // this thread's rows RB + 16u (consecutive lanes read consecutive rows,
+    // so each load instruction reads whole cache lines) and columns C0 + u;
+    // the loads are unconditional (clamped indices), out-of-range values
+    // and the upper part of a diagonal block replaced by zero
+    const int64_t RB = r0 + I * kTB + a, C0 = r0 + J * kTB + 4 * b;
     T x[4][4];
     T vr[4], vcv[4];
 #pragma unroll
     for (int u = 0; u < 4; ++u) {
-        const int64_t rr = R0 + u, cc = C0 + u;
+        const int64_t rr = RB + 16 * u, cc = C0 + u;
         vr[u] = rr < n ? vval(A, lda, c, rr, xi) : T(0);
         vcv[u] = cc < n ? vval(A, lda, c, cc, xi) : T(0);
     }
@@ -209,8 +213,10 @@ __global__ __launch_bounds__(256) void tri_symv_kernel(int64_t n, int64_t c, int
     for (int cc = 0; cc < 4; ++cc)
 #pragma unroll
         for (int rr = 0; rr < 4; ++rr) {
-            const int64_t gi = R0 + rr, gj = C0 + cc;
-            x[cc][rr] = (gi < n && gj < n && gi >= gj) ? A[gi + gj * lda] : T(0);
+            const int64_t gi = RB + 16 * rr, gj = C0 + cc;
+            const int64_t gic = gi < n ? gi : n - 1, gjc = gj < n ? gj : n - 1;
+            const T v = A[gic + gjc * lda];
+            x[cc][rr] = (gi < n && gj < n && gi >= gj) ? v : T(0);
         }
     // row sums (A_IJ v_J) and column sums (A_IJᵀ v_I, strictly lower part
     // only on a diagonal block)
@@ -227,7 +233,7 @@ __global__ __launch_bounds__(256) void tri_symv_kernel(int64_t n, int64_t c, int
         T acc = T(0);
 #pragma unroll
         for (int rr = 0; rr < 4; ++rr) {
-            const bool use = I != J || (R0 + rr > C0 + cc);
+            const bool use = I != J || (RB + 16 * rr > C0 + cc);
             acc += use ? x[cc][rr] * vr[rr] : T(0);
         }
         qc[cc] = acc;
@@ -240,7 +246,7 @@ __global__ __launch_bounds__(256) void tri_symv_kernel(int64_t n, int64_t c, int
     }
     if (lane < 16)
 #pragma unroll
-        for (int rr = 0; rr < 4; ++rr) srow[wv][4 * a + rr] = s[rr];
+        for (int rr = 0; rr < 4; ++rr) srow[wv][a + 16 * rr] = s[rr];
     // qc: reduce over a (lane bits 0..3)
 #pragma unroll
     for (int cc = 0; cc < 4; ++cc) qc[cc] = row_sum16(qc[cc]);
@@ -433,23 +439,31 @@ __global__ __launch_bounds__(256) void bd_gemvt_kernel(int64_t m, int64_t n, int
     if (C <= 0) return;
     const int64_t rb = blockIdx.x / ncb, cb = blockIdx.x % ncb;
     const int t = threadIdx.x, a = t & 15, b = t >> 4, lane = t & 63, wv = t >> 6;
-    const int64_t R0 = i + rb * kTB + 4 * a, C0 = i + 1 + cb * kTB + 4 * b;
+    // rows RB + 16u (whole cache lines per load instruction), columns
+    // C0 + cc; unconditional loads from clamped indices
+    const int64_t RB = i + rb * kTB + a, C0 = i + 1 + cb * kTB + 4 * b;
     T vr[4];
 #pragma unroll
     for (int u = 0; u < 4; ++u) {
-        const int64_t r = R0 + u;
+        const int64_t r = RB + 16 * u;
         vr[u] = r < m ? (r == i ? T(1) : A[r + i * lda] / xi) : T(0);
     }
+    T x[4][4];
+#pragma unroll
+    for (int cc = 0; cc < 4; ++cc)
+#pragma unroll
+        for (int rr = 0; rr < 4; ++rr) {
+            const int64_t gr = RB + 16 * rr, gc = C0 + cc;
+            const int64_t grc = gr < m ? gr : m - 1, gcc = gc < n ? gc : n - 1;
+            const T v = A[grc + gcc * lda];
+            x[cc][rr] = (gr < m && gc < n) ? v : T(0);
+        }
     T qc[4];
 #pragma unroll
     for (int cc = 0; cc < 4; ++cc) {
         T acc = T(0);
-        const int64_t gc = C0 + cc;
 #pragma unroll
-        for (int rr = 0; rr < 4; ++rr) {
-            const int64_t gr = R0 + rr;
-            acc += (gr < m && gc < n) ? A[gr + gc * lda] * vr[rr] : T(0);
-        }
+        for (int rr = 0; rr < 4; ++rr) acc += x[cc][rr] * vr[rr];
         qc[cc] = acc;
     }
 #pragma unroll
@@ -599,23 +613,31 @@ __global__ __launch_bounds__(256) void bd_gemvn_kernel(int64_t m, int64_t n, int
     const int64_t R = m - i - 1;
     const int64_t rb = blockIdx.x / ncb, cb = blockIdx.x % ncb;
     const int t = threadIdx.x, a = t & 15, b = t >> 4, lane = t & 63, wv = t >> 6;
-    const int64_t R0 = i + 1 + rb * kTB + 4 * a, C0 = i + 1 + cb * kTB + 4 * b;
+    // rows RB + 16u (whole cache lines per load instruction), columns
+    // C0 + cc; unconditional loads from clamped indices
+    const int64_t RB = i + 1 + rb * kTB + a, C0 = i + 1 + cb * kTB + 4 * b;
     T uc[4];
 #pragma unroll
     for (int u = 0; u < 4; ++u) {
         const int64_t c = C0 + u;
         uc[u] = c < n ? (c == i + 1 ? T(1) : A[i + c * lda] / xi) : T(0);
     }
+    T x[4][4];
+#pragma unroll
+    for (int cc = 0; cc < 4; ++cc)
+#pragma unroll
+        for (int rr = 0; rr < 4; ++rr) {
+            const int64_t gr = RB + 16 * rr, gc = C0 + cc;
+            const int64_t grc = gr < m ? gr : m - 1, gcc = gc < n ? gc : n - 1;
+            const T v = A[grc + gcc * lda];
+            x[cc][rr] = (gr < m && gc < n) ? v : T(0);
+        }
     T s[4];
 #pragma unroll
     for (int rr = 0; rr < 4; ++rr) {
         T acc = T(0);
-        const int64_t gr = R0 + rr;
 #pragma unroll
-        for (int cc = 0; cc < 4; ++cc) {
-            const int64_t gc = C0 + cc;
-            acc += (gr < m && gc < n) ? A[gr + gc * lda] * uc[cc] : T(0);
-        }
+        for (int cc = 0; cc < 4; ++cc) acc += x[cc][rr] * uc[cc];
         s[rr] = acc;
     }
 #pragma unroll
@@ -625,7 +647,7 @@ __global__ __launch_bounds__(256) void bd_gemvn_kernel(int64_t m, int64_t n, int
     }
     if (lane < 16)
 #pragma unroll
-        for (int rr = 0; rr < 4; ++rr) srow[wv][4 * a + rr] = s[rr];
+        for (int rr = 0; rr < 4; ++rr) srow[wv][a + 16 * rr] = s[rr];
     __syncthreads();
     if (t < kTB) {
         const int64_t r = rb * kTB + t;
