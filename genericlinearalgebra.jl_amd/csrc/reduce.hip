@@ -262,17 +262,28 @@ __global__ __launch_bounds__(256) void tri_symv_kernel(int64_t n, int64_t c, int
             const int64_t r = I * kTB + t;
             if (r < m) P[J * m + r] = srow[0][t] + srow[1][t] + srow[2][t] + srow[3][t] + scol[t];
         }
-        // Wᵀv and Vᵀv over this block's rows
+        // Wᵀv and Vᵀv over this block's rows: wave wv takes l ≡ wv (mod 4),
+        // all loads issued before the sums
         const int64_t gr = r0 + I * kTB + lane;
+        const int64_t grc = gr < n ? gr : n - 1;
         const T vg = gr < n ? vval(A, lda, c, gr, xi) : T(0);
-        for (int l = wv; l < j; l += 4) {
-            T y1 = gr < n ? W[gr + (int64_t)l * ldv] * vg : T(0);
-            T y2 = gr < n ? V[gr + (int64_t)l * ldv] * vg : T(0);
-            y1 = wave_sum(y1);
-            y2 = wave_sum(y2);
-            if (lane == 0) {
-                Y1[I * kPanel + l] = y1;
-                Y2[I * kPanel + l] = y2;
+        T wl[kPanel / 4], vl[kPanel / 4];
+#pragma unroll
+        for (int u = 0; u < kPanel / 4; ++u) {
+            const int l = wv + 4 * u, lc = l < j ? l : 0;
+            wl[u] = W[grc + (int64_t)lc * ldv];
+            vl[u] = V[grc + (int64_t)lc * ldv];
+        }
+#pragma unroll
+        for (int u = 0; u < kPanel / 4; ++u) {
+            const int l = wv + 4 * u;
+            if (l < j) {
+                const T y1 = wave_sum(wl[u] * vg);
+                const T y2 = wave_sum(vl[u] * vg);
+                if (lane == 0) {
+                    Y1[I * kPanel + l] = y1;
+                    Y2[I * kPanel + l] = y2;
+                }
             }
         }
     } else {
@@ -476,15 +487,25 @@ __global__ __launch_bounds__(256) void bd_gemvt_kernel(int64_t m, int64_t n, int
         }
     if (cb == 0) {
         const int64_t gr = i + rb * kTB + lane;
+        const int64_t grc = gr < m ? gr : m - 1;
         const T vg = gr < m ? (gr == i ? T(1) : A[gr + i * lda] / xi) : T(0);
-        for (int l = wv; l < j; l += 4) {
-            T c1 = gr < m ? V[gr + (int64_t)l * ldm] * vg : T(0);
-            T c2 = gr < m ? X[gr + (int64_t)l * ldm] * vg : T(0);
-            c1 = wave_sum(c1);
-            c2 = wave_sum(c2);
-            if (lane == 0) {
-                Cv1[rb * kPanel + l] = c1;
-                Cv2[rb * kPanel + l] = c2;
+        T a1[kPanel / 4], a2[kPanel / 4];
+#pragma unroll
+        for (int u = 0; u < kPanel / 4; ++u) {
+            const int l = wv + 4 * u, lc = l < j ? l : 0;
+            a1[u] = V[grc + (int64_t)lc * ldm];
+            a2[u] = X[grc + (int64_t)lc * ldm];
+        }
+#pragma unroll
+        for (int u = 0; u < kPanel / 4; ++u) {
+            const int l = wv + 4 * u;
+            if (l < j) {
+                const T c1 = wave_sum(a1[u] * vg);
+                const T c2 = wave_sum(a2[u] * vg);
+                if (lane == 0) {
+                    Cv1[rb * kPanel + l] = c1;
+                    Cv2[rb * kPanel + l] = c2;
+                }
             }
         }
     }
@@ -655,15 +676,25 @@ __global__ __launch_bounds__(256) void bd_gemvn_kernel(int64_t m, int64_t n, int
     }
     if (rb == 0) {
         const int64_t gc = i + 1 + cb * kTB + lane;
+        const int64_t gcc = gc < n ? gc : n - 1;
         const T ug = gc < n ? (gc == i + 1 ? T(1) : A[i + gc * lda] / xi) : T(0);
-        for (int l = wv; l <= j; l += 4) {
-            T c1 = gc < n ? Y[gc + (int64_t)l * ldn] * ug : T(0);
-            T c2 = (gc < n && l < j) ? U[gc + (int64_t)l * ldn] * ug : T(0);
-            c1 = wave_sum(c1);
-            c2 = wave_sum(c2);
-            if (lane == 0) {
-                Cu1[cb * kPanel + l] = c1;
-                Cu2[cb * kPanel + l] = c2;
+        T a1[kPanel / 4], a2[kPanel / 4];
+#pragma unroll
+        for (int u = 0; u < kPanel / 4; ++u) {
+            const int l = wv + 4 * u, l1 = l <= j ? l : 0, l2 = l < j ? l : 0;
+            a1[u] = Y[gcc + (int64_t)l1 * ldn];
+            a2[u] = U[gcc + (int64_t)l2 * ldn];
+        }
+#pragma unroll
+        for (int u = 0; u < kPanel / 4; ++u) {
+            const int l = wv + 4 * u;
+            if (l <= j) {
+                const T c1 = wave_sum(a1[u] * ug);
+                const T c2 = wave_sum(l < j ? a2[u] * ug : T(0));
+                if (lane == 0) {
+                    Cu1[cb * kPanel + l] = c1;
+                    Cu2[cb * kPanel + l] = c2;
+                }
             }
         }
     }

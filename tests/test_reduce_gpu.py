@@ -79,14 +79,20 @@ def test_symtri_vs_oracle(n, uplo):
 
 @pytest.mark.parametrize("n", [50, 257])
 def test_symtri_float32(n):
+    """FP32 against the FP64 oracle of the same input (the truth at FP32's
+    scale): within 10·n·eps of FP32, or of twice the FP32 oracle's own
+    error — two FP32 evaluations that sum in different orders can each sit
+    near that bound, so the FP32 oracle is not the yardstick itself."""
     rng = np.random.default_rng(n)
     A = sym(n, rng, np.float32)
+    tol = 10 * n * eps_of(np.float32)
     for uplo in ("L", "U"):
         got, tau, _ = _symtri(A, uplo)
         ref, tref = O.symtri(A, uplo)
+        tru, ttru = O.symtri(A.astype(np.float64), uplo)
         tri = np.tril if uplo == "L" else np.triu
-        assert rel(tri(got), tri(ref)) <= 10 * n * eps_of(np.float32)
-        assert rel(tau, tref) <= 10 * n * eps_of(np.float32)
+        assert rel(tri(got), tri(tru)) <= max(tol, 2 * rel(tri(ref), tri(tru)))
+        assert rel(tau, ttru) <= max(tol, 2 * rel(tref, ttru))
 
 
 @pytest.mark.parametrize("uplo", ["L", "U"])
@@ -176,8 +182,12 @@ def test_bidiag_float32(m, n):
     A = rng.standard_normal((m, n)).astype(np.float32)
     got, tl, tr, _ = _bidiag(A)
     ref, rl, rr = O.bidiag(A)
+    tru, tlt, trt = O.bidiag(A.astype(np.float64))
     tol = 10 * max(m, n) * eps_of(np.float32)
-    assert rel(got, ref) <= tol and rel(tl, rl) <= tol and rel(tr, rr) <= tol
+    # against the FP64 oracle, within the bar or twice the FP32 oracle's error
+    assert rel(got, tru) <= max(tol, 2 * rel(ref, tru))
+    assert rel(tl, tlt) <= max(tol, 2 * rel(rl, tlt))
+    assert rel(tr, trt) <= max(tol, 2 * rel(rr, trt))
 
 
 def test_bidiag_reference_svdvals():
