@@ -476,9 +476,7 @@ constexpr int kZQ = 32;     // widest block column of one panel pass
 template <typename T>
 __device__ __forceinline__ T zwave_sum(T v)
 {
-#pragma unroll
-    for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o);
-    return v;
+    return wave_sum_dpp(v);
 }
 
 template <typename T>
@@ -553,20 +551,30 @@ __global__ __launch_bounds__(256) void zqr_reflect_kernel(int64_t m, int w, int 
             }
         }
     }
-    // dots with the remaining columns (before writing v: column j is not among them)
+    // dots with the remaining columns (before writing v: column j is not
+    // among them), eight columns' loads issued at a time
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-    for (int q = j + 1; q < w; ++q) {
-        T dr = T(0), di = T(0);
-        if (i < m) {
-            const T *b = A + 2 * (i + (int64_t)q * lda);
-            dr = vr * b[0] + vi * b[1];       // conj(v) · b
-            di = vr * b[1] - vi * b[0];
+    const int64_t ic = i < m ? i : m - 1;
+    for (int q0 = j + 1; q0 < w; q0 += 8) {
+        T br[8], bi[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            const int q = q0 + u < w ? q0 + u : q0;
+            const T *b = A + 2 * (ic + (int64_t)q * lda);
+            br[u] = b[0];
+            bi[u] = b[1];
         }
-        dr = zwave_sum(dr);
-        di = zwave_sum(di);
-        if (lane == 0) {
-            red[wv][q][0] = dr;
-            red[wv][q][1] = di;
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            const int q = q0 + u;
+            if (q < w) {
+                const T dr = zwave_sum(vr * br[u] + vi * bi[u]);   // conj(v) · b
+                const T di = zwave_sum(vr * bi[u] - vi * br[u]);
+                if (lane == 0) {
+                    red[wv][q][0] = dr;
+                    red[wv][q][1] = di;
+                }
+            }
         }
     }
     __syncthreads();
@@ -625,12 +633,29 @@ __global__ __launch_bounds__(256) void zqr_apply_kernel(int64_t m, int w, int j,
             vr = a[0];
             vi = a[1];
         }
-        for (int q = j + 1; q < w; ++q) {
-            T *b = A + 2 * (i + (int64_t)q * lda);
-            const T cr = dq[q][0], ci = dq[q][1];
-            b[0] -= vr * cr - vi * ci;
-            b[1] -= vr * ci + vi * cr;
-            if (q == j + 1 && i > j) s = b[0] * b[0] + b[1] * b[1];
+        // eight columns at a time: all loads, then the updates and stores
+        for (int q0 = j + 1; q0 < w; q0 += 8) {
+            T br[8], bi[8];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+                const int q = q0 + u < w ? q0 + u : q0;
+                const T *b = A + 2 * (i + (int64_t)q * lda);
+                br[u] = b[0];
+                bi[u] = b[1];
+            }
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+                const int q = q0 + u;
+                if (q < w) {
+                    const T cr = dq[q][0], ci = dq[q][1];
+                    const T nr = br[u] - (vr * cr - vi * ci);
+                    const T ni = bi[u] - (vr * ci + vi * cr);
+                    T *b = A + 2 * (i + (int64_t)q * lda);
+                    b[0] = nr;
+                    b[1] = ni;
+                    if (q == j + 1 && i > j) s = nr * nr + ni * ni;
+                }
+            }
         }
         if (i == j) {
             T *a = A + 2 * (i + (int64_t)j * lda);
