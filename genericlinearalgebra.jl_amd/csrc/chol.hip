@@ -614,15 +614,22 @@ __device__ __forceinline__ void stage_l_tiles(int nb, const T *__restrict__ L, i
     static_assert(kPanelThreads == 256, "one tile element per thread");
     const int nkb = (nb + 15) / 16;
     const int i = (threadIdx.x >> 4) & 15, j = threadIdx.x & 15;
+    // unconditional loads from clamped indices, so that all of them stay in
+    // flight; out-of-range values replaced afterwards
     T lv[NLT], iv[NBLK];
 #pragma unroll
     for (int q = 0; q < NLT; ++q) {
         const int kb = lt_kb(q), bj = lt_bj(q);
         const int gi = bj * 16 + i, gj = kb * 16 + j;
-        lv[q] = (gi < nb && gj < nb) ? L[gi + (int64_t)gj * ldl] : T(0);
+        const int gic = gi < nb ? gi : nb - 1, gjc = gj < nb ? gj : nb - 1;
+        const T v = L[gic + (int64_t)gjc * ldl];
+        lv[q] = (gi < nb && gj < nb) ? v : T(0);
     }
 #pragma unroll
-    for (int q = 0; q < NBLK; ++q) iv[q] = q < nkb ? LinvG[q * 256 + threadIdx.x] : T(0);
+    for (int q = 0; q < NBLK; ++q) {
+        const T v = LinvG[(q < nkb ? q : nkb - 1) * 256 + threadIdx.x];
+        iv[q] = q < nkb ? v : T(0);
+    }
 #pragma unroll
     for (int q = 0; q < NLT; ++q) Lt[q * TS + i * kXLD + j] = lv[q];
 #pragma unroll
@@ -721,7 +728,10 @@ __global__ __launch_bounds__(kPanelThreads) void chol_solve_kernel(int nb, const
         for (int r = 0; r < 4; ++r) {
             const int64_t gr = row0 + w * 16 + M::row(lane, r);
             const int gj = bj * 16 + lc;
-            acc[bj][r] = (gr < nrows && gj < nb) ? B[gr + (int64_t)gj * ldb] : T(0);
+            const int64_t grc = gr < nrows ? gr : nrows - 1;
+            const int gjc = gj < nb ? gj : nb - 1;
+            const T v = B[grc + (int64_t)gjc * ldb];
+            acc[bj][r] = (gr < nrows && gj < nb) ? v : T(0);
         }
     __syncthreads();
     solve_tiles<T>(nkb, acc, Lt, Li, Xs + w * TS);
@@ -874,7 +884,7 @@ template <typename T>
 int launch_solve(hipStream_t s, int nb, const T *L, int64_t ldl, const T *LinvG, T *B,
                  int64_t ldb, int64_t nrows)
 {
-    if (nrows <= 0) return GLA_OK;
+    if (nrows <= 0 || nb <= 0) return GLA_OK;
     hipLaunchKernelGGL((chol_solve_kernel<T>), dim3((unsigned)cdiv(nrows, kPanelRB)),
                        dim3(kPanelThreads), 0, s, nb, L, ldl, LinvG, B, ldb, nrows, 0, 0, 0);
     GLA_LAUNCH_CHECK();
@@ -898,7 +908,7 @@ int launch_solve_batched(hipStream_t s, int nb, const T *L, int64_t ldl, int64_t
                          const T *LinvG, int64_t sLinv, T *B, int64_t ldb, int64_t sB,
                          int64_t nrows, int64_t batch)
 {
-    if (nrows <= 0 || batch <= 0) return GLA_OK;
+    if (nrows <= 0 || batch <= 0 || nb <= 0) return GLA_OK;
     hipLaunchKernelGGL((chol_solve_kernel<T>), dim3((unsigned)cdiv(nrows, kPanelRB), (unsigned)batch),
                        dim3(kPanelThreads), 0, s, nb, L, ldl, LinvG, B, ldb, nrows, sL, sLinv, sB);
     GLA_LAUNCH_CHECK();
