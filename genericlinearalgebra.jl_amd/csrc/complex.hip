@@ -272,23 +272,25 @@ __global__ __launch_bounds__(256) void zchol_factor_kernel(int nb, T *__restrict
             Sr[zpk(i, j)] = a[0];
             Si[zpk(i, j)] = a[1];
         }
+    // two barriers per column: every thread takes the pivot and its square
+    // root, column k is scaled; after the second barrier the pivot's slot is
+    // written back (nobody reads it again) while the update runs
     for (int k = 0; k < nb; ++k) {
         __syncthreads();
         const T pr = Sr[zpk(k, k)], pi = Si[zpk(k, k)];
         T lr, li;
         csqrt_dev(pr, pi, lr, li);                                   // :5
-        __syncthreads();
-        if (t == 0) {
-            Sr[zpk(k, k)] = lr;
-            Si[zpk(k, k)] = li;
-            if (pr < T(0) && *info == 0) *info = (int)(info_base + k + 1);
-        }
         const T rs = T(1) / lr;                                      // :8 inv(real(A[1,1]))
         if (p == 0 && i > k && i < nb) {
             Sr[zpk(i, k)] *= rs;
             Si[zpk(i, k)] *= rs;
         }
         __syncthreads();
+        if (t == 0) {
+            Sr[zpk(k, k)] = lr;
+            Si[zpk(k, k)] = li;
+            if (pr < T(0) && *info == 0) *info = (int)(info_base + k + 1);
+        }
         if (i > k && i < nb) {                                       // :11 her!
             const T air = Sr[zpk(i, k)], aii = Si[zpk(i, k)];
             int j = k + 1;
@@ -309,27 +311,38 @@ __global__ __launch_bounds__(256) void zchol_factor_kernel(int nb, T *__restrict
             a[0] = Sr[zpk(i, j)];
             a[1] = Si[zpk(i, j)];
         }
-    // L⁻¹ column c by forward substitution (thread c), in LDS
-    if (t < kZB) {
-        const int c = t;
+    // L⁻¹ row by row, every column at once: X[r][c] = (δ_rc − Σ_{c≤l<r}
+    // L[r][l] X[l][c]) / L[r][r].  Lanes 4c..4c+3 take column c, lane q the
+    // terms l ≡ q (mod 4), summed over the quad; a quad lives in one wave, so
+    // each row waits only for the wave's own LDS writes of the rows above.
+    __syncthreads();
+    {
+        const int c = t >> 2, q = t & 3;
         for (int r = 0; r < kZB; ++r) {
-            T xr = T(0), xi = T(0);
-            if (r >= c && r < nb && c < nb) {
-                T sr = r == c ? T(1) : T(0), si = T(0);
-                for (int l = c; l < r; ++l) {
+            T sr = T(0), si = T(0);
+            if (r < nb && c < nb && r > c) {
+                for (int l = c + q; l < r; l += 4) {
                     const T lr_ = Sr[zpk(r, l)], li_ = Si[zpk(r, l)];
                     const T yr = Xr[l * kZB + c], yi = Xi[l * kZB + c];
-                    sr -= lr_ * yr - li_ * yi;
-                    si -= lr_ * yi + li_ * yr;
+                    sr += lr_ * yr - li_ * yi;
+                    si += lr_ * yi + li_ * yr;
                 }
-                // divide by L[r,r]
-                const T dr = Sr[zpk(r, r)], di = Si[zpk(r, r)];
-                const T den = dr * dr + di * di;
-                xr = (sr * dr + si * di) / den;
-                xi = (si * dr - sr * di) / den;
             }
-            Xr[r * kZB + c] = xr;
-            Xi[r * kZB + c] = xi;
+            sr = quad_sum(sr);
+            si = quad_sum(si);
+            if (q == 0) {
+                T xr = T(0), xi = T(0);
+                if (r >= c && r < nb && c < nb) {
+                    const T br = (r == c ? T(1) : T(0)) - sr, bi = -si;
+                    const T dr = Sr[zpk(r, r)], di = Si[zpk(r, r)];
+                    const T den = dr * dr + di * di;
+                    xr = (br * dr + bi * di) / den;
+                    xi = (bi * dr - br * di) / den;
+                }
+                Xr[r * kZB + c] = xr;
+                Xi[r * kZB + c] = xi;
+            }
+            __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
         }
     }
     __syncthreads();
