@@ -262,55 +262,87 @@ __global__ __launch_bounds__(256) void zchol_factor_kernel(int nb, T *__restrict
                                                            T *__restrict__ Linv, int *info,
                                                            int64_t info_base)
 {
-    __shared__ T Sr[kZB * (kZB + 1) / 2], Si[kZB * (kZB + 1) / 2];
-    __shared__ T Xr[kZB * kZB], Xi[kZB * kZB];
+    constexpr int XLD = kZB + 1;               // padded row of X
+    __shared__ T Lr[kZB * kZB], Li[kZB * kZB];   // L, dense row-major (for the inverse)
+    __shared__ T Xr[kZB * XLD], Xi[kZB * XLD];
+    __shared__ T Cr[2][kZB], Ci[2][kZB];       // column k, double-buffered
     const int t = threadIdx.x;
     const int i = t & 63, p = t >> 6;          // row, column class mod 4
-    if (i < nb)
-        for (int j = p; j <= i; j += 4) {
-            const T *a = A + 2 * (i + (int64_t)j * lda);
-            Sr[zpk(i, j)] = a[0];
-            Si[zpk(i, j)] = a[1];
-        }
-    // two barriers per column: every thread takes the pivot and its square
-    // root, column k is scaled; after the second barrier the pivot's slot is
-    // written back (nobody reads it again) while the update runs
+    // thread (i, p) holds row i at the columns j = p + 4m in registers
+    constexpr int NM = kZB / 4;
+    T ar[NM], ai[NM];
+#pragma unroll
+    for (int m = 0; m < NM; ++m) {
+        const int j = p + 4 * m;
+        const bool ok = i < nb && j <= i;
+        const int ic = i < nb ? i : nb - 1, jc = j <= ic ? j : ic;
+        const T *a = A + 2 * (ic + (int64_t)jc * lda);
+        const T vr = a[0], vi = a[1];
+        ar[m] = ok ? vr : T(0);
+        ai[m] = ok ? vi : T(0);
+    }
+    // per column k: the owners of column k publish it (one barrier, the
+    // buffer alternates), every thread takes the pivot's square root and
+    // the scaled column, and updates its own entries j in (k, i]
     for (int k = 0; k < nb; ++k) {
-        __syncthreads();
-        const T pr = Sr[zpk(k, k)], pi = Si[zpk(k, k)];
-        T lr, li;
-        csqrt_dev(pr, pi, lr, li);                                   // :5
-        const T rs = T(1) / lr;                                      // :8 inv(real(A[1,1]))
-        if (p == 0 && i > k && i < nb) {
-            Sr[zpk(i, k)] *= rs;
-            Si[zpk(i, k)] *= rs;
+        const int buf = k & 1, mk = k >> 2;
+        if (p == (k & 3)) {
+            T vr = ar[0], vi = ai[0];
+#pragma unroll
+            for (int m = 1; m < NM; ++m) {
+                vr = m == mk ? ar[m] : vr;
+                vi = m == mk ? ai[m] : vi;
+            }
+            Cr[buf][i] = vr;
+            Ci[buf][i] = vi;
         }
         __syncthreads();
-        if (t == 0) {
-            Sr[zpk(k, k)] = lr;
-            Si[zpk(k, k)] = li;
-            if (pr < T(0) && *info == 0) *info = (int)(info_base + k + 1);
+        const T pr = Cr[buf][k], pi = Ci[buf][k];
+        T lr, li, rs;
+        if (pi == T(0) && pr > T(0)) {
+            // a real positive pivot (every pivot of a Hermitian positive
+            // definite matrix): the complex square root is the real one
+            sqrt_and_rsqrt(pr, lr, rs);
+            li = T(0);
+        } else {
+            csqrt_dev(pr, pi, lr, li);                               // :5
+            rs = T(1) / lr;                                          // :8 inv(real(A[1,1]))
         }
-        if (i > k && i < nb) {                                       // :11 her!
-            const T air = Sr[zpk(i, k)], aii = Si[zpk(i, k)];
-            int j = k + 1;
-            while ((j & 3) != p) ++j;
-            for (; j <= i; j += 4) {
-                const T ajr = Sr[zpk(j, k)], aji = Si[zpk(j, k)];
-                // A[i,j] −= a_i conj(a_j)
-                Sr[zpk(i, j)] -= air * ajr + aii * aji;
-                if (j == i) Si[zpk(i, j)] = T(0);
-                else Si[zpk(i, j)] -= aii * ajr - air * aji;
+        const T lir = Cr[buf][i] * rs, lii = Ci[buf][i] * rs;        // L[i, k]
+        if (t == 0 && pr < T(0) && *info == 0) *info = (int)(info_base + k + 1);
+        const bool row_live = i > k && i < nb;
+#pragma unroll
+        for (int m = 0; m < NM; ++m) {                               // :11 her!
+            const int j = p + 4 * m;
+            if (row_live && j > k && j <= i) {
+                const T ljr = Cr[buf][j] * rs, lji = Ci[buf][j] * rs;
+                // A[i,j] −= L[i,k] conj(L[j,k])
+                ar[m] -= lir * ljr + lii * lji;
+                ai[m] = j == i ? T(0) : ai[m] - (lii * ljr - lir * lji);
+            }
+        }
+        if (p == (k & 3)) {                                          // column k itself
+            const T nr = i == k ? lr : lir, ni = i == k ? li : lii;
+            const bool set = i >= k && i < nb;
+#pragma unroll
+            for (int m = 0; m < NM; ++m) {
+                ar[m] = (set && m == mk) ? nr : ar[m];
+                ai[m] = (set && m == mk) ? ni : ai[m];
             }
         }
     }
-    __syncthreads();
-    if (i < nb)
-        for (int j = p; j <= i; j += 4) {
+    // L to A and to the packed LDS triangle (for the inverse)
+#pragma unroll
+    for (int m = 0; m < NM; ++m) {
+        const int j = p + 4 * m;
+        if (i < nb && j <= i) {
             T *a = A + 2 * (i + (int64_t)j * lda);
-            a[0] = Sr[zpk(i, j)];
-            a[1] = Si[zpk(i, j)];
+            a[0] = ar[m];
+            a[1] = ai[m];
+            Lr[i * kZB + j] = ar[m];
+            Li[i * kZB + j] = ai[m];
         }
+    }
     // L⁻¹ row by row, every column at once: X[r][c] = (δ_rc − Σ_{c≤l<r}
     // L[r][l] X[l][c]) / L[r][r].  Lanes 4c..4c+3 take column c, lane q the
     // terms l ≡ q (mod 4), summed over the quad; a quad lives in one wave, so
@@ -319,14 +351,18 @@ __global__ __launch_bounds__(256) void zchol_factor_kernel(int nb, T *__restrict
     {
         const int c = t >> 2, q = t & 3;
         for (int r = 0; r < kZB; ++r) {
+            // all 16 terms' LDS reads issued together (indices clamped into
+            // the lower triangle, out-of-range terms zeroed)
             T sr = T(0), si = T(0);
-            if (r < nb && c < nb && r > c) {
-                for (int l = c + q; l < r; l += 4) {
-                    const T lr_ = Sr[zpk(r, l)], li_ = Si[zpk(r, l)];
-                    const T yr = Xr[l * kZB + c], yi = Xi[l * kZB + c];
-                    sr += lr_ * yr - li_ * yi;
-                    si += lr_ * yi + li_ * yr;
-                }
+#pragma unroll
+            for (int u = 0; u < kZB / 4; ++u) {
+                const int l = c + q + 4 * u;
+                const bool ok = r < nb && c < nb && l < r;
+                const int lc = ok ? l : c, rr = ok ? r : lc;
+                const T lr_ = Lr[rr * kZB + lc], li_ = Li[rr * kZB + lc];
+                const T yr = Xr[lc * XLD + c], yi = Xi[lc * XLD + c];
+                sr += ok ? lr_ * yr - li_ * yi : T(0);
+                si += ok ? lr_ * yi + li_ * yr : T(0);
             }
             sr = quad_sum(sr);
             si = quad_sum(si);
@@ -334,13 +370,13 @@ __global__ __launch_bounds__(256) void zchol_factor_kernel(int nb, T *__restrict
                 T xr = T(0), xi = T(0);
                 if (r >= c && r < nb && c < nb) {
                     const T br = (r == c ? T(1) : T(0)) - sr, bi = -si;
-                    const T dr = Sr[zpk(r, r)], di = Si[zpk(r, r)];
+                    const T dr = Lr[r * kZB + r], di = Li[r * kZB + r];
                     const T den = dr * dr + di * di;
                     xr = (br * dr + bi * di) / den;
                     xi = (bi * dr - br * di) / den;
                 }
-                Xr[r * kZB + c] = xr;
-                Xi[r * kZB + c] = xi;
+                Xr[r * XLD + c] = xr;
+                Xi[r * XLD + c] = xi;
             }
             __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
         }
@@ -348,8 +384,8 @@ __global__ __launch_bounds__(256) void zchol_factor_kernel(int nb, T *__restrict
     __syncthreads();
     for (int e = t; e < kZB * kZB; e += 256) {
         const int r = e % kZB, c = e / kZB;
-        Linv[2 * e] = Xr[r * kZB + c];
-        Linv[2 * e + 1] = Xi[r * kZB + c];
+        Linv[2 * e] = Xr[r * XLD + c];
+        Linv[2 * e + 1] = Xi[r * XLD + c];
     }
 }
 
