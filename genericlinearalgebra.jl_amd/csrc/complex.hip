@@ -417,11 +417,17 @@ __global__ void zswap_conj_kernel(int64_t n, T *A, int64_t lda)
     }
 }
 
-// Right-looking complex Cholesky with nb-wide steps (nb ≤ 64).
+// Right-looking complex Cholesky with nb-wide steps (nb ≤ 64) and depth-1
+// look-ahead: each step's herk! updates the next block column first on the
+// caller's stream, and the rest of the trailing matrix on the handle's side
+// stream, where it overlaps the next step's diagonal factor and panel solve
+// (which touch only that block column); the next step's updates wait for it.
+// Every element of A22 takes the same K-ordered update as in one product.
 template <typename T>
-int zchol_dev(hipStream_t s, int64_t n, T *A, int64_t lda, int nb, T *Linv, T *W, int64_t ldw,
-              int *info)
+int zchol_dev(Handle *h, hipStream_t s, int64_t n, T *A, int64_t lda, int nb, T *Linv, T *W,
+              int64_t ldw, int *info)
 {
+    bool pending = false;   // a trailing update in flight on the side stream
     for (int64_t k = 0; k < n; k += nb) {
         const int kb = (int)std::min<int64_t>(nb, n - k);
         const int64_t below = n - k - kb;
@@ -437,10 +443,26 @@ int zchol_dev(hipStream_t s, int64_t n, T *A, int64_t lda, int nb, T *Linv, T *W
         // A21 = W (L₁₁⁻¹)ᴴ
         GLA_TRY((zgemm_launch<T, false, true, 0, false>(s, below, kb, kb, T(1), W, ldw, Linv, kZB,
                                                         T(0), A21, lda, 0)));
-        // A22 −= A21 A21ᴴ (lower, herk!)
-        GLA_TRY((zgemm_launch<T, false, true, 1, true>(s, below, below, kb, T(-1), A21, lda, A21,
-                                                       lda, T(1), A21 + 2 * kb * lda, lda, 0)));
+        // A22 −= A21 A21ᴴ (lower, herk!): the previous step's side update
+        // first, then the next block column here and the rest beside it
+        if (pending) GLA_HIP_TRY(hipStreamWaitEvent(s, h->ev_join, 0));
+        pending = false;
+        T *A22 = A21 + 2 * kb * lda;
+        const int64_t kb2 = std::min<int64_t>(nb, below);
+        GLA_TRY((zgemm_launch<T, false, true, 1, true>(s, below, kb2, kb, T(-1), A21, lda, A21,
+                                                       lda, T(1), A22, lda, 0)));
+        const int64_t rest = below - kb2;
+        if (rest > 0) {
+            GLA_HIP_TRY(hipEventRecord(h->ev_fork, s));
+            GLA_HIP_TRY(hipStreamWaitEvent(h->side, h->ev_fork, 0));
+            GLA_TRY((zgemm_launch<T, false, true, 1, true>(
+                h->side, rest, rest, kb, T(-1), A21 + 2 * kb2, lda, A21 + 2 * kb2, lda, T(1),
+                A22 + 2 * (kb2 + kb2 * lda), lda, 0)));
+            GLA_HIP_TRY(hipEventRecord(h->ev_join, h->side));
+            pending = true;
+        }
     }
+    if (pending) GLA_HIP_TRY(hipStreamWaitEvent(s, h->ev_join, 0));
     return GLA_OK;
 }
 
@@ -471,7 +493,7 @@ int zchol_entry(gla_handle_t hh, char uplo, int64_t n, void *Av, int64_t lda, in
             hipLaunchKernelGGL((zswap_conj_kernel<T>), tg, dim3(16, 16), 0, s, n, A, lda);
             GLA_LAUNCH_CHECK();
         }
-        GLA_TRY(zchol_dev<T>(s, n, A, lda, nb, Linv, W, ldw, h->dinfo));
+        GLA_TRY(zchol_dev<T>(h, s, n, A, lda, nb, Linv, W, ldw, h->dinfo));
         if (uplo == 'U') {
             hipLaunchKernelGGL((zswap_conj_kernel<T>), tg, dim3(16, 16), 0, s, n, A, lda);
             GLA_LAUNCH_CHECK();
