@@ -56,7 +56,6 @@ constexpr int kXLD = 17;        // padded row length of the LDS column blocks
     } while (0)
 
 
-__device__ __forceinline__ void wave_order() { __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront"); }
 
 // One wave holds a 16×16 tile with lane (i = l % 16, g = l / 16) owning row
 // i, columns g, g+4, g+8, g+12 (v[t] = tile[i][g + 4t]).  Row i of column c
@@ -569,13 +568,6 @@ __global__ __launch_bounds__(256) void tile_inverse_kernel(int64_t n, const T *_
     for (int t = 0; t < 4; ++t) LinvG[tile * 256 + i * 16 + g + 4 * t] = w[t];
 }
 
-// B (nrows×nb) <- B L⁻ᵀ for the lower nb×nb L (nb ≤ 128) whose 16×16
-// diagonal-tile inverses are in LinvG — rdiv!(A21, LowerTriangular(A11)'),
-// src/cholesky.jl:28,48.  Each workgroup takes 64 rows; wave w holds tile row
-// w of them (8 tiles).  L's 28 strictly-lower tiles and the 8 inverses are
-// read into LDS once; then for each column block kb the wave's tile kb
-// becomes X = B_kb L_kk⁻ᵀ and its tiles j > kb take B_j -= X L_(j,kb)ᵀ, all
-// on MFMA, with no further barrier (every wave touches only its own rows).
 // Tile q of the strictly-lower tiles (bj, kb), bj > kb, of an 8×8-tile
 // block, stored column block after column block: its kb and bj.
 __host__ __device__ constexpr int lt_kb(int q)
@@ -602,8 +594,10 @@ __host__ __device__ constexpr int lt_bj(int q)
 __device__ __forceinline__ int lt_index(int bj, int kb) { return 7 * kb - kb * (kb - 1) / 2 + (bj - kb - 1); }
 
 // Stages the strictly-lower 16×16 tiles of the nb×nb (nb ≤ 128) lower L and
-// the inverses of its diagonal tiles into LDS: thread t stages element
-// (t / 16, t % 16) of every tile, all loads issued before the first store.
+// the inverses of its diagonal tiles into LDS (row-major tiles, padded rows),
+// all loads issued before the first store.  L is read with 16 consecutive
+// rows of one column per 16 lanes (thread t: row t % 16, column t / 16 % 16
+// of every tile), the inverses as stored (row-major, 256 per tile).
 template <typename T>
 __device__ __forceinline__ void stage_l_tiles(int nb, const T *__restrict__ L, int64_t ldl,
                                               const T *__restrict__ LinvG, T *Lt, T *Li)
@@ -613,6 +607,7 @@ __device__ __forceinline__ void stage_l_tiles(int nb, const T *__restrict__ L, i
     constexpr int TS = 16 * kXLD;
     static_assert(kPanelThreads == 256, "one tile element per thread");
     const int nkb = (nb + 15) / 16;
+    const int ri = threadIdx.x & 15, rj = (threadIdx.x >> 4) & 15;
     const int i = (threadIdx.x >> 4) & 15, j = threadIdx.x & 15;
     // unconditional loads from clamped indices, so that all of them stay in
     // flight; out-of-range values replaced afterwards
@@ -620,7 +615,7 @@ __device__ __forceinline__ void stage_l_tiles(int nb, const T *__restrict__ L, i
 #pragma unroll
     for (int q = 0; q < NLT; ++q) {
         const int kb = lt_kb(q), bj = lt_bj(q);
-        const int gi = bj * 16 + i, gj = kb * 16 + j;
+        const int gi = bj * 16 + ri, gj = kb * 16 + rj;
         const int gic = gi < nb ? gi : nb - 1, gjc = gj < nb ? gj : nb - 1;
         const T v = L[gic + (int64_t)gjc * ldl];
         lv[q] = (gi < nb && gj < nb) ? v : T(0);
@@ -631,19 +626,23 @@ __device__ __forceinline__ void stage_l_tiles(int nb, const T *__restrict__ L, i
         iv[q] = q < nkb ? v : T(0);
     }
 #pragma unroll
-    for (int q = 0; q < NLT; ++q) Lt[q * TS + i * kXLD + j] = lv[q];
+    for (int q = 0; q < NLT; ++q) Lt[q * TS + ri * kXLD + rj] = lv[q];
 #pragma unroll
     for (int q = 0; q < NBLK; ++q) Li[q * TS + i * kXLD + j] = iv[q];
 }
 
-// acc (this wave's 16 rows × 128 columns as eight 16×16 tiles) <- acc L⁻ᵀ
-// with L's tiles and tile inverses staged in LDS: for each column block kb
-// the wave's tile kb becomes X = B_kb L_kk⁻ᵀ and its tiles j > kb take
-// B_j -= X L_(j,kb)ᵀ, all on MFMA, with no barrier (every wave touches
-// only its own rows; Xw is the wave's own LDS tile).
+// acc (this wave's 16 rows × 128 columns as eight 16×16 tiles, each held
+// TRANSPOSED: acc[bj] = B_bjᵀ in accumulator layout) <- acc L⁻ᵀ with L's
+// tiles and tile inverses staged in LDS.  The accumulator layout of Bᵀ is
+// the MFMA's B-operand layout with K index row(lane, kq) in step kq, so for
+// each column block kb the wave forms X_kbᵀ = L_kk⁻¹ B_kbᵀ and updates
+// B_jᵀ -= L_(j,kb) X_kbᵀ (j > kb) with the B operands straight from
+// registers and only L read from LDS: no LDS round trip and no barrier
+// (every wave touches only its own rows).  The column-block loop is
+// unrolled, so every tile index is static.
 template <typename T>
 __device__ __forceinline__ void solve_tiles(int nkb, typename PanelMfma<T>::acc_t acc[8],
-                                            const T *Lt, const T *Li, T *Xw)
+                                            const T *Lt, const T *Li)
 {
     typedef PanelMfma<T> M;
     typedef typename M::acc_t acc_t;
@@ -651,41 +650,26 @@ __device__ __forceinline__ void solve_tiles(int nkb, typename PanelMfma<T>::acc_
     constexpr int TS = 16 * kXLD;
     const int lane = threadIdx.x & 63;
     const int lc = lane & 15;
-    for (int kb = 0; kb < nkb; ++kb) {
+#pragma unroll
+    for (int kb = 0; kb < NBLK; ++kb) {
+        if (kb >= nkb) break;
         const T *Lik = Li + kb * TS;
-#pragma unroll
-        for (int bj = 0; bj < NBLK; ++bj)
-            if (bj == kb)
-#pragma unroll
-                for (int r = 0; r < 4; ++r) Xw[M::row(lane, r) * kXLD + lc] = acc[bj][r];
-        wave_order();   // the wave re-reads its own tile in operand layout
         acc_t x = {T(0), T(0), T(0), T(0)};
-        T a[4];
 #pragma unroll
-        for (int kq = 0; kq < 4; ++kq) {
-            const int kk = (lane >> 4) + 4 * kq;
-            x = M::mma(Xw[lc * kXLD + kk], Lik[lc * kXLD + kk], x);
-        }
-        wave_order();
+        for (int kq = 0; kq < 4; ++kq)
+            x = M::mma(Lik[lc * kXLD + M::row(lane, kq)], acc[kb][kq], x);
+        acc[kb] = x;
+        T nx[4];
 #pragma unroll
-        for (int r = 0; r < 4; ++r) Xw[M::row(lane, r) * kXLD + lc] = x[r];
+        for (int kq = 0; kq < 4; ++kq) nx[kq] = -x[kq];
 #pragma unroll
-        for (int bj = 0; bj < NBLK; ++bj)
-            if (bj == kb) acc[bj] = x;
-        wave_order();
-#pragma unroll
-        for (int kq = 0; kq < 4; ++kq) a[kq] = -Xw[lc * kXLD + (lane >> 4) + 4 * kq];
-#pragma unroll
-        for (int bj = 0; bj < NBLK; ++bj) {
-            if (bj <= kb || bj >= nkb) continue;
+        for (int bj = kb + 1; bj < NBLK; ++bj) {
+            if (bj >= nkb) break;
             const T *Lt_jk = Lt + lt_index(bj, kb) * TS;
 #pragma unroll
-            for (int kq = 0; kq < 4; ++kq) {
-                const T b = Lt_jk[lc * kXLD + (lane >> 4) + 4 * kq];
-                acc[bj] = M::mma(a[kq], b, acc[bj]);
-            }
+            for (int kq = 0; kq < 4; ++kq)
+                acc[bj] = M::mma(Lt_jk[lc * kXLD + M::row(lane, kq)], nx[kq], acc[bj]);
         }
-        wave_order();   // Xw is rewritten in the next step
     }
 }
 
@@ -713,34 +697,34 @@ __global__ __launch_bounds__(kPanelThreads) void chol_solve_kernel(int nb, const
     constexpr int TS = 16 * kXLD;                    // one padded tile
     __shared__ T Lt[NLT * TS];                       // tile (bj, kb), bj > kb
     __shared__ T Li[NBLK * TS];                      // inverses of the diagonal tiles
-    __shared__ T Xs[4 * TS];                         // one tile per wave
     const int lane = threadIdx.x & 63;
     const int w = threadIdx.x >> 6;
     const int lc = lane & 15;
     const int64_t row0 = (int64_t)blockIdx.x * kPanelRB;
     const int nkb = (nb + 15) / 16;
 
-    stage_l_tiles<T>(nb, L, ldl, LinvG, Lt, Li);
+    // the wave's rows, tiles transposed (lane: row lc, columns row(lane, r)),
+    // loaded first so that their latency overlaps L's
     acc_t acc[NBLK];
+    const int64_t gr = row0 + w * 16 + lc;
+    const int64_t grc = gr < nrows ? gr : nrows - 1;
 #pragma unroll
     for (int bj = 0; bj < NBLK; ++bj)
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
-            const int64_t gr = row0 + w * 16 + M::row(lane, r);
-            const int gj = bj * 16 + lc;
-            const int64_t grc = gr < nrows ? gr : nrows - 1;
+            const int gj = bj * 16 + M::row(lane, r);
             const int gjc = gj < nb ? gj : nb - 1;
             const T v = B[grc + (int64_t)gjc * ldb];
             acc[bj][r] = (gr < nrows && gj < nb) ? v : T(0);
         }
+    stage_l_tiles<T>(nb, L, ldl, LinvG, Lt, Li);
     __syncthreads();
-    solve_tiles<T>(nkb, acc, Lt, Li, Xs + w * TS);
+    solve_tiles<T>(nkb, acc, Lt, Li);
 #pragma unroll
     for (int bj = 0; bj < NBLK; ++bj)
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
-            const int64_t gr = row0 + w * 16 + M::row(lane, r);
-            const int gj = bj * 16 + lc;
+            const int gj = bj * 16 + M::row(lane, r);
             if (gr < nrows && gj < nb) B[gr + (int64_t)gj * ldb] = acc[bj][r];
         }
 }
@@ -771,25 +755,25 @@ __global__ __launch_bounds__(kPanelThreads) void chol_psolve_kernel(int nb, cons
     constexpr int TS = 16 * kXLD;
     __shared__ T Lt[NLT * TS];
     __shared__ T Li[NBLK * TS];
-    __shared__ T Xs[4 * TS];
     __shared__ T Gx[kPanelRB * kPsolveGLD];   // this workgroup's solved rows, one K slab
     __shared__ T Gl[kPanelNB * kPsolveGLD];   // rows of block c of L, the same K slab
     const int lane = threadIdx.x & 63;
     const int w = threadIdx.x >> 6;
     const int lc = lane & 15;
     const int64_t row0 = (int64_t)blockIdx.x * kPanelRB;
+    const int64_t gr = row0 + w * 16 + lc;   // this lane's row in every tile
     const int nc = nb / kPanelNB;
     for (int c = 0; c < nc; ++c) {
+        // block c of the wave's rows, tiles transposed as solve_tiles takes them
         acc_t acc[NBLK];
 #pragma unroll
         for (int bj = 0; bj < NBLK; ++bj)
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
-                const int64_t gr = row0 + w * 16 + M::row(lane, r);
-                const int64_t gj = (int64_t)c * kPanelNB + bj * 16 + lc;
+                const int64_t gj = (int64_t)c * kPanelNB + bj * 16 + M::row(lane, r);
                 acc[bj][r] = gr < nrows ? B[gr + gj * ldb] : T(0);
             }
-        // B_c -= X[:, 0:128c] L[128c : 128c+128, 0:128c]ᵀ, the global loads
+        // B_cᵀ -= L[128c : 128c+128, 0:128c] X[:, 0:128c]ᵀ, the global loads
         // of slab s + 1 in flight while slab s is multiplied
         const T *Lc = L + (int64_t)c * kPanelNB;
         constexpr int XE = kPanelRB * kPsolveKC / kPanelThreads;   // 8 per thread
@@ -830,10 +814,10 @@ __global__ __launch_bounds__(kPanelThreads) void chol_psolve_kernel(int nb, cons
 #pragma unroll
             for (int kq = 0; kq < kPsolveKC / 4; ++kq) {
                 const int kk = 4 * kq + (lane >> 4);
-                const T a = -Gx[(w * 16 + lc) * kPsolveGLD + kk];
+                const T xb = -Gx[(w * 16 + lc) * kPsolveGLD + kk];
 #pragma unroll
                 for (int bj = 0; bj < NBLK; ++bj)
-                    acc[bj] = M::mma(a, Gl[(bj * 16 + lc) * kPsolveGLD + kk], acc[bj]);
+                    acc[bj] = M::mma(Gl[(bj * 16 + lc) * kPsolveGLD + kk], xb, acc[bj]);
             }
         }
         // X_c = B_c L_cc⁻ᵀ
@@ -841,13 +825,12 @@ __global__ __launch_bounds__(kPanelThreads) void chol_psolve_kernel(int nb, cons
         stage_l_tiles<T>(kPanelNB, L + (int64_t)c * kPanelNB * (1 + ldl), ldl,
                          LinvG + (int64_t)c * NBLK * 256, Lt, Li);
         __syncthreads();
-        solve_tiles<T>(NBLK, acc, Lt, Li, Xs + w * TS);
+        solve_tiles<T>(NBLK, acc, Lt, Li);
 #pragma unroll
         for (int bj = 0; bj < NBLK; ++bj)
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
-                const int64_t gr = row0 + w * 16 + M::row(lane, r);
-                const int64_t gj = (int64_t)c * kPanelNB + bj * 16 + lc;
+                const int64_t gj = (int64_t)c * kPanelNB + bj * 16 + M::row(lane, r);
                 if (gr < nrows) B[gr + gj * ldb] = acc[bj][r];
             }
         // the stores of block c are read back by this workgroup's later
