@@ -4,13 +4,14 @@
 // Kernels
 //   chol_factor_kernel  One workgroup factors a diagonal block (≤128×128) in
 //                       place.  The block is held as 16×16 MFMA accumulator
-//                       tiles.  For each 16-wide column block, wave 0 factors
-//                       the 16×16 diagonal tile in registers column by column
-//                       as cholUnblocked! does (sqrt, reciprocal scale,
-//                       rank-1 update; lane shuffles broadcast the column)
-//                       and inverts it; the tiles below become X = A·L_kk⁻ᵀ
-//                       and the trailing tiles take the rank-16 update
-//                       C -= X_i X_jᵀ, both on MFMA.  The tile inverses are
+//                       tiles; wave 0 holds the diagonal tiles, waves 1–3
+//                       the rest.  For each 16-wide column block the tiles
+//                       below the diagonal become X = A·L_kk⁻ᵀ, then wave 0
+//                       factors the next diagonal tile in registers column
+//                       by column as cholUnblocked! does (sqrt, reciprocal
+//                       scale, rank-1 update; DPP broadcasts move the column)
+//                       and inverts it while waves 1–3 apply the rank-16
+//                       update C -= X_i X_jᵀ on MFMA.  The tile inverses are
 //                       kept for the panel solve.
 //   chol_solve_kernel   rdiv!(A21, LowerTriangular(A11)') (src/cholesky.jl:28,
 //                       48) for a ≤128-wide L11: each workgroup takes 64 rows
@@ -126,115 +127,197 @@ __device__ __forceinline__ void tile_inverse(const T v[4], const T rd[16], T w[4
     tile_inverse_rows<T>(li, rd, w);
 }
 
-// Tile (bi, bj) of the diagonal block held in slot q of wave w: slots 0..w
-// are tile row w (columns 0..w), slots w+1..8 tile row 7-w (columns
+// LDLᴴ tile factor: tile (bi, bj) of the diagonal block held in slot q of
+// wave w: slots 0..w are tile row w (columns 0..w), slots w+1..8 tile row 7-w (columns
 // 0..7-w).  Nine tiles per wave; tile rows are paired so that every 16-wide
 // step gives each wave nearly the same number of updates.
 __host__ __device__ constexpr int fslot_bi(int w, int q) { return q <= w ? w : 7 - w; }
 __host__ __device__ constexpr int fslot_bj(int w, int q) { return q <= w ? q : q - w - 1; }
 
+// Orders one wave's LDS accesses, for data exchanged between its lanes.
+__device__ __forceinline__ void wave_order() { __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront"); }
+
+// Strictly-lower tile rows of waves 1..3: {7, 2}, {6, 3}, {5, 4, 1} (nine,
+// nine and ten tiles), so that each 16-wide step spreads its panel tiles
+// and its trailing updates evenly.  Slot q -> tile (bi, bj); bi = -1 past
+// the wave's last tile.
+__host__ __device__ constexpr int oslot_bi(int w, int q)
+{
+    return w == 1 ? (q < 7 ? 7 : (q < 9 ? 2 : -1))
+         : w == 2 ? (q < 6 ? 6 : (q < 9 ? 3 : -1))
+                  : (q < 5 ? 5 : (q < 9 ? 4 : (q < 10 ? 1 : -1)));
+}
+__host__ __device__ constexpr int oslot_bj(int w, int q)
+{
+    return w == 1 ? (q < 7 ? q : q - 7) : w == 2 ? (q < 6 ? q : q - 6) : (q < 5 ? q : (q < 9 ? q - 5 : q - 9));
+}
+// Cholesky tile factor: slot q of wave W (wave 0: diagonal tile q).
+template <int W>
+__host__ __device__ constexpr int dslot_bi(int q) { return W == 0 ? q : oslot_bi(W, q); }
+template <int W>
+__host__ __device__ constexpr int dslot_bj(int q) { return W == 0 ? q : oslot_bj(W, q); }
+
+// d (a diagonal tile, transposed) -= X_j X_jᵀ for X_j = tile row j of X (LDS)
+template <typename T>
+__device__ __forceinline__ void diag_update(typename PanelMfma<T>::acc_t &d, const T *X, int j)
+{
+    typedef PanelMfma<T> M;
+    const int lane = threadIdx.x & 63;
+    const int lc = lane & 15;
+#pragma unroll
+    for (int kq = 0; kq < 4; ++kq) {
+        const int kk = (lane >> 4) + 4 * kq;
+        const T x = X[(j * 16 + lc) * kXLD + kk];
+        d = M::mma(-x, x, d);
+    }
+}
+
+// Wave 0: factors diagonal tile j (d, transposed) in row layout through the
+// wave's LDS tile D, inverts it, leaves L (transposed) in d, the inverse in Li
+// (row-major) and, when given, LinvG / Linv32.
+template <typename T>
+__device__ __forceinline__ void diag_factor(typename PanelMfma<T>::acc_t &d, int j, int nb,
+                                            int *info, int64_t info_base, T *D, T *Li,
+                                            T *__restrict__ LinvG, float *__restrict__ Linv32)
+{
+    typedef PanelMfma<T> M;
+    const int lane = threadIdx.x & 63;
+    const int lc = lane & 15;
+    const int i = lane & 15, g = lane >> 4;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) D[lc * kXLD + M::row(lane, r)] = d[r];
+    wave_order();
+    T a[16], w[4], rd[16];
+#pragma unroll
+    for (int c = 0; c < 16; ++c) a[c] = c <= i ? D[i * kXLD + c] : T(0);
+    tile_factor_rows<T>(a, rd, nb - j * 16, info, info_base + j * 16);
+    tile_inverse_rows<T>(a, rd, w);
+    wave_order();
+    if (g == 0)
+#pragma unroll
+        for (int c = 0; c < 16; ++c) D[i * kXLD + c] = a[c];
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+        Li[i * kXLD + g + 4 * t] = w[t];
+        if (LinvG) LinvG[j * 256 + i * 16 + g + 4 * t] = w[t];
+        if (Linv32) Linv32[j * 256 + i * 16 + g + 4 * t] = float(w[t]);
+    }
+    wave_order();
+#pragma unroll
+    for (int r = 0; r < 4; ++r) d[r] = D[lc * kXLD + M::row(lane, r)];
+}
+
+// One workgroup factors the nb×nb (nb ≤ 128) diagonal block as 16×16 MFMA
+// tiles held transposed (lane: row bi·16 + l%16 of the tile, columns
+// bj·16 + row(l, r)).  Wave 0 holds the eight diagonal tiles, waves 1–3 the
+// strictly-lower ones.  Each 16-wide step kb takes two barriers: (A) waves
+// 1–3 form the panel tiles X = A_(i,kb) L_kk⁻ᵀ while wave 0 applies the
+// previous step's update to the diagonal tiles beyond kb + 1; (B) wave 0
+// updates, factors and inverts diagonal tile kb + 1 while waves 1–3 apply
+// the rank-16 update to theirs, so the serial tile factor overlaps the
+// trailing update.  Every product is formed from the same operands in the
+// same order as the right-looking sweep cholBlocked! describes
+// (src/cholesky.jl:17-35).
 // TS: storage type of A (T, or FP32 for the mixed-precision driver, which
 // factors its FP32 diagonal tiles in FP64); Linv32: optional FP32 copy of
 // the tile inverses.
 template <typename T, int W, typename TS = T>
 __device__ __forceinline__ void factor_body(int nb, TS *__restrict__ A, int64_t lda, int *info,
-                                            int64_t info_base, T *__restrict__ LinvG, T *Xb,
-                                            T *Db, T *Lib, float *__restrict__ Linv32 = nullptr)
+                                             int64_t info_base, T *__restrict__ LinvG, T *Xb,
+                                             T *Db, T *Lib, float *__restrict__ Linv32 = nullptr)
 {
     typedef PanelMfma<T> M;
     typedef typename M::acc_t acc_t;
-    constexpr int NQ = 9;
+    constexpr int NQ = W == 0 ? 8 : 10;
+    constexpr int NBLK = kPanelNB / 16;
+    constexpr int XS = kPanelNB * kXLD, LS = 16 * kXLD;
     const int lane = threadIdx.x & 63;
     const int lc = lane & 15;
+    const int nkb = (nb + 15) / 16;
 
+    // tiles transposed: lane holds row bi*16 + lc at columns bj*16 + row(lane, r)
     acc_t acc[NQ];
 #pragma unroll
     for (int q = 0; q < NQ; ++q) {
-        const int bi = fslot_bi(W, q), bj = fslot_bj(W, q);
+        const int bi = dslot_bi<W>(q), bj = dslot_bj<W>(q);
+        if (bi < 0) continue;
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
-            const int gi = bi * 16 + M::row(lane, r), gj = bj * 16 + lc;
+            const int gi = bi * 16 + lc, gj = bj * 16 + M::row(lane, r);
             T v = T(0);
             if (gj <= gi) v = gi < nb ? T(A[gi + (int64_t)gj * lda]) : (gi == gj ? T(1) : T(0));
             acc[q][r] = v;
         }
     }
-
-    const int nkb = (nb + 15) / 16;
-    for (int kb = 0; kb < nkb; ++kb) {
-        T *X = Xb + (kb & 1) * (kPanelNB * kXLD);
-        T *D = Db + (kb & 1) * (16 * kXLD);
-        T *Li = Lib + (kb & 1) * (16 * kXLD);
-        // (1) column block kb to LDS: the diagonal tile to D, the tiles below to X
+    if constexpr (W == 0) diag_factor<T>(acc[0], 0, nb, info, info_base, Db, Lib, LinvG, Linv32);
+    __syncthreads();
 #pragma unroll
-        for (int q = 0; q < NQ; ++q) {
-            const int bi = fslot_bi(W, q), bj = fslot_bj(W, q);
-            if (bj != kb) continue;
-            T *dst = bi == kb ? D : X + bi * 16 * kXLD;
+    for (int kb = 0; kb < NBLK; ++kb) {
+        if (kb >= nkb) break;
+        T *X = Xb + (kb & 1) * XS;
+        // (A) waves 1-3: the panel tiles X = A_(i,kb) L_kk⁻ᵀ (rdiv!), as
+        //     Xᵀ = L_kk⁻¹ A_(i,kb)ᵀ with the B operand from registers, then to
+        //     LDS (row-major); wave 0: the diagonal tiles beyond kb + 1 take
+        //     the previous step's update
+        if constexpr (W == 0) {
+            if (kb > 0) {
+                const T *Xp = Xb + ((kb - 1) & 1) * XS;
 #pragma unroll
-            for (int r = 0; r < 4; ++r) dst[M::row(lane, r) * kXLD + lc] = acc[q][r];
-        }
-        __syncthreads();
-        // (2) wave 0 factors the diagonal tile and inverts it
-        if (W == 0) {
-            const int i = lane & 15, g = lane >> 4;
-            T a[16], w[4], rd[16];
+                for (int j = kb + 1; j < NBLK; ++j) {
+                    if (j >= nkb) break;
+                    diag_update<T>(acc[j], Xp, j);
+                }
+            }
+        } else {
+            const T *Lik = Lib + (kb & 1) * LS;
 #pragma unroll
-            for (int j = 0; j < 16; ++j) a[j] = j <= i ? D[i * kXLD + j] : T(0);
-            tile_factor_rows<T>(a, rd, nb - kb * 16, info, info_base + kb * 16);
-            tile_inverse_rows<T>(a, rd, w);
-            if (g == 0)
+            for (int q = 0; q < NQ; ++q) {
+                const int bi = dslot_bi<W>(q), bj = dslot_bj<W>(q);
+                if (bi < 0 || bj != kb) continue;
+                if (bi >= nkb) continue;
+                acc_t x = {T(0), T(0), T(0), T(0)};
 #pragma unroll
-                for (int j = 0; j < 16; ++j) D[i * kXLD + j] = a[j];
+                for (int kq = 0; kq < 4; ++kq)
+                    x = M::mma(Lik[lc * kXLD + M::row(lane, kq)], acc[q][kq], x);
+                acc[q] = x;
 #pragma unroll
-            for (int t = 0; t < 4; ++t) {
-                Li[i * kXLD + g + 4 * t] = w[t];
-                if (LinvG) LinvG[kb * 256 + i * 16 + g + 4 * t] = w[t];
-                if (Linv32) Linv32[kb * 256 + i * 16 + g + 4 * t] = float(w[t]);
+                for (int r = 0; r < 4; ++r) X[(bi * 16 + lc) * kXLD + M::row(lane, r)] = x[r];
             }
         }
         __syncthreads();
-        // (3) the diagonal tile takes L; tiles below: X = A L⁻ᵀ on MFMA (rdiv!)
-#pragma unroll
-        for (int q = 0; q < NQ; ++q) {
-            const int bi = fslot_bi(W, q), bj = fslot_bj(W, q);
-            if (bj != kb) continue;
-            if (bi == kb) {
-#pragma unroll
-                for (int r = 0; r < 4; ++r) acc[q][r] = D[M::row(lane, r) * kXLD + lc];
-                continue;
+        // (B) wave 0: the next diagonal tile takes this step's update and is
+        //     factored and inverted; waves 1-3: rank-16 trailing update of
+        //     their tiles, C_ijᵀ -= X_j X_iᵀ (rankUpdate!)
+        if constexpr (W == 0) {
+            if (kb + 1 < nkb) {
+                diag_update<T>(acc[kb + 1], X, kb + 1);
+                diag_factor<T>(acc[kb + 1], kb + 1, nb, info, info_base, Db,
+                               Lib + ((kb + 1) & 1) * LS, LinvG, Linv32);
             }
-            acc_t x = {T(0), T(0), T(0), T(0)};
+        } else {
 #pragma unroll
-            for (int kq = 0; kq < 4; ++kq) {
-                const int kk = (lane >> 4) + 4 * kq;
-                x = M::mma(X[(bi * 16 + lc) * kXLD + kk], Li[lc * kXLD + kk], x);
+            for (int q = 0; q < NQ; ++q) {
+                const int bi = dslot_bi<W>(q), bj = dslot_bj<W>(q);
+                if (bi < 0 || bj <= kb) continue;
+                if (bi >= nkb) continue;
+#pragma unroll
+                for (int kq = 0; kq < 4; ++kq) {
+                    const int kk = (lane >> 4) + 4 * kq;
+                    const T a = -X[(bj * 16 + lc) * kXLD + kk];
+                    const T b = X[(bi * 16 + lc) * kXLD + kk];
+                    acc[q] = M::mma(a, b, acc[q]);
+                }
             }
-            acc[q] = x;
-#pragma unroll
-            for (int r = 0; r < 4; ++r) X[(bi * 16 + M::row(lane, r)) * kXLD + lc] = x[r];
         }
         __syncthreads();
-        // (4) rank-16 trailing update C_ij -= X_i X_jᵀ (rankUpdate!)
-#pragma unroll
-        for (int q = 0; q < NQ; ++q) {
-            const int bi = fslot_bi(W, q), bj = fslot_bj(W, q);
-            if (bj <= kb) continue;
-#pragma unroll
-            for (int kq = 0; kq < 4; ++kq) {
-                const int kk = (lane >> 4) + 4 * kq;
-                const T a = -X[(bi * 16 + lc) * kXLD + kk];
-                const T b = X[(bj * 16 + lc) * kXLD + kk];
-                acc[q] = M::mma(a, b, acc[q]);
-            }
-        }
     }
 #pragma unroll
     for (int q = 0; q < NQ; ++q) {
-        const int bi = fslot_bi(W, q), bj = fslot_bj(W, q);
+        const int bi = dslot_bi<W>(q), bj = dslot_bj<W>(q);
+        if (bi < 0) continue;
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
-            const int gi = bi * 16 + M::row(lane, r), gj = bj * 16 + lc;
+            const int gi = bi * 16 + lc, gj = bj * 16 + M::row(lane, r);
             if (gi < nb && gj <= gi) A[gi + (int64_t)gj * lda] = TS(acc[q][r]);
         }
     }
