@@ -127,13 +127,6 @@ __device__ __forceinline__ void tile_inverse(const T v[4], const T rd[16], T w[4
     tile_inverse_rows<T>(li, rd, w);
 }
 
-// LDLᴴ tile factor: tile (bi, bj) of the diagonal block held in slot q of
-// wave w: slots 0..w are tile row w (columns 0..w), slots w+1..8 tile row 7-w (columns
-// 0..7-w).  Nine tiles per wave; tile rows are paired so that every 16-wide
-// step gives each wave nearly the same number of updates.
-__host__ __device__ constexpr int fslot_bi(int w, int q) { return q <= w ? w : 7 - w; }
-__host__ __device__ constexpr int fslot_bj(int w, int q) { return q <= w ? q : q - w - 1; }
-
 // Orders one wave's LDS accesses, for data exchanged between its lanes.
 __device__ __forceinline__ void wave_order() { __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront"); }
 
@@ -396,14 +389,74 @@ __device__ __forceinline__ void tile_inverse_unit_rows(const T li[16], T w[4])
     TileInvSteps<0, T>::run(m, w);
 }
 
-// The LDLᴴ of the nb×nb (nb ≤ 128) diagonal block at A, held as 16×16
-// accumulator tiles like factor_body: per 16-column step wave 0 factors the
-// diagonal tile (LdltTileCols) and inverts its unit factor; the tiles below
-// take W = A L_kk⁻ᵀ on MFMA (rdiv! by UnitLowerTriangular, :86) and
-// L = W D_kk⁻¹ (rdiv! by Diagonal, :87, a division); the trailing tiles
-// take C −= W Lᵀ (:88-95) on MFMA.  Writes D on the diagonal and L below
-// it, and the inverses of the unit 16×16 diagonal tiles (row-major, 256
-// elements each) to LinvG.
+// Wave 0: LDLᴴ of diagonal tile j (d, transposed) in row layout through
+// the wave's LDS tile D (LdltTileCols), the inverse of its unit factor to
+// Li (row-major) and LinvG, the pivots to dv; d keeps δ on the diagonal and
+// L below it (transposed).
+template <typename T>
+__device__ __forceinline__ void ldlt_diag_factor(typename PanelMfma<T>::acc_t &d, int j, T *D,
+                                                 T *Li, T *dv, T *__restrict__ LinvG)
+{
+    typedef PanelMfma<T> M;
+    const int lane = threadIdx.x & 63;
+    const int lc = lane & 15;
+    const int i = lane & 15, g = lane >> 4;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) D[lc * kXLD + M::row(lane, r)] = d[r];
+    wave_order();
+    T a[16], w[4], dd[16];
+#pragma unroll
+    for (int c = 0; c < 16; ++c) a[c] = c <= i ? D[i * kXLD + c] : T(0);
+    LdltTileCols<0, T>::run(a, dd, i);
+#pragma unroll
+    for (int c = 0; c < 16; ++c) a[c] = c <= i ? a[c] : T(0);
+    tile_inverse_unit_rows<T>(a, w);
+    wave_order();
+    if (g == 0)
+#pragma unroll
+        for (int c = 0; c < 16; ++c) D[i * kXLD + c] = a[c];
+    if (lane < 16) {
+        T dl = dd[0];   // the pivots are the same in every lane: lane c stores δ_c
+#pragma unroll
+        for (int c = 1; c < 16; ++c) dl = lane == c ? dd[c] : dl;
+        dv[lane] = dl;
+    }
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+        Li[i * kXLD + g + 4 * t] = w[t];
+        if (LinvG) LinvG[j * 256 + i * 16 + g + 4 * t] = w[t];
+    }
+    wave_order();
+#pragma unroll
+    for (int r = 0; r < 4; ++r) d[r] = D[lc * kXLD + M::row(lane, r)];
+}
+
+// d (a diagonal tile, transposed) −= W_j L_jᵀ for tile row j of W (X) and
+// of L (Y) in LDS
+template <typename T>
+__device__ __forceinline__ void ldlt_diag_update(typename PanelMfma<T>::acc_t &d, const T *X,
+                                                 const T *Y, int j)
+{
+    typedef PanelMfma<T> M;
+    const int lane = threadIdx.x & 63;
+    const int lc = lane & 15;
+#pragma unroll
+    for (int kq = 0; kq < 4; ++kq) {
+        const int kk = (lane >> 4) + 4 * kq;
+        d = M::mma(-Y[(j * 16 + lc) * kXLD + kk], X[(j * 16 + lc) * kXLD + kk], d);
+    }
+}
+
+// The LDLᴴ of the nb×nb (nb ≤ 128) diagonal block at A, with the tiles held
+// as in factor_body (transposed; wave 0 the diagonal tiles, waves 1–3 the
+// rest) and the same two-barrier step: (A) waves 1–3 form W = A L_kk⁻ᵀ
+// (rdiv! by UnitLowerTriangular, src/ldlt.jl:86) and L = W D_kk⁻¹ (rdiv! by
+// Diagonal, :87, a division) for their panel tiles while wave 0 applies the
+// previous step's update to the diagonal tiles beyond kb + 1; (B) wave 0
+// updates diagonal tile kb + 1, factors it (LdltTileCols) and inverts its
+// unit factor while waves 1–3 apply C −= W Lᵀ (:88-95) on MFMA.  Writes D
+// on the diagonal and L below it, and the inverses of the unit 16×16
+// diagonal tiles (row-major, 256 elements each) to LinvG.
 template <typename T, int W>
 __device__ __forceinline__ void ldlt_body(int nb, T *__restrict__ A, int64_t lda,
                                           T *__restrict__ LinvG, T *Xb, T *Yb, T *Db, T *Lib,
@@ -411,111 +464,98 @@ __device__ __forceinline__ void ldlt_body(int nb, T *__restrict__ A, int64_t lda
 {
     typedef PanelMfma<T> M;
     typedef typename M::acc_t acc_t;
-    constexpr int NQ = 9;
+    constexpr int NQ = W == 0 ? 8 : 10;
+    constexpr int NBLK = kPanelNB / 16;
+    constexpr int XS = kPanelNB * kXLD, LS = 16 * kXLD;
     const int lane = threadIdx.x & 63;
     const int lc = lane & 15;
+    const int nkb = (nb + 15) / 16;
 
     acc_t acc[NQ];
 #pragma unroll
     for (int q = 0; q < NQ; ++q) {
-        const int bi = fslot_bi(W, q), bj = fslot_bj(W, q);
+        const int bi = dslot_bi<W>(q), bj = dslot_bj<W>(q);
+        if (bi < 0) continue;
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
-            const int gi = bi * 16 + M::row(lane, r), gj = bj * 16 + lc;
+            const int gi = bi * 16 + lc, gj = bj * 16 + M::row(lane, r);
             T v = T(0);
             if (gj <= gi) v = gi < nb ? A[gi + (int64_t)gj * lda] : (gi == gj ? T(1) : T(0));
             acc[q][r] = v;
         }
     }
-
-    const int nkb = (nb + 15) / 16;
-    for (int kb = 0; kb < nkb; ++kb) {
-        T *X = Xb + (kb & 1) * (kPanelNB * kXLD);   // W of column block kb
-        T *Y = Yb + (kb & 1) * (kPanelNB * kXLD);   // L of column block kb
-        T *D = Db + (kb & 1) * (16 * kXLD);
-        T *Li = Lib + (kb & 1) * (16 * kXLD);
-        T *dv = dvs + (kb & 1) * 16;
-        // (1) column block kb to LDS: the diagonal tile to D, the tiles below to X
+    if constexpr (W == 0) ldlt_diag_factor<T>(acc[0], 0, Db, Lib, dvs, LinvG);
+    __syncthreads();
 #pragma unroll
-        for (int q = 0; q < NQ; ++q) {
-            const int bi = fslot_bi(W, q), bj = fslot_bj(W, q);
-            if (bj != kb) continue;
-            T *dst = bi == kb ? D : X + bi * 16 * kXLD;
+    for (int kb = 0; kb < NBLK; ++kb) {
+        if (kb >= nkb) break;
+        T *X = Xb + (kb & 1) * XS;   // W of column block kb
+        T *Y = Yb + (kb & 1) * XS;   // L of column block kb
+        // (A)
+        if constexpr (W == 0) {
+            if (kb > 0) {
+                const T *Xp = Xb + ((kb - 1) & 1) * XS, *Yp = Yb + ((kb - 1) & 1) * XS;
 #pragma unroll
-            for (int r = 0; r < 4; ++r) dst[M::row(lane, r) * kXLD + lc] = acc[q][r];
-        }
-        __syncthreads();
-        // (2) wave 0: LDLᴴ of the diagonal tile and the inverse of its unit factor
-        if (W == 0) {
-            const int i = lane & 15, g = lane >> 4;
-            T a[16], w[4], d[16];
-#pragma unroll
-            for (int j = 0; j < 16; ++j) a[j] = j <= i ? D[i * kXLD + j] : T(0);
-            LdltTileCols<0, T>::run(a, d, i);
-#pragma unroll
-            for (int j = 0; j < 16; ++j) a[j] = j <= i ? a[j] : T(0);
-            tile_inverse_unit_rows<T>(a, w);
-            if (g == 0)
-#pragma unroll
-                for (int j = 0; j < 16; ++j) D[i * kXLD + j] = a[j];
-            if (lane < 16) {
-                T dl = d[0];   // the pivots are the same in every lane: lane j stores δ_j
-#pragma unroll
-                for (int j = 1; j < 16; ++j) dl = lane == j ? d[j] : dl;
-                dv[lane] = dl;
+                for (int j = kb + 1; j < NBLK; ++j) {
+                    if (j >= nkb) break;
+                    ldlt_diag_update<T>(acc[j], Xp, Yp, j);
+                }
             }
+        } else {
+            const T *Lik = Lib + (kb & 1) * LS;
+            const T *dv = dvs + (kb & 1) * 16;
+            T dcol[4];
 #pragma unroll
-            for (int t = 0; t < 4; ++t) {
-                Li[i * kXLD + g + 4 * t] = w[t];
-                if (LinvG) LinvG[kb * 256 + i * 16 + g + 4 * t] = w[t];
+            for (int r = 0; r < 4; ++r) dcol[r] = dv[M::row(lane, r)];
+#pragma unroll
+            for (int q = 0; q < NQ; ++q) {
+                const int bi = dslot_bi<W>(q), bj = dslot_bj<W>(q);
+                if (bi < 0 || bj != kb) continue;
+                if (bi >= nkb) continue;
+                acc_t x = {T(0), T(0), T(0), T(0)};
+#pragma unroll
+                for (int kq = 0; kq < 4; ++kq)
+                    x = M::mma(Lik[lc * kXLD + M::row(lane, kq)], acc[q][kq], x);
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    acc[q][r] = x[r] / dcol[r];
+                    X[(bi * 16 + lc) * kXLD + M::row(lane, r)] = x[r];
+                    Y[(bi * 16 + lc) * kXLD + M::row(lane, r)] = acc[q][r];
+                }
             }
         }
         __syncthreads();
-        // (3) the diagonal tile takes D and L; tiles below: W = A L⁻ᵀ, L = W D⁻¹
-        const T dcol = dv[lc];
-#pragma unroll
-        for (int q = 0; q < NQ; ++q) {
-            const int bi = fslot_bi(W, q), bj = fslot_bj(W, q);
-            if (bj != kb) continue;
-            if (bi == kb) {
-#pragma unroll
-                for (int r = 0; r < 4; ++r) acc[q][r] = D[M::row(lane, r) * kXLD + lc];
-                continue;
+        // (B)
+        if constexpr (W == 0) {
+            if (kb + 1 < nkb) {
+                ldlt_diag_update<T>(acc[kb + 1], X, Y, kb + 1);
+                ldlt_diag_factor<T>(acc[kb + 1], kb + 1, Db, Lib + ((kb + 1) & 1) * LS,
+                                    dvs + ((kb + 1) & 1) * 16, LinvG);
             }
-            acc_t x = {T(0), T(0), T(0), T(0)};
+        } else {
 #pragma unroll
-            for (int kq = 0; kq < 4; ++kq) {
-                const int kk = (lane >> 4) + 4 * kq;
-                x = M::mma(X[(bi * 16 + lc) * kXLD + kk], Li[lc * kXLD + kk], x);
-            }
+            for (int q = 0; q < NQ; ++q) {
+                const int bi = dslot_bi<W>(q), bj = dslot_bj<W>(q);
+                if (bi < 0 || bj <= kb) continue;
+                if (bi >= nkb) continue;
 #pragma unroll
-            for (int r = 0; r < 4; ++r) {
-                acc[q][r] = x[r] / dcol;
-                X[(bi * 16 + M::row(lane, r)) * kXLD + lc] = x[r];
-                Y[(bi * 16 + M::row(lane, r)) * kXLD + lc] = acc[q][r];
+                for (int kq = 0; kq < 4; ++kq) {
+                    const int kk = (lane >> 4) + 4 * kq;
+                    const T a = -Y[(bj * 16 + lc) * kXLD + kk];
+                    const T b = X[(bi * 16 + lc) * kXLD + kk];
+                    acc[q] = M::mma(a, b, acc[q]);
+                }
             }
         }
         __syncthreads();
-        // (4) rank-16 trailing update C_ij −= W_i L_jᵀ
-#pragma unroll
-        for (int q = 0; q < NQ; ++q) {
-            const int bi = fslot_bi(W, q), bj = fslot_bj(W, q);
-            if (bj <= kb) continue;
-#pragma unroll
-            for (int kq = 0; kq < 4; ++kq) {
-                const int kk = (lane >> 4) + 4 * kq;
-                const T a = -X[(bi * 16 + lc) * kXLD + kk];
-                const T b = Y[(bj * 16 + lc) * kXLD + kk];
-                acc[q] = M::mma(a, b, acc[q]);
-            }
-        }
     }
 #pragma unroll
     for (int q = 0; q < NQ; ++q) {
-        const int bi = fslot_bi(W, q), bj = fslot_bj(W, q);
+        const int bi = dslot_bi<W>(q), bj = dslot_bj<W>(q);
+        if (bi < 0) continue;
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
-            const int gi = bi * 16 + M::row(lane, r), gj = bj * 16 + lc;
+            const int gi = bi * 16 + lc, gj = bj * 16 + M::row(lane, r);
             if (gi < nb && gj <= gi) A[gi + (int64_t)gj * lda] = acc[q][r];
         }
     }
