@@ -1142,64 +1142,83 @@ __global__ __launch_bounds__(kHrThreads) void hr_lu_kernel(int k, T *Q, int64_t 
     // one 16×16 tile product on MFMA: acc += A B, A(i, kk) = a(i, kk),
     // B(kk, j) = b(kk, j); element r of lane l of the result sits at
     // (Mf::row(l, r), l % 16)
+    // The diagonal block of column block jb: wave 0 eliminates it, forms the
+    // inverses W = L⁻¹, Z = U⁻¹ and the tile inverse of Un.
+    auto elim = [&](int jb) {
+        const int nbk = min(16, k - jb);
+        T *Wb = Ws[jb >> 4];
+        // lanes 0..15 (row r = lane)
+        const int r = li;
+        T d[16];
+#pragma unroll
+        for (int c = 0; c < 16; ++c) d[c] = M[jb + r + (jb + c) * kHrLD];
+        // scheduling fences between the chained stages of this wave's
+        // work keep their loads from being hoisted into the elimination
+        __builtin_amdgcn_sched_barrier(0);
+        T dv[16], uv[16];
+        HrLuCols<0, T>::run(d, dv, uv, r, nbk);
+        __builtin_amdgcn_sched_barrier(0);
+        if (lane == 0) {
+#pragma unroll
+            for (int c = 0; c < 16; ++c)
+                if (c < nbk) {
+                    Dv[jb + c] = dv[c];
+                    Uv[jb + c] = uv[c];
+                }
+        }
+        if (lane < 16 && jb + r < 128) {
+#pragma unroll
+            for (int c = 0; c < 16; ++c) M[jb + r + (jb + c) * kHrLD] = d[c];
+        }
+        // W = L⁻¹ (unit lower) and Z = U⁻¹ (upper) from the rows in
+        // registers, four columns per lane (wz[t] = X[r][g + 4t]); rows and
+        // columns past nbk (a partial last block) are the identity
+        const int g = lane >> 4;
+        T m[16], wv[4], zv[4];
+#pragma unroll
+        for (int c = 0; c < 16; ++c) m[c] = r > c ? d[c] : T(0);
+#pragma unroll
+        for (int t = 0; t < 4; ++t) wv[t] = zv[t] = (r == g + 4 * t) ? T(1) : T(0);
+        TileInvSteps<0, T>::run(m, wv);
+        T udiag = d[0];
+#pragma unroll
+        for (int c = 1; c < 16; ++c) udiag = r == c ? d[c] : udiag;
+        const T myr = r < nbk ? recip1(udiag) : T(1);
+        // unscaled rows Y[i] = U_ii Z[i]: Y[i] −= (U[i][k]/U[k][k]) Y[k]
+        HrUpMul<0, T>::run(d, myr, r, m);
+        TileInvStepsUp<15, T>::run(m, zv);
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+            const int j = g + 4 * t;
+            zv[t] *= myr;
+            Wb[r + j * kHrTL] = wv[t];
+            Zs[r + j * kHrTL] = zv[t];
+            // the 16×16 diagonal-tile inverse of Un = −Uᵀ, row-major as
+            // chol_solve_kernel reads it: (−U_JJᵀ)⁻¹ = −Zᵀ (the padding of
+            // a partial last block is the identity)
+            UnInv[(jb >> 4) * 256 + j * 16 + r] =
+                (j >= nbk || r >= nbk) ? (j == r ? T(1) : T(0)) : -zv[t];
+        }
+    };
+    // One 16×16 tile of the trailing update M_below,right −= L_below,J U_J,right.
+    auto trail_tile = [&](int jb, int c0, int nct, int it) {
+        const int io = c0 + 16 * (it % nct), jo = c0 + 16 * (it / nct);
+        acc_t acc;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) acc[q] = M[io + Mf::row(lane, q) + (jo + li) * kHrLD];
+#pragma unroll
+        for (int kq = 0; kq < 4; ++kq) {
+            const int kk = 4 * kq + lk;
+            acc = Mf::mma(-M[io + li + (jb + kk) * kHrLD], M[jb + kk + (jo + li) * kHrLD], acc);
+        }
+#pragma unroll
+        for (int q = 0; q < 4; ++q) M[io + Mf::row(lane, q) + (jo + li) * kHrLD] = acc[q];
+    };
+    if (w == 0) elim(0);
+    __syncthreads();
     for (int jb = 0; jb < k; jb += 16) {
         const int nbk = min(16, k - jb), c0 = jb + nbk;
         T *Wb = Ws[jb >> 4];
-        if (w == 0) {
-            // ---- the diagonal block, lanes 0..15 (row r = lane)
-            const int r = li;
-            T d[16];
-#pragma unroll
-            for (int c = 0; c < 16; ++c) d[c] = M[jb + r + (jb + c) * kHrLD];
-            // scheduling fences between the chained stages of this wave's
-            // work keep their loads from being hoisted into the elimination
-            __builtin_amdgcn_sched_barrier(0);
-            T dv[16], uv[16];
-            HrLuCols<0, T>::run(d, dv, uv, r, nbk);
-            __builtin_amdgcn_sched_barrier(0);
-            if (lane == 0) {
-#pragma unroll
-                for (int c = 0; c < 16; ++c)
-                    if (c < nbk) {
-                        Dv[jb + c] = dv[c];
-                        Uv[jb + c] = uv[c];
-                    }
-            }
-            if (lane < 16 && jb + r < 128) {
-#pragma unroll
-                for (int c = 0; c < 16; ++c) M[jb + r + (jb + c) * kHrLD] = d[c];
-            }
-            // W = L⁻¹ (unit lower) and Z = U⁻¹ (upper) from the rows in
-            // registers, four columns per lane (wz[t] = X[r][g + 4t]); rows and
-            // columns past nbk (a partial last block) are the identity
-            const int g = lane >> 4;
-            T m[16], wv[4], zv[4];
-#pragma unroll
-            for (int c = 0; c < 16; ++c) m[c] = r > c ? d[c] : T(0);
-#pragma unroll
-            for (int t = 0; t < 4; ++t) wv[t] = zv[t] = (r == g + 4 * t) ? T(1) : T(0);
-            TileInvSteps<0, T>::run(m, wv);
-            T udiag = d[0];
-#pragma unroll
-            for (int c = 1; c < 16; ++c) udiag = r == c ? d[c] : udiag;
-            const T myr = r < nbk ? recip1(udiag) : T(1);
-            // unscaled rows Y[i] = U_ii Z[i]: Y[i] −= (U[i][k]/U[k][k]) Y[k]
-            HrUpMul<0, T>::run(d, myr, r, m);
-            TileInvStepsUp<15, T>::run(m, zv);
-#pragma unroll
-            for (int t = 0; t < 4; ++t) {
-                const int j = g + 4 * t;
-                zv[t] *= myr;
-                Wb[r + j * kHrTL] = wv[t];
-                Zs[r + j * kHrTL] = zv[t];
-                // the 16×16 diagonal-tile inverse of Un = −Uᵀ, row-major as
-                // chol_solve_kernel reads it: (−U_JJᵀ)⁻¹ = −Zᵀ (the padding of
-                // a partial last block is the identity)
-                UnInv[(jb >> 4) * 256 + j * 16 + r] =
-                    (j >= nbk || r >= nbk) ? (j == r ? T(1) : T(0)) : -zv[t];
-            }
-        }
-        __syncthreads();
         if (c0 >= k) break;
         const int nct = (k - c0 + 15) / 16;
         // ---- U_J,right = W M_J,right and L_below,J = M_below,J Z
@@ -1225,19 +1244,14 @@ __global__ __launch_bounds__(kHrThreads) void hr_lu_kernel(int k, T *Q, int64_t 
             }
         }
         __syncthreads();
-        // ---- M_below,right −= L_below,J U_J,right
-        for (int it = w; it < nct * nct; it += kHrThreads / 64) {
-            const int io = c0 + 16 * (it % nct), jo = c0 + 16 * (it / nct);
-            acc_t acc;
-#pragma unroll
-            for (int q = 0; q < 4; ++q) acc[q] = M[io + Mf::row(lane, q) + (jo + li) * kHrLD];
-#pragma unroll
-            for (int kq = 0; kq < 4; ++kq) {
-                const int kk = 4 * kq + lk;
-                acc = Mf::mma(-M[io + li + (jb + kk) * kHrLD], M[jb + kk + (jo + li) * kHrLD], acc);
-            }
-#pragma unroll
-            for (int q = 0; q < 4; ++q) M[io + Mf::row(lane, q) + (jo + li) * kHrLD] = acc[q];
+        // ---- M_below,right −= L_below,J U_J,right: wave 0 updates the next
+        //      diagonal tile and eliminates it while waves 1–3 update the rest
+        if (w == 0) {
+            trail_tile(jb, c0, nct, 0);
+            __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+            elim(c0);
+        } else {
+            for (int it = w; it < nct * nct; it += kHrThreads / 64 - 1) trail_tile(jb, c0, nct, it);
         }
         __syncthreads();
     }
