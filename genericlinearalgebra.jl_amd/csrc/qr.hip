@@ -671,6 +671,9 @@ __global__ __launch_bounds__(256) void trtri_unit_upper_leaf(int k, T *U, int64_
         const int i = threadIdx.x;
         if (i < j) {
             s = S[i + j * LD];
+            // unrolled so that the LDS reads of eight terms are issued
+            // together; the sum keeps its order
+#pragma unroll 8
             for (int l = i + 1; l < j; ++l) s += S[i + l * LD] * S[l + j * LD];
         }
         __syncthreads();
