@@ -787,24 +787,37 @@ __global__ void zmake_v_kernel(int64_t m, int64_t k, const T *F, int64_t ldf, T 
 // T (k×k, k ≤ 64) from G = VᴴV (k×k, ld ldg) and τ: S = I + diag(τ)
 // striu(G), T = S⁻¹ diag(τ) (the UT transform of src/qr.jl:64-83 with the
 // first term of each entry conjugated, so that I − V T Vᴴ = H₁⋯H_k).
-// One workgroup; thread c computes column c of S⁻¹ by back substitution.
+// One workgroup: the strict upper part of S is formed once into LDS
+// (coalesced reads of G), then thread c computes column c of S⁻¹ by back
+// substitution from LDS alone.
 template <typename T>
 __global__ void zqr_tsmall_kernel(int k, const T *G, int64_t ldg, const T *tau, T *Tm, int64_t ldt)
 {
     __shared__ T Xr[kZB * kZB], Xi[kZB * kZB];
+    __shared__ T Sr[kZB * kZB], Si[kZB * kZB];   // S[r, l] = τ_r G[r, l], r < l
+    for (int e = threadIdx.x; e < k * k; e += blockDim.x) {
+        const int r = e % k, l = e / k;
+        T vr = T(0), vi = T(0);
+        if (r < l) {
+            const T gr = G[2 * (r + (int64_t)l * ldg)], gi = G[2 * (r + (int64_t)l * ldg) + 1];
+            const T tr = tau[2 * r], ti = tau[2 * r + 1];
+            vr = tr * gr - ti * gi;
+            vi = tr * gi + ti * gr;
+        }
+        Sr[r * kZB + l] = vr;
+        Si[r * kZB + l] = vi;
+    }
+    __syncthreads();
     const int c = threadIdx.x;
     if (c < k) {
         // column c of S⁻¹ (unit upper): x_c = 1, x_r = −Σ_{l=r+1..c} S[r,l] x_l for r < c
-        for (int r = kZB - 1; r >= 0; --r) {
+        for (int r = k - 1; r >= 0; --r) {
             T xr = T(0), xi = T(0);
             if (r == c) xr = T(1);
             else if (r < c) {
                 T sr = T(0), si = T(0);
                 for (int l = r + 1; l <= c; ++l) {
-                    // S[r,l] = τ_r G[r,l]
-                    const T gr = G[2 * (r + (int64_t)l * ldg)], gi = G[2 * (r + (int64_t)l * ldg) + 1];
-                    const T tr = tau[2 * r], ti = tau[2 * r + 1];
-                    const T s_r = tr * gr - ti * gi, s_i = tr * gi + ti * gr;
+                    const T s_r = Sr[r * kZB + l], s_i = Si[r * kZB + l];
                     const T yr = Xr[l * kZB + c], yi = Xi[l * kZB + c];
                     sr += s_r * yr - s_i * yi;
                     si += s_r * yi + s_i * yr;
