@@ -144,6 +144,25 @@ __host__ __device__ constexpr int oslot_bj(int w, int q)
 {
     return w == 1 ? (q < 7 ? q : q - 7) : w == 2 ? (q < 6 ? q : q - 6) : (q < 5 ? q : (q < 9 ? q - 5 : q - 9));
 }
+// Every strictly-lower tile of the 8×8-tile block belongs to exactly one
+// slot of waves 1–3, and every slot past a wave's last is marked unused.
+constexpr bool oslots_cover()
+{
+    int seen[8][8] = {};
+    for (int w = 1; w <= 3; ++w)
+        for (int q = 0; q < 10; ++q) {
+            const int bi = oslot_bi(w, q), bj = oslot_bj(w, q);
+            if (bi < 0) continue;
+            if (bi > 7 || bj < 0 || bj >= bi) return false;
+            ++seen[bi][bj];
+        }
+    for (int i = 0; i < 8; ++i)
+        for (int j = 0; j < i; ++j)
+            if (seen[i][j] != 1) return false;
+    return true;
+}
+static_assert(oslots_cover(), "the tile slots of waves 1-3 must cover the strictly-lower tiles once");
+
 // Cholesky tile factor: slot q of wave W (wave 0: diagonal tile q).
 template <int W>
 __host__ __device__ constexpr int dslot_bi(int q) { return W == 0 ? q : oslot_bi(W, q); }
